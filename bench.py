@@ -1,0 +1,196 @@
+"""VarDCT front-end throughput on one MI355X.
+
+A step is one pass of the front end (sRGB u8 -> XYB -> adaptive quant field
+-> AC-strategy search -> forward DCT + quantization) over one 2048x2048 RGB8
+synthetic photo-like image at quality 75, with the input already resident in
+HBM.  Prints one JSON line: whole-job megapixels per second, the dominant
+kernel's roofline position, and the CPU oracle's throughput on the host cores
+of the same machine as a reported baseline.
+
+usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+# Algorithmic HBM bytes per pixel of each stage (reads + writes of the
+# buffers the stage must touch once).
+STAGE_BYTES_PER_PX = {
+    "xyb": 3.0 + 12.0,          # u8 RGB in, 3 f32 planes out
+    "aq": 12.0 + 1.0 / 64,      # 3 planes in (Y twice from cache), u8 field out
+    "select": 12.0 + 2.0 / 64,  # 3 planes in, cells + field out
+    "quantize": 12.0 + 24.0 + 6.0 / 64 + 2.0 / 64,  # planes in, i32+f32 out, dc/qf/cells
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--width", type=int, default=2048)
+    ap.add_argument("--height", type=int, default=2048)
+    ap.add_argument("--quality", type=float, default=75.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile", action="store_true", help="few steps, no CPU leg (for rocprofv3)")
+    return ap.parse_args()
+
+
+def cpu_baseline(rgb, params, threads):
+    """The oracle's whole front end on the host cores (kind "port")."""
+    import jixel_amd as J
+    from oracle import oracle as O
+    h, w, _ = rgb.shape
+    tables = J.default_quant_tables()
+    z = np.zeros(((h + 63) // 64, (w + 63) // 64), np.int8)
+    O.front_end(rgb[:64, :64].copy(), params, tables, z[:1, :1], z[:1, :1], threads)  # warm
+    runs = []
+    t_end = time.perf_counter() + 12.0
+    while len(runs) < 5 and (not runs or time.perf_counter() < t_end):
+        t0 = time.perf_counter()
+        O.front_end(rgb, params, tables, z, z, threads)
+        runs.append(time.perf_counter() - t0)
+    sec = float(np.median(runs))
+    return {
+        "value": round(w * h / 1e6 / sec, 3),
+        "unit": "MP/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"whole front end of the same {w}x{h} image, median of {len(runs)} runs "
+                  f"({sec:.2f} s each) with {threads} threads",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; every box here has one "
+                                   "MI355X and the front end shards by image / DC group "
+                                   "without collectives (see DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+
+    import jixel_amd as J
+    from jixel_amd import synth
+
+    if args.profile:
+        args.steps, args.warmup, args.no_cpu = min(args.steps, 5), min(args.warmup, 2), True
+    w, h = args.width, args.height
+    rgb = synth.photo_like(w, h, 2)
+    cfg = J.EncodeConfig().with_quality(args.quality)
+    p = J.frame_params(cfg)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream(dev)
+    fe = J.FrontEnd(0, stream=stream)
+    d_rgb = torch.from_numpy(rgb).to(dev)
+    outs = fe.outputs(w, h)
+    xb, yb = (w + 7) // 8, (h + 7) // 8
+    qf = fe.empty((yb, xb), torch.uint8)
+    cells = fe.empty((yb, xb), torch.uint8)
+    tiles = ((h + 63) // 64, (w + 63) // 64)
+    ytox = torch.zeros(tiles, dtype=torch.int8, device=dev)
+    ytob = torch.zeros(tiles, dtype=torch.int8, device=dev)
+
+    def step():
+        fe.run(d_rgb, w, h, p, ytox, ytob, outputs=outs, qf=qf, cells=cells)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_gpu = e0.elapsed_time(e1) / args.steps
+    ms_step = max(ms_gpu, wall * 1000.0 / args.steps)
+
+    # Per-stage device time with events on the context's stream.
+    xyb = fe.xyb_buffer(w, h)
+    qf2 = fe.empty((yb, xb), torch.uint8)
+    stage_ms = {k: 0.0 for k in STAGE_BYTES_PER_PX}
+    reps = max(3, min(args.steps, 10))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    for _ in range(reps):
+        ev[0].record(stream)
+        fe.xyb(d_rgb, w, h, fused=bool(p.fused), out=xyb)
+        ev[1].record(stream)
+        fe.fill_quant_field(xyb, w, h, p.distance, 1.0 / p.scale, bool(p.fused), out=qf2)
+        ev[2].record(stream)
+        fe.select_strategy(xyb, w, h, qf2, p, ytox, ytob, cells=cells)
+        ev[3].record(stream)
+        fe.forward_quantize(xyb, w, h, cells, qf2, p, ytox, ytob, outputs=outs)
+        ev[4].record(stream)
+        torch.cuda.synchronize()
+        for i, k in enumerate(stage_ms):
+            stage_ms[k] += ev[i].elapsed_time(ev[i + 1]) / reps
+    px = w * h
+    dominant = max(stage_ms, key=stage_ms.get)
+    alg_bytes = STAGE_BYTES_PER_PX[dominant] * px
+    achieved = alg_bytes / (stage_ms[dominant] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_photo_2048_q75.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dominant)
+
+    result = {
+        "metric": "VarDCT front-end MP/s at q75 (1 GPU vs CPU); ssimulacra2 at equal bytes",
+        "value": round(px / 1e6 / (ms_step * 1e-3), 2),
+        "unit": "MP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"photo_like_{w}x{h}_rgb8_q{args.quality:g}",
+            "width": w, "height": h, "quality": args.quality,
+            "distance": round(p.distance, 4), "speed": "fast",
+            "strategy_search": "square merges DCT8/DCT16X16/DCT32X32 + quant-field aggregation",
+            "fmla": "fused" if p.fused else "unfused",
+            "input": "u8 RGB resident in HBM; outputs (levels, f32 coefficients, DC, field, "
+                     "strategy) left in HBM",
+        },
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": int(alg_bytes),
+        },
+        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        result["cpu_baseline"] = cpu_baseline(rgb, p, threads)
+    else:
+        result["cpu_baseline"] = None
+    fe.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

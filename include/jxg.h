@@ -1,0 +1,294 @@
+/*
+ * jxg.h - C ABI of the MI355X (gfx950) VarDCT encoder front end.
+ *
+ * The front end is the data-parallel middle of jixel's lossy encoder
+ * (awxkee/jixel 0.5.14): sRGB u8 -> linear -> XYB, the adaptive quant field,
+ * the AC-strategy (transform layout) search and forward DCT + quantization
+ * into per-block integer coefficient buffers.  Entropy coding, RDOQ, MA trees,
+ * coefficient orders and the bit writer stay in the caller.
+ *
+ * Every entry point replaces one stage call of the reference's per-encode
+ * dispatch table (src/encoding_context.rs:224-294) at stage granularity, one
+ * call per image instead of one call per 8x8 block.  The replaced reference
+ * function is cited on each declaration.
+ *
+ * Conventions
+ *   - Plain C types only.  All functions return int: JXG_OK (0) or a negative
+ *     JXG_ERR_* code; jxg_last_error() gives a message for the calling thread.
+ *   - *_dev functions take device pointers and run on the context's stream;
+ *     they are asynchronous unless stated.  Host-pointer functions copy in,
+ *     run and copy out synchronously.
+ *   - Images are planar f32 XYB planes of `stride` floats per row and
+ *     `padded_h` rows (see jxg_plane_geometry).  Pixels past the image are
+ *     filled by edge replication (columns repeat the row's last pixel, rows
+ *     repeat the last row), which is exactly the reference's footprint gather
+ *     (src/ac_strategy.rs:451-479), so transforms never need bounds checks.
+ *   - Block maps (quant field, strategy cells, DC) are row-major with
+ *     xblocks = ceil(W/8) columns and yblocks = ceil(H/8) rows.
+ *   - No float atomics: results are independent of launch shape.
+ */
+#ifndef JXG_H_
+#define JXG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mapped by a caller onto jixel's EncodeError,
+ *      src/util.rs:129-160) ---- */
+#define JXG_OK 0
+#define JXG_ERR_INVALID_ARG (-1)      /* bad pointer / stride / enum      */
+#define JXG_ERR_EMPTY_IMAGE (-2)      /* EncodeError::EmptyImage          */
+#define JXG_ERR_TOO_LARGE (-3)        /* EncodeError::DimensionTooLarge   */
+#define JXG_ERR_INVALID_DISTANCE (-4) /* EncodeError::InvalidDistance     */
+#define JXG_ERR_ALLOC (-5)            /* EncodeError::AllocationFailed    */
+#define JXG_ERR_DEVICE (-6)           /* HIP runtime / launch failure     */
+#define JXG_ERR_NO_DEVICE (-7)        /* no gfx950 device visible         */
+
+/* Largest accepted image side (the format's 2^30 limit is far above what a
+ * single HBM-resident image needs; this bound keeps block counts in int32). */
+#define JXG_MAX_DIM (1u << 18)
+
+/* ---- raw AC strategy codes (src/dc_group_data.rs:32-56) ---- */
+enum {
+  JXG_DCT = 0,
+  JXG_DCT16X8 = 1, /* 8 px wide, 16 px tall */
+  JXG_DCT8X16 = 2, /* 16 px wide, 8 px tall */
+  JXG_DCT16X16 = 3,
+  JXG_DCT32X32 = 4,
+  JXG_DCT4X4 = 5,
+  JXG_DCT4X8 = 6,
+  JXG_DCT8X4 = 7,
+  JXG_DCT32X16 = 8, /* 16 wide, 32 tall */
+  JXG_DCT16X32 = 9, /* 32 wide, 16 tall */
+  JXG_AFV0 = 10,
+  JXG_AFV1 = 11,
+  JXG_AFV2 = 12,
+  JXG_AFV3 = 13,
+  JXG_DCT64X64 = 14,
+  JXG_DCT64X32 = 15, /* 32 wide, 64 tall */
+  JXG_DCT32X64 = 16, /* 64 wide, 32 tall */
+  JXG_IDENTITY = 17,
+  JXG_DCT2X2 = 18,
+  JXG_DCT128X128 = 19,
+  JXG_DCT128X64 = 20,
+  JXG_DCT64X128 = 21,
+  JXG_DCT32X8 = 22, /* 8 wide, 32 tall */
+  JXG_DCT8X32 = 23, /* 32 wide, 8 tall */
+  JXG_NUM_STRATEGIES = 24
+};
+
+/* A strategy cell is (raw_strategy << 1) | first_block_bit, the reference's
+ * AcStrategyImage encoding (src/dc_group_data.rs:85-170). */
+#define JXG_CELL(strategy, first) ((uint8_t)(((strategy) << 1) | ((first) ? 1 : 0)))
+
+/* ---- quant-matrix blob ----
+ * One float array holding, for every table kind k, matrix[3][n_k] (step
+ * sizes, DequantMatrices::matrix*) followed by inv[3][n_k] (reciprocals with
+ * inv[c][0] = 0, DequantMatrices::inv_matrix*), kinds in the order below.
+ * This is the data a caller hands over from its DequantMatrices
+ * (src/quant_weights.rs:864-925) so the device uses exactly its tables. */
+enum {
+  JXG_QT_DCT8 = 0,     /* 64   */
+  JXG_QT_DCT16X8 = 1,  /* 128, shared by 16X8 / 8X16, 8 rows x 16 cols */
+  JXG_QT_DCT16X16 = 2, /* 256  */
+  JXG_QT_DCT32X32 = 3, /* 1024 */
+  JXG_QT_DCT32X16 = 4, /* 512, shared, 16 rows x 32 cols */
+  JXG_QT_DCT8X32 = 5,  /* 256, shared, 8 rows x 32 cols */
+  JXG_QT_DCT64X64 = 6, /* 4096 */
+  JXG_QT_DCT64X32 = 7, /* 2048, shared, 32 rows x 64 cols */
+  JXG_QT_IDENTITY = 8, /* 64 */
+  JXG_QT_DCT2X2 = 9,   /* 64 */
+  JXG_QT_DCT4X4 = 10,  /* 64 */
+  JXG_QT_DCT4X8 = 11,  /* 64 */
+  JXG_QT_AFV = 12,     /* 64 */
+  JXG_QT_NUM_KINDS = 13
+};
+/* Total floats in a quant-matrix blob: 6 * sum(n_k). */
+#define JXG_QT_BLOB_FLOATS 52224u
+
+/* Element count and blob offset of table kind k; matrix of channel c starts
+ * at offset + c * n, its inverse at offset + (3 + c) * n. */
+uint32_t jxg_qt_size(int kind);
+uint32_t jxg_qt_offset(int kind);
+/* Table kind used by a raw strategy, or -1 (src/group.rs:1459-1495). */
+int jxg_qt_kind_of(int strategy);
+
+/* ---- frame parameters ----
+ * The scalars the reference reads from DistanceParams
+ * (src/frame/quantization.rs:36-50) and EncodingContext. */
+typedef struct jxg_frame_params {
+  float distance;
+  float scale;    /* global_scale / 65536 */
+  float scale_dc; /* quant_dc * scale */
+  float dc_step[3];
+  int32_t ytob_dc;
+  float cfl_base_x;       /* CflFrame::base_x (0 by default)   */
+  float cfl_base_b;       /* CflFrame::base_b (1 by default)   */
+  float cfl_color_factor; /* CflFrame::color_factor (84)       */
+  uint32_t x_qm_scale;    /* X AC multiplier 1.25^(x_qm_scale-2) */
+  uint32_t b_qm_scale;    /* B AC multiplier table index, 2..7   */
+  float dequant_one[3];   /* DequantBias::one per channel        */
+  float dequant_numerator;
+  int32_t fused;  /* 1: the reference's fmla is a fused multiply-add (aarch64,
+                     FMA-enabled x86 builds); 0: a*b+c (generic x86-64)    */
+  int32_t speed;  /* JXG_SPEED_* */
+  int32_t global_scale;
+  int32_t quant_dc;
+  int32_t epf_iters;
+  int32_t reserved;
+  float channel_weight[3]; /* EncodingContext::channel_weights (X, Y, B) */
+} jxg_frame_params;
+
+/* Speed tiers (src/encode_image.rs Speed, src/effort.rs:305-441). */
+enum {
+  JXG_SPEED_ULTRAFAST = 0,
+  JXG_SPEED_FASTEST = 1,
+  JXG_SPEED_FAST = 2,
+  JXG_SPEED_MEDIUM = 3,
+  JXG_SPEED_MEDIUMSLOW = 4,
+  JXG_SPEED_SLOW = 5,
+  JXG_SPEED_EXTRASLOW = 6
+};
+
+/* XYB arithmetic flavour: the scalar kernel (src/xyb.rs:146-185) or the
+ * x86-64 AVX2 production kernel (src/avx/xyb.rs:234-262: fast cube-root seed
+ * and x*(num/den) Halley step, always fused). */
+enum { JXG_XYB_SCALAR = 0, JXG_XYB_AVX2 = 1 };
+
+/* ---- host-side parameter helpers (no device needed) ---- */
+
+/* distance_from_quality (src/encode_image.rs:1321-1337). */
+float jxg_distance_from_quality(float quality);
+
+/* Fills every field of *out for `distance` and `speed` exactly as
+ * compute_distance_params (src/frame/quantization.rs:89-149) and the
+ * EncodingContext defaults (src/encoding_context.rs:609-700) do; `fused`
+ * selects the fmla semantics.  Returns JXG_ERR_INVALID_DISTANCE for a
+ * non-finite or non-positive distance. */
+int jxg_frame_params_init(float distance, int speed, int fused, jxg_frame_params *out);
+
+/* The 256-entry u8 sRGB -> linear table of TransferLut::build
+ * (src/color.rs:35-45,295-340): computed in f64, stored as f32. */
+void jxg_srgb_lut(float out[256]);
+
+/* The spec-default quant-matrix blob (JPEG XL library tables: the band
+ * parameters of src/quant_weights.rs:123-1230 without distance overrides).
+ * A caller with its own DequantMatrices passes those instead. */
+int jxg_default_quant_tables(float *blob /* JXG_QT_BLOB_FLOATS */);
+
+/* Plane geometry used by the device stages for a W x H image: rows are
+ * padded to a multiple of 64 pixels and the stride to a multiple of 64 floats
+ * (256 B) so every 64x64 tile is whole and every row load is aligned. */
+void jxg_plane_geometry(uint32_t width, uint32_t height, uint32_t *stride,
+                        uint32_t *padded_h);
+
+/* ---- context ---- */
+typedef struct jxg_ctx jxg_ctx;
+
+/* Creates a context on HIP device `device`, launching on `stream`
+ * (a hipStream_t, or NULL for the null stream). */
+int jxg_ctx_create(int device, void *stream, jxg_ctx **out);
+void jxg_ctx_destroy(jxg_ctx *ctx);
+int jxg_ctx_synchronize(jxg_ctx *ctx);
+const char *jxg_last_error(void);
+
+/* Uploads a quant-matrix blob (host pointer) into the context; used by the
+ * strategy and quantize stages until replaced. */
+int jxg_ctx_set_tables(jxg_ctx *ctx, const float *blob);
+
+/* ---- device stages ---- */
+
+/* u8 sRGB (3 or 4 channels, alpha ignored) -> linear -> XYB planes.
+ * Replaces linearize_rgb (src/encode_image.rs:1498) followed by
+ * to_xyb_in_place_with_fn (src/xyb.rs:439) with the selected ToXybBandFn
+ * (src/xyb.rs:347).  `fwd` is XybMatrix::fwd (src/xyb.rs:81-96); `lut` the
+ * 256-entry transfer table.  Writes planes X, Y, B at d_xyb + c*stride*padded_h. */
+int jxg_xyb_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channels,
+                uint32_t width, uint32_t height, const float fwd[9], const float lut[256],
+                int order, int fused, float *d_xyb, uint32_t stride, uint32_t padded_h);
+
+/* Adaptive quant field of the whole image, one DC group (2048 px) at a time.
+ * Replaces (ctx.fill_quant_field) = fill_quant_field_scalar
+ * (src/adaptive_quant.rs:547-729) as called by setup_dc_group
+ * (src/frame/dc_groups.rs:149).  d_qf is xblocks x yblocks bytes. */
+int jxg_fill_quant_field_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
+                             uint32_t padded_h, uint32_t width, uint32_t height,
+                             float distance, float inv_scale, int fused, uint8_t *d_qf);
+
+/* Coefficient-model RD costs of every square candidate the Fast tiers
+ * evaluate: per block its DCT8 cost, per 2x2-aligned site the DCT16X16 cost,
+ * per 4x4-aligned site the DCT32X32 cost (NaN where a site cannot be placed).
+ * Replaces strategy_cost (src/ac_strategy.rs:724 -> :1293) for those
+ * candidates.  Outputs are xblocks x yblocks floats each (the 16/32 costs are
+ * stored at the site's first block).  d_ytox / d_ytob are the per-64x64-tile
+ * CfL maps, tiles_x = ceil(W/64) columns. */
+int jxg_strategy_costs_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
+                           uint32_t padded_h, uint32_t width, uint32_t height,
+                           const uint8_t *d_qf, const int8_t *d_ytox, const int8_t *d_ytob,
+                           const jxg_frame_params *params, float *d_cost8,
+                           float *d_cost16, float *d_cost32);
+
+/* Transform-layout decision over square merges (DCT8 / DCT16X16 /
+ * DCT32X32) and the quant-field aggregation over each merged transform.
+ * Replaces the square-merge core of fill_ac_strategy
+ * (src/ac_strategy/selection.rs:1000-1180 select_band, :238 select_super_block)
+ * and adjust_quant_field (:503).  Writes strategy cells and updates d_qf in
+ * place. */
+int jxg_select_strategy_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
+                            uint32_t padded_h, uint32_t width, uint32_t height,
+                            uint8_t *d_qf, const int8_t *d_ytox, const int8_t *d_ytob,
+                            const jxg_frame_params *params, uint8_t *d_cells);
+
+/* Forward transform + quantization of every first block.  Replaces the
+ * transform/quantize half of write_ac_group (src/group.rs:1259-2240) in its
+ * first pass (no RDOQ): per channel the strategy's DCT, DC from the lowest
+ * frequencies, Y DC quantization, the Y quantize/dequantize round trip, Y LLF
+ * zeroing, CfL subtraction, X/B DC (with CfL) and X/B AC quantization.
+ * Outputs per channel c (X=0, Y=1, B=2):
+ *   d_levels[c]  int32 [yblocks*xblocks*64] quantized AC levels,
+ *   d_coeffs[c]  f32   same layout: Y = source coefficients before the
+ *                round trip, X/B = post-CfL coefficients (RDOQ inputs),
+ *   d_dc[c]      int16 [yblocks*xblocks] quantized DC.
+ * A transform covering cov_x x cov_y blocks at (bx, by) stores its
+ * coefficients (normalised so the long side is the row) in chunks of
+ * cov_x*64 values, chunk j at ((by+j)*xblocks + bx)*64. */
+int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
+                             uint32_t padded_h, uint32_t width, uint32_t height,
+                             const uint8_t *d_cells, const uint8_t *d_qf,
+                             const int8_t *d_ytox, const int8_t *d_ytob,
+                             const jxg_frame_params *params, int32_t *d_levels[3],
+                             float *d_coeffs[3], int16_t *d_dc[3]);
+
+/* The whole front end on device-resident input: XYB -> AQ -> strategy ->
+ * forward DCT + quantize.  CfL maps are inputs (all zero = the maps of an
+ * uncorrelated frame).  Scratch (XYB planes) is owned by the context. */
+int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channels,
+                      uint32_t width, uint32_t height, const jxg_frame_params *params,
+                      const int8_t *d_ytox, const int8_t *d_ytob, uint8_t *d_qf,
+                      uint8_t *d_cells, int32_t *d_levels[3], float *d_coeffs[3],
+                      int16_t *d_dc[3]);
+
+/* Host-buffer convenience form of jxg_front_end_dev for callers that own
+ * host memory (the reference's encode_image input is a host slice,
+ * src/encode_image.rs:1610): uploads, runs, downloads, synchronizes. */
+int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channels,
+                  uint32_t width, uint32_t height, const jxg_frame_params *params,
+                  const int8_t *ytox, const int8_t *ytob, uint8_t *qf, uint8_t *cells,
+                  int32_t *levels[3], float *coeffs[3], int16_t *dc[3]);
+
+/* Status of the last jxg_forward_quantize_dev on this context (synchronizes):
+ * out[0] = transforms whose footprint crossed their 64x64 tile (handled by
+ * the crossing pass), out[1] = 1 if a strategy cell was unsupported or did
+ * not fit the image (those transforms were not written).  The reference
+ * asserts on such cells (src/dc_group_data.rs:219-226); this ABI reports. */
+int jxg_quantize_status(jxg_ctx *ctx, uint32_t out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JXG_H_ */

@@ -1,0 +1,319 @@
+"""MI355X (gfx950) VarDCT encoder front end for jixel.
+
+The product is libjxg.so (C ABI, include/jxg.h): hand-written HIP kernels for
+sRGB -> XYB, the adaptive quant field, the AC-strategy search and forward
+DCT + quantization.  This package is the Python host mirror of the
+reference's configuration surface (EncodeConfig / Speed /
+distance_from_quality, src/encode_image.rs:326-1337) and a thin driver over
+the C ABI.  torch is used only to own device memory and streams.
+
+There is no CPU fallback: every entry point raises when libjxg.so or a
+gfx950 device is missing.
+"""
+import ctypes
+import enum
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from . import _lib
+from ._lib import FrameParams, JxgError, check, lib
+
+__all__ = [
+    "Speed", "EncodeConfig", "FrameParams", "JxgError", "FrontEnd", "FrontEndOutput",
+    "distance_from_quality", "frame_params", "default_quant_tables", "srgb_lut",
+    "plane_geometry", "qt_offset", "qt_size", "encode_front_end", "XYB_SPEC_FWD",
+    "STRATEGY",
+]
+
+
+class Speed(enum.IntEnum):
+    """Encoder speed tiers (src/encode_image.rs Speed, src/effort.rs:427)."""
+    ULTRAFAST = 0
+    FASTEST = 1
+    FAST = 2
+    MEDIUM = 3
+    MEDIUMSLOW = 4
+    SLOW = 5
+    EXTRASLOW = 6
+
+
+# Raw strategy codes (src/dc_group_data.rs:32-56).
+STRATEGY = dict(DCT=0, DCT16X8=1, DCT8X16=2, DCT16X16=3, DCT32X32=4, DCT4X4=5, DCT4X8=6,
+                DCT8X4=7, DCT32X16=8, DCT16X32=9, AFV0=10, AFV1=11, AFV2=12, AFV3=13,
+                DCT64X64=14, DCT64X32=15, DCT32X64=16, IDENTITY=17, DCT2X2=18, DCT128X128=19,
+                DCT128X64=20, DCT64X128=21, DCT32X8=22, DCT8X32=23)
+
+
+def distance_from_quality(quality):
+    """distance_from_quality (src/encode_image.rs:1321-1337); NaN is rejected."""
+    if quality != quality:
+        raise ValueError("quality must not be NaN")
+    return float(lib().jxg_distance_from_quality(float(quality)))
+
+
+@dataclass(frozen=True)
+class EncodeConfig:
+    """The front end's view of jixel's EncodeConfig (src/encode_image.rs:326-359).
+
+    `distance` defaults to 1.0 and `speed` to Fast, as the reference's
+    Default does (:682-740).  `fused` selects the reference build's fmla
+    semantics (aarch64 / FMA-enabled builds fuse, generic x86-64 does not);
+    `xyb_order` selects the scalar XYB kernel or the x86-64 AVX2 one.
+    """
+    distance: float = 1.0
+    speed: Speed = Speed.FAST
+    fused: bool = True
+    xyb_order: int = 0
+
+    def with_quality(self, quality):
+        return replace(self, distance=distance_from_quality(quality))
+
+    def with_distance(self, distance):
+        if not (distance > 0.0) or distance != distance:
+            raise JxgError(-4, f"distance {distance}")
+        return replace(self, distance=float(distance))
+
+    def with_speed(self, speed):
+        return replace(self, speed=Speed(speed))
+
+
+def frame_params(config: EncodeConfig) -> FrameParams:
+    """compute_distance_params + EncodingContext defaults for `config`."""
+    p = FrameParams()
+    check(lib().jxg_frame_params_init(float(config.distance), int(config.speed),
+                                      1 if config.fused else 0, ctypes.byref(p)))
+    return p
+
+
+def qt_size(kind):
+    return int(lib().jxg_qt_size(kind))
+
+
+def qt_offset(kind):
+    return int(lib().jxg_qt_offset(kind))
+
+
+QT_BLOB_FLOATS = 52224
+
+
+def default_quant_tables():
+    blob = np.zeros(QT_BLOB_FLOATS, np.float32)
+    check(lib().jxg_default_quant_tables(blob.ctypes.data))
+    return blob
+
+
+def srgb_lut():
+    out = np.zeros(256, np.float32)
+    lib().jxg_srgb_lut(out.ctypes.data)
+    return out
+
+
+def plane_geometry(width, height):
+    s, h = ctypes.c_uint32(), ctypes.c_uint32()
+    lib().jxg_plane_geometry(width, height, ctypes.byref(s), ctypes.byref(h))
+    return s.value, h.value
+
+
+def _spec_fwd():
+    # XybMatrix::SPEC (src/xyb.rs:35-47,81-96) in f32 arithmetic.
+    f = np.float32
+    m00, m02 = f(0.30), f(0.078)
+    m01 = f(f(1.0) - m02) - m00
+    m10, m12 = f(0.23), f(0.078)
+    m11 = f(f(1.0) - m12) - m10
+    b_bias = f(0.55180986)
+    ratio = f(0.24342269) / f(f(0.24342269) + f(0.20476745))
+    m20 = f(ratio * f(f(1.0) - b_bias))
+    m21 = f(f(f(1.0) - ratio) * f(f(1.0) - b_bias))
+    m22 = f(f(f(1.0) - m20) - m21)
+    return np.array([m00, m01, m02, m10, m11, m12, m20, m21, m22], np.float32)
+
+
+XYB_SPEC_FWD = _spec_fwd()
+
+
+@dataclass
+class FrontEndOutput:
+    """Per-image front-end results in the layouts of include/jxg.h."""
+    qf: np.ndarray      # (yblocks, xblocks) uint8, final quant field
+    cells: np.ndarray   # (yblocks, xblocks) uint8 strategy cells
+    levels: list        # 3 x int32 [yblocks*xblocks*64]
+    coeffs: list        # 3 x float32 [yblocks*xblocks*64]
+    dc: list            # 3 x int16 [yblocks*xblocks]
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class FrontEnd:
+    """A device context (jxg_ctx) plus torch-owned buffers for its stages."""
+
+    def __init__(self, device=0, stream=None, tables=None):
+        import torch
+        self.torch = torch
+        if not torch.cuda.is_available():
+            raise JxgError(-7, "no GPU visible to torch")
+        self.device = torch.device("cuda", device)
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        h = ctypes.c_void_p()
+        check(lib().jxg_ctx_create(device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h)))
+        self.ctx = h
+        if tables is not None:
+            self.set_tables(tables)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            lib().jxg_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tables(self, blob):
+        blob = np.ascontiguousarray(blob, np.float32)
+        assert blob.size == QT_BLOB_FLOATS
+        check(lib().jxg_ctx_set_tables(self.ctx, blob.ctypes.data))
+
+    def synchronize(self):
+        check(lib().jxg_ctx_synchronize(self.ctx))
+
+    # ---- buffers ----
+    def empty(self, shape, dtype):
+        return self.torch.empty(shape, dtype=dtype, device=self.device)
+
+    def xyb_buffer(self, width, height):
+        stride, ph = plane_geometry(width, height)
+        return self.empty((3, ph, stride), self.torch.float32)
+
+    def outputs(self, width, height):
+        t = self.torch
+        xb, yb = (width + 7) // 8, (height + 7) // 8
+        return dict(
+            levels=[self.empty(yb * xb * 64, t.int32) for _ in range(3)],
+            coeffs=[self.empty(yb * xb * 64, t.float32) for _ in range(3)],
+            dc=[self.empty(yb * xb, t.int16) for _ in range(3)],
+        )
+
+    # ---- stages ----
+    def xyb(self, rgb, width, height, fwd=None, lut=None, order=0, fused=True, out=None):
+        """rgb: (H, W, C) uint8 device tensor -> (3, padded_h, stride) XYB."""
+        fwd = XYB_SPEC_FWD if fwd is None else np.ascontiguousarray(fwd, np.float32)
+        lut = srgb_lut() if lut is None else np.ascontiguousarray(lut, np.float32)
+        out = self.xyb_buffer(width, height) if out is None else out
+        ch = rgb.shape[-1]
+        check(lib().jxg_xyb_dev(self.ctx, _ptr(rgb), rgb.stride(0) * rgb.element_size(), ch, width,
+                                height, fwd.ctypes.data, lut.ctypes.data, order, 1 if fused else 0,
+                                _ptr(out), out.shape[2], out.shape[1]))
+        return out
+
+    def fill_quant_field(self, xyb, width, height, distance, inv_scale, fused=True, out=None):
+        xb, yb = (width + 7) // 8, (height + 7) // 8
+        out = self.empty((yb, xb), self.torch.uint8) if out is None else out
+        check(lib().jxg_fill_quant_field_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
+                                             height, float(distance), float(inv_scale),
+                                             1 if fused else 0, _ptr(out)))
+        return out
+
+    def _cmaps(self, width, height, ytox, ytob):
+        t = self.torch
+        n = ((height + 63) // 64, (width + 63) // 64)
+        if ytox is None:
+            ytox = t.zeros(n, dtype=t.int8, device=self.device)
+        if ytob is None:
+            ytob = t.zeros(n, dtype=t.int8, device=self.device)
+        return ytox, ytob
+
+    def strategy_costs(self, xyb, width, height, qf, params, ytox=None, ytob=None):
+        t = self.torch
+        ytox, ytob = self._cmaps(width, height, ytox, ytob)
+        outs = [self.empty(qf.shape, t.float32) for _ in range(3)]
+        check(lib().jxg_strategy_costs_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
+                                           height, _ptr(qf), _ptr(ytox), _ptr(ytob),
+                                           ctypes.byref(params), *[_ptr(o) for o in outs]))
+        return outs
+
+    def select_strategy(self, xyb, width, height, qf, params, ytox=None, ytob=None, cells=None):
+        """Writes strategy cells; updates `qf` in place (adjust_quant_field)."""
+        ytox, ytob = self._cmaps(width, height, ytox, ytob)
+        cells = self.empty(qf.shape, self.torch.uint8) if cells is None else cells
+        check(lib().jxg_select_strategy_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
+                                            height, _ptr(qf), _ptr(ytox), _ptr(ytob),
+                                            ctypes.byref(params), _ptr(cells)))
+        return cells
+
+    def forward_quantize(self, xyb, width, height, cells, qf, params, ytox=None, ytob=None,
+                         outputs=None):
+        ytox, ytob = self._cmaps(width, height, ytox, ytob)
+        o = self.outputs(width, height) if outputs is None else outputs
+        arr = lambda ts: (ctypes.c_void_p * 3)(*[t.data_ptr() for t in ts])
+        check(lib().jxg_forward_quantize_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
+                                             height, _ptr(cells), _ptr(qf), _ptr(ytox), _ptr(ytob),
+                                             ctypes.byref(params), arr(o["levels"]),
+                                             arr(o["coeffs"]), arr(o["dc"])))
+        return o
+
+    def quantize_status(self):
+        out = (ctypes.c_uint32 * 2)()
+        check(lib().jxg_quantize_status(self.ctx, out))
+        return int(out[0]), int(out[1])
+
+    def run(self, rgb, width, height, params, ytox=None, ytob=None, outputs=None, qf=None,
+            cells=None):
+        """The whole front end on a device-resident (H, W, C) uint8 image."""
+        t = self.torch
+        xb, yb = (width + 7) // 8, (height + 7) // 8
+        ytox, ytob = self._cmaps(width, height, ytox, ytob)
+        o = self.outputs(width, height) if outputs is None else outputs
+        qf = self.empty((yb, xb), t.uint8) if qf is None else qf
+        cells = self.empty((yb, xb), t.uint8) if cells is None else cells
+        arr = lambda ts: (ctypes.c_void_p * 3)(*[x.data_ptr() for x in ts])
+        check(lib().jxg_front_end_dev(self.ctx, _ptr(rgb), rgb.stride(0) * rgb.element_size(),
+                                      rgb.shape[-1], width, height, ctypes.byref(params),
+                                      _ptr(ytox), _ptr(ytob), _ptr(qf), _ptr(cells),
+                                      arr(o["levels"]), arr(o["coeffs"]), arr(o["dc"])))
+        o = dict(o)
+        o["qf"] = qf
+        o["cells"] = cells
+        return o
+
+
+def encode_front_end(rgb, config: EncodeConfig = EncodeConfig(), ytox=None, ytob=None,
+                     device=0, tables=None) -> FrontEndOutput:
+    """Host-buffer front end of one RGB(A)8 image (H, W, 3|4) on `device`.
+
+    The drop-in counterpart of the front-end half of jixel's encode_image /
+    encode_image_with_alpha (src/encode_image.rs:1610,3482): the returned
+    buffers are what the CPU back end (RDOQ, tokenization, entropy coding)
+    consumes.
+    """
+    rgb = np.ascontiguousarray(rgb, np.uint8)
+    if rgb.ndim != 3 or rgb.shape[2] not in (3, 4):
+        raise JxgError(-1, f"expected (H, W, 3|4) uint8, got {rgb.shape}")
+    h, w, ch = rgb.shape
+    if h == 0 or w == 0:
+        raise JxgError(-2, "empty image")
+    fe = FrontEnd(device, tables=tables)
+    try:
+        p = frame_params(config)
+        xb, yb = (w + 7) // 8, (h + 7) // 8
+        tiles = ((h + 63) // 64) * ((w + 63) // 64)
+        qf = np.zeros((yb, xb), np.uint8)
+        cells = np.zeros((yb, xb), np.uint8)
+        levels = [np.zeros(yb * xb * 64, np.int32) for _ in range(3)]
+        coeffs = [np.zeros(yb * xb * 64, np.float32) for _ in range(3)]
+        dc = [np.zeros(yb * xb, np.int16) for _ in range(3)]
+        tx = None if ytox is None else np.ascontiguousarray(ytox, np.int8).reshape(tiles)
+        tb = None if ytob is None else np.ascontiguousarray(ytob, np.int8).reshape(tiles)
+        arr = lambda xs: (ctypes.c_void_p * 3)(*[x.ctypes.data for x in xs])
+        check(lib().jxg_front_end(fe.ctx, rgb.ctypes.data, w * ch, ch, w, h, ctypes.byref(p),
+                                  None if tx is None else tx.ctypes.data,
+                                  None if tb is None else tb.ctypes.data, qf.ctypes.data,
+                                  cells.ctypes.data, arr(levels), arr(coeffs), arr(dc)))
+        return FrontEndOutput(qf=qf, cells=cells, levels=levels, coeffs=coeffs, dc=dc)
+    finally:
+        fe.close()

@@ -1,0 +1,46 @@
+"""Builds the gfx950 front-end library (jixel_amd/libjxg.so) in-tree.
+
+hipcc compiles the kernels for gfx950 only; the host-side parameter code is
+plain C++.  Floating-point contraction is disabled everywhere so that every
+a*b+c is rounded exactly as the reference rounds it, and packed-FP32
+instructions are disabled for the device code.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libjxg.so")
+BUILD = os.path.join(HERE, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",
+          "-I", os.path.join(ROOT, "include")]
+DEVICE = ["--offload-arch=gfx950", "-fno-gpu-flush-denormals-to-zero",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    params_o = os.path.join(BUILD, "jxg_params.o")
+    kernels_o = os.path.join(BUILD, "jxg_kernels.o")
+    run([CXX] + COMMON + ["-c", os.path.join(CSRC, "jxg_params.cpp"), "-o", params_o])
+    run([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, "jxg_kernels.hip"), "-o", kernels_o])
+    tmp = LIB + ".tmp"
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp, kernels_o, params_o])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

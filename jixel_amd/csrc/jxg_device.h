@@ -1,0 +1,319 @@
+// jxg_device.h - gfx950 device helpers of the VarDCT front end: the
+// reference's scalar arithmetic (fmla, ties-away rounding, saturating
+// casts, the bit-trick cube root, fast_exp2, dirty_log2f) and the scaled
+// DCT-II butterflies, all in the reference's operation order.
+//
+// The whole library is compiled with -ffp-contract=off; FUSED selects
+// whether the reference's fmla (src/dct.rs:33-53) is a fused multiply-add.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "jxg_internal.h"
+
+namespace jxg {
+
+template <bool F>
+__device__ __forceinline__ float fmla(float a, float b, float c) {
+  if constexpr (F)
+    return __builtin_fmaf(a, b, c);
+  else
+    return a * b + c;
+}
+
+__device__ __forceinline__ float bits_f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t f_bits(float f) { return __float_as_uint(f); }
+
+// Rust `as i32` / `as i16`: saturating, NaN -> 0.
+__device__ __forceinline__ int32_t sat_i32(float v) {
+  if (v != v) return 0;
+  if (v >= 2147483648.0f) return INT32_MAX;
+  if (v <= -2147483648.0f) return INT32_MIN;
+  return (int32_t)v;
+}
+__device__ __forceinline__ int16_t sat_i16(float v) {
+  if (v != v) return 0;
+  if (v >= 32767.0f) return INT16_MAX;
+  if (v <= -32768.0f) return INT16_MIN;
+  return (int16_t)v;
+}
+
+// f32::round: ties away from zero.
+__device__ __forceinline__ float round_away(float x) { return __builtin_roundf(x); }
+
+// ---- cube roots (src/xyb.rs:146-170; src/avx/xyb.rs:128-205) ----
+template <bool F>
+__device__ __forceinline__ float halley(float x, float a) {
+  float tx = x * x * x;
+  return x * fmla<F>(2.0f, a, tx) / fmla<F>(2.0f, tx, a);
+}
+
+template <bool F>
+__device__ __forceinline__ float cbrt_scalar(float x) {
+  if (x == 0.0f) return x;
+  uint32_t ui = f_bits(x);
+  uint32_t hx = (ui & 0x7fffffffu) / 3u + 709958130u;
+  float t = bits_f((ui & 0x80000000u) | hx);
+  t = halley<F>(t, x);
+  return halley<F>(t, x);
+}
+
+__device__ __forceinline__ float cbrt_avx2_positive(float a) {
+  uint32_t hx = f_bits(a) & 0x7fffffffu;
+  float x = bits_f((uint32_t)(((uint64_t)hx * 341u) >> 10) + 709958130u);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    float tx = (x * x) * x;
+    float num = __builtin_fmaf(a, 2.0f, tx);
+    float den = __builtin_fmaf(tx, 2.0f, a);
+    x = x * (num / den);
+  }
+  return a == 0.0f ? 0.0f : x;
+}
+
+// ---- fast_exp2 / dirty_log2f / fast_exp (src/adaptive_quant.rs:739-784,
+//      src/util.rs:342-357) ----
+template <bool F>
+__device__ __forceinline__ float fast_exp2(float d, const float *table) {
+  const float redux = 196608.0f;  // f32::from_bits(0x4b400000) / 64
+  uint32_t ui = f_bits(d + redux);
+  uint32_t i0 = ui + 32u;
+  uint32_t k = i0 / 64u;
+  i0 &= 63u;
+  float uf = bits_f(ui) - redux;
+  float z0 = table[i0];
+  float f = d - uf;
+  float u = 0.24022650695908768f;
+  u = fmla<F>(u, f, 0.69314718055994973f);
+  u *= f;
+  float i2 = bits_f(((uint32_t)((int32_t)k + 0x7f)) << 23);
+  return fmla<F>(u, z0, z0) * i2;
+}
+
+template <bool F>
+__device__ __forceinline__ float dirty_log2f(float d) {
+  uint32_t ix = f_bits(d) + (0x3f800000u - 0x3f3504f3u);
+  int32_t n = (int32_t)(ix >> 23) - 0x7f;
+  ix = (ix & 0x007fffffu) + 0x3f3504f3u;
+  float a = bits_f(ix);
+  float x = (a - 1.0f) / (a + 1.0f);
+  float x2 = x * x;
+  float u = 0.4121985850084821691e+0f;
+  u = fmla<F>(u, x2, 0.5770780163490337802e+0f);
+  u = fmla<F>(u, x2, 0.9617966939259845749e+0f);
+  return fmla<F>(x2 * x, u, fmla<F>(x, 0.2885390081777926802e+1f, (float)n));
+}
+
+__device__ __forceinline__ int32_t round_i32_portable(float v) {
+  float m = fabsf(v);
+  float b = (m >= 0.5f && m < 8388608.0f) ? v + copysignf(0.5f, v) : v;
+  return sat_i32(b);
+}
+
+template <bool F>
+__device__ __forceinline__ float fast_exp(float x) {
+  int32_t e = round_i32_portable(x * 1.44269504088896340736f);
+  float r = fmla<F>((float)e, -0.69314575f, x);
+  r = fmla<F>((float)e, -1.4286068e-6f, r);
+  float sq = r * r;
+  float p = fmla<F>(-0.002775669f, sq, 0.16666707f);
+  p = fmla<F>(p, sq, 2.0f);
+  float scale = bits_f(((uint32_t)(e + 127)) << 23);
+  return (1.0f + 2.0f * r / (p - r)) * scale;
+}
+
+// ---- scaled DCT-II butterflies (src/dct.rs:68-127,687-720,1477-1500) ----
+#define JXG_SQRT2 1.41421356237309504880f
+
+template <bool F>
+__device__ __forceinline__ void dct4(float *b) {
+  float t0 = b[0] + b[3], t1 = b[1] + b[2];
+  float e0 = t0 + t1, e1 = t0 - t1;
+  float t2 = (b[0] - b[3]) * 0.5411961f;
+  float t3 = (b[1] - b[2]) * 1.306563f;
+  float o0 = t2 + t3, o1 = t2 - t3;
+  o0 = fmla<F>(o0, JXG_SQRT2, o1);
+  b[0] = e0;
+  b[2] = e1;
+  b[1] = o0;
+  b[3] = o1;
+}
+
+template <bool F>
+__device__ __forceinline__ void dct8(float *b) {
+  const float wc[4] = {0.5097956f, 0.6013449f, 0.8999762f, 2.5629156f};
+  float t[8];
+#pragma unroll
+  for (int i = 0; i < 4; i++) t[i] = b[i] + b[7 - i];
+  dct4<F>(t);
+#pragma unroll
+  for (int i = 0; i < 4; i++) t[4 + i] = (b[i] - b[7 - i]) * wc[i];
+  dct4<F>(t + 4);
+  t[4] = fmla<F>(t[4], JXG_SQRT2, t[5]);
+  t[5] += t[6];
+  t[6] += t[7];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    b[2 * i] = t[i];
+    b[2 * i + 1] = t[4 + i];
+  }
+}
+
+template <bool F>
+__device__ __forceinline__ void dct16(float *b) {
+  const float wc[8] = {0.5024193f, 0.5224986f, 0.566944f,  0.6468218f,
+                       0.7881546f, 1.0606777f, 1.7224471f, 5.1011486f};
+  float t[16];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    t[i] = b[i] + b[15 - i];
+    t[8 + i] = b[i] - b[15 - i];
+  }
+  dct8<F>(t);
+#pragma unroll
+  for (int i = 0; i < 8; i++) t[8 + i] *= wc[i];
+  dct8<F>(t + 8);
+  t[8] = fmla<F>(t[8], JXG_SQRT2, t[9]);
+#pragma unroll
+  for (int i = 9; i < 15; i++) t[i] += t[i + 1];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    b[2 * i] = t[i];
+    b[2 * i + 1] = t[8 + i];
+  }
+}
+
+template <bool F>
+__device__ __forceinline__ void dct32(float *b) {
+  const float wc[16] = {0.50060298f, 0.50547096f, 0.5154473f, 0.5310426f,
+                        0.5531039f,  0.582935f,   0.6225041f, 0.67480834f,
+                        0.7445363f,  0.83934965f, 0.97256825f, 1.16944f,
+                        1.4841646f,  2.057781f,   3.4076084f, 10.190008f};
+  float t[32];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    t[i] = b[i] + b[31 - i];
+    t[16 + i] = b[i] - b[31 - i];
+  }
+  dct16<F>(t);
+#pragma unroll
+  for (int i = 0; i < 16; i++) t[16 + i] *= wc[i];
+  dct16<F>(t + 16);
+  t[16] = fmla<F>(t[16], JXG_SQRT2, t[17]);
+#pragma unroll
+  for (int i = 17; i < 31; i++) t[i] += t[i + 1];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    b[2 * i] = t[i];
+    b[2 * i + 1] = t[16 + i];
+  }
+}
+
+// One line transform of length N held in LDS with element stride `step`,
+// optionally scaled afterwards (the last pass of a 2-D assembly multiplies
+// by 1/(W*H)).
+template <bool F, int N>
+__device__ __forceinline__ void dct_line(float *p, int step, float scale, bool do_scale) {
+  float b[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) b[i] = p[i * step];
+  if constexpr (N == 8) dct8<F>(b);
+  if constexpr (N == 16) dct16<F>(b);
+  if constexpr (N == 32) dct32<F>(b);
+  if (do_scale) {
+#pragma unroll
+    for (int i = 0; i < N; i++) b[i] = b[i] * scale;
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++) p[i * step] = b[i];
+}
+
+template <bool F>
+__device__ __forceinline__ void dct_line_n(int n, float *p, int step, float scale, bool do_scale) {
+  if (n == 8)
+    dct_line<F, 8>(p, step, scale, do_scale);
+  else if (n == 16)
+    dct_line<F, 16>(p, step, scale, do_scale);
+  else
+    dct_line<F, 32>(p, step, scale, do_scale);
+}
+
+// idct1d_4 (src/dct.rs:2144-2168)
+template <bool F>
+__device__ __forceinline__ void idct4(const float v[4], float out[4]) {
+  float t0 = v[0], t1 = v[2], t2 = v[1], t3 = v[3];
+  float e0 = t0 + t1, e1 = t0 - t1;
+  t3 += t2;
+  t2 *= JXG_SQRT2;
+  float o0 = t2 + t3, o1 = t2 - t3;
+  out[0] = fmla<F>(0.5411961f, o0, e0);
+  out[3] = fmla<F>(-0.5411961f, o0, e0);
+  out[1] = fmla<F>(1.306563f, o1, e1);
+  out[2] = fmla<F>(-1.306563f, o1, e1);
+}
+
+// ---- strategy geometry (src/dc_group_data.rs:140-170) ----
+__device__ __forceinline__ int cov_x(int s) {
+  const uint8_t t[24] = {1, 1, 2, 2, 4, 1, 1, 1, 2, 4, 1, 1, 1, 1, 8, 4, 8, 1, 1, 16, 8, 16, 1, 4};
+  return t[s];
+}
+__device__ __forceinline__ int cov_y(int s) {
+  const uint8_t t[24] = {1, 2, 1, 2, 4, 1, 1, 1, 4, 2, 1, 1, 1, 1, 8, 8, 4, 1, 1, 16, 16, 8, 4, 1};
+  return t[s];
+}
+
+// Whether the 2-D assembly transforms rows first (src/dct.rs: 8x8, 8x16,
+// 16x32, 8x32) or columns first (16x8, 16x16, 32x32, 32x16, 32x8).
+__device__ __forceinline__ bool rows_first(int s) {
+  return s == JXG_DCT || s == JXG_DCT8X16 || s == JXG_DCT16X32 || s == JXG_DCT8X32;
+}
+
+// Storage index of the coefficient with vertical frequency v and horizontal
+// frequency u of a w x h footprint (see the 2-D assemblies of src/dct.rs):
+// the square 8x8 stores u-major, wide shapes v-major, column-first shapes
+// u-major.
+__device__ __forceinline__ int storage_index(int s, int w, int h, int v, int u) {
+  if (s == JXG_DCT) return u * 8 + v;
+  if (rows_first(s)) return v * w + u;
+  return u * h + v;
+}
+
+// Spatial (v, u) of storage index k (inverse of storage_index).
+__device__ __forceinline__ void storage_to_spatial(int s, int w, int h, int k, int *v, int *u) {
+  if (s == JXG_DCT) {
+    *u = k >> 3;
+    *v = k & 7;
+  } else if (rows_first(s)) {
+    *v = k / w;
+    *u = k % w;
+  } else {
+    *u = k / h;
+    *v = k % h;
+  }
+}
+
+// DequantBias::level / level_f32 (src/group.rs:748-790)
+__device__ __forceinline__ float dequant_level(float one, float num, int32_t q) {
+  uint32_t a = q < 0 ? (uint32_t)(-(int64_t)q) : (uint32_t)q;
+  if (a <= 1) return q == 0 ? 0.0f : (q > 0 ? one : -one);
+  float qf = (float)q;
+  if (a < 64) {
+    float t = num / (float)a;
+    return qf - (q < 0 ? -t : t);
+  }
+  return qf - num / qf;
+}
+
+__device__ __forceinline__ float level_f32(float one, float num, float q) {
+  float aq = fabsf(q);
+  if (aq < 1.125f) return q == 0.0f ? 0.0f : copysignf(one, q);
+  uint32_t a = (uint32_t)aq;
+  if (a < 64 && (float)a == aq) {
+    float t = num / (float)a;
+    return q - (q < 0.0f ? -t : t);
+  }
+  return q - num / q;
+}
+
+}  // namespace jxg

@@ -1,0 +1,1707 @@
+// jxg_kernels.hip - gfx950 kernels of the VarDCT encoder front end and the
+// C ABI declared in include/jxg.h.
+//
+// Stage kernels (one launch per stage per image):
+//   k_xyb          u8 sRGB -> LUT linear -> opsin mix -> cube root -> XYB.
+//                  One thread per output pixel of the padded plane; padding
+//                  is edge replication, so later stages need no bounds checks.
+//   k_aq_pre       per 4x4 pixel cell: luma masking pre-pass, 4x4 average.
+//   k_aq_block     per 8x8 block: fuzzy erosion of the pre map (clamped to the
+//                  block's 2048-px DC group) and the per-block modulations.
+//   k_select       one wavefront per 32x32-px region: DCT8 / DCT16X16 /
+//                  DCT32X32 candidate costs (transform, CfL, quantize, SSE and
+//                  rate in LDS, sums in the reference's sequential order), the
+//                  square-merge decisions and the quant-field aggregation.
+//   k_quantize     one 256-thread workgroup per 64x64-px tile: every transform
+//                  whose footprint lies in the tile is transformed in LDS and
+//                  quantized; transforms that cross the tile are queued.
+//   k_quantize_xt  persistent pass over the queued tile-crossing transforms.
+//
+// All arithmetic follows the reference's scalar operation order; see
+// jxg_device.h.  No float atomics are used, so results do not depend on the
+// launch shape.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+
+#include "jxg_device.h"
+#include "jxg_internal.h"
+
+using namespace jxg;
+
+// ---------------------------------------------------------------------------
+// errors
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return set_err(e_ == hipErrorOutOfMemory ? JXG_ERR_ALLOC : JXG_ERR_DEVICE, "%s: %s", \
+                     #expr, hipGetErrorString(e_));                                      \
+  } while (0)
+
+#define LAUNCH_CHECK(name)                                                               \
+  do {                                                                                   \
+    hipError_t e_ = hipGetLastError();                                                   \
+    if (e_ != hipSuccess)                                                                \
+      return set_err(JXG_ERR_DEVICE, "launch %s: %s", name, hipGetErrorString(e_));      \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// XYB
+
+template <bool F, int ORDER>
+__global__ __launch_bounds__(256) void k_xyb(const uint8_t *__restrict__ rgb, size_t row_bytes,
+                                             int ch, int W, int H,
+                                             const float *__restrict__ mlut,
+                                             float *__restrict__ xyb, int stride, int padded_h) {
+  __shared__ float lut[256];
+  __shared__ float m[9];
+  const int t = threadIdx.x;
+  lut[t] = mlut[9 + t];
+  if (t < 9) m[t] = mlut[t];
+  __syncthreads();
+  const int x = blockIdx.x * 256 + t;
+  const int y = blockIdx.y;
+  if (x >= stride) return;
+  const int sx = min(x, W - 1), sy = min(y, H - 1);
+  const uint8_t *p = rgb + (size_t)sy * row_bytes + (size_t)sx * ch;
+  const float r = lut[p[0]], g = lut[p[1]], b = lut[p[2]];
+  const float bias = 0.0037930734f, neg = -0.1559542f;
+  float tm[3];
+#pragma unroll
+  for (int row = 0; row < 3; row++) {
+    const float m0 = m[3 * row], m1 = m[3 * row + 1], m2 = m[3 * row + 2];
+    if constexpr (ORDER == JXG_XYB_AVX2) {
+      // src/avx/xyb.rs:234-262 (SCALAR_ORDER = false, FUSED = true)
+      float mixed = __builtin_fmaf(b, m2, bias);
+      mixed = __builtin_fmaf(g, m1, mixed);
+      mixed = __builtin_fmaf(r, m0, mixed);
+      tm[row] = cbrt_avx2_positive(fmaxf(mixed, 0.0f)) + neg;
+    } else {
+      // src/xyb.rs:172-185
+      float mixed = fmla<F>(m0, r, fmla<F>(m1, g, fmla<F>(m2, b, bias)));
+      tm[row] = cbrt_scalar<F>(fmaxf(mixed, 0.0f)) + neg;
+    }
+  }
+  float X, Y;
+  if constexpr (ORDER == JXG_XYB_AVX2) {
+    X = (tm[0] - tm[1]) * 0.5f;
+    Y = (tm[0] + tm[1]) * 0.5f;
+  } else {
+    X = 0.5f * (tm[0] - tm[1]);
+    Y = 0.5f * (tm[0] + tm[1]);
+  }
+  const size_t plane = (size_t)stride * padded_h;
+  const size_t i = (size_t)y * stride + x;
+  xyb[i] = X;
+  xyb[plane + i] = Y;
+  xyb[2 * plane + i] = tm[2];
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive quantization (src/adaptive_quant.rs:41-206,410-470,547-729)
+
+template <bool INVERT>
+__device__ __forceinline__ float ratio_gamma(float v) {
+  const float k_sg_mul = 226.77216153508914f;
+  const float k_sg_mul2 = 1.0f / 73.377132366608819f;
+  const float k_log2 = 0.693147181f;
+  const float k_ret_mul = k_sg_mul2 * 18.6580932135f * k_log2;
+  const float k_eps = 1e-2f;
+  if (v < 0.0f) v = 0.0f;
+  const float k_num_mul = k_ret_mul * 3.0f * k_sg_mul;
+  const float k_v_offset = 7.7825991679894591f * k_log2 + k_eps;
+  const float k_den_mul = k_log2 * k_sg_mul;
+  float v2 = v * v;
+  float num = k_num_mul * v2 + k_eps;
+  float den = (k_den_mul * v) * v2 + k_v_offset;
+  return INVERT ? num / den : den / num;
+}
+
+template <bool F>
+__device__ __forceinline__ float masking_sqrt(float v) {
+  const float mul_v = 211.66567973503678f * 1e8f;
+  return 0.25f * sqrtf(fmla<F>(v, sqrtf(mul_v), 27.505837037000106f));
+}
+
+template <bool F>
+__global__ __launch_bounds__(256) void k_aq_pre(const float *__restrict__ yp, int stride, int W,
+                                                int H, int pre_w, int pre_h,
+                                                float *__restrict__ pre) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= pre_w || j >= pre_h) return;
+  float acc[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int gy = 4 * j + r;
+    const int gyc = min(gy, H - 1), gy1 = min(max(gy - 1, 0), H - 1), gy2 = min(gy + 1, H - 1);
+    const float *row = yp + (size_t)gyc * stride;
+    const float *row1 = yp + (size_t)gy1 * stride;
+    const float *row2 = yp + (size_t)gy2 * stride;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int gx = 4 * i + c;
+      const int gxc = min(gx, W - 1), gx1 = min(max(gx - 1, 0), W - 1), gx2 = min(gx + 1, W - 1);
+      const float in_y = row[gxc];
+      const float base = 0.25f * (row2[gxc] + row1[gxc] + row[gx1] + row[gx2]);
+      const float gammac = ratio_gamma<false>(in_y + 0.019f);
+      float diff = gammac * (in_y - base);
+      diff *= diff;
+      if (diff >= 0.2f) diff = 0.2f;
+      diff = masking_sqrt<F>(diff);
+      acc[c] = r == 0 ? diff : acc[c] + diff;
+    }
+  }
+  pre[(size_t)j * pre_w + i] = (acc[0] + acc[1] + acc[2] + acc[3]) * 0.25f;
+}
+
+__device__ __forceinline__ void sort4(float *a) {
+#define JXG_CSWAP(i, j)    \
+  if (a[i] > a[j]) {       \
+    float t_ = a[i];       \
+    a[i] = a[j];           \
+    a[j] = t_;             \
+  }
+  JXG_CSWAP(0, 1);
+  JXG_CSWAP(2, 3);
+  JXG_CSWAP(0, 2);
+  JXG_CSWAP(1, 3);
+  JXG_CSWAP(1, 2);
+#undef JXG_CSWAP
+}
+
+__device__ __forceinline__ void store_min4(float v, float *m) {
+  if (v < m[3]) {
+    if (v < m[0]) {
+      m[3] = m[2];
+      m[2] = m[1];
+      m[1] = m[0];
+      m[0] = v;
+    } else if (v < m[1]) {
+      m[3] = m[2];
+      m[2] = m[1];
+      m[1] = v;
+    } else if (v < m[2]) {
+      m[3] = m[2];
+      m[2] = v;
+    } else {
+      m[3] = v;
+    }
+  }
+}
+
+template <bool F>
+__global__ __launch_bounds__(64) void k_aq_block(const float *__restrict__ xyb, int stride,
+                                                 int padded_h, int W, int H,
+                                                 const float *__restrict__ pre, int pre_w,
+                                                 float distance, float inv_scale,
+                                                 const float *__restrict__ exp2_table,
+                                                 uint8_t *__restrict__ qf, int xb, int yb) {
+  const int bx = blockIdx.x * 64 + threadIdx.x;
+  const int by = blockIdx.y;
+  if (bx >= xb || by >= yb) return;
+  const float d = distance;
+  // DC group of this block and its pre-map extent.
+  const int gx = bx >> 8, gy = by >> 8;
+  const int gxb = min(256, xb - gx * 256), gyb = min(256, yb - gy * 256);
+  const int pw = 2 * gxb, ph = 2 * gyb;
+  const int px0 = gx * 512, py0 = gy * 512;
+  // FuzzyErosion weights
+  const float fe_mul = d < 2.0f ? (2.0f - d) * 0.5f : 0.0f;
+  const float fe_base[4] = {0.125f, 0.1f, 0.09f, 0.06f};
+  const float fe_add[4] = {0.0f, -0.1f, -0.09f, -0.06f};
+  float kmul[4];
+  float norm = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    kmul[i] = fe_base[i] + fe_mul * fe_add[i];
+    norm += kmul[i];
+  }
+  const float k_total = 0.29959705784054957f;
+#pragma unroll
+  for (int i = 0; i < 4; i++) kmul[i] *= k_total / norm;
+  float aq = 0.0f;
+  for (int q = 0; q < 4; q++) {
+    const int fy = 2 * (by - gy * 256) + (q >> 1);
+    const int fx = 2 * (bx - gx * 256) + (q & 1);
+    const int ym1 = fy >= 1 ? fy - 1 : fy, yp1 = fy + 1 < ph ? fy + 1 : fy;
+    const int xm1 = fx >= 1 ? fx - 1 : fx, xp1 = fx + 1 < pw ? fx + 1 : fx;
+    const float *rt = pre + (size_t)(py0 + ym1) * pre_w + px0;
+    const float *rm = pre + (size_t)(py0 + fy) * pre_w + px0;
+    const float *rb = pre + (size_t)(py0 + yp1) * pre_w + px0;
+    float mins[4] = {rm[fx], rm[xm1], rm[xp1], rt[xm1]};
+    sort4(mins);
+    store_min4(rt[fx], mins);
+    store_min4(rt[xp1], mins);
+    store_min4(rb[xm1], mins);
+    store_min4(rb[fx], mins);
+    store_min4(rb[xp1], mins);
+    const float v = kmul[0] * mins[0] + kmul[1] * mins[1] + kmul[2] * mins[2] + kmul[3] * mins[3];
+    aq = q == 0 ? v : aq + v;
+  }
+  // Stage 3: per-block modulations.
+  const size_t plane = (size_t)stride * padded_h;
+  const float *px_ = xyb, *py_ = xyb + plane, *pb_ = xyb + 2 * plane;
+  const int x = min(bx * 8, max(W - 8, 0));
+  const int y = min(by * 8, max(H - 8, 0));
+  // compute_mask
+  float mask;
+  {
+    const float k_offset3 = 3.7179635626140772f;
+    const float v1 = fmaxf(aq * 1.07267f, 1e-3f);
+    const float v2 = 1.0f / (v1 + 302.59587815579727f);
+    const float v3 = 1.0f / (v1 * v1 + k_offset3);
+    const float v4 = 1.0f / (v1 * v1 + 0.25f * k_offset3);
+    mask = fmla<F>(9.4708735624378946f, v4,
+                   fmla<F>(17.35036561631863f, v2, fmla<F>(6.7943250517376494f, v3, -0.699765f)));
+  }
+  // gamma_modulation
+  {
+    float overall = 0.0f;
+    for (int dy = 0; dy < 8; dy++) {
+      const int ry = min(y + dy, H - 1);
+      for (int dx = 0; dx < 8; dx++) {
+        const int cx = min(x + dx, W - 1);
+        const size_t i = (size_t)ry * stride + cx;
+        const float iny = py_[i] + 0.16f;
+        const float inx = px_[i];
+        overall = fmla<F>(0.5f, ratio_gamma<true>(iny - inx) + ratio_gamma<true>(iny + inx), overall);
+      }
+    }
+    overall *= 1.0f / 64.0f;
+    mask = fmla<F>(0.119146f, dirty_log2f<F>(overall), mask);
+  }
+  // hf_modulation
+  float out_val;
+  {
+    const float strength = fminf(fmaxf(1.06836f * d, 0.1f), 0.890398f);
+    float sum_y = 0.0f;
+    for (int dy = 0; dy < 8; dy++) {
+      const int ry = min(y + dy, H - 1);
+      const int rn = dy == 7 ? ry : min(y + dy + 1, H - 1);
+      const float *row = py_ + (size_t)ry * stride;
+      const float *nxt = py_ + (size_t)rn * stride;
+      for (int dx = 0; dx < 8; dx++) {
+        const int c0 = min(x + dx, W - 1);
+        const float p = row[c0];
+        if (dx < 7) sum_y += fminf(0.0206f, fabsf(p - row[min(x + dx + 1, W - 1)]));
+        sum_y += fminf(0.0206f, fabsf(p - nxt[c0]));
+      }
+    }
+    out_val = fmla<F>(sum_y, -0.506626f * strength, 0.655866f) + mask;
+  }
+  // blue_modulation
+  {
+    const float k_limit = 0.010474084867598155f;
+    float sum = 0.0f;
+    for (int dy = 0; dy < 8; dy++) {
+      const int ry = min(y + dy, H - 1);
+      for (int dx = 0; dx < 8; dx++) {
+        const int cx = min(x + dx, W - 1);
+        const size_t i = (size_t)ry * stride + cx;
+        const float p_x = px_[i], p_b = pb_[i];
+        const float p_y = py_[i] + 0.0031994768654636393f + fabsf(p_x);
+        if (p_b > p_y) sum += fminf(p_b - p_y, k_limit);
+      }
+    }
+    float s = sum;
+    if (s >= 32.0f * k_limit) s = 64.0f * k_limit - s;
+    const float k_max = 15.463398341612438f;
+    if (s >= k_max * k_limit) s = k_max * k_limit;
+    s *= 0.127191f;
+    out_val = fminf(out_val, s + mask);
+  }
+  // quant field (fill_quant_field_scalar stage 3)
+  const float scale = 0.8294f / d;
+  // aq_schedule of the base level and strength
+  const float knots[7] = {0.5f, 1.0f, 1.4f, 1.75f, 3.0f, 5.0f, 8.0f};
+  const float strength_t[7] = {0.26565f, 0.275459f, 0.058287f, 0.0f, 0.0f, 0.0f, 0.0f};
+  const float base_t[7] = {0.636493f, 0.648351f, 0.606759f, 0.48f, 0.48f, 0.48f, 0.48f};
+  float base_level = base_t[6], dampen = strength_t[6];
+  if (d <= knots[0]) {
+    base_level = base_t[0];
+    dampen = strength_t[0];
+  } else {
+    for (int i = 1; i < 7; i++)
+      if (d <= knots[i]) {
+        const float f = (d - knots[i - 1]) / (knots[i] - knots[i - 1]);
+        base_level = base_t[i - 1] + f * (base_t[i] - base_t[i - 1]);
+        dampen = strength_t[i - 1] + f * (strength_t[i] - strength_t[i - 1]);
+        break;
+      }
+  }
+  dampen = fminf(fmaxf(dampen, 0.0f), 1.0f);
+  base_level = base_level * scale;
+  const float mul = scale * dampen;
+  const float add = (1.0f - dampen) * base_level;
+  const float qv = fast_exp2<F>(out_val * 1.442695041f, exp2_table) * mul + add;
+  int32_t qi = sat_i32(fmla<F>(qv, inv_scale, 0.5f));
+  qi = qi < 1 ? 1 : (qi > 255 ? 255 : qi);
+  qf[(size_t)by * xb + bx] = (uint8_t)qi;
+}
+
+// ---------------------------------------------------------------------------
+// Candidate costs and square-merge selection
+
+#define SEL_PITCH 33  // LDS row pitch of the 32x32 region (bank-conflict free)
+#define RD_LAMBDA 0.08086717f
+#define META_R 4.0f
+
+struct SelLds {
+  float co[3][32 * SEL_PITCH];  // coefficients, spatial layout
+  float mg[3][1024];            // magnitude-rate terms, storage order
+  float sse[16][3], mag[16][3];
+  uint32_t nz[16][3], maxscan[16][3];
+  float cost8[16], cost16[4], cost32;
+  double site_s1[5], site_s2[5];
+  uint32_t site_cnt[5];
+  float qscale;
+};
+
+__device__ __forceinline__ int sel_scan_size(int n) { return n * n; }
+
+// Natural scan position of raw index k of an n x n block (src/coeff_order.rs
+// natural_scan_generic with cx == cy): the table lives in device memory.
+
+// Candidate evaluation over the whole region for square size n (8, 16, 32):
+// all (32/n)^2 sites at once.  qac[site] is the site's quantizer.
+template <bool F>
+__device__ void eval_squares(SelLds &L, int n, const float *__restrict__ xyb, size_t plane,
+                             int stride, int px0, int py0, const float *qac_in, float cmx,
+                             float cmb, const float *__restrict__ tables,
+                             const uint32_t *__restrict__ scan, const jxg_derived *__restrict__ D,
+                             int W, int H, float *cost_out) {
+  const int lane = threadIdx.x;
+  const int sites_1d = 32 / n, nsites = sites_1d * sites_1d, size = n * n;
+  const int strategy = n == 8 ? JXG_DCT : (n == 16 ? JXG_DCT16X16 : JXG_DCT32X32);
+  const int kind = n == 8 ? JXG_QT_DCT8 : (n == 16 ? JXG_QT_DCT16X16 : JXG_QT_DCT32X32);
+  uint32_t qn = 0, qoff = 0;
+  {
+    const uint32_t sizes[13] = {64, 128, 256, 1024, 512, 256, 4096, 2048, 64, 64, 64, 64, 64};
+    qn = sizes[kind];
+    for (int k = 0; k < kind; k++) qoff += 6 * sizes[k];
+  }
+  // load pixels
+  for (int e = lane; e < 3 * 32 * 32; e += 64) {
+    const int c = e >> 10, r = (e >> 5) & 31, col = e & 31;
+    L.co[c][r * SEL_PITCH + col] = xyb[c * plane + (size_t)(py0 + r) * stride + px0 + col];
+  }
+  for (int e = lane; e < 16 * 3; e += 64) {
+    (&L.nz[0][0])[e] = 0;
+    (&L.maxscan[0][0])[e] = 0;
+  }
+  __syncthreads();
+  // 2-D transform in place: 8x8 rows first, 16x16 / 32x32 columns first.
+  const float scale = 1.0f / (float)(n * n);
+  const int nlines = 3 * nsites * n;
+  for (int pass = 0; pass < 2; pass++) {
+    const bool along_rows = (n == 8) ? (pass == 0) : (pass == 1);
+    for (int e = lane; e < nlines; e += 64) {
+      const int c = e / (nsites * n);
+      const int rem = e - c * nsites * n;
+      const int site = rem / n, line = rem - site * n;
+      const int sy = site / sites_1d, sx = site - sy * sites_1d;
+      float *base = &L.co[c][(sy * n) * SEL_PITCH + sx * n];
+      if (along_rows)
+        dct_line_n<F>(n, base + line * SEL_PITCH, 1, scale, pass == 1);
+      else
+        dct_line_n<F>(n, base + line, SEL_PITCH, scale, pass == 1);
+    }
+    __syncthreads();
+  }
+  // CfL against the unquantized luma (apply_cfl_scalar, src/ac_strategy.rs:1076)
+  for (int e = lane; e < 32 * 32; e += 64) {
+    const int a = (e >> 5) * SEL_PITCH + (e & 31);
+    const float yv = L.co[1][a];
+    L.co[0][a] = fmla<F>(-cmx, yv, L.co[0][a]);
+    L.co[2][a] = fmla<F>(-cmb, yv, L.co[2][a]);
+  }
+  __syncthreads();
+  // selection_quantizer::scale for the 32-family (src/ac_strategy/selection_quantizer.rs:82-120)
+  if (lane == 0) {
+    float sc = 1.0f;
+    if (n == 32 && D->selection_quantizer) {
+      const float *inv_y = tables + qoff + 4 * qn;
+      const float qac = qac_in[0];
+      double energy = 0.0;
+      for (int k = 0; k < 1024; k++) {
+        const int u = k >> 5, v = k & 31;  // u-major storage of the 32x32 assembly
+        const int y = k >> 5, x = k & 31;  // storage row / column
+        if (x < 8 && y < 8) continue;
+        const float l = L.co[1][v * SEL_PITCH + u];
+        if (fabsf(l * inv_y[k] * qac) < 0.5f) energy += (double)l * (double)l;
+      }
+      const float erased = (float)energy;
+      const float dd = D->distance;
+      float t = erased / (2e-4f * dd * dd) - 1.0f;
+      t = fminf(fmaxf(t, 0.0f), 1.0f);
+      sc = 1.0f + 0.6f * t;
+    }
+    L.qscale = sc;
+  }
+  __syncthreads();
+  // Per-coefficient terms (sse_and_rate_scalar_impl::<true, false>,
+  // src/inflated_cost.rs:435-500).  d^2 overwrites the coefficient slot in
+  // storage order; the magnitude term goes to mg.
+  const int cls = n == 8 ? 0 : (n == 16 ? 2 : 4);
+  const int half = n / 2, cblk = n / 8;
+  for (int c = 0; c < 3; c++) {
+    const float *inv = tables + qoff + (3 + c) * qn;
+    const float *thr = D->thr[c][cls];
+    const float one = D->dequant_one[c], num = D->dequant_numerator;
+    float dd_r[16];
+    int ks[16];
+    for (int j = 0; j < 16; j++) {
+      const int e = lane + 64 * j;  // 0..1023: site-major, then storage index
+      const int site = e / size, k = e - site * size;
+      const int sy = site / sites_1d, sx = site - sy * sites_1d;
+      int v, u;
+      if (n == 8) {
+        u = k >> 3;
+        v = k & 7;
+      } else {
+        u = k / n;
+        v = k - u * n;
+      }
+      const int row = k / n, col = k - (k / n) * n;  // storage row / column
+      const float coef = L.co[c][(sy * n + v) * SEL_PITCH + sx * n + u];
+      float dd = 0.0f, mgv = 0.0f;
+      if (!(col < cblk && row < cblk)) {
+        const int yfix = row >= n / 2 ? 2 : 0;
+        const float threshold = col >= half ? thr[yfix + 1] : thr[yfix];
+        const float qs = qac_in[site] * L.qscale * D->qm_mult[c];
+        const float a = inv[k] * qs * coef;
+        const float q = fabsf(a) >= threshold ? round_away(a) : 0.0f;
+        const float diff = a - level_f32(one, num, q);
+        dd = diff * diff;
+        if (q != 0.0f) {
+          const float aq = fabsf(q);
+          const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
+          mgv = ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
+          atomicAdd(&L.nz[site][c], 1u);
+          atomicMax(&L.maxscan[site][c], scan[k]);
+        }
+      }
+      dd_r[j] = dd;
+      L.mg[c][e] = mgv;
+      ks[j] = e;
+    }
+    __syncthreads();
+    for (int j = 0; j < 16; j++) {
+      const int e = ks[j];
+      (&L.co[c][0])[e] = dd_r[j];  // storage-ordered terms, contiguous
+    }
+    __syncthreads();
+  }
+  // Sequential sums in the reference's row-major storage order.
+  for (int wk = lane; wk < nsites * 6; wk += 64) {
+    const int site = wk / 6, rest = wk - site * 6, c = rest >> 1, which = rest & 1;
+    const float *src = which == 0 ? &L.co[c][0] + site * size : &L.mg[c][0] + site * size;
+    float acc = 0.0f;
+    for (int k = 0; k < size; k++) acc += src[k];
+    if (which == 0)
+      L.sse[site][c] = acc;
+    else
+      L.mag[site][c] = acc;
+  }
+  __syncthreads();
+  if (lane < nsites) {
+    const int site = lane;
+    const int sy = site / sites_1d, sx = site - sy * sites_1d;
+    float d_total = 0.0f, r_total = 0.0f;
+    for (int c = 0; c < 3; c++) {
+      d_total += D->channel_weight[c] * L.sse[site][c];
+      const uint32_t nzc = L.nz[site][c];
+      const float nzf = (float)nzc;
+      const uint32_t kn = (uint32_t)nzf;
+      const float header = 0.4f * (kn < JXG_RATE_LUT_N ? D->rate_lut[kn] : log2f(1.0f + nzf));
+      float visited = 0.0f;
+      if (nzc != 0) {
+        const int64_t span = (int64_t)L.maxscan[site][c] + 1 - (int64_t)(cblk * cblk + nzc);
+        visited = span > 0 ? (float)span : 0.0f;
+      }
+      r_total += nzf * 1.6f + 1.0f * L.mag[site][c] + header + 0.5f * visited;
+    }
+    // Footprint::visible_fraction (src/inflated_cost.rs:650-660)
+    const int fx = px0 + sx * n, fy = py0 + sy * n;
+    const int vw = min(n, max(W - fx, 0)), vh = min(n, max(H - fy, 0));
+    const float visible = (vw == n && vh == n) ? 1.0f : (float)(vw * vh) / (float)(n * n);
+    cost_out[site] = fmla<F>(RD_LAMBDA, r_total + META_R, d_total * visible);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float mean_max_mixer(float d) {
+  return fmaxf(1.0f - fmaxf(d - 1.54138f, 0.0f) * 0.56391f, 0.0f);
+}
+
+__device__ __forceinline__ uint8_t aggregate_quant(uint8_t max_q, uint32_t sum, int covered,
+                                                   float d) {
+  if (covered < 4) return max_q;
+  const float mixer = mean_max_mixer(d);
+  const float mean = (float)sum / (float)covered;
+  const float mixed = (float)max_q * mixer + mean * (1.0f - mixer);
+  return (uint8_t)fminf(fmaxf(mixed + 0.5f, 1.0f), 255.0f);
+}
+
+template <bool F>
+__device__ __forceinline__ float risk_gated(float k, float accept, float q_min, float q_max,
+                                            float area, const float *table) {
+  if (q_min <= 0.0f || k <= 0.0f) return accept;
+  const float spread = q_max / q_min - 1.0f;
+  return accept * fast_exp2<F>(-k * area * spread, table);
+}
+
+template <bool F>
+__device__ __forceinline__ float site_bias(const jxg_derived *D, double s1, double s2,
+                                           uint32_t cnt) {
+  if (D->site_k == 0.0f) return 1.0f;
+  float var = 0.0f;
+  if (cnt) {
+    const double mean = s1 / (double)cnt;
+    const double v = s2 / (double)cnt - mean * mean;
+    var = (float)(v > 0.0 ? v : 0.0);
+  }
+  var = fmaxf(var, 1e-9f);
+  const float s = fminf(fmaxf(dirty_log2f<F>(var / 0.003f) / 8.0f, -1.0f), 1.0f);
+  return fast_exp<F>(D->site_k * s);
+}
+
+template <bool F>
+__global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, int stride,
+                                               int padded_h, int W, int H,
+                                               uint8_t *__restrict__ qf,
+                                               const int8_t *__restrict__ ytox,
+                                               const int8_t *__restrict__ ytob,
+                                               const float *__restrict__ tables,
+                                               const uint32_t *__restrict__ scans,
+                                               const jxg_derived *__restrict__ D,
+                                               uint8_t *__restrict__ cells, float *__restrict__ c8out,
+                                               float *__restrict__ c16out,
+                                               float *__restrict__ c32out, int xb, int yb) {
+  __shared__ SelLds L;
+  __shared__ uint8_t q_in[16];
+  __shared__ uint8_t cell_out[16];
+  __shared__ uint8_t q_out[16];
+  const int lane = threadIdx.x;
+  const int rbx = blockIdx.x * 4, rby = blockIdx.y * 4;  // region's first block (global)
+  const int gx = rbx >> 8, gy = rby >> 8;
+  const int xsize = min(256, xb - gx * 256), ysize = min(256, yb - gy * 256);
+  const int bx = rbx - gx * 256, by = rby - gy * 256;  // group-relative
+  if (lane < 16) {
+    const int ix = lane & 3, iy = lane >> 2;
+    const bool in = rbx + ix < xb && rby + iy < yb;
+    const uint8_t q = in ? qf[(size_t)(rby + iy) * xb + rbx + ix] : 1;
+    q_in[lane] = q;
+    q_out[lane] = q;
+    cell_out[lane] = JXG_CELL(JXG_DCT, 1);
+  }
+  __syncthreads();
+  const float scale = D->scale, dist = D->distance;
+  const bool four_row = by + 4 <= ysize;
+  const bool four_col = bx + 4 <= xsize;
+  const bool any_row = four_row || by + 1 < ysize;
+  const bool any_col = bx + 1 < xsize;
+  const bool work = D->run_selection && any_row && any_col;
+  if (work) {
+    const size_t plane = (size_t)stride * padded_h;
+    const int px0 = rbx * 8, py0 = rby * 8;
+    // cmap_factors (src/ac_strategy.rs:1838-1855): this region's 64x64 tile,
+    // clamped to the DC group's tile grid.
+    const int tiles_x = (W + 63) / 64;
+    const int ntx = (xsize + 7) / 8, nty = (ysize + 7) / 8;
+    const int tx = min(bx / 8, ntx - 1), ty = min(by / 8, nty - 1);
+    const size_t ti = (size_t)(gy * 32 + ty) * tiles_x + gx * 32 + tx;
+    const float cmx = fmla<F>((float)ytox[ti], D->cfl_inv_color_factor, D->cfl_base_x);
+    const float cmb = fmla<F>((float)ytob[ti], D->cfl_inv_color_factor, D->cfl_base_b);
+    // site luma statistics for MergeTuning::site_bias (f64, raster order,
+    // clipped to the image): sites 0..3 = 16x16, 4 = 32x32.
+    if (lane < 5) {
+      const int n = lane < 4 ? 16 : 32;
+      const int x0 = px0 + (lane < 4 ? (lane & 1) * 16 : 0);
+      const int y0 = py0 + (lane < 4 ? (lane >> 1) * 16 : 0);
+      const float *yp = xyb + plane;
+      double s1 = 0.0, s2 = 0.0;
+      uint32_t cnt = 0;
+      const int ye = min(y0 + n, H), xe = min(x0 + n, W);
+      for (int y = y0; y < ye; y++)
+        for (int x = x0; x < xe; x++) {
+          const double v = (double)yp[(size_t)y * stride + x];
+          s1 += v;
+          s2 += v * v;
+          cnt++;
+        }
+      L.site_s1[lane] = s1;
+      L.site_s2[lane] = s2;
+      L.site_cnt[lane] = cnt;
+    }
+    __syncthreads();
+    float qac8[16], qac16[4], qac32[1];
+    for (int i = 0; i < 16; i++) qac8[i] = scale * (float)q_in[i];
+    // aggregate_qac_2x2 (src/ac_strategy/selection.rs:488-496)
+    for (int s = 0; s < 4; s++) {
+      const int sx = (s & 1) * 2, sy = (s >> 1) * 2;
+      const float a = qac8[sy * 4 + sx], b = qac8[sy * 4 + sx + 1];
+      const float c = qac8[(sy + 1) * 4 + sx], d = qac8[(sy + 1) * 4 + sx + 1];
+      const float max_q = fmaxf(fmaxf(fmaxf(a, b), c), d);
+      const float mixer = mean_max_mixer(dist);
+      const float mean = (a + b + c + d) * 0.25f;
+      const float mixed = (max_q * mixer + mean * (1.0f - mixer)) / scale;
+      qac16[s] = scale * floorf(fminf(fmaxf(mixed + 0.5f, 1.0f), 255.0f));
+    }
+    {
+      // region_qac (src/ac_strategy/selection.rs:898-927)
+      uint8_t max_q = 1;
+      uint32_t sum = 0;
+      for (int i = 0; i < 16; i++) {
+        max_q = q_in[i] > max_q ? q_in[i] : max_q;
+        sum += q_in[i];
+      }
+      qac32[0] = scale * (float)aggregate_quant(max_q, sum, 16, dist);
+    }
+    eval_squares<F>(L, 8, xyb, plane, stride, px0, py0, qac8, cmx, cmb, tables, scans, D, W, H,
+                    L.cost8);
+    eval_squares<F>(L, 16, xyb, plane, stride, px0, py0, qac16, cmx, cmb, tables, scans + 64, D,
+                    W, H, L.cost16);
+    if (four_row && four_col)
+      eval_squares<F>(L, 32, xyb, plane, stride, px0, py0, qac32, cmx, cmb, tables, scans + 320,
+                      D, W, H, &L.cost32);
+    if (lane == 0) {
+      // select_super_block at square scope for each super block, then the
+      // 32x32 merge (src/ac_strategy/selection.rs:238-455, 1040-1150).
+      float children[2][2], baseline[2][2];
+      const int nsy = four_row ? 2 : 1;
+      for (int sy = 0; sy < nsy; sy++)
+        for (int sx = 0; sx < 2; sx++) {
+          children[sy][sx] = 0.0f;
+          baseline[sy][sx] = 0.0f;
+          if (bx + sx * 2 + 1 >= xsize) continue;
+          const int b0 = (sy * 2) * 4 + sx * 2;
+          const float c00 = L.cost8[b0], c01 = L.cost8[b0 + 1];
+          const float c10 = L.cost8[b0 + 4], c11 = L.cost8[b0 + 5];
+          const float left = c00 + c10, right = c01 + c11, top = c00 + c01, bottom = c10 + c11;
+          const float total = left + right;
+          const int s = sy * 2 + sx;
+          const float c16 =
+              D->bias_16x16 * site_bias<F>(D, L.site_s1[s], L.site_s2[s], L.site_cnt[s]) *
+              L.cost16[s];
+          const float cost_16x8 = left + right, cost_8x16 = top + bottom;
+          const float best_rect = fminf(cost_16x8, cost_8x16);
+          float q_min = INFINITY, q_max = 0.0f;
+          const float qs[4] = {qac8[b0], qac8[b0 + 1], qac8[b0 + 4], qac8[b0 + 5]};
+          for (int i = 0; i < 4; i++) {
+            q_min = fminf(q_min, qs[i]);
+            q_max = fmaxf(q_max, qs[i]);
+          }
+          // can_place_strategy(DCT16X16): aligned super blocks always fit
+          // their 64-px tile and 256-px group once inside the image.
+          const bool fits = bx + sx * 2 + 2 <= xsize && by + sy * 2 + 2 <= ysize;
+          const bool pick16 =
+              fits && c16 < best_rect &&
+              c16 < total * risk_gated<F>(D->risk_k, D->accept_16, q_min, q_max, 1.0f,
+                                          D->exp2_table);
+          if (pick16) {
+            cell_out[b0] = JXG_CELL(JXG_DCT16X16, 1);
+            cell_out[b0 + 1] = JXG_CELL(JXG_DCT16X16, 0);
+            cell_out[b0 + 4] = JXG_CELL(JXG_DCT16X16, 0);
+            cell_out[b0 + 5] = JXG_CELL(JXG_DCT16X16, 0);
+            children[sy][sx] = c16;
+          } else {
+            children[sy][sx] = cost_16x8 <= cost_8x16 ? cost_16x8 : cost_8x16;
+          }
+          baseline[sy][sx] = total;
+        }
+      if (four_row && four_col) {
+        const float m32 = site_bias<F>(D, L.site_s1[4], L.site_s2[4], L.site_cnt[4]);
+        const float merged = D->bias_32x32 * m32 * L.cost32;
+        uint8_t qmin = 255, qmax = 0;
+        for (int i = 0; i < 16; i++) {
+          qmin = q_in[i] < qmin ? q_in[i] : qmin;
+          qmax = q_in[i] > qmax ? q_in[i] : qmax;
+        }
+        const float gate =
+            risk_gated<F>(D->risk_k, D->accept_32, (float)qmin, (float)qmax, 2.0f, D->exp2_table);
+        const float child = children[0][0] + children[0][1] + children[1][0] + children[1][1];
+        const float base = baseline[0][0] + baseline[0][1] + baseline[1][0] + baseline[1][1];
+        if (merged < child && merged < base * gate) {
+          for (int i = 0; i < 16; i++) cell_out[i] = JXG_CELL(JXG_DCT32X32, i == 0);
+        }
+      }
+      // adjust_quant_field (src/ac_strategy/selection.rs:503-527)
+      for (int i = 0; i < 16; i++) {
+        const uint8_t cell = cell_out[i];
+        if (!(cell & 1)) continue;
+        const int s = cell >> 1;
+        const int cx = cov_x(s), cy = cov_y(s);
+        if (cx == 1 && cy == 1) continue;
+        uint8_t max_q = 0;
+        uint32_t sum = 0;
+        for (int iy = 0; iy < cy; iy++)
+          for (int ix = 0; ix < cx; ix++) {
+            const uint8_t q = q_in[i + iy * 4 + ix];
+            max_q = q > max_q ? q : max_q;
+            sum += q;
+          }
+        const uint8_t val = aggregate_quant(max_q, sum, cx * cy, dist);
+        for (int iy = 0; iy < cy; iy++)
+          for (int ix = 0; ix < cx; ix++) q_out[i + iy * 4 + ix] = val;
+      }
+    }
+    __syncthreads();
+    if (c8out && lane < 16) {
+      const int ix = lane & 3, iy = lane >> 2;
+      const int sx = ix >> 1, sy = iy >> 1;
+      const bool eval = (four_row ? true : iy < 2) && (bx + sx * 2 + 1 < xsize);
+      if (rbx + ix < xb && rby + iy < yb) {
+        const size_t o = (size_t)(rby + iy) * xb + rbx + ix;
+        c8out[o] = eval ? L.cost8[lane] : NAN;
+        const bool s16 = (ix & 1) == 0 && (iy & 1) == 0 && eval;
+        c16out[o] = s16 ? L.cost16[sy * 2 + sx] : NAN;
+        c32out[o] = (lane == 0 && four_row && four_col) ? L.cost32 : NAN;
+      }
+    }
+  } else if (c8out && lane < 16) {
+    const int ix = lane & 3, iy = lane >> 2;
+    if (rbx + ix < xb && rby + iy < yb) {
+      const size_t o = (size_t)(rby + iy) * xb + rbx + ix;
+      c8out[o] = NAN;
+      c16out[o] = NAN;
+      c32out[o] = NAN;
+    }
+  }
+  if (lane < 16) {
+    const int ix = lane & 3, iy = lane >> 2;
+    if (rbx + ix < xb && rby + iy < yb) {
+      const size_t o = (size_t)(rby + iy) * xb + rbx + ix;
+      if (cells) cells[o] = cell_out[lane];
+      qf[o] = q_out[lane];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward transform + quantization (src/group.rs:1259-2240, first pass)
+
+#define QT_TILE 64
+#define QT_PITCH 65
+#define QT_MAXT 64
+
+struct QLds {
+  float co[3][QT_TILE * QT_PITCH];
+  int n;
+  int s[QT_MAXT];
+  int bx[QT_MAXT], by[QT_MAXT];  // global first block
+  int lx[QT_MAXT], ly[QT_MAXT];  // LDS origin (pixels)
+  int pre1[QT_MAXT + 1];         // prefix of 3 * pass-1 lines
+  int pre2[QT_MAXT + 1];         // prefix of 3 * pass-2 lines
+  int prec[QT_MAXT + 1];         // prefix of coefficient counts
+  int16_t ydc[QT_MAXT][16];
+  float xf[QT_MAXT], bf[QT_MAXT];
+  int quant[QT_MAXT];
+};
+
+__device__ __forceinline__ int find_item(const int *pre, int n, int e) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= e)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// dc_from_coefficients (src/group.rs:183-260; src/dct.rs:930,1033,1864,2170,
+// 2356,2381) on a transform held spatially in LDS.
+template <bool F>
+__device__ void dc_from(int s, const float *co, int pitch, int w, int h, float *dc) {
+  const float r2[2] = {1.0f, 0.9017642f};
+  const float r4[4] = {1.0f, 0.9748868f, 0.9017642f, 0.7870549f};
+  auto at = [&](int k) {
+    int v, u;
+    storage_to_spatial(s, w, h, k, &v, &u);
+    return co[v * pitch + u];
+  };
+  switch (s) {
+    case JXG_DCT16X8:
+    case JXG_DCT8X16: {
+      const float s0 = at(0) * r2[0], s1 = at(1) * r2[1];
+      dc[0] = s0 + s1;
+      dc[1] = s0 - s1;
+      return;
+    }
+    case JXG_DCT16X16: {
+      const float s00 = at(0) * (r2[0] * r2[0]);
+      const float s01 = at(1) * (r2[0] * r2[1]);
+      const float s10 = at(16) * (r2[1] * r2[0]);
+      const float s11 = at(17) * (r2[1] * r2[1]);
+      const float r00 = s00 + s01, r01 = s00 - s01, r10 = s10 + s11, r11 = s10 - s11;
+      dc[0] = r00 + r10;
+      dc[1] = r00 - r10;
+      dc[2] = r01 + r11;
+      dc[3] = r01 - r11;
+      return;
+    }
+    case JXG_DCT32X32: {
+      float rr[4][4];
+      for (int a = 0; a < 4; a++) {
+        float sv[4];
+        for (int b = 0; b < 4; b++) sv[b] = at(a * 32 + b) * r4[a] * r4[b];
+        idct4<F>(sv, rr[a]);
+      }
+      for (int bb = 0; bb < 4; bb++) {
+        const float cin[4] = {rr[0][bb], rr[1][bb], rr[2][bb], rr[3][bb]};
+        float col[4];
+        idct4<F>(cin, col);
+        for (int r = 0; r < 4; r++) dc[bb * 4 + r] = col[r];
+      }
+      return;
+    }
+    case JXG_DCT32X16:
+    case JXG_DCT16X32: {
+      float rr[2][4];
+      for (int a = 0; a < 2; a++) {
+        float sv[4];
+        for (int b = 0; b < 4; b++) sv[b] = at(a * 32 + b) * r2[a] * r4[b];
+        idct4<F>(sv, rr[a]);
+      }
+      for (int bb = 0; bb < 4; bb++) {
+        if (s == JXG_DCT32X16) {
+          dc[bb * 2] = rr[0][bb] + rr[1][bb];
+          dc[bb * 2 + 1] = rr[0][bb] - rr[1][bb];
+        } else {
+          dc[bb] = rr[0][bb] + rr[1][bb];
+          dc[4 + bb] = rr[0][bb] - rr[1][bb];
+        }
+      }
+      return;
+    }
+    case JXG_DCT32X8:
+    case JXG_DCT8X32: {
+      float sv[4];
+      for (int b = 0; b < 4; b++) sv[b] = at(b) * r4[b];
+      idct4<F>(sv, dc);
+      return;
+    }
+    default: dc[0] = at(0); return;
+  }
+}
+
+// Raw strategies the quantize stage transforms.
+__host__ __device__ __forceinline__ bool quantize_supported(int s) {
+  return s == JXG_DCT || s == JXG_DCT16X8 || s == JXG_DCT8X16 || s == JXG_DCT16X16 ||
+         s == JXG_DCT32X32 || s == JXG_DCT32X16 || s == JXG_DCT16X32 || s == JXG_DCT32X8 ||
+         s == JXG_DCT8X32;
+}
+
+struct QArgs {
+  const float *xyb;
+  int stride, padded_h, W, H, xb, yb;
+  const uint8_t *cells, *qf;
+  const int8_t *ytox, *ytob;
+  const float *tables;
+  const jxg_derived *D;
+  int32_t *lv0, *lv1, *lv2;
+  float *co0, *co1, *co2;
+  int16_t *dc0, *dc1, *dc2;
+  uint32_t *xt_count, *xt_list;
+  uint32_t xt_cap;
+  uint32_t *bad;  // set to 1 when an unsupported strategy cell is seen
+};
+
+__device__ __forceinline__ size_t out_index(int bx, int by, int cvx, int xb, int k) {
+  const int chunk = cvx * 64;
+  return ((size_t)(by + k / chunk) * xb + bx) * 64 + (k % chunk);
+}
+
+// Transform + quantize the L.n transforms whose footprints are in L.co.
+template <bool F>
+__device__ void quantize_list(QLds &L, const QArgs &A) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const jxg_derived *D = A.D;
+  const int n = L.n;
+  if (n == 0) return;
+  if (tid == 0) {
+    int p1 = 0, p2 = 0, pc = 0;
+    for (int t = 0; t < n; t++) {
+      const int s = L.s[t];
+      const int w = cov_x(s) * 8, h = cov_y(s) * 8;
+      L.pre1[t] = p1;
+      L.pre2[t] = p2;
+      L.prec[t] = pc;
+      p1 += 3 * (rows_first(s) ? h : w);
+      p2 += 3 * (rows_first(s) ? w : h);
+      pc += w * h;
+    }
+    L.pre1[n] = p1;
+    L.pre2[n] = p2;
+    L.prec[n] = pc;
+  }
+  if (tid < n) {
+    const int t = tid;
+    const int tiles_x = (A.W + 63) / 64;
+    const size_t ti = (size_t)(L.by[t] / 8) * tiles_x + L.bx[t] / 8;
+    L.xf[t] = fmla<F>((float)A.ytox[ti], D->cfl_inv_color_factor, D->cfl_base_x);
+    L.bf[t] = fmla<F>((float)A.ytob[ti], D->cfl_inv_color_factor, D->cfl_base_b);
+    L.quant[t] = A.qf[(size_t)L.by[t] * A.xb + L.bx[t]];
+  }
+  __syncthreads();
+  // Pass 1: rows (8x8, 8x16, 16x32, 8x32) or columns (the rest).
+  for (int e = tid; e < L.pre1[n]; e += nthr) {
+    const int t = find_item(L.pre1, n, e);
+    const int s = L.s[t];
+    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
+    const bool rf = rows_first(s);
+    const int per_c = rf ? h : w;
+    const int local = e - L.pre1[t];
+    const int c = local / per_c, li = local - c * per_c;
+    float *base = &L.co[c][L.ly[t] * QT_PITCH + L.lx[t]];
+    if (rf)
+      dct_line_n<F>(w, base + li * QT_PITCH, 1, 0.0f, false);
+    else
+      dct_line_n<F>(h, base + li, QT_PITCH, 0.0f, false);
+  }
+  __syncthreads();
+  // Pass 2: the other axis, scaled by 1/(w*h).
+  for (int e = tid; e < L.pre2[n]; e += nthr) {
+    const int t = find_item(L.pre2, n, e);
+    const int s = L.s[t];
+    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
+    const bool rf = rows_first(s);
+    const int per_c = rf ? w : h;
+    const int local = e - L.pre2[t];
+    const int c = local / per_c, li = local - c * per_c;
+    const float scale = 1.0f / (float)(w * h);
+    float *base = &L.co[c][L.ly[t] * QT_PITCH + L.lx[t]];
+    if (rf)
+      dct_line_n<F>(h, base + li, QT_PITCH, scale, true);
+    else
+      dct_line_n<F>(w, base + li * QT_PITCH, 1, scale, true);
+  }
+  __syncthreads();
+  // Y DC (dc_from_coefficients, quantize_dc_scalar src/group.rs:290)
+  if (tid < n) {
+    const int t = tid, s = L.s[t];
+    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
+    float dcv[16];
+    dc_from<F>(s, &L.co[1][L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, w, h, dcv);
+    const int cvx = cov_x(s), covered = cvx * cov_y(s);
+    for (int i = 0; i < covered; i++) {
+      const int16_t q = sat_i16(round_away(dcv[i] * D->inv_factor[1]));
+      L.ydc[t][i] = q;
+      A.dc1[(size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx] = q;
+    }
+  }
+  __syncthreads();
+  // Y quantize / dequantize round trip (quantize_roundtrip_y_block,
+  // src/group.rs:795), Y LLF zeroing (:1852) and CfL on X and B (:1863).
+  for (int e = tid; e < L.prec[n]; e += nthr) {
+    const int t = find_item(L.prec, n, e);
+    const int s = L.s[t];
+    const int k = e - L.prec[t];
+    const int cvx = cov_x(s), cvy = cov_y(s);
+    const int w = cvx * 8, h = cvy * 8;
+    const int cx = max(cvx, cvy), cy = min(cvx, cvy);
+    const int wc = cx * 8, hc = cy * 8;
+    int v, u;
+    storage_to_spatial(s, w, h, k, &v, &u);
+    const int a = (L.ly[t] + v) * QT_PITCH + L.lx[t] + u;
+    const int row = k / wc, col = k - row * wc;
+    const size_t out = out_index(L.bx[t], L.by[t], cvx, A.xb, k);
+    const int kind = jxg_qt_kind_dev(s);
+    const float *tb = A.tables + D->qt_off[kind];
+    const int qn = D->qt_n[kind];
+    const float yv = L.co[1][a];
+    A.co1[out] = yv;
+    const float *thr = D->thr[1][jxg_area_class(cx, cy)];
+    const int yfix = row >= hc / 2 ? 2 : 0;
+    const float threshold = col >= wc / 2 ? thr[yfix + 1] : thr[yfix];
+    const int32_t quant = L.quant[t];
+    const float q_scaled = D->scale * (float)quant * 1.0f;
+    const float val = (tb[4 * qn + k] * q_scaled) * yv;
+    const int32_t q = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+    A.lv1[out] = q;
+    const float inv_qac = 1.0f / (D->scale * (float)quant);
+    float y2 =
+        dequant_level(D->dequant_one[1], D->dequant_numerator, q) * tb[1 * qn + k] * inv_qac;
+    if (row < cy && col < cx) y2 = 0.0f;
+    L.co[0][a] = fmla<F>(-L.xf[t], y2, L.co[0][a]);
+    L.co[2][a] = fmla<F>(-L.bf[t], y2, L.co[2][a]);
+  }
+  __syncthreads();
+  // X and B DC after CfL (src/group.rs:1871-2003, 2128-2150)
+  if (tid < n) {
+    const int t = tid, s = L.s[t];
+    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
+    float xd[16], bd[16];
+    dc_from<F>(s, &L.co[0][L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, w, h, xd);
+    dc_from<F>(s, &L.co[2][L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, w, h, bd);
+    const int cvx = cov_x(s), covered = cvx * cov_y(s);
+    for (int i = 0; i < covered; i++) {
+      const float yq = (float)L.ydc[t][i];
+      int16_t xq;
+      if (D->cfl_factor_x != 0.0f)
+        xq = sat_i16(round_away(fmla<F>(xd[i], D->inv_factor[0], -yq * D->cfl_factor_x)));
+      else
+        xq = sat_i16(round_away(xd[i] * D->inv_factor[0]));
+      const int16_t bq =
+          sat_i16(round_away(fmla<F>(bd[i], D->inv_factor[2], -yq * D->cfl_factor_b)));
+      const size_t o = (size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx;
+      A.dc0[o] = xq;
+      A.dc2[o] = bq;
+    }
+  }
+  // X and B AC quantization (quantize_block_ac_scalar, src/group.rs:560-620)
+  for (int e = tid; e < L.prec[n]; e += nthr) {
+    const int t = find_item(L.prec, n, e);
+    const int s = L.s[t];
+    const int k = e - L.prec[t];
+    const int cvx = cov_x(s), cvy = cov_y(s);
+    const int w = cvx * 8, h = cvy * 8;
+    const int cx = max(cvx, cvy), cy = min(cvx, cvy);
+    const int wc = cx * 8, hc = cy * 8;
+    int v, u;
+    storage_to_spatial(s, w, h, k, &v, &u);
+    const int a = (L.ly[t] + v) * QT_PITCH + L.lx[t] + u;
+    const int row = k / wc, col = k - row * wc;
+    const size_t out = out_index(L.bx[t], L.by[t], cvx, A.xb, k);
+    const int kind = jxg_qt_kind_dev(s);
+    const float *tb = A.tables + D->qt_off[kind];
+    const int qn = D->qt_n[kind];
+    const int cls = jxg_area_class(cx, cy);
+    const int yfix = row >= hc / 2 ? 2 : 0;
+    const bool hi = col >= wc / 2;
+    const int32_t quant = L.quant[t];
+    const float xv = L.co[0][a];
+    A.co0[out] = xv;
+    {
+      const float threshold = hi ? D->thr[0][cls][yfix + 1] : D->thr[0][cls][yfix];
+      const float q_scaled = D->scale * (float)quant * D->qm_mult[0];
+      const float val = (tb[3 * qn + k] * q_scaled) * xv;
+      A.lv0[out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+    }
+    const float bv = L.co[2][a];
+    A.co2[out] = bv;
+    {
+      const float threshold = hi ? D->thr[2][cls][yfix + 1] : D->thr[2][cls][yfix];
+      const float q_scaled = D->scale * (float)quant * D->qm_mult[2];
+      const float val = (tb[5 * qn + k] * q_scaled) * bv;
+      A.lv2[out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+    }
+  }
+}
+
+template <bool F>
+__global__ __launch_bounds__(256) void k_quantize(QArgs A) {
+  __shared__ QLds L;
+  const int tid = threadIdx.x;
+  const int bx0 = blockIdx.x * 8, by0 = blockIdx.y * 8;
+  const int nbx = min(8, A.xb - bx0), nby = min(8, A.yb - by0);
+  if (tid == 0) L.n = 0;
+  __syncthreads();
+  if (tid < 64) {
+    const int lx = tid & 7, ly = tid >> 3;
+    if (lx < nbx && ly < nby) {
+      const uint8_t cell = A.cells[(size_t)(by0 + ly) * A.xb + bx0 + lx];
+      if (cell & 1) {
+        const int s = cell >> 1;
+        if (s >= JXG_NUM_STRATEGIES || !quantize_supported(s) || bx0 + lx + cov_x(s) > A.xb ||
+            by0 + ly + cov_y(s) > A.yb) {
+          atomicOr(A.bad, 1u);
+        } else if (lx + cov_x(s) <= 8 && ly + cov_y(s) <= 8) {
+          const int slot = atomicAdd(&L.n, 1);
+          L.s[slot] = s;
+          L.bx[slot] = bx0 + lx;
+          L.by[slot] = by0 + ly;
+          L.lx[slot] = lx * 8;
+          L.ly[slot] = ly * 8;
+        } else {
+          const uint32_t slot = atomicAdd(A.xt_count, 1u);
+          if (slot < A.xt_cap) A.xt_list[slot] = (uint32_t)((by0 + ly) * A.xb + bx0 + lx);
+        }
+      }
+    }
+  }
+  // Load the 64x64 tile of all three planes (padded planes: always in range).
+  const size_t plane = (size_t)A.stride * A.padded_h;
+  const int px0 = bx0 * 8, py0 = by0 * 8;
+  for (int e = tid; e < 3 * 64 * 16; e += 256) {
+    const int c = e / (64 * 16), r = (e >> 4) & 63, q = e & 15;
+    const float4 v =
+        *reinterpret_cast<const float4 *>(A.xyb + c * plane + (size_t)(py0 + r) * A.stride + px0 + 4 * q);
+    float *dst = &L.co[c][r * QT_PITCH + 4 * q];
+    dst[0] = v.x;
+    dst[1] = v.y;
+    dst[2] = v.z;
+    dst[3] = v.w;
+  }
+  __syncthreads();
+  quantize_list<F>(L, A);
+}
+
+// Transforms whose footprint crosses their 64x64 tile (the layout DP's
+// crossing placements, src/dc_group_data.rs:207-217): one workgroup per
+// queued transform, footprint gathered straight from the padded planes.
+template <bool F>
+__global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
+  __shared__ QLds L;
+  const int tid = threadIdx.x;
+  const uint32_t count = min(*A.xt_count, A.xt_cap);
+  const size_t plane = (size_t)A.stride * A.padded_h;
+  for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
+    const uint32_t blk = A.xt_list[item];
+    const int bx = (int)(blk % (uint32_t)A.xb), by = (int)(blk / (uint32_t)A.xb);
+    const int s = A.cells[blk] >> 1;
+    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
+    if (tid == 0) {
+      L.n = 1;
+      L.s[0] = s;
+      L.bx[0] = bx;
+      L.by[0] = by;
+      L.lx[0] = 0;
+      L.ly[0] = 0;
+    }
+    for (int e = tid; e < 3 * w * h; e += 256) {
+      const int c = e / (w * h), r = (e / w) % h, col = e % w;
+      L.co[c][r * QT_PITCH + col] = A.xyb[c * plane + (size_t)(by * 8 + r) * A.stride + bx * 8 + col];
+    }
+    __syncthreads();
+    quantize_list<F>(L, A);
+    __syncthreads();
+  }
+}
+
+// Clears a small counter block (a kernel rather than a memset node, so the
+// sequence can be captured into a graph).
+__global__ void k_zero_u32(uint32_t *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Context and C ABI
+
+struct jxg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float *d_tables = nullptr;
+  bool tables_set = false;
+  jxg_derived *d_derived = nullptr;
+  jxg_derived *h_derived = nullptr;  // pinned staging
+  bool derived_valid = false;
+  jxg_frame_params last_params;
+  uint32_t *d_scans = nullptr;  // 64 + 256 + 1024 natural scan positions
+  float *d_mlut = nullptr;      // fwd matrix (9) + transfer LUT (256)
+  float *h_mlut = nullptr;
+  bool mlut_valid = false;
+  float *d_exp2 = nullptr;      // fast_exp2 table
+  float *d_pre = nullptr;
+  size_t pre_cap = 0;
+  float *d_xyb = nullptr;
+  size_t xyb_cap = 0;
+  uint32_t *d_misc = nullptr;  // [0] crossing count, [1] unsupported flag
+  uint32_t *d_xt = nullptr;
+  size_t xt_cap = 0;
+  int8_t *d_zero_cmap = nullptr;
+  size_t zero_cap = 0;
+  uint8_t *d_rgb = nullptr;
+  size_t rgb_cap = 0;
+};
+
+template <typename T>
+static int ensure(T **p, size_t *cap, size_t n) {
+  if (*cap >= n && *p) return JXG_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  HIP_TRY(hipMalloc((void **)p, n * sizeof(T)));
+  *cap = n;
+  return JXG_OK;
+}
+
+// scan_pos_lut (src/coeff_order.rs:358-415) of the square shapes from the
+// natural order (natural_scan_generic, src/coeff_order.rs:206-248).
+static void natural_scan(int cx, int cy, uint32_t *out) {
+  const int side = cx * 8, xs = cx / cy, xsm = xs - 1;
+  int xss = 0;
+  while ((1 << xss) < xs) xss++;
+  int cur = cx * cy;
+  for (int i = 0; i < side; i++)
+    for (int j = 0; j <= i; j++) {
+      int x = j, y = i - j;
+      if (i & 1) {
+        const int t = x;
+        x = y;
+        y = t;
+      }
+      if (y & xsm) continue;
+      y >>= xss;
+      const int val = (x < cx && y < cy) ? y * cx + x : cur++;
+      out[val] = (uint32_t)(y * side + x);
+    }
+  for (int ip = side - 1; ip >= 1; ip--) {
+    const int i = ip - 1;
+    for (int j = 0; j <= i; j++) {
+      int x = side - 1 - (i - j), y = side - 1 - j;
+      if (i & 1) {
+        const int t = x;
+        x = y;
+        y = t;
+      }
+      if (y & xsm) continue;
+      y >>= xss;
+      out[cur++] = (uint32_t)(y * side + x);
+    }
+  }
+}
+
+static int check_image(uint32_t w, uint32_t h) {
+  if (w == 0 || h == 0) return set_err(JXG_ERR_EMPTY_IMAGE, "empty image %ux%u", w, h);
+  if (w > JXG_MAX_DIM || h > JXG_MAX_DIM)
+    return set_err(JXG_ERR_TOO_LARGE, "image %ux%u exceeds %u", w, h, JXG_MAX_DIM);
+  return JXG_OK;
+}
+
+static int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t padded_h) {
+  uint32_t s, ph;
+  jxg_plane_geometry(w, h, &s, &ph);
+  if (stride < s || padded_h < ph || (stride & 3))
+    return set_err(JXG_ERR_INVALID_ARG, "plane geometry %ux%u too small for %ux%u (need %ux%u)",
+                   stride, padded_h, w, h, s, ph);
+  return JXG_OK;
+}
+
+static int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p) {
+  if (!p) return set_err(JXG_ERR_INVALID_ARG, "null params");
+  if (ctx->derived_valid && memcmp(&ctx->last_params, p, sizeof(*p)) == 0) return JXG_OK;
+  jxg_derived tmp;
+  const int rc = jxg_derive(p, &tmp);
+  if (rc != JXG_OK) return set_err(rc, "invalid frame parameters (distance %g)", (double)p->distance);
+  HIP_TRY(hipStreamSynchronize(ctx->stream));  // staging buffer may still be in flight
+  *ctx->h_derived = tmp;
+  HIP_TRY(hipMemcpyAsync(ctx->d_derived, ctx->h_derived, sizeof(jxg_derived), hipMemcpyHostToDevice,
+                         ctx->stream));
+  ctx->last_params = *p;
+  ctx->derived_valid = true;
+  return JXG_OK;
+}
+
+static int zero_cmaps(jxg_ctx *ctx, uint32_t w, uint32_t h, const int8_t **x, const int8_t **b) {
+  if (*x && *b) return JXG_OK;
+  const size_t n = (size_t)((w + 63) / 64) * ((h + 63) / 64);
+  if (ctx->zero_cap < n) {
+    int rc = ensure(&ctx->d_zero_cmap, &ctx->zero_cap, n);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(ctx->d_zero_cmap, 0, n, ctx->stream));
+  }
+  if (!*x) *x = ctx->d_zero_cmap;
+  if (!*b) *b = ctx->d_zero_cmap;
+  return JXG_OK;
+}
+
+extern "C" {
+
+const char *jxg_last_error(void) { return g_err; }
+
+int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
+  if (!out) return set_err(JXG_ERR_INVALID_ARG, "null out");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    return set_err(JXG_ERR_NO_DEVICE, "no HIP device visible");
+  if (device < 0 || device >= count)
+    return set_err(JXG_ERR_INVALID_ARG, "device %d out of range (%d devices)", device, count);
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return set_err(JXG_ERR_NO_DEVICE, "device %d is %s, this library is built for gfx950", device,
+                   prop.gcnArchName);
+  HIP_TRY(hipSetDevice(device));
+  jxg_ctx *ctx = new jxg_ctx();
+  ctx->device = device;
+  ctx->stream = (hipStream_t)stream;
+  auto fail = [&](int rc) {
+    jxg_ctx_destroy(ctx);
+    return rc;
+  };
+  if (hipMalloc((void **)&ctx->d_tables, JXG_QT_BLOB_FLOATS * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_derived, sizeof(jxg_derived)) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->h_derived, sizeof(jxg_derived), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_scans, (64 + 256 + 1024) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_mlut, (9 + 256) * sizeof(float)) != hipSuccess ||
+      hipHostMalloc((void **)&ctx->h_mlut, (9 + 256) * sizeof(float), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_misc, 4 * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_exp2, 64 * sizeof(float)) != hipSuccess)
+    return fail(set_err(JXG_ERR_ALLOC, "context allocation failed"));
+  // scan position tables of the square shapes, computed once
+  uint32_t scans[64 + 256 + 1024];
+  {
+    uint32_t order[1024];
+    const int sides[3] = {1, 2, 4};
+    int off = 0;
+    for (int i = 0; i < 3; i++) {
+      const int cx = sides[i], n = cx * cx * 64;
+      natural_scan(cx, cx, order);
+      for (int k = 0; k < n; k++) scans[off + order[k]] = (uint32_t)k;
+      off += n;
+    }
+  }
+  if (hipMemcpy(ctx->d_scans, scans, sizeof(scans), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_err(JXG_ERR_DEVICE, "scan table upload failed"));
+  {
+    float e2[64];
+    for (int i = 0; i < 64; i++) e2[i] = (float)exp2((double)(i - 32) / 64.0);
+    if (hipMemcpy(ctx->d_exp2, e2, sizeof(e2), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(set_err(JXG_ERR_DEVICE, "exp2 table upload failed"));
+  }
+  float *blob = new float[JXG_QT_BLOB_FLOATS];
+  jxg_default_quant_tables(blob);
+  const hipError_t e = hipMemcpy(ctx->d_tables, blob, JXG_QT_BLOB_FLOATS * sizeof(float),
+                                 hipMemcpyHostToDevice);
+  delete[] blob;
+  if (e != hipSuccess) return fail(set_err(JXG_ERR_DEVICE, "table upload failed"));
+  *out = ctx;
+  return JXG_OK;
+}
+
+void jxg_ctx_destroy(jxg_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  hipFree(ctx->d_tables);
+  hipFree(ctx->d_derived);
+  hipHostFree(ctx->h_derived);
+  hipFree(ctx->d_scans);
+  hipFree(ctx->d_mlut);
+  hipHostFree(ctx->h_mlut);
+  hipFree(ctx->d_pre);
+  hipFree(ctx->d_xyb);
+  hipFree(ctx->d_misc);
+  hipFree(ctx->d_exp2);
+  hipFree(ctx->d_xt);
+  hipFree(ctx->d_zero_cmap);
+  hipFree(ctx->d_rgb);
+  delete ctx;
+}
+
+int jxg_ctx_synchronize(jxg_ctx *ctx) {
+  if (!ctx) return set_err(JXG_ERR_INVALID_ARG, "null ctx");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return JXG_OK;
+}
+
+int jxg_ctx_set_tables(jxg_ctx *ctx, const float *blob) {
+  if (!ctx || !blob) return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipMemcpy(ctx->d_tables, blob, JXG_QT_BLOB_FLOATS * sizeof(float), hipMemcpyHostToDevice));
+  return JXG_OK;
+}
+
+int jxg_xyb_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channels,
+                uint32_t width, uint32_t height, const float fwd[9], const float lut[256],
+                int order, int fused, float *d_xyb, uint32_t stride, uint32_t padded_h) {
+  if (!ctx || !d_rgb || !fwd || !lut || !d_xyb) return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  int rc = check_image(width, height);
+  if (rc) return rc;
+  if (rc = check_planes(width, height, stride, padded_h)) return rc;
+  if (channels != 3 && channels != 4) return set_err(JXG_ERR_INVALID_ARG, "channels %d", channels);
+  if (row_bytes < (size_t)width * channels) return set_err(JXG_ERR_INVALID_ARG, "row_bytes too small");
+  if (order != JXG_XYB_SCALAR && order != JXG_XYB_AVX2) return set_err(JXG_ERR_INVALID_ARG, "order %d", order);
+  if (!ctx->mlut_valid || memcmp(ctx->h_mlut, fwd, 9 * sizeof(float)) != 0 ||
+      memcmp(ctx->h_mlut + 9, lut, 256 * sizeof(float)) != 0) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));  // staging buffer may be in flight
+    memcpy(ctx->h_mlut, fwd, 9 * sizeof(float));
+    memcpy(ctx->h_mlut + 9, lut, 256 * sizeof(float));
+    HIP_TRY(hipMemcpyAsync(ctx->d_mlut, ctx->h_mlut, 265 * sizeof(float), hipMemcpyHostToDevice,
+                           ctx->stream));
+    ctx->mlut_valid = true;
+  }
+  const dim3 grid((stride + 255) / 256, padded_h);
+  if (order == JXG_XYB_AVX2)
+    k_xyb<true, JXG_XYB_AVX2><<<grid, 256, 0, ctx->stream>>>(d_rgb, row_bytes, channels, width, height, ctx->d_mlut, d_xyb, stride, padded_h);
+  else if (fused)
+    k_xyb<true, JXG_XYB_SCALAR><<<grid, 256, 0, ctx->stream>>>(d_rgb, row_bytes, channels, width, height, ctx->d_mlut, d_xyb, stride, padded_h);
+  else
+    k_xyb<false, JXG_XYB_SCALAR><<<grid, 256, 0, ctx->stream>>>(d_rgb, row_bytes, channels, width, height, ctx->d_mlut, d_xyb, stride, padded_h);
+  LAUNCH_CHECK("k_xyb");
+  return JXG_OK;
+}
+
+static int fill_qf(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h, uint32_t W,
+                   uint32_t H, float distance, float inv_scale, int fused, uint8_t *d_qf) {
+  const int xb = (W + 7) / 8, yb = (H + 7) / 8;
+  const int pre_w = 2 * xb, pre_h = 2 * yb;
+  int rc = ensure(&ctx->d_pre, &ctx->pre_cap, (size_t)pre_w * pre_h);
+  if (rc) return rc;
+  const float *yp = d_xyb + (size_t)stride * padded_h;
+  const dim3 g1((pre_w + 255) / 256, pre_h);
+  if (fused)
+    k_aq_pre<true><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, ctx->d_pre);
+  else
+    k_aq_pre<false><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, ctx->d_pre);
+  LAUNCH_CHECK("k_aq_pre");
+  const dim3 g2((xb + 63) / 64, yb);
+  if (fused)
+    k_aq_block<true><<<g2, 64, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, ctx->d_pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
+  else
+    k_aq_block<false><<<g2, 64, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, ctx->d_pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
+  LAUNCH_CHECK("k_aq_block");
+  return JXG_OK;
+}
+
+int jxg_fill_quant_field_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                             uint32_t width, uint32_t height, float distance, float inv_scale,
+                             int fused, uint8_t *d_qf) {
+  if (!ctx || !d_xyb || !d_qf) return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  int rc = check_image(width, height);
+  if (rc) return rc;
+  if (rc = check_planes(width, height, stride, padded_h)) return rc;
+  if (!(distance > 0.0f) || !(inv_scale > 0.0f))
+    return set_err(JXG_ERR_INVALID_DISTANCE, "distance %g inv_scale %g", (double)distance, (double)inv_scale);
+  return fill_qf(ctx, d_xyb, stride, padded_h, width, height, distance, inv_scale, fused, d_qf);
+}
+
+static int select_impl(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h, uint32_t W,
+                       uint32_t H, uint8_t *d_qf, const int8_t *ytox, const int8_t *ytob,
+                       const jxg_frame_params *p, uint8_t *cells, float *c8, float *c16, float *c32) {
+  int rc = upload_derived(ctx, p);
+  if (rc) return rc;
+  if ((rc = zero_cmaps(ctx, W, H, &ytox, &ytob))) return rc;
+  const int xb = (W + 7) / 8, yb = (H + 7) / 8;
+  const dim3 grid((xb + 3) / 4, (yb + 3) / 4);
+  if (p->fused)
+    k_select<true><<<grid, 64, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+  else
+    k_select<false><<<grid, 64, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+  LAUNCH_CHECK("k_select");
+  return JXG_OK;
+}
+
+int jxg_strategy_costs_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                           uint32_t width, uint32_t height, const uint8_t *d_qf,
+                           const int8_t *d_ytox, const int8_t *d_ytob,
+                           const jxg_frame_params *params, float *d_cost8, float *d_cost16,
+                           float *d_cost32) {
+  if (!ctx || !d_xyb || !d_qf || !d_cost8 || !d_cost16 || !d_cost32)
+    return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  int rc = check_image(width, height);
+  if (rc) return rc;
+  if ((rc = check_planes(width, height, stride, padded_h))) return rc;
+  // The selection kernel rewrites the quant field it reads; give it a copy.
+  const size_t nb = (size_t)((width + 7) / 8) * ((height + 7) / 8);
+  uint8_t *tmp = nullptr;
+  HIP_TRY(hipMallocAsync((void **)&tmp, nb, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(tmp, d_qf, nb, hipMemcpyDeviceToDevice, ctx->stream));
+  rc = select_impl(ctx, d_xyb, stride, padded_h, width, height, tmp, d_ytox, d_ytob, params, nullptr,
+                   d_cost8, d_cost16, d_cost32);
+  (void)hipFreeAsync(tmp, ctx->stream);
+  return rc;
+}
+
+int jxg_select_strategy_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                            uint32_t width, uint32_t height, uint8_t *d_qf, const int8_t *d_ytox,
+                            const int8_t *d_ytob, const jxg_frame_params *params,
+                            uint8_t *d_cells) {
+  if (!ctx || !d_xyb || !d_qf || !d_cells) return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  int rc = check_image(width, height);
+  if (rc) return rc;
+  if ((rc = check_planes(width, height, stride, padded_h))) return rc;
+  return select_impl(ctx, d_xyb, stride, padded_h, width, height, d_qf, d_ytox, d_ytob, params,
+                     d_cells, nullptr, nullptr, nullptr);
+}
+
+int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                             uint32_t width, uint32_t height, const uint8_t *d_cells,
+                             const uint8_t *d_qf, const int8_t *d_ytox, const int8_t *d_ytob,
+                             const jxg_frame_params *params, int32_t *d_levels[3],
+                             float *d_coeffs[3], int16_t *d_dc[3]) {
+  if (!ctx || !d_xyb || !d_cells || !d_qf || !d_levels || !d_coeffs || !d_dc)
+    return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  for (int c = 0; c < 3; c++)
+    if (!d_levels[c] || !d_coeffs[c] || !d_dc[c]) return set_err(JXG_ERR_INVALID_ARG, "null output");
+  int rc = check_image(width, height);
+  if (rc) return rc;
+  if ((rc = check_planes(width, height, stride, padded_h))) return rc;
+  if ((rc = upload_derived(ctx, params))) return rc;
+  if ((rc = zero_cmaps(ctx, width, height, &d_ytox, &d_ytob))) return rc;
+  const int xb = (width + 7) / 8, yb = (height + 7) / 8;
+  // At most one crossing transform per 4 blocks (the smallest multi-block
+  // transform covers two blocks, and a crossing one at least two).
+  const size_t cap = (size_t)xb * yb / 2 + 1;
+  if ((rc = ensure(&ctx->d_xt, &ctx->xt_cap, cap))) return rc;
+  k_zero_u32<<<1, 64, 0, ctx->stream>>>(ctx->d_misc, 4);
+  LAUNCH_CHECK("k_zero_u32");
+  QArgs A;
+  A.xyb = d_xyb;
+  A.stride = stride;
+  A.padded_h = padded_h;
+  A.W = width;
+  A.H = height;
+  A.xb = xb;
+  A.yb = yb;
+  A.cells = d_cells;
+  A.qf = d_qf;
+  A.ytox = d_ytox;
+  A.ytob = d_ytob;
+  A.tables = ctx->d_tables;
+  A.D = ctx->d_derived;
+  A.lv0 = d_levels[0];
+  A.lv1 = d_levels[1];
+  A.lv2 = d_levels[2];
+  A.co0 = d_coeffs[0];
+  A.co1 = d_coeffs[1];
+  A.co2 = d_coeffs[2];
+  A.dc0 = d_dc[0];
+  A.dc1 = d_dc[1];
+  A.dc2 = d_dc[2];
+  A.xt_count = ctx->d_misc;
+  A.bad = ctx->d_misc + 1;
+  A.xt_list = ctx->d_xt;
+  A.xt_cap = (uint32_t)cap;
+  const dim3 grid((xb + 7) / 8, (yb + 7) / 8);
+  if (params->fused)
+    k_quantize<true><<<grid, 256, 0, ctx->stream>>>(A);
+  else
+    k_quantize<false><<<grid, 256, 0, ctx->stream>>>(A);
+  LAUNCH_CHECK("k_quantize");
+  const int xt_grid = 256;
+  if (params->fused)
+    k_quantize_xt<true><<<xt_grid, 256, 0, ctx->stream>>>(A);
+  else
+    k_quantize_xt<false><<<xt_grid, 256, 0, ctx->stream>>>(A);
+  LAUNCH_CHECK("k_quantize_xt");
+  return JXG_OK;
+}
+
+int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channels,
+                      uint32_t width, uint32_t height, const jxg_frame_params *params,
+                      const int8_t *d_ytox, const int8_t *d_ytob, uint8_t *d_qf,
+                      uint8_t *d_cells, int32_t *d_levels[3], float *d_coeffs[3],
+                      int16_t *d_dc[3]) {
+  if (!ctx || !d_rgb || !params || !d_qf || !d_cells) return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  int rc = check_image(width, height);
+  if (rc) return rc;
+  uint32_t stride, ph;
+  jxg_plane_geometry(width, height, &stride, &ph);
+  if ((rc = ensure(&ctx->d_xyb, &ctx->xyb_cap, (size_t)3 * stride * ph))) return rc;
+  // XybMatrix::SPEC (src/xyb.rs:35-47,81-96)
+  const float M00 = 0.30f, M02 = 0.078f, M01 = 1.0f - M02 - M00;
+  const float M10 = 0.23f, M12 = 0.078f, M11 = 1.0f - M12 - M10;
+  const float B_BIAS = 0.55180986f;
+  const float B_R_RATIO = 0.24342269f / (0.24342269f + 0.20476745f);
+  const float M20 = B_R_RATIO * (1.0f - B_BIAS);
+  const float M21 = (1.0f - B_R_RATIO) * (1.0f - B_BIAS);
+  const float M22 = 1.0f - M20 - M21;
+  const float fwd[9] = {M00, M01, M02, M10, M11, M12, M20, M21, M22};
+  float lut[256];
+  jxg_srgb_lut(lut);
+  if ((rc = jxg_xyb_dev(ctx, d_rgb, row_bytes, channels, width, height, fwd, lut, JXG_XYB_SCALAR,
+                        params->fused, ctx->d_xyb, stride, ph)))
+    return rc;
+  if ((rc = upload_derived(ctx, params))) return rc;
+  if ((rc = fill_qf(ctx, ctx->d_xyb, stride, ph, width, height, params->distance,
+                    ctx->h_derived->inv_scale, params->fused, d_qf)))
+    return rc;
+  if ((rc = jxg_select_strategy_dev(ctx, ctx->d_xyb, stride, ph, width, height, d_qf, d_ytox, d_ytob,
+                                    params, d_cells)))
+    return rc;
+  return jxg_forward_quantize_dev(ctx, ctx->d_xyb, stride, ph, width, height, d_cells, d_qf, d_ytox,
+                                  d_ytob, params, d_levels, d_coeffs, d_dc);
+}
+
+int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channels, uint32_t width,
+                  uint32_t height, const jxg_frame_params *params, const int8_t *ytox,
+                  const int8_t *ytob, uint8_t *qf, uint8_t *cells, int32_t *levels[3],
+                  float *coeffs[3], int16_t *dc[3]) {
+  if (!ctx || !rgb || !params || !qf || !cells || !levels || !coeffs || !dc)
+    return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  int rc = check_image(width, height);
+  if (rc) return rc;
+  if (channels != 3 && channels != 4) return set_err(JXG_ERR_INVALID_ARG, "channels %d", channels);
+  const size_t xb = (width + 7) / 8, yb = (height + 7) / 8, nb = xb * yb;
+  const size_t tiles = (size_t)((width + 63) / 64) * ((height + 63) / 64);
+  const size_t in_bytes = (size_t)height * row_bytes;
+  if ((rc = ensure(&ctx->d_rgb, &ctx->rgb_cap, in_bytes))) return rc;
+  // one device block for every output: levels, coeffs, dc, qf, cells, maps
+  const size_t bytes = nb * 64 * 3 * (sizeof(int32_t) + sizeof(float)) + nb * 3 * sizeof(int16_t) +
+                       2 * nb + 2 * tiles + 256;
+  uint8_t *dev = nullptr;
+  HIP_TRY(hipMalloc((void **)&dev, bytes));
+  uint8_t *p = dev;
+  int32_t *d_lv[3];
+  float *d_co[3];
+  int16_t *d_dc[3];
+  for (int c = 0; c < 3; c++) {
+    d_lv[c] = (int32_t *)p;
+    p += nb * 64 * sizeof(int32_t);
+  }
+  for (int c = 0; c < 3; c++) {
+    d_co[c] = (float *)p;
+    p += nb * 64 * sizeof(float);
+  }
+  for (int c = 0; c < 3; c++) {
+    d_dc[c] = (int16_t *)p;
+    p += nb * sizeof(int16_t);
+  }
+  uint8_t *d_qf = p;
+  p += nb;
+  uint8_t *d_cells = p;
+  p += nb;
+  int8_t *d_tx = (int8_t *)p;
+  p += tiles;
+  int8_t *d_tb = (int8_t *)p;
+  auto run = [&]() -> int {
+    HIP_TRY(hipMemcpyAsync(ctx->d_rgb, rgb, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (ytox) HIP_TRY(hipMemcpyAsync(d_tx, ytox, tiles, hipMemcpyHostToDevice, ctx->stream));
+    else HIP_TRY(hipMemsetAsync(d_tx, 0, tiles, ctx->stream));
+    if (ytob) HIP_TRY(hipMemcpyAsync(d_tb, ytob, tiles, hipMemcpyHostToDevice, ctx->stream));
+    else HIP_TRY(hipMemsetAsync(d_tb, 0, tiles, ctx->stream));
+    int r = jxg_front_end_dev(ctx, ctx->d_rgb, row_bytes, channels, width, height, params, d_tx,
+                              d_tb, d_qf, d_cells, d_lv, d_co, d_dc);
+    if (r) return r;
+    for (int c = 0; c < 3; c++) {
+      HIP_TRY(hipMemcpyAsync(levels[c], d_lv[c], nb * 64 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipMemcpyAsync(coeffs[c], d_co[c], nb * 64 * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(hipMemcpyAsync(dc[c], d_dc[c], nb * sizeof(int16_t), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(qf, d_qf, nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(cells, d_cells, nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return JXG_OK;
+  };
+  rc = run();
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(dev);
+  return rc;
+}
+
+// Device-side status of the last jxg_forward_quantize_dev call:
+// out[0] = transforms that crossed their tile, out[1] = 1 if an unsupported
+// strategy cell was seen (its blocks are left unwritten).  Synchronizes.
+int jxg_quantize_status(jxg_ctx *ctx, uint32_t out[2]) {
+  if (!ctx || !out) return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  uint32_t h[4];
+  HIP_TRY(hipMemcpyAsync(h, ctx->d_misc, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  out[0] = h[0];
+  out[1] = h[1];
+  return JXG_OK;
+}
+
+}  // extern "C"
+

@@ -1,0 +1,122 @@
+"""Seeded synthetic images for the front end's workloads.
+
+* photo_like(w, h, seed): 1/f-spectrum noise (random phases, amplitude
+  proportional to 1/|k|, mean 128) + 40 flat axis-aligned rectangles with
+  sharp edges + Gaussian noise sigma 3, clamped to u8 (the 2048x2048 q75
+  workload and the batch/gradient workloads).
+* gradient_sines(w, h, seed): diagonal gradient 0..255 + three sinusoids of
+  amplitude 20 and period 16-64 px in random directions + Gaussian noise
+  sigma 4 (the 256x256 q90 workload).
+* smooth_rgba(w, h, seed): 1/f noise box-smoothed over 32-128 px regions with
+  an alpha of 0/255 inside 20 soft-edged ellipses (the 7680x4320 RGBA case).
+
+PRNG: numpy PCG64 seeded with the workload's seed; all float math in f64,
+rounded to u8 at the end.
+"""
+import hashlib
+
+import numpy as np
+
+
+def _rng(seed):
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def _one_over_f(rng, w, h, channels=3, contrast=48.0):
+    out = np.empty((h, w, channels))
+    fy = np.fft.fftfreq(h)[:, None]
+    fx = np.fft.rfftfreq(w)[None, :]
+    k = np.sqrt(fx * fx + fy * fy)
+    k[0, 0] = 1.0
+    amp = 1.0 / k
+    amp[0, 0] = 0.0
+    for c in range(channels):
+        phase = rng.uniform(0.0, 2.0 * np.pi, size=amp.shape)
+        spec = amp * np.exp(1j * phase)
+        img = np.fft.irfft2(spec, s=(h, w))
+        img /= img.std() + 1e-12
+        out[..., c] = img * contrast
+    # correlate the channels a little, as photos are
+    mix = out.mean(axis=2, keepdims=True)
+    return 128.0 + 0.6 * mix + 0.4 * out
+
+
+def photo_like(w, h, seed):
+    rng = _rng(seed)
+    img = _one_over_f(rng, w, h)
+    for _ in range(40):
+        rw = int(rng.integers(max(2, w // 64), max(3, w // 4)))
+        rh = int(rng.integers(max(2, h // 64), max(3, h // 4)))
+        x0 = int(rng.integers(0, max(1, w - rw)))
+        y0 = int(rng.integers(0, max(1, h - rh)))
+        img[y0:y0 + rh, x0:x0 + rw, :] = rng.uniform(0, 255, size=3)
+    img += rng.normal(0.0, 3.0, size=img.shape)
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8)
+
+
+def gradient_sines(w, h, seed):
+    rng = _rng(seed)
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    base = (x + y) * 255.0 / max(1, (w + h - 2))
+    img = np.repeat(base[..., None], 3, axis=2)
+    for _ in range(3):
+        period = rng.uniform(16.0, 64.0)
+        theta = rng.uniform(0.0, np.pi)
+        phase = rng.uniform(0.0, 2.0 * np.pi)
+        wave = 20.0 * np.sin(2.0 * np.pi * (x * np.cos(theta) + y * np.sin(theta)) / period + phase)
+        img += wave[..., None] * rng.uniform(0.5, 1.0, size=3)
+    img += rng.normal(0.0, 4.0, size=img.shape)
+    return np.clip(np.rint(img), 0, 255).astype(np.uint8)
+
+
+def _box_smooth(img, r):
+    # separable running mean with edge replication
+    out = img
+    for axis in (0, 1):
+        pad = [(0, 0)] * out.ndim
+        pad[axis] = (r, r)
+        p = np.pad(out, pad, mode="edge")
+        c = np.cumsum(p, axis=axis)
+        c = np.concatenate([np.zeros_like(np.take(c, [0], axis=axis)), c], axis=axis)
+        n = out.shape[axis]
+        hi = np.take(c, np.arange(2 * r + 1, 2 * r + 1 + n), axis=axis)
+        lo = np.take(c, np.arange(0, n), axis=axis)
+        out = (hi - lo) / (2 * r + 1)
+    return out
+
+
+def smooth_rgba(w, h, seed):
+    rng = _rng(seed)
+    rgb = _one_over_f(rng, w, h, contrast=60.0) - 128.0
+    r = int(rng.integers(16, 65))
+    rgb = 128.0 + _box_smooth(rgb, r) * 3.0
+    rgb += rng.normal(0.0, 2.0, size=rgb.shape)
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    alpha = np.zeros((h, w))
+    for _ in range(20):
+        cx, cy = rng.uniform(0, w), rng.uniform(0, h)
+        ax, ay = rng.uniform(w / 40, w / 6), rng.uniform(h / 40, h / 6)
+        d = np.sqrt(((x - cx) / ax) ** 2 + ((y - cy) / ay) ** 2)
+        edge = np.clip((1.0 - d) * min(ax, ay) / 4.0, 0.0, 1.0)  # 4 px falloff
+        alpha = np.maximum(alpha, edge)
+    out = np.empty((h, w, 4))
+    out[..., :3] = rgb
+    out[..., 3] = alpha * 255.0
+    return np.clip(np.rint(out), 0, 255).astype(np.uint8)
+
+
+def sha256(img):
+    return hashlib.sha256(np.ascontiguousarray(img).tobytes()).hexdigest()
+
+
+WORKLOADS = {
+    # name: (generator, width, height, seed, quality)
+    "gradient_256_q90": (gradient_sines, 256, 256, 1, 90.0),
+    "photo_2048_q75": (photo_like, 2048, 2048, 2, 75.0),
+    "smooth_rgba_7680x4320_q75": (smooth_rgba, 7680, 4320, 3, 75.0),
+}
+
+
+def workload(name):
+    gen, w, h, seed, q = WORKLOADS[name]
+    return gen(w, h, seed), q

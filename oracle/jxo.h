@@ -1,0 +1,125 @@
+/*
+ * jxo.h - CPU oracle of the VarDCT encoder front end.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * encoder's scalar CPU path (awxkee/jixel 0.5.14, Rust; no Rust toolchain
+ * exists in this environment, so the reference itself cannot run).  Only the
+ * tests, the smoke check and the benchmark's CPU-baseline leg use it, as the
+ * checker; the product library never links it.
+ *
+ * Every function follows the operation order of the reference function it
+ * cites.  The reference's `fmla` (src/dct.rs:33-53) is a fused multiply-add on
+ * aarch64 and FMA-enabled x86 builds and a*b+c elsewhere; jxo_set_fused()
+ * selects which.  Built with -ffp-contract=off so no other operation fuses.
+ *
+ * Images are planar f32 of W x H with stride W (the reference's Image3F,
+ * src/image.rs:89-93); footprints past the image are gathered with edge
+ * replication as the reference does (src/ac_strategy.rs:451-479).
+ */
+#ifndef JXO_H_
+#define JXO_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../include/jxg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void jxo_set_fused(int fused);
+int jxo_get_fused(void);
+
+/* ---- scalar helpers (pinned by the reference's known-answer tests) ---- */
+float jxo_cbrtf(float x);                       /* src/xyb.rs:152 */
+float jxo_fast_exp2(float d);                   /* src/adaptive_quant.rs:739 */
+float jxo_dirty_log2f(float d);                 /* src/adaptive_quant.rs:768 */
+float jxo_fast_exp(float x);                    /* src/util.rs:342 */
+float jxo_distance_from_quality(float quality); /* src/encode_image.rs:1321 */
+void jxo_srgb_lut(float out[256]);              /* src/color.rs:35 */
+
+/* compute_distance_params (src/frame/quantization.rs:89-149). */
+void jxo_distance_params(float distance, int32_t *global_scale, float *scale,
+                         int32_t *quant_dc, float *scale_dc, uint32_t *x_qm_scale,
+                         uint32_t *epf_iters, float dc_step[3]);
+
+/* ---- XYB ---- */
+/* One pixel, scalar order (src/xyb.rs:172) or AVX2 production order
+ * (src/avx/xyb.rs:234, SCALAR_ORDER=false, FUSED=true). */
+void jxo_xyb_pixel(const float m[9], int order, float r, float g, float b, float out[3]);
+/* u8 RGB(A) -> XYB planes (3 * W * H floats, stride W). */
+void jxo_xyb_image(const uint8_t *rgb, size_t row_bytes, int channels, int width,
+                   int height, const float m[9], int order, float *xyb);
+
+/* ---- adaptive quantization ---- */
+/* fill_quant_field_scalar for one DC group (src/adaptive_quant.rs:547-729). */
+void jxo_fill_quant_field_group(const float *xyb, int width, int height, int x0, int y0,
+                                int xblocks, int yblocks, float distance, float inv_scale,
+                                uint8_t *qf /* xblocks*yblocks, stride xblocks */);
+/* All DC groups (2048 px) into one xblocks_total-wide field. */
+void jxo_fill_quant_field_image(const float *xyb, int width, int height, float distance,
+                                float inv_scale, uint8_t *qf);
+
+/* ---- transforms ---- */
+/* Forward transform of `strategy` at pixel (px, py) of one plane
+ * (forward_transform, src/ac_strategy.rs:500); returns size (cx*cy*64). */
+int jxo_forward_transform(int strategy, const float *plane, int width, int height, int px,
+                          int py, float *out);
+/* DC values of one channel's coefficients (dc_from_coefficients,
+ * src/group.rs:183). */
+void jxo_dc_from_coefficients(int strategy, const float *coeffs, float *dc);
+int jxo_covered_x(int strategy);
+int jxo_covered_y(int strategy);
+
+/* ---- quantization ---- */
+void jxo_ac_thresholds(int c, int xsize, int ysize, float distance, float qm_mult,
+                       float thr[4]); /* src/group.rs:420 */
+void jxo_quantize_block_ac(const float *in, int c, const float *qm, int32_t quant,
+                           float scale, float qm_mult, float distance, int xsize,
+                           int ysize, int32_t *out); /* src/group.rs:560 */
+float jxo_dequant_level(float one, float numerator, int32_t q); /* src/group.rs:770 */
+
+/* First-pass transform + quantization of every first block
+ * (write_ac_group transform half, src/group.rs:1259-2240, no RDOQ). */
+void jxo_forward_quantize(const float *xyb, int width, int height, const uint8_t *cells,
+                          const uint8_t *qf, const int8_t *ytox, const int8_t *ytob,
+                          const float *tables, const jxg_frame_params *p, int32_t *levels[3],
+                          float *coeffs[3], int16_t *dc[3]);
+
+/* ---- strategy search ---- */
+/* Natural scan position of every raw index of a w x h coefficient block
+ * (coeff_order::scan_pos_lut, src/coeff_order.rs:358). */
+void jxo_scan_pos_lut(int w, int h, uint32_t *lut);
+/* Natural order (scan position -> raw index) of a cx>=cy block
+ * (natural_scan_generic, src/coeff_order.rs:206). */
+void jxo_natural_scan(int cx, int cy, uint32_t *out);
+/* strategy_cost with the coefficient model (src/ac_strategy.rs:724,1293). */
+float jxo_strategy_cost(const float *xyb, int width, int height, int strategy, int px, int py,
+                        float qac, float meta_r, const float cmap[3], const float *tables,
+                        const jxg_frame_params *p);
+/* Square-merge selection for the whole image (select_band per DC group,
+ * src/ac_strategy/selection.rs:1000) followed by adjust_quant_field (:503).
+ * Updates qf in place and writes strategy cells. */
+void jxo_select_strategy(const float *xyb, int width, int height, uint8_t *qf,
+                         const int8_t *ytox, const int8_t *ytob, const float *tables,
+                         const jxg_frame_params *p, uint8_t *cells);
+/* Square-candidate cost maps (DCT8 per block, DCT16X16 per 2x2 site,
+ * DCT32X32 per 4x4 site; NaN where not evaluated). */
+void jxo_strategy_costs(const float *xyb, int width, int height, const uint8_t *qf,
+                        const int8_t *ytox, const int8_t *ytob, const float *tables,
+                        const jxg_frame_params *p, float *cost8, float *cost16,
+                        float *cost32);
+
+/* Whole front end on the CPU (XYB -> AQ -> selection -> quantize) with
+ * `threads` worker threads; used as the benchmark's CPU baseline. */
+void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width, int height,
+                   const jxg_frame_params *p, const float *tables, const int8_t *ytox,
+                   const int8_t *ytob, int threads, uint8_t *qf, uint8_t *cells,
+                   int32_t *levels[3], float *coeffs[3], int16_t *dc[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JXO_H_ */

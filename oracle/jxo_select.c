@@ -1,0 +1,782 @@
+/*
+ * jxo_select.c - oracle: coefficient-model strategy costs and the
+ * square-merge transform selection (src/ac_strategy.rs,
+ * src/ac_strategy/selection.rs, src/inflated_cost.rs, src/coeff_order.rs),
+ * plus the whole-front-end CPU driver used as the benchmark baseline.
+ * TEST INFRASTRUCTURE ONLY (see jxo.h).
+ */
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "jxo.h"
+#include "jxo_internal.h"
+
+float jxo_b_qm_mul(uint32_t b_qm_scale);
+float jxo_x_qm_mul(uint32_t x_qm_scale);
+void jxo_forward_quantize_rows(const float *xyb, int width, int height, const uint8_t *cells,
+                               const uint8_t *qf, const int8_t *ytox, const int8_t *ytob,
+                               const float *tables, const jxg_frame_params *p, int32_t *levels[3],
+                               float *coeffs[3], int16_t *dc[3], int by0, int by1);
+
+/* ---- quant-table blob layout (include/jxg.h) ---- */
+static const uint32_t QT_SIZES[13] = {64, 128, 256, 1024, 512, 256, 4096, 2048, 64, 64, 64, 64, 64};
+uint32_t jxo_qt_size(int kind) { return QT_SIZES[kind]; }
+uint32_t jxo_qt_offset(int kind) {
+  uint32_t off = 0;
+  for (int k = 0; k < kind; k++) off += 6 * QT_SIZES[k];
+  return off;
+}
+int jxo_qt_kind(int s) {
+  switch (s) {
+    case JXG_DCT: return JXG_QT_DCT8;
+    case JXG_DCT16X8:
+    case JXG_DCT8X16: return JXG_QT_DCT16X8;
+    case JXG_DCT16X16: return JXG_QT_DCT16X16;
+    case JXG_DCT32X32: return JXG_QT_DCT32X32;
+    case JXG_DCT32X16:
+    case JXG_DCT16X32: return JXG_QT_DCT32X16;
+    case JXG_DCT32X8:
+    case JXG_DCT8X32: return JXG_QT_DCT8X32;
+    case JXG_DCT64X64: return JXG_QT_DCT64X64;
+    case JXG_DCT64X32:
+    case JXG_DCT32X64: return JXG_QT_DCT64X32;
+    case JXG_IDENTITY: return JXG_QT_IDENTITY;
+    case JXG_DCT2X2: return JXG_QT_DCT2X2;
+    case JXG_DCT4X4: return JXG_QT_DCT4X4;
+    case JXG_DCT4X8:
+    case JXG_DCT8X4: return JXG_QT_DCT4X8;
+    case JXG_AFV0:
+    case JXG_AFV1:
+    case JXG_AFV2:
+    case JXG_AFV3: return JXG_QT_AFV;
+  }
+  return -1;
+}
+
+/* ---- natural coefficient orders (src/coeff_order.rs:206-248,344-415) ---- */
+void jxo_natural_scan(int cx, int cy, uint32_t *out) {
+  int size = cx * cy * 64, side = cx * 8;
+  int xs = cx / cy, xsm = xs - 1, xss = 0;
+  while ((1 << xss) < xs) xss++;
+  int cur = cx * cy;
+  for (int i = 0; i < side; i++) {
+    for (int j = 0; j <= i; j++) {
+      int x = j, y = i - j;
+      if (i % 2 != 0) {
+        int t = x;
+        x = y;
+        y = t;
+      }
+      if (y & xsm) continue;
+      y >>= xss;
+      int val;
+      if (x < cx && y < cy)
+        val = y * cx + x;
+      else
+        val = cur++;
+      out[val] = (uint32_t)(y * side + x);
+    }
+  }
+  for (int ip = side - 1; ip >= 1; ip--) {
+    int i = ip - 1;
+    for (int j = 0; j <= i; j++) {
+      int x = side - 1 - (i - j), y = side - 1 - j;
+      if (i % 2 != 0) {
+        int t = x;
+        x = y;
+        y = t;
+      }
+      if (y & xsm) continue;
+      y >>= xss;
+      out[cur++] = (uint32_t)(y * side + x);
+    }
+  }
+  (void)size;
+}
+
+void jxo_scan_pos_lut(int w, int h, uint32_t *lut) {
+  int big = w > h ? w : h, small = w > h ? h : w;
+  int size = w * h;
+  uint32_t *order = (uint32_t *)malloc(size * sizeof(uint32_t));
+  uint32_t *canon = (uint32_t *)malloc(size * sizeof(uint32_t));
+  jxo_natural_scan(big / 8, small / 8, order);
+  for (int k = 0; k < size; k++) canon[order[k]] = (uint32_t)k;
+  if (w >= h) {
+    memcpy(lut, canon, size * sizeof(uint32_t));
+  } else {
+    for (int y = 0; y < h; y++)
+      for (int x = 0; x < w; x++) lut[y * w + x] = canon[x * h + y];
+  }
+  free(order);
+  free(canon);
+}
+
+/* ---- rate model (src/inflated_cost.rs:43-76,435-560) ---- */
+static float g_rate_lut[1024];
+static int g_rate_ready = 0;
+static void rate_lut_init(void) {
+  if (g_rate_ready) return;
+  for (int i = 0; i < 1024; i++) g_rate_lut[i] = log2f(1.0f + (float)i);
+  g_rate_ready = 1;
+}
+static float rate_log2(float qabs) {
+  uint32_t k = qabs >= 4294967295.0f ? 0xffffffffu : (uint32_t)qabs;
+  if (k < 1024) return g_rate_lut[k];
+  return log2f(1.0f + qabs);
+}
+
+/* DequantBias::level_f32 (src/group.rs:748-768) */
+static float level_f32(float one, float numerator, float quant) {
+  float aq = fabsf(quant);
+  if (aq < 1.125f) return quant == 0.0f ? 0.0f : copysignf(one, quant);
+  uint32_t a = (uint32_t)aq;
+  if (a < 64 && (float)a == aq) {
+    float t = numerator / (float)a;
+    return quant - (quant < 0.0f ? -t : t);
+  }
+  return quant - numerator / quant;
+}
+
+typedef struct {
+  float sse, mag_bits;
+  uint32_t nzeros, max_scan;
+} sse_rate;
+
+static sse_rate sse_and_rate(const float *coeff, const float *inv, float q_scaled, float one,
+                             float numerator, int width, int height, int cx, int cy,
+                             const float thr[4], const uint32_t *scan_pos) {
+  sse_rate r = {0.0f, 0.0f, 0, 0};
+  int half = width / 2;
+  for (int y = 0; y < height; y++) {
+    int yfix = y >= height / 2 ? 2 : 0;
+    for (int x = 0; x < width; x++) {
+      if (x < cx && y < cy) continue;
+      int i = y * width + x;
+      float threshold = x >= half ? thr[yfix + 1] : thr[yfix];
+      float a = inv[i] * q_scaled * coeff[i];
+      float q = fabsf(a) >= threshold ? roundf(a) : 0.0f;
+      float d = a - level_f32(one, numerator, q);
+      r.sse += d * d;
+      if (q != 0.0f) {
+        r.nzeros++;
+        r.mag_bits += rate_log2(fabsf(q));
+        if (scan_pos[i] > r.max_scan) r.max_scan = scan_pos[i];
+      }
+    }
+  }
+  return r;
+}
+
+static float model_bits(uint32_t nzeros, float mag_bits, uint32_t max_scan, int cx, int cy) {
+  float header = 0.4f * rate_log2((float)nzeros);
+  float visited = 0.0f;
+  if (nzeros != 0) {
+    int64_t span = (int64_t)max_scan + 1 - (int64_t)(cx * cy + nzeros);
+    visited = span > 0 ? (float)span : 0.0f;
+  }
+  return (float)nzeros * 1.6f + 1.0f * mag_bits + header + 0.5f * visited;
+}
+
+/* erased_detail / selection_quantizer::scale
+ * (src/ac_strategy/selection_quantizer.rs:46-120).  The gate `enabled`
+ * holds at the Fast tiers for 1 <= d <= 4 with the SSIMULACRA2 arbiter. */
+static int selection_quantizer_enabled(const jxg_frame_params *p) {
+  if (!(p->distance >= 1.0f && p->distance <= 4.0f)) return 0;
+  return p->speed == JXG_SPEED_FAST || p->speed == JXG_SPEED_FASTEST;
+}
+
+static float selection_scale(int s, const float *luma, const float *inv_y, float qac,
+                             const jxg_frame_params *p, float meta_r) {
+  if (meta_r == 0.0f || !(s == JXG_DCT32X32 || s == JXG_DCT32X16 || s == JXG_DCT16X32) ||
+      !selection_quantizer_enabled(p))
+    return 1.0f;
+  int cxs = jxo_covered_x(s), cys = jxo_covered_y(s);
+  int cx = cxs > cys ? cxs : cys, cy = cxs > cys ? cys : cxs;
+  int width = cx * 8;
+  double energy = 0.0;
+  for (int y = 0; y < cy * 8; y++)
+    for (int x = 0; x < width; x++) {
+      if (x < cx * 2 && y < cy * 2) continue;
+      int i = y * width + x;
+      if (fabsf(luma[i] * inv_y[i] * qac) < 0.5f) energy += (double)luma[i] * (double)luma[i];
+    }
+  float erased = (float)energy;
+  float d = p->distance;
+  float t = erased / (2e-4f * d * d) - 1.0f;
+  t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+  return 1.0f + 0.6f * t;
+}
+
+#define RD_LAMBDA 0.08086717f
+#define META_R 4.0f
+
+static uint32_t *g_scan_luts[64];
+static pthread_mutex_t g_scan_mu = PTHREAD_MUTEX_INITIALIZER;
+static const uint32_t *scan_lut(int w, int h) {
+  int key = (w / 8 - 1) * 8 + (h / 8 - 1);
+  pthread_mutex_lock(&g_scan_mu);
+  if (!g_scan_luts[key]) {
+    uint32_t *l = (uint32_t *)malloc((size_t)w * h * sizeof(uint32_t));
+    jxo_scan_pos_lut(w, h, l);
+    g_scan_luts[key] = l;
+  }
+  pthread_mutex_unlock(&g_scan_mu);
+  return g_scan_luts[key];
+}
+
+float jxo_strategy_cost(const float *xyb, int width, int height, int s, int px, int py,
+                        float qac, float meta_r, const float cmap[3], const float *tables,
+                        const jxg_frame_params *p) {
+  rate_lut_init();
+  size_t plane = (size_t)width * height;
+  static __thread float co[3][64 * 64];
+  int size = 0;
+  for (int c = 0; c < 3; c++)
+    size = jxo_forward_transform(s, xyb + c * plane, width, height, px, py, co[c]);
+  float ncx = -cmap[0], ncb = -cmap[2];
+  for (int i = 0; i < size; i++) {
+    co[0][i] = jxo_fmla(ncx, co[1][i], co[0][i]);
+    co[2][i] = jxo_fmla(ncb, co[1][i], co[2][i]);
+  }
+  int cxs = jxo_covered_x(s), cys = jxo_covered_y(s);
+  int cx = cxs > cys ? cxs : cys, cy = cxs > cys ? cys : cxs;
+  int kind = jxo_qt_kind(s);
+  uint32_t n = jxo_qt_size(kind), off = jxo_qt_offset(kind);
+  const float *inv[3] = {tables + off + 3 * n, tables + off + 4 * n, tables + off + 5 * n};
+  qac = qac * selection_scale(s, co[1], inv[1], qac, p, meta_r);
+  /* Footprint::visible_fraction (src/inflated_cost.rs:650-660) */
+  int fw = cxs * 8, fh = cys * 8;
+  int vw = width - px < fw ? width - px : fw;
+  int vh = height - py < fh ? height - py : fh;
+  float visible = (vw == fw && vh == fh) ? 1.0f : (float)(vw * vh) / (float)(fw * fh);
+  float qm_mult[3] = {jxo_x_qm_mul(p->x_qm_scale), 1.0f, jxo_b_qm_mul(p->b_qm_scale)};
+  const uint32_t *scan = scan_lut(cx * 8, cy * 8);
+  float d_total = 0.0f, r_total = 0.0f;
+  for (int c = 0; c < 3; c++) {
+    float thr[4];
+    jxo_ac_thresholds(c, cx, cy, p->distance, qm_mult[c], thr);
+    sse_rate r = sse_and_rate(co[c], inv[c], qac * qm_mult[c], p->dequant_one[c],
+                              p->dequant_numerator, cx * 8, cy * 8, cx, cy, thr, scan);
+    d_total += p->channel_weight[c] * r.sse;
+    r_total += model_bits(r.nzeros, r.mag_bits, r.max_scan, cx, cy);
+  }
+  return jxo_fmla(RD_LAMBDA, r_total + meta_r, d_total * visible);
+}
+
+/* ---- merge tuning (src/ac_strategy.rs:87-350) ---- */
+typedef struct {
+  float bias_16x16, bias_32x32, accept_16, accept_32, risk_k, site_k;
+} merge_tuning;
+
+static float clampf_(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+static float banded_at(float hq, float base, float d) {
+  float t = clampf_((d - 0.5f) / (1.5f - 0.5f), 0.0f, 1.0f);
+  return jxo_fmla(t, base - hq, hq);
+}
+
+static float merge_margin(float hq, float base, float d) {
+  float fade = clampf_((d - 1.0f) / (4.0f - 1.0f), 0.0f, 1.0f);
+  return banded_at(hq, base, d) * (1.0f - fade * (1.0f - 0.82f));
+}
+
+static merge_tuning merge_tuning_new(const jxg_frame_params *p) {
+  float d = p->distance;
+  merge_tuning t;
+  t.site_k = (p->speed == JXG_SPEED_FAST || p->speed == JXG_SPEED_FASTEST) ? 0.2f : 0.0f;
+  t.bias_16x16 = banded_at(1.2059275f, 1.05f, d);
+  t.bias_32x32 = banded_at(1.2059275f, 1.06f, d);
+  t.accept_16 = 1.0f - merge_margin(0.19243888f, 0.048f, d);
+  t.accept_32 = 1.0f - merge_margin(0.21398508f, 0.234f, d);
+  t.risk_k = jxo_aq_dampen(d) > 0.0f ? 1.44269504f : 0.0f;
+  float vlq_t = clampf_((d - 4.0f) / (7.5f - 4.0f), 0.0f, 1.0f);
+  if (vlq_t > 0.0f) {
+    t.bias_16x16 = jxo_fmla(vlq_t, 0.880f - t.bias_16x16, t.bias_16x16);
+    t.bias_32x32 = jxo_fmla(vlq_t, 0.867f - t.bias_32x32, t.bias_32x32);
+    t.accept_16 = jxo_fmla(vlq_t, (1.0f - 0.067f) - t.accept_16, t.accept_16);
+    t.accept_32 = jxo_fmla(vlq_t, (1.0f - 0.02925f) - t.accept_32, t.accept_32);
+  }
+  if (d >= 1.5f) t.bias_32x32 *= 0.94f;
+  return t;
+}
+
+/* site_variance_rect / MergeTuning::site_bias (src/ac_strategy.rs:224-245,269-282) */
+static float site_bias(const merge_tuning *t, const float *xyb, int width, int height, int x0,
+                       int y0, int n) {
+  if (t->site_k == 0.0f) return 1.0f;
+  const float *py_ = xyb + (size_t)width * height;
+  double s1 = 0.0, s2 = 0.0;
+  size_t cnt = 0;
+  int ye = y0 + n < height ? y0 + n : height;
+  int xe = x0 + n < width ? x0 + n : width;
+  for (int y = y0; y < ye; y++)
+    for (int x = x0; x < xe; x++) {
+      double v = (double)py_[(size_t)y * width + x];
+      s1 += v;
+      s2 += v * v;
+      cnt++;
+    }
+  float var = 0.0f;
+  if (cnt) {
+    double mean = s1 / (double)cnt;
+    double v = s2 / (double)cnt - mean * mean;
+    var = (float)(v > 0.0 ? v : 0.0);
+  }
+  if (var < 1e-9f) var = 1e-9f;
+  float s = clampf_(jxo_dirty_log2f(var / 0.003f) / 8.0f, -1.0f, 1.0f);
+  return jxo_fast_exp(t->site_k * s);
+}
+
+static float risk_gated(float k, float accept, float q_min, float q_max, float area_scale) {
+  if (q_min <= 0.0f || k <= 0.0f) return accept;
+  float spread = q_max / q_min - 1.0f;
+  return accept * jxo_fast_exp2(-k * area_scale * spread);
+}
+
+static float mean_max_mixer(float d) {
+  float v = 1.0f - fmaxf(d - 1.54138f, 0.0f) * 0.56391f;
+  return fmaxf(v, 0.0f);
+}
+
+static uint8_t aggregate_quant(uint8_t max_q, uint32_t sum, int covered, float d) {
+  if (covered < 4) return max_q;
+  float mixer = mean_max_mixer(d);
+  float mean = (float)sum / (float)covered;
+  float mixed = (float)max_q * mixer + mean * (1.0f - mixer);
+  float v = clampf_(mixed + 0.5f, 1.0f, 255.0f);
+  return (uint8_t)v;
+}
+
+static float aggregate_qac_2x2(const float q[2][2], float scale, float d) {
+  float max_q = fmaxf(fmaxf(fmaxf(q[0][0], q[0][1]), q[1][0]), q[1][1]);
+  float mixer = mean_max_mixer(d);
+  float mean = (q[0][0] + q[0][1] + q[1][0] + q[1][1]) * 0.25f;
+  float mixed_q = (max_q * mixer + mean * (1.0f - mixer)) / scale;
+  return scale * floorf(clampf_(mixed_q + 0.5f, 1.0f, 255.0f));
+}
+
+/* can_place_strategy (src/dc_group_data.rs:171-202), group-relative. */
+static int can_place(int x, int y, int s, int xsize, int ysize) {
+  int cx = jxo_covered_x(s), cy = jxo_covered_y(s);
+  if (x + cx > xsize || y + cy > ysize) return 0;
+  if (x / 32 != (x + cx - 1) / 32) return 0;
+  if (y / 32 != (y + cy - 1) / 32) return 0;
+  if (x / 8 != (x + cx - 1) / 8) return 0;
+  if (y / 8 != (y + cy - 1) / 8) return 0;
+  return 1;
+}
+
+typedef struct {
+  const float *xyb;
+  int width, height;
+  int gx0, gy0; /* DC group origin in blocks */
+  int xsize, ysize;
+  const uint8_t *qf; /* whole-image field */
+  int qf_stride;
+  const int8_t *ytox, *ytob;
+  int tiles_x, tiles_y;
+  const float *tables;
+  const jxg_frame_params *p;
+  merge_tuning mt;
+  uint8_t *cells; /* whole image */
+  float *cost8, *cost16, *cost32;
+} group_ctx;
+
+static void cmap_factors(const group_ctx *g, int bx, int by, float out[3]) {
+  /* group-relative block -> this DC group's tile, clamped to its tile grid */
+  int gtx = g->gx0 / 8, gty = g->gy0 / 8;
+  int ntx = (g->xsize + 7) / 8, nty = (g->ysize + 7) / 8;
+  int tx = bx / 8 < ntx - 1 ? bx / 8 : ntx - 1;
+  int ty = by / 8 < nty - 1 ? by / 8 : nty - 1;
+  size_t i = (size_t)(gty + ty) * g->tiles_x + gtx + tx;
+  float inv_cf = 1.0f / g->p->cfl_color_factor;
+  out[0] = jxo_fmla((float)g->ytox[i], inv_cf, g->p->cfl_base_x);
+  out[1] = 0.0f;
+  out[2] = jxo_fmla((float)g->ytob[i], inv_cf, g->p->cfl_base_b);
+}
+
+static uint8_t qf_at(const group_ctx *g, int bx, int by) {
+  return g->qf[(size_t)(g->gy0 + by) * g->qf_stride + g->gx0 + bx];
+}
+
+static void set_first(group_ctx *g, int x, int y, int s) {
+  int cx = jxo_covered_x(s), cy = jxo_covered_y(s);
+  for (int iy = 0; iy < cy; iy++)
+    for (int ix = 0; ix < cx; ix++)
+      g->cells[(size_t)(g->gy0 + y + iy) * g->qf_stride + g->gx0 + x + ix] =
+          (uint8_t)((s << 1) | (ix == 0 && iy == 0 ? 1 : 0));
+}
+
+static float cost_at(group_ctx *g, int s, int bx, int by, float qac, const float cmap[3]) {
+  return jxo_strategy_cost(g->xyb, g->width, g->height, s, (g->gx0 + bx) * 8, (g->gy0 + by) * 8,
+                           qac, META_R, cmap, g->tables, g->p);
+}
+
+/* select_super_block with SearchScope::for_speed at tiers without
+ * rectangles (src/ac_strategy/selection.rs:238-455): rectangles price as
+ * +inf, so only DCT8 and DCT16X16 compete. */
+static void select_super_block(group_ctx *g, int bx0, int by0, float *chosen, float *dct8) {
+  float qac[2][2];
+  for (int dy = 0; dy < 2; dy++)
+    for (int dx = 0; dx < 2; dx++) qac[dy][dx] = g->p->scale * (float)qf_at(g, bx0 + dx, by0 + dy);
+  float cmap[3];
+  cmap_factors(g, bx0, by0, cmap);
+  float c8[2][2];
+  for (int dy = 0; dy < 2; dy++)
+    for (int dx = 0; dx < 2; dx++) {
+      c8[dy][dx] = cost_at(g, JXG_DCT, bx0 + dx, by0 + dy, qac[dy][dx], cmap);
+      if (g->cost8)
+        g->cost8[(size_t)(g->gy0 + by0 + dy) * g->qf_stride + g->gx0 + bx0 + dx] = c8[dy][dx];
+    }
+  float dct8_left = c8[0][0] + c8[1][0];
+  float dct8_right = c8[0][1] + c8[1][1];
+  float dct8_top = c8[0][0] + c8[0][1];
+  float dct8_bottom = c8[1][0] + c8[1][1];
+  float total_dct8 = dct8_left + dct8_right;
+  float c16_raw = cost_at(g, JXG_DCT16X16, bx0, by0, aggregate_qac_2x2(qac, g->p->scale, g->p->distance),
+                          cmap);
+  if (g->cost16) g->cost16[(size_t)(g->gy0 + by0) * g->qf_stride + g->gx0 + bx0] = c16_raw;
+  float c16 = g->mt.bias_16x16 *
+              site_bias(&g->mt, g->xyb, g->width, g->height, (g->gx0 + bx0) * 8, (g->gy0 + by0) * 8,
+                        16) *
+              c16_raw;
+  float cost_16x8 = dct8_left + dct8_right;
+  float cost_8x16 = dct8_top + dct8_bottom;
+  float best_rect = fminf(cost_16x8, cost_8x16);
+  float q_min = INFINITY, q_max = 0.0f;
+  for (int dy = 0; dy < 2; dy++)
+    for (int dx = 0; dx < 2; dx++) {
+      q_min = fminf(q_min, qac[dy][dx]);
+      q_max = fmaxf(q_max, qac[dy][dx]);
+    }
+  int pick16 = can_place(bx0, by0, JXG_DCT16X16, g->xsize, g->ysize) && c16 < best_rect &&
+               c16 < total_dct8 * risk_gated(g->mt.risk_k, g->mt.accept_16, q_min, q_max, 1.0f);
+  if (pick16) {
+    set_first(g, bx0, by0, JXG_DCT16X16);
+    *chosen = c16;
+  } else {
+    *chosen = cost_16x8 <= cost_8x16 ? cost_16x8 : cost_8x16;
+  }
+  *dct8 = total_dct8;
+}
+
+static uint8_t region_quant(const group_ctx *g, int bx, int by, int w, int h) {
+  uint8_t max_q = 1;
+  uint32_t sum = 0;
+  for (int y = by; y < by + h; y++)
+    for (int x = bx; x < bx + w; x++) {
+      uint8_t q = qf_at(g, x, y);
+      if (q > max_q) max_q = q;
+      sum += q;
+    }
+  return aggregate_quant(max_q, sum, w * h, g->p->distance);
+}
+
+/* choose_merge_partition (src/ac_strategy/selection.rs:115-150) restricted
+ * to the square merge: rectangles price +inf, so masks other than the
+ * square can never pass the `merged < incumbent` gate. */
+static int choose_square32(const float children[2][2], const float baseline[2][2], float merge,
+                           float accept) {
+  float child_total = children[0][0] + children[0][1] + children[1][0] + children[1][1];
+  float base_total = baseline[0][0] + baseline[0][1] + baseline[1][0] + baseline[1][1];
+  float limit = base_total * accept;
+  if (!(merge < child_total && merge < limit)) return 0;
+  return merge < child_total;
+}
+
+/* select_band over one DC group (src/ac_strategy/selection.rs:1000-1180),
+ * square scope, followed by adjust_quant_field (:503). */
+static void select_group(group_ctx *g) {
+  const jxg_frame_params *p = g->p;
+  int xsize = g->xsize, ysize = g->ysize;
+  for (int by = 0; by < ysize; by++)
+    for (int bx = 0; bx < xsize; bx++) set_first(g, bx, by, JXG_DCT);
+  int by = 0;
+  while (by + 1 < ysize) {
+    int four_row = (by % 4 == 0) && by + 4 <= ysize;
+    int bx = 0;
+    while (bx + 1 < xsize) {
+      int four_col = bx % 4 == 0 && bx + 4 <= xsize;
+      if (four_row && four_col && can_place(bx, by, JXG_DCT32X32, xsize, ysize)) {
+        float children[2][2], baseline[2][2];
+        for (int sy = 0; sy < 2; sy++)
+          for (int sx = 0; sx < 2; sx++)
+            select_super_block(g, bx + sx * 2, by + sy * 2, &children[sy][sx], &baseline[sy][sx]);
+        float qac32 = p->scale * (float)region_quant(g, bx, by, 4, 4);
+        float cmap[3];
+        cmap_factors(g, bx, by, cmap);
+        float cost32 = cost_at(g, JXG_DCT32X32, bx, by, qac32, cmap);
+        if (g->cost32) g->cost32[(size_t)(g->gy0 + by) * g->qf_stride + g->gx0 + bx] = cost32;
+        float m32 =
+            site_bias(&g->mt, g->xyb, g->width, g->height, (g->gx0 + bx) * 8, (g->gy0 + by) * 8, 32);
+        float merged = g->mt.bias_32x32 * m32 * cost32;
+        uint8_t q_min = 255, q_max = 0;
+        for (int iy = 0; iy < 4; iy++)
+          for (int ix = 0; ix < 4; ix++) {
+            uint8_t q = qf_at(g, bx + ix, by + iy);
+            if (q < q_min) q_min = q;
+            if (q > q_max) q_max = q;
+          }
+        float gate = risk_gated(g->mt.risk_k, g->mt.accept_32, (float)q_min, (float)q_max, 2.0f);
+        if (choose_square32(children, baseline, merged, gate)) set_first(g, bx, by, JXG_DCT32X32);
+        bx += 4;
+      } else if (four_row) {
+        float ch, d8;
+        select_super_block(g, bx, by, &ch, &d8);
+        select_super_block(g, bx, by + 2, &ch, &d8);
+        bx += 2;
+      } else {
+        float ch, d8;
+        select_super_block(g, bx, by, &ch, &d8);
+        bx += 2;
+      }
+    }
+    by += four_row ? 4 : 2;
+  }
+}
+
+static void adjust_quant_field(group_ctx *g, uint8_t *qf_out) {
+  for (int by = 0; by < g->ysize; by++)
+    for (int bx = 0; bx < g->xsize; bx++) {
+      size_t i = (size_t)(g->gy0 + by) * g->qf_stride + g->gx0 + bx;
+      uint8_t cell = g->cells[i];
+      if (!(cell & 1)) continue;
+      int s = cell >> 1;
+      int cx = jxo_covered_x(s), cy = jxo_covered_y(s);
+      if (cx == 1 && cy == 1) continue;
+      uint8_t max_q = 0;
+      uint32_t sum = 0;
+      for (int iy = 0; iy < cy; iy++)
+        for (int ix = 0; ix < cx; ix++) {
+          uint8_t q = qf_at(g, bx + ix, by + iy);
+          if (q > max_q) max_q = q;
+          sum += q;
+        }
+      uint8_t val = aggregate_quant(max_q, sum, cx * cy, g->p->distance);
+      for (int iy = 0; iy < cy; iy++)
+        for (int ix = 0; ix < cx; ix++)
+          qf_out[(size_t)(g->gy0 + by + iy) * g->qf_stride + g->gx0 + bx + ix] = val;
+    }
+}
+
+static void group_init(group_ctx *g, const float *xyb, int width, int height, int gx, int gy,
+                       const uint8_t *qf, const int8_t *ytox, const int8_t *ytob,
+                       const float *tables, const jxg_frame_params *p, uint8_t *cells) {
+  memset(g, 0, sizeof(*g));
+  g->xyb = xyb;
+  g->width = width;
+  g->height = height;
+  int x0 = gx * 2048, y0 = gy * 2048;
+  int xs = width - x0 < 2048 ? width - x0 : 2048;
+  int ys = height - y0 < 2048 ? height - y0 : 2048;
+  g->gx0 = x0 / 8;
+  g->gy0 = y0 / 8;
+  g->xsize = (xs + 7) / 8;
+  g->ysize = (ys + 7) / 8;
+  g->qf = qf;
+  g->qf_stride = (width + 7) / 8;
+  g->ytox = ytox;
+  g->ytob = ytob;
+  g->tiles_x = (width + 63) / 64;
+  g->tiles_y = (height + 63) / 64;
+  g->tables = tables;
+  g->p = p;
+  g->mt = merge_tuning_new(p);
+  g->cells = cells;
+}
+
+/* fill_ac_strategy returns before scoping when the speed has no square
+ * merges or the distance takes the DCT8-only shortcut
+ * (src/ac_strategy/selection.rs:1930-1935, src/ac_strategy.rs:71-79). */
+static int selection_runs(const jxg_frame_params *p) {
+  if (p->distance <= 0.056713393f) return 0;
+  return p->speed >= JXG_SPEED_FASTEST;
+}
+
+void jxo_select_strategy(const float *xyb, int width, int height, uint8_t *qf,
+                         const int8_t *ytox, const int8_t *ytob, const float *tables,
+                         const jxg_frame_params *p, uint8_t *cells) {
+  int xb = (width + 7) / 8, yb = (height + 7) / 8;
+  if (!selection_runs(p)) {
+    memset(cells, JXG_CELL(JXG_DCT, 1), (size_t)xb * yb);
+    return;
+  }
+  uint8_t *qf_in = (uint8_t *)malloc((size_t)xb * yb);
+  memcpy(qf_in, qf, (size_t)xb * yb);
+  for (int gy = 0; gy * 2048 < height; gy++)
+    for (int gx = 0; gx * 2048 < width; gx++) {
+      group_ctx g;
+      group_init(&g, xyb, width, height, gx, gy, qf_in, ytox, ytob, tables, p, cells);
+      select_group(&g);
+      adjust_quant_field(&g, qf);
+    }
+  free(qf_in);
+}
+
+void jxo_strategy_costs(const float *xyb, int width, int height, const uint8_t *qf,
+                        const int8_t *ytox, const int8_t *ytob, const float *tables,
+                        const jxg_frame_params *p, float *cost8, float *cost16, float *cost32) {
+  int xb = (width + 7) / 8, yb = (height + 7) / 8;
+  size_t n = (size_t)xb * yb;
+  for (size_t i = 0; i < n; i++) cost8[i] = cost16[i] = cost32[i] = NAN;
+  uint8_t *cells = (uint8_t *)malloc(n);
+  for (int gy = 0; gy * 2048 < height; gy++)
+    for (int gx = 0; gx * 2048 < width; gx++) {
+      group_ctx g;
+      group_init(&g, xyb, width, height, gx, gy, qf, ytox, ytob, tables, p, cells);
+      g.cost8 = cost8;
+      g.cost16 = cost16;
+      g.cost32 = cost32;
+      select_group(&g);
+    }
+  free(cells);
+}
+
+/* ------------------------------------------------------------------ */
+/* Whole front end with a thread pool over rows of DC groups / bands.  The */
+/* stage order is the reference's: XYB, AQ per DC group, selection per DC  */
+/* group, then transform + quantize.                                      */
+
+typedef struct {
+  int kind; /* 0 xyb rows, 1 aq group, 2 select group, 3 quantize rows */
+  const uint8_t *rgb;
+  size_t row_bytes;
+  int channels, width, height;
+  const jxg_frame_params *p;
+  const float *tables;
+  const int8_t *ytox, *ytob;
+  float *xyb;
+  uint8_t *qf, *qf_in, *cells;
+  int32_t **levels;
+  float **coeffs;
+  int16_t **dc;
+  int next; /* work counter */
+  int total;
+  pthread_mutex_t mu;
+} fe_job;
+
+static int take(fe_job *j) {
+  pthread_mutex_lock(&j->mu);
+  int i = j->next < j->total ? j->next++ : -1;
+  pthread_mutex_unlock(&j->mu);
+  return i;
+}
+
+static void *fe_worker(void *arg) {
+  fe_job *j = (fe_job *)arg;
+  int W = j->width, H = j->height;
+  int xb = (W + 7) / 8;
+  int gxs = (W + 2047) / 2048;
+  size_t plane = (size_t)W * H;
+  float lut[256];
+  jxo_srgb_lut(lut);
+  float m[9];
+  /* XybMatrix::SPEC (src/xyb.rs:35-47,81-96) */
+  {
+    const float M00 = 0.30f, M02 = 0.078f, M01 = 1.0f - M02 - M00;
+    const float M10 = 0.23f, M12 = 0.078f, M11 = 1.0f - M12 - M10;
+    const float B_BIAS = 0.55180986f;
+    const float B_R_RATIO = 0.24342269f / (0.24342269f + 0.20476745f);
+    const float M20 = B_R_RATIO * (1.0f - B_BIAS);
+    const float M21 = (1.0f - B_R_RATIO) * (1.0f - B_BIAS);
+    const float M22 = 1.0f - M20 - M21;
+    float t[9] = {M00, M01, M02, M10, M11, M12, M20, M21, M22};
+    memcpy(m, t, sizeof(m));
+  }
+  int i;
+  while ((i = take(j)) >= 0) {
+    if (j->kind == 0) {
+      for (int y = i * 8; y < (i + 1) * 8 && y < H; y++) {
+        const uint8_t *row = j->rgb + (size_t)y * j->row_bytes;
+        for (int x = 0; x < W; x++) {
+          const uint8_t *px = row + (size_t)x * j->channels;
+          float o[3];
+          jxo_xyb_pixel(m, JXG_XYB_SCALAR, lut[px[0]], lut[px[1]], lut[px[2]], o);
+          size_t k = (size_t)y * W + x;
+          j->xyb[k] = o[0];
+          j->xyb[plane + k] = o[1];
+          j->xyb[2 * plane + k] = o[2];
+        }
+      }
+    } else if (j->kind == 1) {
+      int gx = i % gxs, gy = i / gxs;
+      int x0 = gx * 2048, y0 = gy * 2048;
+      int xs = W - x0 < 2048 ? W - x0 : 2048, ys = H - y0 < 2048 ? H - y0 : 2048;
+      int gxb = (xs + 7) / 8, gyb = (ys + 7) / 8;
+      uint8_t *tmp = (uint8_t *)malloc((size_t)gxb * gyb);
+      jxo_fill_quant_field_group(j->xyb, W, H, x0, y0, gxb, gyb, j->p->distance,
+                                 1.0f / j->p->scale, tmp);
+      for (int by = 0; by < gyb; by++)
+        memcpy(j->qf + (size_t)(y0 / 8 + by) * xb + x0 / 8, tmp + (size_t)by * gxb, gxb);
+      free(tmp);
+    } else if (j->kind == 2) {
+      int gx = i % gxs, gy = i / gxs;
+      group_ctx g;
+      group_init(&g, j->xyb, W, H, gx, gy, j->qf_in, j->ytox, j->ytob, j->tables, j->p, j->cells);
+      select_group(&g);
+      adjust_quant_field(&g, j->qf);
+    } else if (j->kind == 3) {
+      int yb = (H + 7) / 8;
+      int by1 = (i + 1) * 8 < yb ? (i + 1) * 8 : yb;
+      jxo_forward_quantize_rows(j->xyb, W, H, j->cells, j->qf, j->ytox, j->ytob, j->tables, j->p,
+                                j->levels, j->coeffs, j->dc, i * 8, by1);
+    }
+  }
+  return NULL;
+}
+
+static void run_pool(fe_job *j, int kind, int total, int threads) {
+  j->kind = kind;
+  j->next = 0;
+  j->total = total;
+  pthread_t tids[256];
+  if (threads > 256) threads = 256;
+  for (int t = 0; t < threads; t++) pthread_create(&tids[t], NULL, fe_worker, j);
+  for (int t = 0; t < threads; t++) pthread_join(tids[t], NULL);
+}
+
+void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width, int height,
+                   const jxg_frame_params *p, const float *tables, const int8_t *ytox,
+                   const int8_t *ytob, int threads, uint8_t *qf, uint8_t *cells,
+                   int32_t *levels[3], float *coeffs[3], int16_t *dc[3]) {
+  jxo_set_fused(p->fused);
+  fe_job j;
+  memset(&j, 0, sizeof(j));
+  pthread_mutex_init(&j.mu, NULL);
+  j.rgb = rgb;
+  j.row_bytes = row_bytes;
+  j.channels = channels;
+  j.width = width;
+  j.height = height;
+  j.p = p;
+  j.tables = tables;
+  j.ytox = ytox;
+  j.ytob = ytob;
+  size_t plane = (size_t)width * height;
+  int xb = (width + 7) / 8, yb = (height + 7) / 8;
+  j.xyb = (float *)malloc(3 * plane * sizeof(float));
+  j.qf = qf;
+  j.cells = cells;
+  if (threads < 1) threads = 1;
+  int groups = ((width + 2047) / 2048) * ((height + 2047) / 2048);
+  run_pool(&j, 0, (height + 7) / 8, threads);
+  run_pool(&j, 1, groups, threads);
+  if (selection_runs(p)) {
+    j.qf_in = (uint8_t *)malloc((size_t)xb * yb);
+    memcpy(j.qf_in, qf, (size_t)xb * yb);
+    /* selection parallelises over DC groups in this driver; the reference
+     * also splits each group into row bands, which is decision-neutral
+     * (src/ac_strategy/selection.rs:943-948). */
+    run_pool(&j, 2, groups, threads);
+    free(j.qf_in);
+  } else {
+    memset(cells, JXG_CELL(JXG_DCT, 1), (size_t)xb * yb);
+  }
+  j.levels = levels;
+  j.coeffs = coeffs;
+  j.dc = dc;
+  run_pool(&j, 3, (yb + 7) / 8, threads);
+  free(j.xyb);
+  pthread_mutex_destroy(&j.mu);
+}

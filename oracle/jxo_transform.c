@@ -1,0 +1,575 @@
+/*
+ * jxo_transform.c - oracle: forward transforms (scalar butterflies and 2-D
+ * assemblies of src/dct.rs), DC extraction from the lowest frequencies, and
+ * the first-pass quantization of write_ac_group (src/group.rs).
+ * TEST INFRASTRUCTURE ONLY (see jxo.h).
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "jxo.h"
+#include "jxo_internal.h"
+
+#define SQRT2F 1.41421356237309504880f
+
+/* Butterfly constants 1/(2 cos(k pi / 2N)) as stored by the reference
+ * (WC4..WC32, src/dct.rs:55-59,658-680,1432-1450). */
+static const float WC4[2] = {0.5411961f, 1.306563f};
+static const float WC8[4] = {0.5097956f, 0.6013449f, 0.8999762f, 2.5629156f};
+static const float WC16[8] = {0.5024193f, 0.5224986f, 0.566944f,  0.6468218f,
+                              0.7881546f, 1.0606777f, 1.7224471f, 5.1011486f};
+static const float WC32[16] = {0.50060298f, 0.50547096f, 0.5154473f, 0.5310426f,
+                               0.5531039f,  0.582935f,   0.6225041f, 0.67480834f,
+                               0.7445363f,  0.83934965f, 0.97256825f, 1.16944f,
+                               1.4841646f,  2.057781f,   3.4076084f, 10.190008f};
+static const float RS16_2[2] = {1.0f, 0.9017642f};                      /* :925 */
+static const float RS32_4[4] = {1.0f, 0.9748868f, 0.9017642f, 0.7870549f}; /* :2018 */
+
+const float *jxo_wc(int n) {
+  switch (n) {
+    case 4: return WC4;
+    case 8: return WC8;
+    case 16: return WC16;
+    case 32: return WC32;
+  }
+  return 0;
+}
+
+/* dct1d_2 / dct1d_4 / dct1d_8 (src/dct.rs:68-127) */
+static void dct1d_2(float *buf) {
+  float a = buf[0], b = buf[1];
+  buf[0] = a + b;
+  buf[1] = a - b;
+}
+
+static void dct1d_4(float *buf) {
+  float tmp[4];
+  tmp[0] = buf[0] + buf[3];
+  tmp[1] = buf[1] + buf[2];
+  dct1d_2(tmp);
+  tmp[2] = buf[0] - buf[3];
+  tmp[3] = buf[1] - buf[2];
+  tmp[2] *= WC4[0];
+  tmp[3] *= WC4[1];
+  dct1d_2(tmp + 2);
+  tmp[2] = jxo_fmla(tmp[2], SQRT2F, tmp[3]);
+  buf[0] = tmp[0];
+  buf[2] = tmp[1];
+  buf[1] = tmp[2];
+  buf[3] = tmp[3];
+}
+
+static void dct1d_8(float *buf) {
+  float tmp[8];
+  for (int i = 0; i < 4; i++) tmp[i] = buf[i] + buf[7 - i];
+  dct1d_4(tmp);
+  for (int i = 0; i < 4; i++) tmp[4 + i] = (buf[i] - buf[7 - i]) * WC8[i];
+  dct1d_4(tmp + 4);
+  tmp[4] = jxo_fmla(tmp[4], SQRT2F, tmp[5]);
+  tmp[5] += tmp[6];
+  tmp[6] += tmp[7];
+  for (int i = 0; i < 4; i++) {
+    buf[2 * i] = tmp[i];
+    buf[2 * i + 1] = tmp[4 + i];
+  }
+}
+
+/* dct1d_16 (src/dct.rs:687-720) */
+static void dct1d_16(float *buf) {
+  float tmp[16];
+  for (int i = 0; i < 8; i++) {
+    tmp[i] = buf[i] + buf[15 - i];
+    tmp[8 + i] = buf[i] - buf[15 - i];
+  }
+  dct1d_8(tmp);
+  for (int i = 0; i < 8; i++) tmp[8 + i] *= WC16[i];
+  dct1d_8(tmp + 8);
+  tmp[8] = jxo_fmla(tmp[8], SQRT2F, tmp[9]);
+  for (int i = 9; i < 15; i++) tmp[i] += tmp[i + 1];
+  for (int i = 0; i < 8; i++) {
+    buf[2 * i] = tmp[i];
+    buf[2 * i + 1] = tmp[8 + i];
+  }
+}
+
+/* dct1d_32 (src/dct.rs:1477-1500) */
+static void dct1d_32(float *buf) {
+  float tmp[32];
+  for (int i = 0; i < 16; i++) {
+    tmp[i] = buf[i] + buf[31 - i];
+    tmp[16 + i] = buf[i] - buf[31 - i];
+  }
+  dct1d_16(tmp);
+  for (int i = 0; i < 16; i++) tmp[16 + i] *= WC32[i];
+  dct1d_16(tmp + 16);
+  tmp[16] = jxo_fmla(tmp[16], SQRT2F, tmp[17]);
+  for (int i = 17; i < 31; i++) tmp[i] += tmp[i + 1];
+  for (int i = 0; i < 16; i++) {
+    buf[2 * i] = tmp[i];
+    buf[2 * i + 1] = tmp[16 + i];
+  }
+}
+
+static void dct1d(float *buf, int n) {
+  switch (n) {
+    case 8: dct1d_8(buf); break;
+    case 16: dct1d_16(buf); break;
+    case 32: dct1d_32(buf); break;
+    default: abort();
+  }
+}
+
+/* The 2-D assemblies (src/dct.rs:631-655 8x8, :823-845 8x16, :904-925 16x8,
+ * :1010-1030 16x16, :1980-2000 32x32, :2251-2270 32x16, :2333-2352 16x32,
+ * :1742-1760 32x8, :1777-1795 8x32).  `in` is a W x H pixel block with
+ * stride W.  Two pass orders occur: rows first (8x8, 8x16, 16x32, 8x32) and
+ * columns first (all others); the output is the long axis as columns. */
+static void dct_rows_first(const float *in, int W, int H, float *out) {
+  /* row DCT (length W) of each row, then column DCT (length H) of each
+   * column u; output[v * W + u] for wide shapes, output[u * 8 + v] for the
+   * square 8x8 (src/dct.rs:648-653). */
+  float tmp[64 * 64];
+  float row[64], col[64];
+  for (int y = 0; y < H; y++) {
+    memcpy(row, in + y * W, W * sizeof(float));
+    dct1d(row, W);
+    memcpy(tmp + y * W, row, W * sizeof(float));
+  }
+  float scale = 1.0f / (float)(W * H);
+  for (int u = 0; u < W; u++) {
+    for (int i = 0; i < H; i++) col[i] = tmp[i * W + u];
+    dct1d(col, H);
+    for (int v = 0; v < H; v++) {
+      if (W > H)
+        out[v * W + u] = col[v] * scale;
+      else
+        out[u * H + v] = col[v] * scale; /* square 8x8: transposed storage */
+    }
+  }
+}
+
+static void dct_cols_first(const float *in, int W, int H, float *out) {
+  /* column DCT (length H) of each column u into tmp[v*W+u], then row DCT
+   * (length W) of each row v; output[u * H + v] when H >= W, else
+   * output[v * W + u]. */
+  float tmp[64 * 64];
+  float col[64];
+  for (int u = 0; u < W; u++) {
+    for (int i = 0; i < H; i++) col[i] = in[i * W + u];
+    dct1d(col, H);
+    for (int v = 0; v < H; v++) tmp[v * W + u] = col[v];
+  }
+  float scale = 1.0f / (float)(W * H);
+  for (int v = 0; v < H; v++) {
+    float *row = tmp + v * W;
+    dct1d(row, W);
+    for (int u = 0; u < W; u++) {
+      if (H >= W)
+        out[u * H + v] = row[u] * scale;
+      else
+        out[v * W + u] = row[u] * scale;
+    }
+  }
+}
+
+/* 8x8 uses an explicit scale constant 1/64 (src/dct.rs:650) which equals
+ * 1/(W*H) in f32; all shapes share one scale convention 1/(W*H). */
+
+/* identity8x8 (src/dct.rs:185-214) */
+static void identity8x8(const float *in, float *out) {
+  for (int qy = 0; qy < 2; qy++) {
+    for (int qx = 0; qx < 2; qx++) {
+      float anchor = in[(qy * 4 + 1) * 8 + qx * 4 + 1];
+      float mean = 0.0f;
+      for (int iy = 0; iy < 4; iy++) {
+        const float *row = in + (qy * 4 + iy) * 8;
+        for (int ix = 0; ix < 4; ix++) {
+          mean += row[qx * 4 + ix];
+          if (ix == 1 && iy == 1) continue;
+          out[(qy + iy * 2) * 8 + qx + ix * 2] = row[qx * 4 + ix] - anchor;
+        }
+      }
+      out[(qy + 2) * 8 + qx + 2] = out[qy * 8 + qx];
+      out[qy * 8 + qx] = mean * (1.0f / 16.0f);
+    }
+  }
+  float b00 = out[0], b01 = out[1], b10 = out[8], b11 = out[9];
+  out[0] = (b00 + b01 + b10 + b11) * 0.25f;
+  out[1] = (b00 + b01 - b10 - b11) * 0.25f;
+  out[8] = (b00 - b01 + b10 - b11) * 0.25f;
+  out[9] = (b00 - b01 - b10 + b11) * 0.25f;
+}
+
+/* dct2_top_block / dct2x2_8x8 (src/dct.rs:270-306) */
+static void dct2_top_block(const float *in, int S, float *out) {
+  float temp[64];
+  memset(temp, 0, sizeof(temp));
+  int half = S / 2;
+  for (int y = 0; y < half; y++) {
+    const float *row0 = in + (y * 2) * 8;
+    const float *row1 = in + (y * 2 + 1) * 8;
+    for (int x = 0; x < half; x++) {
+      float c00 = row0[x * 2], c01 = row0[x * 2 + 1];
+      float c10 = row1[x * 2], c11 = row1[x * 2 + 1];
+      temp[y * 8 + x] = (c00 + c01 + c10 + c11) * 0.25f;
+      temp[y * 8 + half + x] = (c00 + c01 - c10 - c11) * 0.25f;
+      temp[(y + half) * 8 + x] = (c00 - c01 + c10 - c11) * 0.25f;
+      temp[(y + half) * 8 + half + x] = (c00 - c01 - c10 + c11) * 0.25f;
+    }
+  }
+  for (int y = 0; y < S; y++) memcpy(out + y * 8, temp + y * 8, S * sizeof(float));
+}
+
+static void dct2x2_8x8(const float *in, float *out) {
+  float snap[64];
+  dct2_top_block(in, 8, out);
+  memcpy(snap, out, sizeof(snap));
+  dct2_top_block(snap, 4, out);
+  memcpy(snap, out, sizeof(snap));
+  dct2_top_block(snap, 2, out);
+}
+
+/* ---- strategy geometry (src/dc_group_data.rs:140-170) ---- */
+static const uint8_t COV_X[24] = {1, 1, 2, 2, 4, 1, 1, 1, 2, 4, 1, 1,
+                                  1, 1, 8, 4, 8, 1, 1, 16, 8, 16, 1, 4};
+static const uint8_t COV_Y[24] = {1, 2, 1, 2, 4, 1, 1, 1, 4, 2, 1, 1,
+                                  1, 1, 8, 8, 4, 1, 1, 16, 16, 8, 4, 1};
+int jxo_covered_x(int s) { return COV_X[s]; }
+int jxo_covered_y(int s) { return COV_Y[s]; }
+
+void jxo_strategy_pixel_dims(int s, int *w, int *h) {
+  *w = COV_X[s] * 8;
+  *h = COV_Y[s] * 8;
+}
+
+/* Footprint gather with edge replication (gather_pixels,
+ * src/ac_strategy.rs:451-479). */
+static void gather(const float *plane, int pw, int ph, int px, int py, int w, int h,
+                   float *dst) {
+  for (int v = 0; v < h; v++) {
+    int sy = py + v < ph ? py + v : ph - 1;
+    const float *row = plane + (size_t)sy * pw;
+    for (int u = 0; u < w; u++) {
+      int sx = px + u < pw ? px + u : pw - 1;
+      dst[v * w + u] = row[sx];
+    }
+  }
+}
+
+int jxo_forward_transform(int strategy, const float *plane, int width, int height, int px,
+                          int py, float *out) {
+  int w, h;
+  jxo_strategy_pixel_dims(strategy, &w, &h);
+  float pix[64 * 64];
+  gather(plane, width, height, px, py, w, h, pix);
+  switch (strategy) {
+    case JXG_DCT: dct_rows_first(pix, 8, 8, out); break;
+    case JXG_DCT8X16: dct_rows_first(pix, 16, 8, out); break;   /* W16 H8 */
+    case JXG_DCT16X8: dct_cols_first(pix, 8, 16, out); break;   /* W8 H16 */
+    case JXG_DCT16X16: dct_cols_first(pix, 16, 16, out); break;
+    case JXG_DCT32X32: dct_cols_first(pix, 32, 32, out); break;
+    case JXG_DCT32X16: dct_cols_first(pix, 16, 32, out); break; /* W16 H32 */
+    case JXG_DCT16X32: dct_rows_first(pix, 32, 16, out); break; /* W32 H16 */
+    case JXG_DCT32X8: dct_cols_first(pix, 8, 32, out); break;   /* W8 H32 */
+    case JXG_DCT8X32: dct_rows_first(pix, 32, 8, out); break;   /* W32 H8 */
+    case JXG_IDENTITY: identity8x8(pix, out); break;
+    case JXG_DCT2X2: dct2x2_8x8(pix, out); break;
+    default: abort();
+  }
+  return w * h;
+}
+
+/* idct1d_4 (src/dct.rs:2144-2168) */
+static void idct1d_4(const float v[4], float out[4]) {
+  float t[4] = {v[0], v[2], v[1], v[3]};
+  float e0 = t[0] + t[1], e1 = t[0] - t[1];
+  t[0] = e0;
+  t[1] = e1;
+  t[3] += t[2];
+  t[2] *= SQRT2F;
+  float o0 = t[2] + t[3], o1 = t[2] - t[3];
+  t[2] = o0;
+  t[3] = o1;
+  out[0] = jxo_fmla(WC4[0], t[2], t[0]);
+  out[3] = jxo_fmla(-WC4[0], t[2], t[0]);
+  out[1] = jxo_fmla(WC4[1], t[3], t[1]);
+  out[2] = jxo_fmla(-WC4[1], t[3], t[1]);
+}
+
+/* dc_from_coefficients (src/group.rs:183-260) with dc_from_dct16x8 :930,
+ * dc_from_dct16x16 :1033, dc_from_dct32x32 :2170, dc_from_dct32x16 :2356,
+ * dc_from_dct16x32 :2381, dc_from_dct8x32 :1864. */
+void jxo_dc_from_coefficients(int s, const float *c, float *dc) {
+  switch (s) {
+    case JXG_DCT:
+    case JXG_IDENTITY:
+    case JXG_DCT2X2:
+    case JXG_DCT4X4:
+    case JXG_DCT4X8:
+    case JXG_DCT8X4:
+    case JXG_AFV0:
+    case JXG_AFV1:
+    case JXG_AFV2:
+    case JXG_AFV3: dc[0] = c[0]; return;
+    case JXG_DCT16X8:
+    case JXG_DCT8X16: {
+      float s0 = c[0] * RS16_2[0];
+      float s1 = c[1] * RS16_2[1];
+      dc[0] = s0 + s1;
+      dc[1] = s0 - s1;
+      return;
+    }
+    case JXG_DCT16X16: {
+      float s00 = c[0] * (RS16_2[0] * RS16_2[0]);
+      float s01 = c[1] * (RS16_2[0] * RS16_2[1]);
+      float s10 = c[16] * (RS16_2[1] * RS16_2[0]);
+      float s11 = c[17] * (RS16_2[1] * RS16_2[1]);
+      float r00 = s00 + s01, r01 = s00 - s01, r10 = s10 + s11, r11 = s10 - s11;
+      dc[0] = r00 + r10;
+      dc[1] = r00 - r10;
+      dc[2] = r01 + r11;
+      dc[3] = r01 - r11;
+      return;
+    }
+    case JXG_DCT32X32: {
+      float s4[4][4], rr[4][4];
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) s4[a][b] = c[a * 32 + b] * RS32_4[a] * RS32_4[b];
+      for (int a = 0; a < 4; a++) idct1d_4(s4[a], rr[a]);
+      for (int bb = 0; bb < 4; bb++) {
+        float col_in[4] = {rr[0][bb], rr[1][bb], rr[2][bb], rr[3][bb]}, col[4];
+        idct1d_4(col_in, col);
+        for (int r = 0; r < 4; r++) dc[bb * 4 + r] = col[r];
+      }
+      return;
+    }
+    case JXG_DCT32X16: {
+      float rr[2][4];
+      for (int a = 0; a < 2; a++) {
+        float sv[4];
+        for (int b = 0; b < 4; b++) sv[b] = c[a * 32 + b] * RS16_2[a] * RS32_4[b];
+        idct1d_4(sv, rr[a]);
+      }
+      for (int bb = 0; bb < 4; bb++) {
+        dc[bb * 2] = rr[0][bb] + rr[1][bb];
+        dc[bb * 2 + 1] = rr[0][bb] - rr[1][bb];
+      }
+      return;
+    }
+    case JXG_DCT16X32: {
+      float rr[2][4];
+      for (int a = 0; a < 2; a++) {
+        float sv[4];
+        for (int b = 0; b < 4; b++) sv[b] = c[a * 32 + b] * RS16_2[a] * RS32_4[b];
+        idct1d_4(sv, rr[a]);
+      }
+      for (int bb = 0; bb < 4; bb++) {
+        dc[bb] = rr[0][bb] + rr[1][bb];
+        dc[4 + bb] = rr[0][bb] - rr[1][bb];
+      }
+      return;
+    }
+    case JXG_DCT32X8:
+    case JXG_DCT8X32: {
+      float sv[4];
+      for (int b = 0; b < 4; b++) sv[b] = c[b] * RS32_4[b];
+      idct1d_4(sv, dc);
+      return;
+    }
+    default: abort();
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Quantization (src/group.rs:394-470, 560-620, 682-810).              */
+
+static float clampf_(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+static float x_deadzone_window(float distance) {
+  const float a = 0.5f, b = 0.75f, c = 1.5f, d = 2.0f;
+  float up = clampf_((distance - a) / (b - a), 0.0f, 1.0f);
+  float down = clampf_((d - distance) / (d - c), 0.0f, 1.0f);
+  return fminf(up, down);
+}
+
+void jxo_ac_thresholds(int c, int xsize, int ysize, float distance, float qm_mult,
+                       float thr[4]) {
+  float normal[4] = {0.58f, 0.635f, 0.66f, 0.7f};
+  if (c == 0) {
+    float w = x_deadzone_window(distance);
+    normal[0] += w * (0.5f - normal[0]);
+    for (int i = 1; i < 4; i++) normal[i] += (1.0f - w) * 0.08f;
+  }
+  if (c == 2) {
+    float bt = qm_mult > 1.0f ? 0.65f : 0.75f;
+    for (int i = 1; i < 4; i++) normal[i] = bt;
+  }
+  if (xsize > 1 || ysize > 1) {
+    float delta = clampf_(0.003f * (float)xsize * (float)ysize, 0.0f, c > 0 ? 0.08f : 0.12f);
+    for (int i = 0; i < 4; i++) normal[i] -= delta;
+  }
+  float hq[4];
+  if (c == 1) {
+    hq[0] = 0.50f;
+    hq[1] = 0.51f;
+    hq[2] = 0.52f;
+    hq[3] = 0.54f;
+  } else {
+    for (int i = 0; i < 4; i++) hq[i] = normal[i];
+  }
+  float t = clampf_((distance - 0.1f) / 0.9f, 0.0f, 1.0f);
+  for (int i = 0; i < 4; i++) thr[i] = hq[i] + t * (normal[i] - hq[i]);
+}
+
+void jxo_quantize_block_ac(const float *in, int c, const float *qm, int32_t quant, float scale,
+                           float qm_mult, float distance, int xsize, int ysize, int32_t *out) {
+  float thr[4];
+  jxo_ac_thresholds(c, xsize, ysize, distance, qm_mult, thr);
+  float q_scaled = scale * (float)quant * qm_mult;
+  int width = xsize * 8, height = ysize * 8, half = width / 2;
+  for (int y = 0; y < height; y++) {
+    int yfix = y >= height / 2 ? 2 : 0;
+    for (int x = 0; x < width; x++) {
+      float threshold = x >= half ? thr[yfix + 1] : thr[yfix];
+      int i = y * width + x;
+      float q = qm[i] * q_scaled;
+      float val = q * in[i];
+      out[i] = fabsf(val) >= threshold ? jxo_sat_i32(roundf(val)) : 0;
+    }
+  }
+}
+
+/* DequantBias::level (src/group.rs:770-790); numerator / a is evaluated
+ * directly, which is the same correctly rounded quotient the reference
+ * tabulates for a < 64. */
+float jxo_dequant_level(float one, float numerator, int32_t quant) {
+  uint32_t a = quant < 0 ? (uint32_t)(-(int64_t)quant) : (uint32_t)quant;
+  if (a <= 1) return quant == 0 ? 0.0f : (quant > 0 ? one : -one);
+  float q = (float)quant;
+  if (a < 64) {
+    float t = numerator / (float)a;
+    return q - (quant < 0 ? -t : t);
+  }
+  return q - numerator / q;
+}
+
+static const float INV_DC_QUANT[3] = {4096.0f, 512.0f, 256.0f};
+static const float DC_QUANT3[3] = {1.0f / 4096.0f, 1.0f / 512.0f, 1.0f / 256.0f};
+static const float B_QM_MULS[6] = {1.0f, 1.25f, 1.5625f, 1.953125f, 2.4414062f, 3.0517578f};
+
+float jxo_b_qm_mul(uint32_t b_qm_scale) {
+  uint32_t s = b_qm_scale < 2 ? 2 : (b_qm_scale > 7 ? 7 : b_qm_scale);
+  return B_QM_MULS[s - 2];
+}
+
+float jxo_x_qm_mul(uint32_t x_qm_scale) { return powf(1.25f, (float)x_qm_scale - 2.0f); }
+
+void jxo_forward_quantize_rows(const float *xyb, int width, int height, const uint8_t *cells,
+                               const uint8_t *qf, const int8_t *ytox, const int8_t *ytob,
+                               const float *tables, const jxg_frame_params *p, int32_t *levels[3],
+                               float *coeffs[3], int16_t *dc[3], int by0, int by1) {
+  int xb = (width + 7) / 8;
+  int tiles_x = (width + 63) / 64;
+  size_t plane = (size_t)width * height;
+  float inv_factor[3];
+  for (int c = 0; c < 3; c++) inv_factor[c] = INV_DC_QUANT[c] / p->dc_step[c] * p->scale_dc;
+  float inv_cf = 1.0f / p->cfl_color_factor;
+  /* dc_cfl_factor / dc_cfl_factor_x (src/color_correlation.rs:1598-1610) */
+  float cfl_factor_b = INV_DC_QUANT[2] / p->dc_step[2] * (DC_QUANT3[1] * p->dc_step[1]) *
+                       (p->cfl_base_b + (float)p->ytob_dc / p->cfl_color_factor);
+  float cfl_factor_x =
+      INV_DC_QUANT[0] / p->dc_step[0] * (DC_QUANT3[1] * p->dc_step[1]) * p->cfl_base_x;
+  float x_qm_mul = jxo_x_qm_mul(p->x_qm_scale);
+  float b_qm_mul = jxo_b_qm_mul(p->b_qm_scale);
+  static __thread float co[3][64 * 64];
+  static __thread int32_t qz[3][64 * 64];
+  for (int by = by0; by < by1; by++) {
+    for (int bx = 0; bx < xb; bx++) {
+      uint8_t cell = cells[(size_t)by * xb + bx];
+      if (!(cell & 1)) continue;
+      int s = cell >> 1;
+      int cov_x = COV_X[s], cov_y = COV_Y[s];
+      int cx = cov_x, cy = cov_y;
+      if (cov_y > cov_x) {
+        cx = cov_y;
+        cy = cov_x;
+      }
+      int size = cx * cy * 64;
+      int32_t quant = qf[(size_t)by * xb + bx];
+      for (int c = 0; c < 3; c++)
+        jxo_forward_transform(s, xyb + c * plane, width, height, bx * 8, by * 8, co[c]);
+      int kind = jxo_qt_kind(s);
+      uint32_t n = jxo_qt_size(kind), off = jxo_qt_offset(kind);
+      const float *qm_y = tables + off + 1 * n;
+      const float *inv_qm[3] = {tables + off + 3 * n, tables + off + 4 * n, tables + off + 5 * n};
+      float dcv[256];
+      jxo_dc_from_coefficients(s, co[1], dcv);
+      int covered = cov_x * cov_y;
+      int16_t ydcq[256];
+      for (int i = 0; i < covered; i++) ydcq[i] = jxo_sat_i16(roundf(dcv[i] * inv_factor[1]));
+      /* Y source coefficients (the pass-2 RDOQ input) */
+      for (int j = 0; j < cov_y; j++)
+        memcpy(coeffs[1] + ((size_t)(by + j) * xb + bx) * 64, co[1] + j * cov_x * 64,
+               cov_x * 64 * sizeof(float));
+      /* Y quantize / dequantize round trip (quantize_roundtrip_y_block :795) */
+      jxo_quantize_block_ac(co[1], 1, inv_qm[1], quant, p->scale, 1.0f, p->distance, cx, cy,
+                            qz[1]);
+      float inv_qac = 1.0f / (p->scale * (float)quant);
+      for (int i = 0; i < size; i++)
+        co[1][i] = jxo_dequant_level(p->dequant_one[1], p->dequant_numerator, qz[1][i]) *
+                   qm_y[i] * inv_qac;
+      /* zero Y's LLF (:1852-1860) */
+      int wc = cx * 8;
+      for (int r = 0; r < cy; r++)
+        for (int k = 0; k < cx; k++) co[1][r * wc + k] = 0.0f;
+      /* CfL (apply_cfl_scalar, src/ac_strategy.rs:1076) with the tile's
+       * factors (y_to_x_ratio / y_to_b_ratio, src/color_correlation.rs:1117) */
+      int tx = bx / 8, ty = by / 8;
+      float x_factor = jxo_fmla((float)ytox[ty * tiles_x + tx], inv_cf, p->cfl_base_x);
+      float b_factor = jxo_fmla((float)ytob[ty * tiles_x + tx], inv_cf, p->cfl_base_b);
+      float ncx = -x_factor, ncb = -b_factor;
+      for (int i = 0; i < size; i++) {
+        co[0][i] = jxo_fmla(ncx, co[1][i], co[0][i]);
+        co[2][i] = jxo_fmla(ncb, co[1][i], co[2][i]);
+      }
+      float xdc[256], bdc[256];
+      jxo_dc_from_coefficients(s, co[0], xdc);
+      jxo_dc_from_coefficients(s, co[2], bdc);
+      int16_t xdcq[256], bdcq[256];
+      for (int i = 0; i < covered; i++) {
+        if (cfl_factor_x != 0.0f)
+          xdcq[i] = jxo_sat_i16(
+              roundf(jxo_fmla(xdc[i], inv_factor[0], -(float)ydcq[i] * cfl_factor_x)));
+        else
+          xdcq[i] = jxo_sat_i16(roundf(xdc[i] * inv_factor[0]));
+        bdcq[i] =
+            jxo_sat_i16(roundf(jxo_fmla(bdc[i], inv_factor[2], -(float)ydcq[i] * cfl_factor_b)));
+      }
+      jxo_quantize_block_ac(co[0], 0, inv_qm[0], quant, p->scale, x_qm_mul, p->distance, cx, cy,
+                            qz[0]);
+      jxo_quantize_block_ac(co[2], 2, inv_qm[2], quant, p->scale, b_qm_mul, p->distance, cx, cy,
+                            qz[2]);
+      for (int j = 0; j < cov_y; j++) {
+        size_t slot = ((size_t)(by + j) * xb + bx) * 64;
+        for (int c = 0; c < 3; c++)
+          memcpy(levels[c] + slot, qz[c] + j * cov_x * 64, cov_x * 64 * sizeof(int32_t));
+        memcpy(coeffs[0] + slot, co[0] + j * cov_x * 64, cov_x * 64 * sizeof(float));
+        memcpy(coeffs[2] + slot, co[2] + j * cov_x * 64, cov_x * 64 * sizeof(float));
+        for (int i = 0; i < cov_x; i++) {
+          size_t d = (size_t)(by + j) * xb + bx + i;
+          dc[0][d] = xdcq[j * cov_x + i];
+          dc[1][d] = ydcq[j * cov_x + i];
+          dc[2][d] = bdcq[j * cov_x + i];
+        }
+      }
+    }
+  }
+}
+
+void jxo_forward_quantize(const float *xyb, int width, int height, const uint8_t *cells,
+                          const uint8_t *qf, const int8_t *ytox, const int8_t *ytob,
+                          const float *tables, const jxg_frame_params *p, int32_t *levels[3],
+                          float *coeffs[3], int16_t *dc[3]) {
+  jxo_forward_quantize_rows(xyb, width, height, cells, qf, ytox, ytob, tables, p, levels, coeffs,
+                            dc, 0, (height + 7) / 8);
+}

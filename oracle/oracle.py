@@ -1,0 +1,224 @@
+"""ctypes wrapper of the CPU oracle (oracle/_build/libjxo.so).
+
+TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the benchmark's
+CPU-baseline leg use this as the checker.  The product (jixel_amd) never
+imports it.  Build with `make -C oracle`.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libjxo.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i8p = ctypes.POINTER(ctypes.c_int8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i16p = ctypes.POINTER(ctypes.c_int16)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+def _declare(L):
+    c = ctypes
+    L.jxo_set_fused.argtypes = [c.c_int]
+    L.jxo_cbrtf.restype = c.c_float
+    L.jxo_cbrtf.argtypes = [c.c_float]
+    L.jxo_fast_exp2.restype = c.c_float
+    L.jxo_fast_exp2.argtypes = [c.c_float]
+    L.jxo_dirty_log2f.restype = c.c_float
+    L.jxo_dirty_log2f.argtypes = [c.c_float]
+    L.jxo_fast_exp.restype = c.c_float
+    L.jxo_fast_exp.argtypes = [c.c_float]
+    L.jxo_distance_from_quality.restype = c.c_float
+    L.jxo_distance_from_quality.argtypes = [c.c_float]
+    L.jxo_srgb_lut.argtypes = [_f32p]
+    L.jxo_distance_params.argtypes = [c.c_float, _i32p, _f32p, _i32p, _f32p, _u32p, _u32p, _f32p]
+    L.jxo_xyb_pixel.argtypes = [_f32p, c.c_int, c.c_float, c.c_float, c.c_float, _f32p]
+    L.jxo_xyb_image.argtypes = [_u8p, c.c_size_t, c.c_int, c.c_int, c.c_int, _f32p, c.c_int, _f32p]
+    L.jxo_fill_quant_field_image.argtypes = [_f32p, c.c_int, c.c_int, c.c_float, c.c_float, _u8p]
+    L.jxo_forward_transform.restype = c.c_int
+    L.jxo_forward_transform.argtypes = [c.c_int, _f32p, c.c_int, c.c_int, c.c_int, c.c_int, _f32p]
+    L.jxo_dc_from_coefficients.argtypes = [c.c_int, _f32p, _f32p]
+    L.jxo_ac_thresholds.argtypes = [c.c_int, c.c_int, c.c_int, c.c_float, c.c_float, _f32p]
+    L.jxo_dequant_level.restype = c.c_float
+    L.jxo_dequant_level.argtypes = [c.c_float, c.c_float, c.c_int32]
+    L.jxo_forward_quantize.argtypes = [_f32p, c.c_int, c.c_int, _u8p, _u8p, _i8p, _i8p, _f32p,
+                                       c.c_void_p, c.POINTER(_i32p), c.POINTER(_f32p),
+                                       c.POINTER(_i16p)]
+    L.jxo_scan_pos_lut.argtypes = [c.c_int, c.c_int, _u32p]
+    L.jxo_natural_scan.argtypes = [c.c_int, c.c_int, _u32p]
+    L.jxo_strategy_cost.restype = c.c_float
+    L.jxo_strategy_cost.argtypes = [_f32p, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_float,
+                                    c.c_float, _f32p, _f32p, c.c_void_p]
+    L.jxo_select_strategy.argtypes = [_f32p, c.c_int, c.c_int, _u8p, _i8p, _i8p, _f32p, c.c_void_p,
+                                      _u8p]
+    L.jxo_strategy_costs.argtypes = [_f32p, c.c_int, c.c_int, _u8p, _i8p, _i8p, _f32p, c.c_void_p,
+                                     _f32p, _f32p, _f32p]
+    L.jxo_front_end.argtypes = [_u8p, c.c_size_t, c.c_int, c.c_int, c.c_int, c.c_void_p, _f32p,
+                                _i8p, _i8p, c.c_int, _u8p, _u8p, c.POINTER(_i32p),
+                                c.POINTER(_f32p), c.POINTER(_i16p)]
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def set_fused(fused):
+    lib().jxo_set_fused(1 if fused else 0)
+
+
+def srgb_lut():
+    out = np.zeros(256, np.float32)
+    lib().jxo_srgb_lut(_p(out, _f32p))
+    return out
+
+
+def xyb_image(rgb, fwd, order=0):
+    """rgb: (H, W, C) uint8 -> (3, H, W) float32."""
+    rgb = np.ascontiguousarray(rgb)
+    h, w, ch = rgb.shape
+    out = np.zeros((3, h, w), np.float32)
+    m = np.ascontiguousarray(fwd, np.float32)
+    lib().jxo_xyb_image(_p(rgb, _u8p), w * ch, ch, w, h, _p(m, _f32p), order, _p(out, _f32p))
+    return out
+
+
+def fill_quant_field(xyb, distance, inv_scale):
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    qf = np.zeros(((h + 7) // 8, (w + 7) // 8), np.uint8)
+    lib().jxo_fill_quant_field_image(_p(xyb, _f32p), w, h, distance, inv_scale, _p(qf, _u8p))
+    return qf
+
+
+def forward_transform(strategy, plane, px, py):
+    plane = np.ascontiguousarray(plane, np.float32)
+    h, w = plane.shape
+    out = np.zeros(64 * 64, np.float32)
+    n = lib().jxo_forward_transform(strategy, _p(plane, _f32p), w, h, px, py, _p(out, _f32p))
+    return out[:n]
+
+
+def natural_scan(cx, cy):
+    out = np.zeros(cx * cy * 64, np.uint32)
+    lib().jxo_natural_scan(cx, cy, _p(out, _u32p))
+    return out
+
+
+def scan_pos_lut(w, h):
+    out = np.zeros(w * h, np.uint32)
+    lib().jxo_scan_pos_lut(w, h, _p(out, _u32p))
+    return out
+
+
+def _outputs(xb, yb):
+    levels = [np.zeros(yb * xb * 64, np.int32) for _ in range(3)]
+    coeffs = [np.zeros(yb * xb * 64, np.float32) for _ in range(3)]
+    dc = [np.zeros(yb * xb, np.int16) for _ in range(3)]
+    L = (_i32p * 3)(*[_p(a, _i32p) for a in levels])
+    C = (_f32p * 3)(*[_p(a, _f32p) for a in coeffs])
+    D = (_i16p * 3)(*[_p(a, _i16p) for a in dc])
+    return levels, coeffs, dc, L, C, D
+
+
+def forward_quantize(xyb, cells, qf, ytox, ytob, tables, params):
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    xb, yb = (w + 7) // 8, (h + 7) // 8
+    cells = np.ascontiguousarray(cells, np.uint8)
+    qf = np.ascontiguousarray(qf, np.uint8)
+    ytox = np.ascontiguousarray(ytox, np.int8)
+    ytob = np.ascontiguousarray(ytob, np.int8)
+    tables = np.ascontiguousarray(tables, np.float32)
+    levels, coeffs, dc, L, C, D = _outputs(xb, yb)
+    lib().jxo_forward_quantize(_p(xyb, _f32p), w, h, _p(cells, _u8p), _p(qf, _u8p),
+                               _p(ytox, _i8p), _p(ytob, _i8p), _p(tables, _f32p),
+                               ctypes.byref(params), L, C, D)
+    return levels, coeffs, dc
+
+
+def select_strategy(xyb, qf, ytox, ytob, tables, params):
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    qf = np.array(qf, np.uint8, copy=True)
+    cells = np.zeros(qf.shape, np.uint8)
+    lib().jxo_select_strategy(_p(xyb, _f32p), w, h, _p(qf, _u8p),
+                              _p(np.ascontiguousarray(ytox, np.int8), _i8p),
+                              _p(np.ascontiguousarray(ytob, np.int8), _i8p),
+                              _p(np.ascontiguousarray(tables, np.float32), _f32p),
+                              ctypes.byref(params), _p(cells, _u8p))
+    return cells, qf
+
+
+def strategy_costs(xyb, qf, ytox, ytob, tables, params):
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    qf = np.ascontiguousarray(qf, np.uint8)
+    outs = [np.zeros(qf.shape, np.float32) for _ in range(3)]
+    lib().jxo_strategy_costs(_p(xyb, _f32p), w, h, _p(qf, _u8p),
+                             _p(np.ascontiguousarray(ytox, np.int8), _i8p),
+                             _p(np.ascontiguousarray(ytob, np.int8), _i8p),
+                             _p(np.ascontiguousarray(tables, np.float32), _f32p),
+                             ctypes.byref(params), *[_p(o, _f32p) for o in outs])
+    return outs
+
+
+def strategy_cost(xyb, strategy, px, py, qac, cmap, tables, params, meta_r=4.0):
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    cm = np.ascontiguousarray(cmap, np.float32)
+    return lib().jxo_strategy_cost(_p(xyb, _f32p), w, h, strategy, px, py, qac, meta_r,
+                                   _p(cm, _f32p), _p(np.ascontiguousarray(tables, np.float32), _f32p),
+                                   ctypes.byref(params))
+
+
+def front_end(rgb, params, tables, ytox, ytob, threads=1):
+    """Whole front end on the CPU.  rgb: (H, W, C) uint8."""
+    rgb = np.ascontiguousarray(rgb)
+    h, w, ch = rgb.shape
+    xb, yb = (w + 7) // 8, (h + 7) // 8
+    qf = np.zeros((yb, xb), np.uint8)
+    cells = np.zeros((yb, xb), np.uint8)
+    levels, coeffs, dc, L, C, D = _outputs(xb, yb)
+    lib().jxo_front_end(_p(rgb, _u8p), w * ch, ch, w, h, ctypes.byref(params),
+                        _p(np.ascontiguousarray(tables, np.float32), _f32p),
+                        _p(np.ascontiguousarray(ytox, np.int8), _i8p),
+                        _p(np.ascontiguousarray(ytob, np.int8), _i8p), threads,
+                        _p(qf, _u8p), _p(cells, _u8p), L, C, D)
+    return dict(qf=qf, cells=cells, levels=levels, coeffs=coeffs, dc=dc)
+
+
+def distance_params(distance):
+    c = ctypes
+    gs, qd = c.c_int32(), c.c_int32()
+    scale, scale_dc = c.c_float(), c.c_float()
+    xq, it = c.c_uint32(), c.c_uint32()
+    steps = np.zeros(3, np.float32)
+    lib().jxo_distance_params(distance, c.byref(gs), c.byref(scale), c.byref(qd),
+                              c.byref(scale_dc), c.byref(xq), c.byref(it), _p(steps, _f32p))
+    return dict(global_scale=gs.value, scale=scale.value, quant_dc=qd.value,
+                scale_dc=scale_dc.value, x_qm_scale=xq.value, epf_iters=it.value,
+                dc_step=steps)
+
+
+def scalar(name, x):
+    return getattr(lib(), "jxo_" + name)(x)

@@ -1,0 +1,95 @@
+"""Extract the reference's constant tables into a JSON fixture.
+
+Reads the reference's Rust sources as text (nothing is compiled or run) and
+records the numeric tables the tests pin the oracle and the product's host
+tables against:
+
+* EXP2FT (src/adaptive_quant.rs:737): the 64 (value, correction) bit pairs of
+  fast_exp2's table.
+* K_COEFF_ORDER_8X8 / 16X8 / 16X16 / 32X16 / 32X32 (src/ac_context.rs:164-340):
+  the natural coefficient orders.
+* DEQUANT_MATRIX_8X8 / DEQUANT_MATRIX_16X8 (src/quant_weights.rs:246,455):
+  the stored DCT8 and 16x8 step tables.
+* WC8 / WC16 / WC32 (src/dct.rs): butterfly constants.
+
+Usage: python tests/golden/make_reference_tables.py [/root/reference]
+"""
+import json
+import os
+import re
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_tables.json")
+
+NUM = r"-?\d[\d_]*\.?[\d_]*(?:[eE][-+]?\d+)?"
+
+
+def read(rel):
+    with open(os.path.join(REF, rel), encoding="utf-8") as f:
+        return f.read()
+
+
+def block_after(text, marker):
+    """Text of the bracketed initializer that follows `marker`."""
+    i = text.index(marker)
+    i = text.index("=", i)
+    start = text.index("[", i)
+    depth = 0
+    for j in range(start, len(text)):
+        if text[j] == "[":
+            depth += 1
+        elif text[j] == "]":
+            depth -= 1
+            if depth == 0:
+                return text[start : j + 1]
+    raise ValueError(marker)
+
+
+def strip_comments(s):
+    return re.sub(r"//[^\n]*", "", s)
+
+
+def numbers(s):
+    return [float(x.replace("_", "")) for x in re.findall(NUM, strip_comments(s))]
+
+
+def main():
+    out = {}
+    aq = read("src/adaptive_quant.rs")
+    exp2 = block_after(aq, "static EXP2FT")
+    pairs = re.findall(r"\(0x([0-9A-Fa-f]+),\s*0x([0-9A-Fa-f]+)\)", exp2)
+    out["EXP2FT"] = [[int(a, 16), int(b, 16)] for a, b in pairs]
+    assert len(out["EXP2FT"]) == 64
+
+    ctx = read("src/ac_context.rs")
+    for name, n in [("K_COEFF_ORDER_8X8", 64), ("K_COEFF_ORDER_16X8", 128),
+                    ("K_COEFF_ORDER_16X16", 256), ("K_COEFF_ORDER_32X16", 512),
+                    ("K_COEFF_ORDER_32X32", 1024)]:
+        m = re.search(r"static\s+" + name + r"\s*:", ctx)
+        vals = [int(v) for v in numbers(block_after(ctx[m.start():], name))]
+        # the type annotation's length is parsed too; drop it
+        vals = vals[-n:] if len(vals) > n else vals
+        assert len(vals) == n, (name, len(vals))
+        out[name] = vals
+
+    qw = read("src/quant_weights.rs")
+    for name, n in [("DEQUANT_MATRIX_8X8", 64), ("DEQUANT_MATRIX_16X8", 128)]:
+        vals = numbers(block_after(qw, "static " + name))
+        vals = vals[-3 * n:]
+        assert len(vals) == 3 * n, (name, len(vals))
+        out[name] = [vals[c * n:(c + 1) * n] for c in range(3)]
+
+    dct = read("src/dct.rs")
+    for name, n in [("WC8", 4), ("WC16", 8), ("WC32", 16)]:
+        m = re.search(r"const " + name + r": \[f32; \d+\] = ", dct)
+        vals = numbers(block_after(dct[m.start():], name))[-n:]
+        out[name] = vals
+
+    with open(OUT, "w") as f:
+        json.dump(out, f, indent=0)
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()

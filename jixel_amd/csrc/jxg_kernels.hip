@@ -363,7 +363,7 @@ __global__ __launch_bounds__(64) void k_aq_block(const float *__restrict__ xyb, 
 #define META_R 4.0f
 
 struct SelLds {
-  float co[3][32 * SEL_PITCH];  // coefficients, spatial layout
+  float co[3][32 * SEL_PITCH];  // coefficients (spatial), then d^2 terms (storage order)
   float mg[3][1024];            // magnitude-rate terms, storage order
   float sse[16][3], mag[16][3];
   uint32_t nz[16][3], maxscan[16][3];
@@ -373,40 +373,37 @@ struct SelLds {
   float qscale;
 };
 
-__device__ __forceinline__ int sel_scan_size(int n) { return n * n; }
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
-// Natural scan position of raw index k of an n x n block (src/coeff_order.rs
-// natural_scan_generic with cx == cy): the table lives in device memory.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+  return v;
+}
 
 // Candidate evaluation over the whole region for square size n (8, 16, 32):
-// all (32/n)^2 sites at once.  qac[site] is the site's quantizer.
+// all (32/n)^2 sites at once.  qac_in[site] is the site's quantizer.  The
+// region's pixels must already be in L.co (spatial, pitch SEL_PITCH).
 template <bool F>
-__device__ void eval_squares(SelLds &L, int n, const float *__restrict__ xyb, size_t plane,
-                             int stride, int px0, int py0, const float *qac_in, float cmx,
-                             float cmb, const float *__restrict__ tables,
-                             const uint32_t *__restrict__ scan, const jxg_derived *__restrict__ D,
-                             int W, int H, float *cost_out) {
+__device__ void eval_squares(SelLds &L, int n, const float *qac_in, float cmx, float cmb,
+                             const float *__restrict__ tables, const uint32_t *__restrict__ scan,
+                             const jxg_derived *__restrict__ D, int W, int H, int px0, int py0,
+                             float *cost_out) {
   const int lane = threadIdx.x;
   const int sites_1d = 32 / n, nsites = sites_1d * sites_1d, size = n * n;
-  const int strategy = n == 8 ? JXG_DCT : (n == 16 ? JXG_DCT16X16 : JXG_DCT32X32);
+  const int lg = n == 8 ? 3 : (n == 16 ? 4 : 5);  // log2(n)
   const int kind = n == 8 ? JXG_QT_DCT8 : (n == 16 ? JXG_QT_DCT16X16 : JXG_QT_DCT32X32);
-  uint32_t qn = 0, qoff = 0;
-  {
-    const uint32_t sizes[13] = {64, 128, 256, 1024, 512, 256, 4096, 2048, 64, 64, 64, 64, 64};
-    qn = sizes[kind];
-    for (int k = 0; k < kind; k++) qoff += 6 * sizes[k];
+  const uint32_t qoff = D->qt_off[kind], qn = D->qt_n[kind];
+  if (lane < 48) {
+    (&L.nz[0][0])[lane] = 0;
+    (&L.maxscan[0][0])[lane] = 0;
   }
-  // load pixels
-  for (int e = lane; e < 3 * 32 * 32; e += 64) {
-    const int c = e >> 10, r = (e >> 5) & 31, col = e & 31;
-    L.co[c][r * SEL_PITCH + col] = xyb[c * plane + (size_t)(py0 + r) * stride + px0 + col];
-  }
-  for (int e = lane; e < 16 * 3; e += 64) {
-    (&L.nz[0][0])[e] = 0;
-    (&L.maxscan[0][0])[e] = 0;
-  }
-  __syncthreads();
-  // 2-D transform in place: 8x8 rows first, 16x16 / 32x32 columns first.
+  // 2-D transform in place: 8x8 rows first, 16x16 / 32x32 columns first
+  // (src/dct.rs:631-655,1010-1030,1980-2000).
   const float scale = 1.0f / (float)(n * n);
   const int nlines = 3 * nsites * n;
   for (int pass = 0; pass < 2; pass++) {
@@ -414,7 +411,7 @@ __device__ void eval_squares(SelLds &L, int n, const float *__restrict__ xyb, si
     for (int e = lane; e < nlines; e += 64) {
       const int c = e / (nsites * n);
       const int rem = e - c * nsites * n;
-      const int site = rem / n, line = rem - site * n;
+      const int site = rem >> lg, line = rem & (n - 1);
       const int sy = site / sites_1d, sx = site - sy * sites_1d;
       float *base = &L.co[c][(sy * n) * SEL_PITCH + sx * n];
       if (along_rows)
@@ -432,59 +429,50 @@ __device__ void eval_squares(SelLds &L, int n, const float *__restrict__ xyb, si
     L.co[2][a] = fmla<F>(-cmb, yv, L.co[2][a]);
   }
   __syncthreads();
-  // selection_quantizer::scale for the 32-family (src/ac_strategy/selection_quantizer.rs:82-120)
-  if (lane == 0) {
-    float sc = 1.0f;
-    if (n == 32 && D->selection_quantizer) {
-      const float *inv_y = tables + qoff + 4 * qn;
-      const float qac = qac_in[0];
-      double energy = 0.0;
-      for (int k = 0; k < 1024; k++) {
-        const int u = k >> 5, v = k & 31;  // u-major storage of the 32x32 assembly
-        const int y = k >> 5, x = k & 31;  // storage row / column
-        if (x < 8 && y < 8) continue;
-        const float l = L.co[1][v * SEL_PITCH + u];
-        if (fabsf(l * inv_y[k] * qac) < 0.5f) energy += (double)l * (double)l;
-      }
-      const float erased = (float)energy;
-      const float dd = D->distance;
-      float t = erased / (2e-4f * dd * dd) - 1.0f;
-      t = fminf(fmaxf(t, 0.0f), 1.0f);
-      sc = 1.0f + 0.6f * t;
+  // selection_quantizer::scale of the 32-family
+  // (src/ac_strategy/selection_quantizer.rs:82-120): erased luma energy in f64.
+  float qscale = 1.0f;
+  if (n == 32 && D->selection_quantizer) {
+    const float *inv_y = tables + qoff + 4 * qn;
+    const float qac = qac_in[0];
+    double energy = 0.0;
+    for (int j = 0; j < 16; j++) {
+      const int k = lane + 64 * j;
+      const int y = k >> 5, x = k & 31;  // storage row / column (= u, v)
+      if (x < 8 && y < 8) continue;
+      const float l = L.co[1][x * SEL_PITCH + y];
+      if (fabsf(l * inv_y[k] * qac) < 0.5f) energy += (double)l * (double)l;
     }
-    L.qscale = sc;
+    const float erased = (float)wave_sum_f64(energy);
+    const float dd = D->distance;
+    const float t = fminf(fmaxf(erased / (2e-4f * dd * dd) - 1.0f, 0.0f), 1.0f);
+    qscale = 1.0f + 0.6f * t;
   }
-  __syncthreads();
   // Per-coefficient terms (sse_and_rate_scalar_impl::<true, false>,
-  // src/inflated_cost.rs:435-500).  d^2 overwrites the coefficient slot in
-  // storage order; the magnitude term goes to mg.
-  const int cls = n == 8 ? 0 : (n == 16 ? 2 : 4);
+  // src/inflated_cost.rs:435-500).  Element e = lane + 64 j: every j covers
+  // 64 consecutive storage indices of one site.
   const int half = n / 2, cblk = n / 8;
   for (int c = 0; c < 3; c++) {
     const float *inv = tables + qoff + (3 + c) * qn;
-    const float *thr = D->thr[c][cls];
+    const float *thr = D->thr[c][n == 8 ? 0 : (n == 16 ? 2 : 4)];
     const float one = D->dequant_one[c], num = D->dequant_numerator;
+    const float qm = D->qm_mult[c];
     float dd_r[16];
-    int ks[16];
+#pragma unroll
     for (int j = 0; j < 16; j++) {
-      const int e = lane + 64 * j;  // 0..1023: site-major, then storage index
-      const int site = e / size, k = e - site * size;
+      const int e = lane + 64 * j;
+      const int site = (64 * j) >> (2 * lg);
+      const int k = e - site * size;
       const int sy = site / sites_1d, sx = site - sy * sites_1d;
-      int v, u;
-      if (n == 8) {
-        u = k >> 3;
-        v = k & 7;
-      } else {
-        u = k / n;
-        v = k - u * n;
-      }
-      const int row = k / n, col = k - (k / n) * n;  // storage row / column
+      const int u = k >> lg, v = k & (n - 1);  // square storage is u-major
+      const int row = u, col = v;              // storage row / column
       const float coef = L.co[c][(sy * n + v) * SEL_PITCH + sx * n + u];
       float dd = 0.0f, mgv = 0.0f;
+      bool nzf = false;
       if (!(col < cblk && row < cblk)) {
-        const int yfix = row >= n / 2 ? 2 : 0;
+        const int yfix = row >= half ? 2 : 0;
         const float threshold = col >= half ? thr[yfix + 1] : thr[yfix];
-        const float qs = qac_in[site] * L.qscale * D->qm_mult[c];
+        const float qs = qac_in[site] * qscale * qm;
         const float a = inv[k] * qs * coef;
         const float q = fabsf(a) >= threshold ? round_away(a) : 0.0f;
         const float diff = a - level_f32(one, num, q);
@@ -493,27 +481,36 @@ __device__ void eval_squares(SelLds &L, int n, const float *__restrict__ xyb, si
           const float aq = fabsf(q);
           const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
           mgv = ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
-          atomicAdd(&L.nz[site][c], 1u);
-          atomicMax(&L.maxscan[site][c], scan[k]);
+          nzf = true;
         }
       }
       dd_r[j] = dd;
       L.mg[c][e] = mgv;
-      ks[j] = e;
+      const uint32_t cnt = (uint32_t)__popcll(__ballot(nzf));
+      const uint32_t mx = wave_max_u32(nzf ? scan[k] : 0u);
+      if (lane == 0) {
+        L.nz[site][c] += cnt;
+        L.maxscan[site][c] = max(L.maxscan[site][c], mx);
+      }
     }
     __syncthreads();
-    for (int j = 0; j < 16; j++) {
-      const int e = ks[j];
-      (&L.co[c][0])[e] = dd_r[j];  // storage-ordered terms, contiguous
-    }
-    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 16; j++) (&L.co[c][0])[lane + 64 * j] = dd_r[j];
   }
-  // Sequential sums in the reference's row-major storage order.
+  __syncthreads();
+  // Sums in the reference's sequential row-major storage order (f32).
   for (int wk = lane; wk < nsites * 6; wk += 64) {
     const int site = wk / 6, rest = wk - site * 6, c = rest >> 1, which = rest & 1;
-    const float *src = which == 0 ? &L.co[c][0] + site * size : &L.mg[c][0] + site * size;
+    const float4 *src = reinterpret_cast<const float4 *>(
+        which == 0 ? &L.co[c][0] + site * size : &L.mg[c][0] + site * size);
     float acc = 0.0f;
-    for (int k = 0; k < size; k++) acc += src[k];
+    for (int k = 0; k < (size >> 2); k++) {
+      const float4 v4 = src[k];
+      acc += v4.x;
+      acc += v4.y;
+      acc += v4.z;
+      acc += v4.w;
+    }
     if (which == 0)
       L.sse[site][c] = acc;
     else
@@ -528,8 +525,7 @@ __device__ void eval_squares(SelLds &L, int n, const float *__restrict__ xyb, si
       d_total += D->channel_weight[c] * L.sse[site][c];
       const uint32_t nzc = L.nz[site][c];
       const float nzf = (float)nzc;
-      const uint32_t kn = (uint32_t)nzf;
-      const float header = 0.4f * (kn < JXG_RATE_LUT_N ? D->rate_lut[kn] : log2f(1.0f + nzf));
+      const float header = 0.4f * (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf));
       float visited = 0.0f;
       if (nzc != 0) {
         const int64_t span = (int64_t)L.maxscan[site][c] + 1 - (int64_t)(cblk * cblk + nzc);
@@ -542,6 +538,23 @@ __device__ void eval_squares(SelLds &L, int n, const float *__restrict__ xyb, si
     const int vw = min(n, max(W - fx, 0)), vh = min(n, max(H - fy, 0));
     const float visible = (vw == n && vh == n) ? 1.0f : (float)(vw * vh) / (float)(n * n);
     cost_out[site] = fmla<F>(RD_LAMBDA, r_total + META_R, d_total * visible);
+  }
+  __syncthreads();
+}
+
+// Loads the region's three 32x32 pixel planes into L.co.
+__device__ __forceinline__ void load_region(SelLds &L, const float *__restrict__ xyb, size_t plane,
+                                            int stride, int px0, int py0) {
+  const int lane = threadIdx.x;
+  for (int e = lane; e < 3 * 32 * 8; e += 64) {
+    const int c = e >> 8, r = (e >> 3) & 31, q = e & 7;
+    const float4 v = *reinterpret_cast<const float4 *>(xyb + c * plane + (size_t)(py0 + r) * stride +
+                                                       px0 + 4 * q);
+    float *dst = &L.co[c][r * SEL_PITCH + 4 * q];
+    dst[0] = v.x;
+    dst[1] = v.y;
+    dst[2] = v.z;
+    dst[3] = v.w;
   }
   __syncthreads();
 }
@@ -629,26 +642,35 @@ __global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, in
     const size_t ti = (size_t)(gy * 32 + ty) * tiles_x + gx * 32 + tx;
     const float cmx = fmla<F>((float)ytox[ti], D->cfl_inv_color_factor, D->cfl_base_x);
     const float cmb = fmla<F>((float)ytob[ti], D->cfl_inv_color_factor, D->cfl_base_b);
-    // site luma statistics for MergeTuning::site_bias (f64, raster order,
-    // clipped to the image): sites 0..3 = 16x16, 4 = 32x32.
-    if (lane < 5) {
-      const int n = lane < 4 ? 16 : 32;
-      const int x0 = px0 + (lane < 4 ? (lane & 1) * 16 : 0);
-      const int y0 = py0 + (lane < 4 ? (lane >> 1) * 16 : 0);
-      const float *yp = xyb + plane;
-      double s1 = 0.0, s2 = 0.0;
-      uint32_t cnt = 0;
-      const int ye = min(y0 + n, H), xe = min(x0 + n, W);
-      for (int y = y0; y < ye; y++)
-        for (int x = x0; x < xe; x++) {
-          const double v = (double)yp[(size_t)y * stride + x];
-          s1 += v;
-          s2 += v * v;
-          cnt++;
+    // Site luma statistics for MergeTuning::site_bias (site_variance_rect,
+    // src/ac_strategy.rs:224-245): f64 sums of the in-image pixels of the
+    // four 16x16 sites and the 32x32 region, reduced across the wavefront.
+    load_region(L, xyb, plane, stride, px0, py0);
+    {
+      double s1[5] = {0, 0, 0, 0, 0}, s2[5] = {0, 0, 0, 0, 0};
+      uint32_t cnt[5] = {0, 0, 0, 0, 0};
+      for (int j = 0; j < 16; j++) {
+        const int e = lane + 64 * j, r = e >> 5, col = e & 31;
+        if (py0 + r >= H || px0 + col >= W) continue;
+        const double v = (double)L.co[1][r * SEL_PITCH + col];
+        const int q = (r >> 4) * 2 + (col >> 4);
+        s1[q] += v;
+        s2[q] += v * v;
+        cnt[q]++;
+        s1[4] += v;
+        s2[4] += v * v;
+        cnt[4]++;
+      }
+      for (int q = 0; q < 5; q++) {
+        const double a1 = wave_sum_f64(s1[q]), a2 = wave_sum_f64(s2[q]);
+        uint32_t cc = cnt[q];
+        for (int off = 32; off > 0; off >>= 1) cc += (uint32_t)__shfl_xor((int)cc, off, 64);
+        if (lane == 0) {
+          L.site_s1[q] = a1;
+          L.site_s2[q] = a2;
+          L.site_cnt[q] = cc;
         }
-      L.site_s1[lane] = s1;
-      L.site_s2[lane] = s2;
-      L.site_cnt[lane] = cnt;
+      }
     }
     __syncthreads();
     float qac8[16], qac16[4], qac32[1];
@@ -674,13 +696,13 @@ __global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, in
       }
       qac32[0] = scale * (float)aggregate_quant(max_q, sum, 16, dist);
     }
-    eval_squares<F>(L, 8, xyb, plane, stride, px0, py0, qac8, cmx, cmb, tables, scans, D, W, H,
-                    L.cost8);
-    eval_squares<F>(L, 16, xyb, plane, stride, px0, py0, qac16, cmx, cmb, tables, scans + 64, D,
-                    W, H, L.cost16);
-    if (four_row && four_col)
-      eval_squares<F>(L, 32, xyb, plane, stride, px0, py0, qac32, cmx, cmb, tables, scans + 320,
-                      D, W, H, &L.cost32);
+    eval_squares<F>(L, 8, qac8, cmx, cmb, tables, scans, D, W, H, px0, py0, L.cost8);
+    load_region(L, xyb, plane, stride, px0, py0);
+    eval_squares<F>(L, 16, qac16, cmx, cmb, tables, scans + 64, D, W, H, px0, py0, L.cost16);
+    if (four_row && four_col) {
+      load_region(L, xyb, plane, stride, px0, py0);
+      eval_squares<F>(L, 32, qac32, cmx, cmb, tables, scans + 320, D, W, H, px0, py0, &L.cost32);
+    }
     if (lane == 0) {
       // select_super_block at square scope for each super block, then the
       // 32x32 merge (src/ac_strategy/selection.rs:238-455, 1040-1150).
@@ -809,7 +831,7 @@ struct QLds {
   int lx[QT_MAXT], ly[QT_MAXT];  // LDS origin (pixels)
   int pre1[QT_MAXT + 1];         // prefix of 3 * pass-1 lines
   int pre2[QT_MAXT + 1];         // prefix of 3 * pass-2 lines
-  int prec[QT_MAXT + 1];         // prefix of coefficient counts
+  int prec[QT_MAXT + 1];         // prefix of 64-coefficient chunks
   int16_t ydc[QT_MAXT][16];
   float xf[QT_MAXT], bf[QT_MAXT];
   int quant[QT_MAXT];
@@ -925,15 +947,47 @@ struct QArgs {
   uint32_t *bad;  // set to 1 when an unsupported strategy cell is seen
 };
 
-__device__ __forceinline__ size_t out_index(int bx, int by, int cvx, int xb, int k) {
-  const int chunk = cvx * 64;
-  return ((size_t)(by + k / chunk) * xb + bx) * 64 + (k % chunk);
+__device__ __forceinline__ int ctz32(int v) { return __builtin_ctz((unsigned)v); }
+
+// Geometry of one coefficient of transform t: LDS address, storage row /
+// column and output index (all sizes are powers of two).
+struct CoefGeom {
+  int a, row, col, hc, wc;
+  size_t out;
+};
+
+__device__ __forceinline__ CoefGeom coef_geom(const QLds &L, int t, int k, int xb) {
+  const int s = L.s[t];
+  const int cvx = cov_x(s), cvy = cov_y(s);
+  const int lw = 3 + ctz32(cvx), lh = 3 + ctz32(cvy);
+  int v, u;
+  if (s == JXG_DCT) {
+    u = k >> 3;
+    v = k & 7;
+  } else if (rows_first(s)) {
+    v = k >> lw;
+    u = k & ((1 << lw) - 1);
+  } else {
+    u = k >> lh;
+    v = k & ((1 << lh) - 1);
+  }
+  const int lwc = lw > lh ? lw : lh;
+  CoefGeom g;
+  g.a = (L.ly[t] + v) * QT_PITCH + L.lx[t] + u;
+  g.row = k >> lwc;
+  g.col = k & ((1 << lwc) - 1);
+  g.wc = 1 << lwc;
+  g.hc = 1 << (lw > lh ? lh : lw);
+  const int lch = 6 + ctz32(cvx);
+  g.out = ((size_t)(L.by[t] + (k >> lch)) * xb + L.bx[t]) * 64 + (k & ((1 << lch) - 1));
+  return g;
 }
 
 // Transform + quantize the L.n transforms whose footprints are in L.co.
 template <bool F>
 __device__ void quantize_list(QLds &L, const QArgs &A) {
   const int tid = threadIdx.x, nthr = blockDim.x;
+  const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
   const jxg_derived *D = A.D;
   const int n = L.n;
   if (n == 0) return;
@@ -947,7 +1001,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A) {
       L.prec[t] = pc;
       p1 += 3 * (rows_first(s) ? h : w);
       p2 += 3 * (rows_first(s) ? w : h);
-      pc += w * h;
+      pc += cov_x(s) * cov_y(s);
     }
     L.pre1[n] = p1;
     L.pre2[n] = p2;
@@ -968,9 +1022,9 @@ __device__ void quantize_list(QLds &L, const QArgs &A) {
     const int s = L.s[t];
     const int w = cov_x(s) * 8, h = cov_y(s) * 8;
     const bool rf = rows_first(s);
-    const int per_c = rf ? h : w;
+    const int lper = 3 + ctz32(rf ? cov_y(s) : cov_x(s));
     const int local = e - L.pre1[t];
-    const int c = local / per_c, li = local - c * per_c;
+    const int c = local >> lper, li = local & ((1 << lper) - 1);
     float *base = &L.co[c][L.ly[t] * QT_PITCH + L.lx[t]];
     if (rf)
       dct_line_n<F>(w, base + li * QT_PITCH, 1, 0.0f, false);
@@ -984,9 +1038,9 @@ __device__ void quantize_list(QLds &L, const QArgs &A) {
     const int s = L.s[t];
     const int w = cov_x(s) * 8, h = cov_y(s) * 8;
     const bool rf = rows_first(s);
-    const int per_c = rf ? w : h;
+    const int lper = 3 + ctz32(rf ? cov_x(s) : cov_y(s));
     const int local = e - L.pre2[t];
-    const int c = local / per_c, li = local - c * per_c;
+    const int c = local >> lper, li = local & ((1 << lper) - 1);
     const float scale = 1.0f / (float)(w * h);
     float *base = &L.co[c][L.ly[t] * QT_PITCH + L.lx[t]];
     if (rf)
@@ -1002,47 +1056,44 @@ __device__ void quantize_list(QLds &L, const QArgs &A) {
     float dcv[16];
     dc_from<F>(s, &L.co[1][L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, w, h, dcv);
     const int cvx = cov_x(s), covered = cvx * cov_y(s);
-    for (int i = 0; i < covered; i++) {
-      const int16_t q = sat_i16(round_away(dcv[i] * D->inv_factor[1]));
-      L.ydc[t][i] = q;
-      A.dc1[(size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx] = q;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      if (i < covered) {
+        const int16_t q = sat_i16(round_away(dcv[i] * D->inv_factor[1]));
+        L.ydc[t][i] = q;
+        A.dc1[(size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx] = q;
+      }
     }
   }
   __syncthreads();
   // Y quantize / dequantize round trip (quantize_roundtrip_y_block,
   // src/group.rs:795), Y LLF zeroing (:1852) and CfL on X and B (:1863).
-  for (int e = tid; e < L.prec[n]; e += nthr) {
-    const int t = find_item(L.prec, n, e);
+  // Each wavefront takes 64 consecutive coefficients of one transform.
+  for (int ch = wave; ch < L.prec[n]; ch += nwaves) {
+    const int t = find_item(L.prec, n, ch);
     const int s = L.s[t];
-    const int k = e - L.prec[t];
-    const int cvx = cov_x(s), cvy = cov_y(s);
-    const int w = cvx * 8, h = cvy * 8;
-    const int cx = max(cvx, cvy), cy = min(cvx, cvy);
-    const int wc = cx * 8, hc = cy * 8;
-    int v, u;
-    storage_to_spatial(s, w, h, k, &v, &u);
-    const int a = (L.ly[t] + v) * QT_PITCH + L.lx[t] + u;
-    const int row = k / wc, col = k - row * wc;
-    const size_t out = out_index(L.bx[t], L.by[t], cvx, A.xb, k);
+    const int k = ((ch - L.prec[t]) << 6) + lane;
+    const CoefGeom g = coef_geom(L, t, k, A.xb);
+    const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
     const int kind = jxg_qt_kind_dev(s);
     const float *tb = A.tables + D->qt_off[kind];
     const int qn = D->qt_n[kind];
-    const float yv = L.co[1][a];
-    A.co1[out] = yv;
+    const float yv = L.co[1][g.a];
+    A.co1[g.out] = yv;
     const float *thr = D->thr[1][jxg_area_class(cx, cy)];
-    const int yfix = row >= hc / 2 ? 2 : 0;
-    const float threshold = col >= wc / 2 ? thr[yfix + 1] : thr[yfix];
+    const int yfix = g.row >= g.hc / 2 ? 2 : 0;
+    const float threshold = g.col >= g.wc / 2 ? thr[yfix + 1] : thr[yfix];
     const int32_t quant = L.quant[t];
     const float q_scaled = D->scale * (float)quant * 1.0f;
     const float val = (tb[4 * qn + k] * q_scaled) * yv;
     const int32_t q = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
-    A.lv1[out] = q;
+    A.lv1[g.out] = q;
     const float inv_qac = 1.0f / (D->scale * (float)quant);
     float y2 =
         dequant_level(D->dequant_one[1], D->dequant_numerator, q) * tb[1 * qn + k] * inv_qac;
-    if (row < cy && col < cx) y2 = 0.0f;
-    L.co[0][a] = fmla<F>(-L.xf[t], y2, L.co[0][a]);
-    L.co[2][a] = fmla<F>(-L.bf[t], y2, L.co[2][a]);
+    if (g.row < cy && g.col < cx) y2 = 0.0f;
+    L.co[0][g.a] = fmla<F>(-L.xf[t], y2, L.co[0][g.a]);
+    L.co[2][g.a] = fmla<F>(-L.bf[t], y2, L.co[2][g.a]);
   }
   __syncthreads();
   // X and B DC after CfL (src/group.rs:1871-2003, 2128-2150)
@@ -1053,56 +1104,52 @@ __device__ void quantize_list(QLds &L, const QArgs &A) {
     dc_from<F>(s, &L.co[0][L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, w, h, xd);
     dc_from<F>(s, &L.co[2][L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, w, h, bd);
     const int cvx = cov_x(s), covered = cvx * cov_y(s);
-    for (int i = 0; i < covered; i++) {
-      const float yq = (float)L.ydc[t][i];
-      int16_t xq;
-      if (D->cfl_factor_x != 0.0f)
-        xq = sat_i16(round_away(fmla<F>(xd[i], D->inv_factor[0], -yq * D->cfl_factor_x)));
-      else
-        xq = sat_i16(round_away(xd[i] * D->inv_factor[0]));
-      const int16_t bq =
-          sat_i16(round_away(fmla<F>(bd[i], D->inv_factor[2], -yq * D->cfl_factor_b)));
-      const size_t o = (size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx;
-      A.dc0[o] = xq;
-      A.dc2[o] = bq;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      if (i < covered) {
+        const float yq = (float)L.ydc[t][i];
+        int16_t xq;
+        if (D->cfl_factor_x != 0.0f)
+          xq = sat_i16(round_away(fmla<F>(xd[i], D->inv_factor[0], -yq * D->cfl_factor_x)));
+        else
+          xq = sat_i16(round_away(xd[i] * D->inv_factor[0]));
+        const int16_t bq =
+            sat_i16(round_away(fmla<F>(bd[i], D->inv_factor[2], -yq * D->cfl_factor_b)));
+        const size_t o = (size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx;
+        A.dc0[o] = xq;
+        A.dc2[o] = bq;
+      }
     }
   }
   // X and B AC quantization (quantize_block_ac_scalar, src/group.rs:560-620)
-  for (int e = tid; e < L.prec[n]; e += nthr) {
-    const int t = find_item(L.prec, n, e);
+  for (int ch = wave; ch < L.prec[n]; ch += nwaves) {
+    const int t = find_item(L.prec, n, ch);
     const int s = L.s[t];
-    const int k = e - L.prec[t];
-    const int cvx = cov_x(s), cvy = cov_y(s);
-    const int w = cvx * 8, h = cvy * 8;
-    const int cx = max(cvx, cvy), cy = min(cvx, cvy);
-    const int wc = cx * 8, hc = cy * 8;
-    int v, u;
-    storage_to_spatial(s, w, h, k, &v, &u);
-    const int a = (L.ly[t] + v) * QT_PITCH + L.lx[t] + u;
-    const int row = k / wc, col = k - row * wc;
-    const size_t out = out_index(L.bx[t], L.by[t], cvx, A.xb, k);
+    const int k = ((ch - L.prec[t]) << 6) + lane;
+    const CoefGeom g = coef_geom(L, t, k, A.xb);
+    const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
     const int kind = jxg_qt_kind_dev(s);
     const float *tb = A.tables + D->qt_off[kind];
     const int qn = D->qt_n[kind];
     const int cls = jxg_area_class(cx, cy);
-    const int yfix = row >= hc / 2 ? 2 : 0;
-    const bool hi = col >= wc / 2;
+    const int yfix = g.row >= g.hc / 2 ? 2 : 0;
+    const bool hi = g.col >= g.wc / 2;
     const int32_t quant = L.quant[t];
-    const float xv = L.co[0][a];
-    A.co0[out] = xv;
+    const float xv = L.co[0][g.a];
+    A.co0[g.out] = xv;
     {
       const float threshold = hi ? D->thr[0][cls][yfix + 1] : D->thr[0][cls][yfix];
       const float q_scaled = D->scale * (float)quant * D->qm_mult[0];
       const float val = (tb[3 * qn + k] * q_scaled) * xv;
-      A.lv0[out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+      A.lv0[g.out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
     }
-    const float bv = L.co[2][a];
-    A.co2[out] = bv;
+    const float bv = L.co[2][g.a];
+    A.co2[g.out] = bv;
     {
       const float threshold = hi ? D->thr[2][cls][yfix + 1] : D->thr[2][cls][yfix];
       const float q_scaled = D->scale * (float)quant * D->qm_mult[2];
       const float val = (tb[5 * qn + k] * q_scaled) * bv;
-      A.lv2[out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+      A.lv2[g.out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
     }
   }
 }
@@ -1384,19 +1431,19 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   else (void)hipDeviceSynchronize();
-  hipFree(ctx->d_tables);
-  hipFree(ctx->d_derived);
-  hipHostFree(ctx->h_derived);
-  hipFree(ctx->d_scans);
-  hipFree(ctx->d_mlut);
-  hipHostFree(ctx->h_mlut);
-  hipFree(ctx->d_pre);
-  hipFree(ctx->d_xyb);
-  hipFree(ctx->d_misc);
-  hipFree(ctx->d_exp2);
-  hipFree(ctx->d_xt);
-  hipFree(ctx->d_zero_cmap);
-  hipFree(ctx->d_rgb);
+  (void)hipFree(ctx->d_tables);
+  (void)hipFree(ctx->d_derived);
+  (void)hipHostFree(ctx->h_derived);
+  (void)hipFree(ctx->d_scans);
+  (void)hipFree(ctx->d_mlut);
+  (void)hipHostFree(ctx->h_mlut);
+  (void)hipFree(ctx->d_pre);
+  (void)hipFree(ctx->d_xyb);
+  (void)hipFree(ctx->d_misc);
+  (void)hipFree(ctx->d_exp2);
+  (void)hipFree(ctx->d_xt);
+  (void)hipFree(ctx->d_zero_cmap);
+  (void)hipFree(ctx->d_rgb);
   delete ctx;
 }
 
@@ -1418,7 +1465,7 @@ int jxg_xyb_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channe
   if (!ctx || !d_rgb || !fwd || !lut || !d_xyb) return set_err(JXG_ERR_INVALID_ARG, "null argument");
   int rc = check_image(width, height);
   if (rc) return rc;
-  if (rc = check_planes(width, height, stride, padded_h)) return rc;
+  if ((rc = check_planes(width, height, stride, padded_h))) return rc;
   if (channels != 3 && channels != 4) return set_err(JXG_ERR_INVALID_ARG, "channels %d", channels);
   if (row_bytes < (size_t)width * channels) return set_err(JXG_ERR_INVALID_ARG, "row_bytes too small");
   if (order != JXG_XYB_SCALAR && order != JXG_XYB_AVX2) return set_err(JXG_ERR_INVALID_ARG, "order %d", order);
@@ -1467,10 +1514,10 @@ static int fill_qf(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t p
 int jxg_fill_quant_field_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
                              uint32_t width, uint32_t height, float distance, float inv_scale,
                              int fused, uint8_t *d_qf) {
+  int rc = check_image(width, height);  // EmptyImage before argument checks, as the reference
+  if (rc) return rc;                    // validates (src/encode_image.rs:1785-1802)
   if (!ctx || !d_xyb || !d_qf) return set_err(JXG_ERR_INVALID_ARG, "null argument");
-  int rc = check_image(width, height);
-  if (rc) return rc;
-  if (rc = check_planes(width, height, stride, padded_h)) return rc;
+  if ((rc = check_planes(width, height, stride, padded_h))) return rc;
   if (!(distance > 0.0f) || !(inv_scale > 0.0f))
     return set_err(JXG_ERR_INVALID_DISTANCE, "distance %g inv_scale %g", (double)distance, (double)inv_scale);
   return fill_qf(ctx, d_xyb, stride, padded_h, width, height, distance, inv_scale, fused, d_qf);

@@ -359,23 +359,35 @@ __global__ __launch_bounds__(64) void k_aq_block(const float *__restrict__ xyb, 
 // Candidate costs and square-merge selection
 
 #define SEL_PITCH 33  // LDS row pitch of the 32x32 region (bank-conflict free)
+#define SEL_T 256     // threads per region
 #define RD_LAMBDA 0.08086717f
 #define META_R 4.0f
 
 struct SelLds {
   float co[3][32 * SEL_PITCH];  // coefficients (spatial), then d^2 terms (storage order)
   float mg[3][1024];            // magnitude-rate terms, storage order
+  float quot[3][64];            // per channel: 0, 1 - one, numerator / a
+  float rate[256];              // log2(1 + i)
   float sse[16][3], mag[16][3];
   uint32_t nz[16][3], maxscan[16][3];
   float cost8[16], cost16[4], cost32;
+  double red1[4][5], red2[4][5];
+  uint32_t redc[4][5];
   double site_s1[5], site_s2[5];
   uint32_t site_cnt[5];
-  float qscale;
+  double erased[4];
+  float qs[3][16];  // per channel, per site: quantizer * selection scale * qm multiplier
 };
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
   return v;
 }
 
@@ -385,50 +397,71 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   return v;
 }
 
+// DequantBias::level_f32 (src/group.rs:748-768) with the quotient table the
+// reference keeps for integral levels below 64.
+__device__ __forceinline__ float level_f32_tab(float one, float num, float q, const float *quot) {
+  const float aq = fabsf(q);
+  if (aq < 1.125f) return q == 0.0f ? 0.0f : copysignf(one, q);
+  const uint32_t a = (uint32_t)aq;
+  if (a < 64 && (float)a == aq) {
+    const float t = quot[a];
+    return q - (q < 0.0f ? -t : t);
+  }
+  return q - num / q;
+}
+
+__device__ __noinline__ float rate_log2_slow(const jxg_derived *D, float aq) {
+  const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
+  return ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
+}
+
 // Candidate evaluation over the whole region for square size n (8, 16, 32):
 // all (32/n)^2 sites at once.  qac_in[site] is the site's quantizer.  The
 // region's pixels must already be in L.co (spatial, pitch SEL_PITCH).
-template <bool F>
-__device__ void eval_squares(SelLds &L, int n, const float *qac_in, float cmx, float cmb,
+template <bool F, int LG>
+__device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cmb,
                              const float *__restrict__ tables, const uint32_t *__restrict__ scan,
                              const jxg_derived *__restrict__ D, int W, int H, int px0, int py0,
                              float *cost_out) {
-  const int lane = threadIdx.x;
-  const int sites_1d = 32 / n, nsites = sites_1d * sites_1d, size = n * n;
-  const int lg = n == 8 ? 3 : (n == 16 ? 4 : 5);  // log2(n)
-  const int kind = n == 8 ? JXG_QT_DCT8 : (n == 16 ? JXG_QT_DCT16X16 : JXG_QT_DCT32X32);
+  constexpr int n = 1 << LG;
+  constexpr int sites_1d = 32 / n, nsites = sites_1d * sites_1d, size = n * n;
+  constexpr int kind = n == 8 ? JXG_QT_DCT8 : (n == 16 ? JXG_QT_DCT16X16 : JXG_QT_DCT32X32);
+  constexpr int cls = n == 8 ? 0 : (n == 16 ? 2 : 4);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t qoff = D->qt_off[kind], qn = D->qt_n[kind];
-  if (lane < 48) {
-    (&L.nz[0][0])[lane] = 0;
-    (&L.maxscan[0][0])[lane] = 0;
+  if (tid < 48) {
+    (&L.nz[0][0])[tid] = 0;
+    (&L.maxscan[0][0])[tid] = 0;
   }
   // 2-D transform in place: 8x8 rows first, 16x16 / 32x32 columns first
   // (src/dct.rs:631-655,1010-1030,1980-2000).
   const float scale = 1.0f / (float)(n * n);
-  const int nlines = 3 * nsites * n;
+  constexpr int nlines = 3 * nsites * n;
+#pragma unroll
   for (int pass = 0; pass < 2; pass++) {
     const bool along_rows = (n == 8) ? (pass == 0) : (pass == 1);
-    for (int e = lane; e < nlines; e += 64) {
+    for (int e = tid; e < nlines; e += SEL_T) {
       const int c = e / (nsites * n);
       const int rem = e - c * nsites * n;
-      const int site = rem >> lg, line = rem & (n - 1);
+      const int site = rem >> LG, line = rem & (n - 1);
       const int sy = site / sites_1d, sx = site - sy * sites_1d;
       float *base = &L.co[c][(sy * n) * SEL_PITCH + sx * n];
       if (along_rows)
-        dct_line_n<F>(n, base + line * SEL_PITCH, 1, scale, pass == 1);
+        dct_line<F, n>(base + line * SEL_PITCH, 1, scale, pass == 1);
       else
-        dct_line_n<F>(n, base + line, SEL_PITCH, scale, pass == 1);
+        dct_line<F, n>(base + line, SEL_PITCH, scale, pass == 1);
     }
     __syncthreads();
   }
   // CfL against the unquantized luma (apply_cfl_scalar, src/ac_strategy.rs:1076)
-  for (int e = lane; e < 32 * 32; e += 64) {
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int e = tid + SEL_T * j;
     const int a = (e >> 5) * SEL_PITCH + (e & 31);
     const float yv = L.co[1][a];
     L.co[0][a] = fmla<F>(-cmx, yv, L.co[0][a]);
     L.co[2][a] = fmla<F>(-cmb, yv, L.co[2][a]);
   }
-  __syncthreads();
   // selection_quantizer::scale of the 32-family
   // (src/ac_strategy/selection_quantizer.rs:82-120): erased luma energy in f64.
   float qscale = 1.0f;
@@ -436,70 +469,82 @@ __device__ void eval_squares(SelLds &L, int n, const float *qac_in, float cmx, f
     const float *inv_y = tables + qoff + 4 * qn;
     const float qac = qac_in[0];
     double energy = 0.0;
-    for (int j = 0; j < 16; j++) {
-      const int k = lane + 64 * j;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int k = tid + SEL_T * j;
       const int y = k >> 5, x = k & 31;  // storage row / column (= u, v)
       if (x < 8 && y < 8) continue;
       const float l = L.co[1][x * SEL_PITCH + y];
       if (fabsf(l * inv_y[k] * qac) < 0.5f) energy += (double)l * (double)l;
     }
-    const float erased = (float)wave_sum_f64(energy);
+    energy = wave_sum_f64(energy);
+    if (lane == 0) L.erased[wave] = energy;
+    __syncthreads();
+    const float erased = (float)(((L.erased[0] + L.erased[1]) + L.erased[2]) + L.erased[3]);
     const float dd = D->distance;
     const float t = fminf(fmaxf(erased / (2e-4f * dd * dd) - 1.0f, 0.0f), 1.0f);
     qscale = 1.0f + 0.6f * t;
   }
+  __syncthreads();
   // Per-coefficient terms (sse_and_rate_scalar_impl::<true, false>,
-  // src/inflated_cost.rs:435-500).  Element e = lane + 64 j: every j covers
-  // 64 consecutive storage indices of one site.
-  const int half = n / 2, cblk = n / 8;
+  // src/inflated_cost.rs:435-500).  Element e = tid + 256 j lies in site
+  // e / size (one site per wavefront and j); within the site it is storage
+  // index k.
+  constexpr int half = n / 2, cblk = n / 8;
+  if (tid < 3 * nsites) {
+    const int c = tid / nsites, site = tid - c * nsites;
+    L.qs[c][site] = qac_in[site] * qscale * D->qm_mult[c];
+  }
+  __syncthreads();
+#pragma unroll
   for (int c = 0; c < 3; c++) {
     const float *inv = tables + qoff + (3 + c) * qn;
-    const float *thr = D->thr[c][n == 8 ? 0 : (n == 16 ? 2 : 4)];
+    const float *thr = D->thr[c][cls];
     const float one = D->dequant_one[c], num = D->dequant_numerator;
-    const float qm = D->qm_mult[c];
-    float dd_r[16];
+    float dd_r[4];
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const int e = lane + 64 * j;
-      const int site = (64 * j) >> (2 * lg);
-      const int k = e - site * size;
+    for (int j = 0; j < 4; j++) {
+      const int e = tid + SEL_T * j;
+      const int site = __builtin_amdgcn_readfirstlane(e >> (2 * LG));
+      const int k = e & (size - 1);
       const int sy = site / sites_1d, sx = site - sy * sites_1d;
-      const int u = k >> lg, v = k & (n - 1);  // square storage is u-major
+      const int u = k >> LG, v = k & (n - 1);  // square storage is u-major
       const int row = u, col = v;              // storage row / column
       const float coef = L.co[c][(sy * n + v) * SEL_PITCH + sx * n + u];
       float dd = 0.0f, mgv = 0.0f;
-      bool nzf = false;
+      uint32_t spv = 0;
       if (!(col < cblk && row < cblk)) {
         const int yfix = row >= half ? 2 : 0;
         const float threshold = col >= half ? thr[yfix + 1] : thr[yfix];
-        const float qs = qac_in[site] * qscale * qm;
-        const float a = inv[k] * qs * coef;
+        const float a = inv[k] * L.qs[c][site] * coef;
         const float q = fabsf(a) >= threshold ? round_away(a) : 0.0f;
-        const float diff = a - level_f32(one, num, q);
+        const float aq = fabsf(q);
+        // level_f32 with the extended quotient table (quot[0] = 0,
+        // quot[1] = 1 - one): bit-identical to src/group.rs:748-768.
+        const float lev = aq < 64.0f ? q - copysignf(L.quot[c][(int)aq], q) : q - num / q;
+        const float diff = a - lev;
         dd = diff * diff;
-        if (q != 0.0f) {
-          const float aq = fabsf(q);
-          const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
-          mgv = ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
-          nzf = true;
+        if (aq != 0.0f) {
+          mgv = aq < 256.0f ? L.rate[(int)aq] : rate_log2_slow(D, aq);
+          spv = scan[k] + 1;
         }
       }
       dd_r[j] = dd;
       L.mg[c][e] = mgv;
-      const uint32_t cnt = (uint32_t)__popcll(__ballot(nzf));
-      const uint32_t mx = wave_max_u32(nzf ? scan[k] : 0u);
-      if (lane == 0) {
-        L.nz[site][c] += cnt;
-        L.maxscan[site][c] = max(L.maxscan[site][c], mx);
+      const uint32_t cnt = (uint32_t)__popcll(__ballot(spv != 0));
+      const uint32_t mx = wave_max_u32(spv);
+      if (lane == 0 && cnt) {
+        atomicAdd(&L.nz[site][c], cnt);
+        atomicMax(&L.maxscan[site][c], mx);
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 16; j++) (&L.co[c][0])[lane + 64 * j] = dd_r[j];
+    for (int j = 0; j < 4; j++) (&L.co[c][0])[tid + SEL_T * j] = dd_r[j];
   }
   __syncthreads();
   // Sums in the reference's sequential row-major storage order (f32).
-  for (int wk = lane; wk < nsites * 6; wk += 64) {
+  for (int wk = tid; wk < nsites * 6; wk += SEL_T) {
     const int site = wk / 6, rest = wk - site * 6, c = rest >> 1, which = rest & 1;
     const float4 *src = reinterpret_cast<const float4 *>(
         which == 0 ? &L.co[c][0] + site * size : &L.mg[c][0] + site * size);
@@ -517,8 +562,8 @@ __device__ void eval_squares(SelLds &L, int n, const float *qac_in, float cmx, f
       L.mag[site][c] = acc;
   }
   __syncthreads();
-  if (lane < nsites) {
-    const int site = lane;
+  if (tid < nsites) {
+    const int site = tid;
     const int sy = site / sites_1d, sx = site - sy * sites_1d;
     float d_total = 0.0f, r_total = 0.0f;
     for (int c = 0; c < 3; c++) {
@@ -528,7 +573,8 @@ __device__ void eval_squares(SelLds &L, int n, const float *qac_in, float cmx, f
       const float header = 0.4f * (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf));
       float visited = 0.0f;
       if (nzc != 0) {
-        const int64_t span = (int64_t)L.maxscan[site][c] + 1 - (int64_t)(cblk * cblk + nzc);
+        // maxscan holds 1 + the last nonzero's scan position
+        const int64_t span = (int64_t)L.maxscan[site][c] - (int64_t)(cblk * cblk + nzc);
         visited = span > 0 ? (float)span : 0.0f;
       }
       r_total += nzf * 1.6f + 1.0f * L.mag[site][c] + header + 0.5f * visited;
@@ -545,8 +591,8 @@ __device__ void eval_squares(SelLds &L, int n, const float *qac_in, float cmx, f
 // Loads the region's three 32x32 pixel planes into L.co.
 __device__ __forceinline__ void load_region(SelLds &L, const float *__restrict__ xyb, size_t plane,
                                             int stride, int px0, int py0) {
-  const int lane = threadIdx.x;
-  for (int e = lane; e < 3 * 32 * 8; e += 64) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < 3 * 32 * 8; e += SEL_T) {
     const int c = e >> 8, r = (e >> 3) & 31, q = e & 7;
     const float4 v = *reinterpret_cast<const float4 *>(xyb + c * plane + (size_t)(py0 + r) * stride +
                                                        px0 + 4 * q);
@@ -596,33 +642,42 @@ __device__ __forceinline__ float site_bias(const jxg_derived *D, double s1, doub
 }
 
 template <bool F>
-__global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, int stride,
-                                               int padded_h, int W, int H,
-                                               uint8_t *__restrict__ qf,
-                                               const int8_t *__restrict__ ytox,
-                                               const int8_t *__restrict__ ytob,
-                                               const float *__restrict__ tables,
-                                               const uint32_t *__restrict__ scans,
-                                               const jxg_derived *__restrict__ D,
-                                               uint8_t *__restrict__ cells, float *__restrict__ c8out,
-                                               float *__restrict__ c16out,
-                                               float *__restrict__ c32out, int xb, int yb) {
+__global__ __launch_bounds__(SEL_T) void k_select(const float *__restrict__ xyb, int stride,
+                                                  int padded_h, int W, int H,
+                                                  uint8_t *__restrict__ qf,
+                                                  const int8_t *__restrict__ ytox,
+                                                  const int8_t *__restrict__ ytob,
+                                                  const float *__restrict__ tables,
+                                                  const uint32_t *__restrict__ scans,
+                                                  const jxg_derived *__restrict__ D,
+                                                  uint8_t *__restrict__ cells,
+                                                  float *__restrict__ c8out,
+                                                  float *__restrict__ c16out,
+                                                  float *__restrict__ c32out, int xb, int yb) {
   __shared__ SelLds L;
   __shared__ uint8_t q_in[16];
   __shared__ uint8_t cell_out[16];
   __shared__ uint8_t q_out[16];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int rbx = blockIdx.x * 4, rby = blockIdx.y * 4;  // region's first block (global)
   const int gx = rbx >> 8, gy = rby >> 8;
   const int xsize = min(256, xb - gx * 256), ysize = min(256, yb - gy * 256);
   const int bx = rbx - gx * 256, by = rby - gy * 256;  // group-relative
-  if (lane < 16) {
-    const int ix = lane & 3, iy = lane >> 2;
+  if (tid < 16) {
+    const int ix = tid & 3, iy = tid >> 2;
     const bool in = rbx + ix < xb && rby + iy < yb;
     const uint8_t q = in ? qf[(size_t)(rby + iy) * xb + rbx + ix] : 1;
-    q_in[lane] = q;
-    q_out[lane] = q;
-    cell_out[lane] = JXG_CELL(JXG_DCT, 1);
+    q_in[tid] = q;
+    q_out[tid] = q;
+    cell_out[tid] = JXG_CELL(JXG_DCT, 1);
+  }
+  {
+    const float num = D->dequant_numerator;
+    if (tid < 192) {
+      const int c = tid >> 6, a = tid & 63;
+      L.quot[c][a] = a >= 2 ? num / (float)a : (a == 1 ? 1.0f - D->dequant_one[c] : 0.0f);
+    }
+    L.rate[tid] = D->rate_lut[tid];
   }
   __syncthreads();
   const float scale = D->scale, dist = D->distance;
@@ -644,38 +699,51 @@ __global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, in
     const float cmb = fmla<F>((float)ytob[ti], D->cfl_inv_color_factor, D->cfl_base_b);
     // Site luma statistics for MergeTuning::site_bias (site_variance_rect,
     // src/ac_strategy.rs:224-245): f64 sums of the in-image pixels of the
-    // four 16x16 sites and the 32x32 region, reduced across the wavefront.
+    // four 16x16 sites and the 32x32 region, reduced over the workgroup.
     load_region(L, xyb, plane, stride, px0, py0);
     {
       double s1[5] = {0, 0, 0, 0, 0}, s2[5] = {0, 0, 0, 0, 0};
       uint32_t cnt[5] = {0, 0, 0, 0, 0};
-      for (int j = 0; j < 16; j++) {
-        const int e = lane + 64 * j, r = e >> 5, col = e & 31;
-        if (py0 + r >= H || px0 + col >= W) continue;
-        const double v = (double)L.co[1][r * SEL_PITCH + col];
-        const int q = (r >> 4) * 2 + (col >> 4);
-        s1[q] += v;
-        s2[q] += v * v;
-        cnt[q]++;
-        s1[4] += v;
-        s2[4] += v * v;
-        cnt[4]++;
-      }
-      for (int q = 0; q < 5; q++) {
-        const double a1 = wave_sum_f64(s1[q]), a2 = wave_sum_f64(s2[q]);
-        uint32_t cc = cnt[q];
-        for (int off = 32; off > 0; off >>= 1) cc += (uint32_t)__shfl_xor((int)cc, off, 64);
-        if (lane == 0) {
-          L.site_s1[q] = a1;
-          L.site_s2[q] = a2;
-          L.site_cnt[q] = cc;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int e = tid + SEL_T * j, r = e >> 5, col = e & 31;
+        if (py0 + r < H && px0 + col < W) {
+          const double v = (double)L.co[1][r * SEL_PITCH + col];
+          const int q = (r >> 4) * 2 + (col >> 4);
+#pragma unroll
+          for (int s = 0; s < 4; s++)
+            if (s == q) {
+              s1[s] += v;
+              s2[s] += v * v;
+              cnt[s]++;
+            }
+          s1[4] += v;
+          s2[4] += v * v;
+          cnt[4]++;
         }
       }
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        const double a1 = wave_sum_f64(s1[q]), a2 = wave_sum_f64(s2[q]);
+        const uint32_t cc = wave_sum_u32(cnt[q]);
+        if (lane == 0) {
+          L.red1[wave][q] = a1;
+          L.red2[wave][q] = a2;
+          L.redc[wave][q] = cc;
+        }
+      }
+      __syncthreads();
+      if (tid < 5) {
+        L.site_s1[tid] = ((L.red1[0][tid] + L.red1[1][tid]) + L.red1[2][tid]) + L.red1[3][tid];
+        L.site_s2[tid] = ((L.red2[0][tid] + L.red2[1][tid]) + L.red2[2][tid]) + L.red2[3][tid];
+        L.site_cnt[tid] = L.redc[0][tid] + L.redc[1][tid] + L.redc[2][tid] + L.redc[3][tid];
+      }
     }
-    __syncthreads();
     float qac8[16], qac16[4], qac32[1];
+#pragma unroll
     for (int i = 0; i < 16; i++) qac8[i] = scale * (float)q_in[i];
     // aggregate_qac_2x2 (src/ac_strategy/selection.rs:488-496)
+#pragma unroll
     for (int s = 0; s < 4; s++) {
       const int sx = (s & 1) * 2, sy = (s >> 1) * 2;
       const float a = qac8[sy * 4 + sx], b = qac8[sy * 4 + sx + 1];
@@ -696,14 +764,14 @@ __global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, in
       }
       qac32[0] = scale * (float)aggregate_quant(max_q, sum, 16, dist);
     }
-    eval_squares<F>(L, 8, qac8, cmx, cmb, tables, scans, D, W, H, px0, py0, L.cost8);
+    eval_squares<F, 3>(L, qac8, cmx, cmb, tables, scans, D, W, H, px0, py0, L.cost8);
     load_region(L, xyb, plane, stride, px0, py0);
-    eval_squares<F>(L, 16, qac16, cmx, cmb, tables, scans + 64, D, W, H, px0, py0, L.cost16);
+    eval_squares<F, 4>(L, qac16, cmx, cmb, tables, scans + 64, D, W, H, px0, py0, L.cost16);
     if (four_row && four_col) {
       load_region(L, xyb, plane, stride, px0, py0);
-      eval_squares<F>(L, 32, qac32, cmx, cmb, tables, scans + 320, D, W, H, px0, py0, &L.cost32);
+      eval_squares<F, 5>(L, qac32, cmx, cmb, tables, scans + 320, D, W, H, px0, py0, &L.cost32);
     }
-    if (lane == 0) {
+    if (tid == 0) {
       // select_super_block at square scope for each super block, then the
       // 32x32 merge (src/ac_strategy/selection.rs:238-455, 1040-1150).
       float children[2][2], baseline[2][2];
@@ -758,6 +826,8 @@ __global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, in
         }
         const float gate =
             risk_gated<F>(D->risk_k, D->accept_32, (float)qmin, (float)qmax, 2.0f, D->exp2_table);
+        // choose_merge_partition (src/ac_strategy/selection.rs:115-150) with
+        // rectangles priced +inf: only the square merge can pass.
         const float child = children[0][0] + children[0][1] + children[1][0] + children[1][1];
         const float base = baseline[0][0] + baseline[0][1] + baseline[1][0] + baseline[1][1];
         if (merged < child && merged < base * gate) {
@@ -785,20 +855,20 @@ __global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, in
       }
     }
     __syncthreads();
-    if (c8out && lane < 16) {
-      const int ix = lane & 3, iy = lane >> 2;
+    if (c8out && tid < 16) {
+      const int ix = tid & 3, iy = tid >> 2;
       const int sx = ix >> 1, sy = iy >> 1;
       const bool eval = (four_row ? true : iy < 2) && (bx + sx * 2 + 1 < xsize);
       if (rbx + ix < xb && rby + iy < yb) {
         const size_t o = (size_t)(rby + iy) * xb + rbx + ix;
-        c8out[o] = eval ? L.cost8[lane] : NAN;
+        c8out[o] = eval ? L.cost8[tid] : NAN;
         const bool s16 = (ix & 1) == 0 && (iy & 1) == 0 && eval;
         c16out[o] = s16 ? L.cost16[sy * 2 + sx] : NAN;
-        c32out[o] = (lane == 0 && four_row && four_col) ? L.cost32 : NAN;
+        c32out[o] = (tid == 0 && four_row && four_col) ? L.cost32 : NAN;
       }
     }
-  } else if (c8out && lane < 16) {
-    const int ix = lane & 3, iy = lane >> 2;
+  } else if (c8out && tid < 16) {
+    const int ix = tid & 3, iy = tid >> 2;
     if (rbx + ix < xb && rby + iy < yb) {
       const size_t o = (size_t)(rby + iy) * xb + rbx + ix;
       c8out[o] = NAN;
@@ -806,12 +876,12 @@ __global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, in
       c32out[o] = NAN;
     }
   }
-  if (lane < 16) {
-    const int ix = lane & 3, iy = lane >> 2;
+  if (tid < 16) {
+    const int ix = tid & 3, iy = tid >> 2;
     if (rbx + ix < xb && rby + iy < yb) {
       const size_t o = (size_t)(rby + iy) * xb + rbx + ix;
-      if (cells) cells[o] = cell_out[lane];
-      qf[o] = q_out[lane];
+      if (cells) cells[o] = cell_out[tid];
+      qf[o] = q_out[tid];
     }
   }
 }
@@ -824,17 +894,22 @@ __global__ __launch_bounds__(64) void k_select(const float *__restrict__ xyb, in
 #define QT_MAXT 64
 
 struct QLds {
-  float co[3][QT_TILE * QT_PITCH];
+  float y2[QT_TILE * QT_PITCH];   // dequantized luma with zeroed LLF (CfL source)
+  float cur[QT_TILE * QT_PITCH];  // the channel being transformed
   int n;
   int s[QT_MAXT];
   int bx[QT_MAXT], by[QT_MAXT];  // global first block
   int lx[QT_MAXT], ly[QT_MAXT];  // LDS origin (pixels)
-  int pre1[QT_MAXT + 1];         // prefix of 3 * pass-1 lines
-  int pre2[QT_MAXT + 1];         // prefix of 3 * pass-2 lines
+  int pre1[QT_MAXT + 1];         // prefix of pass-1 lines (one channel)
+  int pre2[QT_MAXT + 1];         // prefix of pass-2 lines
   int prec[QT_MAXT + 1];         // prefix of 64-coefficient chunks
   int16_t ydc[QT_MAXT][16];
   float xf[QT_MAXT], bf[QT_MAXT];
   int quant[QT_MAXT];
+  float inv_qac[QT_MAXT];  // 1 / (scale * quant), src/group.rs:806
+  float quot[64];          // 0, 1 - one, numerator / a for a >= 2 (see dequant_level_tab)
+  uint8_t line1_t[512], line2_t[512];  // line -> transform, per pass
+  uint8_t chunk_t[64];                 // 64-coefficient chunk -> transform
 };
 
 __device__ __forceinline__ int find_item(const int *pre, int n, int e) {
@@ -949,6 +1024,17 @@ struct QArgs {
 
 __device__ __forceinline__ int ctz32(int v) { return __builtin_ctz((unsigned)v); }
 
+// DequantBias::level (src/group.rs:770-790) with an extended quotient
+// table: quot[0] = 0, quot[1] = 1 - one, quot[a] = numerator / a.  For
+// |q| = 1 the result q - copysign(1 - one, q) is exactly +-one (1 - one is
+// exact for one in [0.5, 1]), so the table form is bit-identical.
+__device__ __forceinline__ float dequant_level_tab(float num, int32_t q, const float *quot) {
+  const uint32_t a = q < 0 ? (uint32_t)(-(int64_t)q) : (uint32_t)q;
+  const float qf = (float)q;
+  if (a >= 64) return qf - num / qf;
+  return qf - copysignf(quot[a], qf);
+}
+
 // Geometry of one coefficient of transform t: LDS address, storage row /
 // column and output index (all sizes are powers of two).
 struct CoefGeom {
@@ -983,11 +1069,62 @@ __device__ __forceinline__ CoefGeom coef_geom(const QLds &L, int t, int k, int x
   return g;
 }
 
-// Transform + quantize the L.n transforms whose footprints are in L.co.
+// Copies channel c of the w x h pixel rectangle at (px, py) of the padded
+// planes into L.cur at LDS origin (0, 0).  w is a multiple of 8.
+__device__ __forceinline__ void load_plane(QLds &L, const QArgs &A, int c, int px, int py, int w,
+                                           int h) {
+  const size_t plane = (size_t)A.stride * A.padded_h;
+  const float *src = A.xyb + c * plane;
+  const int q4 = w >> 2;
+  for (int e = threadIdx.x; e < h * q4; e += blockDim.x) {
+    const int r = e / q4, q = e - r * q4;
+    const float4 v = *reinterpret_cast<const float4 *>(src + (size_t)(py + r) * A.stride + px + 4 * q);
+    float *dst = &L.cur[r * QT_PITCH + 4 * q];
+    dst[0] = v.x;
+    dst[1] = v.y;
+    dst[2] = v.z;
+    dst[3] = v.w;
+  }
+}
+
+// Both 1-D passes of every transform on L.cur (one channel).
 template <bool F>
-__device__ void quantize_list(QLds &L, const QArgs &A) {
+__device__ void transform_plane(QLds &L, int n) {
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
+  for (int e = tid; e < L.pre1[n]; e += nthr) {
+    const int t = L.line1_t[e];
+    const int s = L.s[t];
+    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
+    const int li = e - L.pre1[t];
+    float *base = &L.cur[L.ly[t] * QT_PITCH + L.lx[t]];
+    if (rows_first(s))
+      dct_line_n<F>(w, base + li * QT_PITCH, 1, 0.0f, false);
+    else
+      dct_line_n<F>(h, base + li, QT_PITCH, 0.0f, false);
+  }
+  __syncthreads();
+  for (int e = tid; e < L.pre2[n]; e += nthr) {
+    const int t = L.line2_t[e];
+    const int s = L.s[t];
+    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
+    const int li = e - L.pre2[t];
+    const float scale = 1.0f / (float)(w * h);
+    float *base = &L.cur[L.ly[t] * QT_PITCH + L.lx[t]];
+    if (rows_first(s))
+      dct_line_n<F>(h, base + li, QT_PITCH, scale, true);
+    else
+      dct_line_n<F>(w, base + li * QT_PITCH, 1, scale, true);
+  }
+  __syncthreads();
+}
+
+// Transform + quantize the L.n transforms listed in L, one channel at a
+// time: Y first (its dequantized form stays in L.y2 for CfL), then X and B.
+// (px0, py0, win_w, win_h) is the pixel rectangle that L.cur holds.
+template <bool F>
+__device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win_w, int win_h) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
   const jxg_derived *D = A.D;
   const int n = L.n;
   if (n == 0) return;
@@ -999,8 +1136,8 @@ __device__ void quantize_list(QLds &L, const QArgs &A) {
       L.pre1[t] = p1;
       L.pre2[t] = p2;
       L.prec[t] = pc;
-      p1 += 3 * (rows_first(s) ? h : w);
-      p2 += 3 * (rows_first(s) ? w : h);
+      p1 += rows_first(s) ? h : w;
+      p2 += rows_first(s) ? w : h;
       pc += cov_x(s) * cov_y(s);
     }
     L.pre1[n] = p1;
@@ -1013,48 +1150,29 @@ __device__ void quantize_list(QLds &L, const QArgs &A) {
     const size_t ti = (size_t)(L.by[t] / 8) * tiles_x + L.bx[t] / 8;
     L.xf[t] = fmla<F>((float)A.ytox[ti], D->cfl_inv_color_factor, D->cfl_base_x);
     L.bf[t] = fmla<F>((float)A.ytob[ti], D->cfl_inv_color_factor, D->cfl_base_b);
-    L.quant[t] = A.qf[(size_t)L.by[t] * A.xb + L.bx[t]];
+    const int quant = A.qf[(size_t)L.by[t] * A.xb + L.bx[t]];
+    L.quant[t] = quant;
+    L.inv_qac[t] = 1.0f / (D->scale * (float)quant);
   }
+  if (tid < 64)
+    L.quot[tid] = tid >= 2 ? D->dequant_numerator / (float)tid
+                           : (tid == 1 ? 1.0f - D->dequant_one[1] : 0.0f);
   __syncthreads();
-  // Pass 1: rows (8x8, 8x16, 16x32, 8x32) or columns (the rest).
-  for (int e = tid; e < L.pre1[n]; e += nthr) {
-    const int t = find_item(L.pre1, n, e);
-    const int s = L.s[t];
-    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
-    const bool rf = rows_first(s);
-    const int lper = 3 + ctz32(rf ? cov_y(s) : cov_x(s));
-    const int local = e - L.pre1[t];
-    const int c = local >> lper, li = local & ((1 << lper) - 1);
-    float *base = &L.co[c][L.ly[t] * QT_PITCH + L.lx[t]];
-    if (rf)
-      dct_line_n<F>(w, base + li * QT_PITCH, 1, 0.0f, false);
-    else
-      dct_line_n<F>(h, base + li, QT_PITCH, 0.0f, false);
+  // line / chunk -> transform tables
+  if (tid < n) {
+    for (int e = L.pre1[tid]; e < L.pre1[tid + 1]; e++) L.line1_t[e] = (uint8_t)tid;
+    for (int e = L.pre2[tid]; e < L.pre2[tid + 1]; e++) L.line2_t[e] = (uint8_t)tid;
+    for (int e = L.prec[tid]; e < L.prec[tid + 1]; e++) L.chunk_t[e] = (uint8_t)tid;
   }
+  // ---- Y ----
+  load_plane(L, A, 1, px0, py0, win_w, win_h);
   __syncthreads();
-  // Pass 2: the other axis, scaled by 1/(w*h).
-  for (int e = tid; e < L.pre2[n]; e += nthr) {
-    const int t = find_item(L.pre2, n, e);
-    const int s = L.s[t];
-    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
-    const bool rf = rows_first(s);
-    const int lper = 3 + ctz32(rf ? cov_x(s) : cov_y(s));
-    const int local = e - L.pre2[t];
-    const int c = local >> lper, li = local & ((1 << lper) - 1);
-    const float scale = 1.0f / (float)(w * h);
-    float *base = &L.co[c][L.ly[t] * QT_PITCH + L.lx[t]];
-    if (rf)
-      dct_line_n<F>(h, base + li, QT_PITCH, scale, true);
-    else
-      dct_line_n<F>(w, base + li * QT_PITCH, 1, scale, true);
-  }
-  __syncthreads();
+  transform_plane<F>(L, n);
   // Y DC (dc_from_coefficients, quantize_dc_scalar src/group.rs:290)
   if (tid < n) {
     const int t = tid, s = L.s[t];
-    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
     float dcv[16];
-    dc_from<F>(s, &L.co[1][L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, w, h, dcv);
+    dc_from<F>(s, &L.cur[L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, cov_x(s) * 8, cov_y(s) * 8, dcv);
     const int cvx = cov_x(s), covered = cvx * cov_y(s);
 #pragma unroll
     for (int i = 0; i < 16; i++) {
@@ -1065,92 +1183,126 @@ __device__ void quantize_list(QLds &L, const QArgs &A) {
       }
     }
   }
-  __syncthreads();
   // Y quantize / dequantize round trip (quantize_roundtrip_y_block,
-  // src/group.rs:795), Y LLF zeroing (:1852) and CfL on X and B (:1863).
-  // Each wavefront takes 64 consecutive coefficients of one transform.
-  for (int ch = wave; ch < L.prec[n]; ch += nwaves) {
-    const int t = find_item(L.prec, n, ch);
-    const int s = L.s[t];
-    const int k = ((ch - L.prec[t]) << 6) + lane;
-    const CoefGeom g = coef_geom(L, t, k, A.xb);
-    const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
-    const int kind = jxg_qt_kind_dev(s);
-    const float *tb = A.tables + D->qt_off[kind];
-    const int qn = D->qt_n[kind];
-    const float yv = L.co[1][g.a];
-    A.co1[g.out] = yv;
-    const float *thr = D->thr[1][jxg_area_class(cx, cy)];
-    const int yfix = g.row >= g.hc / 2 ? 2 : 0;
-    const float threshold = g.col >= g.wc / 2 ? thr[yfix + 1] : thr[yfix];
-    const int32_t quant = L.quant[t];
-    const float q_scaled = D->scale * (float)quant * 1.0f;
-    const float val = (tb[4 * qn + k] * q_scaled) * yv;
-    const int32_t q = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
-    A.lv1[g.out] = q;
-    const float inv_qac = 1.0f / (D->scale * (float)quant);
-    float y2 =
-        dequant_level(D->dequant_one[1], D->dequant_numerator, q) * tb[1 * qn + k] * inv_qac;
-    if (g.row < cy && g.col < cx) y2 = 0.0f;
-    L.co[0][g.a] = fmla<F>(-L.xf[t], y2, L.co[0][g.a]);
-    L.co[2][g.a] = fmla<F>(-L.bf[t], y2, L.co[2][g.a]);
-  }
-  __syncthreads();
-  // X and B DC after CfL (src/group.rs:1871-2003, 2128-2150)
-  if (tid < n) {
-    const int t = tid, s = L.s[t];
-    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
-    float xd[16], bd[16];
-    dc_from<F>(s, &L.co[0][L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, w, h, xd);
-    dc_from<F>(s, &L.co[2][L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, w, h, bd);
-    const int cvx = cov_x(s), covered = cvx * cov_y(s);
+  // src/group.rs:795) and Y LLF zeroing (:1852).  Each wavefront takes 64
+  // consecutive coefficients of one transform; four chunks per round so the
+  // table loads of a round are in flight together.
+  const int nch = L.prec[n];
+  for (int base = wave; base < nch; base += 4 * nwaves) {
+    CoefGeom g[4];
+    float tq[4], td[4], yv[4];
+    int tt[4], kk[4];
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-      if (i < covered) {
-        const float yq = (float)L.ydc[t][i];
-        int16_t xq;
-        if (D->cfl_factor_x != 0.0f)
-          xq = sat_i16(round_away(fmla<F>(xd[i], D->inv_factor[0], -yq * D->cfl_factor_x)));
-        else
-          xq = sat_i16(round_away(xd[i] * D->inv_factor[0]));
-        const int16_t bq =
-            sat_i16(round_away(fmla<F>(bd[i], D->inv_factor[2], -yq * D->cfl_factor_b)));
-        const size_t o = (size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx;
-        A.dc0[o] = xq;
-        A.dc2[o] = bq;
+    for (int u = 0; u < 4; u++) {
+      const int ch = base + u * nwaves;
+      if (ch < nch) {
+        const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
+        const int s = L.s[t];
+        const int k = ((ch - L.prec[t]) << 6) + lane;
+        const int kind = jxg_qt_kind_dev(s);
+        const float *tb = A.tables + D->qt_off[kind];
+        const int qn = D->qt_n[kind];
+        g[u] = coef_geom(L, t, k, A.xb);
+        tq[u] = tb[4 * qn + k];
+        td[u] = tb[1 * qn + k];
+        yv[u] = L.cur[g[u].a];
+        tt[u] = t;
+        kk[u] = k;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int ch = base + u * nwaves;
+      if (ch < nch) {
+        const int t = tt[u], s = L.s[t];
+        const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
+        A.co1[g[u].out] = yv[u];
+        const float *thr = D->thr[1][jxg_area_class(cx, cy)];
+        const int yfix = g[u].row >= g[u].hc / 2 ? 2 : 0;
+        const float threshold = g[u].col >= g[u].wc / 2 ? thr[yfix + 1] : thr[yfix];
+        const float q_scaled = D->scale * (float)L.quant[t] * 1.0f;
+        const float val = (tq[u] * q_scaled) * yv[u];
+        const int32_t q = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+        A.lv1[g[u].out] = q;
+        float y2 = dequant_level_tab(D->dequant_numerator, q, L.quot) * td[u] * L.inv_qac[t];
+        if (g[u].row < cy && g[u].col < cx) y2 = 0.0f;
+        L.y2[g[u].a] = y2;
+        (void)kk;
       }
     }
   }
-  // X and B AC quantization (quantize_block_ac_scalar, src/group.rs:560-620)
-  for (int ch = wave; ch < L.prec[n]; ch += nwaves) {
-    const int t = find_item(L.prec, n, ch);
-    const int s = L.s[t];
-    const int k = ((ch - L.prec[t]) << 6) + lane;
-    const CoefGeom g = coef_geom(L, t, k, A.xb);
-    const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
-    const int kind = jxg_qt_kind_dev(s);
-    const float *tb = A.tables + D->qt_off[kind];
-    const int qn = D->qt_n[kind];
-    const int cls = jxg_area_class(cx, cy);
-    const int yfix = g.row >= g.hc / 2 ? 2 : 0;
-    const bool hi = g.col >= g.wc / 2;
-    const int32_t quant = L.quant[t];
-    const float xv = L.co[0][g.a];
-    A.co0[g.out] = xv;
-    {
-      const float threshold = hi ? D->thr[0][cls][yfix + 1] : D->thr[0][cls][yfix];
-      const float q_scaled = D->scale * (float)quant * D->qm_mult[0];
-      const float val = (tb[3 * qn + k] * q_scaled) * xv;
-      A.lv0[g.out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+  __syncthreads();
+  // ---- X (c = 0) and B (c = 2): CfL (:1863), DC with CfL (:1871-2003,
+  // 2128-2150) and AC quantization (quantize_block_ac_scalar :560) ----
+  for (int c = 0; c <= 2; c += 2) {
+    load_plane(L, A, c, px0, py0, win_w, win_h);
+    __syncthreads();
+    transform_plane<F>(L, n);
+    int32_t *lv = c == 0 ? A.lv0 : A.lv2;
+    float *co = c == 0 ? A.co0 : A.co2;
+    const float qm = D->qm_mult[c];
+    for (int base = wave; base < nch; base += 4 * nwaves) {
+      CoefGeom g[4];
+      float tq[4];
+      int tt[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int ch = base + u * nwaves;
+        if (ch < nch) {
+          const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
+          const int s = L.s[t];
+          const int k = ((ch - L.prec[t]) << 6) + lane;
+          const int kind = jxg_qt_kind_dev(s);
+          g[u] = coef_geom(L, t, k, A.xb);
+          tq[u] = A.tables[D->qt_off[kind] + (3 + c) * D->qt_n[kind] + k];
+          tt[u] = t;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int ch = base + u * nwaves;
+        if (ch < nch) {
+          const int t = tt[u], s = L.s[t];
+          const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
+          const float f = c == 0 ? L.xf[t] : L.bf[t];
+          const float v = fmla<F>(-f, L.y2[g[u].a], L.cur[g[u].a]);
+          L.cur[g[u].a] = v;
+          co[g[u].out] = v;
+          const int cls = jxg_area_class(cx, cy);
+          const int yfix = g[u].row >= g[u].hc / 2 ? 2 : 0;
+          const float threshold =
+              g[u].col >= g[u].wc / 2 ? D->thr[c][cls][yfix + 1] : D->thr[c][cls][yfix];
+          const float q_scaled = D->scale * (float)L.quant[t] * qm;
+          const float val = (tq[u] * q_scaled) * v;
+          lv[g[u].out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+        }
+      }
     }
-    const float bv = L.co[2][g.a];
-    A.co2[g.out] = bv;
-    {
-      const float threshold = hi ? D->thr[2][cls][yfix + 1] : D->thr[2][cls][yfix];
-      const float q_scaled = D->scale * (float)quant * D->qm_mult[2];
-      const float val = (tb[5 * qn + k] * q_scaled) * bv;
-      A.lv2[g.out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+    __syncthreads();
+    if (tid < n) {
+      const int t = tid, s = L.s[t];
+      float xd[16];
+      dc_from<F>(s, &L.cur[L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, cov_x(s) * 8, cov_y(s) * 8, xd);
+      const int cvx = cov_x(s), covered = cvx * cov_y(s);
+      int16_t *dc = c == 0 ? A.dc0 : A.dc2;
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        if (i < covered) {
+          const float yq = (float)L.ydc[t][i];
+          int16_t qd;
+          if (c == 0) {
+            if (D->cfl_factor_x != 0.0f)
+              qd = sat_i16(round_away(fmla<F>(xd[i], D->inv_factor[0], -yq * D->cfl_factor_x)));
+            else
+              qd = sat_i16(round_away(xd[i] * D->inv_factor[0]));
+          } else {
+            qd = sat_i16(round_away(fmla<F>(xd[i], D->inv_factor[2], -yq * D->cfl_factor_b)));
+          }
+          dc[(size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx] = qd;
+        }
+      }
     }
+    __syncthreads();
   }
 }
 
@@ -1185,21 +1337,8 @@ __global__ __launch_bounds__(256) void k_quantize(QArgs A) {
       }
     }
   }
-  // Load the 64x64 tile of all three planes (padded planes: always in range).
-  const size_t plane = (size_t)A.stride * A.padded_h;
-  const int px0 = bx0 * 8, py0 = by0 * 8;
-  for (int e = tid; e < 3 * 64 * 16; e += 256) {
-    const int c = e / (64 * 16), r = (e >> 4) & 63, q = e & 15;
-    const float4 v =
-        *reinterpret_cast<const float4 *>(A.xyb + c * plane + (size_t)(py0 + r) * A.stride + px0 + 4 * q);
-    float *dst = &L.co[c][r * QT_PITCH + 4 * q];
-    dst[0] = v.x;
-    dst[1] = v.y;
-    dst[2] = v.z;
-    dst[3] = v.w;
-  }
   __syncthreads();
-  quantize_list<F>(L, A);
+  quantize_list<F>(L, A, bx0 * 8, by0 * 8, 64, 64);
 }
 
 // Transforms whose footprint crosses their 64x64 tile (the layout DP's
@@ -1210,12 +1349,10 @@ __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
   __shared__ QLds L;
   const int tid = threadIdx.x;
   const uint32_t count = min(*A.xt_count, A.xt_cap);
-  const size_t plane = (size_t)A.stride * A.padded_h;
   for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
     const uint32_t blk = A.xt_list[item];
     const int bx = (int)(blk % (uint32_t)A.xb), by = (int)(blk / (uint32_t)A.xb);
     const int s = A.cells[blk] >> 1;
-    const int w = cov_x(s) * 8, h = cov_y(s) * 8;
     if (tid == 0) {
       L.n = 1;
       L.s[0] = s;
@@ -1224,12 +1361,8 @@ __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
       L.lx[0] = 0;
       L.ly[0] = 0;
     }
-    for (int e = tid; e < 3 * w * h; e += 256) {
-      const int c = e / (w * h), r = (e / w) % h, col = e % w;
-      L.co[c][r * QT_PITCH + col] = A.xyb[c * plane + (size_t)(by * 8 + r) * A.stride + bx * 8 + col];
-    }
     __syncthreads();
-    quantize_list<F>(L, A);
+    quantize_list<F>(L, A, bx * 8, by * 8, cov_x(s) * 8, cov_y(s) * 8);
     __syncthreads();
   }
 }
@@ -1532,9 +1665,9 @@ static int select_impl(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32
   const int xb = (W + 7) / 8, yb = (H + 7) / 8;
   const dim3 grid((xb + 3) / 4, (yb + 3) / 4);
   if (p->fused)
-    k_select<true><<<grid, 64, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+    k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_derived, cells, c8, c16, c32, xb, yb);
   else
-    k_select<false><<<grid, 64, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+    k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_derived, cells, c8, c16, c32, xb, yb);
   LAUNCH_CHECK("k_select");
   return JXG_OK;
 }

@@ -32,6 +32,10 @@ STAGE_BYTES_PER_PX = {
 }
 
 
+EXTRA_BYTES_PER_PX = {
+    "gaborish": 12.0 + 12.0,  # 3 planes in, 3 planes out
+}
+
 CAL_BYTES = 256 << 20  # calibration copy size for the byte counters
 
 # Kernel of each stage, as rocprofv3 names it.
@@ -92,6 +96,24 @@ def cpu_baseline(rgb, params, threads):
         "sample": f"whole front end of the same {w}x{h} image, median of {len(runs)} runs "
                   f"({sec:.2f} s each) with {threads} threads",
     }
+
+
+def time_kernel(fn, stream, reps, alg_bytes):
+    """Mean device time of fn() over reps launches (HIP events on the
+    context's stream) and its HBM roofline position."""
+    import torch
+    fn()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbs = alg_bytes / (ms * 1e-3) / 1e9
+    return {"ms": round(ms, 4), "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": int(alg_bytes)}
 
 
 def host_path_rate(fe, rgb, params, reps=5):
@@ -203,6 +225,16 @@ def main():
         torch.cuda.synchronize()
         for i, k in enumerate(stage_ms):
             stage_ms[k] += ev[i].elapsed_time(ev[i + 1]) / reps
+    # Kernels beside the measured chain (not part of the Fast-tier step),
+    # each timed alone on the same image with its algorithmic bytes.
+    extra = {}
+    gab_out = torch.empty_like(xyb)
+
+    def gab():
+        fe.gaborish_inverse(xyb, w, h, fused=bool(p.fused), out=gab_out)
+    extra["gaborish"] = time_kernel(gab, stream, reps, EXTRA_BYTES_PER_PX["gaborish"] * w * h)
+    del gab_out
+
     px = w * h
     dominant = max(stage_ms, key=stage_ms.get)
     alg_bytes = STAGE_BYTES_PER_PX[dominant] * px
@@ -244,6 +276,7 @@ def main():
             "algorithmic_bytes_per_launch": int(alg_bytes),
         },
         "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "other_kernels": extra,
     }
     if not args.profile:
         result["host_buffer_path"] = host_path_rate(fe, rgb, p)

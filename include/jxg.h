@@ -280,6 +280,19 @@ int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channe
                   const int8_t *ytox, const int8_t *ytob, uint8_t *qf, uint8_t *cells,
                   int32_t *levels[3], float *coeffs[3], int16_t *dc[3]);
 
+/* Gaborish pre-inversion of the three XYB planes: a normalised symmetric
+ * 5x5 sharpening with mirror-reflected image edges, so that the decoder's
+ * gaborish smoothing reproduces the input.  Replaces gaborish_inverse
+ * (src/gaborish.rs:44) as called by gaborize (src/frame/quantization.rs:241,
+ * mul 0.9908511).  d_dst may equal d_src (in place, as the reference runs
+ * it; staged through context scratch) or be a separate set of planes of the
+ * same geometry.  Pixels past the image are written as the edge replication
+ * of the filtered image.  mul must be finite and >= 0 (the reference
+ * debug-asserts mul >= 0). */
+int jxg_gaborish_inverse_dev(jxg_ctx *ctx, const float *d_src, float *d_dst, uint32_t stride,
+                             uint32_t padded_h, uint32_t width, uint32_t height, float mul,
+                             int fused);
+
 /* Status of the last jxg_forward_quantize_dev on this context (synchronizes):
  * out[0] = transforms whose footprint crossed their 64x64 tile (handled by
  * the crossing pass), out[1] = 1 if a strategy cell was unsupported or did

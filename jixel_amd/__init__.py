@@ -44,6 +44,9 @@ STRATEGY = dict(DCT=0, DCT16X8=1, DCT8X16=2, DCT16X16=3, DCT32X32=4, DCT4X4=5, D
                 DCT64X64=14, DCT64X32=15, DCT32X64=16, IDENTITY=17, DCT2X2=18, DCT128X128=19,
                 DCT128X64=20, DCT64X128=21, DCT32X8=22, DCT8X32=23)
 
+# gaborize's multiplier (src/frame/quantization.rs:243).
+GABORISH_MUL = 0.9908511
+
 
 def distance_from_quality(quality):
     """distance_from_quality (src/encode_image.rs:1321-1337); NaN is rejected."""
@@ -256,6 +259,15 @@ class FrontEnd:
                                              ctypes.byref(params), arr(o["levels"]),
                                              arr(o["coeffs"]), arr(o["dc"])))
         return o
+
+    def gaborish_inverse(self, xyb, width, height, mul=GABORISH_MUL, fused=True, out=None):
+        """Gaborish pre-inversion of (3, padded_h, stride) planes; in place
+        unless `out` (same geometry) is given."""
+        out = xyb if out is None else out
+        check(lib().jxg_gaborish_inverse_dev(self.ctx, _ptr(xyb), _ptr(out), xyb.shape[2],
+                                             xyb.shape[1], width, height, float(mul),
+                                             1 if fused else 0))
+        return out
 
     def quantize_status(self):
         out = (ctypes.c_uint32 * 2)()

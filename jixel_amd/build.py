@@ -24,6 +24,9 @@ DEVICE = ["--offload-arch=gfx950", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+HIP_SOURCES = ["jxg_kernels.hip", "jxg_filters.hip"]
+
+
 def run(cmd):
     print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
@@ -32,11 +35,14 @@ def run(cmd):
 def build(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
     params_o = os.path.join(BUILD, "jxg_params.o")
-    kernels_o = os.path.join(BUILD, "jxg_kernels.o")
     run([CXX] + COMMON + ["-c", os.path.join(CSRC, "jxg_params.cpp"), "-o", params_o])
-    run([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, "jxg_kernels.hip"), "-o", kernels_o])
+    objs = [params_o]
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        run([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     tmp = LIB + ".tmp"
-    run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp, kernels_o, params_o])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     return LIB
 

@@ -1,0 +1,81 @@
+// jxg_ctx.h - host-side context and error plumbing shared by the kernel
+// translation units of libjxg.
+#ifndef JXG_CTX_H_
+#define JXG_CTX_H_
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "jxg.h"
+#include "jxg_internal.h"
+
+#define JXG_HIDDEN __attribute__((visibility("hidden")))
+
+// Records a message for jxg_last_error() on the calling thread; returns code.
+JXG_HIDDEN int set_err(int code, const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return set_err(e_ == hipErrorOutOfMemory ? JXG_ERR_ALLOC : JXG_ERR_DEVICE, "%s: %s", \
+                     #expr, hipGetErrorString(e_));                                      \
+  } while (0)
+
+#define LAUNCH_CHECK(name)                                                               \
+  do {                                                                                   \
+    hipError_t e_ = hipGetLastError();                                                   \
+    if (e_ != hipSuccess)                                                                \
+      return set_err(JXG_ERR_DEVICE, "launch %s: %s", name, hipGetErrorString(e_));      \
+  } while (0)
+
+struct jxg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float *d_tables = nullptr;
+  bool tables_set = false;
+  jxg_derived *d_derived = nullptr;
+  jxg_derived *h_derived = nullptr;  // pinned staging
+  bool derived_valid = false;
+  jxg_frame_params last_params;
+  uint32_t *d_scans = nullptr;  // 64 + 256 + 1024 natural scan positions
+  float *d_mlut = nullptr;      // fwd matrix (9) + transfer LUT (256)
+  float *h_mlut = nullptr;
+  bool mlut_valid = false;
+  float *d_exp2 = nullptr;      // fast_exp2 table
+  float *d_pre = nullptr;
+  size_t pre_cap = 0;
+  float *d_xyb = nullptr;
+  size_t xyb_cap = 0;
+  uint32_t *d_misc = nullptr;  // [0] crossing count, [1] unsupported flag
+  uint32_t *d_xt = nullptr;
+  size_t xt_cap = 0;
+  int8_t *d_zero_cmap = nullptr;
+  size_t zero_cap = 0;
+  uint8_t *d_rgb = nullptr;
+  size_t rgb_cap = 0;
+  float *d_tmp = nullptr;  // out-of-place staging of in-place filters
+  size_t tmp_cap = 0;
+};
+
+template <typename T>
+inline int ensure(T **p, size_t *cap, size_t n) {
+  if (*cap >= n && *p) return JXG_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  HIP_TRY(hipMalloc((void **)p, n * sizeof(T)));
+  *cap = n;
+  return JXG_OK;
+}
+
+// Shared argument checks and per-frame uploads (jxg_kernels.hip).
+JXG_HIDDEN int check_image(uint32_t w, uint32_t h);
+JXG_HIDDEN int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t padded_h);
+JXG_HIDDEN int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p);
+JXG_HIDDEN int fill_u8(jxg_ctx *ctx, uint8_t *p, size_t n, uint8_t v);
+JXG_HIDDEN int zero_cmaps(jxg_ctx *ctx, uint32_t w, uint32_t h, const int8_t **x,
+                          const int8_t **b);
+
+#endif  // JXG_CTX_H_

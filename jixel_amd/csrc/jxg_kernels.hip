@@ -27,6 +27,7 @@
 
 #include <mutex>
 
+#include "jxg_ctx.h"
 #include "jxg_device.h"
 #include "jxg_internal.h"
 
@@ -37,28 +38,13 @@ using namespace jxg;
 
 static thread_local char g_err[512] = "";
 
-static int set_err(int code, const char *fmt, ...) {
+int set_err(int code, const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
   return code;
 }
-
-#define HIP_TRY(expr)                                                                    \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess)                                                                \
-      return set_err(e_ == hipErrorOutOfMemory ? JXG_ERR_ALLOC : JXG_ERR_DEVICE, "%s: %s", \
-                     #expr, hipGetErrorString(e_));                                      \
-  } while (0)
-
-#define LAUNCH_CHECK(name)                                                               \
-  do {                                                                                   \
-    hipError_t e_ = hipGetLastError();                                                   \
-    if (e_ != hipSuccess)                                                                \
-      return set_err(JXG_ERR_DEVICE, "launch %s: %s", name, hipGetErrorString(e_));      \
-  } while (0)
 
 // ---------------------------------------------------------------------------
 // XYB
@@ -1369,50 +1355,28 @@ __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
 
 // Clears a small counter block (a kernel rather than a memset node, so the
 // sequence can be captured into a graph).
+// Byte fill, used instead of hipMemsetAsync so that streams captured into a
+// graph hold kernel nodes only.
+__global__ void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+JXG_HIDDEN int fill_u8(jxg_ctx *ctx, uint8_t *p, size_t n, uint8_t v) {
+  if (n == 0) return JXG_OK;
+  const size_t blocks = (n + 255) / 256;
+  k_fill_u8<<<(unsigned)(blocks < 4096 ? blocks : 4096), 256, 0, ctx->stream>>>(p, n, v);
+  LAUNCH_CHECK("k_fill_u8");
+  return JXG_OK;
+}
+
 __global__ void k_zero_u32(uint32_t *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
 // Context and C ABI
-
-struct jxg_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  float *d_tables = nullptr;
-  bool tables_set = false;
-  jxg_derived *d_derived = nullptr;
-  jxg_derived *h_derived = nullptr;  // pinned staging
-  bool derived_valid = false;
-  jxg_frame_params last_params;
-  uint32_t *d_scans = nullptr;  // 64 + 256 + 1024 natural scan positions
-  float *d_mlut = nullptr;      // fwd matrix (9) + transfer LUT (256)
-  float *h_mlut = nullptr;
-  bool mlut_valid = false;
-  float *d_exp2 = nullptr;      // fast_exp2 table
-  float *d_pre = nullptr;
-  size_t pre_cap = 0;
-  float *d_xyb = nullptr;
-  size_t xyb_cap = 0;
-  uint32_t *d_misc = nullptr;  // [0] crossing count, [1] unsupported flag
-  uint32_t *d_xt = nullptr;
-  size_t xt_cap = 0;
-  int8_t *d_zero_cmap = nullptr;
-  size_t zero_cap = 0;
-  uint8_t *d_rgb = nullptr;
-  size_t rgb_cap = 0;
-};
-
-template <typename T>
-static int ensure(T **p, size_t *cap, size_t n) {
-  if (*cap >= n && *p) return JXG_OK;
-  if (*p) (void)hipFree(*p);
-  *p = nullptr;
-  *cap = 0;
-  HIP_TRY(hipMalloc((void **)p, n * sizeof(T)));
-  *cap = n;
-  return JXG_OK;
-}
 
 // scan_pos_lut (src/coeff_order.rs:358-415) of the square shapes from the
 // natural order (natural_scan_generic, src/coeff_order.rs:206-248).
@@ -1450,14 +1414,14 @@ static void natural_scan(int cx, int cy, uint32_t *out) {
   }
 }
 
-static int check_image(uint32_t w, uint32_t h) {
+int check_image(uint32_t w, uint32_t h) {
   if (w == 0 || h == 0) return set_err(JXG_ERR_EMPTY_IMAGE, "empty image %ux%u", w, h);
   if (w > JXG_MAX_DIM || h > JXG_MAX_DIM)
     return set_err(JXG_ERR_TOO_LARGE, "image %ux%u exceeds %u", w, h, JXG_MAX_DIM);
   return JXG_OK;
 }
 
-static int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t padded_h) {
+int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t padded_h) {
   uint32_t s, ph;
   jxg_plane_geometry(w, h, &s, &ph);
   if (stride < s || padded_h < ph || (stride & 3))
@@ -1466,7 +1430,7 @@ static int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t padded
   return JXG_OK;
 }
 
-static int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p) {
+int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p) {
   if (!p) return set_err(JXG_ERR_INVALID_ARG, "null params");
   if (ctx->derived_valid && memcmp(&ctx->last_params, p, sizeof(*p)) == 0) return JXG_OK;
   jxg_derived tmp;
@@ -1481,13 +1445,14 @@ static int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p) {
   return JXG_OK;
 }
 
-static int zero_cmaps(jxg_ctx *ctx, uint32_t w, uint32_t h, const int8_t **x, const int8_t **b) {
+int zero_cmaps(jxg_ctx *ctx, uint32_t w, uint32_t h, const int8_t **x, const int8_t **b) {
   if (*x && *b) return JXG_OK;
   const size_t n = (size_t)((w + 63) / 64) * ((h + 63) / 64);
   if (ctx->zero_cap < n) {
     int rc = ensure(&ctx->d_zero_cmap, &ctx->zero_cap, n);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(ctx->d_zero_cmap, 0, n, ctx->stream));
+    rc = fill_u8(ctx, (uint8_t *)ctx->d_zero_cmap, n, 0);
+    if (rc) return rc;
   }
   if (!*x) *x = ctx->d_zero_cmap;
   if (!*b) *b = ctx->d_zero_cmap;
@@ -1577,6 +1542,7 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_xt);
   (void)hipFree(ctx->d_zero_cmap);
   (void)hipFree(ctx->d_rgb);
+  (void)hipFree(ctx->d_tmp);
   delete ctx;
 }
 
@@ -1848,9 +1814,9 @@ int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channe
   auto run = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(ctx->d_rgb, rgb, in_bytes, hipMemcpyHostToDevice, ctx->stream));
     if (ytox) HIP_TRY(hipMemcpyAsync(d_tx, ytox, tiles, hipMemcpyHostToDevice, ctx->stream));
-    else HIP_TRY(hipMemsetAsync(d_tx, 0, tiles, ctx->stream));
+    else if (int r0 = fill_u8(ctx, (uint8_t *)d_tx, tiles, 0)) return r0;
     if (ytob) HIP_TRY(hipMemcpyAsync(d_tb, ytob, tiles, hipMemcpyHostToDevice, ctx->stream));
-    else HIP_TRY(hipMemsetAsync(d_tb, 0, tiles, ctx->stream));
+    else if (int r0 = fill_u8(ctx, (uint8_t *)d_tb, tiles, 0)) return r0;
     int r = jxg_front_end_dev(ctx, ctx->d_rgb, row_bytes, channels, width, height, params, d_tx,
                               d_tb, d_qf, d_cells, d_lv, d_co, d_dc);
     if (r) return r;

@@ -111,6 +111,14 @@ void jxo_strategy_costs(const float *xyb, int width, int height, const uint8_t *
                         const jxg_frame_params *p, float *cost8, float *cost16,
                         float *cost32);
 
+/* ---- whole-plane filters ---- */
+/* Normalised gaborish weights {centre, r, d, R, L, D} for `mul`
+ * (src/gaborish.rs:44-70). */
+void jxo_gaborish_weights(float mul, float w[6]);
+/* gaborish_inverse on three W x H planes (stride W), in place
+ * (src/gaborish.rs:44-77). */
+void jxo_gaborish_inverse(float *xyb, int width, int height, float mul);
+
 /* Whole front end on the CPU (XYB -> AQ -> selection -> quantize) with
  * `threads` worker threads; used as the benchmark's CPU baseline. */
 void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width, int height,

@@ -55,6 +55,8 @@ def _declare(L):
     L.jxo_xyb_pixel.argtypes = [_f32p, c.c_int, c.c_float, c.c_float, c.c_float, _f32p]
     L.jxo_xyb_image.argtypes = [_u8p, c.c_size_t, c.c_int, c.c_int, c.c_int, _f32p, c.c_int, _f32p]
     L.jxo_fill_quant_field_image.argtypes = [_f32p, c.c_int, c.c_int, c.c_float, c.c_float, _u8p]
+    L.jxo_gaborish_weights.argtypes = [c.c_float, _f32p]
+    L.jxo_gaborish_inverse.argtypes = [_f32p, c.c_int, c.c_int, c.c_float]
     L.jxo_forward_transform.restype = c.c_int
     L.jxo_forward_transform.argtypes = [c.c_int, _f32p, c.c_int, c.c_int, c.c_int, c.c_int, _f32p]
     L.jxo_dc_from_coefficients.argtypes = [c.c_int, _f32p, _f32p]
@@ -222,3 +224,17 @@ def distance_params(distance):
 
 def scalar(name, x):
     return getattr(lib(), "jxo_" + name)(x)
+
+
+def gaborish_weights(mul):
+    w = np.zeros(6, np.float32)
+    lib().jxo_gaborish_weights(mul, _p(w, _f32p))
+    return w
+
+
+def gaborish_inverse(xyb, mul=0.9908511):
+    """xyb: (3, H, W) float32; returns the filtered copy."""
+    out = np.ascontiguousarray(xyb, np.float32).copy()
+    _, h, w = out.shape
+    lib().jxo_gaborish_inverse(_p(out, _f32p), w, h, mul)
+    return out

@@ -30,6 +30,7 @@
 #include "jxg_ctx.h"
 #include "jxg_device.h"
 #include "jxg_internal.h"
+#include "jxg_sub8.h"
 
 using namespace jxg;
 
@@ -896,6 +897,7 @@ struct QLds {
   float quot[64];          // 0, 1 - one, numerator / a for a >= 2 (see dequant_level_tab)
   uint8_t line1_t[512], line2_t[512];  // line -> transform, per pass
   uint8_t chunk_t[64];                 // 64-coefficient chunk -> transform
+  float sub8_px[32][64];               // pixel copies for sub-8x8 transforms
 };
 
 __device__ __forceinline__ int find_item(const int *pre, int n, int e) {
@@ -990,7 +992,7 @@ __device__ void dc_from(int s, const float *co, int pitch, int w, int h, float *
 __host__ __device__ __forceinline__ bool quantize_supported(int s) {
   return s == JXG_DCT || s == JXG_DCT16X8 || s == JXG_DCT8X16 || s == JXG_DCT16X16 ||
          s == JXG_DCT32X32 || s == JXG_DCT32X16 || s == JXG_DCT16X32 || s == JXG_DCT32X8 ||
-         s == JXG_DCT8X32;
+         s == JXG_DCT8X32 || is_sub8(s);
 }
 
 struct QArgs {
@@ -1036,7 +1038,7 @@ __device__ __forceinline__ CoefGeom coef_geom(const QLds &L, int t, int k, int x
   if (s == JXG_DCT) {
     u = k >> 3;
     v = k & 7;
-  } else if (rows_first(s)) {
+  } else if (rows_first(s) || is_sub8(s)) {
     v = k >> lw;
     u = k & ((1 << lw) - 1);
   } else {
@@ -1073,10 +1075,19 @@ __device__ __forceinline__ void load_plane(QLds &L, const QArgs &A, int c, int p
   }
 }
 
-// Both 1-D passes of every transform on L.cur (one channel).
+// Every transform of the list on L.cur (one channel): sub-8x8 transforms
+// one block per thread, the DCTs as two passes of 1-D lines.
 template <bool F>
 __device__ void transform_plane(QLds &L, int n) {
   const int tid = threadIdx.x, nthr = blockDim.x;
+  // 32 threads, each owning one pixel-copy slot, take the sub-8x8 blocks.
+  if (tid < 32) {
+    for (int t = tid; t < n; t += 32) {
+      const int s = L.s[t];
+      if (is_sub8(s))
+        sub8_transform<F>(s, &L.cur[L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, L.sub8_px[tid]);
+    }
+  }
   for (int e = tid; e < L.pre1[n]; e += nthr) {
     const int t = L.line1_t[e];
     const int s = L.s[t];
@@ -1122,8 +1133,10 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
       L.pre1[t] = p1;
       L.pre2[t] = p2;
       L.prec[t] = pc;
-      p1 += rows_first(s) ? h : w;
-      p2 += rows_first(s) ? w : h;
+      if (!is_sub8(s)) {  // sub-8x8 transforms have no 1-D lines
+        p1 += rows_first(s) ? h : w;
+        p2 += rows_first(s) ? w : h;
+      }
       pc += cov_x(s) * cov_y(s);
     }
     L.pre1[n] = p1;

@@ -229,6 +229,165 @@ static void dct2x2_8x8(const float *in, float *out) {
   dct2_top_block(snap, 2, out);
 }
 
+/* dct4x4_2d (src/dct.rs:1061-1076): rows, then columns stored transposed
+ * (out[hf*4 + vf]), scaled 1/16. */
+static void dct4x4_2d(const float *in, float *out) {
+  float tmp[16];
+  memcpy(tmp, in, sizeof(tmp));
+  for (int r = 0; r < 4; r++) dct1d_4(tmp + r * 4);
+  for (int x = 0; x < 4; x++) {
+    float col[4] = {tmp[x], tmp[4 + x], tmp[8 + x], tmp[12 + x]};
+    dct1d_4(col);
+    for (int i = 0; i < 4; i++) out[x * 4 + i] = col[i] * (1.0f / 16.0f);
+  }
+}
+
+/* dct4x8_2d (src/dct.rs:1198-1220): a 4-tall x 8-wide half; columns
+ * (4-point) then rows (8-point), out[vf*8 + hf], scaled 1/32. */
+static void dct4x8_2d(const float *in, float *out) {
+  float tmp[32];
+  memcpy(tmp, in, sizeof(tmp));
+  for (int c = 0; c < 8; c++) {
+    float col[4] = {tmp[c], tmp[8 + c], tmp[16 + c], tmp[24 + c]};
+    dct1d_4(col);
+    for (int vf = 0; vf < 4; vf++) tmp[vf * 8 + c] = col[vf];
+  }
+  for (int vf = 0; vf < 4; vf++) {
+    dct1d_8(tmp + vf * 8);
+    for (int i = 0; i < 8; i++) tmp[vf * 8 + i] *= 1.0f / 32.0f;
+  }
+  memcpy(out, tmp, sizeof(tmp));
+}
+
+/* dct8x4_2d (src/dct.rs:1267-1300): an 8-tall x 4-wide half; columns
+ * (8-point) then rows (4-point), stored out[hf*8 + vf], scaled 1/32. */
+static void dct8x4_2d(const float *in, float *out) {
+  float tmp[32];
+  for (int c = 0; c < 4; c++) {
+    float col[8];
+    for (int i = 0; i < 8; i++) col[i] = in[i * 4 + c];
+    dct1d_8(col);
+    for (int vf = 0; vf < 8; vf++) tmp[vf * 4 + c] = col[vf];
+  }
+  for (int vf = 0; vf < 8; vf++) {
+    float row[4] = {tmp[vf * 4], tmp[vf * 4 + 1], tmp[vf * 4 + 2], tmp[vf * 4 + 3]};
+    dct1d_4(row);
+    for (int hf = 0; hf < 4; hf++) out[hf * 8 + vf] = row[hf] * (1.0f / 32.0f);
+  }
+}
+
+/* dct4x4_scalar_input (src/dct.rs:1150-1180): four quadrant DCT4x4s
+ * interleaved (quadrant (qy, qx) at rows qy+2iy, columns qx+2ix), then a
+ * 2x2 Hadamard of the four sub-DCs. */
+static void dct4x4_8x8(const float *in, float *out) {
+  for (int qy = 0; qy < 2; qy++)
+    for (int qx = 0; qx < 2; qx++) {
+      float blk[16], d[16];
+      for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) blk[r * 4 + c] = in[(qy * 4 + r) * 8 + qx * 4 + c];
+      dct4x4_2d(blk, d);
+      for (int iy = 0; iy < 4; iy++)
+        for (int ix = 0; ix < 4; ix++) out[(qy + iy * 2) * 8 + qx + ix * 2] = d[iy * 4 + ix];
+    }
+  float b00 = out[0], b01 = out[1], b10 = out[8], b11 = out[9];
+  out[0] = (b00 + b01 + b10 + b11) * 0.25f;
+  out[1] = (b00 + b01 - b10 - b11) * 0.25f;
+  out[8] = (b00 - b01 + b10 - b11) * 0.25f;
+  out[9] = (b00 - b01 - b10 + b11) * 0.25f;
+}
+
+/* dct4x8_scalar_input (src/dct.rs:1230-1255): the two 4-tall halves
+ * interleaved on even / odd rows, then a 2-point Hadamard of their DCs. */
+static void dct4x8_8x8(const float *in, float *out) {
+  for (int y = 0; y < 2; y++) {
+    float d[32];
+    dct4x8_2d(in + y * 32, d);
+    for (int iy = 0; iy < 4; iy++)
+      for (int ix = 0; ix < 8; ix++) out[(y + iy * 2) * 8 + ix] = d[iy * 8 + ix];
+  }
+  float b0 = out[0], b1 = out[8];
+  out[0] = (b0 + b1) * 0.5f;
+  out[8] = (b0 - b1) * 0.5f;
+}
+
+/* dct8x4_scalar_input (src/dct.rs:1310-1340): the two 4-wide halves. */
+static void dct8x4_8x8(const float *in, float *out) {
+  for (int x = 0; x < 2; x++) {
+    float half[32], d[32];
+    for (int r = 0; r < 8; r++)
+      for (int c = 0; c < 4; c++) half[r * 4 + c] = in[r * 8 + x * 4 + c];
+    dct8x4_2d(half, d);
+    for (int iy = 0; iy < 4; iy++)
+      for (int ix = 0; ix < 8; ix++) out[(x + iy * 2) * 8 + ix] = d[iy * 8 + ix];
+  }
+  float b0 = out[0], b1 = out[8];
+  out[0] = (b0 + b1) * 0.5f;
+  out[8] = (b0 - b1) * 0.5f;
+}
+
+/* The JPEG XL AFV basis (the format's 4x4 "AFV DCT" basis; the same values
+ * as src/afv.rs:48 AFV_BASIS_TRANSPOSE, f32): row j = pixel j, column i =
+ * basis vector i.  Orthonormal; column 0 is the constant 1/4. */
+static const float AFV_BASIS[256] = {
+    0.25f, 0.876902938f, 0.0f, 0.0f, 0.0f, -0.41053775f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f,
+    0.25f, 0.220651805f, 0.0f, 0.0f, -0.707106769f, 0.623548508f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f,
+    0.25f, -0.101400502f, 0.40670076f, -0.21255748f, 0.0f, -0.0643507168f, -0.451755673f, -0.304684758f, 0.301792949f, 0.408248305f, 0.174786702f, -0.211056009f, -0.142660841f, -0.138135403f, -0.174376026f, 0.113549873f,
+    0.25f, -0.101400502f, 0.444448173f, 0.308549702f, 0.0f, -0.0643507168f, 0.158545032f, 0.511261642f, 0.257923633f, 0.0f, 0.0812611207f, 0.185671806f, -0.341644675f, 0.330228269f, 0.0702790692f, -0.0741750449f,
+    0.25f, 0.220651805f, 0.0f, 0.0f, 0.707106769f, 0.623548508f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f,
+    0.25f, -0.101400502f, 0.0f, 0.470670223f, 0.0f, -0.0643507168f, -0.0403851531f, 0.0f, 0.162723407f, 0.0f, 0.0f, 0.0f, 0.736749768f, 0.0875511467f, -0.292102665f, 0.194028929f,
+    0.25f, -0.101400502f, 0.195743993f, -0.162120521f, 0.0f, -0.0643507168f, 0.00741822645f, -0.290480137f, 0.0952002257f, 0.0f, -0.367539793f, 0.492158592f, 0.246271074f, -0.0794670656f, 0.362381727f, -0.435190499f,
+    0.25f, -0.101400502f, 0.29291001f, 0.0f, 0.0f, -0.0643507168f, 0.393510342f, -0.0657870173f, 0.0f, -0.408248305f, -0.30788222f, -0.385250151f, -0.0857401937f, -0.461337477f, 0.0f, 0.219186842f,
+    0.25f, -0.101400502f, -0.40670076f, -0.21255748f, 0.0f, -0.0643507168f, -0.451755673f, 0.304684758f, 0.301792949f, -0.408248305f, -0.174786702f, 0.211056009f, -0.142660841f, -0.138135403f, -0.174376026f, 0.113549873f,
+    0.25f, -0.101400502f, -0.195743993f, -0.162120521f, 0.0f, -0.0643507168f, 0.00741822645f, 0.290480137f, 0.0952002257f, 0.0f, 0.367539793f, -0.492158592f, 0.246271074f, -0.0794670656f, 0.362381727f, -0.435190499f,
+    0.25f, -0.101400502f, 0.0f, -0.470670223f, 0.0f, -0.0643507168f, 0.11074166f, 0.0f, -0.162723407f, 0.0f, 0.0f, 0.0f, 0.14883399f, 0.497246474f, 0.292102665f, 0.555044353f,
+    0.25f, -0.101400502f, 0.113790743f, -0.146429181f, 0.0f, -0.0643507168f, 0.0829816312f, -0.238897741f, -0.353123844f, -0.408248305f, 0.482668906f, 0.174194127f, -0.0476868041f, 0.125380591f, -0.432660788f, -0.254682779f,
+    0.25f, -0.101400502f, -0.444448173f, 0.308549702f, 0.0f, -0.0643507168f, 0.158545032f, -0.511261642f, 0.257923633f, 0.0f, -0.0812611207f, -0.185671806f, -0.341644675f, 0.330228269f, 0.0702790692f, -0.0741750449f,
+    0.25f, -0.101400502f, -0.29291001f, 0.0f, 0.0f, -0.0643507168f, 0.393510342f, 0.0657870173f, 0.0f, 0.408248305f, 0.30788222f, 0.385250151f, -0.0857401937f, -0.461337477f, 0.0f, 0.219186842f,
+    0.25f, -0.101400502f, -0.113790743f, -0.146429181f, 0.0f, -0.0643507168f, 0.0829816312f, 0.238897741f, -0.353123844f, 0.408248305f, -0.482668906f, -0.174194127f, -0.0476868041f, 0.125380591f, -0.432660788f, -0.254682779f,
+    0.25f, -0.101400502f, 0.0f, 0.425114959f, 0.0f, -0.0643507168f, -0.451755673f, 0.0f, -0.603585899f, 0.0f, 0.0f, 0.0f, -0.142660841f, -0.138135403f, 0.348752052f, 0.113549873f,
+};
+
+const float *jxo_afv_basis(void) { return AFV_BASIS; }
+
+/* afv_dct4x4 (src/afv.rs:216-225): coeffs[i] = sum_j pixels[j] * B[j][i],
+ * accumulated in pixel order with fmla. */
+static void afv_dct4x4(const float *px, float *out) {
+  for (int i = 0; i < 16; i++) {
+    float acc = 0.0f;
+    for (int j = 0; j < 16; j++) acc = jxo_fmla(px[j], AFV_BASIS[j * 16 + i], acc);
+    out[i] = acc;
+  }
+}
+
+/* afv_from_pixels (src/afv.rs:207-256).  kind bit 0: AFV quadrant on the
+ * right; bit 1: on the bottom. */
+static void afv_8x8(int kind, const float *in, float *out) {
+  const int afv_x = kind & 1, afv_y = kind >> 1;
+  float quad[16], coeff[16], d44[16], half[32], d48[32];
+  for (int i = 0; i < 16; i++) {
+    const int dy = i / 4, dx = i % 4;
+    const int sy = afv_y ? 3 - dy : dy, sx = afv_x ? 3 - dx : dx;
+    quad[i] = in[(sy + 4 * afv_y) * 8 + sx + 4 * afv_x];
+  }
+  afv_dct4x4(quad, coeff);
+  for (int iy = 0; iy < 4; iy++)
+    for (int ix = 0; ix < 4; ix++) out[iy * 2 * 8 + ix * 2] = coeff[iy * 4 + ix];
+  const int qx = afv_x ? 0 : 4;
+  for (int i = 0; i < 16; i++) quad[i] = in[(i / 4 + 4 * afv_y) * 8 + qx + i % 4];
+  dct4x4_2d(quad, d44);
+  for (int iy = 0; iy < 4; iy++)
+    for (int ix = 0; ix < 4; ix++) out[iy * 2 * 8 + ix * 2 + 1] = d44[iy * 4 + ix];
+  const int hy = afv_y ? 0 : 4;
+  for (int i = 0; i < 32; i++) half[i] = in[(hy + i / 8) * 8 + i % 8];
+  dct4x8_2d(half, d48);
+  for (int iy = 0; iy < 4; iy++)
+    for (int ix = 0; ix < 8; ix++) out[(1 + iy * 2) * 8 + ix] = d48[iy * 8 + ix];
+  const float block00 = out[0] * 0.25f, block01 = out[1], block10 = out[8];
+  out[0] = (block00 + block01 + 2.0f * block10) * 0.25f;
+  out[1] = (block00 - block01) * 0.5f;
+  out[8] = (block00 + block01 - 2.0f * block10) * 0.25f;
+}
+
 /* ---- strategy geometry (src/dc_group_data.rs:140-170) ---- */
 static const uint8_t COV_X[24] = {1, 1, 2, 2, 4, 1, 1, 1, 2, 4, 1, 1,
                                   1, 1, 8, 4, 8, 1, 1, 16, 8, 16, 1, 4};
@@ -274,6 +433,13 @@ int jxo_forward_transform(int strategy, const float *plane, int width, int heigh
     case JXG_DCT8X32: dct_rows_first(pix, 32, 8, out); break;   /* W32 H8 */
     case JXG_IDENTITY: identity8x8(pix, out); break;
     case JXG_DCT2X2: dct2x2_8x8(pix, out); break;
+    case JXG_DCT4X4: dct4x4_8x8(pix, out); break;
+    case JXG_DCT4X8: dct4x8_8x8(pix, out); break;
+    case JXG_DCT8X4: dct8x4_8x8(pix, out); break;
+    case JXG_AFV0:
+    case JXG_AFV1:
+    case JXG_AFV2:
+    case JXG_AFV3: afv_8x8(strategy - JXG_AFV0, pix, out); break;
     default: abort();
   }
   return w * h;

@@ -11,6 +11,7 @@ tables against:
 * DEQUANT_MATRIX_8X8 / DEQUANT_MATRIX_16X8 (src/quant_weights.rs:246,455):
   the stored DCT8 and 16x8 step tables.
 * WC8 / WC16 / WC32 (src/dct.rs): butterfly constants.
+* AFV_BASIS_TRANSPOSE (src/afv.rs:48): the 16x16 AFV basis, row j = pixel j.
 
 Usage: python tests/golden/make_reference_tables.py [/root/reference]
 """
@@ -85,6 +86,11 @@ def main():
         m = re.search(r"const " + name + r": \[f32; \d+\] = ", dct)
         vals = numbers(block_after(dct[m.start():], name))[-n:]
         out[name] = vals
+
+    afv = read("src/afv.rs")
+    vals = numbers(block_after(afv, "static AFV_BASIS_TRANSPOSE"))[-256:]
+    assert len(vals) == 256
+    out["AFV_BASIS_TRANSPOSE"] = [vals[j * 16:(j + 1) * 16] for j in range(16)]
 
     with open(OUT, "w") as f:
         json.dump(out, f, indent=0)

@@ -160,21 +160,28 @@ def test_strategy_selection_bit_exact(J, fe, torch, oracle, w, h, distance, fuse
 
 COV = {0: (1, 1), 1: (1, 2), 2: (2, 1), 3: (2, 2), 4: (4, 4), 8: (2, 4), 9: (4, 2), 22: (1, 4),
        23: (4, 1)}
+SUB8 = [5, 6, 7, 10, 11, 12, 13, 17, 18]  # DCT4X4, 4X8, 8X4, AFV0-3, IDENTITY, DCT2X2
+COV.update({s: (1, 1) for s in SUB8})
 
 
-def random_layout(xb, yb, seed, crossing=False):
+def random_layout(xb, yb, seed, crossing=False, sub8=False):
     """A valid strategy image: multi-block transforms inside the image and
-    their 256-px group, and inside their 64-px tile unless `crossing`."""
+    their 256-px group, and inside their 64-px tile unless `crossing`;
+    `sub8` mixes in the single-block sub-8x8 transforms."""
     rng = np.random.default_rng(seed)
     cells = np.zeros((yb, xb), np.uint8)
     taken = np.zeros((yb, xb), bool)
     kinds = [0, 1, 2, 3, 4, 8, 9, 22, 23]
+    prob = [0.3, 0.1, 0.1, 0.15, 0.1, 0.05, 0.05, 0.075, 0.075]
+    if sub8:
+        kinds = kinds + SUB8
+        prob = [0.5 * x for x in prob] + [0.5 / len(SUB8)] * len(SUB8)
     for by in range(yb):
         for bx in range(xb):
             if taken[by, bx]:
                 continue
             for _ in range(4):
-                s = int(rng.choice(kinds, p=[0.3, 0.1, 0.1, 0.15, 0.1, 0.05, 0.05, 0.075, 0.075]))
+                s = int(rng.choice(kinds, p=prob))
                 cx, cy = COV[s]
                 if bx + cx > xb or by + cy > yb:
                     continue
@@ -211,6 +218,31 @@ def test_forward_quantize_bit_exact(J, fe, torch, oracle, w, h, distance, crossi
     assert bad == 0
     if not crossing:
         assert n_cross == 0
+    for c in range(3):
+        for name, g, wnt in [("levels", o["levels"][c], lv_w[c]), ("coeffs", o["coeffs"][c], co_w[c]),
+                             ("dc", o["dc"][c], dc_w[c])]:
+            g = g.cpu().numpy()
+            assert same_bits(g, wnt), (name, c, first_mismatch(g, wnt))
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("kind", SUB8 + ["mixed"])
+def test_forward_quantize_sub8_bit_exact(J, fe, torch, oracle, kind, fused):
+    """Every block one sub-8x8 transform (or a random mix with the DCTs)."""
+    w, h = (200, 136) if kind == "mixed" else (72, 40)
+    log("quantize sub8", kind, w, h, fused)
+    rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, 1.4, 21, fused, cmap=True)
+    xb, yb = (w + 7) // 8, (h + 7) // 8
+    if kind == "mixed":
+        cells = random_layout(xb, yb, seed=77, crossing=True, sub8=True)
+    else:
+        cells = np.full((yb, xb), (kind << 1) | 1, np.uint8)
+    tables = J.default_quant_tables()
+    lv_w, co_w, dc_w = oracle.forward_quantize(xyb, cells, qf, ytox, ytob, tables, p)
+    oracle.set_fused(True)
+    o = fe.forward_quantize(to_dev(torch, padded(J, xyb)), w, h, to_dev(torch, cells),
+                            to_dev(torch, qf), p, to_dev(torch, ytox), to_dev(torch, ytob))
+    assert fe.quantize_status()[1] == 0
     for c in range(3):
         for name, g, wnt in [("levels", o["levels"][c], lv_w[c]), ("coeffs", o["coeffs"][c], co_w[c]),
                              ("dc", o["dc"][c], dc_w[c])]:

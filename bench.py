@@ -1,7 +1,7 @@
 """VarDCT front-end throughput on one MI355X.
 
 A step is one pass of the front end (sRGB u8 -> XYB -> adaptive quant field
--> AC-strategy search -> forward DCT + quantization) over one 2048x2048 RGB8
+-> CfL maps -> AC-strategy search -> forward DCT + quantization) over one 2048x2048 RGB8
 synthetic photo-like image at quality 75, with the input already resident in
 HBM.  Prints one JSON line: whole-job megapixels per second, the dominant
 kernel's roofline position, and the CPU oracle's throughput on the host cores
@@ -27,6 +27,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 STAGE_BYTES_PER_PX = {
     "xyb": 3.0 + 12.0,          # u8 RGB in, 3 f32 planes out
     "aq": 12.0 + 1.0 / 64,      # 3 planes in (Y twice from cache), u8 field out
+    "cfl": 12.0,                # 3 planes in, two i8 per 64x64 tile out
     "select": 12.0 + 2.0 / 64,  # 3 planes in, cells + field out
     "quantize": 12.0 + 24.0 + 6.0 / 64 + 2.0 / 64,  # planes in, i32+f32 out, dc/qf/cells
 }
@@ -39,7 +40,7 @@ EXTRA_BYTES_PER_PX = {
 CAL_BYTES = 256 << 20  # calibration copy size for the byte counters
 
 # Kernel of each stage, as rocprofv3 names it.
-STAGE_KERNEL = {"xyb": "k_xyb", "aq": "k_aq_block", "select": "k_select",
+STAGE_KERNEL = {"xyb": "k_xyb", "aq": "k_aq_block", "cfl": "k_cmap", "select": "k_select",
                 "quantize": "k_quantize"}
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_photo_2048x2048_q75.txt")
@@ -79,13 +80,12 @@ def cpu_baseline(rgb, params, threads):
     from oracle import oracle as O
     h, w, _ = rgb.shape
     tables = J.default_quant_tables()
-    z = np.zeros(((h + 63) // 64, (w + 63) // 64), np.int8)
-    O.front_end(rgb[:64, :64].copy(), params, tables, z[:1, :1], z[:1, :1], threads)  # warm
+    O.front_end(rgb[:64, :64].copy(), params, tables, threads)  # warm
     runs = []
     t_end = time.perf_counter() + 12.0
     while len(runs) < 5 and (not runs or time.perf_counter() < t_end):
         t0 = time.perf_counter()
-        O.front_end(rgb, params, tables, z, z, threads)
+        O.front_end(rgb, params, tables, threads)
         runs.append(time.perf_counter() - t0)
     sec = float(np.median(runs))
     return {
@@ -179,7 +179,7 @@ def main():
     ytob = torch.zeros(tiles, dtype=torch.int8, device=dev)
 
     def step():
-        fe.run(d_rgb, w, h, p, ytox, ytob, outputs=outs, qf=qf, cells=cells)
+        fe.run(d_rgb, w, h, p, outputs=outs, qf=qf, cells=cells, ytox=ytox, ytob=ytob)
 
     if args.profile:
         # Calibration for the HBM byte counters: one plain copy kernel that
@@ -211,17 +211,19 @@ def main():
     qf2 = fe.empty((yb, xb), torch.uint8)
     stage_ms = {k: 0.0 for k in STAGE_BYTES_PER_PX}
     reps = max(3, min(args.steps, 10))
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
     for _ in range(reps):
         ev[0].record(stream)
         fe.xyb(d_rgb, w, h, fused=bool(p.fused), out=xyb)
         ev[1].record(stream)
         fe.fill_quant_field(xyb, w, h, p.distance, 1.0 / p.scale, bool(p.fused), out=qf2)
         ev[2].record(stream)
-        fe.select_strategy(xyb, w, h, qf2, p, ytox, ytob, cells=cells)
+        fe.fill_cmap(xyb, w, h, p, ytox, ytob)
         ev[3].record(stream)
-        fe.forward_quantize(xyb, w, h, cells, qf2, p, ytox, ytob, outputs=outs)
+        fe.select_strategy(xyb, w, h, qf2, p, ytox, ytob, cells=cells)
         ev[4].record(stream)
+        fe.forward_quantize(xyb, w, h, cells, qf2, p, ytox, ytob, outputs=outs)
+        ev[5].record(stream)
         torch.cuda.synchronize()
         for i, k in enumerate(stage_ms):
             stage_ms[k] += ev[i].elapsed_time(ev[i + 1]) / reps
@@ -261,6 +263,7 @@ def main():
             "width": w, "height": h, "quality": args.quality,
             "distance": round(p.distance, 4), "speed": "fast",
             "strategy_search": "square merges DCT8/DCT16X16/DCT32X32 + quant-field aggregation",
+            "cfl": "per-tile regression maps (fill_cmap, Fast path)",
             "fmla": "fused" if p.fused else "unfused",
             "input": "u8 RGB resident in HBM; outputs (levels, f32 coefficients, DC, field, "
                      "strategy) left in HBM",

@@ -263,22 +263,39 @@ int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
                              const jxg_frame_params *params, int32_t *d_levels[3],
                              float *d_coeffs[3], int16_t *d_dc[3]);
 
-/* The whole front end on device-resident input: XYB -> AQ -> strategy ->
- * forward DCT + quantize.  CfL maps are inputs (all zero = the maps of an
- * uncorrelated frame).  Scratch (XYB planes) is owned by the context. */
+/* Chroma-from-luma slope maps: per 64x64 tile the Y-to-X and Y-to-B
+ * multipliers (i8, tiles_x = ceil(W/64) per row) by the closed-form
+ * regression over the tile's DCT8 coefficients.  Replaces fill_cmap
+ * (src/color_correlation.rs:901) on its regression path (compute_cmap_tile
+ * :441), the path XYB frames take at Fast and faster, as called by
+ * setup_dc_group (src/frame/dc_groups.rs:171). */
+int jxg_fill_cmap_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                      uint32_t width, uint32_t height, const jxg_frame_params *params,
+                      int8_t *d_ytox, int8_t *d_ytob);
+
+/* The whole front end on device-resident input: XYB -> AQ -> CfL maps ->
+ * strategy -> forward DCT + quantize.  d_ytox / d_ytob receive the CfL maps
+ * (NULL: kept in context scratch).  XYB planes live in context scratch. */
 int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channels,
                       uint32_t width, uint32_t height, const jxg_frame_params *params,
-                      const int8_t *d_ytox, const int8_t *d_ytob, uint8_t *d_qf,
-                      uint8_t *d_cells, int32_t *d_levels[3], float *d_coeffs[3],
-                      int16_t *d_dc[3]);
+                      int8_t *d_ytox, int8_t *d_ytob, uint8_t *d_qf, uint8_t *d_cells,
+                      int32_t *d_levels[3], float *d_coeffs[3], int16_t *d_dc[3]);
 
 /* Host-buffer convenience form of jxg_front_end_dev for callers that own
  * host memory (the reference's encode_image input is a host slice,
- * src/encode_image.rs:1610): uploads, runs, downloads, synchronizes. */
+ * src/encode_image.rs:1610): uploads, runs, downloads, synchronizes.
+ * ytox / ytob may be NULL when the maps are not wanted. */
 int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channels,
                   uint32_t width, uint32_t height, const jxg_frame_params *params,
-                  const int8_t *ytox, const int8_t *ytob, uint8_t *qf, uint8_t *cells,
+                  int8_t *ytox, int8_t *ytob, uint8_t *qf, uint8_t *cells,
                   int32_t *levels[3], float *coeffs[3], int16_t *dc[3]);
+
+/* Status of the last jxg_forward_quantize_dev on this context (synchronizes):
+ * out[0] = transforms whose footprint crossed their 64x64 tile (handled by
+ * the crossing pass), out[1] = 1 if a strategy cell was unsupported or did
+ * not fit the image (those transforms were not written).  The reference
+ * asserts on such cells (src/dc_group_data.rs:219-226); this ABI reports. */
+int jxg_quantize_status(jxg_ctx *ctx, uint32_t out[2]);
 
 /* Gaborish pre-inversion of the three XYB planes: a normalised symmetric
  * 5x5 sharpening with mirror-reflected image edges, so that the decoder's
@@ -292,13 +309,6 @@ int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channe
 int jxg_gaborish_inverse_dev(jxg_ctx *ctx, const float *d_src, float *d_dst, uint32_t stride,
                              uint32_t padded_h, uint32_t width, uint32_t height, float mul,
                              int fused);
-
-/* Status of the last jxg_forward_quantize_dev on this context (synchronizes):
- * out[0] = transforms whose footprint crossed their 64x64 tile (handled by
- * the crossing pass), out[1] = 1 if a strategy cell was unsupported or did
- * not fit the image (those transforms were not written).  The reference
- * asserts on such cells (src/dc_group_data.rs:219-226); this ABI reports. */
-int jxg_quantize_status(jxg_ctx *ctx, uint32_t out[2]);
 
 #ifdef __cplusplus
 }

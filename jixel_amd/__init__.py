@@ -141,6 +141,8 @@ class FrontEndOutput:
     """Per-image front-end results in the layouts of include/jxg.h."""
     qf: np.ndarray      # (yblocks, xblocks) uint8, final quant field
     cells: np.ndarray   # (yblocks, xblocks) uint8 strategy cells
+    ytox: np.ndarray    # (ytiles, xtiles) int8 CfL map, Y to X
+    ytob: np.ndarray    # (ytiles, xtiles) int8 CfL map, Y to B
     levels: list        # 3 x int32 [yblocks*xblocks*64]
     coeffs: list        # 3 x float32 [yblocks*xblocks*64]
     dc: list            # 3 x int16 [yblocks*xblocks]
@@ -274,15 +276,29 @@ class FrontEnd:
         check(lib().jxg_quantize_status(self.ctx, out))
         return int(out[0]), int(out[1])
 
-    def run(self, rgb, width, height, params, ytox=None, ytob=None, outputs=None, qf=None,
-            cells=None):
-        """The whole front end on a device-resident (H, W, C) uint8 image."""
+    def fill_cmap(self, xyb, width, height, params, ytox=None, ytob=None):
+        """CfL maps (ytox, ytob) of (3, padded_h, stride) planes."""
+        t = self.torch
+        n = ((height + 63) // 64, (width + 63) // 64)
+        ytox = self.empty(n, t.int8) if ytox is None else ytox
+        ytob = self.empty(n, t.int8) if ytob is None else ytob
+        check(lib().jxg_fill_cmap_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
+                                      height, ctypes.byref(params), _ptr(ytox), _ptr(ytob)))
+        return ytox, ytob
+
+    def run(self, rgb, width, height, params, outputs=None, qf=None, cells=None, ytox=None,
+            ytob=None):
+        """The whole front end on a device-resident (H, W, C) uint8 image.
+        Returns levels / coeffs / dc plus the quant field, strategy cells and
+        CfL maps; pass preallocated tensors to reuse them."""
         t = self.torch
         xb, yb = (width + 7) // 8, (height + 7) // 8
-        ytox, ytob = self._cmaps(width, height, ytox, ytob)
+        n = ((height + 63) // 64, (width + 63) // 64)
         o = self.outputs(width, height) if outputs is None else outputs
         qf = self.empty((yb, xb), t.uint8) if qf is None else qf
         cells = self.empty((yb, xb), t.uint8) if cells is None else cells
+        ytox = self.empty(n, t.int8) if ytox is None else ytox
+        ytob = self.empty(n, t.int8) if ytob is None else ytob
         arr = lambda ts: (ctypes.c_void_p * 3)(*[x.data_ptr() for x in ts])
         check(lib().jxg_front_end_dev(self.ctx, _ptr(rgb), rgb.stride(0) * rgb.element_size(),
                                       rgb.shape[-1], width, height, ctypes.byref(params),
@@ -291,11 +307,13 @@ class FrontEnd:
         o = dict(o)
         o["qf"] = qf
         o["cells"] = cells
+        o["ytox"] = ytox
+        o["ytob"] = ytob
         return o
 
 
-def encode_front_end(rgb, config: EncodeConfig = EncodeConfig(), ytox=None, ytob=None,
-                     device=0, tables=None) -> FrontEndOutput:
+def encode_front_end(rgb, config: EncodeConfig = EncodeConfig(), device=0,
+                     tables=None) -> FrontEndOutput:
     """Host-buffer front end of one RGB(A)8 image (H, W, 3|4) on `device`.
 
     The drop-in counterpart of the front-end half of jixel's encode_image /
@@ -313,19 +331,19 @@ def encode_front_end(rgb, config: EncodeConfig = EncodeConfig(), ytox=None, ytob
     try:
         p = frame_params(config)
         xb, yb = (w + 7) // 8, (h + 7) // 8
-        tiles = ((h + 63) // 64) * ((w + 63) // 64)
+        tiles = ((h + 63) // 64, (w + 63) // 64)
         qf = np.zeros((yb, xb), np.uint8)
         cells = np.zeros((yb, xb), np.uint8)
+        ytox = np.zeros(tiles, np.int8)
+        ytob = np.zeros(tiles, np.int8)
         levels = [np.zeros(yb * xb * 64, np.int32) for _ in range(3)]
         coeffs = [np.zeros(yb * xb * 64, np.float32) for _ in range(3)]
         dc = [np.zeros(yb * xb, np.int16) for _ in range(3)]
-        tx = None if ytox is None else np.ascontiguousarray(ytox, np.int8).reshape(tiles)
-        tb = None if ytob is None else np.ascontiguousarray(ytob, np.int8).reshape(tiles)
         arr = lambda xs: (ctypes.c_void_p * 3)(*[x.ctypes.data for x in xs])
         check(lib().jxg_front_end(fe.ctx, rgb.ctypes.data, w * ch, ch, w, h, ctypes.byref(p),
-                                  None if tx is None else tx.ctypes.data,
-                                  None if tb is None else tb.ctypes.data, qf.ctypes.data,
+                                  ytox.ctypes.data, ytob.ctypes.data, qf.ctypes.data,
                                   cells.ctypes.data, arr(levels), arr(coeffs), arr(dc)))
-        return FrontEndOutput(qf=qf, cells=cells, levels=levels, coeffs=coeffs, dc=dc)
+        return FrontEndOutput(qf=qf, cells=cells, ytox=ytox, ytob=ytob, levels=levels,
+                              coeffs=coeffs, dc=dc)
     finally:
         fe.close()

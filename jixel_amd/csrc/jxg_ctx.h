@@ -57,6 +57,8 @@ struct jxg_ctx {
   size_t rgb_cap = 0;
   float *d_tmp = nullptr;  // out-of-place staging of in-place filters
   size_t tmp_cap = 0;
+  int8_t *d_cmap = nullptr;  // CfL maps when the caller does not want them
+  size_t cmap_cap = 0;
 };
 
 template <typename T>
@@ -74,6 +76,9 @@ inline int ensure(T **p, size_t *cap, size_t n) {
 JXG_HIDDEN int check_image(uint32_t w, uint32_t h);
 JXG_HIDDEN int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t padded_h);
 JXG_HIDDEN int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p);
+JXG_HIDDEN int fill_cmap(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                         uint32_t width, uint32_t height, const jxg_frame_params *p,
+                         int8_t *d_ytox, int8_t *d_ytob);  // jxg_cfl.hip
 JXG_HIDDEN int fill_u8(jxg_ctx *ctx, uint8_t *p, size_t n, uint8_t v);
 JXG_HIDDEN int zero_cmaps(jxg_ctx *ctx, uint32_t w, uint32_t h, const int8_t **x,
                           const int8_t **b);

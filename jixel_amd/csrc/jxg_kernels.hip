@@ -1556,6 +1556,7 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_zero_cmap);
   (void)hipFree(ctx->d_rgb);
   (void)hipFree(ctx->d_tmp);
+  (void)hipFree(ctx->d_cmap);
   delete ctx;
 }
 
@@ -1749,15 +1750,20 @@ int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, 
 
 int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channels,
                       uint32_t width, uint32_t height, const jxg_frame_params *params,
-                      const int8_t *d_ytox, const int8_t *d_ytob, uint8_t *d_qf,
-                      uint8_t *d_cells, int32_t *d_levels[3], float *d_coeffs[3],
-                      int16_t *d_dc[3]) {
+                      int8_t *d_ytox, int8_t *d_ytob, uint8_t *d_qf, uint8_t *d_cells,
+                      int32_t *d_levels[3], float *d_coeffs[3], int16_t *d_dc[3]) {
   if (!ctx || !d_rgb || !params || !d_qf || !d_cells) return set_err(JXG_ERR_INVALID_ARG, "null argument");
   int rc = check_image(width, height);
   if (rc) return rc;
   uint32_t stride, ph;
   jxg_plane_geometry(width, height, &stride, &ph);
   if ((rc = ensure(&ctx->d_xyb, &ctx->xyb_cap, (size_t)3 * stride * ph))) return rc;
+  const size_t tiles = (size_t)((width + 63) / 64) * ((height + 63) / 64);
+  if (!d_ytox || !d_ytob) {
+    if ((rc = ensure(&ctx->d_cmap, &ctx->cmap_cap, 2 * tiles))) return rc;
+    if (!d_ytox) d_ytox = ctx->d_cmap;
+    if (!d_ytob) d_ytob = ctx->d_cmap + tiles;
+  }
   // XybMatrix::SPEC (src/xyb.rs:35-47,81-96)
   const float M00 = 0.30f, M02 = 0.078f, M01 = 1.0f - M02 - M00;
   const float M10 = 0.23f, M12 = 0.078f, M11 = 1.0f - M12 - M10;
@@ -1776,6 +1782,10 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
   if ((rc = fill_qf(ctx, ctx->d_xyb, stride, ph, width, height, params->distance,
                     ctx->h_derived->inv_scale, params->fused, d_qf)))
     return rc;
+  // CfL slopes before the strategy search, as setup_dc_group does
+  // (src/frame/dc_groups.rs:166-184).
+  if ((rc = fill_cmap(ctx, ctx->d_xyb, stride, ph, width, height, params, d_ytox, d_ytob)))
+    return rc;
   if ((rc = jxg_select_strategy_dev(ctx, ctx->d_xyb, stride, ph, width, height, d_qf, d_ytox, d_ytob,
                                     params, d_cells)))
     return rc;
@@ -1784,9 +1794,9 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
 }
 
 int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channels, uint32_t width,
-                  uint32_t height, const jxg_frame_params *params, const int8_t *ytox,
-                  const int8_t *ytob, uint8_t *qf, uint8_t *cells, int32_t *levels[3],
-                  float *coeffs[3], int16_t *dc[3]) {
+                  uint32_t height, const jxg_frame_params *params, int8_t *ytox, int8_t *ytob,
+                  uint8_t *qf, uint8_t *cells, int32_t *levels[3], float *coeffs[3],
+                  int16_t *dc[3]) {
   if (!ctx || !rgb || !params || !qf || !cells || !levels || !coeffs || !dc)
     return set_err(JXG_ERR_INVALID_ARG, "null argument");
   int rc = check_image(width, height);
@@ -1826,10 +1836,6 @@ int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channe
   int8_t *d_tb = (int8_t *)p;
   auto run = [&]() -> int {
     HIP_TRY(hipMemcpyAsync(ctx->d_rgb, rgb, in_bytes, hipMemcpyHostToDevice, ctx->stream));
-    if (ytox) HIP_TRY(hipMemcpyAsync(d_tx, ytox, tiles, hipMemcpyHostToDevice, ctx->stream));
-    else if (int r0 = fill_u8(ctx, (uint8_t *)d_tx, tiles, 0)) return r0;
-    if (ytob) HIP_TRY(hipMemcpyAsync(d_tb, ytob, tiles, hipMemcpyHostToDevice, ctx->stream));
-    else if (int r0 = fill_u8(ctx, (uint8_t *)d_tb, tiles, 0)) return r0;
     int r = jxg_front_end_dev(ctx, ctx->d_rgb, row_bytes, channels, width, height, params, d_tx,
                               d_tb, d_qf, d_cells, d_lv, d_co, d_dc);
     if (r) return r;
@@ -1840,6 +1846,8 @@ int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channe
     }
     HIP_TRY(hipMemcpyAsync(qf, d_qf, nb, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(cells, d_cells, nb, hipMemcpyDeviceToHost, ctx->stream));
+    if (ytox) HIP_TRY(hipMemcpyAsync(ytox, d_tx, tiles, hipMemcpyDeviceToHost, ctx->stream));
+    if (ytob) HIP_TRY(hipMemcpyAsync(ytob, d_tb, tiles, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return JXG_OK;
   };

@@ -113,6 +113,16 @@ void jxo_strategy_costs(const float *xyb, int width, int height, const uint8_t *
                         const jxg_frame_params *p, float *cost8, float *cost16,
                         float *cost32);
 
+/* ---- chroma from luma ---- */
+/* One tile's (ytox, ytob) by the closed-form regression
+ * (compute_cmap_tile, src/color_correlation.rs:441). */
+void jxo_cmap_tile(const float *xyb, int width, int height, int tx, int ty, const float *tables,
+                   const jxg_frame_params *p, int *ytox, int *ytob);
+/* Both maps for the whole image, tiles_x = ceil(W/64) per row
+ * (fill_cmap, src/color_correlation.rs:901, regression path). */
+void jxo_fill_cmap(const float *xyb, int width, int height, const float *tables,
+                   const jxg_frame_params *p, int8_t *ytox, int8_t *ytob);
+
 /* ---- whole-plane filters ---- */
 /* Normalised gaborish weights {centre, r, d, R, L, D} for `mul`
  * (src/gaborish.rs:44-70). */
@@ -121,12 +131,13 @@ void jxo_gaborish_weights(float mul, float w[6]);
  * (src/gaborish.rs:44-77). */
 void jxo_gaborish_inverse(float *xyb, int width, int height, float mul);
 
-/* Whole front end on the CPU (XYB -> AQ -> selection -> quantize) with
- * `threads` worker threads; used as the benchmark's CPU baseline. */
+/* Whole front end on the CPU (XYB -> AQ -> CfL maps -> selection ->
+ * quantize) with `threads` worker threads; used as the benchmark's CPU
+ * baseline.  ytox / ytob receive the CfL maps (ceil(H/64) x ceil(W/64)). */
 void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width, int height,
-                   const jxg_frame_params *p, const float *tables, const int8_t *ytox,
-                   const int8_t *ytob, int threads, uint8_t *qf, uint8_t *cells,
-                   int32_t *levels[3], float *coeffs[3], int16_t *dc[3]);
+                   const jxg_frame_params *p, const float *tables, int8_t *ytox, int8_t *ytob,
+                   int threads, uint8_t *qf, uint8_t *cells, int32_t *levels[3],
+                   float *coeffs[3], int16_t *dc[3]);
 
 #ifdef __cplusplus
 }

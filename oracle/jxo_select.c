@@ -639,13 +639,13 @@ void jxo_strategy_costs(const float *xyb, int width, int height, const uint8_t *
 /* group, then transform + quantize.                                      */
 
 typedef struct {
-  int kind; /* 0 xyb rows, 1 aq group, 2 select group, 3 quantize rows */
+  int kind; /* 0 xyb rows, 1 aq group, 2 select group, 3 quantize rows, 4 cmap tile row */
   const uint8_t *rgb;
   size_t row_bytes;
   int channels, width, height;
   const jxg_frame_params *p;
   const float *tables;
-  const int8_t *ytox, *ytob;
+  int8_t *ytox, *ytob;
   float *xyb;
   uint8_t *qf, *qf_in, *cells;
   int32_t **levels;
@@ -716,6 +716,14 @@ static void *fe_worker(void *arg) {
       group_init(&g, j->xyb, W, H, gx, gy, j->qf_in, j->ytox, j->ytob, j->tables, j->p, j->cells);
       select_group(&g);
       adjust_quant_field(&g, j->qf);
+    } else if (j->kind == 4) {
+      const int tiles_x = (W + 63) / 64;
+      for (int tx = 0; tx < tiles_x; tx++) {
+        int x, b;
+        jxo_cmap_tile(j->xyb, W, H, tx, i, j->tables, j->p, &x, &b);
+        j->ytox[i * tiles_x + tx] = (int8_t)x;
+        j->ytob[i * tiles_x + tx] = (int8_t)b;
+      }
     } else if (j->kind == 3) {
       int yb = (H + 7) / 8;
       int by1 = (i + 1) * 8 < yb ? (i + 1) * 8 : yb;
@@ -737,8 +745,8 @@ static void run_pool(fe_job *j, int kind, int total, int threads) {
 }
 
 void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width, int height,
-                   const jxg_frame_params *p, const float *tables, const int8_t *ytox,
-                   const int8_t *ytob, int threads, uint8_t *qf, uint8_t *cells,
+                   const jxg_frame_params *p, const float *tables, int8_t *ytox, int8_t *ytob,
+                   int threads, uint8_t *qf, uint8_t *cells,
                    int32_t *levels[3], float *coeffs[3], int16_t *dc[3]) {
   jxo_set_fused(p->fused);
   fe_job j;
@@ -762,6 +770,7 @@ void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width
   int groups = ((width + 2047) / 2048) * ((height + 2047) / 2048);
   run_pool(&j, 0, (height + 7) / 8, threads);
   run_pool(&j, 1, groups, threads);
+  run_pool(&j, 4, (height + 63) / 64, threads);
   if (selection_runs(p)) {
     j.qf_in = (uint8_t *)malloc((size_t)xb * yb);
     memcpy(j.qf_in, qf, (size_t)xb * yb);

@@ -75,6 +75,9 @@ def _declare(L):
                                       _u8p]
     L.jxo_strategy_costs.argtypes = [_f32p, c.c_int, c.c_int, _u8p, _i8p, _i8p, _f32p, c.c_void_p,
                                      _f32p, _f32p, _f32p]
+    L.jxo_fill_cmap.argtypes = [_f32p, c.c_int, c.c_int, _f32p, c.c_void_p, _i8p, _i8p]
+    L.jxo_cmap_tile.argtypes = [_f32p, c.c_int, c.c_int, c.c_int, c.c_int, _f32p, c.c_void_p,
+                                c.POINTER(c.c_int), c.POINTER(c.c_int)]
     L.jxo_front_end.argtypes = [_u8p, c.c_size_t, c.c_int, c.c_int, c.c_int, c.c_void_p, _f32p,
                                 _i8p, _i8p, c.c_int, _u8p, _u8p, c.POINTER(_i32p),
                                 c.POINTER(_f32p), c.POINTER(_i16p)]
@@ -193,20 +196,45 @@ def strategy_cost(xyb, strategy, px, py, qac, cmap, tables, params, meta_r=4.0):
                                    ctypes.byref(params))
 
 
-def front_end(rgb, params, tables, ytox, ytob, threads=1):
-    """Whole front end on the CPU.  rgb: (H, W, C) uint8."""
+def front_end(rgb, params, tables, threads=1):
+    """Whole front end on the CPU.  rgb: (H, W, C) uint8.  Returns the quant
+    field, strategy cells, CfL maps, levels, coefficients and DC."""
     rgb = np.ascontiguousarray(rgb)
     h, w, ch = rgb.shape
     xb, yb = (w + 7) // 8, (h + 7) // 8
     qf = np.zeros((yb, xb), np.uint8)
     cells = np.zeros((yb, xb), np.uint8)
+    tiles = ((h + 63) // 64, (w + 63) // 64)
+    ytox = np.zeros(tiles, np.int8)
+    ytob = np.zeros(tiles, np.int8)
     levels, coeffs, dc, L, C, D = _outputs(xb, yb)
     lib().jxo_front_end(_p(rgb, _u8p), w * ch, ch, w, h, ctypes.byref(params),
                         _p(np.ascontiguousarray(tables, np.float32), _f32p),
-                        _p(np.ascontiguousarray(ytox, np.int8), _i8p),
-                        _p(np.ascontiguousarray(ytob, np.int8), _i8p), threads,
+                        _p(ytox, _i8p), _p(ytob, _i8p), threads,
                         _p(qf, _u8p), _p(cells, _u8p), L, C, D)
-    return dict(qf=qf, cells=cells, levels=levels, coeffs=coeffs, dc=dc)
+    return dict(qf=qf, cells=cells, ytox=ytox, ytob=ytob, levels=levels, coeffs=coeffs, dc=dc)
+
+
+def fill_cmap(xyb, tables, params):
+    """CfL maps of (3, H, W) XYB planes: (ytox, ytob), each ceil(H/64) x ceil(W/64)."""
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    tiles = ((h + 63) // 64, (w + 63) // 64)
+    ytox = np.zeros(tiles, np.int8)
+    ytob = np.zeros(tiles, np.int8)
+    lib().jxo_fill_cmap(_p(xyb, _f32p), w, h, _p(np.ascontiguousarray(tables, np.float32), _f32p),
+                        ctypes.byref(params), _p(ytox, _i8p), _p(ytob, _i8p))
+    return ytox, ytob
+
+
+def cmap_tile(xyb, tables, params, tx, ty):
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    x, b = ctypes.c_int(), ctypes.c_int()
+    lib().jxo_cmap_tile(_p(xyb, _f32p), w, h, tx, ty,
+                        _p(np.ascontiguousarray(tables, np.float32), _f32p), ctypes.byref(params),
+                        ctypes.byref(x), ctypes.byref(b))
+    return x.value, b.value
 
 
 def distance_params(distance):

@@ -269,11 +269,11 @@ def test_front_end_bit_exact(J, fe, torch, oracle, name, w, h, q, fused):
     cfg = J.EncodeConfig(fused=fused).with_quality(q)
     p = J.frame_params(cfg)
     tables = J.default_quant_tables()
-    tiles = ((h + 63) // 64, (w + 63) // 64)
-    z = np.zeros(tiles, np.int8)
-    want = oracle.front_end(rgb, p, tables, z, z, threads=4)
+    want = oracle.front_end(rgb, p, tables, threads=4)
     o = fe.run(to_dev(torch, rgb), w, h, p)
     assert same_bits(o["qf"].cpu().numpy(), want["qf"])
+    assert same_bits(o["ytox"].cpu().numpy(), want["ytox"])
+    assert same_bits(o["ytob"].cpu().numpy(), want["ytob"])
     assert same_bits(o["cells"].cpu().numpy(), want["cells"])
     for c in range(3):
         assert same_bits(o["levels"][c].cpu().numpy(), want["levels"][c]), c
@@ -287,11 +287,25 @@ def test_host_front_end_matches_device_path(J, oracle):
     cfg = J.EncodeConfig().with_quality(75)
     out = J.encode_front_end(rgb, cfg)
     p = J.frame_params(cfg)
-    z = np.zeros((2, 3), np.int8)
-    want = oracle.front_end(rgb, p, J.default_quant_tables(), z, z, threads=2)
+    want = oracle.front_end(rgb, p, J.default_quant_tables(), threads=2)
     assert same_bits(out.qf, want["qf"]) and same_bits(out.cells, want["cells"])
+    assert same_bits(out.ytox, want["ytox"]) and same_bits(out.ytob, want["ytob"])
     for c in range(3):
         assert same_bits(out.levels[c], want["levels"][c])
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("w,h,distance", [(64, 64, 2.35), (200, 136, 0.8), (130, 75, 1.5),
+                                          (1000, 520, 2.35), (13, 70, 2.35)])
+def test_cmap_bit_exact(J, fe, torch, oracle, w, h, distance, fused):
+    log("cmap", w, h, distance, fused)
+    rgb, xyb, p, qf, _, _ = _stage_inputs(J, oracle, w, h, distance, 31, fused)
+    want_x, want_b = oracle.fill_cmap(xyb, J.default_quant_tables(), p)
+    oracle.set_fused(True)
+    gx, gb = fe.fill_cmap(to_dev(torch, padded(J, xyb)), w, h, p)
+    assert same_bits(gx.cpu().numpy(), want_x), first_mismatch(gx.cpu().numpy(), want_x)
+    assert same_bits(gb.cpu().numpy(), want_b), first_mismatch(gb.cpu().numpy(), want_b)
+    assert np.any(want_x != 0) or np.any(want_b != 0) or w < 64 or h < 16
 
 
 def test_error_paths(J, fe, torch):

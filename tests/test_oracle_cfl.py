@@ -1,0 +1,124 @@
+"""Oracle pins for the chroma-from-luma slope maps (fill_cmap regression
+path, src/color_correlation.rs:333-508).
+
+An independent FP64 restatement of the regression (matrix DCT8, the same
+inverse quant weights, the closed-form slope -cb / (ca + num*5e-4)) must give
+the same integer slopes up to a rounding boundary, and channels built as
+exact multiples of luma must come out at the slope the format defines
+(84 steps per unit)."""
+import math
+
+import numpy as np
+import pytest
+
+
+def C8():
+    k = np.arange(8)[:, None]
+    i = np.arange(8)[None, :]
+    m = np.cos(np.pi * k * (2 * i + 1) / 16)
+    m[1:] *= math.sqrt(2.0)
+    return m
+
+
+def dct8_storage(block):
+    """DCT8 of an 8x8 block in the reference's storage (u-major: k = u*8+v)."""
+    d = C8() @ block @ C8().T / 64.0  # [v, u]
+    return d.T.reshape(64)
+
+
+def qt_inv(J, tables, c):
+    off = J.qt_offset(0)
+    return np.asarray(tables[off + (3 + c) * 64: off + (4 + c) * 64], np.float64)
+
+
+def cmap_f64(J, xyb, tables, distance, tx, ty):
+    _, h, w = xyb.shape
+    xb, yb = (w + 7) // 8, (h + 7) // 8
+    qx, qb = qt_inv(J, tables, 0), qt_inv(J, tables, 2)
+    ca_x = cb_x = ca_b = cb_b = 0.0
+    num = 0
+    for by in range(ty * 8, min(ty * 8 + 8, yb)):
+        for bx in range(tx * 8, min(tx * 8 + 8, xb)):
+            px, py = bx * 8, by * 8
+            if px + 8 > w or py + 8 > h:
+                continue
+            y, x, b = (dct8_storage(xyb[c, py:py + 8, px:px + 8].astype(np.float64)) for c in (1, 0, 2))
+            y[0] = x[0] = b[0] = 0.0
+            a_x = y * qx / 84.0
+            b_x = -x * qx
+            a_b = y * qb / 84.0
+            b_b = y * qb - b * qb
+            ca_x += np.sum(a_x * a_x)
+            cb_x += np.sum(a_x * b_x)
+            ca_b += np.sum(a_b * a_b)
+            cb_b += np.sum(a_b * b_b)
+            num += 64
+    dz = 1.5 if distance <= 1 else (0.0 if distance >= 2 else 1.5 * (2 - distance))
+
+    def solve(ca, cb):
+        if num == 0:
+            return 0, 0.0
+        v = -cb / (num * 1e-3 * 0.5 + ca)
+        if abs(v) < dz:
+            v = 0.0
+        return int(np.clip(math.floor(abs(v) + 0.5) * np.sign(v), -128, 127)), v
+    return solve(ca_x, cb_x), solve(ca_b, cb_b)
+
+
+@pytest.fixture(scope="module")
+def J():
+    import jixel_amd
+    return jixel_amd
+
+
+@pytest.mark.parametrize("distance", [0.8, 1.5, 2.35])
+def test_cmap_matches_fp64_regression(J, oracle, distance):
+    from jixel_amd.synth import photo_like
+    rgb = photo_like(200, 136, seed=4)
+    oracle.set_fused(True)
+    xyb = oracle.xyb_image(rgb, J.XYB_SPEC_FWD)
+    p = J.frame_params(J.EncodeConfig(distance=distance))
+    tables = J.default_quant_tables()
+    ytox, ytob = oracle.fill_cmap(xyb, tables, p)
+    exact = 0
+    for ty in range(ytox.shape[0]):
+        for tx in range(ytox.shape[1]):
+            (wx, vx), (wb, vb) = cmap_f64(J, xyb, tables, distance, tx, ty)
+            for got, want, v in ((ytox[ty, tx], wx, vx), (ytob[ty, tx], wb, vb)):
+                assert abs(int(got) - want) <= 1, (tx, ty, got, want, v)
+                exact += int(got) == want
+    assert exact == 2 * ytox.size
+
+
+@pytest.mark.parametrize("slope", [0.05, -0.1, 0.4])
+def test_proportional_chroma_recovers_the_slope(J, oracle, slope):
+    """X = slope * Y and B = (1 - slope) * Y in the pixel domain: the X map
+    is round(84 slope) and the B map round(-84 slope); luma is strongly
+    textured so the ridge term num * 5e-4 is negligible."""
+    rng = np.random.default_rng(7)
+    h, w = 64, 128
+    y = rng.uniform(0.0, 20.0, (h, w)).astype(np.float32)
+    xyb = np.stack([slope * y, y, (1.0 - slope) * y]).astype(np.float32)
+    p = J.frame_params(J.EncodeConfig(distance=2.35))
+    ytox, ytob = oracle.fill_cmap(xyb, J.default_quant_tables(), p)
+    assert np.all(ytox == round(84 * slope)), ytox
+    assert np.all(ytob == round(-84 * slope)), ytob
+
+
+def test_deadzone_snaps_small_slopes_at_high_quality(J, oracle):
+    rng = np.random.default_rng(8)
+    y = rng.uniform(0.2, 0.8, (64, 64)).astype(np.float32)
+    xyb = np.stack([0.012 * y, y, y]).astype(np.float32)  # slope ~1 step < 1.5
+    for d, want in ((0.9, 0), (2.5, 1)):
+        p = J.frame_params(J.EncodeConfig(distance=d))
+        ytox, _ = oracle.fill_cmap(xyb, J.default_quant_tables(), p)
+        assert ytox[0, 0] == want, (d, ytox)
+
+
+def test_flat_and_partial_tiles(J, oracle):
+    """Blocks that stick out of the image are skipped; a tile with none left
+    (or flat content) has slope 0."""
+    xyb = np.full((3, 13, 70), 0.3, np.float32)
+    p = J.frame_params(J.EncodeConfig(distance=2.35))
+    ytox, ytob = oracle.fill_cmap(xyb, J.default_quant_tables(), p)
+    assert ytox.shape == (1, 2) and not ytox.any() and not ytob.any()

@@ -210,6 +210,98 @@ __device__ __forceinline__ void dct32(float *b) {
   }
 }
 
+// WC64 (src/dct.rs:2404-2437), the reference's literal constants.
+static __constant__ float WC64_DEV[32] = {
+    0.500150621f, 0.501358449f, 0.50378871f,  0.507471144f, 0.51245147f,  0.518792689f,
+    0.526577294f, 0.535909832f, 0.546920419f, 0.559769809f, 0.574655175f, 0.591818511f,
+    0.611557364f, 0.634238958f, 0.660319805f, 0.690372109f, 0.725120544f, 0.765494168f,
+    0.812702119f, 0.868344724f, 0.934583604f, 1.01440823f,  1.11207163f,  1.23383272f,
+    1.38929391f,  1.59397233f,  1.87467599f,  2.28204989f,  2.9246285f,   4.08461094f,
+    6.79675055f,  20.3738785f};
+
+// dct1d_64 (src/dct.rs:2529-2546) on a line held in LDS (element stride
+// `step`), computed in place so that only 32 values are in registers at a
+// time.  The reference's final even / odd interleave is not performed:
+// position i < 32 holds frequency 2i and position 32 + i frequency 2i + 1
+// (dct64_pos gives the position of a frequency).
+template <bool F>
+__device__ __forceinline__ void dct64_line(float *p, int step, float scale, bool do_scale) {
+  // tmp[i] = buf[i] + buf[63-i], tmp[32+i] = (buf[i] - buf[63-i]) * WC64[i];
+  // positions {i, 31-i, 32+i, 63-i} are read and written by one group.
+  for (int i = 0; i < 16; i++) {
+    const float a = p[i * step], b = p[(63 - i) * step];
+    const float c = p[(31 - i) * step], d = p[(32 + i) * step];
+    p[i * step] = a + b;
+    p[(32 + i) * step] = (a - b) * WC64_DEV[i];
+    p[(31 - i) * step] = c + d;
+    p[(63 - i) * step] = (c - d) * WC64_DEV[31 - i];
+  }
+#pragma unroll
+  for (int half = 0; half < 2; half++) {
+    float b[32];
+    float *q = p + half * 32 * step;
+#pragma unroll
+    for (int i = 0; i < 32; i++) b[i] = q[i * step];
+    dct32<F>(b);
+#pragma unroll
+    for (int i = 0; i < 32; i++) q[i * step] = b[i];
+  }
+  p[32 * step] = fmla<F>(p[32 * step], JXG_SQRT2, p[33 * step]);
+  for (int i = 33; i < 63; i++) p[i * step] += p[(i + 1) * step];
+  if (do_scale)
+    for (int i = 0; i < 64; i++) p[i * step] = p[i * step] * scale;
+}
+
+// LDS position of frequency f along an axis transformed by dct64_line.
+__host__ __device__ __forceinline__ int dct64_pos(int f) { return (f & 1) ? 32 + (f >> 1) : f >> 1; }
+
+// inv_dct1d_2 / inv_dct1d_4 / inv_dct1d_8 (src/dct.rs:1500-1543).
+__device__ __forceinline__ void inv_dct2(float *b) {
+  const float s = b[0], d = b[1];
+  b[0] = (s + d) * 0.5f;
+  b[1] = (s - d) * 0.5f;
+}
+__device__ __forceinline__ void inv_dct4(float *b) {
+  const float IS2 = 0.70710678118654752440f;
+  float t[4] = {b[0], b[2], b[1], b[3]};
+  t[2] = (t[2] - t[3]) * IS2;
+  inv_dct2(t + 2);
+  t[2] *= 1.0f / 0.5411961f;
+  t[3] *= 1.0f / 1.306563f;
+  inv_dct2(t);
+  b[0] = (t[0] + t[2]) * 0.5f;
+  b[3] = (t[0] - t[2]) * 0.5f;
+  b[1] = (t[1] + t[3]) * 0.5f;
+  b[2] = (t[1] - t[3]) * 0.5f;
+}
+__device__ __forceinline__ void inv_dct8(float *b) {
+  const float IS2 = 0.70710678118654752440f;
+  const float iwc[4] = {1.0f / 0.5097956f, 1.0f / 0.6013449f, 1.0f / 0.8999762f,
+                        1.0f / 2.5629156f};
+  float t[8];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    t[i] = b[2 * i];
+    t[4 + i] = b[2 * i + 1];
+  }
+  t[6] -= t[7];
+  t[5] -= t[6];
+  t[4] = (t[4] - t[5]) * IS2;
+  inv_dct4(t + 4);
+#pragma unroll
+  for (int i = 0; i < 4; i++) t[4 + i] *= iwc[i];
+  inv_dct4(t);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    b[i] = (t[i] + t[4 + i]) * 0.5f;
+    b[7 - i] = (t[i] - t[4 + i]) * 0.5f;
+  }
+}
+
+// RESAMPLE_SCALE_64_TO_8 (src/dct.rs:2005-2014)
+static __constant__ float RS64_8_DEV[8] = {
+    1.0f, 0.9936866f, 0.97488683f, 0.94401807f, 0.9017642f, 0.8490575f, 0.7870549f, 0.71710813f};
+
 // One line transform of length N held in LDS with element stride `step`,
 // optionally scaled afterwards (the last pass of a 2-D assembly multiplies
 // by 1/(W*H)).
@@ -229,14 +321,19 @@ __device__ __forceinline__ void dct_line(float *p, int step, float scale, bool d
   for (int i = 0; i < N; i++) p[i * step] = b[i];
 }
 
-template <bool F>
+// Line of length n (8, 16, 32, or 64 when BIG).  The 64-point line is only
+// compiled into kernels that take DCT64-family transforms, so it does not
+// raise the register budget of the others.
+template <bool F, bool BIG = false>
 __device__ __forceinline__ void dct_line_n(int n, float *p, int step, float scale, bool do_scale) {
   if (n == 8)
     dct_line<F, 8>(p, step, scale, do_scale);
   else if (n == 16)
     dct_line<F, 16>(p, step, scale, do_scale);
-  else
+  else if (!BIG || n == 32)
     dct_line<F, 32>(p, step, scale, do_scale);
+  else if constexpr (BIG)
+    dct64_line<F>(p, step, scale, do_scale);
 }
 
 // idct1d_4 (src/dct.rs:2144-2168)
@@ -264,9 +361,16 @@ __device__ __forceinline__ int cov_y(int s) {
 }
 
 // Whether the 2-D assembly transforms rows first (src/dct.rs: 8x8, 8x16,
-// 16x32, 8x32) or columns first (16x8, 16x16, 32x32, 32x16, 32x8).
+// 16x32, 8x32, 32x64) or columns first (16x8, 16x16, 32x32, 32x16, 32x8,
+// 64x64, 64x32).
 __device__ __forceinline__ bool rows_first(int s) {
-  return s == JXG_DCT || s == JXG_DCT8X16 || s == JXG_DCT16X32 || s == JXG_DCT8X32;
+  return s == JXG_DCT || s == JXG_DCT8X16 || s == JXG_DCT16X32 || s == JXG_DCT8X32 ||
+         s == JXG_DCT32X64;
+}
+
+// The DCT64 family (64X64, 64X32, 32X64).
+__host__ __device__ __forceinline__ bool is_dct64(int s) {
+  return s == JXG_DCT64X64 || s == JXG_DCT64X32 || s == JXG_DCT32X64;
 }
 
 // Storage index of the coefficient with vertical frequency v and horizontal

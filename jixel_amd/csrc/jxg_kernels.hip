@@ -898,6 +898,11 @@ struct QLds {
   uint8_t line1_t[512], line2_t[512];  // line -> transform, per pass
   uint8_t chunk_t[64];                 // 64-coefficient chunk -> transform
   float sub8_px[32][64];               // pixel copies for sub-8x8 transforms
+  int n64;                             // DCT64-family transforms in the list (<= 2)
+  int t64[2];
+  float dcs[2][64];                    // DCT64 DC: inverse-transform scratch
+  float dc64v[2][64];                  // DCT64 DC values of the current channel
+  int16_t ydc64[2][64];                // quantized Y DC of the DCT64 transforms
 };
 
 __device__ __forceinline__ int find_item(const int *pre, int n, int e) {
@@ -992,7 +997,7 @@ __device__ void dc_from(int s, const float *co, int pitch, int w, int h, float *
 __host__ __device__ __forceinline__ bool quantize_supported(int s) {
   return s == JXG_DCT || s == JXG_DCT16X8 || s == JXG_DCT8X16 || s == JXG_DCT16X16 ||
          s == JXG_DCT32X32 || s == JXG_DCT32X16 || s == JXG_DCT16X32 || s == JXG_DCT32X8 ||
-         s == JXG_DCT8X32 || is_sub8(s);
+         s == JXG_DCT8X32 || is_sub8(s) || is_dct64(s);
 }
 
 struct QArgs {
@@ -1045,6 +1050,9 @@ __device__ __forceinline__ CoefGeom coef_geom(const QLds &L, int t, int k, int x
     u = k >> lh;
     v = k & ((1 << lh) - 1);
   }
+  // 64-long axes keep dct64_line's even / odd split order in LDS
+  if (cvy == 8) v = dct64_pos(v);
+  if (cvx == 8) u = dct64_pos(u);
   const int lwc = lw > lh ? lw : lh;
   CoefGeom g;
   g.a = (L.ly[t] + v) * QT_PITCH + L.lx[t] + u;
@@ -1077,7 +1085,7 @@ __device__ __forceinline__ void load_plane(QLds &L, const QArgs &A, int c, int p
 
 // Every transform of the list on L.cur (one channel): sub-8x8 transforms
 // one block per thread, the DCTs as two passes of 1-D lines.
-template <bool F>
+template <bool F, bool BIG>
 __device__ void transform_plane(QLds &L, int n) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   // 32 threads, each owning one pixel-copy slot, take the sub-8x8 blocks.
@@ -1095,9 +1103,9 @@ __device__ void transform_plane(QLds &L, int n) {
     const int li = e - L.pre1[t];
     float *base = &L.cur[L.ly[t] * QT_PITCH + L.lx[t]];
     if (rows_first(s))
-      dct_line_n<F>(w, base + li * QT_PITCH, 1, 0.0f, false);
+      dct_line_n<F, BIG>(w, base + li * QT_PITCH, 1, 0.0f, false);
     else
-      dct_line_n<F>(h, base + li, QT_PITCH, 0.0f, false);
+      dct_line_n<F, BIG>(h, base + li, QT_PITCH, 0.0f, false);
   }
   __syncthreads();
   for (int e = tid; e < L.pre2[n]; e += nthr) {
@@ -1108,17 +1116,97 @@ __device__ void transform_plane(QLds &L, int n) {
     const float scale = 1.0f / (float)(w * h);
     float *base = &L.cur[L.ly[t] * QT_PITCH + L.lx[t]];
     if (rows_first(s))
-      dct_line_n<F>(h, base + li, QT_PITCH, scale, true);
+      dct_line_n<F, BIG>(h, base + li, QT_PITCH, scale, true);
     else
-      dct_line_n<F>(w, base + li * QT_PITCH, 1, scale, true);
+      dct_line_n<F, BIG>(w, base + li * QT_PITCH, 1, scale, true);
   }
   __syncthreads();
+}
+
+// DC values of the DCT64-family transforms of the list from the channel's
+// coefficients in L.cur (src/dct.rs:2602-2611 dc_from_dct64x64,
+// :2767-2805 dc_from_dct64x32 / dc_from_dct32x64), into L.dc64v in the
+// covered-block order iy * cov_x + ix.  Eight lanes per transform follow the
+// reference's two inverse passes; called by every thread of the workgroup.
+template <bool F>
+__device__ void dc64_values(QLds &L) {
+  const int tid = threadIdx.x;
+  const int j = tid >> 3, i = tid & 7;
+  const float r4[4] = {1.0f, 0.9748868f, 0.9017642f, 0.7870549f};
+  if (tid < 16 && j < L.n64) {
+    const int t = L.t64[j], s = L.s[t];
+    const float *co = &L.cur[L.ly[t] * QT_PITCH + L.lx[t]];
+    const int W = cov_x(s) * 8, H = cov_y(s) * 8;
+    auto C = [&](int vf, int hf) {
+      return co[(H == 64 ? dct64_pos(vf) : vf) * QT_PITCH + (W == 64 ? dct64_pos(hf) : hf)];
+    };
+    if (s == JXG_DCT64X64) {
+      // inv_dct8x8 of low * 64: first the column of horizontal frequency i
+      float col[8];
+#pragma unroll
+      for (int v = 0; v < 8; v++) col[v] = (C(v, i) * (RS64_8_DEV[v] * RS64_8_DEV[i])) * 64.0f;
+      inv_dct8(col);
+#pragma unroll
+      for (int v = 0; v < 8; v++) L.dcs[j][v * 8 + i] = col[v];
+    } else if (i < 4) {
+      // rows[a] = idct1d_8_unnormalized over the long axis, a = short-axis frequency
+      const int a = i;
+      float r[8];
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const float c = s == JXG_DCT64X32 ? C(b, a) : C(a, b);
+        r[b] = c * (r4[a] * RS64_8_DEV[b]);
+      }
+#pragma unroll
+      for (int b = 0; b < 8; b++) r[b] *= 8.0f;
+      inv_dct8(r);
+#pragma unroll
+      for (int b = 0; b < 8; b++) L.dcs[j][a * 8 + b] = r[b];
+    }
+  }
+  __syncthreads();
+  if (tid < 16 && j < L.n64) {
+    const int s = L.s[L.t64[j]];
+    if (s == JXG_DCT64X64) {
+      float row[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) row[u] = L.dcs[j][i * 8 + u];
+      inv_dct8(row);
+#pragma unroll
+      for (int u = 0; u < 8; u++) L.dc64v[j][i * 8 + u] = row[u];
+    } else {
+      const float cin[4] = {L.dcs[j][i], L.dcs[j][8 + i], L.dcs[j][16 + i], L.dcs[j][24 + i]};
+      float col[4];
+      idct4<F>(cin, col);
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        if (s == JXG_DCT64X32)
+          L.dc64v[j][i * 4 + y] = col[y];  // 8 rows x 4 columns
+        else
+          L.dc64v[j][y * 8 + i] = col[y];  // 4 rows x 8 columns
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// X / B DC with the luma term (src/group.rs:2003,2148 quantize_dc_cfl).
+template <bool F>
+__device__ __forceinline__ int16_t dc_cfl_q(int c, float xd, float yq, const jxg_derived *D) {
+  if (c == 0) {
+    if (D->cfl_factor_x != 0.0f)
+      return sat_i16(round_away(fmla<F>(xd, D->inv_factor[0], -yq * D->cfl_factor_x)));
+    return sat_i16(round_away(xd * D->inv_factor[0]));
+  }
+  return sat_i16(round_away(fmla<F>(xd, D->inv_factor[2], -yq * D->cfl_factor_b)));
 }
 
 // Transform + quantize the L.n transforms listed in L, one channel at a
 // time: Y first (its dequantized form stays in L.y2 for CfL), then X and B.
 // (px0, py0, win_w, win_h) is the pixel rectangle that L.cur holds.
-template <bool F>
+// BIG: the list may hold DCT64-family transforms (the crossing / large
+// transform pass); the tile pass is compiled without that code.
+template <bool F, bool BIG>
 __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win_w, int win_h) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
@@ -1126,9 +1214,10 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   const int n = L.n;
   if (n == 0) return;
   if (tid == 0) {
-    int p1 = 0, p2 = 0, pc = 0;
+    int p1 = 0, p2 = 0, pc = 0, n64 = 0;
     for (int t = 0; t < n; t++) {
       const int s = L.s[t];
+      if (is_dct64(s) && n64 < 2) L.t64[n64++] = t;
       const int w = cov_x(s) * 8, h = cov_y(s) * 8;
       L.pre1[t] = p1;
       L.pre2[t] = p2;
@@ -1142,6 +1231,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     L.pre1[n] = p1;
     L.pre2[n] = p2;
     L.prec[n] = pc;
+    L.n64 = n64;
   }
   if (tid < n) {
     const int t = tid;
@@ -1166,9 +1256,21 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   // ---- Y ----
   load_plane(L, A, 1, px0, py0, win_w, win_h);
   __syncthreads();
-  transform_plane<F>(L, n);
+  transform_plane<F, BIG>(L, n);
   // Y DC (dc_from_coefficients, quantize_dc_scalar src/group.rs:290)
-  if (tid < n) {
+  if constexpr (BIG) dc64_values<F>(L);
+  if (BIG && tid < 128) {
+    const int j = tid >> 6, i = tid & 63;
+    if (j < L.n64) {
+      const int t = L.t64[j], cvx = cov_x(L.s[t]);
+      if (i < cvx * cov_y(L.s[t])) {
+        const int16_t q = sat_i16(round_away(L.dc64v[j][i] * D->inv_factor[1]));
+        L.ydc64[j][i] = q;
+        A.dc1[(size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx] = q;
+      }
+    }
+  }
+  if (tid < n && !is_dct64(L.s[tid])) {
     const int t = tid, s = L.s[t];
     float dcv[16];
     dc_from<F>(s, &L.cur[L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, cov_x(s) * 8, cov_y(s) * 8, dcv);
@@ -1236,7 +1338,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   for (int c = 0; c <= 2; c += 2) {
     load_plane(L, A, c, px0, py0, win_w, win_h);
     __syncthreads();
-    transform_plane<F>(L, n);
+    transform_plane<F, BIG>(L, n);
     int32_t *lv = c == 0 ? A.lv0 : A.lv2;
     float *co = c == 0 ? A.co0 : A.co2;
     const float qm = D->qm_mult[c];
@@ -1278,7 +1380,19 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
       }
     }
     __syncthreads();
-    if (tid < n) {
+    if constexpr (BIG) dc64_values<F>(L);
+    if (BIG && tid < 128) {
+      const int j = tid >> 6, i = tid & 63;
+      if (j < L.n64) {
+        const int t = L.t64[j], cvx = cov_x(L.s[t]);
+        if (i < cvx * cov_y(L.s[t])) {
+          int16_t *dc = c == 0 ? A.dc0 : A.dc2;
+          dc[(size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx] =
+              dc_cfl_q<F>(c, L.dc64v[j][i], (float)L.ydc64[j][i], D);
+        }
+      }
+    }
+    if (tid < n && !is_dct64(L.s[tid])) {
       const int t = tid, s = L.s[t];
       float xd[16];
       dc_from<F>(s, &L.cur[L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, cov_x(s) * 8, cov_y(s) * 8, xd);
@@ -1322,7 +1436,7 @@ __global__ __launch_bounds__(256) void k_quantize(QArgs A) {
         if (s >= JXG_NUM_STRATEGIES || !quantize_supported(s) || bx0 + lx + cov_x(s) > A.xb ||
             by0 + ly + cov_y(s) > A.yb) {
           atomicOr(A.bad, 1u);
-        } else if (lx + cov_x(s) <= 8 && ly + cov_y(s) <= 8) {
+        } else if (lx + cov_x(s) <= 8 && ly + cov_y(s) <= 8 && !is_dct64(s)) {
           const int slot = atomicAdd(&L.n, 1);
           L.s[slot] = s;
           L.bx[slot] = bx0 + lx;
@@ -1337,12 +1451,13 @@ __global__ __launch_bounds__(256) void k_quantize(QArgs A) {
     }
   }
   __syncthreads();
-  quantize_list<F>(L, A, bx0 * 8, by0 * 8, 64, 64);
+  quantize_list<F, false>(L, A, bx0 * 8, by0 * 8, 64, 64);
 }
 
 // Transforms whose footprint crosses their 64x64 tile (the layout DP's
-// crossing placements, src/dc_group_data.rs:207-217): one workgroup per
-// queued transform, footprint gathered straight from the padded planes.
+// crossing placements, src/dc_group_data.rs:207-217) and the DCT64-family
+// transforms: one workgroup per queued transform, footprint gathered
+// straight from the padded planes.
 template <bool F>
 __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
   __shared__ QLds L;
@@ -1361,7 +1476,7 @@ __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
       L.ly[0] = 0;
     }
     __syncthreads();
-    quantize_list<F>(L, A, bx * 8, by * 8, cov_x(s) * 8, cov_y(s) * 8);
+    quantize_list<F, true>(L, A, bx * 8, by * 8, cov_x(s) * 8, cov_y(s) * 8);
     __syncthreads();
   }
 }

@@ -70,6 +70,8 @@ int jxo_forward_transform(int strategy, const float *plane, int width, int heigh
  * src/group.rs:183). */
 void jxo_dc_from_coefficients(int strategy, const float *coeffs, float *dc);
 int jxo_covered_x(int strategy);
+/* The DCT64 butterfly constants the oracle uses (32 floats). */
+const float *jxo_wc64(void);
 /* The AFV basis the oracle uses (16 x 16, row j = pixel j). */
 const float *jxo_afv_basis(void);
 int jxo_covered_y(int strategy);

@@ -110,11 +110,42 @@ static void dct1d_32(float *buf) {
   }
 }
 
+/* WC64 (src/dct.rs:2404-2437): the reference's literal butterfly constants
+ * (a few differ by one ulp from 1 / (2 cos((2i+1) pi / 128)) rounded to f32,
+ * so the literals themselves are used). */
+static const float WC64[32] = {
+    0.500150621f, 0.501358449f, 0.50378871f,  0.507471144f, 0.51245147f,  0.518792689f,
+    0.526577294f, 0.535909832f, 0.546920419f, 0.559769809f, 0.574655175f, 0.591818511f,
+    0.611557364f, 0.634238958f, 0.660319805f, 0.690372109f, 0.725120544f, 0.765494168f,
+    0.812702119f, 0.868344724f, 0.934583604f, 1.01440823f,  1.11207163f,  1.23383272f,
+    1.38929391f,  1.59397233f,  1.87467599f,  2.28204989f,  2.9246285f,   4.08461094f,
+    6.79675055f,  20.3738785f};
+
+const float *jxo_wc64(void) { return WC64; }
+
+/* dct1d_64 (src/dct.rs:2529-2546) */
+static void dct1d_64(float *buf) {
+  float tmp[64];
+  for (int i = 0; i < 32; i++) {
+    tmp[i] = buf[i] + buf[63 - i];
+    tmp[32 + i] = (buf[i] - buf[63 - i]) * WC64[i];
+  }
+  dct1d_32(tmp);
+  dct1d_32(tmp + 32);
+  tmp[32] = jxo_fmla(tmp[32], SQRT2F, tmp[33]);
+  for (int i = 33; i < 63; i++) tmp[i] += tmp[i + 1];
+  for (int i = 0; i < 32; i++) {
+    buf[2 * i] = tmp[i];
+    buf[2 * i + 1] = tmp[32 + i];
+  }
+}
+
 static void dct1d(float *buf, int n) {
   switch (n) {
     case 8: dct1d_8(buf); break;
     case 16: dct1d_16(buf); break;
     case 32: dct1d_32(buf); break;
+    case 64: dct1d_64(buf); break;
     default: abort();
   }
 }
@@ -431,6 +462,9 @@ int jxo_forward_transform(int strategy, const float *plane, int width, int heigh
     case JXG_DCT16X32: dct_rows_first(pix, 32, 16, out); break; /* W32 H16 */
     case JXG_DCT32X8: dct_cols_first(pix, 8, 32, out); break;   /* W8 H32 */
     case JXG_DCT8X32: dct_rows_first(pix, 32, 8, out); break;   /* W32 H8 */
+    case JXG_DCT64X64: dct_cols_first(pix, 64, 64, out); break; /* src/dct.rs:2576 */
+    case JXG_DCT64X32: dct_cols_first(pix, 32, 64, out); break; /* W32 H64, :2660 */
+    case JXG_DCT32X64: dct_rows_first(pix, 64, 32, out); break; /* W64 H32, :2688 */
     case JXG_IDENTITY: identity8x8(pix, out); break;
     case JXG_DCT2X2: dct2x2_8x8(pix, out); break;
     case JXG_DCT4X4: dct4x4_8x8(pix, out); break;
@@ -460,6 +494,86 @@ static void idct1d_4(const float v[4], float out[4]) {
   out[3] = jxo_fmla(-WC4[0], t[2], t[0]);
   out[1] = jxo_fmla(WC4[1], t[3], t[1]);
   out[2] = jxo_fmla(-WC4[1], t[3], t[1]);
+}
+
+/* inv_dct1d_2 / inv_dct1d_4 / inv_dct1d_8 (src/dct.rs:1500-1543) */
+static const float INV_WC4[2] = {1.0f / 0.5411961f, 1.0f / 1.306563f};
+static void inv_dct1d_2(float *b) {
+  float s = b[0], d = b[1];
+  b[0] = (s + d) * 0.5f;
+  b[1] = (s - d) * 0.5f;
+}
+static void inv_dct1d_4(float *b) {
+  const float IS2 = 0.70710678118654752440f;
+  float t[4] = {b[0], b[2], b[1], b[3]};
+  t[2] = (t[2] - t[3]) * IS2;
+  inv_dct1d_2(t + 2);
+  t[2] *= INV_WC4[0];
+  t[3] *= INV_WC4[1];
+  inv_dct1d_2(t);
+  b[0] = (t[0] + t[2]) * 0.5f;
+  b[3] = (t[0] - t[2]) * 0.5f;
+  b[1] = (t[1] + t[3]) * 0.5f;
+  b[2] = (t[1] - t[3]) * 0.5f;
+}
+static void inv_dct1d_8(float *b) {
+  const float IS2 = 0.70710678118654752440f;
+  const float INV_WC8[4] = {1.0f / WC8[0], 1.0f / WC8[1], 1.0f / WC8[2], 1.0f / WC8[3]};
+  float t[8];
+  for (int i = 0; i < 4; i++) {
+    t[i] = b[2 * i];
+    t[4 + i] = b[2 * i + 1];
+  }
+  t[6] -= t[7];
+  t[5] -= t[6];
+  t[4] = (t[4] - t[5]) * IS2;
+  inv_dct1d_4(t + 4);
+  for (int i = 0; i < 4; i++) t[4 + i] *= INV_WC8[i];
+  inv_dct1d_4(t);
+  for (int i = 0; i < 4; i++) {
+    b[i] = (t[i] + t[4 + i]) * 0.5f;
+    b[7 - i] = (t[i] - t[4 + i]) * 0.5f;
+  }
+}
+
+/* RESAMPLE_SCALE_64_TO_8 (src/dct.rs:2005-2014) */
+static const float RS64_8[8] = {1.0f,       0.9936866f, 0.97488683f, 0.94401807f,
+                                0.9017642f, 0.8490575f, 0.7870549f,  0.71710813f};
+
+/* dc_from_dct64x64 (src/dct.rs:2602-2611): the 8x8 lowest frequencies,
+ * rescaled, through inv_dct8x8 (inv_dct_square, :1594-1621). */
+static void dc_from_dct64x64(const float *c, float *dc) {
+  float low[64], out[64], col[8];
+  for (int y = 0; y < 8; y++)
+    for (int x = 0; x < 8; x++) low[y * 8 + x] = c[y * 64 + x] * (RS64_8[x] * RS64_8[y]);
+  for (int a = 0; a < 8; a++)
+    for (int b = 0; b < 8; b++) out[b * 8 + a] = low[a * 8 + b] * 64.0f;
+  for (int u = 0; u < 8; u++) {
+    for (int v = 0; v < 8; v++) col[v] = out[v * 8 + u];
+    inv_dct1d_8(col);
+    for (int v = 0; v < 8; v++) out[v * 8 + u] = col[v];
+  }
+  for (int v = 0; v < 8; v++) inv_dct1d_8(out + v * 8);
+  memcpy(dc, out, sizeof(out));
+}
+
+/* dc_from_dct64x32_normalized (src/dct.rs:2767-2785): rows[a] is the
+ * unnormalised 8-point inverse of the a-th long-axis frequency row, then a
+ * 4-point inverse down each column; out[y][x], y < 4, x < 8. */
+static void idct1d_4(const float v[4], float out[4]);
+static void dc_64x32_normalized(const float *c, float out[4][8]) {
+  float rows[4][8];
+  for (int a = 0; a < 4; a++) {
+    for (int b = 0; b < 8; b++) rows[a][b] = c[a * 64 + b] * (RS32_4[a] * RS64_8[b]);
+    for (int b = 0; b < 8; b++) rows[a][b] *= 8.0f; /* idct1d_8_unnormalized :2756 */
+    inv_dct1d_8(rows[a]);
+  }
+  for (int x = 0; x < 8; x++) {
+    const float col_in[4] = {rows[0][x], rows[1][x], rows[2][x], rows[3][x]};
+    float col[4];
+    idct1d_4(col_in, col);
+    for (int y = 0; y < 4; y++) out[y][x] = col[y];
+  }
 }
 
 /* dc_from_coefficients (src/group.rs:183-260) with dc_from_dct16x8 :930,
@@ -540,6 +654,21 @@ void jxo_dc_from_coefficients(int s, const float *c, float *dc) {
       float sv[4];
       for (int b = 0; b < 4; b++) sv[b] = c[b] * RS32_4[b];
       idct1d_4(sv, dc);
+      return;
+    }
+    case JXG_DCT64X64: dc_from_dct64x64(c, dc); return;
+    case JXG_DCT64X32: { /* 8 rows x 4 columns (src/dct.rs:2789-2796) */
+      float n[4][8];
+      dc_64x32_normalized(c, n);
+      for (int y = 0; y < 8; y++)
+        for (int x = 0; x < 4; x++) dc[y * 4 + x] = n[x][y];
+      return;
+    }
+    case JXG_DCT32X64: { /* 4 rows x 8 columns (:2800-2805) */
+      float n[4][8];
+      dc_64x32_normalized(c, n);
+      for (int y = 0; y < 4; y++)
+        for (int x = 0; x < 8; x++) dc[y * 8 + x] = n[y][x];
       return;
     }
     default: abort();

@@ -10,7 +10,8 @@ tables against:
   the natural coefficient orders.
 * DEQUANT_MATRIX_8X8 / DEQUANT_MATRIX_16X8 (src/quant_weights.rs:246,455):
   the stored DCT8 and 16x8 step tables.
-* WC8 / WC16 / WC32 (src/dct.rs): butterfly constants.
+* WC8 / WC16 / WC32 / WC64 (src/dct.rs): butterfly constants.
+* RESAMPLE_SCALE_64_TO_8 (src/dct.rs:2005): DCT64 DC resampling scales.
 * AFV_BASIS_TRANSPOSE (src/afv.rs:48): the 16x16 AFV basis, row j = pixel j.
 
 Usage: python tests/golden/make_reference_tables.py [/root/reference]
@@ -82,10 +83,13 @@ def main():
         out[name] = [vals[c * n:(c + 1) * n] for c in range(3)]
 
     dct = read("src/dct.rs")
-    for name, n in [("WC8", 4), ("WC16", 8), ("WC32", 16)]:
+    for name, n in [("WC8", 4), ("WC16", 8), ("WC32", 16), ("WC64", 32)]:
         m = re.search(r"const " + name + r": \[f32; \d+\] = ", dct)
         vals = numbers(block_after(dct[m.start():], name))[-n:]
         out[name] = vals
+
+    m = re.search(r"const RESAMPLE_SCALE_64_TO_8: \[f32; 8\] = ", dct)
+    out["RESAMPLE_SCALE_64_TO_8"] = numbers(block_after(dct[m.start():], "RESAMPLE_SCALE_64_TO_8"))[-8:]
 
     afv = read("src/afv.rs")
     vals = numbers(block_after(afv, "static AFV_BASIS_TRANSPOSE"))[-256:]

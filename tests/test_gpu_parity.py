@@ -162,9 +162,11 @@ COV = {0: (1, 1), 1: (1, 2), 2: (2, 1), 3: (2, 2), 4: (4, 4), 8: (2, 4), 9: (4, 
        23: (4, 1)}
 SUB8 = [5, 6, 7, 10, 11, 12, 13, 17, 18]  # DCT4X4, 4X8, 8X4, AFV0-3, IDENTITY, DCT2X2
 COV.update({s: (1, 1) for s in SUB8})
+BIG = [14, 15, 16]  # DCT64X64, DCT64X32 (32 wide, 64 tall), DCT32X64 (64 wide, 32 tall)
+COV.update({14: (8, 8), 15: (4, 8), 16: (8, 4)})
 
 
-def random_layout(xb, yb, seed, crossing=False, sub8=False):
+def random_layout(xb, yb, seed, crossing=False, sub8=False, big=False):
     """A valid strategy image: multi-block transforms inside the image and
     their 256-px group, and inside their 64-px tile unless `crossing`;
     `sub8` mixes in the single-block sub-8x8 transforms."""
@@ -176,6 +178,9 @@ def random_layout(xb, yb, seed, crossing=False, sub8=False):
     if sub8:
         kinds = kinds + SUB8
         prob = [0.5 * x for x in prob] + [0.5 / len(SUB8)] * len(SUB8)
+    if big:
+        kinds = kinds + BIG
+        prob = [0.4 * x for x in prob] + [0.2, 0.2, 0.2]
     for by in range(yb):
         for bx in range(xb):
             if taken[by, bx]:
@@ -234,9 +239,36 @@ def test_forward_quantize_sub8_bit_exact(J, fe, torch, oracle, kind, fused):
     rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, 1.4, 21, fused, cmap=True)
     xb, yb = (w + 7) // 8, (h + 7) // 8
     if kind == "mixed":
-        cells = random_layout(xb, yb, seed=77, crossing=True, sub8=True)
+        cells = random_layout(xb, yb, seed=77, crossing=True, sub8=True, big=True)
     else:
         cells = np.full((yb, xb), (kind << 1) | 1, np.uint8)
+    tables = J.default_quant_tables()
+    lv_w, co_w, dc_w = oracle.forward_quantize(xyb, cells, qf, ytox, ytob, tables, p)
+    oracle.set_fused(True)
+    o = fe.forward_quantize(to_dev(torch, padded(J, xyb)), w, h, to_dev(torch, cells),
+                            to_dev(torch, qf), p, to_dev(torch, ytox), to_dev(torch, ytob))
+    assert fe.quantize_status()[1] == 0
+    for c in range(3):
+        for name, g, wnt in [("levels", o["levels"][c], lv_w[c]), ("coeffs", o["coeffs"][c], co_w[c]),
+                             ("dc", o["dc"][c], dc_w[c])]:
+            g = g.cpu().numpy()
+            assert same_bits(g, wnt), (name, c, first_mismatch(g, wnt))
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("w,h,seed", [(64, 64, 1), (128, 72, 2), (200, 136, 3), (264, 200, 4)])
+def test_forward_quantize_dct64_bit_exact(J, fe, torch, oracle, w, h, seed, fused):
+    """DCT64X64 / DCT64X32 / DCT32X64 (src/dct.rs:2529-2805) beside the
+    smaller transforms, on tile-aligned and crossing placements."""
+    log("quantize dct64", w, h, seed, fused)
+    rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, 2.35, 40 + seed, fused, cmap=True)
+    xb, yb = (w + 7) // 8, (h + 7) // 8
+    if seed == 1:
+        cells = np.full((yb, xb), 14 << 1, np.uint8)
+        cells[0, 0] |= 1
+    else:
+        cells = random_layout(xb, yb, seed=seed, crossing=seed % 2 == 0, big=True)
+    assert any((c >> 1) in BIG for c in cells.reshape(-1) if c & 1)
     tables = J.default_quant_tables()
     lv_w, co_w, dc_w = oracle.forward_quantize(xyb, cells, qf, ytox, ytob, tables, p)
     oracle.set_fused(True)
@@ -254,7 +286,7 @@ def test_unsupported_cells_are_reported(J, fe, torch, oracle):
     w, h = 64, 64
     rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, 2.35, 1)
     cells = np.full((8, 8), 1, np.uint8)
-    cells[0, 0] = (14 << 1) | 1  # DCT64X64: not a quantize-stage strategy yet
+    cells[0, 0] = (19 << 1) | 1  # DCT128X128: not a quantize-stage strategy
     fe.forward_quantize(to_dev(torch, padded(J, xyb)), w, h, to_dev(torch, cells), to_dev(torch, qf), p)
     assert fe.quantize_status()[1] == 1
 

@@ -1,5 +1,6 @@
 """Pins the CPU oracle against the reference's own tables, known answers and
 closed forms (CPU only)."""
+import ctypes
 import math
 import struct
 
@@ -42,12 +43,16 @@ def test_exp2_table_matches_reference(oracle, golden):
     assert [bits(t[i]) for i in range(64)] == [p[0] for p in golden["EXP2FT"]]
 
 
-def test_butterfly_constants_are_the_reference_constants(golden):
-    # WC_N[i] = 1 / (2 cos((2i+1) pi / 2N)) rounded to f32 (src/dct.rs:55-59)
-    for n, name in [(8, "WC8"), (16, "WC16"), (32, "WC32")]:
+def test_butterfly_constants_are_the_reference_constants(oracle, golden):
+    # WC_N[i] = 1 / (2 cos((2i+1) pi / 2N)) to within an f32 ulp (src/dct.rs:55-59,2404)
+    for n, name in [(8, "WC8"), (16, "WC16"), (32, "WC32"), (64, "WC64")]:
         for i, v in enumerate(golden[name]):
             exact = 1.0 / (2.0 * math.cos((2 * i + 1) * math.pi / (2 * n)))
             assert abs(f32(v) - exact) <= 2e-7 * exact, (name, i)
+    L = oracle.lib()
+    L.jxo_wc64.restype = ctypes.POINTER(ctypes.c_float)
+    w = L.jxo_wc64()
+    assert [bits(w[i]) for i in range(32)] == [bits(f32(v)) for v in golden["WC64"]]
 
 
 # ---- known answers --------------------------------------------------------
@@ -132,7 +137,7 @@ def _dct_mat(n):
 
 
 SHAPES = {0: (8, 8), 1: (8, 16), 2: (16, 8), 3: (16, 16), 4: (32, 32), 8: (16, 32),
-          9: (32, 16), 22: (8, 32), 23: (32, 8)}
+          9: (32, 16), 22: (8, 32), 23: (32, 8), 14: (64, 64), 15: (32, 64), 16: (64, 32)}
 
 
 @pytest.mark.parametrize("strategy", sorted(SHAPES))
@@ -165,11 +170,11 @@ def test_dc_of_constant_block_is_the_value(oracle):
     for s, (w, h) in SHAPES.items():
         img = np.full((h, w), 0.375, np.float32)
         co = oracle.forward_transform(s, img, 0, 0)
-        dc = np.zeros(16, np.float32)
+        dc = np.zeros(64, np.float32)
         L.jxo_dc_from_coefficients(s, co.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                    dc.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
         n = (w // 8) * (h // 8)
-        assert np.allclose(dc[:n], 0.375, atol=2e-6), (s, dc[:n])
+        assert np.allclose(dc[:n], 0.375, atol=4e-6), (s, dc[:n])
 
 
 def test_xyb_closed_form(oracle):
@@ -242,3 +247,23 @@ def test_quantizer_brute_force(oracle):
                 v = vals[i]
                 want = 0 if abs(v) < t else int(math.copysign(math.floor(abs(v) + 0.5), v))
                 assert out[i] == want, (c, xs, i, v, t)
+
+
+@pytest.mark.parametrize("strategy", [14, 15, 16])
+def test_dct64_dc_is_the_block_mean(oracle, strategy):
+    """DC of each covered 8x8 block equals that block's mean for content
+    whose spectrum lives in the lowest 8x8 frequencies (a smooth ramp plus
+    one cosine of the 8-block grid period), which pins the resample scales,
+    the inverse DCT8 and the output order (src/dct.rs:2602,2789,2800)."""
+    w, h = SHAPES[strategy]
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    img = (0.2 + 0.3 * np.cos(np.pi * (2 * xx + 1) * 2 / (2 * w))
+           + 0.25 * np.cos(np.pi * (2 * yy + 1) * 1 / (2 * h))).astype(np.float32)
+    co = oracle.forward_transform(strategy, img, 0, 0)
+    L = oracle.lib()
+    dc = np.zeros(64, np.float32)
+    L.jxo_dc_from_coefficients(strategy, co.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               dc.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    bw, bh = w // 8, h // 8
+    means = img.astype(np.float64).reshape(bh, 8, bw, 8).mean(axis=(1, 3)).reshape(-1)
+    assert np.abs(dc[:bw * bh] - means).max() < 2e-3, (dc[:bw * bh], means)

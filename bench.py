@@ -116,6 +116,47 @@ def time_kernel(fn, stream, reps, alg_bytes):
             "algorithmic_bytes": int(alg_bytes)}
 
 
+def dct64_forms(fe, xyb, w, h, stream, reps):
+    """DCT64X64 of every 64x64 block of the image in both forms (FP32
+    butterflies; split-bf16 x3 on the matrix cores), each with its HBM and
+    matrix-core roofline position and its error against the FP64 closed form
+    on a sample of blocks."""
+    import math
+    import torch
+    nblk = ((w + 63) // 64) * ((h + 63) // 64)
+    px = 64 * 64 * nblk
+    outs = {m: fe.dct64_image(xyb, w, h, m) for m in (0, 1)}
+    k = np.arange(64)[:, None]
+    cm = np.cos(np.pi * k * (2 * np.arange(64)[None, :] + 1) / 128)
+    cm[1:] *= math.sqrt(2.0)
+    planes = xyb.cpu().numpy()
+    xbk = 8 * ((w + 63) // 64)
+    res = {}
+    for m, name in ((0, "dct64_fp32_butterflies"), (1, "dct64_mfma_bf16x3")):
+        r = time_kernel(lambda: fe.dct64_image(xyb, w, h, m, out=outs[m]), stream, reps,
+                        24.0 * px)
+        flops = 3 * nblk * 2 * 3 * 2 * 64 ** 3  # 3 channels, 2 products, 3 split terms
+        if m == 1:
+            r["mfma_tflops"] = round(flops / (r["ms"] * 1e-3) / 1e12, 1)
+            r["mfma_frac_bf16_peak"] = round(flops / (r["ms"] * 1e-3) / 2.5e15, 4)
+        got = [o.cpu().numpy().reshape(-1, xbk, 64) for o in outs[m]]
+        worst = 0.0
+        for (tx, ty) in [(0, 0), (3, 5), (7, 2), (12, 9), (31, 31), (20, 27)]:
+            if tx * 64 >= w or ty * 64 >= h:
+                continue
+            for c in range(3):
+                x = planes[c, ty * 64:ty * 64 + 64, tx * 64:tx * 64 + 64].astype(np.float64)
+                want = (cm @ x @ cm.T / 4096.0).T.reshape(-1)
+                g = np.concatenate([got[c][ty * 8 + j, tx * 8:tx * 8 + 8, :].reshape(512)
+                                    for j in range(8)])
+                worst = max(worst, float(np.abs(g - want).max() / np.abs(want).max()))
+        r["max_err_vs_fp64_rel_to_max_coef"] = float("%.3g" % worst)
+        res[name] = r
+    del outs
+    torch.cuda.synchronize()
+    return res
+
+
 def host_path_rate(fe, rgb, params, reps=5):
     """PCIe-inclusive rate of the host-buffer entry point (jxg_front_end):
     pageable host RGB in, every output copied back to pageable host memory.
@@ -236,6 +277,7 @@ def main():
         fe.gaborish_inverse(xyb, w, h, fused=bool(p.fused), out=gab_out)
     extra["gaborish"] = time_kernel(gab, stream, reps, EXTRA_BYTES_PER_PX["gaborish"] * w * h)
     del gab_out
+    extra.update(dct64_forms(fe, xyb, w, h, stream, reps))
 
     px = w * h
     dominant = max(stage_ms, key=stage_ms.get)

@@ -297,6 +297,23 @@ int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channe
  * asserts on such cells (src/dc_group_data.rs:219-226); this ABI reports. */
 int jxg_quantize_status(jxg_ctx *ctx, uint32_t out[2]);
 
+/* DCT64X64 of every 64x64 block of the padded planes (ceil(W/64) x
+ * ceil(H/64) blocks, edge replication past the image), the large-transform
+ * workload: method JXG_DCT64_PLAIN runs the reference's FP32 butterflies in
+ * its operation order (dct64x64_scalar_input, src/dct.rs:2549; bit-identical
+ * to the quantize stage's DCT64X64); JXG_DCT64_MFMA_BF16X3 computes
+ * C X C^T / 4096 on the bf16 matrix cores with split-precision operands
+ * (hi + lo bf16, three products, f32 accumulation), an approximation whose
+ * error against FP64 the caller measures.  d_out[c] receives
+ * 64 * (8 ceil(W/64)) * (8 ceil(H/64)) floats: the DCT64X64 coefficient
+ * storage (u-major) in the chunk layout of jxg_forward_quantize_dev with
+ * xblocks = 8 ceil(W/64).  No counterpart entry point in the reference (its
+ * DCT64 runs per transform inside write_ac_group, src/group.rs:1534-1640). */
+enum { JXG_DCT64_PLAIN = 0, JXG_DCT64_MFMA_BF16X3 = 1 };
+int jxg_dct64_image_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                        uint32_t width, uint32_t height, int method, int fused,
+                        float *d_out[3]);
+
 /* Gaborish pre-inversion of the three XYB planes: a normalised symmetric
  * 5x5 sharpening with mirror-reflected image edges, so that the decoder's
  * gaborish smoothing reproduces the input.  Replaces gaborish_inverse

@@ -271,6 +271,18 @@ class FrontEnd:
                                              1 if fused else 0))
         return out
 
+    def dct64_image(self, xyb, width, height, method=0, fused=True, out=None):
+        """DCT64X64 of every 64x64 block (method 0: FP32 butterflies in the
+        reference's order, 1: split-bf16 x3 on the matrix cores).  Returns 3
+        float tensors of 64 * (8 ceil(W/64)) * (8 ceil(H/64)) coefficients in
+        the quantize-stage chunk layout with xblocks = 8 ceil(W/64)."""
+        n = 64 * 64 * ((width + 63) // 64) * ((height + 63) // 64)
+        out = [self.empty(n, self.torch.float32) for _ in range(3)] if out is None else out
+        arr = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in out])
+        check(lib().jxg_dct64_image_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
+                                        height, int(method), 1 if fused else 0, arr))
+        return out
+
     def quantize_status(self):
         out = (ctypes.c_uint32 * 2)()
         check(lib().jxg_quantize_status(self.ctx, out))

@@ -23,6 +23,7 @@ EXPORTED = [
     "jxg_ctx_set_tables", "jxg_xyb_dev", "jxg_fill_quant_field_dev", "jxg_strategy_costs_dev",
     "jxg_select_strategy_dev", "jxg_forward_quantize_dev", "jxg_front_end_dev", "jxg_front_end",
     "jxg_quantize_status", "jxg_gaborish_inverse_dev", "jxg_fill_cmap_dev",
+    "jxg_dct64_image_dev",
 ]
 
 
@@ -112,6 +113,8 @@ def _declare(L):
     L.jxg_front_end.argtypes = [vp, vp, c.c_size_t, c.c_int, c.c_uint32, c.c_uint32,
                                 c.POINTER(FrameParams), vp, vp, vp, vp, vp, vp, vp]
     L.jxg_quantize_status.argtypes = [vp, vp]
+    L.jxg_dct64_image_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
+                                      c.c_int, c.c_int, vp]
     L.jxg_fill_cmap_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
                                     c.POINTER(FrameParams), vp, vp]
     L.jxg_gaborish_inverse_dev.argtypes = [vp, vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,

@@ -24,7 +24,7 @@ DEVICE = ["--offload-arch=gfx950", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-HIP_SOURCES = ["jxg_kernels.hip", "jxg_filters.hip", "jxg_cfl.hip"]
+HIP_SOURCES = ["jxg_kernels.hip", "jxg_filters.hip", "jxg_cfl.hip", "jxg_large.hip"]
 
 
 def run(cmd):

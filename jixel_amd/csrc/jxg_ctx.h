@@ -59,6 +59,7 @@ struct jxg_ctx {
   size_t tmp_cap = 0;
   int8_t *d_cmap = nullptr;  // CfL maps when the caller does not want them
   size_t cmap_cap = 0;
+  void *d_c64 = nullptr;  // split-bf16 DCT64 matrix (hi, lo), built on first use
 };
 
 template <typename T>

@@ -1672,6 +1672,7 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_rgb);
   (void)hipFree(ctx->d_tmp);
   (void)hipFree(ctx->d_cmap);
+  (void)hipFree(ctx->d_c64);
   delete ctx;
 }
 

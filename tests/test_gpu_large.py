@@ -1,0 +1,82 @@
+"""DCT64X64 over whole images: the plain FP32 butterfly form against the
+oracle bit for bit, and the split-bf16 x3 matrix-core form against the FP64
+closed form Y = C X C^T / 4096 (stated tolerance 2e-5 of the block's largest
+coefficient; the measured error is printed)."""
+import math
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import J, fe, torch, padded, same_bits, first_mismatch, log  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+
+def C64():
+    k = np.arange(64)[:, None]
+    i = np.arange(64)[None, :]
+    m = np.cos(np.pi * k * (2 * i + 1) / 128)
+    m[1:] *= math.sqrt(2.0)
+    return m
+
+
+def block_coeffs(out, w, h, tx, ty):
+    """The 4096 storage-order coefficients of 64x64 block (tx, ty)."""
+    xb = 8 * ((w + 63) // 64)
+    o = out.reshape(-1, xb, 64)  # rows of blocks, block, 64 values
+    parts = [o[ty * 8 + j, tx * 8:tx * 8 + 8, :].reshape(512) for j in range(8)]
+    return np.concatenate(parts)
+
+
+def image(w, h, seed):
+    from jixel_amd.synth import photo_like
+    from oracle import oracle as O
+    rgb = photo_like(w, h, seed=seed)
+    return O.xyb_image(rgb, J_mod().XYB_SPEC_FWD)
+
+
+def J_mod():
+    import jixel_amd
+    return jixel_amd
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("w,h", [(64, 64), (200, 136), (130, 70)])
+def test_dct64_plain_bit_exact(J, fe, torch, oracle, w, h, fused):
+    log("dct64 plain", w, h, fused)
+    oracle.set_fused(fused)
+    xyb = image(w, h, 5)
+    pad = padded(J, xyb)
+    out = [o.cpu().numpy() for o in fe.dct64_image(torch.from_numpy(pad).cuda(), w, h, 0, fused)]
+    for ty in range((h + 63) // 64):
+        for tx in range((w + 63) // 64):
+            for c in range(3):
+                want = oracle.forward_transform(14, np.ascontiguousarray(pad[c]), tx * 64, ty * 64)
+                got = block_coeffs(out[c], w, h, tx, ty)
+                assert same_bits(got, want), (tx, ty, c, first_mismatch(got, want))
+    oracle.set_fused(True)
+
+
+@pytest.mark.parametrize("w,h", [(64, 64), (256, 192)])
+def test_dct64_mfma_split_bf16_error(J, fe, torch, oracle, w, h):
+    xyb = image(w, h, 6)
+    pad = padded(J, xyb)
+    out = [o.cpu().numpy() for o in fe.dct64_image(torch.from_numpy(pad).cuda(), w, h, 1)]
+    plain = [o.cpu().numpy() for o in fe.dct64_image(torch.from_numpy(pad).cuda(), w, h, 0)]
+    Cm = C64()
+    worst = worst_plain = 0.0
+    for ty in range((h + 63) // 64):
+        for tx in range((w + 63) // 64):
+            for c in range(3):
+                x = pad[c, ty * 64:ty * 64 + 64, tx * 64:tx * 64 + 64].astype(np.float64)
+                y = Cm @ x @ Cm.T / 4096.0  # [v, u]
+                want = y.T.reshape(-1)  # storage u * 64 + v
+                scale = np.abs(want).max()
+                got = block_coeffs(out[c], w, h, tx, ty).astype(np.float64)
+                gp = block_coeffs(plain[c], w, h, tx, ty).astype(np.float64)
+                worst = max(worst, np.abs(got - want).max() / scale)
+                worst_plain = max(worst_plain, np.abs(gp - want).max() / scale)
+    log(f"dct64 {w}x{h}: max error / max |coef| vs FP64: split-bf16x3 {worst:.3e}, "
+        f"FP32 butterflies {worst_plain:.3e}")
+    assert worst <= 2e-5
+    assert worst_plain <= 1e-5

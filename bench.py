@@ -35,6 +35,7 @@ STAGE_BYTES_PER_PX = {
 
 EXTRA_BYTES_PER_PX = {
     "gaborish": 12.0 + 12.0,  # 3 planes in, 3 planes out
+    "epf_proxy_compare": 8.0,  # source + candidate luma (the pyramids are 1/4 and 1/16 of that)
 }
 
 CAL_BYTES = 256 << 20  # calibration copy size for the byte counters
@@ -276,6 +277,13 @@ def main():
     def gab():
         fe.gaborish_inverse(xyb, w, h, fused=bool(p.fused), out=gab_out)
     extra["gaborish"] = time_kernel(gab, stream, reps, EXTRA_BYTES_PER_PX["gaborish"] * w * h)
+    # EPF proxy: source = XYB luma, candidate = the gaborish-sharpened luma
+    proxy = J.LumaProxy(fe, xyb[1, :h, :w])
+    cand = gab_out[1, :h, :w]
+    extra["epf_proxy_compare"] = time_kernel(lambda: proxy.compare(cand), stream, reps,
+                                             EXTRA_BYTES_PER_PX["epf_proxy_compare"] * w * h)
+    extra["epf_proxy_compare"]["note"] = "host-synchronous call (score read back)"
+    proxy.close()
     del gab_out
     extra.update(dct64_forms(fe, xyb, w, h, stream, reps))
 

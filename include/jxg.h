@@ -314,6 +314,26 @@ int jxg_dct64_image_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint3
                         uint32_t width, uint32_t height, int method, int fused,
                         float *d_out[3]);
 
+/* ---- EPF luma proxy ----
+ * The six-term score the EPF fit ranks reconstructions with: structure pools
+ * of the half- and quarter-resolution luma (5x5 mirrored box moments) and
+ * error / excess / fourth-power pools of adjacent-pixel gradients against the
+ * source.  Per-pixel values follow the reference's f32 order; pooled sums are
+ * accumulated in f64 (the reference uses one f32 running sum). */
+typedef struct jxg_epf_proxy jxg_epf_proxy;
+/* LumaProxy::new (src/epf_proxy.rs:151; callers src/epf_filter.rs:832,1098):
+ * copies the W x H source luma (row_stride floats per row; W, H >= 4, the
+ * reference asserts) and builds the reference scales on the context's
+ * stream. */
+int jxg_epf_proxy_new(jxg_ctx *ctx, const float *d_y, uint32_t row_stride, uint32_t width,
+                      uint32_t height, jxg_epf_proxy **out);
+void jxg_epf_proxy_destroy(jxg_epf_proxy *proxy);
+/* LumaProxy::compare (src/epf_proxy.rs:173): score and, if terms is not
+ * NULL, the six terms {quarter structure, half structure, half structure
+ * L4, gradient error, gradient excess, gradient loss L4}.  Synchronizes. */
+int jxg_epf_proxy_compare(jxg_epf_proxy *proxy, const float *d_y, uint32_t row_stride,
+                          float *score, float terms[6]);
+
 /* Gaborish pre-inversion of the three XYB planes: a normalised symmetric
  * 5x5 sharpening with mirror-reflected image edges, so that the decoder's
  * gaborish smoothing reproduces the input.  Replaces gaborish_inverse

@@ -20,7 +20,7 @@ from . import _lib
 from ._lib import FrameParams, JxgError, check, lib
 
 __all__ = [
-    "Speed", "EncodeConfig", "FrameParams", "JxgError", "FrontEnd", "FrontEndOutput",
+    "Speed", "EncodeConfig", "FrameParams", "JxgError", "FrontEnd", "FrontEndOutput", "LumaProxy",
     "distance_from_quality", "frame_params", "default_quant_tables", "srgb_lut",
     "plane_geometry", "qt_offset", "qt_size", "encode_front_end", "XYB_SPEC_FWD",
     "STRATEGY",
@@ -322,6 +322,39 @@ class FrontEnd:
         o["ytox"] = ytox
         o["ytob"] = ytob
         return o
+
+
+class LumaProxy:
+    """The EPF ranking proxy on the GPU (LumaProxy, src/epf_proxy.rs:151):
+    LumaProxy(front_end, y) with y a (H, W) float32 device tensor (or a 2-D
+    view with unit column stride); compare(candidate) -> (score, terms)."""
+
+    def __init__(self, fe, y):
+        self.fe = fe
+        h, w = y.shape
+        self.shape = (h, w)
+        self.ptr = ctypes.c_void_p()
+        check(lib().jxg_epf_proxy_new(fe.ctx, _ptr(y), y.stride(0), w, h, ctypes.byref(self.ptr)))
+
+    def compare(self, y):
+        if tuple(y.shape) != self.shape:
+            raise JxgError(-1, f"candidate shape {tuple(y.shape)} != {self.shape}")
+        score = ctypes.c_float()
+        terms = np.zeros(6, np.float32)
+        check(lib().jxg_epf_proxy_compare(self.ptr, _ptr(y), y.stride(0), ctypes.byref(score),
+                                          terms.ctypes.data))
+        return float(score.value), terms
+
+    def close(self):
+        if self.ptr:
+            lib().jxg_epf_proxy_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def encode_front_end(rgb, config: EncodeConfig = EncodeConfig(), device=0,

@@ -23,7 +23,7 @@ EXPORTED = [
     "jxg_ctx_set_tables", "jxg_xyb_dev", "jxg_fill_quant_field_dev", "jxg_strategy_costs_dev",
     "jxg_select_strategy_dev", "jxg_forward_quantize_dev", "jxg_front_end_dev", "jxg_front_end",
     "jxg_quantize_status", "jxg_gaborish_inverse_dev", "jxg_fill_cmap_dev",
-    "jxg_dct64_image_dev",
+    "jxg_dct64_image_dev", "jxg_epf_proxy_new", "jxg_epf_proxy_destroy", "jxg_epf_proxy_compare",
 ]
 
 
@@ -115,6 +115,10 @@ def _declare(L):
     L.jxg_quantize_status.argtypes = [vp, vp]
     L.jxg_dct64_image_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
                                       c.c_int, c.c_int, vp]
+    L.jxg_epf_proxy_new.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.POINTER(vp)]
+    L.jxg_epf_proxy_destroy.argtypes = [vp]
+    L.jxg_epf_proxy_destroy.restype = None
+    L.jxg_epf_proxy_compare.argtypes = [vp, vp, c.c_uint32, c.POINTER(c.c_float), vp]
     L.jxg_fill_cmap_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
                                     c.POINTER(FrameParams), vp, vp]
     L.jxg_gaborish_inverse_dev.argtypes = [vp, vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,

@@ -24,7 +24,8 @@ DEVICE = ["--offload-arch=gfx950", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-HIP_SOURCES = ["jxg_kernels.hip", "jxg_filters.hip", "jxg_cfl.hip", "jxg_large.hip"]
+HIP_SOURCES = ["jxg_kernels.hip", "jxg_filters.hip", "jxg_cfl.hip", "jxg_large.hip",
+               "jxg_epf.hip"]
 
 
 def run(cmd):

@@ -133,6 +133,17 @@ void jxo_gaborish_weights(float mul, float w[6]);
  * (src/gaborish.rs:44-77). */
 void jxo_gaborish_inverse(float *xyb, int width, int height, float mul);
 
+/* ---- EPF luma proxy (LumaProxy, src/epf_proxy.rs:151-245) ---- */
+typedef struct jxo_epf jxo_epf;
+/* Reference scales of a W x H luma plane (W, H >= 4). */
+jxo_epf *jxo_epf_new(const float *y, int w, int h);
+void jxo_epf_free(jxo_epf *p);
+/* Six terms and the weighted score of a candidate plane; pooled sums in
+ * pixel order in f32 (wide = 0, the reference) or in f64 (wide = 1). */
+float jxo_epf_compare(const jxo_epf *p, const float *y, int wide, float terms[6]);
+/* The 5x5 mirrored box mean (original_box5, src/epf_proxy/box_filter.rs:466). */
+void jxo_epf_box5(const float *src, int w, int h, float *dst);
+
 /* Whole front end on the CPU (XYB -> AQ -> CfL maps -> selection ->
  * quantize) with `threads` worker threads; used as the benchmark's CPU
  * baseline.  ytox / ytob receive the CfL maps (ceil(H/64) x ceil(W/64)). */

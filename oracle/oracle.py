@@ -75,6 +75,12 @@ def _declare(L):
                                       _u8p]
     L.jxo_strategy_costs.argtypes = [_f32p, c.c_int, c.c_int, _u8p, _i8p, _i8p, _f32p, c.c_void_p,
                                      _f32p, _f32p, _f32p]
+    L.jxo_epf_new.restype = c.c_void_p
+    L.jxo_epf_new.argtypes = [_f32p, c.c_int, c.c_int]
+    L.jxo_epf_free.argtypes = [c.c_void_p]
+    L.jxo_epf_compare.restype = c.c_float
+    L.jxo_epf_compare.argtypes = [c.c_void_p, _f32p, c.c_int, _f32p]
+    L.jxo_epf_box5.argtypes = [_f32p, c.c_int, c.c_int, _f32p]
     L.jxo_fill_cmap.argtypes = [_f32p, c.c_int, c.c_int, _f32p, c.c_void_p, _i8p, _i8p]
     L.jxo_cmap_tile.argtypes = [_f32p, c.c_int, c.c_int, c.c_int, c.c_int, _f32p, c.c_void_p,
                                 c.POINTER(c.c_int), c.POINTER(c.c_int)]
@@ -265,4 +271,27 @@ def gaborish_inverse(xyb, mul=0.9908511):
     out = np.ascontiguousarray(xyb, np.float32).copy()
     _, h, w = out.shape
     lib().jxo_gaborish_inverse(_p(out, _f32p), w, h, mul)
+    return out
+
+
+def epf_compare(y_ref, y_cand, wide=False):
+    """(score, terms[6]) of LumaProxy::new(y_ref).compare(y_cand)."""
+    y_ref = np.ascontiguousarray(y_ref, np.float32)
+    y_cand = np.ascontiguousarray(y_cand, np.float32)
+    h, w = y_ref.shape
+    L = lib()
+    p = L.jxo_epf_new(_p(y_ref, _f32p), w, h)
+    try:
+        terms = np.zeros(6, np.float32)
+        score = L.jxo_epf_compare(p, _p(y_cand, _f32p), 1 if wide else 0, _p(terms, _f32p))
+        return float(np.float32(score)), terms
+    finally:
+        L.jxo_epf_free(p)
+
+
+def epf_box5(src):
+    src = np.ascontiguousarray(src, np.float32)
+    h, w = src.shape
+    out = np.zeros_like(src)
+    lib().jxo_epf_box5(_p(src, _f32p), w, h, _p(out, _f32p))
     return out

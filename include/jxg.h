@@ -202,7 +202,10 @@ int jxg_ctx_set_tables(jxg_ctx *ctx, const float *blob);
 
 /* ---- device stages ---- */
 
-/* u8 sRGB (3 or 4 channels, alpha ignored) -> linear -> XYB planes.
+/* u8 sRGB (3 or 4 channels, alpha ignored; 1 or 2 channels = gray (+ alpha),
+ * decoded to equal R = G = B as linearize_gray does, src/encode_image.rs:1571,
+ * and snapped to X = 0, B = Y as snap_achromatic_xyb does,
+ * src/frame/encoder.rs:351) -> linear -> XYB planes.
  * Replaces linearize_rgb (src/encode_image.rs:1498) followed by
  * to_xyb_in_place_with_fn (src/xyb.rs:439) with the selected ToXybBandFn
  * (src/xyb.rs:347).  `fwd` is XybMatrix::fwd (src/xyb.rs:81-96); `lut` the

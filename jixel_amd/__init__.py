@@ -359,7 +359,8 @@ class LumaProxy:
 
 def encode_front_end(rgb, config: EncodeConfig = EncodeConfig(), device=0,
                      tables=None) -> FrontEndOutput:
-    """Host-buffer front end of one RGB(A)8 image (H, W, 3|4) on `device`.
+    """Host-buffer front end of one 8-bit image on `device`: (H, W, 3|4) RGB(A)
+    or (H, W[, 1|2]) gray (+ alpha), the gray path of encode_image_gray.
 
     The drop-in counterpart of the front-end half of jixel's encode_image /
     encode_image_with_alpha (src/encode_image.rs:1610,3482): the returned
@@ -367,8 +368,10 @@ def encode_front_end(rgb, config: EncodeConfig = EncodeConfig(), device=0,
     consumes.
     """
     rgb = np.ascontiguousarray(rgb, np.uint8)
-    if rgb.ndim != 3 or rgb.shape[2] not in (3, 4):
-        raise JxgError(-1, f"expected (H, W, 3|4) uint8, got {rgb.shape}")
+    if rgb.ndim == 2:
+        rgb = rgb[:, :, None]
+    if rgb.ndim != 3 or rgb.shape[2] not in (1, 2, 3, 4):
+        raise JxgError(-1, f"expected (H, W[, 1|2|3|4]) uint8, got {rgb.shape}")
     h, w, ch = rgb.shape
     if h == 0 or w == 0:
         raise JxgError(-2, "empty image")

@@ -66,7 +66,10 @@ __global__ __launch_bounds__(256) void k_xyb(const uint8_t *__restrict__ rgb, si
   if (x >= stride) return;
   const int sx = min(x, W - 1), sy = min(y, H - 1);
   const uint8_t *p = rgb + (size_t)sy * row_bytes + (size_t)sx * ch;
-  const float r = lut[p[0]], g = lut[p[1]], b = lut[p[2]];
+  // Gray (+ alpha) input decodes one sample to all three channels
+  // (linearize_gray, src/encode_image.rs:1571-1599).
+  const bool gray = ch <= 2;
+  const float r = lut[p[0]], g = gray ? r : lut[p[1]], b = gray ? r : lut[p[2]];
   const float bias = 0.0037930734f, neg = -0.1559542f;
   float tm[3];
 #pragma unroll
@@ -94,9 +97,10 @@ __global__ __launch_bounds__(256) void k_xyb(const uint8_t *__restrict__ rgb, si
   }
   const size_t plane = (size_t)stride * padded_h;
   const size_t i = (size_t)y * stride + x;
-  xyb[i] = X;
+  // snap_achromatic_xyb (src/frame/encoder.rs:347-357): X = 0, B = Y.
+  xyb[i] = gray ? 0.0f : X;
   xyb[plane + i] = Y;
-  xyb[2 * plane + i] = tm[2];
+  xyb[2 * plane + i] = gray ? Y : tm[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -1695,7 +1699,7 @@ int jxg_xyb_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channe
   int rc = check_image(width, height);
   if (rc) return rc;
   if ((rc = check_planes(width, height, stride, padded_h))) return rc;
-  if (channels != 3 && channels != 4) return set_err(JXG_ERR_INVALID_ARG, "channels %d", channels);
+  if (channels < 1 || channels > 4) return set_err(JXG_ERR_INVALID_ARG, "channels %d", channels);
   if (row_bytes < (size_t)width * channels) return set_err(JXG_ERR_INVALID_ARG, "row_bytes too small");
   if (order != JXG_XYB_SCALAR && order != JXG_XYB_AVX2) return set_err(JXG_ERR_INVALID_ARG, "order %d", order);
   if (!ctx->mlut_valid || memcmp(ctx->h_mlut, fwd, 9 * sizeof(float)) != 0 ||
@@ -1917,7 +1921,7 @@ int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channe
     return set_err(JXG_ERR_INVALID_ARG, "null argument");
   int rc = check_image(width, height);
   if (rc) return rc;
-  if (channels != 3 && channels != 4) return set_err(JXG_ERR_INVALID_ARG, "channels %d", channels);
+  if (channels < 1 || channels > 4) return set_err(JXG_ERR_INVALID_ARG, "channels %d", channels);
   const size_t xb = (width + 7) / 8, yb = (height + 7) / 8, nb = xb * yb;
   const size_t tiles = (size_t)((width + 63) / 64) * ((height + 63) / 64);
   const size_t in_bytes = (size_t)height * row_bytes;

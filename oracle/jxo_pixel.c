@@ -349,7 +349,15 @@ void jxo_xyb_image(const uint8_t *rgb, size_t row_bytes, int channels, int width
     for (int x = 0; x < width; x++) {
       const uint8_t *px = row + (size_t)x * channels;
       float o[3];
-      jxo_xyb_pixel(m, order, lut[px[0]], lut[px[1]], lut[px[2]], o);
+      if (channels <= 2) {
+        /* linearize_gray (src/encode_image.rs:1571-1599), then
+         * snap_achromatic_xyb (src/frame/encoder.rs:347-357) */
+        jxo_xyb_pixel(m, order, lut[px[0]], lut[px[0]], lut[px[0]], o);
+        o[0] = 0.0f;
+        o[2] = o[1];
+      } else {
+        jxo_xyb_pixel(m, order, lut[px[0]], lut[px[1]], lut[px[2]], o);
+      }
       size_t i = (size_t)y * width + x;
       xyb[i] = o[0];
       xyb[plane + i] = o[1];

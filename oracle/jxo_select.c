@@ -692,7 +692,13 @@ static void *fe_worker(void *arg) {
         for (int x = 0; x < W; x++) {
           const uint8_t *px = row + (size_t)x * j->channels;
           float o[3];
-          jxo_xyb_pixel(m, JXG_XYB_SCALAR, lut[px[0]], lut[px[1]], lut[px[2]], o);
+          if (j->channels <= 2) { /* gray: linearize_gray + snap_achromatic_xyb */
+            jxo_xyb_pixel(m, JXG_XYB_SCALAR, lut[px[0]], lut[px[0]], lut[px[0]], o);
+            o[0] = 0.0f;
+            o[2] = o[1];
+          } else {
+            jxo_xyb_pixel(m, JXG_XYB_SCALAR, lut[px[0]], lut[px[1]], lut[px[2]], o);
+          }
           size_t k = (size_t)y * W + x;
           j->xyb[k] = o[0];
           j->xyb[plane + k] = o[1];

@@ -76,7 +76,8 @@ SIZES = [(1, 1), (7, 9), (65, 33), (100, 75), (256, 256)]
 
 
 @pytest.mark.parametrize("order,fused", [(0, True), (0, False), (1, True)])
-@pytest.mark.parametrize("w,h,ch", [(1, 1, 3), (7, 9, 4), (65, 33, 3), (300, 70, 4)])
+@pytest.mark.parametrize("w,h,ch", [(1, 1, 3), (7, 9, 4), (65, 33, 3), (300, 70, 4), (40, 30, 1),
+                                    (33, 17, 2)])
 def test_xyb_bit_exact(J, fe, torch, oracle, w, h, ch, order, fused):
     log("xyb", w, h, ch, order, fused)
     rgb = rand_rgb(h, w, ch, seed=w * 31 + h)
@@ -311,6 +312,23 @@ def test_front_end_bit_exact(J, fe, torch, oracle, name, w, h, q, fused):
         assert same_bits(o["levels"][c].cpu().numpy(), want["levels"][c]), c
         assert same_bits(o["coeffs"][c].cpu().numpy(), want["coeffs"][c]), c
         assert same_bits(o["dc"][c].cpu().numpy(), want["dc"][c]), c
+
+
+@pytest.mark.parametrize("ch", [1, 2])
+def test_front_end_gray_bit_exact(J, fe, torch, oracle, ch):
+    """Gray (+ alpha) input: the achromatic path (X = 0, B = Y) end to end."""
+    from jixel_amd import synth
+    rgb = synth.photo_like(136, 72, 8)
+    gray = rgb[:, :, :1].copy() if ch == 1 else rgb[:, :, :2].copy()
+    p = J.frame_params(J.EncodeConfig().with_quality(80.0))
+    want = oracle.front_end(gray, p, J.default_quant_tables(), threads=2)
+    o = fe.run(to_dev(torch, gray), 136, 72, p)
+    for k in ("qf", "cells", "ytox", "ytob"):
+        assert same_bits(o[k].cpu().numpy(), want[k]), k
+    for c in range(3):
+        assert same_bits(o["levels"][c].cpu().numpy(), want["levels"][c]), c
+        assert same_bits(o["dc"][c].cpu().numpy(), want["dc"][c]), c
+    assert not o["levels"][0].cpu().numpy().any()  # X carries nothing
 
 
 def test_host_front_end_matches_device_path(J, oracle):

@@ -267,3 +267,21 @@ def test_dct64_dc_is_the_block_mean(oracle, strategy):
     bw, bh = w // 8, h // 8
     means = img.astype(np.float64).reshape(bh, 8, bw, 8).mean(axis=(1, 3)).reshape(-1)
     assert np.abs(dc[:bw * bh] - means).max() < 2e-3, (dc[:bw * bh], means)
+
+
+def test_gray_input_is_snapped_achromatic(oracle):
+    """Gray samples decode to R = G = B (linearize_gray) and snap to X = 0,
+    B = Y (snap_achromatic_xyb, src/frame/encoder.rs:347-357); Y equals the
+    Y of the same gray as RGB."""
+    import jixel_amd as J
+    rng = np.random.default_rng(9)
+    g = rng.integers(0, 256, (5, 7, 1), dtype=np.uint8)
+    ga = np.concatenate([g, rng.integers(0, 256, (5, 7, 1), dtype=np.uint8)], axis=2)
+    rgb = np.repeat(g, 3, axis=2)
+    x1 = oracle.xyb_image(g, J.XYB_SPEC_FWD)
+    x2 = oracle.xyb_image(ga, J.XYB_SPEC_FWD)
+    x3 = oracle.xyb_image(rgb, J.XYB_SPEC_FWD)
+    assert np.array_equal(x1, x2)
+    assert not x1[0].any() and np.array_equal(x1[2], x1[1])
+    assert np.array_equal(x1[1], x3[1])
+    assert np.abs(x3[0]).max() < 1e-6 and np.abs(x3[2] - x3[1]).max() < 1e-6

@@ -884,30 +884,42 @@ __global__ __launch_bounds__(SEL_T) void k_select(const float *__restrict__ xyb,
 #define QT_PITCH 65
 #define QT_MAXT 64
 
+#define QT_SUB8_SLOTS 12
+
+// Sized so that four workgroups fit the 160 KB of LDS of a CU.
 struct QLds {
   float y2[QT_TILE * QT_PITCH];   // dequantized luma with zeroed LLF (CfL source)
   float cur[QT_TILE * QT_PITCH];  // the channel being transformed
   int n;
-  int s[QT_MAXT];
-  int bx[QT_MAXT], by[QT_MAXT];  // global first block
-  int lx[QT_MAXT], ly[QT_MAXT];  // LDS origin (pixels)
-  int pre1[QT_MAXT + 1];         // prefix of pass-1 lines (one channel)
-  int pre2[QT_MAXT + 1];         // prefix of pass-2 lines
-  int prec[QT_MAXT + 1];         // prefix of 64-coefficient chunks
-  int16_t ydc[QT_MAXT][16];
+  uint8_t s[QT_MAXT];
+  uint16_t bx[QT_MAXT], by[QT_MAXT];  // global first block
+  uint8_t lx[QT_MAXT], ly[QT_MAXT];   // LDS origin (pixels)
+  uint16_t pre1[QT_MAXT + 1];         // prefix of pass-1 lines (one channel)
+  uint16_t pre2[QT_MAXT + 1];         // prefix of pass-2 lines
+  uint16_t prec[QT_MAXT + 1];         // prefix of 64-coefficient chunks
+  int16_t ydc[64];                    // quantized Y DC per 8x8 block of the window
   float xf[QT_MAXT], bf[QT_MAXT];
-  int quant[QT_MAXT];
+  uint8_t quant[QT_MAXT];
   float inv_qac[QT_MAXT];  // 1 / (scale * quant), src/group.rs:806
   float quot[64];          // 0, 1 - one, numerator / a for a >= 2 (see dequant_level_tab)
   uint8_t line1_t[512], line2_t[512];  // line -> transform, per pass
   uint8_t chunk_t[64];                 // 64-coefficient chunk -> transform
-  float sub8_px[32][64];               // pixel copies for sub-8x8 transforms
-  int n64;                             // DCT64-family transforms in the list (<= 2)
+  union {
+    float sub8_px[QT_SUB8_SLOTS][64];  // pixel copies for sub-8x8 transforms
+    struct {
+      float dcs[2][64];    // DCT64 DC: inverse-transform scratch
+      float dc64v[2][64];  // DCT64 DC values of the current channel
+    };
+  };
+  int n64;  // DCT64-family transforms in the list (<= 2)
   int t64[2];
-  float dcs[2][64];                    // DCT64 DC: inverse-transform scratch
-  float dc64v[2][64];                  // DCT64 DC values of the current channel
-  int16_t ydc64[2][64];                // quantized Y DC of the DCT64 transforms
+  int16_t ydc64[2][64];  // quantized Y DC of the DCT64 transforms
 };
+
+// Index of the 8x8 block of the window holding DC value i of transform t.
+__device__ __forceinline__ int win_block(const QLds &L, int t, int i, int cvx) {
+  return ((L.ly[t] >> 3) + i / cvx) * 8 + (L.lx[t] >> 3) + i % cvx;
+}
 
 __device__ __forceinline__ int find_item(const int *pre, int n, int e) {
   int lo = 0, hi = n - 1;
@@ -1092,9 +1104,9 @@ __device__ __forceinline__ void load_plane(QLds &L, const QArgs &A, int c, int p
 template <bool F, bool BIG>
 __device__ void transform_plane(QLds &L, int n) {
   const int tid = threadIdx.x, nthr = blockDim.x;
-  // 32 threads, each owning one pixel-copy slot, take the sub-8x8 blocks.
-  if (tid < 32) {
-    for (int t = tid; t < n; t += 32) {
+  // A few threads, each owning one pixel-copy slot, take the sub-8x8 blocks.
+  if (tid < QT_SUB8_SLOTS) {
+    for (int t = tid; t < n; t += QT_SUB8_SLOTS) {
       const int s = L.s[t];
       if (is_sub8(s))
         sub8_transform<F>(s, &L.cur[L.ly[t] * QT_PITCH + L.lx[t]], QT_PITCH, L.sub8_px[tid]);
@@ -1283,7 +1295,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     for (int i = 0; i < 16; i++) {
       if (i < covered) {
         const int16_t q = sat_i16(round_away(dcv[i] * D->inv_factor[1]));
-        L.ydc[t][i] = q;
+        L.ydc[win_block(L, t, i, cvx)] = q;
         A.dc1[(size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx] = q;
       }
     }
@@ -1405,7 +1417,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
 #pragma unroll
       for (int i = 0; i < 16; i++) {
         if (i < covered) {
-          const float yq = (float)L.ydc[t][i];
+          const float yq = (float)L.ydc[win_block(L, t, i, cvx)];
           int16_t qd;
           if (c == 0) {
             if (D->cfl_factor_x != 0.0f)
@@ -1424,7 +1436,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
 }
 
 template <bool F>
-__global__ __launch_bounds__(256) void k_quantize(QArgs A) {
+__global__ __launch_bounds__(256, 4) void k_quantize(QArgs A) {
   __shared__ QLds L;
   const int tid = threadIdx.x;
   const int bx0 = blockIdx.x * 8, by0 = blockIdx.y * 8;

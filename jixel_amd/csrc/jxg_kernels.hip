@@ -196,23 +196,93 @@ __device__ __forceinline__ void store_min4(float v, float *m) {
   }
 }
 
+// Per-block modulations (fill_quant_field_scalar stage 3,
+// src/adaptive_quant.rs:692-729) for a strip of AQ_STRIP blocks of one block
+// row.  Map, then fold: every pixel's gamma / HF / blue term is computed in
+// parallel (rows across the strip, so loads coalesce), then one lane per
+// (block, modulation) folds the terms in the reference's loop order, and one
+// lane per block finishes the block.  The per-pixel terms and every sum keep
+// the reference's f32 operation order.
+#define AQ_STRIP 32
+
 template <bool F>
-__global__ __launch_bounds__(64) void k_aq_block(const float *__restrict__ xyb, int stride,
-                                                 int padded_h, int W, int H,
-                                                 const float *__restrict__ pre, int pre_w,
-                                                 float distance, float inv_scale,
-                                                 const float *__restrict__ exp2_table,
-                                                 uint8_t *__restrict__ qf, int xb, int yb) {
-  const int bx = blockIdx.x * 64 + threadIdx.x;
-  const int by = blockIdx.y;
+__global__ __launch_bounds__(256) void k_aq_block(const float *__restrict__ xyb, int stride,
+                                                  int padded_h, int W, int H,
+                                                  const float *__restrict__ pre, int pre_w,
+                                                  float distance, float inv_scale,
+                                                  const float *__restrict__ exp2_table,
+                                                  uint8_t *__restrict__ qf, int xb, int yb) {
+  __shared__ float tg[8][AQ_STRIP * 8];  // gamma pair sum, [dy][block * 8 + dx]
+  __shared__ float thr_[8][AQ_STRIP * 8];  // HF term to the right
+  __shared__ float thd[8][AQ_STRIP * 8];   // HF term downwards
+  __shared__ float tbl[8][AQ_STRIP * 8];   // blue term
+  __shared__ float sums[3][AQ_STRIP];
+  const int tid = threadIdx.x;
+  const int bx0 = blockIdx.x * AQ_STRIP, by = blockIdx.y;
+  const size_t plane = (size_t)stride * padded_h;
+  const float *px_ = xyb, *py_ = xyb + plane, *pb_ = xyb + 2 * plane;
+  const int y = min(by * 8, max(H - 8, 0));
+  // ---- map: thread = (block, dx) of the strip, one row dy at a time ----
+  {
+    const int b = tid >> 3, dx = tid & 7, bx = bx0 + b;
+    if (bx < xb) {
+      const int x = min(bx * 8, max(W - 8, 0));
+      const int cx = min(x + dx, W - 1), cr = min(x + dx + 1, W - 1);
+      const float k_limit = 0.010474084867598155f;
+#pragma unroll 2
+      for (int dy = 0; dy < 8; dy++) {
+        const int ry = min(y + dy, H - 1);
+        const int rn = dy == 7 ? ry : min(y + dy + 1, H - 1);
+        const size_t i = (size_t)ry * stride + cx;
+        const float vy = py_[i], vx = px_[i], vb = pb_[i];
+        // gamma_modulation (src/adaptive_quant.rs:410-435)
+        const float iny = vy + 0.16f, inx = vx;
+        tg[dy][tid] = ratio_gamma<true>(iny - inx) + ratio_gamma<true>(iny + inx);
+        // hf_modulation (:127-160)
+        thr_[dy][tid] = fminf(0.0206f, fabsf(vy - py_[(size_t)ry * stride + cr]));
+        thd[dy][tid] = fminf(0.0206f, fabsf(vy - py_[(size_t)rn * stride + cx]));
+        // blue_modulation (:163-206)
+        const float p_y = vy + 0.0031994768654636393f + fabsf(vx);
+        tbl[dy][tid] = vb > p_y ? fminf(vb - p_y, k_limit) : 0.0f;
+      }
+    }
+  }
+  __syncthreads();
+  // ---- fold: lane (which, block) runs one sum in the reference's order ----
+  if (tid < 3 * AQ_STRIP) {
+    const int which = tid / AQ_STRIP, b = tid - which * AQ_STRIP;
+    if (bx0 + b < xb) {
+      float acc = 0.0f;
+      if (which == 0) {
+        for (int dy = 0; dy < 8; dy++)
+#pragma unroll
+          for (int dx = 0; dx < 8; dx++) acc = fmla<F>(0.5f, tg[dy][b * 8 + dx], acc);
+      } else if (which == 1) {
+        for (int dy = 0; dy < 8; dy++)
+#pragma unroll
+          for (int dx = 0; dx < 8; dx++) {
+            if (dx < 7) acc += thr_[dy][b * 8 + dx];
+            acc += thd[dy][b * 8 + dx];
+          }
+      } else {
+        for (int dy = 0; dy < 8; dy++)
+#pragma unroll
+          for (int dx = 0; dx < 8; dx++) acc += tbl[dy][b * 8 + dx];
+      }
+      sums[which][b] = acc;
+    }
+  }
+  __syncthreads();
+  if (tid >= AQ_STRIP) return;
+  const int bx = bx0 + tid;
   if (bx >= xb || by >= yb) return;
   const float d = distance;
   // DC group of this block and its pre-map extent.
   const int gx = bx >> 8, gy = by >> 8;
   const int gxb = min(256, xb - gx * 256), gyb = min(256, yb - gy * 256);
   const int pw = 2 * gxb, ph = 2 * gyb;
-  const int px0 = gx * 512, py0 = gy * 512;
-  // FuzzyErosion weights
+  const int ppx0 = gx * 512, ppy0 = gy * 512;
+  // FuzzyErosion (src/adaptive_quant.rs:437-470, 640-690)
   const float fe_mul = d < 2.0f ? (2.0f - d) * 0.5f : 0.0f;
   const float fe_base[4] = {0.125f, 0.1f, 0.09f, 0.06f};
   const float fe_add[4] = {0.0f, -0.1f, -0.09f, -0.06f};
@@ -232,9 +302,9 @@ __global__ __launch_bounds__(64) void k_aq_block(const float *__restrict__ xyb, 
     const int fx = 2 * (bx - gx * 256) + (q & 1);
     const int ym1 = fy >= 1 ? fy - 1 : fy, yp1 = fy + 1 < ph ? fy + 1 : fy;
     const int xm1 = fx >= 1 ? fx - 1 : fx, xp1 = fx + 1 < pw ? fx + 1 : fx;
-    const float *rt = pre + (size_t)(py0 + ym1) * pre_w + px0;
-    const float *rm = pre + (size_t)(py0 + fy) * pre_w + px0;
-    const float *rb = pre + (size_t)(py0 + yp1) * pre_w + px0;
+    const float *rt = pre + (size_t)(ppy0 + ym1) * pre_w + ppx0;
+    const float *rm = pre + (size_t)(ppy0 + fy) * pre_w + ppx0;
+    const float *rb = pre + (size_t)(ppy0 + yp1) * pre_w + ppx0;
     float mins[4] = {rm[fx], rm[xm1], rm[xp1], rt[xm1]};
     sort4(mins);
     store_min4(rt[fx], mins);
@@ -245,12 +315,7 @@ __global__ __launch_bounds__(64) void k_aq_block(const float *__restrict__ xyb, 
     const float v = kmul[0] * mins[0] + kmul[1] * mins[1] + kmul[2] * mins[2] + kmul[3] * mins[3];
     aq = q == 0 ? v : aq + v;
   }
-  // Stage 3: per-block modulations.
-  const size_t plane = (size_t)stride * padded_h;
-  const float *px_ = xyb, *py_ = xyb + plane, *pb_ = xyb + 2 * plane;
-  const int x = min(bx * 8, max(W - 8, 0));
-  const int y = min(by * 8, max(H - 8, 0));
-  // compute_mask
+  // compute_mask (:67-84)
   float mask;
   {
     const float k_offset3 = 3.7179635626140772f;
@@ -263,17 +328,7 @@ __global__ __launch_bounds__(64) void k_aq_block(const float *__restrict__ xyb, 
   }
   // gamma_modulation
   {
-    float overall = 0.0f;
-    for (int dy = 0; dy < 8; dy++) {
-      const int ry = min(y + dy, H - 1);
-      for (int dx = 0; dx < 8; dx++) {
-        const int cx = min(x + dx, W - 1);
-        const size_t i = (size_t)ry * stride + cx;
-        const float iny = py_[i] + 0.16f;
-        const float inx = px_[i];
-        overall = fmla<F>(0.5f, ratio_gamma<true>(iny - inx) + ratio_gamma<true>(iny + inx), overall);
-      }
-    }
+    float overall = sums[0][tid];
     overall *= 1.0f / 64.0f;
     mask = fmla<F>(0.119146f, dirty_log2f<F>(overall), mask);
   }
@@ -281,36 +336,12 @@ __global__ __launch_bounds__(64) void k_aq_block(const float *__restrict__ xyb, 
   float out_val;
   {
     const float strength = fminf(fmaxf(1.06836f * d, 0.1f), 0.890398f);
-    float sum_y = 0.0f;
-    for (int dy = 0; dy < 8; dy++) {
-      const int ry = min(y + dy, H - 1);
-      const int rn = dy == 7 ? ry : min(y + dy + 1, H - 1);
-      const float *row = py_ + (size_t)ry * stride;
-      const float *nxt = py_ + (size_t)rn * stride;
-      for (int dx = 0; dx < 8; dx++) {
-        const int c0 = min(x + dx, W - 1);
-        const float p = row[c0];
-        if (dx < 7) sum_y += fminf(0.0206f, fabsf(p - row[min(x + dx + 1, W - 1)]));
-        sum_y += fminf(0.0206f, fabsf(p - nxt[c0]));
-      }
-    }
-    out_val = fmla<F>(sum_y, -0.506626f * strength, 0.655866f) + mask;
+    out_val = fmla<F>(sums[1][tid], -0.506626f * strength, 0.655866f) + mask;
   }
   // blue_modulation
   {
     const float k_limit = 0.010474084867598155f;
-    float sum = 0.0f;
-    for (int dy = 0; dy < 8; dy++) {
-      const int ry = min(y + dy, H - 1);
-      for (int dx = 0; dx < 8; dx++) {
-        const int cx = min(x + dx, W - 1);
-        const size_t i = (size_t)ry * stride + cx;
-        const float p_x = px_[i], p_b = pb_[i];
-        const float p_y = py_[i] + 0.0031994768654636393f + fabsf(p_x);
-        if (p_b > p_y) sum += fminf(p_b - p_y, k_limit);
-      }
-    }
-    float s = sum;
+    float s = sums[2][tid];
     if (s >= 32.0f * k_limit) s = 64.0f * k_limit - s;
     const float k_max = 15.463398341612438f;
     if (s >= k_max * k_limit) s = k_max * k_limit;
@@ -1747,11 +1778,11 @@ static int fill_qf(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t p
   else
     k_aq_pre<false><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, ctx->d_pre);
   LAUNCH_CHECK("k_aq_pre");
-  const dim3 g2((xb + 63) / 64, yb);
+  const dim3 g2((xb + AQ_STRIP - 1) / AQ_STRIP, yb);
   if (fused)
-    k_aq_block<true><<<g2, 64, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, ctx->d_pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
+    k_aq_block<true><<<g2, 256, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, ctx->d_pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
   else
-    k_aq_block<false><<<g2, 64, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, ctx->d_pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
+    k_aq_block<false><<<g2, 256, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, ctx->d_pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
   LAUNCH_CHECK("k_aq_block");
   return JXG_OK;
 }

@@ -2,12 +2,14 @@
 // src/color_correlation.rs:901-960, regression path compute_cmap_tile
 // :441-508, the path XYB frames take at Fast and faster).
 //
-//   k_cmap  one 256-thread workgroup per 64x64-px tile: the three planes of
-//           the tile are staged in LDS, every 8x8 block gets its three DCT8s
-//           as 1-D lines (rows, then columns with the 1/64 scale), one thread
-//           per block runs the regression sums in coefficient order, and one
-//           thread accumulates the blocks in raster order and solves the two
-//           slopes, so every sum keeps the reference's sequential f32 order.
+//   k_cmap_blocks  one 128-thread workgroup per quarter tile (64 x 16 px): the
+//                  three planes staged in LDS, every 8x8 block's three DCT8s
+//                  as 1-D lines (rows, then columns with the 1/64 scale), and
+//                  the block's four regression sums, one lane each, in
+//                  coefficient order.
+//   k_cmap_solve   one wavefront per tile: the block sums in raster order and
+//                  the two slopes.  Every sum keeps the reference's
+//                  sequential f32 order.
 #include <hip/hip_runtime.h>
 
 #include "jxg_ctx.h"
@@ -45,23 +47,28 @@ __device__ __forceinline__ int solve_multiplier(float ca, float cb, int num, flo
   return (int)r;
 }
 
+// A quarter tile (64 px wide, 16 px tall = 2 block rows of 8 blocks) per
+// 128-thread workgroup: DCT8s of the three planes in LDS, then four lanes per
+// block run the four regression sums (cfl_regression_scalar,
+// src/color_correlation.rs:333-357) in coefficient order.  Block sums go to
+// `part`; k_cmap_solve adds a tile's blocks in raster order and solves.
+#define CM_ROWS 16
+#define CM_T 128
+
 template <bool F>
-__global__ __launch_bounds__(256) void k_cmap(const float *__restrict__ xyb, int stride,
-                                              int padded_h, int W, int H, int tiles_x,
-                                              const float *__restrict__ qm_x,
-                                              const float *__restrict__ qm_b, CmapArgs a,
-                                              int8_t *__restrict__ ytox,
-                                              int8_t *__restrict__ ytob) {
-  __shared__ float t[3][64 * CM_P];  // X, Y, B
-  __shared__ float sums[64][4];
-  __shared__ int valid[64];
+__global__ __launch_bounds__(CM_T) void k_cmap_blocks(const float *__restrict__ xyb, int stride,
+                                                      int padded_h, int W, int H,
+                                                      const float *__restrict__ qm_x,
+                                                      const float *__restrict__ qm_b, CmapArgs a,
+                                                      float *__restrict__ part) {
+  __shared__ float t[3][CM_ROWS * CM_P];  // X, Y, B
   const int tid = threadIdx.x;
-  const int tx = blockIdx.x, ty = blockIdx.y;
-  const int x0 = tx * 64, y0 = ty * 64;
+  const int tx = blockIdx.x, ty = blockIdx.y >> 2, quarter = blockIdx.y & 3;
+  const int x0 = tx * 64, y0 = ty * 64 + quarter * CM_ROWS;
   const size_t plane = (size_t)stride * padded_h;
-  // stride and padded_h are multiples of 64: the whole tile is in bounds.
-  for (int e = tid; e < 3 * 64 * 16; e += 256) {
-    const int c = e / (64 * 16), r = (e / 16) % 64, q = e % 16;
+  // stride and padded_h are multiples of 64: the whole quarter tile is in bounds.
+  for (int e = tid; e < 3 * CM_ROWS * 16; e += CM_T) {
+    const int c = e / (CM_ROWS * 16), r = (e / 16) % CM_ROWS, q = e % 16;
     const float4 v =
         *reinterpret_cast<const float4 *>(xyb + c * plane + (size_t)(y0 + r) * stride + x0 + 4 * q);
     float *d = &t[c][r * CM_P + 4 * q];
@@ -72,69 +79,78 @@ __global__ __launch_bounds__(256) void k_cmap(const float *__restrict__ xyb, int
   }
   __syncthreads();
   // DCT8 rows (src/dct.rs:631-660: rows first, columns scaled by 1/64)
-  for (int e = tid; e < 3 * 64 * 8; e += 256) {
-    const int c = e / 512, rem = e % 512, b = rem >> 3, r = rem & 7;
+  constexpr int NB = CM_ROWS / 8 * 8;  // blocks per workgroup
+  for (int e = tid; e < 3 * NB * 8; e += CM_T) {
+    const int c = e / (NB * 8), rem = e % (NB * 8), b = rem >> 3, r = rem & 7;
     dct_line<F, 8>(&t[c][((b >> 3) * 8 + r) * CM_P + (b & 7) * 8], 1, 0.0f, false);
   }
   __syncthreads();
-  for (int e = tid; e < 3 * 64 * 8; e += 256) {
-    const int c = e / 512, rem = e % 512, b = rem >> 3, col = rem & 7;
+  for (int e = tid; e < 3 * NB * 8; e += CM_T) {
+    const int c = e / (NB * 8), rem = e % (NB * 8), b = rem >> 3, col = rem & 7;
     dct_line<F, 8>(&t[c][((b >> 3) * 8) * CM_P + (b & 7) * 8 + col], CM_P, 1.0f / 64.0f, true);
   }
   __syncthreads();
-  if (tid < 64) {
-    const int bx = tid & 7, by = tid >> 3;
-    const int px = x0 + bx * 8, py = y0 + by * 8;
-    const bool ok = px + 8 <= W && py + 8 <= H;
-    valid[tid] = ok;
-    if (ok) {
-      const float K_INV = 1.0f / 84.0f;
-      const float sx = -a.base_x, sb = 1.0f - a.base_b;
-      float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-      // coefficient k = u * 8 + v of the square DCT8 sits at LDS (v, u)
-      for (int k = 0; k < 64; k++) {
-        const int at = (by * 8 + (k & 7)) * CM_P + bx * 8 + (k >> 3);
-        float y = t[1][at], x = t[0][at], b = t[2][at];
-        if (k == 0) y = x = b = 0.0f;  // the regression ignores the DC
-        if (sx != 0.0f) x = fmla<F>(sx, y, x);
-        if (sb != 0.0f) b = fmla<F>(sb, y, b);
-        // cfl_regression_scalar (src/color_correlation.rs:333-357)
-        const float m_x = y * qm_x[k], s_x = x * qm_x[k];
-        const float a_x = K_INV * m_x, b_x = -s_x;
-        s0 = fmla<F>(a_x, a_x, s0);
-        s1 = fmla<F>(a_x, b_x, s1);
-        const float m_b = y * qm_b[k], s_b = b * qm_b[k];
-        const float a_b = K_INV * m_b, b_b = fmla<F>(1.0f, m_b, -s_b);
-        s2 = fmla<F>(a_b, a_b, s2);
-        s3 = fmla<F>(a_b, b_b, s3);
-      }
-      sums[tid][0] = s0;
-      sums[tid][1] = s1;
-      sums[tid][2] = s2;
-      sums[tid][3] = s3;
+  if (tid < 4 * NB) {
+    const int b = tid >> 2, which = tid & 3;
+    const int bx = b & 7, by = b >> 3;
+    const float K_INV = 1.0f / 84.0f;
+    const float shift = which < 2 ? -a.base_x : 1.0f - a.base_b;
+    const float *oc = which < 2 ? t[0] : t[2];
+    float acc = 0.0f;
+    // coefficient k = u * 8 + v of the square DCT8 sits at LDS (v, u); the
+    // lanes of a block differ only in selects, so the wavefront stays uniform
+#pragma unroll 16
+    for (int k = 0; k < 64; k++) {
+      const int at = (by * 8 + (k & 7)) * CM_P + bx * 8 + (k >> 3);
+      float y = t[1][at], o = oc[at];
+      if (k == 0) y = o = 0.0f;  // the regression ignores the DC
+      const float fo = fmla<F>(shift, y, o);
+      o = shift != 0.0f ? fo : o;
+      const float qm = which < 2 ? qm_x[k] : qm_b[k];
+      const float m = y * qm, s_o = o * qm;
+      const float am = K_INV * m;
+      // which 0 / 2: a * a; 1: a * (-s_x); 3: a * (m_b - s_b) (fmla(1, m, -s) is exact)
+      const float second = (which & 1) ? (which == 1 ? -s_o : fmla<F>(1.0f, m, -s_o)) : am;
+      acc = fmla<F>(am, second, acc);
     }
+    const int tiles_x = gridDim.x;
+    part[((size_t)(ty * tiles_x + tx) * 64 + quarter * NB + b) * 4 + which] = acc;
   }
+}
+
+// The tile's block sums in raster order, then the two slopes
+// (compute_cmap_tile, src/color_correlation.rs:470-508).  One wavefront per
+// tile: lane b loads block b's four sums, lane 0 adds them in order.
+template <bool F>
+__global__ __launch_bounds__(64) void k_cmap_solve(const float *__restrict__ part, int W, int H,
+                                                   int tiles_x, int tiles_y, CmapArgs a,
+                                                   int8_t *__restrict__ ytox,
+                                                   int8_t *__restrict__ ytob) {
+  __shared__ float4 s4[64];
+  const int ti = blockIdx.x, lane = threadIdx.x;
+  const int tx = ti % tiles_x, ty = ti / tiles_x;
+  s4[lane] = reinterpret_cast<const float4 *>(part)[(size_t)ti * 64 + lane];
   __syncthreads();
-  if (tid == 0) {
-    const int xb = (W + 7) / 8, yb = (H + 7) / 8;
-    const int bxc = min(8, xb - tx * 8), byc = min(8, yb - ty * 8);
-    float ca_x = 0.0f, cb_x = 0.0f, ca_b = 0.0f, cb_b = 0.0f;
-    int num = 0;
-    for (int by = 0; by < byc; by++)
-      for (int bx = 0; bx < bxc; bx++) {
-        const int b = by * 8 + bx;
-        if (!valid[b]) continue;
-        ca_x += sums[b][0];
-        cb_x += sums[b][1];
-        ca_b += sums[b][2];
-        cb_b += sums[b][3];
-        num += 64;
-      }
-    const float dz = cfl_deadzone(a.distance);
-    const float units = a.color_factor / 84.0f;
-    ytox[ty * tiles_x + tx] = (int8_t)solve_multiplier<F>(ca_x, cb_x, num, dz, units);
-    ytob[ty * tiles_x + tx] = (int8_t)solve_multiplier<F>(ca_b, cb_b, num, dz, units);
-  }
+  if (lane != 0) return;
+  const int xb = (W + 7) / 8, yb = (H + 7) / 8;
+  const int bxc = min(8, xb - tx * 8), byc = min(8, yb - ty * 8);
+  float ca_x = 0.0f, cb_x = 0.0f, ca_b = 0.0f, cb_b = 0.0f;
+  int num = 0;
+  for (int by = 0; by < byc; by++)
+    for (int bx = 0; bx < bxc; bx++) {
+      const int px = tx * 64 + bx * 8, py = ty * 64 + by * 8;
+      if (px + 8 > W || py + 8 > H) continue;
+      const float4 s = s4[by * 8 + bx];
+      ca_x += s.x;
+      cb_x += s.y;
+      ca_b += s.z;
+      cb_b += s.w;
+      num += 64;
+    }
+  const float dz = cfl_deadzone(a.distance);
+  const float units = a.color_factor / 84.0f;
+  ytox[ti] = (int8_t)solve_multiplier<F>(ca_x, cb_x, num, dz, units);
+  ytob[ti] = (int8_t)solve_multiplier<F>(ca_b, cb_b, num, dz, units);
 }
 
 }  // namespace
@@ -143,19 +159,28 @@ JXG_HIDDEN int fill_cmap(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint
                          uint32_t width, uint32_t height, const jxg_frame_params *p,
                          int8_t *d_ytox, int8_t *d_ytob) {
   const int tiles_x = (int)((width + 63) / 64), tiles_y = (int)((height + 63) / 64);
+  const size_t need = (size_t)tiles_x * tiles_y * 64 * 4;
+  int rc = ensure(&ctx->d_tmp, &ctx->tmp_cap, need);
+  if (rc) return rc;
   const uint32_t off = jxg_qt_offset(JXG_QT_DCT8);
   const float *qm_x = ctx->d_tables + off + 3 * 64;  // inv_matrix(0) of DCT8
   const float *qm_b = ctx->d_tables + off + 5 * 64;  // inv_matrix(2)
   const CmapArgs a{p->distance, p->cfl_base_x, p->cfl_base_b, p->cfl_color_factor};
-  const dim3 grid(tiles_x, tiles_y);
-  if (p->fused)
-    k_cmap<true><<<grid, 256, 0, ctx->stream>>>(d_xyb, (int)stride, (int)padded_h, (int)width,
-                                                (int)height, tiles_x, qm_x, qm_b, a, d_ytox,
-                                                d_ytob);
-  else
-    k_cmap<false><<<grid, 256, 0, ctx->stream>>>(d_xyb, (int)stride, (int)padded_h, (int)width,
-                                                 (int)height, tiles_x, qm_x, qm_b, a, d_ytox,
-                                                 d_ytob);
+  const dim3 grid(tiles_x, 4 * tiles_y);
+  const unsigned sgrid = (unsigned)(tiles_x * tiles_y);
+  if (p->fused) {
+    k_cmap_blocks<true><<<grid, CM_T, 0, ctx->stream>>>(d_xyb, (int)stride, (int)padded_h,
+                                                       (int)width, (int)height, qm_x, qm_b, a,
+                                                       ctx->d_tmp);
+    k_cmap_solve<true><<<sgrid, 64, 0, ctx->stream>>>(ctx->d_tmp, (int)width, (int)height,
+                                                      tiles_x, tiles_y, a, d_ytox, d_ytob);
+  } else {
+    k_cmap_blocks<false><<<grid, CM_T, 0, ctx->stream>>>(d_xyb, (int)stride, (int)padded_h,
+                                                        (int)width, (int)height, qm_x, qm_b, a,
+                                                        ctx->d_tmp);
+    k_cmap_solve<false><<<sgrid, 64, 0, ctx->stream>>>(ctx->d_tmp, (int)width, (int)height,
+                                                       tiles_x, tiles_y, a, d_ytox, d_ytob);
+  }
   LAUNCH_CHECK("k_cmap");
   return JXG_OK;
 }

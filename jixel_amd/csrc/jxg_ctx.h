@@ -39,7 +39,7 @@ struct jxg_ctx {
   jxg_derived *h_derived = nullptr;  // pinned staging
   bool derived_valid = false;
   jxg_frame_params last_params;
-  uint32_t *d_scans = nullptr;  // 64 + 256 + 1024 natural scan positions
+  uint32_t *d_scans = nullptr;  // natural orders of DCT8/16/32: scan position -> storage index
   float *d_mlut = nullptr;      // fwd matrix (9) + transfer LUT (256)
   float *h_mlut = nullptr;
   bool mlut_valid = false;
@@ -48,7 +48,8 @@ struct jxg_ctx {
   size_t pre_cap = 0;
   float *d_xyb = nullptr;
   size_t xyb_cap = 0;
-  uint32_t *d_misc = nullptr;  // [0] crossing count, [1] unsupported flag
+  uint32_t *d_misc = nullptr;  // [0] crossing count, [1] unsupported flag (zero between
+                               // calls), [4..5] their last values, [6] finish ticket
   uint32_t *d_xt = nullptr;
   size_t xt_cap = 0;
   int8_t *d_zero_cmap = nullptr;

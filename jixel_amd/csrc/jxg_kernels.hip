@@ -518,27 +518,34 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
     L.qs[c][site] = qac_in[site] * qscale * D->qm_mult[c];
   }
   __syncthreads();
+  // Lanes take coefficients in scan order: within a site, e's low bits are
+  // the natural-scan position p and order[p] the storage index, so a
+  // wavefront's 64 lanes hold 64 consecutive scan positions of one site and
+  // the nonzero count and last nonzero position come from one ballot.
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const float *inv = tables + qoff + (3 + c) * qn;
-    const float *thr = D->thr[c][cls];
-    const float one = D->dequant_one[c], num = D->dequant_numerator;
+    const float thr0 = D->thr[c][cls][0], thr1 = D->thr[c][cls][1];
+    const float thr2 = D->thr[c][cls][2], thr3 = D->thr[c][cls][3];
+    const float num = D->dequant_numerator;
     float dd_r[4];
+    int kk_r[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int e = tid + SEL_T * j;
       const int site = __builtin_amdgcn_readfirstlane(e >> (2 * LG));
-      const int k = e & (size - 1);
+      const int p = e & (size - 1);
+      const int k = (int)scan[p];  // storage index of scan position p
       const int sy = site / sites_1d, sx = site - sy * sites_1d;
       const int u = k >> LG, v = k & (n - 1);  // square storage is u-major
       const int row = u, col = v;              // storage row / column
       const float coef = L.co[c][(sy * n + v) * SEL_PITCH + sx * n + u];
+      const float qsv = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.qs[c][site])));
       float dd = 0.0f, mgv = 0.0f;
-      uint32_t spv = 0;
+      bool nzq = false;
       if (!(col < cblk && row < cblk)) {
-        const int yfix = row >= half ? 2 : 0;
-        const float threshold = col >= half ? thr[yfix + 1] : thr[yfix];
-        const float a = inv[k] * L.qs[c][site] * coef;
+        const float threshold = row >= half ? (col >= half ? thr3 : thr2) : (col >= half ? thr1 : thr0);
+        const float a = inv[k] * qsv * coef;
         const float q = fabsf(a) >= threshold ? round_away(a) : 0.0f;
         const float aq = fabsf(q);
         // level_f32 with the extended quotient table (quot[0] = 0,
@@ -548,21 +555,22 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
         dd = diff * diff;
         if (aq != 0.0f) {
           mgv = aq < 256.0f ? L.rate[(int)aq] : rate_log2_slow(D, aq);
-          spv = scan[k] + 1;
+          nzq = true;
         }
       }
       dd_r[j] = dd;
-      L.mg[c][e] = mgv;
-      const uint32_t cnt = (uint32_t)__popcll(__ballot(spv != 0));
-      const uint32_t mx = wave_max_u32(spv);
-      if (lane == 0 && cnt) {
-        atomicAdd(&L.nz[site][c], cnt);
-        atomicMax(&L.maxscan[site][c], mx);
+      kk_r[j] = site * size + k;
+      L.mg[c][site * size + k] = mgv;
+      const uint64_t m = __ballot(nzq);
+      if (lane == 0 && m) {
+        const uint32_t p0 = (uint32_t)(e & (size - 1));  // lane 0's scan position
+        atomicAdd(&L.nz[site][c], (uint32_t)__popcll(m));
+        atomicMax(&L.maxscan[site][c], p0 + 64u - (uint32_t)__clzll(m));
       }
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < 4; j++) (&L.co[c][0])[tid + SEL_T * j] = dd_r[j];
+    for (int j = 0; j < 4; j++) (&L.co[c][0])[kk_r[j]] = dd_r[j];
   }
   __syncthreads();
   // Sums in the reference's sequential row-major storage order (f32).
@@ -570,13 +578,20 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
     const int site = wk / 6, rest = wk - site * 6, c = rest >> 1, which = rest & 1;
     const float4 *src = reinterpret_cast<const float4 *>(
         which == 0 ? &L.co[c][0] + site * size : &L.mg[c][0] + site * size);
+    // Four float4 loads per step keep the LDS reads ahead of the
+    // (necessarily sequential) f32 accumulation.
     float acc = 0.0f;
-    for (int k = 0; k < (size >> 2); k++) {
-      const float4 v4 = src[k];
-      acc += v4.x;
-      acc += v4.y;
-      acc += v4.z;
-      acc += v4.w;
+    for (int k0 = 0; k0 < (size >> 2); k0 += 4) {
+      float4 v4[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) v4[i] = src[k0 + i];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        acc += v4[i].x;
+        acc += v4[i].y;
+        acc += v4[i].z;
+        acc += v4[i].w;
+      }
     }
     if (which == 0)
       L.sse[site][c] = acc;
@@ -664,7 +679,7 @@ __device__ __forceinline__ float site_bias(const jxg_derived *D, double s1, doub
 }
 
 template <bool F>
-__global__ __launch_bounds__(SEL_T) void k_select(const float *__restrict__ xyb, int stride,
+__global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ xyb, int stride,
                                                   int padded_h, int W, int H,
                                                   uint8_t *__restrict__ qf,
                                                   const int8_t *__restrict__ ytox,
@@ -1058,6 +1073,7 @@ struct QArgs {
   float *co0, *co1, *co2;
   int16_t *dc0, *dc1, *dc2;
   uint32_t *xt_count, *xt_list;
+  uint32_t *status;  // [0..1] copy of {count, bad} for the caller, [2] finish ticket
   uint32_t xt_cap;
   uint32_t *bad;  // set to 1 when an unsupported strategy cell is seen
 };
@@ -1526,10 +1542,21 @@ __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
     quantize_list<F, true>(L, A, bx * 8, by * 8, cov_x(s) * 8, cov_y(s) * 8);
     __syncthreads();
   }
+  // The last workgroup to finish publishes the counters and clears them for
+  // the next call (every workgroup has read them by then), so no separate
+  // clearing launch is needed.
+  if (tid == 0) {
+    __threadfence();
+    const uint32_t ticket = atomicAdd(&A.status[2], 1u);
+    if (ticket == gridDim.x - 1) {
+      A.status[0] = atomicExch(A.xt_count, 0u);
+      A.status[1] = atomicExch(A.bad, 0u);
+      A.status[2] = 0u;
+      __threadfence();
+    }
+  }
 }
 
-// Clears a small counter block (a kernel rather than a memset node, so the
-// sequence can be captured into a graph).
 // Byte fill, used instead of hipMemsetAsync so that streams captured into a
 // graph hold kernel nodes only.
 __global__ void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
@@ -1544,10 +1571,6 @@ JXG_HIDDEN int fill_u8(jxg_ctx *ctx, uint8_t *p, size_t n, uint8_t v) {
   k_fill_u8<<<(unsigned)(blocks < 4096 ? blocks : 4096), 256, 0, ctx->stream>>>(p, n, v);
   LAUNCH_CHECK("k_fill_u8");
   return JXG_OK;
-}
-
-__global__ void k_zero_u32(uint32_t *p, int n) {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1665,7 +1688,7 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
       hipMalloc((void **)&ctx->d_scans, (64 + 256 + 1024) * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc((void **)&ctx->d_mlut, (9 + 256) * sizeof(float)) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_mlut, (9 + 256) * sizeof(float), hipHostMallocDefault) != hipSuccess ||
-      hipMalloc((void **)&ctx->d_misc, 4 * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_misc, 8 * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc((void **)&ctx->d_exp2, 64 * sizeof(float)) != hipSuccess)
     return fail(set_err(JXG_ERR_ALLOC, "context allocation failed"));
   // scan position tables of the square shapes, computed once
@@ -1677,7 +1700,7 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
     for (int i = 0; i < 3; i++) {
       const int cx = sides[i], n = cx * cx * 64;
       natural_scan(cx, cx, order);
-      for (int k = 0; k < n; k++) scans[off + order[k]] = (uint32_t)k;
+      for (int k = 0; k < n; k++) scans[off + k] = order[k];  // scan position -> storage index
       off += n;
     }
   }
@@ -1688,6 +1711,11 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
     for (int i = 0; i < 64; i++) e2[i] = (float)exp2((double)(i - 32) / 64.0);
     if (hipMemcpy(ctx->d_exp2, e2, sizeof(e2), hipMemcpyHostToDevice) != hipSuccess)
       return fail(set_err(JXG_ERR_DEVICE, "exp2 table upload failed"));
+  }
+  {
+    const uint32_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (hipMemcpy(ctx->d_misc, zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
+      return fail(set_err(JXG_ERR_DEVICE, "counter initialisation failed"));
   }
   float *blob = new float[JXG_QT_BLOB_FLOATS];
   jxg_default_quant_tables(blob);
@@ -1867,8 +1895,6 @@ int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, 
   // transform covers two blocks, and a crossing one at least two).
   const size_t cap = (size_t)xb * yb / 2 + 1;
   if ((rc = ensure(&ctx->d_xt, &ctx->xt_cap, cap))) return rc;
-  k_zero_u32<<<1, 64, 0, ctx->stream>>>(ctx->d_misc, 4);
-  LAUNCH_CHECK("k_zero_u32");
   QArgs A;
   A.xyb = d_xyb;
   A.stride = stride;
@@ -1892,8 +1918,9 @@ int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, 
   A.dc0 = d_dc[0];
   A.dc1 = d_dc[1];
   A.dc2 = d_dc[2];
-  A.xt_count = ctx->d_misc;
+  A.xt_count = ctx->d_misc;  // zero between calls (k_quantize_xt clears it)
   A.bad = ctx->d_misc + 1;
+  A.status = ctx->d_misc + 4;
   A.xt_list = ctx->d_xt;
   A.xt_cap = (uint32_t)cap;
   const dim3 grid((xb + 7) / 8, (yb + 7) / 8);
@@ -2025,8 +2052,8 @@ int jxg_front_end(jxg_ctx *ctx, const uint8_t *rgb, size_t row_bytes, int channe
 // strategy cell was seen (its blocks are left unwritten).  Synchronizes.
 int jxg_quantize_status(jxg_ctx *ctx, uint32_t out[2]) {
   if (!ctx || !out) return set_err(JXG_ERR_INVALID_ARG, "null argument");
-  uint32_t h[4];
-  HIP_TRY(hipMemcpyAsync(h, ctx->d_misc, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  uint32_t h[2];
+  HIP_TRY(hipMemcpyAsync(h, ctx->d_misc + 4, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   out[0] = h[0];
   out[1] = h[1];

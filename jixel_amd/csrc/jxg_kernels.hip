@@ -382,6 +382,7 @@ __global__ __launch_bounds__(256) void k_aq_block(const float *__restrict__ xyb,
 
 #define SEL_PITCH 33  // LDS row pitch of the 32x32 region (bank-conflict free)
 #define SEL_T 256     // threads per region
+#define SELQ_N (64 + 256 + 1024)  // scan-ordered inverse weights per channel
 #define RD_LAMBDA 0.08086717f
 #define META_R 4.0f
 
@@ -443,8 +444,8 @@ __device__ __noinline__ float rate_log2_slow(const jxg_derived *D, float aq) {
 template <bool F, int LG>
 __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cmb,
                              const float *__restrict__ tables, const uint32_t *__restrict__ scan,
-                             const jxg_derived *__restrict__ D, int W, int H, int px0, int py0,
-                             float *cost_out) {
+                             const float *__restrict__ selq, const jxg_derived *__restrict__ D,
+                             int W, int H, int px0, int py0, float *cost_out) {
   constexpr int n = 1 << LG;
   constexpr int sites_1d = 32 / n, nsites = sites_1d * sites_1d, size = n * n;
   constexpr int kind = n == 8 ? JXG_QT_DCT8 : (n == 16 ? JXG_QT_DCT16X16 : JXG_QT_DCT32X32);
@@ -524,7 +525,7 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
   // the nonzero count and last nonzero position come from one ballot.
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const float *inv = tables + qoff + (3 + c) * qn;
+    const float *invp = selq + c * SELQ_N;  // inverse weights by scan position
     const float thr0 = D->thr[c][cls][0], thr1 = D->thr[c][cls][1];
     const float thr2 = D->thr[c][cls][2], thr3 = D->thr[c][cls][3];
     const float num = D->dequant_numerator;
@@ -545,7 +546,7 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
       bool nzq = false;
       if (!(col < cblk && row < cblk)) {
         const float threshold = row >= half ? (col >= half ? thr3 : thr2) : (col >= half ? thr1 : thr0);
-        const float a = inv[k] * qsv * coef;
+        const float a = invp[p] * qsv * coef;
         const float q = fabsf(a) >= threshold ? round_away(a) : 0.0f;
         const float aq = fabsf(q);
         // level_f32 with the extended quotient table (quot[0] = 0,
@@ -686,6 +687,7 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ x
                                                   const int8_t *__restrict__ ytob,
                                                   const float *__restrict__ tables,
                                                   const uint32_t *__restrict__ scans,
+                                                  const float *__restrict__ selq,
                                                   const jxg_derived *__restrict__ D,
                                                   uint8_t *__restrict__ cells,
                                                   float *__restrict__ c8out,
@@ -801,12 +803,14 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ x
       }
       qac32[0] = scale * (float)aggregate_quant(max_q, sum, 16, dist);
     }
-    eval_squares<F, 3>(L, qac8, cmx, cmb, tables, scans, D, W, H, px0, py0, L.cost8);
+    eval_squares<F, 3>(L, qac8, cmx, cmb, tables, scans, selq, D, W, H, px0, py0, L.cost8);
     load_region(L, xyb, plane, stride, px0, py0);
-    eval_squares<F, 4>(L, qac16, cmx, cmb, tables, scans + 64, D, W, H, px0, py0, L.cost16);
+    eval_squares<F, 4>(L, qac16, cmx, cmb, tables, scans + 64, selq + 64, D, W, H, px0, py0,
+                       L.cost16);
     if (four_row && four_col) {
       load_region(L, xyb, plane, stride, px0, py0);
-      eval_squares<F, 5>(L, qac32, cmx, cmb, tables, scans + 320, D, W, H, px0, py0, &L.cost32);
+      eval_squares<F, 5>(L, qac32, cmx, cmb, tables, scans + 320, selq + 320, D, W, H, px0, py0,
+                         &L.cost32);
     }
     if (tid == 0) {
       // select_super_block at square scope for each super block, then the
@@ -1576,6 +1580,23 @@ JXG_HIDDEN int fill_u8(jxg_ctx *ctx, uint8_t *p, size_t n, uint8_t v) {
 // ---------------------------------------------------------------------------
 // Context and C ABI
 
+// Uploads a quant-table blob and the scan-ordered copy of the DCT8 /
+// DCT16X16 / DCT32X32 inverse weights the strategy search reads.
+static int upload_tables(jxg_ctx *ctx, const float *blob) {
+  HIP_TRY(hipMemcpy(ctx->d_tables, blob, JXG_QT_BLOB_FLOATS * sizeof(float), hipMemcpyHostToDevice));
+  static const int kinds[3] = {JXG_QT_DCT8, JXG_QT_DCT16X16, JXG_QT_DCT32X32};
+  static const int offs[3] = {0, 64, 320};
+  float selq[3 * SELQ_N];
+  for (int i = 0; i < 3; i++) {
+    const uint32_t off = jxg_qt_offset(kinds[i]), n = jxg_qt_size(kinds[i]);
+    for (int c = 0; c < 3; c++)
+      for (uint32_t p = 0; p < n; p++)
+        selq[c * SELQ_N + offs[i] + p] = blob[off + (3 + c) * n + ctx->h_order[offs[i] + p]];
+  }
+  HIP_TRY(hipMemcpy(ctx->d_selq, selq, sizeof(selq), hipMemcpyHostToDevice));
+  return JXG_OK;
+}
+
 // scan_pos_lut (src/coeff_order.rs:358-415) of the square shapes from the
 // natural order (natural_scan_generic, src/coeff_order.rs:206-248).
 static void natural_scan(int cx, int cy, uint32_t *out) {
@@ -1686,6 +1707,7 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
       hipMalloc((void **)&ctx->d_derived, sizeof(jxg_derived)) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_derived, sizeof(jxg_derived), hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void **)&ctx->d_scans, (64 + 256 + 1024) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_selq, 3 * SELQ_N * sizeof(float)) != hipSuccess ||
       hipMalloc((void **)&ctx->d_mlut, (9 + 256) * sizeof(float)) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_mlut, (9 + 256) * sizeof(float), hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void **)&ctx->d_misc, 8 * sizeof(uint32_t)) != hipSuccess ||
@@ -1701,6 +1723,7 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
       const int cx = sides[i], n = cx * cx * 64;
       natural_scan(cx, cx, order);
       for (int k = 0; k < n; k++) scans[off + k] = order[k];  // scan position -> storage index
+      for (int k = 0; k < n; k++) ctx->h_order[off + k] = order[k];
       off += n;
     }
   }
@@ -1719,10 +1742,9 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
   }
   float *blob = new float[JXG_QT_BLOB_FLOATS];
   jxg_default_quant_tables(blob);
-  const hipError_t e = hipMemcpy(ctx->d_tables, blob, JXG_QT_BLOB_FLOATS * sizeof(float),
-                                 hipMemcpyHostToDevice);
+  const int urc = upload_tables(ctx, blob);
   delete[] blob;
-  if (e != hipSuccess) return fail(set_err(JXG_ERR_DEVICE, "table upload failed"));
+  if (urc != JXG_OK) return fail(urc);
   *out = ctx;
   return JXG_OK;
 }
@@ -1736,6 +1758,7 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_derived);
   (void)hipHostFree(ctx->h_derived);
   (void)hipFree(ctx->d_scans);
+  (void)hipFree(ctx->d_selq);
   (void)hipFree(ctx->d_mlut);
   (void)hipHostFree(ctx->h_mlut);
   (void)hipFree(ctx->d_pre);
@@ -1759,8 +1782,9 @@ int jxg_ctx_synchronize(jxg_ctx *ctx) {
 
 int jxg_ctx_set_tables(jxg_ctx *ctx, const float *blob) {
   if (!ctx || !blob) return set_err(JXG_ERR_INVALID_ARG, "null argument");
-  HIP_TRY(hipMemcpy(ctx->d_tables, blob, JXG_QT_BLOB_FLOATS * sizeof(float), hipMemcpyHostToDevice));
-  return JXG_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));  // kernels in flight may read the tables
+  return upload_tables(ctx, blob);
 }
 
 int jxg_xyb_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channels,
@@ -1836,9 +1860,9 @@ static int select_impl(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32
   const int xb = (W + 7) / 8, yb = (H + 7) / 8;
   const dim3 grid((xb + 3) / 4, (yb + 3) / 4);
   if (p->fused)
-    k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+    k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
   else
-    k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+    k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
   LAUNCH_CHECK("k_select");
   return JXG_OK;
 }

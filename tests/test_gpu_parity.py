@@ -159,6 +159,32 @@ def test_strategy_selection_bit_exact(J, fe, torch, oracle, w, h, distance, fuse
     log("  strategies", np.bincount(cells_w.reshape(-1) >> 1, minlength=5)[:5])
 
 
+def test_caller_tables_reach_every_stage(J, torch, oracle):
+    """A caller's DequantMatrices (jxg_ctx_set_tables) drive selection and
+    quantization exactly as the oracle with the same tables."""
+    w, h = 200, 136
+    rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, 2.35, 12, True, cmap=True)
+    tables = J.default_quant_tables().copy()
+    rng = np.random.default_rng(4)
+    tables *= rng.uniform(0.8, 1.25, tables.shape).astype(np.float32)
+    f2 = J.FrontEnd(0, tables=tables)
+    try:
+        cells_w, qf_w = oracle.select_strategy(xyb, qf, ytox, ytob, tables, p)
+        qf_d = to_dev(torch, qf)
+        cells = f2.select_strategy(to_dev(torch, padded(J, xyb)), w, h, qf_d, p,
+                                   to_dev(torch, ytox), to_dev(torch, ytob))
+        assert same_bits(cells.cpu().numpy(), cells_w)
+        assert same_bits(qf_d.cpu().numpy(), qf_w)
+        lv_w, co_w, dc_w = oracle.forward_quantize(xyb, cells_w, qf_w, ytox, ytob, tables, p)
+        o = f2.forward_quantize(to_dev(torch, padded(J, xyb)), w, h, cells, qf_d, p,
+                                to_dev(torch, ytox), to_dev(torch, ytob))
+        for c in range(3):
+            assert same_bits(o["levels"][c].cpu().numpy(), lv_w[c]), c
+            assert same_bits(o["dc"][c].cpu().numpy(), dc_w[c]), c
+    finally:
+        f2.close()
+
+
 COV = {0: (1, 1), 1: (1, 2), 2: (2, 1), 3: (2, 2), 4: (4, 4), 8: (2, 4), 9: (4, 2), 22: (1, 4),
        23: (4, 1)}
 SUB8 = [5, 6, 7, 10, 11, 12, 13, 17, 18]  # DCT4X4, 4X8, 8X4, AFV0-3, IDENTITY, DCT2X2

@@ -139,9 +139,22 @@ typedef struct jxg_frame_params {
   int32_t global_scale;
   int32_t quant_dc;
   int32_t epf_iters;
-  int32_t reserved;
+  int32_t flags;  /* JXG_FLAG_* (alpha handling of 2- and 4-channel input) */
   float channel_weight[3]; /* EncodingContext::channel_weights (X, Y, B) */
 } jxg_frame_params;
+
+/* jxg_frame_params::flags.  Both are on after jxg_frame_params_init, as in
+ * EncodeConfig's defaults (src/encode_image.rs:682-740).  They act only on
+ * 2- and 4-channel input (gray + alpha, RGBA):
+ *   JXG_FLAG_LOSSY_ALPHA     alpha_fringe::clear on the alpha plane first
+ *                            (src/alpha_fringe.rs:77; the reference runs it
+ *                            unless alpha_distance <= 0, src/encode_image.rs:3650)
+ *   JXG_FLAG_FILL_INVISIBLE  fill_invisible on XYB (src/invisible.rs:34,
+ *                            src/frame/encoder.rs:152) and, for non-constant
+ *                            alpha, the alpha block factors on the final quant
+ *                            field (src/frame/encoder.rs:60-69,
+ *                            src/frame/dc_groups.rs:305). */
+enum { JXG_FLAG_FILL_INVISIBLE = 1, JXG_FLAG_LOSSY_ALPHA = 2 };
 
 /* Speed tiers (src/encode_image.rs Speed, src/effort.rs:305-441). */
 enum {
@@ -349,6 +362,48 @@ int jxg_epf_proxy_compare(jxg_epf_proxy *proxy, const float *d_y, uint32_t row_s
 int jxg_gaborish_inverse_dev(jxg_ctx *ctx, const float *d_src, float *d_dst, uint32_t stride,
                              uint32_t padded_h, uint32_t width, uint32_t height, float mul,
                              int fused);
+
+/* ---- alpha stages (2- and 4-channel input) ----
+ * jxg_front_end_dev runs these itself according to params->flags; they are
+ * exported one by one for callers that stage the frame themselves.  Alpha
+ * planes are W x H bytes, row stride W. */
+
+/* alpha_fringe::clear (src/alpha_fringe.rs:77) in place: alpha samples 1..4
+ * whose clipped 9x9 window of such samples has mean <= 1/2 become 0; then,
+ * when near-opaque samples (251..254) are at least 1/12 of the plane, the
+ * same sweep over 255 - alpha fills them with 255 if it changes at least
+ * 1/12 of the plane.  Callers: the RGBA and gray + alpha paths of
+ * encode_image_with_alpha (src/encode_image.rs:3650,3980). */
+int jxg_alpha_fringe_clear_dev(jxg_ctx *ctx, uint8_t *d_alpha, uint32_t width, uint32_t height);
+
+/* fill_invisible (src/invisible.rs:34) on the three planes, as called by
+ * encode_frame (src/frame/encoder.rs:152): every alpha-0 sample, in
+ * row-major order, becomes the weighted mean of its filled left and
+ * upper-right neighbours and its visible right, lower-right, upper and lower
+ * neighbours.  Bit-identical to the sequential scan.  The edge padding is
+ * rewritten from the filled edge samples. */
+int jxg_fill_invisible_dev(jxg_ctx *ctx, float *d_xyb, uint32_t stride, uint32_t padded_h,
+                           uint32_t width, uint32_t height, const uint8_t *d_alpha);
+
+/* alpha_block_factors (src/adaptive_quant.rs:953): per 8x8 block
+ * (ceil(W/8) x ceil(H/8) floats) the largest alpha / 255, 1 for fully
+ * transparent or opaque blocks, at least 0.1 otherwise. */
+int jxg_alpha_block_factors_dev(jxg_ctx *ctx, const uint8_t *d_alpha, uint32_t width,
+                                uint32_t height, float *d_factors);
+
+/* apply_alpha_factors (src/adaptive_quant.rs:1036) on the final quant field:
+ * every transform's quant value times the largest factor of its footprint,
+ * rounded and clamped to 1..255.  The reference applies it only for
+ * non-constant alpha (src/frame/encoder.rs:60-69; caller
+ * src/frame/dc_groups.rs:305). */
+int jxg_apply_alpha_factors_dev(jxg_ctx *ctx, const uint8_t *d_cells, const float *d_factors,
+                                uint32_t width, uint32_t height, uint8_t *d_qf);
+
+/* The alpha plane the last jxg_front_end_dev / jxg_front_end call used
+ * (after fringe clearing), W x H bytes, copied to dst (host memory when
+ * to_host, synchronously; device memory otherwise, stream-ordered).  The
+ * caller's CPU back end encodes this plane as the alpha channel. */
+int jxg_alpha_plane(jxg_ctx *ctx, uint8_t *dst, size_t bytes, int to_host);
 
 #ifdef __cplusplus
 }

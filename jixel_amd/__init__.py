@@ -63,11 +63,19 @@ class EncodeConfig:
     Default does (:682-740).  `fused` selects the reference build's fmla
     semantics (aarch64 / FMA-enabled builds fuse, generic x86-64 does not);
     `xyb_order` selects the scalar XYB kernel or the x86-64 AVX2 one.
+    `fill_invisible` (on by default, :693) fills the colour under alpha 0
+    and scales the quant field by the alpha block factors; `lossy_alpha`
+    (alpha_distance unset or > 0) clears the alpha fringe first.
     """
     distance: float = 1.0
     speed: Speed = Speed.FAST
     fused: bool = True
     xyb_order: int = 0
+    fill_invisible: bool = True
+    lossy_alpha: bool = True
+
+    def with_fill_invisible(self, on):
+        return replace(self, fill_invisible=bool(on))
 
     def with_quality(self, quality):
         return replace(self, distance=distance_from_quality(quality))
@@ -86,6 +94,8 @@ def frame_params(config: EncodeConfig) -> FrameParams:
     p = FrameParams()
     check(lib().jxg_frame_params_init(float(config.distance), int(config.speed),
                                       1 if config.fused else 0, ctypes.byref(p)))
+    p.flags = ((FLAG_FILL_INVISIBLE if config.fill_invisible else 0) |
+               (FLAG_LOSSY_ALPHA if config.lossy_alpha else 0))
     return p
 
 
@@ -136,6 +146,10 @@ def _spec_fwd():
 XYB_SPEC_FWD = _spec_fwd()
 
 
+FLAG_FILL_INVISIBLE = 1  # JXG_FLAG_FILL_INVISIBLE
+FLAG_LOSSY_ALPHA = 2     # JXG_FLAG_LOSSY_ALPHA
+
+
 @dataclass
 class FrontEndOutput:
     """Per-image front-end results in the layouts of include/jxg.h."""
@@ -146,6 +160,7 @@ class FrontEndOutput:
     levels: list        # 3 x int32 [yblocks*xblocks*64]
     coeffs: list        # 3 x float32 [yblocks*xblocks*64]
     dc: list            # 3 x int16 [yblocks*xblocks]
+    alpha: np.ndarray = None  # (H, W) uint8 alpha plane as the stages used it
 
 
 def _ptr(t):
@@ -283,6 +298,36 @@ class FrontEnd:
                                         height, int(method), 1 if fused else 0, arr))
         return out
 
+    def alpha_fringe_clear(self, alpha, width, height):
+        """alpha_fringe::clear of a (H, W) uint8 device plane, in place."""
+        check(lib().jxg_alpha_fringe_clear_dev(self.ctx, _ptr(alpha), width, height))
+        return alpha
+
+    def fill_invisible(self, xyb, width, height, alpha):
+        """fill_invisible of (3, padded_h, stride) planes under a (H, W) uint8
+        alpha plane, in place (edge padding rewritten)."""
+        check(lib().jxg_fill_invisible_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
+                                           height, _ptr(alpha)))
+        return xyb
+
+    def alpha_block_factors(self, alpha, width, height, out=None):
+        t = self.torch
+        out = self.empty(((height + 7) // 8, (width + 7) // 8), t.float32) if out is None else out
+        check(lib().jxg_alpha_block_factors_dev(self.ctx, _ptr(alpha), width, height, _ptr(out)))
+        return out
+
+    def apply_alpha_factors(self, cells, factors, width, height, qf):
+        """apply_alpha_factors on a device quant field, in place."""
+        check(lib().jxg_apply_alpha_factors_dev(self.ctx, _ptr(cells), _ptr(factors), width,
+                                                height, _ptr(qf)))
+        return qf
+
+    def alpha_plane(self, width, height):
+        """The alpha plane the last front end call used, as a device tensor."""
+        out = self.empty((height, width), self.torch.uint8)
+        check(lib().jxg_alpha_plane(self.ctx, _ptr(out), width * height, 0))
+        return out
+
     def quantize_status(self):
         out = (ctypes.c_uint32 * 2)()
         check(lib().jxg_quantize_status(self.ctx, out))
@@ -391,7 +436,11 @@ def encode_front_end(rgb, config: EncodeConfig = EncodeConfig(), device=0,
         check(lib().jxg_front_end(fe.ctx, rgb.ctypes.data, w * ch, ch, w, h, ctypes.byref(p),
                                   ytox.ctypes.data, ytob.ctypes.data, qf.ctypes.data,
                                   cells.ctypes.data, arr(levels), arr(coeffs), arr(dc)))
+        alpha = None
+        if ch in (2, 4):
+            alpha = np.zeros((h, w), np.uint8)
+            check(lib().jxg_alpha_plane(fe.ctx, alpha.ctypes.data, alpha.size, 1))
         return FrontEndOutput(qf=qf, cells=cells, ytox=ytox, ytob=ytob, levels=levels,
-                              coeffs=coeffs, dc=dc)
+                              coeffs=coeffs, dc=dc, alpha=alpha)
     finally:
         fe.close()

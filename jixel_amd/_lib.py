@@ -24,6 +24,8 @@ EXPORTED = [
     "jxg_select_strategy_dev", "jxg_forward_quantize_dev", "jxg_front_end_dev", "jxg_front_end",
     "jxg_quantize_status", "jxg_gaborish_inverse_dev", "jxg_fill_cmap_dev",
     "jxg_dct64_image_dev", "jxg_epf_proxy_new", "jxg_epf_proxy_destroy", "jxg_epf_proxy_compare",
+    "jxg_alpha_fringe_clear_dev", "jxg_fill_invisible_dev", "jxg_alpha_block_factors_dev",
+    "jxg_apply_alpha_factors_dev", "jxg_alpha_plane",
 ]
 
 
@@ -47,7 +49,7 @@ class FrameParams(ctypes.Structure):
         ("global_scale", ctypes.c_int32),
         ("quant_dc", ctypes.c_int32),
         ("epf_iters", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
         ("channel_weight", ctypes.c_float * 3),
     ]
 
@@ -123,6 +125,11 @@ def _declare(L):
                                     c.POINTER(FrameParams), vp, vp]
     L.jxg_gaborish_inverse_dev.argtypes = [vp, vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
                                            c.c_float, c.c_int]
+    L.jxg_alpha_fringe_clear_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32]
+    L.jxg_fill_invisible_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32, vp]
+    L.jxg_alpha_block_factors_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, vp]
+    L.jxg_apply_alpha_factors_dev.argtypes = [vp, vp, vp, c.c_uint32, c.c_uint32, vp]
+    L.jxg_alpha_plane.argtypes = [vp, vp, c.c_size_t, c.c_int]
 
 
 def check(rc):

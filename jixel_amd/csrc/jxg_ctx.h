@@ -63,6 +63,16 @@ struct jxg_ctx {
   int8_t *d_cmap = nullptr;  // CfL maps when the caller does not want them
   size_t cmap_cap = 0;
   void *d_c64 = nullptr;  // split-bf16 DCT64 matrix (hi, lo), built on first use
+  uint32_t *d_astat = nullptr;  // alpha-stage counters
+  uint8_t *d_alpha = nullptr;   // alpha plane of the last front end call
+  size_t alpha_cap = 0;
+  size_t alpha_used = 0;        // its size in bytes (0: that call had no alpha)
+  uint8_t *d_alpha2 = nullptr;  // fringe sweep outputs
+  size_t alpha2_cap = 0;
+  uint32_t *d_vis = nullptr;  // visibility bits
+  size_t vis_cap = 0;
+  float *d_afac = nullptr;  // alpha block factors
+  size_t afac_cap = 0;
 };
 
 template <typename T>
@@ -84,6 +94,13 @@ JXG_HIDDEN int fill_cmap(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint
                          uint32_t width, uint32_t height, const jxg_frame_params *p,
                          int8_t *d_ytox, int8_t *d_ytob);  // jxg_cfl.hip
 JXG_HIDDEN int fill_u8(jxg_ctx *ctx, uint8_t *p, size_t n, uint8_t v);
+// jxg_alpha.hip: the alpha stages of 2- and 4-channel input after XYB, and
+// the alpha factors on the final quant field.
+JXG_HIDDEN int alpha_front(jxg_ctx *ctx, const uint8_t *d_in, size_t row_bytes, int ch,
+                           uint32_t W, uint32_t H, int flags, float *xyb, uint32_t stride,
+                           uint32_t ph);
+JXG_HIDDEN int alpha_finish(jxg_ctx *ctx, uint32_t W, uint32_t H, const uint8_t *cells,
+                            uint8_t *qf);
 JXG_HIDDEN int zero_cmaps(jxg_ctx *ctx, uint32_t w, uint32_t h, const int8_t **x,
                           const int8_t **b);
 

@@ -1766,6 +1766,11 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_misc);
   (void)hipFree(ctx->d_exp2);
   (void)hipFree(ctx->d_xt);
+  (void)hipFree(ctx->d_astat);
+  (void)hipFree(ctx->d_alpha);
+  (void)hipFree(ctx->d_alpha2);
+  (void)hipFree(ctx->d_vis);
+  (void)hipFree(ctx->d_afac);
   (void)hipFree(ctx->d_zero_cmap);
   (void)hipFree(ctx->d_rgb);
   (void)hipFree(ctx->d_tmp);
@@ -1992,6 +1997,18 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
   if ((rc = jxg_xyb_dev(ctx, d_rgb, row_bytes, channels, width, height, fwd, lut, JXG_XYB_SCALAR,
                         params->fused, ctx->d_xyb, stride, ph)))
     return rc;
+  // Alpha of gray + alpha and RGBA input: fringe clearing, fill_invisible on
+  // XYB and the alpha block factors (src/frame/encoder.rs:60-69,152-157).
+  // k_xyb already snapped gray input to X = 0, B = Y; the fill is the same
+  // weighted mean on every plane, so snapping first gives identical planes.
+  const bool has_alpha = channels == 2 || channels == 4;
+  ctx->alpha_used = 0;
+  if (has_alpha) {
+    if ((rc = alpha_front(ctx, d_rgb, row_bytes, channels, width, height, params->flags,
+                          ctx->d_xyb, stride, ph)))
+      return rc;
+    ctx->alpha_used = (size_t)width * height;
+  }
   if ((rc = upload_derived(ctx, params))) return rc;
   if ((rc = fill_qf(ctx, ctx->d_xyb, stride, ph, width, height, params->distance,
                     ctx->h_derived->inv_scale, params->fused, d_qf)))
@@ -2002,6 +2019,10 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
     return rc;
   if ((rc = jxg_select_strategy_dev(ctx, ctx->d_xyb, stride, ph, width, height, d_qf, d_ytox, d_ytob,
                                     params, d_cells)))
+    return rc;
+  // once every layout and quant-field decision is made (src/frame/dc_groups.rs:302-312)
+  if (has_alpha && (params->flags & JXG_FLAG_FILL_INVISIBLE) &&
+      (rc = alpha_finish(ctx, width, height, d_cells, d_qf)))
     return rc;
   return jxg_forward_quantize_dev(ctx, ctx->d_xyb, stride, ph, width, height, d_cells, d_qf, d_ytox,
                                   d_ytob, params, d_levels, d_coeffs, d_dc);

@@ -345,6 +345,8 @@ int jxg_frame_params_init(float distance, int speed, int fused, jxg_frame_params
   out->dequant_numerator = 0.145f;
   out->fused = f ? 1 : 0;
   out->speed = speed;
+  // EncodeConfig::default: fill_invisible on, alpha_distance unset (lossy alpha)
+  out->flags = JXG_FLAG_FILL_INVISIBLE | JXG_FLAG_LOSSY_ALPHA;
   // channel_weights_for_bias at the spec matrix (src/encoding_context.rs:39-51)
   out->channel_weight[0] = 0.30f;
   out->channel_weight[1] = 1.0f;

@@ -144,13 +144,30 @@ float jxo_epf_compare(const jxo_epf *p, const float *y, int wide, float terms[6]
 /* The 5x5 mirrored box mean (original_box5, src/epf_proxy/box_filter.rs:466). */
 void jxo_epf_box5(const float *src, int w, int h, float *dst);
 
+/* ---- alpha (jxo_alpha.c) ---- */
+/* alpha_fringe::clear for u8 samples (src/alpha_fringe.rs:77-225). */
+void jxo_alpha_fringe_clear(uint8_t *alpha, int width, int height);
+/* fill_invisible on one plane (src/invisible.rs:34-146); alpha is W x H. */
+void jxo_fill_invisible_plane(float *plane, size_t stride, int width, int height,
+                              const uint8_t *alpha);
+int jxo_alpha_is_constant(const uint8_t *alpha, size_t n);
+/* alpha_block_factors (src/adaptive_quant.rs:953-1030): ceil(W/8) x ceil(H/8). */
+void jxo_alpha_block_factors(const uint8_t *alpha, int width, int height, float *factors);
+/* apply_alpha_factors (src/adaptive_quant.rs:1036-1070) over a whole image. */
+void jxo_apply_alpha_factors(uint8_t *qf, const uint8_t *cells, int xb, int yb,
+                             const float *factors);
+
 /* Whole front end on the CPU (XYB -> AQ -> CfL maps -> selection ->
  * quantize) with `threads` worker threads; used as the benchmark's CPU
- * baseline.  ytox / ytob receive the CfL maps (ceil(H/64) x ceil(W/64)). */
+ * baseline.  ytox / ytob receive the CfL maps (ceil(H/64) x ceil(W/64)).
+ * With alpha (2 or 4 channels) and p->flags: JXG_FLAG_LOSSY_ALPHA clears
+ * the alpha fringe, JXG_FLAG_FILL_INVISIBLE fills the colour under alpha 0
+ * and scales the quant field by the alpha block factors; `alpha_out`
+ * (W x H, may be NULL) receives the alpha plane the stages used. */
 void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width, int height,
                    const jxg_frame_params *p, const float *tables, int8_t *ytox, int8_t *ytob,
                    int threads, uint8_t *qf, uint8_t *cells, int32_t *levels[3],
-                   float *coeffs[3], int16_t *dc[3]);
+                   float *coeffs[3], int16_t *dc[3], uint8_t *alpha_out);
 
 #ifdef __cplusplus
 }

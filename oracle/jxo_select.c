@@ -692,10 +692,8 @@ static void *fe_worker(void *arg) {
         for (int x = 0; x < W; x++) {
           const uint8_t *px = row + (size_t)x * j->channels;
           float o[3];
-          if (j->channels <= 2) { /* gray: linearize_gray + snap_achromatic_xyb */
+          if (j->channels <= 2) { /* gray: linearize_gray, snapped after the alpha fill */
             jxo_xyb_pixel(m, JXG_XYB_SCALAR, lut[px[0]], lut[px[0]], lut[px[0]], o);
-            o[0] = 0.0f;
-            o[2] = o[1];
           } else {
             jxo_xyb_pixel(m, JXG_XYB_SCALAR, lut[px[0]], lut[px[1]], lut[px[2]], o);
           }
@@ -753,7 +751,7 @@ static void run_pool(fe_job *j, int kind, int total, int threads) {
 void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width, int height,
                    const jxg_frame_params *p, const float *tables, int8_t *ytox, int8_t *ytob,
                    int threads, uint8_t *qf, uint8_t *cells,
-                   int32_t *levels[3], float *coeffs[3], int16_t *dc[3]) {
+                   int32_t *levels[3], float *coeffs[3], int16_t *dc[3], uint8_t *alpha_out) {
   jxo_set_fused(p->fused);
   fe_job j;
   memset(&j, 0, sizeof(j));
@@ -775,6 +773,32 @@ void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width
   if (threads < 1) threads = 1;
   int groups = ((width + 2047) / 2048) * ((height + 2047) / 2048);
   run_pool(&j, 0, (height + 7) / 8, threads);
+  /* Alpha of 2- and 4-channel input: fringe clearing (encode_image.rs:3650,
+   * 3980), then fill_invisible and snap_achromatic_xyb in the order of
+   * encode_frame (src/frame/encoder.rs:152-157). */
+  uint8_t *alpha = NULL;
+  float *factors = NULL;
+  if (channels == 2 || channels == 4) {
+    alpha = (uint8_t *)malloc(plane);
+    for (int y = 0; y < height; y++)
+      for (int x = 0; x < width; x++)
+        alpha[(size_t)y * width + x] = rgb[(size_t)y * row_bytes + (size_t)x * channels + channels - 1];
+    if (p->flags & JXG_FLAG_LOSSY_ALPHA) jxo_alpha_fringe_clear(alpha, width, height);
+    if (p->flags & JXG_FLAG_FILL_INVISIBLE) {
+      for (int c = 0; c < 3; c++)
+        jxo_fill_invisible_plane(j.xyb + c * plane, (size_t)width, width, height, alpha);
+      if (!jxo_alpha_is_constant(alpha, plane)) {
+        factors = (float *)malloc((size_t)xb * yb * sizeof(float));
+        jxo_alpha_block_factors(alpha, width, height, factors);
+      }
+    }
+    if (alpha_out) memcpy(alpha_out, alpha, plane);
+  }
+  if (channels <= 2)
+    for (size_t k = 0; k < plane; k++) {
+      j.xyb[k] = 0.0f;
+      j.xyb[2 * plane + k] = j.xyb[plane + k];
+    }
   run_pool(&j, 1, groups, threads);
   run_pool(&j, 4, (height + 63) / 64, threads);
   if (selection_runs(p)) {
@@ -788,6 +812,10 @@ void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width
   } else {
     memset(cells, JXG_CELL(JXG_DCT, 1), (size_t)xb * yb);
   }
+  /* after every layout and quant-field decision (src/frame/dc_groups.rs:302-312) */
+  if (factors) jxo_apply_alpha_factors(qf, cells, xb, yb, factors);
+  free(factors);
+  free(alpha);
   j.levels = levels;
   j.coeffs = coeffs;
   j.dc = dc;

@@ -86,7 +86,11 @@ def _declare(L):
                                 c.POINTER(c.c_int), c.POINTER(c.c_int)]
     L.jxo_front_end.argtypes = [_u8p, c.c_size_t, c.c_int, c.c_int, c.c_int, c.c_void_p, _f32p,
                                 _i8p, _i8p, c.c_int, _u8p, _u8p, c.POINTER(_i32p),
-                                c.POINTER(_f32p), c.POINTER(_i16p)]
+                                c.POINTER(_f32p), c.POINTER(_i16p), _u8p]
+    L.jxo_alpha_fringe_clear.argtypes = [_u8p, c.c_int, c.c_int]
+    L.jxo_fill_invisible_plane.argtypes = [_f32p, c.c_size_t, c.c_int, c.c_int, _u8p]
+    L.jxo_alpha_block_factors.argtypes = [_u8p, c.c_int, c.c_int, _f32p]
+    L.jxo_apply_alpha_factors.argtypes = [_u8p, _u8p, c.c_int, c.c_int, _f32p]
 
 
 def _p(a, t):
@@ -214,11 +218,49 @@ def front_end(rgb, params, tables, threads=1):
     ytox = np.zeros(tiles, np.int8)
     ytob = np.zeros(tiles, np.int8)
     levels, coeffs, dc, L, C, D = _outputs(xb, yb)
+    alpha = np.zeros((h, w), np.uint8)
     lib().jxo_front_end(_p(rgb, _u8p), w * ch, ch, w, h, ctypes.byref(params),
                         _p(np.ascontiguousarray(tables, np.float32), _f32p),
                         _p(ytox, _i8p), _p(ytob, _i8p), threads,
-                        _p(qf, _u8p), _p(cells, _u8p), L, C, D)
-    return dict(qf=qf, cells=cells, ytox=ytox, ytob=ytob, levels=levels, coeffs=coeffs, dc=dc)
+                        _p(qf, _u8p), _p(cells, _u8p), L, C, D, _p(alpha, _u8p))
+    out = dict(qf=qf, cells=cells, ytox=ytox, ytob=ytob, levels=levels, coeffs=coeffs, dc=dc)
+    if ch in (2, 4):
+        out["alpha"] = alpha
+    return out
+
+
+def alpha_fringe_clear(alpha):
+    """alpha_fringe::clear of a (H, W) uint8 plane; returns the cleared copy."""
+    a = np.ascontiguousarray(alpha, np.uint8).copy()
+    h, w = a.shape
+    lib().jxo_alpha_fringe_clear(_p(a, _u8p), w, h)
+    return a
+
+
+def fill_invisible(planes, alpha):
+    """fill_invisible of (C, H, W) float32 planes under a (H, W) uint8 alpha."""
+    out = np.ascontiguousarray(planes, np.float32).copy()
+    a = np.ascontiguousarray(alpha, np.uint8)
+    h, w = a.shape
+    for c in range(out.shape[0]):
+        lib().jxo_fill_invisible_plane(_p(out[c], _f32p), w, w, h, _p(a, _u8p))
+    return out
+
+
+def alpha_block_factors(alpha):
+    a = np.ascontiguousarray(alpha, np.uint8)
+    h, w = a.shape
+    f = np.zeros(((h + 7) // 8, (w + 7) // 8), np.float32)
+    lib().jxo_alpha_block_factors(_p(a, _u8p), w, h, _p(f, _f32p))
+    return f
+
+
+def apply_alpha_factors(qf, cells, factors):
+    q = np.ascontiguousarray(qf, np.uint8).copy()
+    yb, xb = q.shape
+    lib().jxo_apply_alpha_factors(_p(q, _u8p), _p(np.ascontiguousarray(cells, np.uint8), _u8p),
+                                  xb, yb, _p(np.ascontiguousarray(factors, np.float32), _f32p))
+    return q
 
 
 def fill_cmap(xyb, tables, params):

@@ -139,13 +139,14 @@ typedef struct jxg_frame_params {
   int32_t global_scale;
   int32_t quant_dc;
   int32_t epf_iters;
-  int32_t flags;  /* JXG_FLAG_* (alpha handling of 2- and 4-channel input) */
+  int32_t flags;  /* JXG_FLAG_* stage switches */
   float channel_weight[3]; /* EncodingContext::channel_weights (X, Y, B) */
 } jxg_frame_params;
 
-/* jxg_frame_params::flags.  Both are on after jxg_frame_params_init, as in
- * EncodeConfig's defaults (src/encode_image.rs:682-740).  They act only on
- * 2- and 4-channel input (gray + alpha, RGBA):
+/* jxg_frame_params::flags.  The first two are on after
+ * jxg_frame_params_init, as in EncodeConfig's defaults
+ * (src/encode_image.rs:682-740), and act only on 2- and 4-channel input
+ * (gray + alpha, RGBA):
  *   JXG_FLAG_LOSSY_ALPHA     alpha_fringe::clear on the alpha plane first
  *                            (src/alpha_fringe.rs:77; the reference runs it
  *                            unless alpha_distance <= 0, src/encode_image.rs:3650)
@@ -154,7 +155,12 @@ typedef struct jxg_frame_params {
  *                            alpha, the alpha block factors on the final quant
  *                            field (src/frame/encoder.rs:60-69,
  *                            src/frame/dc_groups.rs:305). */
-enum { JXG_FLAG_FILL_INVISIBLE = 1, JXG_FLAG_LOSSY_ALPHA = 2 };
+enum { JXG_FLAG_FILL_INVISIBLE = 1, JXG_FLAG_LOSSY_ALPHA = 2, JXG_FLAG_CHROMA_HF = 4 };
+/* JXG_FLAG_CHROMA_HF (off after jxg_frame_params_init): apply the chroma-HF
+ * protection to the AQ field, as setup_dc_group does when the effort has the
+ * chroma policy and the frame classifier marked it x- or b-heavy
+ * (src/frame/dc_groups.rs:159, flags set in src/frame/chroma.rs:442-443; the
+ * classifier stays with the caller). */
 
 /* Speed tiers (src/encode_image.rs Speed, src/effort.rs:305-441). */
 enum {
@@ -362,6 +368,17 @@ int jxg_epf_proxy_compare(jxg_epf_proxy *proxy, const float *d_y, uint32_t row_s
 int jxg_gaborish_inverse_dev(jxg_ctx *ctx, const float *d_src, float *d_dst, uint32_t stride,
                              uint32_t padded_h, uint32_t width, uint32_t height, float mul,
                              int fused);
+
+/* apply_chroma_hf_protection (src/adaptive_quant.rs:303) on the quant
+ * field of the whole image, in place: blocks whose chroma is strong, whose
+ * chroma gradients are real and carry a large share of the edge energy get
+ * their quant value raised by up to (0.35 + 1.4 share^2) * strength, the
+ * strength fading in from distance 0.35 and out by 2.0.  Statistics and
+ * weights follow the scalar path (chroma_hf_stats_scalar :222); `fused`
+ * selects the fmla semantics. */
+int jxg_chroma_hf_protection_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
+                                 uint32_t padded_h, uint32_t width, uint32_t height,
+                                 float distance, int fused, uint8_t *d_qf);
 
 /* ---- alpha stages (2- and 4-channel input) ----
  * jxg_front_end_dev runs these itself according to params->flags; they are

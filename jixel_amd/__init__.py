@@ -148,6 +148,7 @@ XYB_SPEC_FWD = _spec_fwd()
 
 FLAG_FILL_INVISIBLE = 1  # JXG_FLAG_FILL_INVISIBLE
 FLAG_LOSSY_ALPHA = 2     # JXG_FLAG_LOSSY_ALPHA
+FLAG_CHROMA_HF = 4       # JXG_FLAG_CHROMA_HF
 
 
 @dataclass
@@ -297,6 +298,13 @@ class FrontEnd:
         check(lib().jxg_dct64_image_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
                                         height, int(method), 1 if fused else 0, arr))
         return out
+
+    def chroma_hf_protection(self, xyb, width, height, qf, distance, fused=True):
+        """apply_chroma_hf_protection on a device quant field, in place."""
+        check(lib().jxg_chroma_hf_protection_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1],
+                                                 width, height, float(distance),
+                                                 1 if fused else 0, _ptr(qf)))
+        return qf
 
     def alpha_fringe_clear(self, alpha, width, height):
         """alpha_fringe::clear of a (H, W) uint8 device plane, in place."""

@@ -101,6 +101,9 @@ JXG_HIDDEN int alpha_front(jxg_ctx *ctx, const uint8_t *d_in, size_t row_bytes, 
                            uint32_t ph);
 JXG_HIDDEN int alpha_finish(jxg_ctx *ctx, uint32_t W, uint32_t H, const uint8_t *cells,
                             uint8_t *qf);
+// jxg_chroma.hip: chroma-HF protection of the quant field.
+JXG_HIDDEN int chroma_hf(jxg_ctx *ctx, const float *xyb, uint32_t stride, uint32_t ph, uint32_t W,
+                         uint32_t H, float distance, int fused, uint8_t *qf);
 JXG_HIDDEN int zero_cmaps(jxg_ctx *ctx, uint32_t w, uint32_t h, const int8_t **x,
                           const int8_t **b);
 

@@ -2013,6 +2013,10 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
   if ((rc = fill_qf(ctx, ctx->d_xyb, stride, ph, width, height, params->distance,
                     ctx->h_derived->inv_scale, params->fused, d_qf)))
     return rc;
+  if ((params->flags & JXG_FLAG_CHROMA_HF) &&
+      (rc = chroma_hf(ctx, ctx->d_xyb, stride, ph, width, height, params->distance, params->fused,
+                      d_qf)))
+    return rc;
   // CfL slopes before the strategy search, as setup_dc_group does
   // (src/frame/dc_groups.rs:166-184).
   if ((rc = fill_cmap(ctx, ctx->d_xyb, stride, ph, width, height, params, d_ytox, d_ytob)))

@@ -144,6 +144,18 @@ float jxo_epf_compare(const jxo_epf *p, const float *y, int wide, float terms[6]
 /* The 5x5 mirrored box mean (original_box5, src/epf_proxy/box_filter.rs:466). */
 void jxo_epf_box5(const float *src, int w, int h, float *dst);
 
+/* ---- chroma-HF protection (jxo_pixel.c) ---- */
+/* chroma_hf_stats_scalar (src/adaptive_quant.rs:222): {chroma_sum,
+ * chroma_grad, luma_grad} and the edge count of a w x h block at (px, py). */
+void jxo_chroma_hf_stats(const float *xyb, int width, int height, int px, int py, int w, int h,
+                         float out[3], int *edges);
+/* The distance strength of apply_chroma_hf_protection (0 outside 0.35..2). */
+float jxo_chroma_hf_strength(float distance);
+/* apply_chroma_hf_protection (src/adaptive_quant.rs:303-408) on the whole
+ * image's quant field (ceil(W/8) x ceil(H/8)). */
+void jxo_chroma_hf_protection(const float *xyb, int width, int height, uint8_t *qf,
+                              float distance);
+
 /* ---- alpha (jxo_alpha.c) ---- */
 /* alpha_fringe::clear for u8 samples (src/alpha_fringe.rs:77-225). */
 void jxo_alpha_fringe_clear(uint8_t *alpha, int width, int height);

@@ -666,3 +666,85 @@ void jxo_fill_quant_field_image(const float *xyb, int width, int height, float d
     }
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* Chroma-HF protection (apply_chroma_hf_protection_with_stats,         */
+/* src/adaptive_quant.rs:344-408, stats chroma_hf_stats_scalar :222-279) */
+
+void jxo_chroma_hf_stats(const float *xyb, int width, int height, int px, int py, int w, int h,
+                         float out[3], int *edges) {
+  const size_t plane = (size_t)width * height;
+  const float *X = xyb, *Y = xyb + plane, *B = xyb + 2 * plane;
+  float chroma_sum = 0.0f, chroma_grad = 0.0f, luma_grad = 0.0f;
+  int e = 0;
+  for (int y = py; y < py + h; y++)
+    for (int x = px; x < px + w; x++) {
+      const size_t i = (size_t)y * width + x;
+      chroma_sum += fabsf(X[i]) + fabsf(B[i] - Y[i]);
+    }
+  if (w > 1) {
+    for (int y = py; y < py + h; y++)
+      for (int x = px; x + 1 < px + w; x++) {
+        const size_t i = (size_t)y * width + x, j = i + 1;
+        chroma_grad += fabsf(X[j] - X[i]) + fabsf((B[j] - Y[j]) - (B[i] - Y[i]));
+        luma_grad += fabsf(Y[j] - Y[i]);
+      }
+    e += (w - 1) * h;
+  }
+  if (h > 1) {
+    for (int y = py; y + 1 < py + h; y++)
+      for (int x = px; x < px + w; x++) {
+        const size_t i = (size_t)y * width + x, j = i + (size_t)width;
+        chroma_grad += fabsf(X[j] - X[i]) + fabsf((B[j] - Y[j]) - (B[i] - Y[i]));
+        luma_grad += fabsf(Y[j] - Y[i]);
+      }
+    e += w * (h - 1);
+  }
+  out[0] = chroma_sum;
+  out[1] = chroma_grad;
+  out[2] = luma_grad;
+  *edges = e;
+}
+
+static float clamp01(float v) { return v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v); }
+
+float jxo_chroma_hf_strength(float distance) {
+  const float fade_in_start = 0.35f, fade_in_width = 0.40f, fade_out_start = 1.0f,
+              fade_out_width = 1.0f, strength = 0.40f;
+  if (distance < fade_in_start) return 0.0f;
+  const float fade_in = clamp01((distance - fade_in_start) / fade_in_width);
+  const float fade_out = 1.0f - clamp01((distance - fade_out_start) / fade_out_width);
+  return strength * fade_in * fade_out;
+}
+
+void jxo_chroma_hf_protection(const float *xyb, int width, int height, uint8_t *qf,
+                              float distance) {
+  const float ceil_base = 0.35f, ceil_share = 1.4f;
+  const float ds = jxo_chroma_hf_strength(distance);
+  if (ds <= 0.0f) return;
+  const int xb = (width + 7) / 8, yb = (height + 7) / 8;
+  for (int by = 0; by < yb; by++)
+    for (int bx = 0; bx < xb; bx++) {
+      const int px = bx * 8, py = by * 8;
+      const int w = mini(width - px, 8), h = mini(height - py, 8);
+      float s[3];
+      int edges;
+      jxo_chroma_hf_stats(xyb, width, height, px, py, w, h, s, &edges);
+      if (edges == 0) continue;
+      const float mean_chroma = s[0] / (float)(w * h);
+      const float mean_gradient = s[1] / (float)edges;
+      const float chroma_share = s[1] / (s[1] + s[2] + 1e-4f);
+      const float chroma_w = clamp01((mean_chroma - 0.025f) / 0.075f);
+      const float gradient_w = clamp01((mean_gradient - 0.006f) / 0.030f);
+      const float share_w = clamp01((chroma_share - 0.25f) / 0.55f);
+      const float risk = chroma_w * gradient_w * share_w;
+      if (risk > 0.0f) {
+        const float ceiling = jxo_fmla(ceil_share, share_w * share_w, ceil_base);
+        const float gain = jxo_fmla(ceiling * ds, risk, 1.0f);
+        uint8_t *q = qf + (size_t)by * xb + bx;
+        float v = roundf((float)*q * gain);
+        v = v < 1.0f ? 1.0f : (v > 255.0f ? 255.0f : v);
+        *q = (uint8_t)v;
+      }
+    }
+}

@@ -800,6 +800,8 @@ void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width
       j.xyb[2 * plane + k] = j.xyb[plane + k];
     }
   run_pool(&j, 1, groups, threads);
+  /* chroma-HF protection after the AQ field (src/frame/dc_groups.rs:159-167) */
+  if (p->flags & JXG_FLAG_CHROMA_HF) jxo_chroma_hf_protection(j.xyb, width, height, qf, p->distance);
   run_pool(&j, 4, (height + 63) / 64, threads);
   if (selection_runs(p)) {
     j.qf_in = (uint8_t *)malloc((size_t)xb * yb);

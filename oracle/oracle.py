@@ -87,6 +87,11 @@ def _declare(L):
     L.jxo_front_end.argtypes = [_u8p, c.c_size_t, c.c_int, c.c_int, c.c_int, c.c_void_p, _f32p,
                                 _i8p, _i8p, c.c_int, _u8p, _u8p, c.POINTER(_i32p),
                                 c.POINTER(_f32p), c.POINTER(_i16p), _u8p]
+    L.jxo_chroma_hf_stats.argtypes = [_f32p, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int,
+                                      _f32p, c.POINTER(c.c_int)]
+    L.jxo_chroma_hf_strength.restype = c.c_float
+    L.jxo_chroma_hf_strength.argtypes = [c.c_float]
+    L.jxo_chroma_hf_protection.argtypes = [_f32p, c.c_int, c.c_int, _u8p, c.c_float]
     L.jxo_alpha_fringe_clear.argtypes = [_u8p, c.c_int, c.c_int]
     L.jxo_fill_invisible_plane.argtypes = [_f32p, c.c_size_t, c.c_int, c.c_int, _u8p]
     L.jxo_alpha_block_factors.argtypes = [_u8p, c.c_int, c.c_int, _f32p]
@@ -337,3 +342,25 @@ def epf_box5(src):
     out = np.zeros_like(src)
     lib().jxo_epf_box5(_p(src, _f32p), w, h, _p(out, _f32p))
     return out
+
+
+def chroma_hf_stats(xyb, px, py, w, h):
+    """(chroma_sum, chroma_grad, luma_grad, edges) of a block of (3, H, W) XYB."""
+    x = np.ascontiguousarray(xyb, np.float32)
+    out = np.zeros(3, np.float32)
+    e = ctypes.c_int()
+    lib().jxo_chroma_hf_stats(_p(x, _f32p), x.shape[2], x.shape[1], px, py, w, h, _p(out, _f32p),
+                              ctypes.byref(e))
+    return float(out[0]), float(out[1]), float(out[2]), e.value
+
+
+def chroma_hf_strength(distance):
+    return float(lib().jxo_chroma_hf_strength(float(distance)))
+
+
+def chroma_hf_protection(xyb, qf, distance):
+    x = np.ascontiguousarray(xyb, np.float32)
+    q = np.ascontiguousarray(qf, np.uint8).copy()
+    lib().jxo_chroma_hf_protection(_p(x, _f32p), x.shape[2], x.shape[1], _p(q, _u8p),
+                                   float(distance))
+    return q

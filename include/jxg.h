@@ -254,6 +254,27 @@ int jxg_strategy_costs_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
                            const jxg_frame_params *params, float *d_cost8,
                            float *d_cost16, float *d_cost32);
 
+/* Coefficient-model RD cost of one strategy at a list of sites: the cost
+ * table the slower tiers' layout search (rectangles, sub-8x8 and DCT64
+ * passes) compares.  Replaces strategy_cost (src/ac_strategy.rs:724 ->
+ * strategy_cost_impl :1293) for every strategy up to 64x64, the DCT64
+ * family's strategy_cost64 (:646, callers src/ac_strategy/selection.rs:
+ * 2077-2189) and sub8_strategy_costs' per-candidate evaluation (:1746,
+ * caller selection.rs:1266).  Site i is the first block
+ * d_sites[i] = by << 16 | bx with quantizer d_qac[i] (the reference's qac,
+ * scale * quant value); meta_r is the candidate's metadata charge (META_R = 4
+ * at the chooser's call sites, 0 for refinement).  Costs use the model rate
+ * (learned rate prices, src/ac_strategy/rate_prices.rs, stay on the CPU);
+ * the selection quantizer applies to the 32-family as at the Fast tiers.
+ * d_cost[i] receives the cost, NaN for a site whose footprint leaves the
+ * padded planes.  DCT128 strategies are rejected. */
+int jxg_strategy_cost_sites_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
+                                uint32_t padded_h, uint32_t width, uint32_t height,
+                                const int8_t *d_ytox, const int8_t *d_ytob,
+                                const jxg_frame_params *params, int strategy, uint32_t n_sites,
+                                const uint32_t *d_sites, const float *d_qac, float meta_r,
+                                float *d_cost);
+
 /* Transform-layout decision over square merges (DCT8 / DCT16X16 /
  * DCT32X32) and the quant-field aggregation over each merged transform.
  * Replaces the square-merge core of fill_ac_strategy

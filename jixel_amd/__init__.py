@@ -299,6 +299,24 @@ class FrontEnd:
                                         height, int(method), 1 if fused else 0, arr))
         return out
 
+    def strategy_cost_sites(self, xyb, width, height, params, strategy, sites, qac,
+                            meta_r=4.0, ytox=None, ytob=None, out=None):
+        """Coefficient-model cost of `strategy` at each (bx, by) in `sites`
+        ((n, 2) int tensor or array) with quantizers `qac` (n floats)."""
+        t = self.torch
+        sites = t.as_tensor(np.asarray(sites, np.int64)) if not t.is_tensor(sites) else sites
+        packed = ((sites[:, 1].to(t.int64) << 16) | sites[:, 0].to(t.int64)).to(t.int32)
+        packed = packed.to(self.device).contiguous()
+        qac = t.as_tensor(qac, dtype=t.float32).to(self.device).contiguous()
+        n = int(packed.shape[0])
+        out = self.empty(n, t.float32) if out is None else out
+        ytox, ytob = self._cmaps(width, height, ytox, ytob)
+        check(lib().jxg_strategy_cost_sites_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1],
+                                                width, height, _ptr(ytox), _ptr(ytob),
+                                                ctypes.byref(params), int(strategy), n,
+                                                _ptr(packed), _ptr(qac), float(meta_r), _ptr(out)))
+        return out
+
     def chroma_hf_protection(self, xyb, width, height, qf, distance, fused=True):
         """apply_chroma_hf_protection on a device quant field, in place."""
         check(lib().jxg_chroma_hf_protection_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1],

@@ -26,6 +26,7 @@ EXPORTED = [
     "jxg_dct64_image_dev", "jxg_epf_proxy_new", "jxg_epf_proxy_destroy", "jxg_epf_proxy_compare",
     "jxg_alpha_fringe_clear_dev", "jxg_fill_invisible_dev", "jxg_alpha_block_factors_dev",
     "jxg_apply_alpha_factors_dev", "jxg_alpha_plane", "jxg_chroma_hf_protection_dev",
+    "jxg_strategy_cost_sites_dev",
 ]
 
 
@@ -130,6 +131,9 @@ def _declare(L):
     L.jxg_alpha_block_factors_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, vp]
     L.jxg_apply_alpha_factors_dev.argtypes = [vp, vp, vp, c.c_uint32, c.c_uint32, vp]
     L.jxg_alpha_plane.argtypes = [vp, vp, c.c_size_t, c.c_int]
+    L.jxg_strategy_cost_sites_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32,
+                                              c.c_uint32, vp, vp, c.POINTER(FrameParams), c.c_int,
+                                              c.c_uint32, vp, vp, c.c_float, vp]
     L.jxg_chroma_hf_protection_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32,
                                                c.c_uint32, c.c_float, c.c_int, vp]
 

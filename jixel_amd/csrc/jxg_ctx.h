@@ -40,6 +40,7 @@ struct jxg_ctx {
   bool derived_valid = false;
   jxg_frame_params last_params;
   uint32_t *d_scans = nullptr;  // natural orders of DCT8/16/32: scan position -> storage index
+  uint16_t *d_scanpos = nullptr;  // storage index -> scan position, every shape up to 64x64
   uint32_t h_order[64 + 256 + 1024];
   float *d_selq = nullptr;  // inverse quant weights of DCT8/16/32 per channel, in scan order
   float *d_mlut = nullptr;      // fwd matrix (9) + transfer LUT (256)

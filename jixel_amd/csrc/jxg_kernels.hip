@@ -1561,6 +1561,231 @@ __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Candidate costs at listed sites: strategy_cost with the coefficient model
+// (src/ac_strategy.rs:724 -> strategy_cost_impl :1293-1377) for any strategy
+// up to 64x64, the DCT64 family in strategy_cost64's final form (:646-720),
+// sub-8x8 strategies with the arithmetic of sub8_strategy_costs' evaluate
+// (:1768-1795, identical to strategy_cost_impl for them).  One workgroup per
+// site: the three channels' footprints are transformed in LDS (the quantize
+// stage's transforms), CfL is subtracted from X and B, every coefficient's
+// squared error and rate term is stored, and six lanes fold the six sums in
+// the reference's sequential storage order.
+
+struct SiteArgs {
+  const float *xyb;
+  int stride, padded_h, W, H, tiles_x, tiles_y;
+  const int8_t *ytox, *ytob;
+  const float *tables;
+  const jxg_derived *D;
+  const uint16_t *scanpos;  // scan position of each storage index, per shape
+  int s;
+  uint32_t n;
+  const uint32_t *sites;  // by << 16 | bx
+  const float *qac;
+  float meta_r;
+  float *cost;
+};
+
+// Offset of shape (cx >= cy) in the scan-position table: (1,1) (2,1) (2,2)
+// (4,1) (4,2) (4,4) (8,4) (8,8).
+__host__ __device__ __forceinline__ int scanpos_offset(int cx, int cy) {
+  const int a = cx * cy;
+  switch (a) {
+    case 1: return 0;
+    case 2: return 64;
+    case 4: return cx == 2 ? 192 : 448;
+    case 8: return 704;
+    case 16: return 1216;
+    case 32: return 2240;
+    default: return 4288;
+  }
+}
+#define SCANPOS_TOTAL (4288 + 4096)
+
+template <bool F, bool BIG>
+__global__ __launch_bounds__(256) void k_site_cost(SiteArgs S) {
+  constexpr int NMAX = BIG ? 4096 : 1024;
+  __shared__ QLds L;
+  __shared__ float terms[3][2][NMAX];  // per channel: squared errors, rate terms
+  __shared__ double erased[4];
+  __shared__ float sums[3][2];
+  __shared__ uint32_t nz[3], mx[3];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const jxg_derived *D = S.D;
+  const int s = S.s;
+  const uint32_t site = S.sites[blockIdx.x];
+  const int bx = (int)(site & 0xffffu), by = (int)(site >> 16);
+  const int cvx = cov_x(s), cvy = cov_y(s), w = cvx * 8, h = cvy * 8;
+  const int cx = max(cvx, cvy), cy = min(cvx, cvy), size = cx * cy * 64;
+  if ((bx + cvx) * 8 > S.stride || (by + cvy) * 8 > S.padded_h) {
+    if (tid == 0) S.cost[blockIdx.x] = __int_as_float(0x7fc00000);  // NaN: site off the planes
+    return;
+  }
+  QArgs A;
+  A.xyb = S.xyb;
+  A.stride = S.stride;
+  A.padded_h = S.padded_h;
+  if (tid == 0) {
+    L.n = 1;
+    L.s[0] = (uint8_t)s;
+    L.bx[0] = (uint16_t)bx;
+    L.by[0] = (uint16_t)by;
+    L.lx[0] = 0;
+    L.ly[0] = 0;
+    L.pre1[0] = 0;
+    L.pre2[0] = 0;
+    L.pre1[1] = is_sub8(s) ? 0 : (rows_first(s) ? h : w);
+    L.pre2[1] = is_sub8(s) ? 0 : (rows_first(s) ? w : h);
+  }
+  if (tid < 3) {
+    nz[tid] = 0;
+    mx[tid] = 0;
+  }
+  if (tid < 64) {
+    L.line1_t[tid] = 0;
+    L.line2_t[tid] = 0;
+  }
+  const int kind = jxg_qt_kind_dev(s);
+  const float *tb = S.tables + D->qt_off[kind];
+  const int qn = D->qt_n[kind];
+  const uint16_t *scanpos = S.scanpos + scanpos_offset(cx, cy);
+  // cmap_factors (src/ac_strategy.rs:1838-1855)
+  const int tx = min(bx / 8, S.tiles_x - 1), ty = min(by / 8, S.tiles_y - 1);
+  const float xf = fmla<F>((float)S.ytox[ty * S.tiles_x + tx], D->cfl_inv_color_factor, D->cfl_base_x);
+  const float bf = fmla<F>((float)S.ytob[ty * S.tiles_x + tx], D->cfl_inv_color_factor, D->cfl_base_b);
+  // ---- Y: transform, keep the coefficients in L.y2 ----
+  load_plane(L, A, 1, bx * 8, by * 8, w, h);
+  __syncthreads();
+  transform_plane<F, BIG>(L, 1);
+  for (int e = tid; e < h * w; e += 256) {
+    const int r = e / w, c = e - r * w;
+    L.y2[r * QT_PITCH + c] = L.cur[r * QT_PITCH + c];
+  }
+  __syncthreads();
+  // selection_quantizer::scale (src/ac_strategy/selection_quantizer.rs:82-120)
+  float qac = S.qac[blockIdx.x];
+  if ((s == JXG_DCT32X32 || s == JXG_DCT32X16 || s == JXG_DCT16X32) && S.meta_r != 0.0f &&
+      D->selection_quantizer) {
+    const float *inv_y = tb + 4 * qn;
+    double energy = 0.0;
+    for (int k = tid; k < size; k += 256) {
+      const CoefGeom g = coef_geom(L, 0, k, 0);
+      if (g.col < cx && g.row < cy) continue;
+      const float l = L.y2[g.a];
+      if (fabsf(l * inv_y[k] * qac) < 0.5f) energy += (double)l * (double)l;
+    }
+    energy = wave_sum_f64(energy);
+    if (lane == 0) erased[wave] = energy;
+    __syncthreads();
+    const float e32 = (float)(((erased[0] + erased[1]) + erased[2]) + erased[3]);
+    const float dd = D->distance;
+    const float t = fminf(fmaxf(e32 / (2e-4f * dd * dd) - 1.0f, 0.0f), 1.0f);
+    qac = qac * (1.0f + 0.6f * t);
+  }
+  const int cls = jxg_area_class(cx, cy);
+  // Per-coefficient terms of channel c (sse_and_rate_scalar_impl,
+  // src/inflated_cost.rs:435-500); src = L.cur (X / B, CfL applied) or
+  // L.y2 (Y).
+  auto channel_terms = [&](int c) {
+    const float *inv = tb + (3 + c) * qn;
+    const float qs = qac * D->qm_mult[c];
+    const float one = D->dequant_one[c], num = D->dequant_numerator;
+    const float *thr = D->thr[c][cls];
+    const float f = c == 0 ? xf : bf;
+    uint32_t cnt = 0, best = 0;
+    for (int k0 = 0; k0 < size; k0 += 256) {
+      const int k = k0 + tid;
+      bool nzq = false;
+      uint32_t pos1 = 0;
+      if (k < size) {
+        const CoefGeom g = coef_geom(L, 0, k, 0);
+        float coef = c == 1 ? L.y2[g.a] : fmla<F>(-f, L.y2[g.a], L.cur[g.a]);
+        float dd = 0.0f, mg = 0.0f;
+        if (!(g.col < cx && g.row < cy)) {
+          const int yfix = g.row >= cy * 4 ? 2 : 0;
+          const float threshold = g.col >= cx * 4 ? thr[yfix + 1] : thr[yfix];
+          const float a = inv[k] * qs * coef;
+          const float q = fabsf(a) >= threshold ? round_away(a) : 0.0f;
+          const float diff = a - level_f32(one, num, q);
+          dd = diff * diff;
+          if (q != 0.0f) {
+            mg = rate_log2_slow(D, fabsf(q));
+            nzq = true;
+            pos1 = (uint32_t)scanpos[k] + 1u;
+          }
+        }
+        terms[c][0][k] = dd;
+        terms[c][1][k] = mg;
+      }
+      cnt += (uint32_t)__popcll(__ballot(nzq));
+      best = max(best, pos1);
+    }
+    for (int o = 32; o > 0; o >>= 1) best = max(best, (uint32_t)__shfl_xor((int)best, o));
+    if (lane == 0) {
+      atomicAdd(&nz[c], cnt);
+      atomicMax(&mx[c], best);
+    }
+  };
+  channel_terms(1);
+  // ---- X, then B ----
+  for (int c = 0; c <= 2; c += 2) {
+    __syncthreads();
+    load_plane(L, A, c, bx * 8, by * 8, w, h);
+    __syncthreads();
+    transform_plane<F, BIG>(L, 1);
+    channel_terms(c);
+  }
+  __syncthreads();
+  // six sequential f32 folds in storage order
+  {
+    const int which = tid >> 5;  // lanes 0 and 32 of each of the first three waves
+    if ((tid & 31) == 0 && which < 6) {
+      const int c = which >> 1, part = which & 1;
+      const float4 *src = reinterpret_cast<const float4 *>(&terms[c][part][0]);
+      float acc = 0.0f;
+      for (int k4 = 0; k4 < (size >> 2); k4 += 2) {
+        const float4 a0 = src[k4], a1 = src[k4 + 1];
+        acc += a0.x;
+        acc += a0.y;
+        acc += a0.z;
+        acc += a0.w;
+        acc += a1.x;
+        acc += a1.y;
+        acc += a1.z;
+        acc += a1.w;
+      }
+      sums[c][part] = acc;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    float d_total = 0.0f, r_total = 0.0f;
+    for (int c = 0; c < 3; c++) {
+      d_total += D->channel_weight[c] * sums[c][0];
+      const uint32_t nzc = nz[c];
+      const float nzf = (float)nzc;
+      const float header = 0.4f * (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf));
+      float visited = 0.0f;
+      if (nzc != 0) {  // mx holds 1 + the last nonzero's scan position
+        const int64_t span = (int64_t)mx[c] - (int64_t)(cx * cy + nzc);
+        visited = span > 0 ? (float)span : 0.0f;
+      }
+      r_total += nzf * 1.6f + 1.0f * sums[c][1] + header + 0.5f * visited;
+    }
+    // Footprint::visible_fraction (src/inflated_cost.rs:650-660)
+    const int fx = bx * 8, fy = by * 8;
+    const int vw = min(w, max(S.W - fx, 0)), vh = min(h, max(S.H - fy, 0));
+    const float visible = (vw == w && vh == h) ? 1.0f : (float)(vw * vh) / (float)(w * h);
+    float cost;
+    if (is_dct64(s))
+      cost = d_total * visible + RD_LAMBDA * (r_total + S.meta_r);  // strategy_cost64
+    else
+      cost = fmla<F>(RD_LAMBDA, r_total + S.meta_r, d_total * visible);
+    S.cost[blockIdx.x] = cost;
+  }
+}
+
 // Byte fill, used instead of hipMemsetAsync so that streams captured into a
 // graph hold kernel nodes only.
 __global__ void k_fill_u8(uint8_t *p, size_t n, uint8_t v) {
@@ -1730,6 +1955,28 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
   if (hipMemcpy(ctx->d_scans, scans, sizeof(scans), hipMemcpyHostToDevice) != hipSuccess)
     return fail(set_err(JXG_ERR_DEVICE, "scan table upload failed"));
   {
+    // scan position of every storage index, for each coefficient shape
+    static const int shapes[8][2] = {{1, 1}, {2, 1}, {2, 2}, {4, 1}, {4, 2}, {4, 4}, {8, 4}, {8, 8}};
+    uint16_t *pos = (uint16_t *)malloc(SCANPOS_TOTAL * sizeof(uint16_t));
+    uint32_t *order = (uint32_t *)malloc(4096 * sizeof(uint32_t));
+    if (!pos || !order) {
+      free(pos);
+      free(order);
+      return fail(set_err(JXG_ERR_ALLOC, "context allocation failed"));
+    }
+    for (int i = 0; i < 8; i++) {
+      const int cx = shapes[i][0], cy = shapes[i][1], n = cx * cy * 64;
+      natural_scan(cx, cy, order);
+      for (int p = 0; p < n; p++) pos[scanpos_offset(cx, cy) + order[p]] = (uint16_t)p;
+    }
+    const bool ok = hipMalloc((void **)&ctx->d_scanpos, SCANPOS_TOTAL * sizeof(uint16_t)) == hipSuccess &&
+                    hipMemcpy(ctx->d_scanpos, pos, SCANPOS_TOTAL * sizeof(uint16_t),
+                              hipMemcpyHostToDevice) == hipSuccess;
+    free(pos);
+    free(order);
+    if (!ok) return fail(set_err(JXG_ERR_DEVICE, "scan position upload failed"));
+  }
+  {
     float e2[64];
     for (int i = 0; i < 64; i++) e2[i] = (float)exp2((double)(i - 32) / 64.0);
     if (hipMemcpy(ctx->d_exp2, e2, sizeof(e2), hipMemcpyHostToDevice) != hipSuccess)
@@ -1766,6 +2013,7 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_misc);
   (void)hipFree(ctx->d_exp2);
   (void)hipFree(ctx->d_xt);
+  (void)hipFree(ctx->d_scanpos);
   (void)hipFree(ctx->d_astat);
   (void)hipFree(ctx->d_alpha);
   (void)hipFree(ctx->d_alpha2);
@@ -1964,6 +2212,59 @@ int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, 
   else
     k_quantize_xt<false><<<xt_grid, 256, 0, ctx->stream>>>(A);
   LAUNCH_CHECK("k_quantize_xt");
+  return JXG_OK;
+}
+
+int jxg_strategy_cost_sites_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
+                                uint32_t padded_h, uint32_t width, uint32_t height,
+                                const int8_t *d_ytox, const int8_t *d_ytob,
+                                const jxg_frame_params *params, int strategy, uint32_t n_sites,
+                                const uint32_t *d_sites, const float *d_qac, float meta_r,
+                                float *d_cost) {
+  if (!ctx || !d_xyb || !params || (n_sites && (!d_sites || !d_qac || !d_cost)))
+    return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  int rc = check_image(width, height);
+  if (rc) return rc;
+  if ((rc = check_planes(width, height, stride, padded_h))) return rc;
+  if (strategy < 0 || strategy >= JXG_NUM_STRATEGIES || strategy == JXG_DCT128X128 ||
+      strategy == JXG_DCT128X64 || strategy == JXG_DCT64X128)
+    return set_err(JXG_ERR_INVALID_ARG, "strategy %d has no cost kernel", strategy);
+  if (n_sites == 0) return JXG_OK;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if ((rc = upload_derived(ctx, params))) return rc;
+  if ((rc = zero_cmaps(ctx, width, height, &d_ytox, &d_ytob))) return rc;
+  SiteArgs S;
+  S.xyb = d_xyb;
+  S.stride = (int)stride;
+  S.padded_h = (int)padded_h;
+  S.W = (int)width;
+  S.H = (int)height;
+  S.tiles_x = (int)((width + 63) / 64);
+  S.tiles_y = (int)((height + 63) / 64);
+  S.ytox = d_ytox;
+  S.ytob = d_ytob;
+  S.tables = ctx->d_tables;
+  S.D = ctx->d_derived;
+  S.scanpos = ctx->d_scanpos;
+  S.s = strategy;
+  S.n = n_sites;
+  S.sites = d_sites;
+  S.qac = d_qac;
+  S.meta_r = meta_r;
+  S.cost = d_cost;
+  const bool big = is_dct64(strategy);
+  if (params->fused) {
+    if (big)
+      k_site_cost<true, true><<<n_sites, 256, 0, ctx->stream>>>(S);
+    else
+      k_site_cost<true, false><<<n_sites, 256, 0, ctx->stream>>>(S);
+  } else {
+    if (big)
+      k_site_cost<false, true><<<n_sites, 256, 0, ctx->stream>>>(S);
+    else
+      k_site_cost<false, false><<<n_sites, 256, 0, ctx->stream>>>(S);
+  }
+  LAUNCH_CHECK("k_site_cost");
   return JXG_OK;
 }
 

@@ -261,6 +261,9 @@ float jxo_strategy_cost(const float *xyb, int width, int height, int s, int px, 
     d_total += p->channel_weight[c] * r.sse;
     r_total += model_bits(r.nzeros, r.mag_bits, r.max_scan, cx, cy);
   }
+  /* strategy_cost64 (src/ac_strategy.rs:646-720) adds without fmla */
+  if (s == JXG_DCT64X64 || s == JXG_DCT64X32 || s == JXG_DCT32X64)
+    return d_total * visible + RD_LAMBDA * (r_total + meta_r);
   return jxo_fmla(RD_LAMBDA, r_total + meta_r, d_total * visible);
 }
 

@@ -324,6 +324,54 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
                       int8_t *d_ytox, int8_t *d_ytob, uint8_t *d_qf, uint8_t *d_cells,
                       int32_t *d_levels[3], float *d_coeffs[3], int16_t *d_dc[3]);
 
+/* ---- batched front end ----
+ * Many images of uneven sizes and qualities through the front end with one
+ * launch per stage for the whole batch (eight launches in all), the stage
+ * chain captured once into a HIP graph and replayed.  The reference encodes
+ * one image per encode_image call (src/encode_image.rs:1610); a batch is N
+ * such calls' front ends.  Per image the outputs are bit-identical to
+ * jxg_front_end_dev with the same parameters.  Batches take 1- and
+ * 3-channel input (gray, RGB) without JXG_FLAG_CHROMA_HF; all images share
+ * params.fused. */
+typedef struct jxg_batch_image {
+  const uint8_t *d_rgb; /* device input, row_bytes per row */
+  size_t row_bytes;
+  int32_t channels; /* 1 or 3 */
+  uint32_t width, height;
+  jxg_frame_params params;
+  int8_t *d_ytox, *d_ytob; /* CfL maps, ceil(W/64) x ceil(H/64) */
+  uint8_t *d_qf, *d_cells; /* xblocks x yblocks */
+  int32_t *d_levels[3];
+  float *d_coeffs[3];
+  int16_t *d_dc[3];
+} jxg_batch_image;
+typedef struct jxg_batch jxg_batch;
+
+/* Validates the images, allocates the batch's scratch (XYB planes, AQ and
+ * CfL staging) and uploads the per-image descriptors.  The device buffers
+ * named in `images` must stay valid for the batch's lifetime; their
+ * contents are read / written at every run. */
+int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jxg_batch **out);
+void jxg_batch_destroy(jxg_batch *batch);
+/* Enqueues the batch's stage chain on the context's stream: kernel launches
+ * only, so a caller may capture it into its own graph. */
+int jxg_batch_run(jxg_batch *batch);
+/* Runs the chain as a HIP graph: captured on the first call (and after a
+ * retune) on the batch's own stream, then replayed; ordered after prior
+ * work on the context's stream and before later work. */
+int jxg_batch_launch(jxg_batch *batch);
+/* Status of the last run (synchronizes): as jxg_quantize_status, summed over
+ * the batch. */
+int jxg_batch_status(jxg_batch *batch, uint32_t out[2]);
+/* Autotuner: times every candidate tile shape of the tunable stages (AQ
+ * strip of 16 / 32 / 64 blocks, CfL sub-tiles of 16 / 32 rows, 64 / 256 /
+ * 1024 workgroups for the crossing-transform queue) on this batch, `reps`
+ * graph replays each, keeps the fastest (results do not depend on it) and
+ * writes a one-line-per-candidate report (may be NULL). */
+int jxg_batch_autotune(jxg_batch *batch, int reps, char *report, size_t report_len);
+/* The tile shapes in use: {aq_strip, cm_rows, xt_grid}. */
+int jxg_batch_config(jxg_batch *batch, int out[3]);
+
 /* Host-buffer convenience form of jxg_front_end_dev for callers that own
  * host memory (the reference's encode_image input is a host slice,
  * src/encode_image.rs:1610): uploads, runs, downloads, synchronizes.

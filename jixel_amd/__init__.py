@@ -21,6 +21,7 @@ from ._lib import FrameParams, JxgError, check, lib
 
 __all__ = [
     "Speed", "EncodeConfig", "FrameParams", "JxgError", "FrontEnd", "FrontEndOutput", "LumaProxy",
+    "Batch",
     "distance_from_quality", "frame_params", "default_quant_tables", "srgb_lut",
     "plane_geometry", "qt_offset", "qt_size", "encode_front_end", "XYB_SPEC_FWD",
     "STRATEGY",
@@ -393,6 +394,85 @@ class FrontEnd:
         o["ytox"] = ytox
         o["ytob"] = ytob
         return o
+
+
+class Batch:
+    """Many images through the front end at once (jxg_batch_*): one launch per
+    stage for the whole batch, replayed as a HIP graph by launch().
+
+    images: list of (rgb, params) with rgb a (H, W, 1|3) uint8 device tensor
+    and params its FrameParams.  outputs(i) gives image i's results in the
+    layout of FrontEnd.run.
+    """
+
+    def __init__(self, fe, images):
+        from ._lib import BatchImage
+        t = fe.torch
+        self.fe = fe
+        self.inputs = [rgb for rgb, _ in images]  # kept alive: the batch reads them
+        self.outs = []
+        arr = (BatchImage * len(images))()
+        for i, (rgb, params) in enumerate(images):
+            h, w, ch = int(rgb.shape[0]), int(rgb.shape[1]), int(rgb.shape[2])
+            xb, yb = (w + 7) // 8, (h + 7) // 8
+            tiles = ((h + 63) // 64, (w + 63) // 64)
+            o = fe.outputs(w, h)
+            o["qf"] = fe.empty((yb, xb), t.uint8)
+            o["cells"] = fe.empty((yb, xb), t.uint8)
+            o["ytox"] = fe.empty(tiles, t.int8)
+            o["ytob"] = fe.empty(tiles, t.int8)
+            self.outs.append(o)
+            d = arr[i]
+            d.d_rgb = rgb.data_ptr()
+            d.row_bytes = rgb.stride(0) * rgb.element_size()
+            d.channels = ch
+            d.width, d.height = w, h
+            d.params = params
+            d.d_ytox, d.d_ytob = o["ytox"].data_ptr(), o["ytob"].data_ptr()
+            d.d_qf, d.d_cells = o["qf"].data_ptr(), o["cells"].data_ptr()
+            for c in range(3):
+                d.d_levels[c] = o["levels"][c].data_ptr()
+                d.d_coeffs[c] = o["coeffs"][c].data_ptr()
+                d.d_dc[c] = o["dc"][c].data_ptr()
+        self.handle = ctypes.c_void_p()
+        check(lib().jxg_batch_create(fe.ctx, len(images), arr, ctypes.byref(self.handle)))
+
+    def run(self):
+        """Enqueues the stage chain on the context's stream."""
+        check(lib().jxg_batch_run(self.handle))
+
+    def launch(self):
+        """Runs the chain as a HIP graph (captured on first use)."""
+        check(lib().jxg_batch_launch(self.handle))
+
+    def status(self):
+        out = (ctypes.c_uint32 * 2)()
+        check(lib().jxg_batch_status(self.handle, out))
+        return int(out[0]), int(out[1])
+
+    def autotune(self, reps=3):
+        buf = ctypes.create_string_buffer(4096)
+        check(lib().jxg_batch_autotune(self.handle, int(reps), buf, len(buf)))
+        return buf.value.decode()
+
+    def config(self):
+        out = (ctypes.c_int * 3)()
+        check(lib().jxg_batch_config(self.handle, out))
+        return dict(aq_strip=out[0], cm_rows=out[1], xt_grid=out[2])
+
+    def outputs(self, i):
+        return self.outs[i]
+
+    def close(self):
+        if self.handle:
+            lib().jxg_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class LumaProxy:

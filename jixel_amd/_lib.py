@@ -26,7 +26,8 @@ EXPORTED = [
     "jxg_dct64_image_dev", "jxg_epf_proxy_new", "jxg_epf_proxy_destroy", "jxg_epf_proxy_compare",
     "jxg_alpha_fringe_clear_dev", "jxg_fill_invisible_dev", "jxg_alpha_block_factors_dev",
     "jxg_apply_alpha_factors_dev", "jxg_alpha_plane", "jxg_chroma_hf_protection_dev",
-    "jxg_strategy_cost_sites_dev",
+    "jxg_strategy_cost_sites_dev", "jxg_batch_create", "jxg_batch_destroy", "jxg_batch_run",
+    "jxg_batch_launch", "jxg_batch_status", "jxg_batch_autotune", "jxg_batch_config",
 ]
 
 
@@ -52,6 +53,25 @@ class FrameParams(ctypes.Structure):
         ("epf_iters", ctypes.c_int32),
         ("flags", ctypes.c_int32),
         ("channel_weight", ctypes.c_float * 3),
+    ]
+
+
+class BatchImage(ctypes.Structure):
+    """Mirror of jxg_batch_image (include/jxg.h)."""
+    _fields_ = [
+        ("d_rgb", ctypes.c_void_p),
+        ("row_bytes", ctypes.c_size_t),
+        ("channels", ctypes.c_int32),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("params", FrameParams),
+        ("d_ytox", ctypes.c_void_p),
+        ("d_ytob", ctypes.c_void_p),
+        ("d_qf", ctypes.c_void_p),
+        ("d_cells", ctypes.c_void_p),
+        ("d_levels", ctypes.c_void_p * 3),
+        ("d_coeffs", ctypes.c_void_p * 3),
+        ("d_dc", ctypes.c_void_p * 3),
     ]
 
 
@@ -131,6 +151,14 @@ def _declare(L):
     L.jxg_alpha_block_factors_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, vp]
     L.jxg_apply_alpha_factors_dev.argtypes = [vp, vp, vp, c.c_uint32, c.c_uint32, vp]
     L.jxg_alpha_plane.argtypes = [vp, vp, c.c_size_t, c.c_int]
+    L.jxg_batch_create.argtypes = [vp, c.c_uint32, c.POINTER(BatchImage), c.POINTER(vp)]
+    L.jxg_batch_destroy.argtypes = [vp]
+    L.jxg_batch_destroy.restype = None
+    L.jxg_batch_run.argtypes = [vp]
+    L.jxg_batch_launch.argtypes = [vp]
+    L.jxg_batch_status.argtypes = [vp, vp]
+    L.jxg_batch_autotune.argtypes = [vp, c.c_int, c.c_char_p, c.c_size_t]
+    L.jxg_batch_config.argtypes = [vp, vp]
     L.jxg_strategy_cost_sites_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32,
                                               c.c_uint32, vp, vp, c.POINTER(FrameParams), c.c_int,
                                               c.c_uint32, vp, vp, c.c_float, vp]

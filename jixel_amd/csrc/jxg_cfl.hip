@@ -12,6 +12,7 @@
 //                  sequential f32 order.
 #include <hip/hip_runtime.h>
 
+#include "jxg_batch.h"
 #include "jxg_ctx.h"
 #include "jxg_device.h"
 
@@ -55,20 +56,21 @@ __device__ __forceinline__ int solve_multiplier(float ca, float cb, int num, flo
 #define CM_ROWS 16
 #define CM_T 128
 
-template <bool F>
-__global__ __launch_bounds__(CM_T) void k_cmap_blocks(const float *__restrict__ xyb, int stride,
+template <bool F, int ROWS>
+__device__ __forceinline__ void cmap_blocks_body(const float *__restrict__ xyb, int stride,
                                                       int padded_h, int W, int H,
                                                       const float *__restrict__ qm_x,
                                                       const float *__restrict__ qm_b, CmapArgs a,
-                                                      float *__restrict__ part) {
-  __shared__ float t[3][CM_ROWS * CM_P];  // X, Y, B
+                                                      float *__restrict__ part, int tiles_x_, int bix, int biy) {
+  __shared__ float t[3][ROWS * CM_P];  // X, Y, B
   const int tid = threadIdx.x;
-  const int tx = blockIdx.x, ty = blockIdx.y >> 2, quarter = blockIdx.y & 3;
-  const int x0 = tx * 64, y0 = ty * 64 + quarter * CM_ROWS;
+  constexpr int PARTS = 64 / ROWS;  // sub-tiles per 64x64 tile
+  const int tx = bix, ty = biy / PARTS, quarter = biy % PARTS;
+  const int x0 = tx * 64, y0 = ty * 64 + quarter * ROWS;
   const size_t plane = (size_t)stride * padded_h;
   // stride and padded_h are multiples of 64: the whole quarter tile is in bounds.
-  for (int e = tid; e < 3 * CM_ROWS * 16; e += CM_T) {
-    const int c = e / (CM_ROWS * 16), r = (e / 16) % CM_ROWS, q = e % 16;
+  for (int e = tid; e < 3 * ROWS * 16; e += CM_T) {
+    const int c = e / (ROWS * 16), r = (e / 16) % ROWS, q = e % 16;
     const float4 v =
         *reinterpret_cast<const float4 *>(xyb + c * plane + (size_t)(y0 + r) * stride + x0 + 4 * q);
     float *d = &t[c][r * CM_P + 4 * q];
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(CM_T) void k_cmap_blocks(const float *__restrict__ 
   }
   __syncthreads();
   // DCT8 rows (src/dct.rs:631-660: rows first, columns scaled by 1/64)
-  constexpr int NB = CM_ROWS / 8 * 8;  // blocks per workgroup
+  constexpr int NB = ROWS / 8 * 8;  // blocks per workgroup
   for (int e = tid; e < 3 * NB * 8; e += CM_T) {
     const int c = e / (NB * 8), rem = e % (NB * 8), b = rem >> 3, r = rem & 7;
     dct_line<F, 8>(&t[c][((b >> 3) * 8 + r) * CM_P + (b & 7) * 8], 1, 0.0f, false);
@@ -113,21 +115,31 @@ __global__ __launch_bounds__(CM_T) void k_cmap_blocks(const float *__restrict__ 
       const float second = (which & 1) ? (which == 1 ? -s_o : fmla<F>(1.0f, m, -s_o)) : am;
       acc = fmla<F>(am, second, acc);
     }
-    const int tiles_x = gridDim.x;
+    const int tiles_x = tiles_x_;
     part[((size_t)(ty * tiles_x + tx) * 64 + quarter * NB + b) * 4 + which] = acc;
   }
+}
+
+template <bool F>
+__global__ __launch_bounds__(CM_T) void k_cmap_blocks(const float *__restrict__ xyb, int stride,
+                                                      int padded_h, int W, int H,
+                                                      const float *__restrict__ qm_x,
+                                                      const float *__restrict__ qm_b, CmapArgs a,
+                                                      float *__restrict__ part) {
+  cmap_blocks_body<F, CM_ROWS>(xyb, stride, padded_h, W, H, qm_x, qm_b, a, part, (int)gridDim.x,
+                               blockIdx.x, blockIdx.y);
 }
 
 // The tile's block sums in raster order, then the two slopes
 // (compute_cmap_tile, src/color_correlation.rs:470-508).  One wavefront per
 // tile: lane b loads block b's four sums, lane 0 adds them in order.
 template <bool F>
-__global__ __launch_bounds__(64) void k_cmap_solve(const float *__restrict__ part, int W, int H,
+__device__ __forceinline__ void cmap_solve_body(const float *__restrict__ part, int W, int H,
                                                    int tiles_x, int tiles_y, CmapArgs a,
                                                    int8_t *__restrict__ ytox,
-                                                   int8_t *__restrict__ ytob) {
+                                                   int8_t *__restrict__ ytob, int bix, int biy) {
   __shared__ float4 s4[64];
-  const int ti = blockIdx.x, lane = threadIdx.x;
+  const int ti = bix, lane = threadIdx.x;
   const int tx = ti % tiles_x, ty = ti / tiles_x;
   s4[lane] = reinterpret_cast<const float4 *>(part)[(size_t)ti * 64 + lane];
   __syncthreads();
@@ -153,7 +165,70 @@ __global__ __launch_bounds__(64) void k_cmap_solve(const float *__restrict__ par
   ytob[ti] = (int8_t)solve_multiplier<F>(ca_b, cb_b, num, dz, units);
 }
 
+template <bool F>
+__global__ __launch_bounds__(64) void k_cmap_solve(const float *__restrict__ part, int W, int H,
+                                                   int tiles_x, int tiles_y, CmapArgs a,
+                                                   int8_t *__restrict__ ytox,
+                                                   int8_t *__restrict__ ytob) {
+  cmap_solve_body<F>(part, W, H, tiles_x, tiles_y, a, ytox, ytob, blockIdx.x, blockIdx.y);
+}
+
+// Batched forms: one launch per stage over every image (jxg_batch.h).
+// ROWS (16 or 32) pixel rows per workgroup: a tile shape the batch
+// autotuner chooses.
+template <bool F, int ROWS>
+__global__ __launch_bounds__(CM_T) void k_cmap_blocks_b(const BatchImg *__restrict__ img,
+                                                        const uint32_t *__restrict__ pfx, int n,
+                                                        const float *__restrict__ qm_x,
+                                                        const float *__restrict__ qm_b) {
+  const int i = batch_image(pfx, n, blockIdx.x);
+  const BatchImg &I = img[i];
+  const uint32_t l = blockIdx.x - pfx[i];
+  const int gw = I.gw[BS_CMAPB];
+  const CmapArgs a{I.distance, I.cfl_base_x, I.cfl_base_b, I.cfl_color_factor};
+  cmap_blocks_body<F, ROWS>(I.xyb, I.stride, I.ph, I.W, I.H, qm_x, qm_b, a, I.part, gw,
+                            (int)(l % gw), (int)(l / gw));
+}
+
+template <bool F>
+__global__ __launch_bounds__(64) void k_cmap_solve_b(const BatchImg *__restrict__ img,
+                                                     const uint32_t *__restrict__ pfx, int n) {
+  const int i = batch_image(pfx, n, blockIdx.x);
+  const BatchImg &I = img[i];
+  const CmapArgs a{I.distance, I.cfl_base_x, I.cfl_base_b, I.cfl_color_factor};
+  cmap_solve_body<F>(I.part, I.W, I.H, I.tiles_x, I.tiles_y, a, I.ytox, I.ytob,
+                     (int)(blockIdx.x - pfx[i]), 0);
+}
+
 }  // namespace
+
+template <bool F>
+static void launch_cmap(hipStream_t st, const BatchImg *d_img, const uint32_t *d_pfx, int n,
+                        const uint32_t total[BS_N], const float *qm_x, const float *qm_b,
+                        int cm_rows) {
+  const int m = n + 1;
+  if (cm_rows == 32)
+    k_cmap_blocks_b<F, 32><<<total[BS_CMAPB], CM_T, 0, st>>>(d_img, d_pfx + BS_CMAPB * m, n, qm_x,
+                                                             qm_b);
+  else
+    k_cmap_blocks_b<F, 16><<<total[BS_CMAPB], CM_T, 0, st>>>(d_img, d_pfx + BS_CMAPB * m, n, qm_x,
+                                                             qm_b);
+  k_cmap_solve_b<F><<<total[BS_CMAPS], 64, 0, st>>>(d_img, d_pfx + BS_CMAPS * m, n);
+}
+
+JXG_HIDDEN int batch_cmap(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                          const uint32_t *d_pfx, int n, const uint32_t total[BS_N], int fused,
+                          int cm_rows) {
+  const uint32_t off = jxg_qt_offset(JXG_QT_DCT8);
+  const float *qm_x = ctx->d_tables + off + 3 * 64;
+  const float *qm_b = ctx->d_tables + off + 5 * 64;
+  if (fused)
+    launch_cmap<true>(st, d_img, d_pfx, n, total, qm_x, qm_b, cm_rows);
+  else
+    launch_cmap<false>(st, d_img, d_pfx, n, total, qm_x, qm_b, cm_rows);
+  LAUNCH_CHECK("batch cmap");
+  return JXG_OK;
+}
 
 JXG_HIDDEN int fill_cmap(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
                          uint32_t width, uint32_t height, const jxg_frame_params *p,

@@ -102,6 +102,20 @@ JXG_HIDDEN int alpha_front(jxg_ctx *ctx, const uint8_t *d_in, size_t row_bytes, 
                            uint32_t ph);
 JXG_HIDDEN int alpha_finish(jxg_ctx *ctx, uint32_t W, uint32_t H, const uint8_t *cells,
                             uint8_t *qf);
+// Batched stage launches (jxg_kernels.hip, jxg_cfl.hip); see jxg_batch.h.
+struct BatchImg;
+JXG_HIDDEN int batch_stages_pre(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                                const uint32_t *d_pfx, int n, const uint32_t total[],
+                                int fused, const float *mlut, int aq_strip);
+JXG_HIDDEN int batch_cmap(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                          const uint32_t *d_pfx, int n, const uint32_t total[], int fused,
+                          int cm_rows);
+JXG_HIDDEN int batch_stages_post(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                                 const uint32_t *d_pfx, int n, const uint32_t total[],
+                                 int fused, uint32_t *xt_list, uint32_t *xt_img, uint32_t xt_cap,
+                                 uint32_t *counters, int xt_grid);
+JXG_HIDDEN void batch_grids(uint32_t W, uint32_t H, uint32_t stride, uint32_t ph, int aq_strip,
+                            int cm_rows, uint32_t gw[], uint32_t gh[]);
 // jxg_chroma.hip: chroma-HF protection of the quant field.
 JXG_HIDDEN int chroma_hf(jxg_ctx *ctx, const float *xyb, uint32_t stride, uint32_t ph, uint32_t W,
                          uint32_t H, float distance, int fused, uint8_t *qf);

@@ -27,6 +27,7 @@
 
 #include <mutex>
 
+#include "jxg_batch.h"
 #include "jxg_ctx.h"
 #include "jxg_device.h"
 #include "jxg_internal.h"
@@ -51,18 +52,18 @@ int set_err(int code, const char *fmt, ...) {
 // XYB
 
 template <bool F, int ORDER>
-__global__ __launch_bounds__(256) void k_xyb(const uint8_t *__restrict__ rgb, size_t row_bytes,
+__device__ __forceinline__ void xyb_body(const uint8_t *__restrict__ rgb, size_t row_bytes,
                                              int ch, int W, int H,
                                              const float *__restrict__ mlut,
-                                             float *__restrict__ xyb, int stride, int padded_h) {
+                                             float *__restrict__ xyb, int stride, int padded_h, int bix, int biy) {
   __shared__ float lut[256];
   __shared__ float m[9];
   const int t = threadIdx.x;
   lut[t] = mlut[9 + t];
   if (t < 9) m[t] = mlut[t];
   __syncthreads();
-  const int x = blockIdx.x * 256 + t;
-  const int y = blockIdx.y;
+  const int x = bix * 256 + t;
+  const int y = biy;
   if (x >= stride) return;
   const int sx = min(x, W - 1), sy = min(y, H - 1);
   const uint8_t *p = rgb + (size_t)sy * row_bytes + (size_t)sx * ch;
@@ -103,6 +104,14 @@ __global__ __launch_bounds__(256) void k_xyb(const uint8_t *__restrict__ rgb, si
   xyb[2 * plane + i] = gray ? Y : tm[2];
 }
 
+template <bool F, int ORDER>
+__global__ __launch_bounds__(256) void k_xyb(const uint8_t *__restrict__ rgb, size_t row_bytes,
+                                             int ch, int W, int H,
+                                             const float *__restrict__ mlut,
+                                             float *__restrict__ xyb, int stride, int padded_h) {
+  xyb_body<F, ORDER>(rgb, row_bytes, ch, W, H, mlut, xyb, stride, padded_h, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Adaptive quantization (src/adaptive_quant.rs:41-206,410-470,547-729)
 
@@ -130,11 +139,11 @@ __device__ __forceinline__ float masking_sqrt(float v) {
 }
 
 template <bool F>
-__global__ __launch_bounds__(256) void k_aq_pre(const float *__restrict__ yp, int stride, int W,
+__device__ __forceinline__ void aq_pre_body(const float *__restrict__ yp, int stride, int W,
                                                 int H, int pre_w, int pre_h,
-                                                float *__restrict__ pre) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  const int j = blockIdx.y;
+                                                float *__restrict__ pre, int bix, int biy) {
+  const int i = bix * 256 + threadIdx.x;
+  const int j = biy;
   if (i >= pre_w || j >= pre_h) return;
   float acc[4];
 #pragma unroll
@@ -159,6 +168,13 @@ __global__ __launch_bounds__(256) void k_aq_pre(const float *__restrict__ yp, in
     }
   }
   pre[(size_t)j * pre_w + i] = (acc[0] + acc[1] + acc[2] + acc[3]) * 0.25f;
+}
+
+template <bool F>
+__global__ __launch_bounds__(256) void k_aq_pre(const float *__restrict__ yp, int stride, int W,
+                                                int H, int pre_w, int pre_h,
+                                                float *__restrict__ pre) {
+  aq_pre_body<F>(yp, stride, W, H, pre_w, pre_h, pre, blockIdx.x, blockIdx.y);
 }
 
 __device__ __forceinline__ void sort4(float *a) {
@@ -205,20 +221,20 @@ __device__ __forceinline__ void store_min4(float v, float *m) {
 // the reference's f32 operation order.
 #define AQ_STRIP 32
 
-template <bool F>
-__global__ __launch_bounds__(256) void k_aq_block(const float *__restrict__ xyb, int stride,
+template <bool F, int STRIP>
+__device__ __forceinline__ void aq_block_body(const float *__restrict__ xyb, int stride,
                                                   int padded_h, int W, int H,
                                                   const float *__restrict__ pre, int pre_w,
                                                   float distance, float inv_scale,
                                                   const float *__restrict__ exp2_table,
-                                                  uint8_t *__restrict__ qf, int xb, int yb) {
-  __shared__ float tg[8][AQ_STRIP * 8];  // gamma pair sum, [dy][block * 8 + dx]
-  __shared__ float thr_[8][AQ_STRIP * 8];  // HF term to the right
-  __shared__ float thd[8][AQ_STRIP * 8];   // HF term downwards
-  __shared__ float tbl[8][AQ_STRIP * 8];   // blue term
-  __shared__ float sums[3][AQ_STRIP];
+                                                  uint8_t *__restrict__ qf, int xb, int yb, int bix, int biy) {
+  __shared__ float tg[8][STRIP * 8];  // gamma pair sum, [dy][block * 8 + dx]
+  __shared__ float thr_[8][STRIP * 8];  // HF term to the right
+  __shared__ float thd[8][STRIP * 8];   // HF term downwards
+  __shared__ float tbl[8][STRIP * 8];   // blue term
+  __shared__ float sums[3][STRIP];
   const int tid = threadIdx.x;
-  const int bx0 = blockIdx.x * AQ_STRIP, by = blockIdx.y;
+  const int bx0 = bix * STRIP, by = biy;
   const size_t plane = (size_t)stride * padded_h;
   const float *px_ = xyb, *py_ = xyb + plane, *pb_ = xyb + 2 * plane;
   const int y = min(by * 8, max(H - 8, 0));
@@ -249,8 +265,8 @@ __global__ __launch_bounds__(256) void k_aq_block(const float *__restrict__ xyb,
   }
   __syncthreads();
   // ---- fold: lane (which, block) runs one sum in the reference's order ----
-  if (tid < 3 * AQ_STRIP) {
-    const int which = tid / AQ_STRIP, b = tid - which * AQ_STRIP;
+  if (tid < 3 * STRIP) {
+    const int which = tid / STRIP, b = tid - which * STRIP;
     if (bx0 + b < xb) {
       float acc = 0.0f;
       if (which == 0) {
@@ -273,7 +289,7 @@ __global__ __launch_bounds__(256) void k_aq_block(const float *__restrict__ xyb,
     }
   }
   __syncthreads();
-  if (tid >= AQ_STRIP) return;
+  if (tid >= STRIP) return;
   const int bx = bx0 + tid;
   if (bx >= xb || by >= yb) return;
   const float d = distance;
@@ -375,6 +391,16 @@ __global__ __launch_bounds__(256) void k_aq_block(const float *__restrict__ xyb,
   int32_t qi = sat_i32(fmla<F>(qv, inv_scale, 0.5f));
   qi = qi < 1 ? 1 : (qi > 255 ? 255 : qi);
   qf[(size_t)by * xb + bx] = (uint8_t)qi;
+}
+
+template <bool F>
+__global__ __launch_bounds__(256) void k_aq_block(const float *__restrict__ xyb, int stride,
+                                                  int padded_h, int W, int H,
+                                                  const float *__restrict__ pre, int pre_w,
+                                                  float distance, float inv_scale,
+                                                  const float *__restrict__ exp2_table,
+                                                  uint8_t *__restrict__ qf, int xb, int yb) {
+  aq_block_body<F, AQ_STRIP>(xyb, stride, padded_h, W, H, pre, pre_w, distance, inv_scale, exp2_table, qf, xb, yb, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -680,7 +706,7 @@ __device__ __forceinline__ float site_bias(const jxg_derived *D, double s1, doub
 }
 
 template <bool F>
-__global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ xyb, int stride,
+__device__ __forceinline__ void select_body(const float *__restrict__ xyb, int stride,
                                                   int padded_h, int W, int H,
                                                   uint8_t *__restrict__ qf,
                                                   const int8_t *__restrict__ ytox,
@@ -692,13 +718,13 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ x
                                                   uint8_t *__restrict__ cells,
                                                   float *__restrict__ c8out,
                                                   float *__restrict__ c16out,
-                                                  float *__restrict__ c32out, int xb, int yb) {
+                                                  float *__restrict__ c32out, int xb, int yb, int bix, int biy) {
   __shared__ SelLds L;
   __shared__ uint8_t q_in[16];
   __shared__ uint8_t cell_out[16];
   __shared__ uint8_t q_out[16];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int rbx = blockIdx.x * 4, rby = blockIdx.y * 4;  // region's first block (global)
+  const int rbx = bix * 4, rby = biy * 4;  // region's first block (global)
   const int gx = rbx >> 8, gy = rby >> 8;
   const int xsize = min(256, xb - gx * 256), ysize = min(256, yb - gy * 256);
   const int bx = rbx - gx * 256, by = rby - gy * 256;  // group-relative
@@ -927,6 +953,23 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ x
   }
 }
 
+template <bool F>
+__global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ xyb, int stride,
+                                                  int padded_h, int W, int H,
+                                                  uint8_t *__restrict__ qf,
+                                                  const int8_t *__restrict__ ytox,
+                                                  const int8_t *__restrict__ ytob,
+                                                  const float *__restrict__ tables,
+                                                  const uint32_t *__restrict__ scans,
+                                                  const float *__restrict__ selq,
+                                                  const jxg_derived *__restrict__ D,
+                                                  uint8_t *__restrict__ cells,
+                                                  float *__restrict__ c8out,
+                                                  float *__restrict__ c16out,
+                                                  float *__restrict__ c32out, int xb, int yb) {
+  select_body<F>(xyb, stride, padded_h, W, H, qf, ytox, ytob, tables, scans, selq, D, cells, c8out, c16out, c32out, xb, yb, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // Forward transform + quantization (src/group.rs:1259-2240, first pass)
 
@@ -1080,6 +1123,8 @@ struct QArgs {
   uint32_t *status;  // [0..1] copy of {count, bad} for the caller, [2] finish ticket
   uint32_t xt_cap;
   uint32_t *bad;  // set to 1 when an unsupported strategy cell is seen
+  uint32_t *xt_img = nullptr;  // batch: the image of each queued transform
+  int img = 0;                 // batch: this image's index
 };
 
 __device__ __forceinline__ int ctz32(int v) { return __builtin_ctz((unsigned)v); }
@@ -1153,7 +1198,7 @@ __device__ __forceinline__ void load_plane(QLds &L, const QArgs &A, int c, int p
 // Every transform of the list on L.cur (one channel): sub-8x8 transforms
 // one block per thread, the DCTs as two passes of 1-D lines.
 template <bool F, bool BIG>
-__device__ void transform_plane(QLds &L, int n) {
+__device__ __forceinline__ void transform_plane(QLds &L, int n) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   // A few threads, each owning one pixel-copy slot, take the sub-8x8 blocks.
   if (tid < QT_SUB8_SLOTS) {
@@ -1487,10 +1532,10 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
 }
 
 template <bool F>
-__global__ __launch_bounds__(256, 4) void k_quantize(QArgs A) {
+__device__ __forceinline__ void quantize_body(QArgs A, int bix, int biy) {
   __shared__ QLds L;
   const int tid = threadIdx.x;
-  const int bx0 = blockIdx.x * 8, by0 = blockIdx.y * 8;
+  const int bx0 = bix * 8, by0 = biy * 8;
   const int nbx = min(8, A.xb - bx0), nby = min(8, A.yb - by0);
   if (tid == 0) L.n = 0;
   __syncthreads();
@@ -1512,13 +1557,21 @@ __global__ __launch_bounds__(256, 4) void k_quantize(QArgs A) {
           L.ly[slot] = ly * 8;
         } else {
           const uint32_t slot = atomicAdd(A.xt_count, 1u);
-          if (slot < A.xt_cap) A.xt_list[slot] = (uint32_t)((by0 + ly) * A.xb + bx0 + lx);
+          if (slot < A.xt_cap) {
+            A.xt_list[slot] = (uint32_t)((by0 + ly) * A.xb + bx0 + lx);
+            if (A.xt_img) A.xt_img[slot] = (uint32_t)A.img;
+          }
         }
       }
     }
   }
   __syncthreads();
   quantize_list<F, false>(L, A, bx0 * 8, by0 * 8, 64, 64);
+}
+
+template <bool F>
+__global__ __launch_bounds__(256, 4) void k_quantize(QArgs A) {
+  quantize_body<F>(A, blockIdx.x, blockIdx.y);
 }
 
 // Transforms whose footprint crosses their 64x64 tile (the layout DP's
@@ -1559,6 +1612,222 @@ __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
       __threadfence();
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Batched stages: one launch per stage over every image of a batch (see
+// jxg_batch.h).  Each workgroup maps its flat index to (image, local 2-D
+// block) and runs the single-image stage body.
+
+#define BATCH_BLOCK(stage)                                   \
+  const int bi_ = batch_image(pfx, n, blockIdx.x);           \
+  const BatchImg &I = img[bi_];                              \
+  const uint32_t bl_ = blockIdx.x - pfx[bi_];                \
+  const int bx_ = (int)(bl_ % (uint32_t)I.gw[stage]);        \
+  const int by_ = (int)(bl_ / (uint32_t)I.gw[stage]);
+
+template <bool F>
+__global__ __launch_bounds__(256) void k_xyb_b(const BatchImg *__restrict__ img,
+                                               const uint32_t *__restrict__ pfx, int n,
+                                               const float *__restrict__ mlut) {
+  BATCH_BLOCK(BS_XYB)
+  xyb_body<F, JXG_XYB_SCALAR>(I.rgb, (size_t)I.row_bytes, I.ch, I.W, I.H, mlut, I.xyb, I.stride,
+                              I.ph, bx_, by_);
+}
+
+template <bool F>
+__global__ __launch_bounds__(256) void k_aq_pre_b(const BatchImg *__restrict__ img,
+                                                  const uint32_t *__restrict__ pfx, int n) {
+  BATCH_BLOCK(BS_AQPRE)
+  aq_pre_body<F>(I.xyb + (size_t)I.stride * I.ph, I.stride, I.W, I.H, I.pre_w, I.pre_h, I.pre, bx_,
+                 by_);
+}
+
+// STRIP blocks per workgroup (STRIP * 8 threads): a tile shape the batch
+// autotuner chooses.
+template <bool F, int STRIP>
+__global__ __launch_bounds__(STRIP * 8) void k_aq_block_b(const BatchImg *__restrict__ img,
+                                                          const uint32_t *__restrict__ pfx, int n,
+                                                          const float *__restrict__ exp2_table) {
+  BATCH_BLOCK(BS_AQBLK)
+  aq_block_body<F, STRIP>(I.xyb, I.stride, I.ph, I.W, I.H, I.pre, I.pre_w, I.distance, I.inv_scale,
+                          exp2_table, I.qf, I.xb, I.yb, bx_, by_);
+}
+
+template <bool F>
+__global__ __launch_bounds__(SEL_T, 5) void k_select_b(const BatchImg *__restrict__ img,
+                                                    const uint32_t *__restrict__ pfx, int n,
+                                                    const float *__restrict__ tables,
+                                                    const uint32_t *__restrict__ scans,
+                                                    const float *__restrict__ selq) {
+  BATCH_BLOCK(BS_SEL)
+  select_body<F>(I.xyb, I.stride, I.ph, I.W, I.H, I.qf, I.ytox, I.ytob, tables, scans, selq, I.D,
+                 I.cells, nullptr, nullptr, nullptr, I.xb, I.yb, bx_, by_);
+}
+
+struct BatchQueue {
+  const float *tables;
+  uint32_t *xt_count, *xt_list, *xt_img, *status, *bad;
+  uint32_t xt_cap;
+};
+
+__device__ __forceinline__ QArgs batch_qargs(const BatchImg &I, const BatchQueue &Q, int i) {
+  QArgs A;
+  A.xyb = I.xyb;
+  A.stride = I.stride;
+  A.padded_h = I.ph;
+  A.W = I.W;
+  A.H = I.H;
+  A.xb = I.xb;
+  A.yb = I.yb;
+  A.cells = I.cells;
+  A.qf = I.qf;
+  A.ytox = I.ytox;
+  A.ytob = I.ytob;
+  A.tables = Q.tables;
+  A.D = I.D;
+  A.lv0 = I.lv[0];
+  A.lv1 = I.lv[1];
+  A.lv2 = I.lv[2];
+  A.co0 = I.co[0];
+  A.co1 = I.co[1];
+  A.co2 = I.co[2];
+  A.dc0 = I.dc[0];
+  A.dc1 = I.dc[1];
+  A.dc2 = I.dc[2];
+  A.xt_count = Q.xt_count;
+  A.xt_list = Q.xt_list;
+  A.xt_img = Q.xt_img;
+  A.status = Q.status;
+  A.xt_cap = Q.xt_cap;
+  A.bad = Q.bad;
+  A.img = i;
+  return A;
+}
+
+template <bool F>
+__global__ __launch_bounds__(256, 4) void k_quantize_b(const BatchImg *__restrict__ img,
+                                                       const uint32_t *__restrict__ pfx, int n,
+                                                       BatchQueue Q) {
+  BATCH_BLOCK(BS_QUANT)
+  quantize_body<F>(batch_qargs(I, Q, bi_), bx_, by_);
+}
+
+// The crossing / DCT64 queue of the whole batch; the last workgroup
+// publishes and clears the counters as k_quantize_xt does.
+template <bool F>
+__global__ __launch_bounds__(256) void k_quantize_xt_b(const BatchImg *__restrict__ img,
+                                                       BatchQueue Q) {
+  __shared__ QLds L;
+  const int tid = threadIdx.x;
+  const uint32_t count = min(*Q.xt_count, Q.xt_cap);
+  for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
+    const int i = (int)Q.xt_img[item];
+    const BatchImg &I = img[i];
+    const QArgs A = batch_qargs(I, Q, i);
+    const uint32_t blk = Q.xt_list[item];
+    const int bx = (int)(blk % (uint32_t)A.xb), by = (int)(blk / (uint32_t)A.xb);
+    const int s = A.cells[blk] >> 1;
+    if (tid == 0) {
+      L.n = 1;
+      L.s[0] = s;
+      L.bx[0] = bx;
+      L.by[0] = by;
+      L.lx[0] = 0;
+      L.ly[0] = 0;
+    }
+    __syncthreads();
+    quantize_list<F, true>(L, A, bx * 8, by * 8, cov_x(s) * 8, cov_y(s) * 8);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    __threadfence();
+    const uint32_t ticket = atomicAdd(&Q.status[2], 1u);
+    if (ticket == gridDim.x - 1) {
+      Q.status[0] = atomicExch(Q.xt_count, 0u);
+      Q.status[1] = atomicExch(Q.bad, 0u);
+      Q.status[2] = 0u;
+      __threadfence();
+    }
+  }
+}
+
+// Stage launches of a batch (the CfL stages live in jxg_cfl.hip).  total[s]
+// is the number of workgroups of stage s; pfx holds BS_N prefixes of n + 1.
+template <bool F>
+static void launch_pre(hipStream_t st, const BatchImg *d_img, const uint32_t *p, int n,
+                       const uint32_t total[BS_N], const float *mlut, const float *exp2,
+                       int aq_strip) {
+  const int m = n + 1;
+  k_xyb_b<F><<<total[BS_XYB], 256, 0, st>>>(d_img, p + BS_XYB * m, n, mlut);
+  k_aq_pre_b<F><<<total[BS_AQPRE], 256, 0, st>>>(d_img, p + BS_AQPRE * m, n);
+  if (aq_strip == 16)
+    k_aq_block_b<F, 16><<<total[BS_AQBLK], 128, 0, st>>>(d_img, p + BS_AQBLK * m, n, exp2);
+  else if (aq_strip == 64)
+    k_aq_block_b<F, 64><<<total[BS_AQBLK], 512, 0, st>>>(d_img, p + BS_AQBLK * m, n, exp2);
+  else
+    k_aq_block_b<F, 32><<<total[BS_AQBLK], 256, 0, st>>>(d_img, p + BS_AQBLK * m, n, exp2);
+}
+
+JXG_HIDDEN int batch_stages_pre(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                                const uint32_t *d_pfx, int n, const uint32_t total[BS_N],
+                                int fused, const float *mlut, int aq_strip) {
+  if (fused)
+    launch_pre<true>(st, d_img, d_pfx, n, total, mlut, ctx->d_exp2, aq_strip);
+  else
+    launch_pre<false>(st, d_img, d_pfx, n, total, mlut, ctx->d_exp2, aq_strip);
+  LAUNCH_CHECK("batch xyb / aq");
+  return JXG_OK;
+}
+
+JXG_HIDDEN int batch_stages_post(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                                 const uint32_t *d_pfx, int n, const uint32_t total[BS_N],
+                                 int fused, uint32_t *xt_list, uint32_t *xt_img, uint32_t xt_cap,
+                                 uint32_t *counters, int xt_grid) {
+  const uint32_t *p = d_pfx;
+  const int m = n + 1;
+  BatchQueue Q;
+  Q.tables = ctx->d_tables;
+  Q.xt_count = counters;
+  Q.bad = counters + 1;
+  Q.status = counters + 4;
+  Q.xt_list = xt_list;
+  Q.xt_img = xt_img;
+  Q.xt_cap = xt_cap;
+  if (fused) {
+    k_select_b<true><<<total[BS_SEL], SEL_T, 0, st>>>(d_img, p + BS_SEL * m, n, ctx->d_tables,
+                                                      ctx->d_scans, ctx->d_selq);
+    k_quantize_b<true><<<total[BS_QUANT], 256, 0, st>>>(d_img, p + BS_QUANT * m, n, Q);
+    k_quantize_xt_b<true><<<xt_grid, 256, 0, st>>>(d_img, Q);
+  } else {
+    k_select_b<false><<<total[BS_SEL], SEL_T, 0, st>>>(d_img, p + BS_SEL * m, n, ctx->d_tables,
+                                                       ctx->d_scans, ctx->d_selq);
+    k_quantize_b<false><<<total[BS_QUANT], 256, 0, st>>>(d_img, p + BS_QUANT * m, n, Q);
+    k_quantize_xt_b<false><<<xt_grid, 256, 0, st>>>(d_img, Q);
+  }
+  LAUNCH_CHECK("batch select / quantize");
+  return JXG_OK;
+}
+
+// Workgroup grids of the single-image stages for a W x H image (the batch
+// uses the same shapes).
+JXG_HIDDEN void batch_grids(uint32_t W, uint32_t H, uint32_t stride, uint32_t ph, int aq_strip,
+                            int cm_rows, uint32_t gw[BS_N], uint32_t gh[BS_N]) {
+  const uint32_t xb = (W + 7) / 8, yb = (H + 7) / 8, tx = (W + 63) / 64, ty = (H + 63) / 64;
+  gw[BS_XYB] = (stride + 255) / 256;
+  gh[BS_XYB] = ph;
+  gw[BS_AQPRE] = (2 * xb + 255) / 256;
+  gh[BS_AQPRE] = 2 * yb;
+  gw[BS_AQBLK] = (xb + aq_strip - 1) / aq_strip;
+  gh[BS_AQBLK] = yb;
+  gw[BS_CMAPB] = tx;
+  gh[BS_CMAPB] = (64 / cm_rows) * ty;
+  gw[BS_CMAPS] = tx * ty;
+  gh[BS_CMAPS] = 1;
+  gw[BS_SEL] = (xb + 3) / 4;
+  gh[BS_SEL] = (yb + 3) / 4;
+  gw[BS_QUANT] = (xb + 7) / 8;
+  gh[BS_QUANT] = (yb + 7) / 8;
 }
 
 // ---------------------------------------------------------------------------

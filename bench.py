@@ -72,6 +72,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile", action="store_true", help="few steps, no CPU leg (for rocprofv3)")
+    ap.add_argument("--batch", type=int, default=512,
+                    help="images in the batched-front-end workload (0: skip it)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched workload (for rocprofv3)")
     return ap.parse_args()
 
 
@@ -189,6 +193,82 @@ def host_path_rate(fe, rgb, params, reps=5):
             "ms": round(sec * 1e3, 3), "note": "pageable host buffers, copies included"}
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def batch_workload(J, torch, fe, stream, n, reps):
+    """The batch workload: n RGB8 images, sides uniform in [64, 2048] and not
+    multiples of 8, quality uniform in [50, 95], photo-like content (seed
+    1000 + i, generated on the device).  Autotunes the batch's tile shapes,
+    then times graph replays of the whole batch, and the same images as a
+    loop of single-image front-end calls."""
+    from jixel_amd import synth
+    dev = torch.device("cuda", 0)
+    sizes = synth.batch_sizes(n, 4)
+    field = synth.one_over_f_field(4096, 4, dev)
+    images = []
+    for i, (w, h, q) in enumerate(sizes):
+        img = synth.photo_like_device(w, h, 1000 + i, dev, field)
+        images.append((img, J.frame_params(J.EncodeConfig().with_quality(q))))
+    del field
+    torch.cuda.synchronize()
+    px = sum(w * h for w, h, _ in sizes)
+    log(f"batch: {n} images, {px / 1e6:.1f} MP generated")
+    b = J.Batch(fe, images)
+    report = b.autotune(reps=2)
+    log("batch: autotuned " + str(b.config()))
+    b.launch()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(reps):
+        b.launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1000.0 / reps
+    ms = max(e0.elapsed_time(e1) / reps, wall)
+    status = b.status()
+    log(f"batch: {ms:.3f} ms per batch")
+    # the same images one front-end call each, into the same buffers
+    def loop():
+        for i, (img, p) in enumerate(images):
+            o = b.outputs(i)
+            fe.run(img, img.shape[1], img.shape[0], p, outputs=o, qf=o["qf"], cells=o["cells"],
+                   ytox=o["ytox"], ytob=o["ytob"])
+    loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    loop()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_loop = max(e0.elapsed_time(e1), (time.perf_counter() - t0) * 1000.0)
+    log(f"batch: single-image loop {ms_loop:.3f} ms")
+    out = {
+        "workload": f"batch_{n}_photo_like_rgb8_sides_64_2048_q50_95",
+        "images": n,
+        "megapixels": round(px / 1e6, 2),
+        "ms_per_batch": round(ms, 3),
+        "mp_per_s": round(px / 1e6 / (ms * 1e-3), 1),
+        "single_image_loop_ms": round(ms_loop, 3),
+        "single_image_loop_mp_per_s": round(px / 1e6 / (ms_loop * 1e-3), 1),
+        "tile_config": b.config(),
+        "autotune_ms": {line.split(" ")[0] + " " + line.split(" ")[1] + " " + line.split(" ")[2]:
+                        float(line.split(" ")[3]) for line in report.strip().splitlines()},
+        "launches_per_batch": 8,
+        "graph": True,
+        "status": list(status),
+        "data": "synthetic, generated on the device (photo-like recipe, seeds 1000 + i)",
+    }
+    b.close()
+    del images
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -203,6 +283,15 @@ def main():
 
     if args.profile:
         args.steps, args.warmup, args.no_cpu = min(args.steps, 5), min(args.warmup, 2), True
+    if args.batch_only:
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        stream = torch.cuda.current_stream(dev)
+        fe = J.FrontEnd(0, stream=stream)
+        print(json.dumps({"batch": batch_workload(J, torch, fe, stream, max(1, args.batch),
+                                                  max(1, min(args.steps, 3)))}), flush=True)
+        fe.close()
+        return
     w, h = args.width, args.height
     rgb = synth.photo_like(w, h, 2)
     cfg = J.EncodeConfig().with_quality(args.quality)
@@ -333,6 +422,8 @@ def main():
     }
     if not args.profile:
         result["host_buffer_path"] = host_path_rate(fe, rgb, p)
+        if args.batch > 0:
+            result["batch"] = batch_workload(J, torch, fe, stream, args.batch, 3)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(rgb, p, threads)

@@ -54,6 +54,64 @@ def photo_like(w, h, seed):
     return np.clip(np.rint(img), 0, 255).astype(np.uint8)
 
 
+def one_over_f_field(side, seed, device):
+    """A 3-channel 1/f-spectrum field of side x side on the device (torch;
+    one FFT size), from which batch images take crops."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    fy = torch.fft.fftfreq(side, device=device, dtype=torch.float64)[:, None]
+    fx = torch.fft.rfftfreq(side, device=device, dtype=torch.float64)[None, :]
+    k = torch.sqrt(fx * fx + fy * fy)
+    k[0, 0] = 1.0
+    amp = 1.0 / k
+    amp[0, 0] = 0.0
+    chans = []
+    for _ in range(3):
+        phase = torch.rand(amp.shape, generator=g, device=device, dtype=torch.float64) * (2.0 * np.pi)
+        chans.append(torch.fft.irfft2(torch.polar(amp, phase), s=(side, side)))
+    return torch.stack(chans, dim=2)
+
+
+def photo_like_device(w, h, seed, device, field):
+    """photo_like's recipe on the device: a crop of a shared 1/f field at a
+    seeded offset, normalized per channel like photo_like, with 40 flat
+    rectangles and Gaussian noise sigma 3 (torch random stream: the same
+    distribution as photo_like, different pixels).  Used for the 512-image
+    batch workload, whose host generation would take minutes."""
+    import torch
+    rng = _rng(seed)
+    side = field.shape[0]
+    ox, oy = int(rng.integers(0, side - w + 1)), int(rng.integers(0, side - h + 1))
+    crop = field[oy:oy + h, ox:ox + w, :]
+    out = crop / (crop.std(dim=(0, 1), keepdim=True) + 1e-12) * 48.0
+    img = 128.0 + 0.6 * out.mean(dim=2, keepdim=True) + 0.4 * out
+    rects = torch.tensor(rng.uniform(0, 255, size=(40, 3)), device=device)
+    for r in range(40):
+        rw = int(rng.integers(max(2, w // 64), max(3, w // 4)))
+        rh = int(rng.integers(max(2, h // 64), max(3, h // 4)))
+        x0 = int(rng.integers(0, max(1, w - rw)))
+        y0 = int(rng.integers(0, max(1, h - rh)))
+        img[y0:y0 + rh, x0:x0 + rw, :] = rects[r]
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    img += torch.randn(img.shape, generator=g, device=device, dtype=torch.float64) * 3.0
+    return img.round().clamp(0, 255).to(torch.uint8).contiguous()
+
+
+def batch_sizes(n, seed):
+    """(width, height, quality) of the batch workload: sides uniform in
+    [64, 2048] and not multiples of 8, quality uniform in [50, 95]."""
+    rng = _rng(seed)
+    out = []
+    for _ in range(n):
+        w, h = (int(v) for v in rng.integers(64, 2049, size=2))
+        w = w - 1 if w % 8 == 0 else w
+        h = h - 1 if h % 8 == 0 else h
+        out.append((w, h, float(rng.uniform(50.0, 95.0))))
+    return out
+
+
 def gradient_sines(w, h, seed):
     rng = _rng(seed)
     y, x = np.mgrid[0:h, 0:w].astype(np.float64)

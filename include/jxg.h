@@ -372,6 +372,44 @@ int jxg_batch_autotune(jxg_batch *batch, int reps, char *report, size_t report_l
 /* The tile shapes in use: {aq_strip, cm_rows, xt_grid}. */
 int jxg_batch_config(jxg_batch *batch, int out[3]);
 
+/* ---- quant-field tuning by gradient descent ----
+ * No counterpart in the reference (its quant field comes from the AQ
+ * heuristics and the strategy search).  A differentiable proxy of the coding
+ * loss of a fixed layout as a function of the per-block quant field theta:
+ * for every DCT8..DCT32 transform T (theta_T = mean over its blocks) and
+ * channel c, the AC coefficients get additive uniform noise of the
+ * quantizer's step s_k = 1 / (inv_c[k] * scale * theta_T * qm_c), the exact
+ * inverse transform reconstructs the block, and
+ *   L = sum_c w_c |Xhat - X|^2 (in-image pixels) + lambda sum_k log2(1 + |Chat_k / s_k|).
+ * The gradient comes from a hand-written adjoint (the adjoint of the
+ * inverse transform is the unscaled forward product); Adam runs on
+ * ln theta; the result is rounded back to an integer field that is
+ * constant over every transform, so the quantize stage and the bitstream
+ * take it as they take the heuristic field.  Transforms whose footprint
+ * leaves their 32x32 region, sub-8x8 and DCT64 transforms keep their field
+ * (counted by jxg_qf_tuner_excluded).  d_xyb and d_cells must stay valid
+ * for the tuner's lifetime. */
+typedef struct jxg_qf_tuner jxg_qf_tuner;
+int jxg_qf_tuner_create(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                        uint32_t width, uint32_t height, const uint8_t *d_cells,
+                        const jxg_frame_params *params, float lambda, uint32_t seed,
+                        uint32_t max_steps, jxg_qf_tuner **out);
+void jxg_qf_tuner_destroy(jxg_qf_tuner *tuner);
+/* Transforms left out of the objective (synchronizes). */
+int jxg_qf_tuner_excluded(jxg_qf_tuner *tuner, uint32_t *count);
+/* L and dL/dtheta per block (xblocks x yblocks floats, 0 for left-out
+ * blocks) at the field d_theta with the noise of step `step`; *loss (host,
+ * may be NULL) synchronizes. */
+int jxg_qf_tuner_loss_grad(jxg_qf_tuner *tuner, const float *d_theta, uint32_t step,
+                           float *d_grad, float *loss);
+/* `steps` Adam steps (b1 0.9, b2 0.999, eps 1e-8) at learning rate lr on
+ * ln theta from the field d_qf_in, then the rounded field into d_qf_out
+ * (may equal d_qf_in).  d_theta_out (may be NULL) receives the final real
+ * field; loss_trace (host, steps + 1 floats, may be NULL; synchronizes)
+ * the loss before each step and at the final field. */
+int jxg_qf_tuner_run(jxg_qf_tuner *tuner, const uint8_t *d_qf_in, uint32_t steps, float lr,
+                     uint8_t *d_qf_out, float *d_theta_out, float *loss_trace);
+
 /* Host-buffer convenience form of jxg_front_end_dev for callers that own
  * host memory (the reference's encode_image input is a host slice,
  * src/encode_image.rs:1610): uploads, runs, downloads, synchronizes.

@@ -21,7 +21,7 @@ from ._lib import FrameParams, JxgError, check, lib
 
 __all__ = [
     "Speed", "EncodeConfig", "FrameParams", "JxgError", "FrontEnd", "FrontEndOutput", "LumaProxy",
-    "Batch",
+    "Batch", "QfTuner",
     "distance_from_quality", "frame_params", "default_quant_tables", "srgb_lut",
     "plane_geometry", "qt_offset", "qt_size", "encode_front_end", "XYB_SPEC_FWD",
     "STRATEGY",
@@ -466,6 +466,56 @@ class Batch:
     def close(self):
         if self.handle:
             lib().jxg_batch_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class QfTuner:
+    """Gradient tuning of the quant field (jxg_qf_tuner_*) for one image:
+    QfTuner(fe, xyb, width, height, cells, params, lam=0.01, seed=1).
+    loss_grad(theta, step) -> (loss, grad); run(qf, steps, lr) -> (integer
+    field, real field, loss trace)."""
+
+    def __init__(self, fe, xyb, width, height, cells, params, lam=0.01, seed=1, max_steps=64):
+        self.fe = fe
+        self.keep = (xyb, cells)  # the tuner reads them at every step
+        self.width, self.height = width, height
+        self.handle = ctypes.c_void_p()
+        check(lib().jxg_qf_tuner_create(fe.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
+                                        height, _ptr(cells), ctypes.byref(params), float(lam),
+                                        int(seed), int(max_steps), ctypes.byref(self.handle)))
+
+    def excluded(self):
+        n = ctypes.c_uint32()
+        check(lib().jxg_qf_tuner_excluded(self.handle, ctypes.byref(n)))
+        return int(n.value)
+
+    def loss_grad(self, theta, step=0):
+        t = self.fe.torch
+        theta = theta.to(t.float32).contiguous()
+        grad = t.empty_like(theta)
+        loss = ctypes.c_float()
+        check(lib().jxg_qf_tuner_loss_grad(self.handle, _ptr(theta), int(step), _ptr(grad),
+                                           ctypes.byref(loss)))
+        return float(loss.value), grad
+
+    def run(self, qf, steps=20, lr=0.05):
+        t = self.fe.torch
+        out = t.empty_like(qf)
+        theta = t.empty(qf.shape, dtype=t.float32, device=qf.device)
+        trace = (ctypes.c_float * (steps + 1))()
+        check(lib().jxg_qf_tuner_run(self.handle, _ptr(qf), int(steps), float(lr), _ptr(out),
+                                     _ptr(theta), trace))
+        return out, theta, [float(v) for v in trace]
+
+    def close(self):
+        if self.handle:
+            lib().jxg_qf_tuner_destroy(self.handle)
             self.handle = ctypes.c_void_p()
 
     def __del__(self):

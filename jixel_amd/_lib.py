@@ -28,6 +28,8 @@ EXPORTED = [
     "jxg_apply_alpha_factors_dev", "jxg_alpha_plane", "jxg_chroma_hf_protection_dev",
     "jxg_strategy_cost_sites_dev", "jxg_batch_create", "jxg_batch_destroy", "jxg_batch_run",
     "jxg_batch_launch", "jxg_batch_status", "jxg_batch_autotune", "jxg_batch_config",
+    "jxg_qf_tuner_create", "jxg_qf_tuner_destroy", "jxg_qf_tuner_excluded",
+    "jxg_qf_tuner_loss_grad", "jxg_qf_tuner_run",
 ]
 
 
@@ -151,6 +153,14 @@ def _declare(L):
     L.jxg_alpha_block_factors_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, vp]
     L.jxg_apply_alpha_factors_dev.argtypes = [vp, vp, vp, c.c_uint32, c.c_uint32, vp]
     L.jxg_alpha_plane.argtypes = [vp, vp, c.c_size_t, c.c_int]
+    L.jxg_qf_tuner_create.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32, vp,
+                                      c.POINTER(FrameParams), c.c_float, c.c_uint32, c.c_uint32,
+                                      c.POINTER(vp)]
+    L.jxg_qf_tuner_destroy.argtypes = [vp]
+    L.jxg_qf_tuner_destroy.restype = None
+    L.jxg_qf_tuner_excluded.argtypes = [vp, c.POINTER(c.c_uint32)]
+    L.jxg_qf_tuner_loss_grad.argtypes = [vp, vp, c.c_uint32, vp, c.POINTER(c.c_float)]
+    L.jxg_qf_tuner_run.argtypes = [vp, vp, c.c_uint32, c.c_float, vp, vp, vp]
     L.jxg_batch_create.argtypes = [vp, c.c_uint32, c.POINTER(BatchImage), c.POINTER(vp)]
     L.jxg_batch_destroy.argtypes = [vp]
     L.jxg_batch_destroy.restype = None

@@ -156,6 +156,16 @@ float jxo_chroma_hf_strength(float distance);
 void jxo_chroma_hf_protection(const float *xyb, int width, int height, uint8_t *qf,
                               float distance);
 
+/* ---- quant-field tuning objective (jxo_adjoint.c) ---- */
+/* The tuning noise u in [-1/2, 1/2) of (seed, step, channel, first block,
+ * storage index). */
+float jxo_tune_noise(uint32_t seed, uint32_t step, int c, uint32_t first_block, int k);
+int jxo_tune_supported(int strategy);
+/* FP64 loss and per-block gradient dL/dtheta of the tuning objective. */
+double jxo_tune_loss_grad(const float *xyb, int width, int height, const uint8_t *cells,
+                          const double *theta, const float *tables, const jxg_frame_params *p,
+                          double lambda, uint32_t seed, uint32_t step, double *grad);
+
 /* ---- alpha (jxo_alpha.c) ---- */
 /* alpha_fringe::clear for u8 samples (src/alpha_fringe.rs:77-225). */
 void jxo_alpha_fringe_clear(uint8_t *alpha, int width, int height);

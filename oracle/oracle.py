@@ -87,6 +87,12 @@ def _declare(L):
     L.jxo_front_end.argtypes = [_u8p, c.c_size_t, c.c_int, c.c_int, c.c_int, c.c_void_p, _f32p,
                                 _i8p, _i8p, c.c_int, _u8p, _u8p, c.POINTER(_i32p),
                                 c.POINTER(_f32p), c.POINTER(_i16p), _u8p]
+    L.jxo_tune_noise.restype = c.c_float
+    L.jxo_tune_noise.argtypes = [c.c_uint32, c.c_uint32, c.c_int, c.c_uint32, c.c_int]
+    L.jxo_tune_loss_grad.restype = c.c_double
+    L.jxo_tune_loss_grad.argtypes = [_f32p, c.c_int, c.c_int, _u8p, c.POINTER(c.c_double), _f32p,
+                                     c.c_void_p, c.c_double, c.c_uint32, c.c_uint32,
+                                     c.POINTER(c.c_double)]
     L.jxo_chroma_hf_stats.argtypes = [_f32p, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int,
                                       _f32p, c.POINTER(c.c_int)]
     L.jxo_chroma_hf_strength.restype = c.c_float
@@ -364,3 +370,22 @@ def chroma_hf_protection(xyb, qf, distance):
     lib().jxo_chroma_hf_protection(_p(x, _f32p), x.shape[2], x.shape[1], _p(q, _u8p),
                                    float(distance))
     return q
+
+
+def tune_loss_grad(xyb, cells, theta, tables, params, lam, seed, step, with_grad=True):
+    """FP64 tuning loss and per-block gradient of (3, H, W) XYB."""
+    x = np.ascontiguousarray(xyb, np.float32)
+    th = np.ascontiguousarray(theta, np.float64)
+    g = np.zeros_like(th) if with_grad else None
+    dp = ctypes.POINTER(ctypes.c_double)
+    loss = lib().jxo_tune_loss_grad(_p(x, _f32p), x.shape[2], x.shape[1],
+                                    _p(np.ascontiguousarray(cells, np.uint8), _u8p),
+                                    th.ctypes.data_as(dp),
+                                    _p(np.ascontiguousarray(tables, np.float32), _f32p),
+                                    ctypes.byref(params), float(lam), int(seed), int(step),
+                                    g.ctypes.data_as(dp) if g is not None else None)
+    return loss, g
+
+
+def tune_noise(seed, step, c, first, k):
+    return float(lib().jxo_tune_noise(seed, step, c, first, k))

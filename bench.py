@@ -76,6 +76,9 @@ def parse():
                     help="images in the batched-front-end workload (0: skip it)")
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the batched workload (for rocprofv3)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the quant-field tuning workload")
+    ap.add_argument("--tune-only", action="store_true",
+                    help="run only the quant-field tuning workload (for rocprofv3)")
     return ap.parse_args()
 
 
@@ -269,6 +272,58 @@ def batch_workload(J, torch, fe, stream, n, reps):
     return out
 
 
+def tune_workload(J, torch, fe, stream, n=16, side=1024, steps=20, lr=0.05, lam=0.01):
+    """Quant-field tuning: n photo-like side x side RGB8 images at q75 through
+    the front end, then `steps` Adam steps of the gradient tuner per image
+    (lambda, lr as given) and the rounded integer field.  Reports the device
+    time of the tuning (forward + adjoint + Adam) and the proxy loss before
+    and after."""
+    from jixel_amd import synth
+    dev = torch.device("cuda", 0)
+    field = synth.one_over_f_field(4096, 5, dev)
+    p = J.frame_params(J.EncodeConfig().with_quality(75.0))
+    jobs = []
+    for i in range(n):
+        rgb = synth.photo_like_device(side, side, 5000 + i, dev, field)
+        o = fe.run(rgb, side, side, p)
+        xyb = fe.xyb(rgb, side, side)
+        jobs.append((xyb, o["cells"], o["qf"]))
+    del field
+    torch.cuda.synchronize()
+    log(f"tune: {n} images through the front end")
+    tuners = [J.QfTuner(fe, xyb, side, side, cells, p, lam=lam, seed=7, max_steps=steps)
+              for xyb, cells, _ in jobs]
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    traces = []
+    for t, (_, _, qf) in zip(tuners, jobs):
+        _, _, trace = t.run(qf, steps=steps, lr=lr)
+        traces.append(trace)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = max(e0.elapsed_time(e1), (time.perf_counter() - t0) * 1000.0)
+    log(f"tune: {ms:.2f} ms")
+    for t in tuners:
+        t.close()
+    before = sum(tr[0] for tr in traces)
+    after = sum(tr[-1] for tr in traces)
+    px = n * side * side
+    return {
+        "workload": f"qf_tune_{n}x{side}x{side}_photo_like_q75",
+        "images": n, "steps": steps, "lr": lr, "lambda": lam,
+        "ms_total": round(ms, 3),
+        "ms_per_adam_step_per_mp": round(ms / steps / (px / 1e6), 4),
+        "mp_per_s_per_step": round(px / 1e6 * steps / (ms * 1e-3), 1),
+        "loss_before": round(before, 2),
+        "loss_after": round(after, 2),
+        "note": "proxy loss (channel-weighted squared error + lambda * log-rate); the "
+                "SSIMULACRA2-at-equal-bytes target needs the CPU back end and is not measured",
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -283,6 +338,14 @@ def main():
 
     if args.profile:
         args.steps, args.warmup, args.no_cpu = min(args.steps, 5), min(args.warmup, 2), True
+    if args.tune_only:
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        stream = torch.cuda.current_stream(dev)
+        fe = J.FrontEnd(0, stream=stream)
+        print(json.dumps({"qf_tune": tune_workload(J, torch, fe, stream)}), flush=True)
+        fe.close()
+        return
     if args.batch_only:
         dev = torch.device("cuda", 0)
         torch.cuda.set_device(dev)
@@ -424,6 +487,8 @@ def main():
         result["host_buffer_path"] = host_path_rate(fe, rgb, p)
         if args.batch > 0:
             result["batch"] = batch_workload(J, torch, fe, stream, args.batch, 3)
+        if not args.no_tune:
+            result["qf_tune"] = tune_workload(J, torch, fe, stream)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(rgb, p, threads)

@@ -443,6 +443,20 @@ int jxg_dct64_image_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint3
                         uint32_t width, uint32_t height, int method, int fused,
                         float *d_out[3]);
 
+/* DCT128X128 of every 128x128 block (ceil(W/128) x ceil(H/128) blocks; the
+ * planes must reach whole 128-px blocks: stride >= 128 ceil(W/128) and
+ * padded_h >= 128 ceil(H/128), edge-replicated), the 128 family's square
+ * transform: method JXG_DCT64_PLAIN runs the reference's FP32 butterflies
+ * in its order (dct128x128_scalar_input, src/dct128.rs:385; 128-point lines
+ * dct1d_128 :330 over two 64-point halves), JXG_DCT64_MFMA_BF16X3 computes
+ * C X C^T / 16384 on the bf16 matrix cores with split operands.  d_out[c]
+ * receives 16384 coefficients per block (out[u * 128 + v]) in the chunk
+ * layout with xblocks = 16 ceil(W/128).  The reference builds the 128
+ * family only with its `extraslow` feature. */
+int jxg_dct128_image_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                         uint32_t width, uint32_t height, int method, int fused,
+                         float *d_out[3]);
+
 /* ---- EPF luma proxy ----
  * The six-term score the EPF fit ranks reconstructions with: structure pools
  * of the half- and quarter-resolution luma (5x5 mirrored box moments) and

@@ -355,6 +355,23 @@ class FrontEnd:
         check(lib().jxg_alpha_plane(self.ctx, _ptr(out), width * height, 0))
         return out
 
+    def dct128_image(self, xyb, width, height, method=0, fused=True, out=None):
+        """DCT128X128 of every 128x128 block (method 0: FP32 butterflies in
+        the reference's order, 1: split-bf16 x3 on the matrix cores); xyb
+        must reach whole 128-px blocks (see xyb_buffer128).  Returns 3 float
+        tensors in the chunk layout with xblocks = 16 ceil(W/128)."""
+        n = 16384 * ((width + 127) // 128) * ((height + 127) // 128)
+        out = [self.empty(n, self.torch.float32) for _ in range(3)] if out is None else out
+        arr = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in out])
+        check(lib().jxg_dct128_image_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
+                                         height, int(method), 1 if fused else 0, arr))
+        return out
+
+    def xyb_buffer128(self, width, height):
+        """XYB planes padded to whole 128-px blocks (for dct128_image)."""
+        return self.empty((3, (height + 127) // 128 * 128, (width + 127) // 128 * 128),
+                          self.torch.float32)
+
     def quantize_status(self):
         out = (ctypes.c_uint32 * 2)()
         check(lib().jxg_quantize_status(self.ctx, out))

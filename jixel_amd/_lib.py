@@ -29,7 +29,7 @@ EXPORTED = [
     "jxg_strategy_cost_sites_dev", "jxg_batch_create", "jxg_batch_destroy", "jxg_batch_run",
     "jxg_batch_launch", "jxg_batch_status", "jxg_batch_autotune", "jxg_batch_config",
     "jxg_qf_tuner_create", "jxg_qf_tuner_destroy", "jxg_qf_tuner_excluded",
-    "jxg_qf_tuner_loss_grad", "jxg_qf_tuner_run",
+    "jxg_qf_tuner_loss_grad", "jxg_qf_tuner_run", "jxg_dct128_image_dev",
 ]
 
 
@@ -153,6 +153,8 @@ def _declare(L):
     L.jxg_alpha_block_factors_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, vp]
     L.jxg_apply_alpha_factors_dev.argtypes = [vp, vp, vp, c.c_uint32, c.c_uint32, vp]
     L.jxg_alpha_plane.argtypes = [vp, vp, c.c_size_t, c.c_int]
+    L.jxg_dct128_image_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
+                                       c.c_int, c.c_int, vp]
     L.jxg_qf_tuner_create.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32, vp,
                                       c.POINTER(FrameParams), c.c_float, c.c_uint32, c.c_uint32,
                                       c.POINTER(vp)]

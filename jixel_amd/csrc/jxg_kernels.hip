@@ -2283,6 +2283,8 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_exp2);
   (void)hipFree(ctx->d_xt);
   (void)hipFree(ctx->d_scanpos);
+  (void)hipFree(ctx->d_c128);
+  (void)hipFree(ctx->d_wc128);
   (void)hipFree(ctx->d_astat);
   (void)hipFree(ctx->d_alpha);
   (void)hipFree(ctx->d_alpha2);

@@ -75,6 +75,11 @@ const float *jxo_wc64(void);
 /* The AFV basis the oracle uses (16 x 16, row j = pixel j). */
 const float *jxo_afv_basis(void);
 int jxo_covered_y(int strategy);
+/* The 128-point butterfly constants the oracle uses (64 floats). */
+const float *jxo_wc128(void);
+/* 128-family forward transforms: kind 0 128x128, 1 128 tall x 64 wide,
+ * 2 64 tall x 128 wide; in = footprint with stride = its width. */
+void jxo_dct128(int kind, const float *in, float *out);
 
 /* ---- quantization ---- */
 void jxo_ac_thresholds(int c, int xsize, int ysize, float distance, float qm_mult,

@@ -4,6 +4,7 @@
  * the first-pass quantization of write_ac_group (src/group.rs).
  * TEST INFRASTRUCTURE ONLY (see jxo.h).
  */
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -137,6 +138,79 @@ static void dct1d_64(float *buf) {
   for (int i = 0; i < 32; i++) {
     buf[2 * i] = tmp[i];
     buf[2 * i + 1] = tmp[32 + i];
+  }
+}
+
+/* WC128: 1 / (2 cos((2i+1) pi / 256)) rounded to f32, which equals the
+ * reference's literals (src/dct128.rs:76-141) at every index. */
+static float WC128_[64];
+static pthread_once_t wc128_once = PTHREAD_ONCE_INIT;
+static void wc128_init(void) {
+  for (int i = 0; i < 64; i++) WC128_[i] = (float)(0.5 / cos((2 * i + 1) * M_PI / 256.0));
+}
+const float *jxo_wc128(void) {
+  pthread_once(&wc128_once, wc128_init);
+  return WC128_;
+}
+
+/* dct1d_128 (src/dct128.rs:330-352) */
+static void dct1d_128(float *buf) {
+  const float *wc = jxo_wc128();
+  float tmp[128];
+  for (int i = 0; i < 64; i++) {
+    tmp[i] = buf[i] + buf[127 - i];
+    tmp[64 + i] = (buf[i] - buf[127 - i]) * wc[i];
+  }
+  dct1d_64(tmp);
+  dct1d_64(tmp + 64);
+  tmp[64] = jxo_fmla(tmp[64], SQRT2F, tmp[65]);
+  for (int i = 65; i < 127; i++) tmp[i] += tmp[i + 1];
+  for (int i = 0; i < 64; i++) {
+    buf[2 * i] = tmp[i];
+    buf[2 * i + 1] = tmp[64 + i];
+  }
+}
+
+/* The 128-family 2-D assemblies: kind 0 DCT128X128 (dct128x128_scalar_input,
+ * src/dct128.rs:385-410: columns, then rows scaled 1/16384, out[u*128+v]),
+ * kind 1 DCT128X64 (64 wide, 128 tall; :416-441: 128-point columns, 64-point
+ * rows, out[u*128+v], 1/8192), kind 2 DCT64X128 (128 wide, 64 tall;
+ * :446-468: 128-point rows, 64-point columns, out[v*128+u], 1/8192).  `in`
+ * has stride W (the footprint's width). */
+void jxo_dct128(int kind, const float *in, float *out) {
+  float line[128];
+  if (kind == 0 || kind == 1) {
+    const int W = kind == 0 ? 128 : 64;
+    float *after = (float *)malloc((size_t)128 * W * sizeof(float));
+    for (int x = 0; x < W; x++) {
+      for (int y = 0; y < 128; y++) line[y] = in[(size_t)y * W + x];
+      dct1d_128(line);
+      for (int v = 0; v < 128; v++) after[(size_t)v * W + x] = line[v];
+    }
+    const float scale = kind == 0 ? 1.0f / 16384.0f : 1.0f / 8192.0f;
+    for (int v = 0; v < 128; v++) {
+      float *row = after + (size_t)v * W;
+      if (kind == 0)
+        dct1d_128(row);
+      else
+        dct1d_64(row);
+      for (int u = 0; u < W; u++) out[u * 128 + v] = row[u] * scale;
+    }
+    free(after);
+  } else {
+    float *after = (float *)malloc((size_t)64 * 128 * sizeof(float));
+    for (int y = 0; y < 64; y++) {
+      memcpy(after + y * 128, in + (size_t)y * 128, 128 * sizeof(float));
+      dct1d_128(after + y * 128);
+    }
+    const float scale = 1.0f / 8192.0f;
+    for (int u = 0; u < 128; u++) {
+      float col[64];
+      for (int y = 0; y < 64; y++) col[y] = after[y * 128 + u];
+      dct1d_64(col);
+      for (int v = 0; v < 64; v++) out[v * 128 + u] = col[v] * scale;
+    }
+    free(after);
   }
 }
 

@@ -87,6 +87,8 @@ def _declare(L):
     L.jxo_front_end.argtypes = [_u8p, c.c_size_t, c.c_int, c.c_int, c.c_int, c.c_void_p, _f32p,
                                 _i8p, _i8p, c.c_int, _u8p, _u8p, c.POINTER(_i32p),
                                 c.POINTER(_f32p), c.POINTER(_i16p), _u8p]
+    L.jxo_wc128.restype = _f32p
+    L.jxo_dct128.argtypes = [c.c_int, _f32p, _f32p]
     L.jxo_tune_noise.restype = c.c_float
     L.jxo_tune_noise.argtypes = [c.c_uint32, c.c_uint32, c.c_int, c.c_uint32, c.c_int]
     L.jxo_tune_loss_grad.restype = c.c_double
@@ -389,3 +391,17 @@ def tune_loss_grad(xyb, cells, theta, tables, params, lam, seed, step, with_grad
 
 def tune_noise(seed, step, c, first, k):
     return float(lib().jxo_tune_noise(seed, step, c, first, k))
+
+
+def wc128():
+    p = lib().jxo_wc128()
+    return np.array([p[i] for i in range(64)], np.float32)
+
+
+def dct128(kind, block):
+    """128-family transform of a footprint ((128, 128), (128, 64) or (64, 128)
+    rows x cols for kind 0, 1, 2) in the reference's storage order."""
+    b = np.ascontiguousarray(block, np.float32)
+    out = np.zeros(b.size, np.float32)
+    lib().jxo_dct128(int(kind), _p(b, _f32p), _p(out, _f32p))
+    return out

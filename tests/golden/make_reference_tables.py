@@ -12,6 +12,7 @@ tables against:
   the stored DCT8 and 16x8 step tables.
 * WC8 / WC16 / WC32 / WC64 (src/dct.rs): butterfly constants.
 * RESAMPLE_SCALE_64_TO_8 (src/dct.rs:2005): DCT64 DC resampling scales.
+* WC128 (src/dct128.rs:76): the 128-point butterfly constants.
 * AFV_BASIS_TRANSPOSE (src/afv.rs:48): the 16x16 AFV basis, row j = pixel j.
 
 Usage: python tests/golden/make_reference_tables.py [/root/reference]
@@ -90,6 +91,12 @@ def main():
 
     m = re.search(r"const RESAMPLE_SCALE_64_TO_8: \[f32; 8\] = ", dct)
     out["RESAMPLE_SCALE_64_TO_8"] = numbers(block_after(dct[m.start():], "RESAMPLE_SCALE_64_TO_8"))[-8:]
+
+    d128 = read("src/dct128.rs")
+    m = re.search(r"static WC128: \[f32; 64\] = ", d128)
+    vals = numbers(block_after(d128[m.start():], "WC128"))[-64:]
+    assert len(vals) == 64
+    out["WC128"] = vals
 
     afv = read("src/afv.rs")
     vals = numbers(block_after(afv, "static AFV_BASIS_TRANSPOSE"))[-256:]

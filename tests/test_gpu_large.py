@@ -80,3 +80,61 @@ def test_dct64_mfma_split_bf16_error(J, fe, torch, oracle, w, h):
         f"FP32 butterflies {worst_plain:.3e}")
     assert worst <= 2e-5
     assert worst_plain <= 1e-5
+
+
+def padded128(xyb):
+    _, h, w = xyb.shape
+    H, W = (h + 127) // 128 * 128, (w + 127) // 128 * 128
+    return np.ascontiguousarray(np.pad(xyb, ((0, 0), (0, H - h), (0, W - w)), mode="edge"))
+
+
+def block128(out, w, tx, ty):
+    """The 16384 storage-order coefficients of 128x128 block (tx, ty)."""
+    xb = 16 * ((w + 127) // 128)
+    o = out.reshape(-1, xb, 64)
+    parts = [o[ty * 16 + j, tx * 16:tx * 16 + 16, :].reshape(1024) for j in range(16)]
+    return np.concatenate(parts)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("w,h", [(128, 128), (300, 200)])
+def test_dct128_plain_bit_exact(J, fe, torch, oracle, w, h, fused):
+    log("dct128 plain", w, h, fused)
+    oracle.set_fused(fused)
+    pad = padded128(image(w, h, 7))
+    out = [o.cpu().numpy() for o in fe.dct128_image(torch.from_numpy(pad).cuda(), w, h, 0, fused)]
+    for ty in range((h + 127) // 128):
+        for tx in range((w + 127) // 128):
+            for c in range(3):
+                blk = pad[c, ty * 128:ty * 128 + 128, tx * 128:tx * 128 + 128]
+                want = oracle.dct128(0, blk)
+                got = block128(out[c], w, tx, ty)
+                assert same_bits(got, want), (tx, ty, c, first_mismatch(got, want))
+    oracle.set_fused(True)
+
+
+def test_dct128_mfma_split_bf16_error(J, fe, torch, oracle):
+    w, h = 384, 256
+    pad = padded128(image(w, h, 8))
+    d = torch.from_numpy(pad).cuda()
+    out = [o.cpu().numpy() for o in fe.dct128_image(d, w, h, 1)]
+    plain = [o.cpu().numpy() for o in fe.dct128_image(d, w, h, 0)]
+    k = np.arange(128)[:, None]
+    i = np.arange(128)[None, :]
+    Cm = np.cos(np.pi * k * (2 * i + 1) / 256)
+    Cm[1:] *= math.sqrt(2.0)
+    worst = worst_plain = 0.0
+    for ty in range(h // 128):
+        for tx in range(w // 128):
+            for c in range(3):
+                x = pad[c, ty * 128:ty * 128 + 128, tx * 128:tx * 128 + 128].astype(np.float64)
+                want = (Cm @ x @ Cm.T / 16384.0).T.reshape(-1)  # storage u * 128 + v
+                scale = np.abs(want).max()
+                got = block128(out[c], w, tx, ty).astype(np.float64)
+                gp = block128(plain[c], w, tx, ty).astype(np.float64)
+                worst = max(worst, np.abs(got - want).max() / scale)
+                worst_plain = max(worst_plain, np.abs(gp - want).max() / scale)
+    log(f"dct128 {w}x{h}: max error / max |coef| vs FP64: split-bf16x3 {worst:.3e}, "
+        f"FP32 butterflies {worst_plain:.3e}")
+    assert worst <= 2e-5
+    assert worst_plain <= 1e-5

@@ -165,6 +165,45 @@ def dct64_forms(fe, xyb, w, h, stream, reps):
     return res
 
 
+def dct128_forms(fe, xyb, w, h, stream, reps):
+    """DCT128X128 of every 128x128 block in both forms, as dct64_forms.  The
+    planes of a 2048-multiple image already reach whole 128-px blocks."""
+    import math
+    import torch
+    nblk = ((w + 127) // 128) * ((h + 127) // 128)
+    px = 128 * 128 * nblk
+    outs = {m: fe.dct128_image(xyb, w, h, m) for m in (0, 1)}
+    k = np.arange(128)[:, None]
+    cm = np.cos(np.pi * k * (2 * np.arange(128)[None, :] + 1) / 256)
+    cm[1:] *= math.sqrt(2.0)
+    planes = xyb.cpu().numpy()
+    xbk = 16 * ((w + 127) // 128)
+    res = {}
+    for m, name in ((0, "dct128_fp32_butterflies"), (1, "dct128_mfma_bf16x3")):
+        r = time_kernel(lambda: fe.dct128_image(xyb, w, h, m, out=outs[m]), stream, reps,
+                        24.0 * px)
+        flops = 3 * nblk * 2 * 3 * 2 * 128 ** 3
+        if m == 1:
+            r["mfma_tflops"] = round(flops / (r["ms"] * 1e-3) / 1e12, 1)
+            r["mfma_frac_bf16_peak"] = round(flops / (r["ms"] * 1e-3) / 2.5e15, 4)
+        got = [o.cpu().numpy().reshape(-1, xbk, 64) for o in outs[m]]
+        worst = 0.0
+        for (tx, ty) in [(0, 0), (3, 5), (7, 2), (15, 15), (10, 13)]:
+            if tx * 128 >= w or ty * 128 >= h:
+                continue
+            for c in range(3):
+                x = planes[c, ty * 128:ty * 128 + 128, tx * 128:tx * 128 + 128].astype(np.float64)
+                want = (cm @ x @ cm.T / 16384.0).T.reshape(-1)
+                g = np.concatenate([got[c][ty * 16 + j, tx * 16:tx * 16 + 16, :].reshape(1024)
+                                    for j in range(16)])
+                worst = max(worst, float(np.abs(g - want).max() / np.abs(want).max()))
+        r["max_err_vs_fp64_rel_to_max_coef"] = float("%.3g" % worst)
+        res[name] = r
+    del outs
+    torch.cuda.synchronize()
+    return res
+
+
 def host_path_rate(fe, rgb, params, reps=5):
     """PCIe-inclusive rate of the host-buffer entry point (jxg_front_end):
     pageable host RGB in, every output copied back to pageable host memory.
@@ -438,6 +477,8 @@ def main():
     proxy.close()
     del gab_out
     extra.update(dct64_forms(fe, xyb, w, h, stream, reps))
+    if w % 128 == 0 and h % 128 == 0:
+        extra.update(dct128_forms(fe, xyb, w, h, stream, reps))
 
     px = w * h
     dominant = max(stage_ms, key=stage_ms.get)

@@ -416,39 +416,40 @@ __global__ __launch_bounds__(256) void k_epf_pools(PoolArgs A) {
   }
 }
 
-// Fixed-order f64 sum of n strided partials (256 threads, contiguous chunks
-// then a tree).
-__device__ double block_sum(const double *p, int n, int stride, int off, double *sh) {
-  const int tid = threadIdx.x;
-  const int per = (n + 255) / 256;
-  double acc = 0.0;
-  for (int k = tid * per; k < min(n, (tid + 1) * per); k++) acc += p[(size_t)k * stride + off];
-  sh[tid] = acc;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (tid < o) sh[tid] += sh[tid + o];
-    __syncthreads();
-  }
-  const double r = sh[0];
-  __syncthreads();
-  return r;
-}
-
-// The six terms and the weighted score (src/epf_proxy.rs:186-214).
+// The six terms and the weighted score (src/epf_proxy.rs:186-214).  The
+// seven pooled sums are reduced together in a fixed order: each thread sums
+// a contiguous run of partials, then one tree over the 256 threads.
 __global__ __launch_bounds__(256) void k_epf_final(const double *__restrict__ ph, int nbh, int hn,
                                                    const double *__restrict__ pq, int nbq, int qn,
                                                    const double *__restrict__ pg, int nbg,
                                                    float ng, float *__restrict__ out) {
-  __shared__ double sh[256];
+  __shared__ double sh[7][256];
+  const int tid = threadIdx.x;
+  double acc[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int per_h = (nbh + 255) / 256, per_q = (nbq + 255) / 256, per_g = (nbg + 255) / 256;
+  for (int k = tid * per_h; k < min(nbh, (tid + 1) * per_h); k++) {
+    acc[0] += ph[2 * (size_t)k];
+    acc[1] += ph[2 * (size_t)k + 1];
+  }
+  for (int k = tid * per_q; k < min(nbq, (tid + 1) * per_q); k++) {
+    acc[2] += pq[2 * (size_t)k];
+    acc[3] += pq[2 * (size_t)k + 1];
+  }
+  for (int k = tid * per_g; k < min(nbg, (tid + 1) * per_g); k++) {
+    acc[4] += pg[3 * (size_t)k];
+    acc[5] += pg[3 * (size_t)k + 1];
+    acc[6] += pg[3 * (size_t)k + 2];
+  }
+  for (int i = 0; i < 7; i++) sh[i][tid] = acc[i];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o)
+      for (int i = 0; i < 7; i++) sh[i][tid] += sh[i][tid + o];
+    __syncthreads();
+  }
+  if (tid != 0) return;
   double s[7];
-  s[0] = block_sum(ph, nbh, 2, 0, sh);
-  s[1] = block_sum(ph, nbh, 2, 1, sh);
-  s[2] = block_sum(pq, nbq, 2, 0, sh);
-  s[3] = block_sum(pq, nbq, 2, 1, sh);
-  s[4] = block_sum(pg, nbg, 3, 0, sh);
-  s[5] = block_sum(pg, nbg, 3, 1, sh);
-  s[6] = block_sum(pg, nbg, 3, 2, sh);
-  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 7; i++) s[i] = sh[i][0];
   float t[6];
   t[0] = (float)s[2] / (float)qn;
   t[1] = (float)s[0] / (float)hn;

@@ -149,14 +149,19 @@ def smooth_rgba(w, h, seed):
     r = int(rng.integers(16, 65))
     rgb = 128.0 + _box_smooth(rgb, r) * 3.0
     rgb += rng.normal(0.0, 2.0, size=rgb.shape)
-    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
     alpha = np.zeros((h, w))
     for _ in range(20):
         cx, cy = rng.uniform(0, w), rng.uniform(0, h)
         ax, ay = rng.uniform(w / 40, w / 6), rng.uniform(h / 40, h / 6)
+        # outside the bounding box d > 1 and the falloff is 0, so only the box is visited
+        x0, x1 = max(0, int(np.floor(cx - ax))), min(w, int(np.ceil(cx + ax)) + 1)
+        y0, y1 = max(0, int(np.floor(cy - ay))), min(h, int(np.ceil(cy + ay)) + 1)
+        if x0 >= x1 or y0 >= y1:
+            continue
+        y, x = np.mgrid[y0:y1, x0:x1].astype(np.float64)
         d = np.sqrt(((x - cx) / ax) ** 2 + ((y - cy) / ay) ** 2)
         edge = np.clip((1.0 - d) * min(ax, ay) / 4.0, 0.0, 1.0)  # 4 px falloff
-        alpha = np.maximum(alpha, edge)
+        alpha[y0:y1, x0:x1] = np.maximum(alpha[y0:y1, x0:x1], edge)
     out = np.empty((h, w, 4))
     out[..., :3] = rgb
     out[..., 3] = alpha * 255.0

@@ -414,7 +414,8 @@ __global__ __launch_bounds__(256) void k_aq_block(const float *__restrict__ xyb,
 
 struct SelLds {
   float co[3][32 * SEL_PITCH];  // coefficients (spatial), then d^2 terms (storage order)
-  float mg[3][1024];            // magnitude-rate terms, storage order
+  float mg[3][1024 + 4];        // magnitude-rate terms, storage order (rows padded: the
+                                // six 32x32 sums read their arrays on distinct banks)
   float quot[3][64];            // per channel: 0, 1 - one, numerator / a
   float rate[256];              // log2(1 + i)
   float sse[16][3], mag[16][3];
@@ -426,6 +427,9 @@ struct SelLds {
   uint32_t site_cnt[5];
   double erased[4];
   float qs[3][16];  // per channel, per site: quantizer * selection scale * qm multiplier
+  float sbias[5], sgate[5];  // site biases and risk gates: four 16x16 sites, the region
+  float child[4], base[4];   // per 16x16 site: cost of its choice, cost of its four DCT8s
+  int pick16[4], merge32;
 };
 
 __device__ __forceinline__ double wave_sum_f64(double v) {
@@ -502,6 +506,18 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
     }
     __syncthreads();
   }
+  // This thread's four coefficients (element e = tid + 256 j, scan position
+  // p = e mod size): storage index and the three channels' inverse weights,
+  // loaded here so they arrive while CfL and the quantizer scale run.
+  int kq[4];
+  float ipq[3][4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int p = (tid + SEL_T * j) & (size - 1);
+    kq[j] = (int)scan[p];
+#pragma unroll
+    for (int c = 0; c < 3; c++) ipq[c][j] = selq[c * SELQ_N + p];
+  }
   // CfL against the unquantized luma (apply_cfl_scalar, src/ac_strategy.rs:1076)
 #pragma unroll
   for (int j = 0; j < 4; j++) {
@@ -548,21 +564,21 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
   // Lanes take coefficients in scan order: within a site, e's low bits are
   // the natural-scan position p and order[p] the storage index, so a
   // wavefront's 64 lanes hold 64 consecutive scan positions of one site and
-  // the nonzero count and last nonzero position come from one ballot.
+  // the nonzero count and last nonzero position come from one ballot.  All
+  // three channels are read before one barrier and their d^2 terms written
+  // over the coefficients after it.
+  float dd_r[3][4];
+  int kk_r[4];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const float *invp = selq + c * SELQ_N;  // inverse weights by scan position
     const float thr0 = D->thr[c][cls][0], thr1 = D->thr[c][cls][1];
     const float thr2 = D->thr[c][cls][2], thr3 = D->thr[c][cls][3];
     const float num = D->dequant_numerator;
-    float dd_r[4];
-    int kk_r[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int e = tid + SEL_T * j;
       const int site = __builtin_amdgcn_readfirstlane(e >> (2 * LG));
-      const int p = e & (size - 1);
-      const int k = (int)scan[p];  // storage index of scan position p
+      const int k = kq[j];  // storage index of scan position e mod size
       const int sy = site / sites_1d, sx = site - sy * sites_1d;
       const int u = k >> LG, v = k & (n - 1);  // square storage is u-major
       const int row = u, col = v;              // storage row / column
@@ -572,7 +588,7 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
       bool nzq = false;
       if (!(col < cblk && row < cblk)) {
         const float threshold = row >= half ? (col >= half ? thr3 : thr2) : (col >= half ? thr1 : thr0);
-        const float a = invp[p] * qsv * coef;
+        const float a = ipq[c][j] * qsv * coef;
         const float q = fabsf(a) >= threshold ? round_away(a) : 0.0f;
         const float aq = fabsf(q);
         // level_f32 with the extended quotient table (quot[0] = 0,
@@ -585,7 +601,7 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
           nzq = true;
         }
       }
-      dd_r[j] = dd;
+      dd_r[c][j] = dd;
       kk_r[j] = site * size + k;
       L.mg[c][site * size + k] = mgv;
       const uint64_t m = __ballot(nzq);
@@ -595,25 +611,27 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
         atomicMax(&L.maxscan[site][c], p0 + 64u - (uint32_t)__clzll(m));
       }
     }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 4; j++) (&L.co[c][0])[kk_r[j]] = dd_r[j];
   }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) (&L.co[c][0])[kk_r[j]] = dd_r[c][j];
   __syncthreads();
   // Sums in the reference's sequential row-major storage order (f32).
   for (int wk = tid; wk < nsites * 6; wk += SEL_T) {
     const int site = wk / 6, rest = wk - site * 6, c = rest >> 1, which = rest & 1;
     const float4 *src = reinterpret_cast<const float4 *>(
         which == 0 ? &L.co[c][0] + site * size : &L.mg[c][0] + site * size);
-    // Four float4 loads per step keep the LDS reads ahead of the
+    // Eight float4 loads per step keep the LDS reads ahead of the
     // (necessarily sequential) f32 accumulation.
     float acc = 0.0f;
-    for (int k0 = 0; k0 < (size >> 2); k0 += 4) {
-      float4 v4[4];
+    for (int k0 = 0; k0 < (size >> 2); k0 += 8) {
+      float4 v4[8];
 #pragma unroll
-      for (int i = 0; i < 4; i++) v4[i] = src[k0 + i];
+      for (int i = 0; i < 8; i++) v4[i] = src[k0 + i];
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
+      for (int i = 0; i < 8; i++) {
         acc += v4[i].x;
         acc += v4[i].y;
         acc += v4[i].z;
@@ -653,8 +671,8 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
 }
 
 // Loads the region's three 32x32 pixel planes into L.co.
-__device__ __forceinline__ void load_region(SelLds &L, const float *__restrict__ xyb, size_t plane,
-                                            int stride, int px0, int py0) {
+__device__ __forceinline__ void load_region_nosync(SelLds &L, const float *__restrict__ xyb,
+                                                   size_t plane, int stride, int px0, int py0) {
   const int tid = threadIdx.x;
   for (int e = tid; e < 3 * 32 * 8; e += SEL_T) {
     const int c = e >> 8, r = (e >> 3) & 31, q = e & 7;
@@ -666,6 +684,11 @@ __device__ __forceinline__ void load_region(SelLds &L, const float *__restrict__
     dst[2] = v.z;
     dst[3] = v.w;
   }
+}
+
+__device__ __forceinline__ void load_region(SelLds &L, const float *__restrict__ xyb, size_t plane,
+                                            int stride, int px0, int py0) {
+  load_region_nosync(L, xyb, plane, stride, px0, py0);
   __syncthreads();
 }
 
@@ -744,16 +767,18 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
     }
     L.rate[tid] = D->rate_lut[tid];
   }
-  __syncthreads();
   const float scale = D->scale, dist = D->distance;
   const bool four_row = by + 4 <= ysize;
   const bool four_col = bx + 4 <= xsize;
   const bool any_row = four_row || by + 1 < ysize;
   const bool any_col = bx + 1 < xsize;
   const bool work = D->run_selection && any_row && any_col;
+  const size_t plane = (size_t)stride * padded_h;
+  const int px0 = rbx * 8, py0 = rby * 8;
+  // the region's pixels, loaded under the table set-up (one barrier for both)
+  if (work) load_region_nosync(L, xyb, plane, stride, px0, py0);
+  __syncthreads();
   if (work) {
-    const size_t plane = (size_t)stride * padded_h;
-    const int px0 = rbx * 8, py0 = rby * 8;
     // cmap_factors (src/ac_strategy.rs:1838-1855): this region's 64x64 tile,
     // clamped to the DC group's tile grid.
     const int tiles_x = (W + 63) / 64;
@@ -765,7 +790,6 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
     // Site luma statistics for MergeTuning::site_bias (site_variance_rect,
     // src/ac_strategy.rs:224-245): f64 sums of the in-image pixels of the
     // four 16x16 sites and the 32x32 region, reduced over the workgroup.
-    load_region(L, xyb, plane, stride, px0, py0);
     {
       double s1[5] = {0, 0, 0, 0, 0}, s2[5] = {0, 0, 0, 0, 0};
       uint32_t cnt[5] = {0, 0, 0, 0, 0};
@@ -838,87 +862,101 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
       eval_squares<F, 5>(L, qac32, cmx, cmb, tables, scans + 320, selq + 320, D, W, H, px0, py0,
                          &L.cost32);
     }
-    if (tid == 0) {
-      // select_super_block at square scope for each super block, then the
-      // 32x32 merge (src/ac_strategy/selection.rs:238-455, 1040-1150).
-      float children[2][2], baseline[2][2];
-      const int nsy = four_row ? 2 : 1;
-      for (int sy = 0; sy < nsy; sy++)
-        for (int sx = 0; sx < 2; sx++) {
-          children[sy][sx] = 0.0f;
-          baseline[sy][sx] = 0.0f;
-          if (bx + sx * 2 + 1 >= xsize) continue;
-          const int b0 = (sy * 2) * 4 + sx * 2;
-          const float c00 = L.cost8[b0], c01 = L.cost8[b0 + 1];
-          const float c10 = L.cost8[b0 + 4], c11 = L.cost8[b0 + 5];
-          const float left = c00 + c10, right = c01 + c11, top = c00 + c01, bottom = c10 + c11;
-          const float total = left + right;
-          const int s = sy * 2 + sx;
-          const float c16 =
-              D->bias_16x16 * site_bias<F>(D, L.site_s1[s], L.site_s2[s], L.site_cnt[s]) *
-              L.cost16[s];
-          const float cost_16x8 = left + right, cost_8x16 = top + bottom;
-          const float best_rect = fminf(cost_16x8, cost_8x16);
-          float q_min = INFINITY, q_max = 0.0f;
-          const float qs[4] = {qac8[b0], qac8[b0 + 1], qac8[b0 + 4], qac8[b0 + 5]};
-          for (int i = 0; i < 4; i++) {
-            q_min = fminf(q_min, qs[i]);
-            q_max = fmaxf(q_max, qs[i]);
-          }
-          // can_place_strategy(DCT16X16): aligned super blocks always fit
-          // their 64-px tile and 256-px group once inside the image.
-          const bool fits = bx + sx * 2 + 2 <= xsize && by + sy * 2 + 2 <= ysize;
-          const bool pick16 =
-              fits && c16 < best_rect &&
-              c16 < total * risk_gated<F>(D->risk_k, D->accept_16, q_min, q_max, 1.0f,
-                                          D->exp2_table);
-          if (pick16) {
-            cell_out[b0] = JXG_CELL(JXG_DCT16X16, 1);
-            cell_out[b0 + 1] = JXG_CELL(JXG_DCT16X16, 0);
-            cell_out[b0 + 4] = JXG_CELL(JXG_DCT16X16, 0);
-            cell_out[b0 + 5] = JXG_CELL(JXG_DCT16X16, 0);
-            children[sy][sx] = c16;
-          } else {
-            children[sy][sx] = cost_16x8 <= cost_8x16 ? cost_16x8 : cost_8x16;
-          }
-          baseline[sy][sx] = total;
+    // The five site biases and risk gates (four 16x16 sites, then the
+    // region), one thread each, ahead of the decisions.
+    if (tid < 5) {
+      L.sbias[tid] = site_bias<F>(D, L.site_s1[tid], L.site_s2[tid], L.site_cnt[tid]);
+      if (tid < 4) {
+        const int b0 = (tid >> 1) * 8 + (tid & 1) * 2;
+        float q_min = INFINITY, q_max = 0.0f;
+        // the qac8 values of the site's blocks, recomputed from q_in (no
+        // thread-dependent index into the register array)
+        const float qs[4] = {scale * (float)q_in[b0], scale * (float)q_in[b0 + 1],
+                             scale * (float)q_in[b0 + 4], scale * (float)q_in[b0 + 5]};
+        for (int i = 0; i < 4; i++) {
+          q_min = fminf(q_min, qs[i]);
+          q_max = fmaxf(q_max, qs[i]);
         }
-      if (four_row && four_col) {
-        const float m32 = site_bias<F>(D, L.site_s1[4], L.site_s2[4], L.site_cnt[4]);
-        const float merged = D->bias_32x32 * m32 * L.cost32;
+        L.sgate[tid] = risk_gated<F>(D->risk_k, D->accept_16, q_min, q_max, 1.0f, D->exp2_table);
+      } else {
         uint8_t qmin = 255, qmax = 0;
         for (int i = 0; i < 16; i++) {
           qmin = q_in[i] < qmin ? q_in[i] : qmin;
           qmax = q_in[i] > qmax ? q_in[i] : qmax;
         }
-        const float gate =
+        L.sgate[4] =
             risk_gated<F>(D->risk_k, D->accept_32, (float)qmin, (float)qmax, 2.0f, D->exp2_table);
-        // choose_merge_partition (src/ac_strategy/selection.rs:115-150) with
-        // rectangles priced +inf: only the square merge can pass.
-        const float child = children[0][0] + children[0][1] + children[1][0] + children[1][1];
-        const float base = baseline[0][0] + baseline[0][1] + baseline[1][0] + baseline[1][1];
-        if (merged < child && merged < base * gate) {
-          for (int i = 0; i < 16; i++) cell_out[i] = JXG_CELL(JXG_DCT32X32, i == 0);
-        }
       }
-      // adjust_quant_field (src/ac_strategy/selection.rs:503-527)
-      for (int i = 0; i < 16; i++) {
-        const uint8_t cell = cell_out[i];
-        if (!(cell & 1)) continue;
-        const int s = cell >> 1;
-        const int cx = cov_x(s), cy = cov_y(s);
-        if (cx == 1 && cy == 1) continue;
+    }
+    __syncthreads();
+    // select_super_block at square scope (src/ac_strategy/selection.rs:238-455),
+    // one thread per 16x16 super block.
+    if (tid < 4) {
+      const int sx = tid & 1, sy = tid >> 1;
+      float child = 0.0f, base = 0.0f;
+      int pick = 0;
+      if ((four_row || sy == 0) && bx + sx * 2 + 1 < xsize) {
+        const int b0 = (sy * 2) * 4 + sx * 2;
+        const float c00 = L.cost8[b0], c01 = L.cost8[b0 + 1];
+        const float c10 = L.cost8[b0 + 4], c11 = L.cost8[b0 + 5];
+        const float left = c00 + c10, right = c01 + c11, top = c00 + c01, bottom = c10 + c11;
+        const float total = left + right;
+        const float c16 = D->bias_16x16 * L.sbias[tid] * L.cost16[tid];
+        const float cost_16x8 = left + right, cost_8x16 = top + bottom;
+        const float best_rect = fminf(cost_16x8, cost_8x16);
+        // can_place_strategy(DCT16X16): aligned super blocks always fit
+        // their 64-px tile and 256-px group once inside the image.
+        const bool fits = bx + sx * 2 + 2 <= xsize && by + sy * 2 + 2 <= ysize;
+        pick = fits && c16 < best_rect && c16 < total * L.sgate[tid];
+        child = pick ? c16 : (cost_16x8 <= cost_8x16 ? cost_16x8 : cost_8x16);
+        base = total;
+      }
+      L.child[tid] = child;
+      L.base[tid] = base;
+      L.pick16[tid] = pick;
+    }
+    __syncthreads();
+    // The 32x32 merge (choose_merge_partition, src/ac_strategy/selection.rs:
+    // 115-150 and 1040-1150, with rectangles priced +inf: only the square
+    // merge can pass).
+    if (tid == 0) {
+      int m32 = 0;
+      if (four_row && four_col) {
+        const float merged = D->bias_32x32 * L.sbias[4] * L.cost32;
+        const float child = L.child[0] + L.child[1] + L.child[2] + L.child[3];
+        const float base = L.base[0] + L.base[1] + L.base[2] + L.base[3];
+        m32 = merged < child && merged < base * L.sgate[4];
+      }
+      L.merge32 = m32;
+    }
+    __syncthreads();
+    // Each block's cell, and adjust_quant_field (src/ac_strategy/selection.rs:
+    // 503-527): the aggregate quant value over its transform's blocks.
+    if (tid < 16) {
+      const int ix = tid & 3, iy = tid >> 2;
+      const int site = (iy >> 1) * 2 + (ix >> 1);
+      int first = tid, cv = 1;
+      uint8_t cell = JXG_CELL(JXG_DCT, 1);
+      if (L.merge32) {
+        first = 0;
+        cv = 4;
+        cell = JXG_CELL(JXG_DCT32X32, tid == 0);
+      } else if (L.pick16[site]) {
+        first = (iy >> 1) * 8 + (ix >> 1) * 2;
+        cv = 2;
+        cell = JXG_CELL(JXG_DCT16X16, tid == first);
+      }
+      cell_out[tid] = cell;
+      if (cv > 1) {
         uint8_t max_q = 0;
         uint32_t sum = 0;
-        for (int iy = 0; iy < cy; iy++)
-          for (int ix = 0; ix < cx; ix++) {
-            const uint8_t q = q_in[i + iy * 4 + ix];
+        for (int jy = 0; jy < cv; jy++)
+          for (int jx = 0; jx < cv; jx++) {
+            const uint8_t q = q_in[first + jy * 4 + jx];
             max_q = q > max_q ? q : max_q;
             sum += q;
           }
-        const uint8_t val = aggregate_quant(max_q, sum, cx * cy, dist);
-        for (int iy = 0; iy < cy; iy++)
-          for (int ix = 0; ix < cx; ix++) q_out[i + iy * 4 + ix] = val;
+        q_out[tid] = aggregate_quant(max_q, sum, cv * cv, dist);
       }
     }
     __syncthreads();

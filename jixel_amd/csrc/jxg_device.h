@@ -351,13 +351,51 @@ __device__ __forceinline__ void idct4(const float v[4], float out[4]) {
 }
 
 // ---- strategy geometry (src/dc_group_data.rs:140-170) ----
+// Per-strategy values live in 4-bit fields of two 64-bit constants (codes
+// 0..15, then 16..23), so a lookup by a per-lane strategy code is a shift in
+// registers; an array indexed that way would be a memory load per lane.
+namespace jxg_geom {
+constexpr uint8_t kCovX[24] = {1, 1, 2, 2, 4, 1, 1, 1, 2, 4, 1, 1, 1, 1, 8, 4, 8, 1, 1, 16, 8, 16, 1, 4};
+constexpr uint8_t kCovY[24] = {1, 2, 1, 2, 4, 1, 1, 1, 4, 2, 1, 1, 1, 1, 8, 8, 4, 1, 1, 16, 16, 8, 4, 1};
+constexpr int log2i(int v) { return v <= 1 ? 0 : 1 + log2i(v >> 1); }
+constexpr uint64_t pack_log2(const uint8_t *v, int lo) {
+  uint64_t r = 0;
+  for (int i = 0; i < 16 && lo + i < 24; i++) r |= (uint64_t)log2i(v[lo + i]) << (4 * i);
+  return r;
+}
+constexpr uint64_t pack_kind(int lo) {
+  uint64_t r = 0;
+  for (int i = 0; i < 16 && lo + i < 24; i++) r |= (uint64_t)jxg_qt_kind_dev(lo + i) << (4 * i);
+  return r;
+}
+constexpr uint64_t kLogCovX[2] = {pack_log2(kCovX, 0), pack_log2(kCovX, 16)};
+constexpr uint64_t kLogCovY[2] = {pack_log2(kCovY, 0), pack_log2(kCovY, 16)};
+constexpr uint64_t kKind[2] = {pack_kind(0), pack_kind(16)};
+constexpr int field(const uint64_t (&w)[2], int s) {
+  return (int)((s < 16 ? w[0] >> (4 * s) : w[1] >> (4 * (s - 16))) & 15u);
+}
+constexpr bool packed_ok() {
+  for (int s = 0; s < 24; s++)
+    if ((1 << field(kLogCovX, s)) != kCovX[s] || (1 << field(kLogCovY, s)) != kCovY[s] ||
+        field(kKind, s) != jxg_qt_kind_dev(s))
+      return false;
+  return true;
+}
+static_assert(packed_ok(), "packed strategy geometry");
+}  // namespace jxg_geom
+
+__device__ __forceinline__ int nibble24(uint64_t lo, uint64_t hi, int s) {
+  return (int)((s < 16 ? lo >> (4 * s) : hi >> (4 * (s - 16))) & 15u);
+}
 __device__ __forceinline__ int cov_x(int s) {
-  const uint8_t t[24] = {1, 1, 2, 2, 4, 1, 1, 1, 2, 4, 1, 1, 1, 1, 8, 4, 8, 1, 1, 16, 8, 16, 1, 4};
-  return t[s];
+  return 1 << nibble24(jxg_geom::kLogCovX[0], jxg_geom::kLogCovX[1], s);
 }
 __device__ __forceinline__ int cov_y(int s) {
-  const uint8_t t[24] = {1, 2, 1, 2, 4, 1, 1, 1, 4, 2, 1, 1, 1, 1, 8, 8, 4, 1, 1, 16, 16, 8, 4, 1};
-  return t[s];
+  return 1 << nibble24(jxg_geom::kLogCovY[0], jxg_geom::kLogCovY[1], s);
+}
+// jxg_qt_kind_dev as a register lookup.
+__device__ __forceinline__ int qt_kind(int s) {
+  return nibble24(jxg_geom::kKind[0], jxg_geom::kKind[1], s);
 }
 
 // Whether the 2-D assembly transforms rows first (src/dct.rs: 8x8, 8x16,

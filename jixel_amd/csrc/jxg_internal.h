@@ -61,7 +61,7 @@ static inline JXG_HD int jxg_area_class(int cx, int cy) {
 }
 
 // Quant-table kind of a raw strategy (src/group.rs:1459-1495).
-static inline JXG_HD int jxg_qt_kind_dev(int s) {
+static inline constexpr JXG_HD int jxg_qt_kind_dev(int s) {
   switch (s) {
     case JXG_DCT: return JXG_QT_DCT8;
     case JXG_DCT16X8:

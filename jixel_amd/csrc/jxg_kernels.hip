@@ -1363,25 +1363,50 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   const jxg_derived *D = A.D;
   const int n = L.n;
   if (n == 0) return;
-  if (tid == 0) {
-    int p1 = 0, p2 = 0, pc = 0, n64 = 0;
-    for (int t = 0; t < n; t++) {
+  // Prefix sums of the transforms' pass-1 lines, pass-2 lines and
+  // 64-coefficient chunks (n <= 64: one wavefront, lane = transform), and the
+  // first two DCT64-family transforms in list order.
+  if (tid < 64) {
+    const int t = tid;
+    int l1 = 0, l2 = 0, cc = 0;
+    bool d64 = false;
+    if (t < n) {
       const int s = L.s[t];
-      if (is_dct64(s) && n64 < 2) L.t64[n64++] = t;
+      d64 = is_dct64(s);
       const int w = cov_x(s) * 8, h = cov_y(s) * 8;
-      L.pre1[t] = p1;
-      L.pre2[t] = p2;
-      L.prec[t] = pc;
       if (!is_sub8(s)) {  // sub-8x8 transforms have no 1-D lines
-        p1 += rows_first(s) ? h : w;
-        p2 += rows_first(s) ? w : h;
+        l1 = rows_first(s) ? h : w;
+        l2 = rows_first(s) ? w : h;
       }
-      pc += cov_x(s) * cov_y(s);
+      cc = cov_x(s) * cov_y(s);
     }
-    L.pre1[n] = p1;
-    L.pre2[n] = p2;
-    L.prec[n] = pc;
-    L.n64 = n64;
+    int i1 = l1, i2 = l2, ic = cc;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int a1 = __shfl_up(i1, o, 64), a2 = __shfl_up(i2, o, 64), ac = __shfl_up(ic, o, 64);
+      if (t >= o) {
+        i1 += a1;
+        i2 += a2;
+        ic += ac;
+      }
+    }
+    if (t < n) {
+      L.pre1[t] = i1 - l1;
+      L.pre2[t] = i2 - l2;
+      L.prec[t] = ic - cc;
+    }
+    if (t == n - 1) {
+      L.pre1[n] = i1;
+      L.pre2[n] = i2;
+      L.prec[n] = ic;
+    }
+    const uint64_t m = __ballot(d64);
+    if (t == 0) {
+      const int n64 = __popcll(m) < 2 ? __popcll(m) : 2;
+      if (n64 > 0) L.t64[0] = __builtin_ctzll(m);
+      if (n64 > 1) L.t64[1] = __builtin_ctzll(m & (m - 1));
+      L.n64 = n64;
+    }
   }
   if (tid < n) {
     const int t = tid;
@@ -1450,7 +1475,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
         const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
         const int s = L.s[t];
         const int k = ((ch - L.prec[t]) << 6) + lane;
-        const int kind = jxg_qt_kind_dev(s);
+        const int kind = qt_kind(s);
         const float *tb = A.tables + D->qt_off[kind];
         const int qn = D->qt_n[kind];
         g[u] = coef_geom(L, t, k, A.xb);
@@ -1503,7 +1528,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
           const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
           const int s = L.s[t];
           const int k = ((ch - L.prec[t]) << 6) + lane;
-          const int kind = jxg_qt_kind_dev(s);
+          const int kind = qt_kind(s);
           g[u] = coef_geom(L, t, k, A.xb);
           tq[u] = A.tables[D->qt_off[kind] + (3 + c) * D->qt_n[kind] + k];
           tt[u] = t;
@@ -1620,7 +1645,19 @@ template <bool F>
 __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
   __shared__ QLds L;
   const int tid = threadIdx.x;
-  const uint32_t count = min(*A.xt_count, A.xt_cap);
+  const uint32_t queued = *A.xt_count;
+  if (queued == 0) {
+    // Nothing crossed (the common case): every workgroup read the same 0,
+    // written by the finished quantize launch, so workgroup 0 alone
+    // publishes the status; the queue counter is already clear.
+    if (blockIdx.x == 0 && tid == 0) {
+      A.status[0] = 0u;
+      A.status[1] = atomicExch(A.bad, 0u);
+      __threadfence();
+    }
+    return;
+  }
+  const uint32_t count = min(queued, A.xt_cap);
   for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
     const uint32_t blk = A.xt_list[item];
     const int bx = (int)(blk % (uint32_t)A.xb), by = (int)(blk / (uint32_t)A.xb);
@@ -1758,7 +1795,16 @@ __global__ __launch_bounds__(256) void k_quantize_xt_b(const BatchImg *__restric
                                                        BatchQueue Q) {
   __shared__ QLds L;
   const int tid = threadIdx.x;
-  const uint32_t count = min(*Q.xt_count, Q.xt_cap);
+  const uint32_t queued = *Q.xt_count;
+  if (queued == 0) {  // as in k_quantize_xt: workgroup 0 alone publishes
+    if (blockIdx.x == 0 && tid == 0) {
+      Q.status[0] = 0u;
+      Q.status[1] = atomicExch(Q.bad, 0u);
+      __threadfence();
+    }
+    return;
+  }
+  const uint32_t count = min(queued, Q.xt_cap);
   for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
     const int i = (int)Q.xt_img[item];
     const BatchImg &I = img[i];
@@ -1953,7 +1999,7 @@ __global__ __launch_bounds__(256) void k_site_cost(SiteArgs S) {
     L.line1_t[tid] = 0;
     L.line2_t[tid] = 0;
   }
-  const int kind = jxg_qt_kind_dev(s);
+  const int kind = qt_kind(s);
   const float *tb = S.tables + D->qt_off[kind];
   const int qn = D->qt_n[kind];
   const uint16_t *scanpos = S.scanpos + scanpos_offset(cx, cy);

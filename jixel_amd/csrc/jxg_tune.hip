@@ -218,7 +218,7 @@ __global__ __launch_bounds__(256) void k_tune_step(TuneArgs A, const float *thet
         const int v = py - L.oy[b], u = px - L.ox[b];
         if (!(v < h / 8 && u < w / 8)) {
           const int k = storage_index(s, w, h, v, u);
-          const int kind = jxg_qt_kind_dev(s);
+          const int kind = qt_kind(s);
           const float iv = A.tables[D->qt_off[kind] + (3 + c) * D->qt_n[kind] + k];
           const float qs = D->scale * L.theta[b] * D->qm_mult[c];
           const float stp = 1.0f / (iv * qs);

@@ -1032,6 +1032,7 @@ struct QLds {
   float xf[QT_MAXT], bf[QT_MAXT];
   uint8_t quant[QT_MAXT];
   float inv_qac[QT_MAXT];  // 1 / (scale * quant), src/group.rs:806
+  uint16_t tqo[QT_MAXT], tqn[QT_MAXT];  // quant-table offset and size of the transform
   float quot[64];          // 0, 1 - one, numerator / a for a >= 2 (see dequant_level_tab)
   uint8_t line1_t[512], line2_t[512];  // line -> transform, per pass
   uint8_t chunk_t[64];                 // 64-coefficient chunk -> transform
@@ -1185,8 +1186,7 @@ struct CoefGeom {
   size_t out;
 };
 
-__device__ __forceinline__ CoefGeom coef_geom(const QLds &L, int t, int k, int xb) {
-  const int s = L.s[t];
+__device__ __forceinline__ CoefGeom coef_geom(const QLds &L, int t, int s, int k, int xb) {
   const int cvx = cov_x(s), cvy = cov_y(s);
   const int lw = 3 + ctz32(cvx), lh = 3 + ctz32(cvy);
   int v, u;
@@ -1417,6 +1417,9 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     const int quant = A.qf[(size_t)L.by[t] * A.xb + L.bx[t]];
     L.quant[t] = quant;
     L.inv_qac[t] = 1.0f / (D->scale * (float)quant);
+    const int kind = qt_kind(L.s[t]);
+    L.tqo[t] = (uint16_t)D->qt_off[kind];
+    L.tqn[t] = (uint16_t)D->qt_n[kind];
   }
   if (tid < 64)
     L.quot[tid] = tid >= 2 ? D->dequant_numerator / (float)tid
@@ -1472,13 +1475,13 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     for (int u = 0; u < 4; u++) {
       const int ch = base + u * nwaves;
       if (ch < nch) {
+        // transform and strategy are uniform over the chunk: scalar registers
         const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
-        const int s = L.s[t];
+        const int s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
         const int k = ((ch - L.prec[t]) << 6) + lane;
-        const int kind = qt_kind(s);
-        const float *tb = A.tables + D->qt_off[kind];
-        const int qn = D->qt_n[kind];
-        g[u] = coef_geom(L, t, k, A.xb);
+        const float *tb = A.tables + __builtin_amdgcn_readfirstlane((int)L.tqo[t]);
+        const int qn = __builtin_amdgcn_readfirstlane((int)L.tqn[t]);
+        g[u] = coef_geom(L, t, s, k, A.xb);
         tq[u] = tb[4 * qn + k];
         td[u] = tb[1 * qn + k];
         yv[u] = L.cur[g[u].a];
@@ -1490,7 +1493,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     for (int u = 0; u < 4; u++) {
       const int ch = base + u * nwaves;
       if (ch < nch) {
-        const int t = tt[u], s = L.s[t];
+        const int t = __builtin_amdgcn_readfirstlane(tt[u]), s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
         const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
         A.co1[g[u].out] = yv[u];
         const float *thr = D->thr[1][jxg_area_class(cx, cy)];
@@ -1526,11 +1529,12 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
         const int ch = base + u * nwaves;
         if (ch < nch) {
           const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
-          const int s = L.s[t];
+          const int s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
           const int k = ((ch - L.prec[t]) << 6) + lane;
-          const int kind = qt_kind(s);
-          g[u] = coef_geom(L, t, k, A.xb);
-          tq[u] = A.tables[D->qt_off[kind] + (3 + c) * D->qt_n[kind] + k];
+          g[u] = coef_geom(L, t, s, k, A.xb);
+          const int qo = __builtin_amdgcn_readfirstlane((int)L.tqo[t]);
+          const int qn = __builtin_amdgcn_readfirstlane((int)L.tqn[t]);
+          tq[u] = A.tables[qo + (3 + c) * qn + k];
           tt[u] = t;
         }
       }
@@ -1538,7 +1542,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
       for (int u = 0; u < 4; u++) {
         const int ch = base + u * nwaves;
         if (ch < nch) {
-          const int t = tt[u], s = L.s[t];
+          const int t = __builtin_amdgcn_readfirstlane(tt[u]), s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
           const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
           const float f = c == 0 ? L.xf[t] : L.bf[t];
           const float v = fmla<F>(-f, L.y2[g[u].a], L.cur[g[u].a]);
@@ -2023,7 +2027,7 @@ __global__ __launch_bounds__(256) void k_site_cost(SiteArgs S) {
     const float *inv_y = tb + 4 * qn;
     double energy = 0.0;
     for (int k = tid; k < size; k += 256) {
-      const CoefGeom g = coef_geom(L, 0, k, 0);
+      const CoefGeom g = coef_geom(L, 0, s, k, 0);
       if (g.col < cx && g.row < cy) continue;
       const float l = L.y2[g.a];
       if (fabsf(l * inv_y[k] * qac) < 0.5f) energy += (double)l * (double)l;
@@ -2052,7 +2056,7 @@ __global__ __launch_bounds__(256) void k_site_cost(SiteArgs S) {
       bool nzq = false;
       uint32_t pos1 = 0;
       if (k < size) {
-        const CoefGeom g = coef_geom(L, 0, k, 0);
+        const CoefGeom g = coef_geom(L, 0, s, k, 0);
         float coef = c == 1 ? L.y2[g.a] : fmla<F>(-f, L.y2[g.a], L.cur[g.a]);
         float dd = 0.0f, mg = 0.0f;
         if (!(g.col < cx && g.row < cy)) {

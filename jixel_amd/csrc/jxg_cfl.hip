@@ -232,7 +232,7 @@ JXG_HIDDEN int batch_cmap(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
 
 JXG_HIDDEN int fill_cmap(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
                          uint32_t width, uint32_t height, const jxg_frame_params *p,
-                         int8_t *d_ytox, int8_t *d_ytob) {
+                         int8_t *d_ytox, int8_t *d_ytob, hipStream_t st) {
   const int tiles_x = (int)((width + 63) / 64), tiles_y = (int)((height + 63) / 64);
   const size_t need = (size_t)tiles_x * tiles_y * 64 * 4;
   int rc = ensure(&ctx->d_tmp, &ctx->tmp_cap, need);
@@ -244,16 +244,16 @@ JXG_HIDDEN int fill_cmap(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint
   const dim3 grid(tiles_x, 4 * tiles_y);
   const unsigned sgrid = (unsigned)(tiles_x * tiles_y);
   if (p->fused) {
-    k_cmap_blocks<true><<<grid, CM_T, 0, ctx->stream>>>(d_xyb, (int)stride, (int)padded_h,
+    k_cmap_blocks<true><<<grid, CM_T, 0, st>>>(d_xyb, (int)stride, (int)padded_h,
                                                        (int)width, (int)height, qm_x, qm_b, a,
                                                        ctx->d_tmp);
-    k_cmap_solve<true><<<sgrid, 64, 0, ctx->stream>>>(ctx->d_tmp, (int)width, (int)height,
+    k_cmap_solve<true><<<sgrid, 64, 0, st>>>(ctx->d_tmp, (int)width, (int)height,
                                                       tiles_x, tiles_y, a, d_ytox, d_ytob);
   } else {
-    k_cmap_blocks<false><<<grid, CM_T, 0, ctx->stream>>>(d_xyb, (int)stride, (int)padded_h,
+    k_cmap_blocks<false><<<grid, CM_T, 0, st>>>(d_xyb, (int)stride, (int)padded_h,
                                                         (int)width, (int)height, qm_x, qm_b, a,
                                                         ctx->d_tmp);
-    k_cmap_solve<false><<<sgrid, 64, 0, ctx->stream>>>(ctx->d_tmp, (int)width, (int)height,
+    k_cmap_solve<false><<<sgrid, 64, 0, st>>>(ctx->d_tmp, (int)width, (int)height,
                                                        tiles_x, tiles_y, a, d_ytox, d_ytob);
   }
   LAUNCH_CHECK("k_cmap");
@@ -272,7 +272,8 @@ int jxg_fill_cmap_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_
   if ((rc = check_planes(width, height, stride, padded_h))) return rc;
   if ((rc = upload_derived(ctx, params))) return rc;  // validates the parameters
   HIP_TRY(hipSetDevice(ctx->device));
-  return fill_cmap(ctx, d_xyb, stride, padded_h, width, height, params, d_ytox, d_ytob);
+  return fill_cmap(ctx, d_xyb, stride, padded_h, width, height, params, d_ytox, d_ytob,
+                   ctx->stream);
 }
 
 }  // extern "C"

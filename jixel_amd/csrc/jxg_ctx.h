@@ -33,6 +33,11 @@ JXG_HIDDEN int set_err(int code, const char *fmt, ...);
 struct jxg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // A second stream (non-blocking, default priority) for the CfL maps, which
+  // run beside the quant field; ev_fork / ev_join order it after XYB and
+  // before the strategy search on `stream`.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   float *d_tables = nullptr;
   bool tables_set = false;
   jxg_derived *d_derived = nullptr;
@@ -95,7 +100,7 @@ JXG_HIDDEN int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t pa
 JXG_HIDDEN int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p);
 JXG_HIDDEN int fill_cmap(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
                          uint32_t width, uint32_t height, const jxg_frame_params *p,
-                         int8_t *d_ytox, int8_t *d_ytob);  // jxg_cfl.hip
+                         int8_t *d_ytox, int8_t *d_ytob, hipStream_t st);  // jxg_cfl.hip
 JXG_HIDDEN int fill_u8(jxg_ctx *ctx, uint8_t *p, size_t n, uint8_t v);
 // jxg_alpha.hip: the alpha stages of 2- and 4-channel input after XYB, and
 // the alpha factors on the final quant field.

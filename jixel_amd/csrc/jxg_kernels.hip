@@ -1215,6 +1215,53 @@ __device__ __forceinline__ CoefGeom coef_geom(const QLds &L, int t, int s, int k
   return g;
 }
 
+// coef_geom split for a chunk of 64 consecutive storage indices kc ..
+// kc + 63 of transform t: everything but the lane's own position is uniform
+// (scalar registers), and the chunk's outputs are contiguous (chunks start
+// at multiples of 64 and an output row holds cov_x * 64 >= 64 values).
+struct ChunkGeom {
+  int kc, lw, lh, lwc, mode, origin;  // mode 0: 8x8 u-major, 1: v-major, 2: u-major
+  bool d64x, d64y;
+  size_t out;  // output index of storage index kc
+};
+
+__device__ __forceinline__ ChunkGeom chunk_geom(const QLds &L, int t, int s, int kc, int xb) {
+  ChunkGeom G;
+  const int cvx = cov_x(s), cvy = cov_y(s);
+  G.kc = kc;
+  G.lw = 3 + ctz32(cvx);
+  G.lh = 3 + ctz32(cvy);
+  G.lwc = G.lw > G.lh ? G.lw : G.lh;
+  G.mode = s == JXG_DCT ? 0 : ((rows_first(s) || is_sub8(s)) ? 1 : 2);
+  G.d64x = cvx == 8;
+  G.d64y = cvy == 8;
+  G.origin = L.ly[t] * QT_PITCH + L.lx[t];
+  const int lch = 6 + ctz32(cvx);
+  G.out = ((size_t)(L.by[t] + (kc >> lch)) * xb + L.bx[t]) * 64 + (kc & ((1 << lch) - 1));
+  return G;
+}
+
+// The lane's LDS address, storage row and column (as coef_geom).
+__device__ __forceinline__ void lane_geom(const ChunkGeom &G, int lane, int &a, int &row, int &col) {
+  const int k = G.kc + lane;
+  int v, u;
+  if (G.mode == 0) {
+    u = k >> 3;
+    v = k & 7;
+  } else if (G.mode == 1) {
+    v = k >> G.lw;
+    u = k & ((1 << G.lw) - 1);
+  } else {
+    u = k >> G.lh;
+    v = k & ((1 << G.lh) - 1);
+  }
+  if (G.d64y) v = dct64_pos(v);
+  if (G.d64x) u = dct64_pos(u);
+  a = G.origin + v * QT_PITCH + u;
+  row = k >> G.lwc;
+  col = k & ((1 << G.lwc) - 1);
+}
+
 // Copies channel c of the w x h pixel rectangle at (px, py) of the padded
 // planes into L.cur at LDS origin (0, 0).  w is a multiple of 8.
 __device__ __forceinline__ void load_plane(QLds &L, const QArgs &A, int c, int px, int py, int w,
@@ -1468,9 +1515,8 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   // table loads of a round are in flight together.
   const int nch = L.prec[n];
   for (int base = wave; base < nch; base += 4 * nwaves) {
-    CoefGeom g[4];
     float tq[4], td[4], yv[4];
-    int tt[4], kk[4];
+    int tt[4], aa[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int ch = base + u * nwaves;
@@ -1478,35 +1524,41 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
         // transform and strategy are uniform over the chunk: scalar registers
         const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
         const int s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
-        const int k = ((ch - L.prec[t]) << 6) + lane;
-        const float *tb = A.tables + __builtin_amdgcn_readfirstlane((int)L.tqo[t]);
+        const int kc = (ch - L.prec[t]) << 6;
+        const float *tb = A.tables + __builtin_amdgcn_readfirstlane((int)L.tqo[t]) + kc + lane;
         const int qn = __builtin_amdgcn_readfirstlane((int)L.tqn[t]);
-        g[u] = coef_geom(L, t, s, k, A.xb);
-        tq[u] = tb[4 * qn + k];
-        td[u] = tb[1 * qn + k];
-        yv[u] = L.cur[g[u].a];
+        const ChunkGeom G = chunk_geom(L, t, s, kc, A.xb);
+        int row, col;
+        lane_geom(G, lane, aa[u], row, col);
+        tq[u] = tb[4 * qn];
+        td[u] = tb[1 * qn];
+        yv[u] = L.cur[aa[u]];
         tt[u] = t;
-        kk[u] = k;
       }
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int ch = base + u * nwaves;
       if (ch < nch) {
-        const int t = __builtin_amdgcn_readfirstlane(tt[u]), s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
+        const int t = __builtin_amdgcn_readfirstlane(tt[u]);
+        const int s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
+        const ChunkGeom G = chunk_geom(L, t, s, (ch - L.prec[t]) << 6, A.xb);
+        int a, row, col;
+        lane_geom(G, lane, a, row, col);
         const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
-        A.co1[g[u].out] = yv[u];
+        A.co1[G.out + lane] = yv[u];
         const float *thr = D->thr[1][jxg_area_class(cx, cy)];
-        const int yfix = g[u].row >= g[u].hc / 2 ? 2 : 0;
-        const float threshold = g[u].col >= g[u].wc / 2 ? thr[yfix + 1] : thr[yfix];
+        const float thr0 = thr[0], thr1 = thr[1], thr2 = thr[2], thr3 = thr[3];
+        const int hc = 1 << (G.lw > G.lh ? G.lh : G.lw), wc = 1 << G.lwc;
+        const float threshold = row >= hc / 2 ? (col >= wc / 2 ? thr3 : thr2)
+                                              : (col >= wc / 2 ? thr1 : thr0);
         const float q_scaled = D->scale * (float)L.quant[t] * 1.0f;
         const float val = (tq[u] * q_scaled) * yv[u];
         const int32_t q = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
-        A.lv1[g[u].out] = q;
+        A.lv1[G.out + lane] = q;
         float y2 = dequant_level_tab(D->dequant_numerator, q, L.quot) * td[u] * L.inv_qac[t];
-        if (g[u].row < cy && g[u].col < cx) y2 = 0.0f;
-        L.y2[g[u].a] = y2;
-        (void)kk;
+        if (row < cy && col < cx) y2 = 0.0f;
+        L.y2[aa[u]] = y2;
       }
     }
   }
@@ -1521,7 +1573,6 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     float *co = c == 0 ? A.co0 : A.co2;
     const float qm = D->qm_mult[c];
     for (int base = wave; base < nch; base += 4 * nwaves) {
-      CoefGeom g[4];
       float tq[4];
       int tt[4];
 #pragma unroll
@@ -1529,12 +1580,10 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
         const int ch = base + u * nwaves;
         if (ch < nch) {
           const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
-          const int s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
-          const int k = ((ch - L.prec[t]) << 6) + lane;
-          g[u] = coef_geom(L, t, s, k, A.xb);
+          const int kc = (ch - L.prec[t]) << 6;
           const int qo = __builtin_amdgcn_readfirstlane((int)L.tqo[t]);
           const int qn = __builtin_amdgcn_readfirstlane((int)L.tqn[t]);
-          tq[u] = A.tables[qo + (3 + c) * qn + k];
+          tq[u] = A.tables[qo + (3 + c) * qn + kc + lane];
           tt[u] = t;
         }
       }
@@ -1542,19 +1591,24 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
       for (int u = 0; u < 4; u++) {
         const int ch = base + u * nwaves;
         if (ch < nch) {
-          const int t = __builtin_amdgcn_readfirstlane(tt[u]), s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
+          const int t = __builtin_amdgcn_readfirstlane(tt[u]);
+          const int s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
+          const ChunkGeom G = chunk_geom(L, t, s, (ch - L.prec[t]) << 6, A.xb);
+          int a, row, col;
+          lane_geom(G, lane, a, row, col);
           const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
           const float f = c == 0 ? L.xf[t] : L.bf[t];
-          const float v = fmla<F>(-f, L.y2[g[u].a], L.cur[g[u].a]);
-          L.cur[g[u].a] = v;
-          co[g[u].out] = v;
-          const int cls = jxg_area_class(cx, cy);
-          const int yfix = g[u].row >= g[u].hc / 2 ? 2 : 0;
-          const float threshold =
-              g[u].col >= g[u].wc / 2 ? D->thr[c][cls][yfix + 1] : D->thr[c][cls][yfix];
+          const float v = fmla<F>(-f, L.y2[a], L.cur[a]);
+          L.cur[a] = v;
+          co[G.out + lane] = v;
+          const float *thr = D->thr[c][jxg_area_class(cx, cy)];
+          const float thr0 = thr[0], thr1 = thr[1], thr2 = thr[2], thr3 = thr[3];
+          const int hc = 1 << (G.lw > G.lh ? G.lh : G.lw), wc = 1 << G.lwc;
+          const float threshold = row >= hc / 2 ? (col >= wc / 2 ? thr3 : thr2)
+                                                : (col >= wc / 2 ? thr1 : thr0);
           const float q_scaled = D->scale * (float)L.quant[t] * qm;
           const float val = (tq[u] * q_scaled) * v;
-          lv[g[u].out] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
+          lv[G.out + lane] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
         }
       }
     }
@@ -2295,6 +2349,10 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
       hipMalloc((void **)&ctx->d_misc, 8 * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc((void **)&ctx->d_exp2, 64 * sizeof(float)) != hipSuccess)
     return fail(set_err(JXG_ERR_ALLOC, "context allocation failed"));
+  if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail(set_err(JXG_ERR_DEVICE, "context stream / event creation failed"));
   // scan position tables of the square shapes, computed once
   uint32_t scans[64 + 256 + 1024];
   {
@@ -2358,6 +2416,12 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   else (void)hipDeviceSynchronize();
+  if (ctx->side) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
+  }
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   (void)hipFree(ctx->d_tables);
   (void)hipFree(ctx->d_derived);
   (void)hipHostFree(ctx->h_derived);
@@ -2670,6 +2734,15 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
     ctx->alpha_used = (size_t)width * height;
   }
   if ((rc = upload_derived(ctx, params))) return rc;
+  // The CfL slopes (needed before the strategy search, as setup_dc_group
+  // computes them, src/frame/dc_groups.rs:166-184) depend only on XYB: they
+  // run on the side stream beside the quant field.
+  HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
+  HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  if ((rc = fill_cmap(ctx, ctx->d_xyb, stride, ph, width, height, params, d_ytox, d_ytob,
+                      ctx->side)))
+    return rc;
+  HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
   if ((rc = fill_qf(ctx, ctx->d_xyb, stride, ph, width, height, params->distance,
                     ctx->h_derived->inv_scale, params->fused, d_qf)))
     return rc;
@@ -2677,10 +2750,7 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
       (rc = chroma_hf(ctx, ctx->d_xyb, stride, ph, width, height, params->distance, params->fused,
                       d_qf)))
     return rc;
-  // CfL slopes before the strategy search, as setup_dc_group does
-  // (src/frame/dc_groups.rs:166-184).
-  if ((rc = fill_cmap(ctx, ctx->d_xyb, stride, ph, width, height, params, d_ytox, d_ytob)))
-    return rc;
+  HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   if ((rc = jxg_select_strategy_dev(ctx, ctx->d_xyb, stride, ph, width, height, d_qf, d_ytox, d_ytob,
                                     params, d_cells)))
     return rc;

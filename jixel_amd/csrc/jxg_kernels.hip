@@ -51,26 +51,19 @@ int set_err(int code, const char *fmt, ...) {
 // ---------------------------------------------------------------------------
 // XYB
 
-template <bool F, int ORDER>
-__device__ __forceinline__ void xyb_body(const uint8_t *__restrict__ rgb, size_t row_bytes,
-                                             int ch, int W, int H,
-                                             const float *__restrict__ mlut,
-                                             float *__restrict__ xyb, int stride, int padded_h, int bix, int biy) {
-  __shared__ float lut[256];
-  __shared__ float m[9];
-  const int t = threadIdx.x;
-  lut[t] = mlut[9 + t];
-  if (t < 9) m[t] = mlut[t];
-  __syncthreads();
-  const int x = bix * 256 + t;
-  const int y = biy;
-  if (x >= stride) return;
-  const int sx = min(x, W - 1), sy = min(y, H - 1);
-  const uint8_t *p = rgb + (size_t)sy * row_bytes + (size_t)sx * ch;
-  // Gray (+ alpha) input decodes one sample to all three channels
-  // (linearize_gray, src/encode_image.rs:1571-1599).
-  const bool gray = ch <= 2;
-  const float r = lut[p[0]], g = gray ? r : lut[p[1]], b = gray ? r : lut[p[2]];
+// A workgroup covers XYB_PX_WG x XYB_ROWS pixels: each thread four
+// consecutive pixels of XYB_ROWS rows, read as ch aligned dwords and stored
+// as 16-byte vectors.
+#define XYB_PIX 4
+#define XYB_PX_WG (256 * XYB_PIX)
+#ifndef XYB_ROWS
+#define XYB_ROWS 1
+#endif
+
+// One pixel's linear r, g, b (LUT values): X, Y, B.
+template <bool F, int ORDER, bool GRAY>
+__device__ __forceinline__ void xyb_pixel(float r, float g, float b, const float *m,
+                                          float &Xo, float &Yo, float &Bo) {
   const float bias = 0.0037930734f, neg = -0.1559542f;
   float tm[3];
 #pragma unroll
@@ -96,12 +89,86 @@ __device__ __forceinline__ void xyb_body(const uint8_t *__restrict__ rgb, size_t
     X = 0.5f * (tm[0] - tm[1]);
     Y = 0.5f * (tm[0] + tm[1]);
   }
-  const size_t plane = (size_t)stride * padded_h;
-  const size_t i = (size_t)y * stride + x;
   // snap_achromatic_xyb (src/frame/encoder.rs:347-357): X = 0, B = Y.
-  xyb[i] = gray ? 0.0f : X;
-  xyb[plane + i] = Y;
-  xyb[2 * plane + i] = gray ? Y : tm[2];
+  Xo = GRAY ? 0.0f : X;
+  Yo = Y;
+  Bo = GRAY ? Y : tm[2];
+}
+
+// The XYB_PIX pixels of one thread on row y, CH bytes per pixel.
+template <bool F, int ORDER, int CH>
+__device__ __forceinline__ void xyb_run(const uint8_t *__restrict__ rgb, size_t row_bytes,
+                                        int W, int H, const float *lut, const float *m,
+                                        float *__restrict__ xyb, size_t plane, int stride,
+                                        int x0, int y, bool vec) {
+  // Gray (+ alpha) input decodes one sample to all three channels
+  // (linearize_gray, src/encode_image.rs:1571-1599).
+  constexpr bool GRAY = CH <= 2;
+  const uint8_t *row = rgb + (size_t)min(y, H - 1) * row_bytes;
+  const uint8_t *p = row + (size_t)x0 * CH;
+  uint8_t px[XYB_PIX * CH];
+  if (x0 + XYB_PIX <= W && ((uintptr_t)p & 3) == 0) {
+    // XYB_PIX * CH bytes = CH dwords
+    uint32_t w[CH];
+#pragma unroll
+    for (int i = 0; i < CH; i++) w[i] = reinterpret_cast<const uint32_t *>(p)[i];
+#pragma unroll
+    for (int i = 0; i < XYB_PIX * CH; i++) px[i] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+  } else {
+#pragma unroll
+    for (int k = 0; k < XYB_PIX; k++) {
+      const uint8_t *q = row + (size_t)min(x0 + k, W - 1) * CH;
+#pragma unroll
+      for (int c = 0; c < CH; c++) px[k * CH + c] = q[c];
+    }
+  }
+  float Xo[XYB_PIX], Yo[XYB_PIX], Bo[XYB_PIX];
+#pragma unroll
+  for (int k = 0; k < XYB_PIX; k++) {
+    const float r = lut[px[k * CH]];
+    const float g = GRAY ? r : lut[px[k * CH + 1]], b = GRAY ? r : lut[px[k * CH + 2]];
+    xyb_pixel<F, ORDER, GRAY>(r, g, b, m, Xo[k], Yo[k], Bo[k]);
+  }
+  float *o = xyb + (size_t)y * stride + x0;
+  if (vec) {
+    *reinterpret_cast<float4 *>(o) = make_float4(Xo[0], Xo[1], Xo[2], Xo[3]);
+    *reinterpret_cast<float4 *>(o + plane) = make_float4(Yo[0], Yo[1], Yo[2], Yo[3]);
+    *reinterpret_cast<float4 *>(o + 2 * plane) = make_float4(Bo[0], Bo[1], Bo[2], Bo[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < XYB_PIX; k++) {
+      o[k] = Xo[k];
+      o[plane + k] = Yo[k];
+      o[2 * plane + k] = Bo[k];
+    }
+  }
+}
+
+template <bool F, int ORDER>
+__device__ __forceinline__ void xyb_body(const uint8_t *__restrict__ rgb, size_t row_bytes,
+                                             int ch, int W, int H,
+                                             const float *__restrict__ mlut,
+                                             float *__restrict__ xyb, int stride, int padded_h, int bix, int biy) {
+  __shared__ float lut[256];
+  __shared__ float m[9];
+  const int t = threadIdx.x;
+  lut[t] = mlut[9 + t];
+  if (t < 9) m[t] = mlut[t];
+  __syncthreads();
+  const int x0 = bix * XYB_PX_WG + t * XYB_PIX;  // stride is a multiple of 64
+  if (x0 >= stride) return;
+  const size_t plane = (size_t)stride * padded_h;
+  const bool vec = ((uintptr_t)xyb & 15) == 0;  // planes a caller passes may not be
+  for (int r = 0; r < XYB_ROWS; r++) {
+    const int y = biy * XYB_ROWS + r;  // padded_h is a multiple of 64
+    if (y >= padded_h) break;
+    switch (ch) {
+      case 1: xyb_run<F, ORDER, 1>(rgb, row_bytes, W, H, lut, m, xyb, plane, stride, x0, y, vec); break;
+      case 2: xyb_run<F, ORDER, 2>(rgb, row_bytes, W, H, lut, m, xyb, plane, stride, x0, y, vec); break;
+      case 3: xyb_run<F, ORDER, 3>(rgb, row_bytes, W, H, lut, m, xyb, plane, stride, x0, y, vec); break;
+      default: xyb_run<F, ORDER, 4>(rgb, row_bytes, W, H, lut, m, xyb, plane, stride, x0, y, vec); break;
+    }
+  }
 }
 
 template <bool F, int ORDER>
@@ -1956,8 +2023,8 @@ JXG_HIDDEN int batch_stages_post(jxg_ctx *ctx, hipStream_t st, const BatchImg *d
 JXG_HIDDEN void batch_grids(uint32_t W, uint32_t H, uint32_t stride, uint32_t ph, int aq_strip,
                             int cm_rows, uint32_t gw[BS_N], uint32_t gh[BS_N]) {
   const uint32_t xb = (W + 7) / 8, yb = (H + 7) / 8, tx = (W + 63) / 64, ty = (H + 63) / 64;
-  gw[BS_XYB] = (stride + 255) / 256;
-  gh[BS_XYB] = ph;
+  gw[BS_XYB] = (stride + XYB_PX_WG - 1) / XYB_PX_WG;
+  gh[BS_XYB] = (ph + XYB_ROWS - 1) / XYB_ROWS;
   gw[BS_AQPRE] = (2 * xb + 255) / 256;
   gh[BS_AQPRE] = 2 * yb;
   gw[BS_AQBLK] = (xb + aq_strip - 1) / aq_strip;
@@ -2482,7 +2549,7 @@ int jxg_xyb_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channe
                            ctx->stream));
     ctx->mlut_valid = true;
   }
-  const dim3 grid((stride + 255) / 256, padded_h);
+  const dim3 grid((stride + XYB_PX_WG - 1) / XYB_PX_WG, (padded_h + XYB_ROWS - 1) / XYB_ROWS);
   if (order == JXG_XYB_AVX2)
     k_xyb<true, JXG_XYB_AVX2><<<grid, 256, 0, ctx->stream>>>(d_rgb, row_bytes, channels, width, height, ctx->d_mlut, d_xyb, stride, padded_h);
   else if (fused)

@@ -486,7 +486,7 @@ struct SelLds {
   float quot[3][64];            // per channel: 0, 1 - one, numerator / a
   float rate[256];              // log2(1 + i)
   float sse[16][3], mag[16][3];
-  uint32_t nz[16][3], maxscan[16][3];
+  uint32_t nzp[3][16], msp[3][16];  // per slot 4 j + wave: nonzeros, 1 + last nonzero scan position
   float cost8[16], cost16[4], cost32;
   double red1[4][5], red2[4][5];
   uint32_t redc[4][5];
@@ -530,10 +530,12 @@ __device__ __forceinline__ float level_f32_tab(float one, float num, float q, co
   return q - num / q;
 }
 
-__device__ __noinline__ float rate_log2_slow(const jxg_derived *D, float aq) {
+__device__ __forceinline__ float rate_log2_big(const jxg_derived *D, float aq) {
   const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
   return ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
 }
+
+__device__ __noinline__ float rate_log2_slow(const jxg_derived *D, float aq) { return rate_log2_big(D, aq); }
 
 // Candidate evaluation over the whole region for square size n (8, 16, 32):
 // all (32/n)^2 sites at once.  qac_in[site] is the site's quantizer.  The
@@ -549,10 +551,6 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
   constexpr int cls = n == 8 ? 0 : (n == 16 ? 2 : 4);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t qoff = D->qt_off[kind], qn = D->qt_n[kind];
-  if (tid < 48) {
-    (&L.nz[0][0])[tid] = 0;
-    (&L.maxscan[0][0])[tid] = 0;
-  }
   // 2-D transform in place: 8x8 rows first, 16x16 / 32x32 columns first
   // (src/dct.rs:631-655,1010-1030,1980-2000).
   const float scale = 1.0f / (float)(n * n);
@@ -634,50 +632,80 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
   // the nonzero count and last nonzero position come from one ballot.  All
   // three channels are read before one barrier and their d^2 terms written
   // over the coefficients after it.
+  // A channel's four coefficients are read together and every table lookup
+  // is clamped into its table, so the LDS reads of a channel are in flight
+  // together; the two out-of-table cases (|q| >= 64: the quotient, |q| >=
+  // 256: the rate) are patched afterwards, only when a lane has one.
   float dd_r[3][4];
-  int kk_r[4];
+  int kk_r[4], off[4], tsel[4];
+  bool ac[4];
+  float qsv[3][4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int e = tid + SEL_T * j;
+    const int site = __builtin_amdgcn_readfirstlane(e >> (2 * LG));
+    const int k = kq[j];  // storage index of scan position e mod size
+    const int sy = site / sites_1d, sx = site - sy * sites_1d;
+    const int u = k >> LG, v = k & (n - 1);  // square storage is u-major
+    const int row = u, col = v;              // storage row / column
+    off[j] = (sy * n + v) * SEL_PITCH + sx * n + u;
+    kk_r[j] = site * size + k;
+    ac[j] = !(col < cblk && row < cblk);
+    tsel[j] = (row >= half ? 2 : 0) + (col >= half ? 1 : 0);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      qsv[c][j] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.qs[c][site])));
+  }
+  uint32_t vn = 0, vm = 0;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const float thr0 = D->thr[c][cls][0], thr1 = D->thr[c][cls][1];
     const float thr2 = D->thr[c][cls][2], thr3 = D->thr[c][cls][3];
-    const float num = D->dequant_numerator;
+    float a[4], q[4], qt[4], rt[4], lev[4];
+    bool slow = false;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const int e = tid + SEL_T * j;
-      const int site = __builtin_amdgcn_readfirstlane(e >> (2 * LG));
-      const int k = kq[j];  // storage index of scan position e mod size
-      const int sy = site / sites_1d, sx = site - sy * sites_1d;
-      const int u = k >> LG, v = k & (n - 1);  // square storage is u-major
-      const int row = u, col = v;              // storage row / column
-      const float coef = L.co[c][(sy * n + v) * SEL_PITCH + sx * n + u];
-      const float qsv = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(L.qs[c][site])));
-      float dd = 0.0f, mgv = 0.0f;
-      bool nzq = false;
-      if (!(col < cblk && row < cblk)) {
-        const float threshold = row >= half ? (col >= half ? thr3 : thr2) : (col >= half ? thr1 : thr0);
-        const float a = ipq[c][j] * qsv * coef;
-        const float q = fabsf(a) >= threshold ? round_away(a) : 0.0f;
-        const float aq = fabsf(q);
-        // level_f32 with the extended quotient table (quot[0] = 0,
-        // quot[1] = 1 - one): bit-identical to src/group.rs:748-768.
-        const float lev = aq < 64.0f ? q - copysignf(L.quot[c][(int)aq], q) : q - num / q;
-        const float diff = a - lev;
-        dd = diff * diff;
-        if (aq != 0.0f) {
-          mgv = aq < 256.0f ? L.rate[(int)aq] : rate_log2_slow(D, aq);
-          nzq = true;
-        }
-      }
-      dd_r[c][j] = dd;
-      kk_r[j] = site * size + k;
-      L.mg[c][site * size + k] = mgv;
-      const uint64_t m = __ballot(nzq);
-      if (lane == 0 && m) {
-        const uint32_t p0 = (uint32_t)(e & (size - 1));  // lane 0's scan position
-        atomicAdd(&L.nz[site][c], (uint32_t)__popcll(m));
-        atomicMax(&L.maxscan[site][c], p0 + 64u - (uint32_t)__clzll(m));
+      const float coef = L.co[c][off[j]];
+      const float threshold = tsel[j] >= 2 ? (tsel[j] == 3 ? thr3 : thr2) : (tsel[j] == 1 ? thr1 : thr0);
+      a[j] = ipq[c][j] * qsv[c][j] * coef;
+      q[j] = (ac[j] && fabsf(a[j]) >= threshold) ? round_away(a[j]) : 0.0f;
+      const float aq = fabsf(q[j]);
+      qt[j] = L.quot[c][(int)fminf(aq, 63.0f)];
+      rt[j] = L.rate[(int)fminf(aq, 255.0f)];
+      slow |= aq >= 64.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      // level_f32 with the extended quotient table (quot[0] = 0,
+      // quot[1] = 1 - one): bit-identical to src/group.rs:748-768.
+      lev[j] = q[j] - copysignf(qt[j], q[j]);
+    if (__builtin_expect(__ballot(slow) != 0, 0)) {
+      const float num = D->dequant_numerator;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const float aq = fabsf(q[j]);
+        if (aq >= 64.0f) lev[j] = q[j] - num / q[j];
+        if (aq >= 256.0f) rt[j] = rate_log2_big(D, aq);
       }
     }
+    // Per wavefront and j: the nonzero count and 1 + the last nonzero's
+    // scan position of its 64 scan positions, one slot (4 j + wave) each.
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const float diff = a[j] - lev[j];
+      dd_r[c][j] = ac[j] ? diff * diff : 0.0f;
+      const bool nzq = q[j] != 0.0f;
+      L.mg[c][kk_r[j]] = nzq ? rt[j] : 0.0f;
+      const uint64_t m = __ballot(nzq);
+      const uint32_t p0 = (uint32_t)((wave * 64 + SEL_T * j) & (size - 1));  // lane 0's scan position
+      const bool mine = lane == 4 * c + j;
+      vn = mine ? (uint32_t)__popcll(m) : vn;
+      vm = mine ? (m ? p0 + 64u - (uint32_t)__clzll(m) : 0u) : vm;
+    }
+  }
+  if (lane < 12) {
+    L.nzp[lane >> 2][4 * (lane & 3) + wave] = vn;
+    L.msp[lane >> 2][4 * (lane & 3) + wave] = vm;
   }
   __syncthreads();
 #pragma unroll
@@ -717,13 +745,20 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
     float d_total = 0.0f, r_total = 0.0f;
     for (int c = 0; c < 3; c++) {
       d_total += D->channel_weight[c] * L.sse[site][c];
-      const uint32_t nzc = L.nz[site][c];
+      // site s of nsites holds the 16 / nsites slots from s * 16 / nsites
+      constexpr int per = 16 / nsites;
+      uint32_t nzc = 0, mscan = 0;
+#pragma unroll
+      for (int i = 0; i < per; i++) {
+        nzc += L.nzp[c][site * per + i];
+        mscan = max(mscan, L.msp[c][site * per + i]);
+      }
       const float nzf = (float)nzc;
       const float header = 0.4f * (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf));
       float visited = 0.0f;
       if (nzc != 0) {
         // maxscan holds 1 + the last nonzero's scan position
-        const int64_t span = (int64_t)L.maxscan[site][c] - (int64_t)(cblk * cblk + nzc);
+        const int64_t span = (int64_t)mscan - (int64_t)(cblk * cblk + nzc);
         visited = span > 0 ? (float)span : 0.0f;
       }
       r_total += nzf * 1.6f + 1.0f * L.mag[site][c] + header + 0.5f * visited;

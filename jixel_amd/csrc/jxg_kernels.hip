@@ -1123,6 +1123,7 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ x
 struct QLds {
   float y2[QT_TILE * QT_PITCH];   // dequantized luma with zeroed LLF (CfL source)
   float cur[QT_TILE * QT_PITCH];  // the channel being transformed
+  uint4 cdesc[64];                 // 64-coefficient chunk -> packed descriptor (chunk_desc)
   int n;
   uint8_t s[QT_MAXT];
   uint16_t bx[QT_MAXT], by[QT_MAXT];  // global first block
@@ -1134,10 +1135,8 @@ struct QLds {
   float xf[QT_MAXT], bf[QT_MAXT];
   uint8_t quant[QT_MAXT];
   float inv_qac[QT_MAXT];  // 1 / (scale * quant), src/group.rs:806
-  uint16_t tqo[QT_MAXT], tqn[QT_MAXT];  // quant-table offset and size of the transform
   float quot[64];          // 0, 1 - one, numerator / a for a >= 2 (see dequant_level_tab)
   uint8_t line1_t[512], line2_t[512];  // line -> transform, per pass
-  uint8_t chunk_t[64];                 // 64-coefficient chunk -> transform
   union {
     float sub8_px[QT_SUB8_SLOTS][64];  // pixel copies for sub-8x8 transforms
     struct {
@@ -1327,23 +1326,63 @@ struct ChunkGeom {
   size_t out;  // output index of storage index kc
 };
 
-__device__ __forceinline__ ChunkGeom chunk_geom(const QLds &L, int t, int s, int kc, int xb) {
+// A chunk's uniform data, packed once per tile into four words:
+//   d0: LDS origin (12 bits) | mode << 12 | (lw - 3) << 14 | (lh - 3) << 16 |
+//       d64x << 18 | d64y << 19 | area class << 20 | kc mod 2^lch << 23
+//   d1: block row of kc (by + kc >> lch) | block column << 16
+//   d2: kc | transform << 12 | quant-table size << 18
+//   d3: quant-table offset + kc
+// (lch = 6 + log2 cov_x: an output row of blocks holds 2^lch values; block
+// coordinates are below 2^15 and the LDS origin below 2^12.)
+// Writes the descriptors of transform t's chunks prec[t] .. prec[t + 1] - 1
+// (the per-transform fields once, then the per-chunk ones).
+__device__ __forceinline__ void chunk_pack(QLds &L, int t, uint32_t qoff, uint32_t qn) {
+  const int s = L.s[t], cvx = cov_x(s), cvy = cov_y(s);
+  const int lw = 3 + ctz32(cvx), lh = 3 + ctz32(cvy), lch = 6 + ctz32(cvx);
+  const int mode = s == JXG_DCT ? 0 : ((rows_first(s) || is_sub8(s)) ? 1 : 2);
+  const int cls = jxg_area_class(max(cvx, cvy), min(cvx, cvy));
+  const uint32_t x0 = (uint32_t)(L.ly[t] * QT_PITCH + L.lx[t]) | (uint32_t)mode << 12 |
+                      (uint32_t)(lw - 3) << 14 | (uint32_t)(lh - 3) << 16 | (uint32_t)(cvx == 8) << 18 |
+                      (uint32_t)(cvy == 8) << 19 | (uint32_t)cls << 20;
+  const uint32_t by = L.by[t], bx16 = (uint32_t)L.bx[t] << 16, z0 = (uint32_t)t << 12 | qn << 18;
+  const int e0 = L.prec[t], e1 = L.prec[t + 1];
+  for (int e = e0; e < e1; e++) {
+    const uint32_t kc = (uint32_t)(e - e0) << 6;
+    L.cdesc[e] = make_uint4(x0 | (kc & ((1u << lch) - 1)) << 23, (by + (kc >> lch)) | bx16, kc | z0,
+                            qoff + kc);
+  }
+}
+
+struct ChunkDesc {
   ChunkGeom G;
-  const int cvx = cov_x(s), cvy = cov_y(s);
-  G.kc = kc;
-  G.lw = 3 + ctz32(cvx);
-  G.lh = 3 + ctz32(cvy);
-  G.lwc = G.lw > G.lh ? G.lw : G.lh;
-  G.mode = s == JXG_DCT ? 0 : ((rows_first(s) || is_sub8(s)) ? 1 : 2);
-  G.d64x = cvx == 8;
-  G.d64y = cvy == 8;
-  G.origin = L.ly[t] * QT_PITCH + L.lx[t];
-  const int lch = 6 + ctz32(cvx);
-  G.out = ((size_t)(L.by[t] + (kc >> lch)) * xb + L.bx[t]) * 64 + (kc & ((1 << lch) - 1));
-  return G;
+  int t, cls, qn;
+  uint32_t toff;  // quant-table offset of storage index kc
+};
+
+// Chunk ch's descriptor in scalar registers.
+__device__ __forceinline__ ChunkDesc chunk_desc(const QLds &L, int ch, int xb) {
+  const uint4 d = L.cdesc[ch];
+  const uint32_t d0 = __builtin_amdgcn_readfirstlane(d.x), d1 = __builtin_amdgcn_readfirstlane(d.y);
+  const uint32_t d2 = __builtin_amdgcn_readfirstlane(d.z), d3 = __builtin_amdgcn_readfirstlane(d.w);
+  ChunkDesc C;
+  C.G.origin = (int)(d0 & 0xfffu);
+  C.G.mode = (int)((d0 >> 12) & 3u);
+  C.G.lw = 3 + (int)((d0 >> 14) & 3u);
+  C.G.lh = 3 + (int)((d0 >> 16) & 3u);
+  C.G.lwc = C.G.lw > C.G.lh ? C.G.lw : C.G.lh;
+  C.G.d64x = (d0 >> 18) & 1u;
+  C.G.d64y = (d0 >> 19) & 1u;
+  C.cls = (int)((d0 >> 20) & 7u);
+  C.G.kc = (int)(d2 & 0xfffu);
+  C.t = (int)((d2 >> 12) & 63u);
+  C.qn = (int)(d2 >> 18);
+  C.toff = d3;
+  C.G.out = (size_t)((d1 & 0xffffu) * (uint32_t)xb + (d1 >> 16)) * 64 + (d0 >> 23);
+  return C;
 }
 
 // The lane's LDS address, storage row and column (as coef_geom).
+template <bool BIG>
 __device__ __forceinline__ void lane_geom(const ChunkGeom &G, int lane, int &a, int &row, int &col) {
   const int k = G.kc + lane;
   int v, u;
@@ -1357,8 +1396,9 @@ __device__ __forceinline__ void lane_geom(const ChunkGeom &G, int lane, int &a, 
     u = k >> G.lh;
     v = k & ((1 << G.lh) - 1);
   }
-  if (G.d64y) v = dct64_pos(v);
-  if (G.d64x) u = dct64_pos(u);
+  // only the DCT64 family has 64-long axes
+  if (BIG && G.d64y) v = dct64_pos(v);
+  if (BIG && G.d64x) u = dct64_pos(u);
   a = G.origin + v * QT_PITCH + u;
   row = k >> G.lwc;
   col = k & ((1 << G.lwc) - 1);
@@ -1557,6 +1597,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
       L.n64 = n64;
     }
   }
+  uint32_t t_qoff = 0, t_qn = 0;  // this thread's transform: quant-table offset and size
   if (tid < n) {
     const int t = tid;
     const int tiles_x = (A.W + 63) / 64;
@@ -1567,8 +1608,8 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     L.quant[t] = quant;
     L.inv_qac[t] = 1.0f / (D->scale * (float)quant);
     const int kind = qt_kind(L.s[t]);
-    L.tqo[t] = (uint16_t)D->qt_off[kind];
-    L.tqn[t] = (uint16_t)D->qt_n[kind];
+    t_qoff = D->qt_off[kind];
+    t_qn = D->qt_n[kind];
   }
   if (tid < 64)
     L.quot[tid] = tid >= 2 ? D->dequant_numerator / (float)tid
@@ -1576,9 +1617,10 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   __syncthreads();
   // line / chunk -> transform tables
   if (tid < n) {
-    for (int e = L.pre1[tid]; e < L.pre1[tid + 1]; e++) L.line1_t[e] = (uint8_t)tid;
-    for (int e = L.pre2[tid]; e < L.pre2[tid + 1]; e++) L.line2_t[e] = (uint8_t)tid;
-    for (int e = L.prec[tid]; e < L.prec[tid + 1]; e++) L.chunk_t[e] = (uint8_t)tid;
+    const int a1 = L.pre1[tid], b1 = L.pre1[tid + 1], a2 = L.pre2[tid], b2 = L.pre2[tid + 1];
+    for (int e = a1; e < b1; e++) L.line1_t[e] = (uint8_t)tid;
+    for (int e = a2; e < b2; e++) L.line2_t[e] = (uint8_t)tid;
+    chunk_pack(L, tid, t_qoff, t_qn);
   }
   // ---- Y ----
   load_plane(L, A, 1, px0, py0, win_w, win_h);
@@ -1616,51 +1658,54 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   // consecutive coefficients of one transform; four chunks per round so the
   // table loads of a round are in flight together.
   const int nch = L.prec[n];
+  const float num = D->dequant_numerator;
   for (int base = wave; base < nch; base += 4 * nwaves) {
     float tq[4], td[4], yv[4];
-    int tt[4], aa[4];
+    ChunkDesc C[4];
+    int aa[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int ch = base + u * nwaves;
       if (ch < nch) {
-        // transform and strategy are uniform over the chunk: scalar registers
-        const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
-        const int s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
-        const int kc = (ch - L.prec[t]) << 6;
-        const float *tb = A.tables + __builtin_amdgcn_readfirstlane((int)L.tqo[t]) + kc + lane;
-        const int qn = __builtin_amdgcn_readfirstlane((int)L.tqn[t]);
-        const ChunkGeom G = chunk_geom(L, t, s, kc, A.xb);
+        C[u] = chunk_desc(L, ch, A.xb);
         int row, col;
-        lane_geom(G, lane, aa[u], row, col);
-        tq[u] = tb[4 * qn];
-        td[u] = tb[1 * qn];
+        lane_geom<BIG>(C[u].G, lane, aa[u], row, col);
+        const float *tb = A.tables + C[u].toff + lane;
+        tq[u] = tb[4 * C[u].qn];
+        td[u] = tb[1 * C[u].qn];
         yv[u] = L.cur[aa[u]];
-        tt[u] = t;
       }
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int ch = base + u * nwaves;
       if (ch < nch) {
-        const int t = __builtin_amdgcn_readfirstlane(tt[u]);
-        const int s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
-        const ChunkGeom G = chunk_geom(L, t, s, (ch - L.prec[t]) << 6, A.xb);
+        const ChunkGeom &G = C[u].G;
+        const int t = C[u].t;
         int a, row, col;
-        lane_geom(G, lane, a, row, col);
-        const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
+        lane_geom<BIG>(G, lane, a, row, col);
+        const int lmin = G.lw > G.lh ? G.lh : G.lw;
+        const int cx = 1 << (G.lwc - 3), cy = 1 << (lmin - 3);
         A.co1[G.out + lane] = yv[u];
-        const float *thr = D->thr[1][jxg_area_class(cx, cy)];
+        const float *thr = D->thr[1][C[u].cls];
         const float thr0 = thr[0], thr1 = thr[1], thr2 = thr[2], thr3 = thr[3];
-        const int hc = 1 << (G.lw > G.lh ? G.lh : G.lw), wc = 1 << G.lwc;
+        const int hc = 1 << lmin, wc = 1 << G.lwc;
         const float threshold = row >= hc / 2 ? (col >= wc / 2 ? thr3 : thr2)
                                               : (col >= wc / 2 ? thr1 : thr0);
         const float q_scaled = D->scale * (float)L.quant[t] * 1.0f;
         const float val = (tq[u] * q_scaled) * yv[u];
         const int32_t q = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
         A.lv1[G.out + lane] = q;
-        float y2 = dequant_level_tab(D->dequant_numerator, q, L.quot) * td[u] * L.inv_qac[t];
+        // dequant_level_tab with the table lookup clamped and |q| >= 64
+        // patched only when a lane of the chunk has one
+        const uint32_t aq = q < 0 ? (uint32_t)(-(int64_t)q) : (uint32_t)q;
+        const float qf = (float)q;
+        float lev = qf - copysignf(L.quot[aq < 63u ? aq : 63u], qf);
+        if (__builtin_expect(__ballot(aq >= 64u) != 0, 0))
+          if (aq >= 64u) lev = qf - num / qf;
+        float y2 = lev * td[u] * L.inv_qac[t];
         if (row < cy && col < cx) y2 = 0.0f;
-        L.y2[aa[u]] = y2;
+        L.y2[a] = y2;
       }
     }
   }
@@ -1675,35 +1720,34 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     float *co = c == 0 ? A.co0 : A.co2;
     const float qm = D->qm_mult[c];
     for (int base = wave; base < nch; base += 4 * nwaves) {
-      float tq[4];
-      int tt[4];
+      float tq[4], yq[4], cv[4];
+      ChunkDesc C[4];
+      int aa[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int ch = base + u * nwaves;
         if (ch < nch) {
-          const int t = __builtin_amdgcn_readfirstlane((int)L.chunk_t[ch]);
-          const int kc = (ch - L.prec[t]) << 6;
-          const int qo = __builtin_amdgcn_readfirstlane((int)L.tqo[t]);
-          const int qn = __builtin_amdgcn_readfirstlane((int)L.tqn[t]);
-          tq[u] = A.tables[qo + (3 + c) * qn + kc + lane];
-          tt[u] = t;
+          C[u] = chunk_desc(L, ch, A.xb);
+          int row, col;
+          lane_geom<BIG>(C[u].G, lane, aa[u], row, col);
+          tq[u] = A.tables[C[u].toff + (3 + c) * C[u].qn + lane];
+          yq[u] = L.y2[aa[u]];
+          cv[u] = L.cur[aa[u]];
         }
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int ch = base + u * nwaves;
         if (ch < nch) {
-          const int t = __builtin_amdgcn_readfirstlane(tt[u]);
-          const int s = __builtin_amdgcn_readfirstlane((int)L.s[t]);
-          const ChunkGeom G = chunk_geom(L, t, s, (ch - L.prec[t]) << 6, A.xb);
+          const ChunkGeom &G = C[u].G;
+          const int t = C[u].t;
           int a, row, col;
-          lane_geom(G, lane, a, row, col);
-          const int cx = max(cov_x(s), cov_y(s)), cy = min(cov_x(s), cov_y(s));
+          lane_geom<BIG>(G, lane, a, row, col);
           const float f = c == 0 ? L.xf[t] : L.bf[t];
-          const float v = fmla<F>(-f, L.y2[a], L.cur[a]);
+          const float v = fmla<F>(-f, yq[u], cv[u]);
           L.cur[a] = v;
           co[G.out + lane] = v;
-          const float *thr = D->thr[c][jxg_area_class(cx, cy)];
+          const float *thr = D->thr[c][C[u].cls];
           const float thr0 = thr[0], thr1 = thr[1], thr2 = thr[2], thr3 = thr[3];
           const int hc = 1 << (G.lw > G.lh ? G.lh : G.lw), wc = 1 << G.lwc;
           const float threshold = row >= hc / 2 ? (col >= wc / 2 ? thr3 : thr2)

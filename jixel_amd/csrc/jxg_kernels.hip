@@ -549,7 +549,7 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
   constexpr int sites_1d = 32 / n, nsites = sites_1d * sites_1d, size = n * n;
   constexpr int kind = n == 8 ? JXG_QT_DCT8 : (n == 16 ? JXG_QT_DCT16X16 : JXG_QT_DCT32X32);
   constexpr int cls = n == 8 ? 0 : (n == 16 ? 2 : 4);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint32_t qoff = D->qt_off[kind], qn = D->qt_n[kind];
   // 2-D transform in place: 8x8 rows first, 16x16 / 32x32 columns first
   // (src/dct.rs:631-655,1010-1030,1980-2000).
@@ -848,7 +848,7 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
   __shared__ uint8_t q_in[16];
   __shared__ uint8_t cell_out[16];
   __shared__ uint8_t q_out[16];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int rbx = bix * 4, rby = biy * 4;  // region's first block (global)
   const int gx = rbx >> 8, gy = rby >> 8;
   const int xsize = min(256, xb - gx * 256), ysize = min(256, yb - gy * 256);
@@ -1321,16 +1321,18 @@ __device__ __forceinline__ CoefGeom coef_geom(const QLds &L, int t, int s, int k
 // (scalar registers), and the chunk's outputs are contiguous (chunks start
 // at multiples of 64 and an output row holds cov_x * 64 >= 64 values).
 struct ChunkGeom {
-  int kc, lw, lh, lwc, mode, origin;  // mode 0: 8x8 u-major, 1: v-major, 2: u-major
-  bool d64x, d64y;
+  // storage index k splits into k >> sh and k mod 2^sh: (u, v) in that order,
+  // (v, u) when swap (8x8: sh = 3; v-major: sh = lw, swap; u-major: sh = lh)
+  int kc, lw, lh, lwc, sh, origin;
+  bool swap, d64x, d64y;
   size_t out;  // output index of storage index kc
 };
 
 // A chunk's uniform data, packed once per tile into four words:
-//   d0: LDS origin (12 bits) | mode << 12 | (lw - 3) << 14 | (lh - 3) << 16 |
+//   d0: LDS origin (12 bits) | (sh - 3) << 12 | (lw - 3) << 14 | (lh - 3) << 16 |
 //       d64x << 18 | d64y << 19 | area class << 20 | kc mod 2^lch << 23
 //   d1: block row of kc (by + kc >> lch) | block column << 16
-//   d2: kc | transform << 12 | quant-table size << 18
+//   d2: kc | transform << 12 | quant-table size << 18 | swap << 31
 //   d3: quant-table offset + kc
 // (lch = 6 + log2 cov_x: an output row of blocks holds 2^lch values; block
 // coordinates are below 2^15 and the LDS origin below 2^12.)
@@ -1339,12 +1341,14 @@ struct ChunkGeom {
 __device__ __forceinline__ void chunk_pack(QLds &L, int t, uint32_t qoff, uint32_t qn) {
   const int s = L.s[t], cvx = cov_x(s), cvy = cov_y(s);
   const int lw = 3 + ctz32(cvx), lh = 3 + ctz32(cvy), lch = 6 + ctz32(cvx);
-  const int mode = s == JXG_DCT ? 0 : ((rows_first(s) || is_sub8(s)) ? 1 : 2);
+  const bool swap = s != JXG_DCT && (rows_first(s) || is_sub8(s));  // v-major storage
+  const int sh = s == JXG_DCT ? 3 : (swap ? lw : lh);
   const int cls = jxg_area_class(max(cvx, cvy), min(cvx, cvy));
-  const uint32_t x0 = (uint32_t)(L.ly[t] * QT_PITCH + L.lx[t]) | (uint32_t)mode << 12 |
+  const uint32_t x0 = (uint32_t)(L.ly[t] * QT_PITCH + L.lx[t]) | (uint32_t)(sh - 3) << 12 |
                       (uint32_t)(lw - 3) << 14 | (uint32_t)(lh - 3) << 16 | (uint32_t)(cvx == 8) << 18 |
                       (uint32_t)(cvy == 8) << 19 | (uint32_t)cls << 20;
-  const uint32_t by = L.by[t], bx16 = (uint32_t)L.bx[t] << 16, z0 = (uint32_t)t << 12 | qn << 18;
+  const uint32_t by = L.by[t], bx16 = (uint32_t)L.bx[t] << 16;
+  const uint32_t z0 = (uint32_t)t << 12 | qn << 18 | (uint32_t)swap << 31;
   const int e0 = L.prec[t], e1 = L.prec[t + 1];
   for (int e = e0; e < e1; e++) {
     const uint32_t kc = (uint32_t)(e - e0) << 6;
@@ -1366,7 +1370,8 @@ __device__ __forceinline__ ChunkDesc chunk_desc(const QLds &L, int ch, int xb) {
   const uint32_t d2 = __builtin_amdgcn_readfirstlane(d.z), d3 = __builtin_amdgcn_readfirstlane(d.w);
   ChunkDesc C;
   C.G.origin = (int)(d0 & 0xfffu);
-  C.G.mode = (int)((d0 >> 12) & 3u);
+  C.G.sh = 3 + (int)((d0 >> 12) & 3u);
+  C.G.swap = d2 >> 31;
   C.G.lw = 3 + (int)((d0 >> 14) & 3u);
   C.G.lh = 3 + (int)((d0 >> 16) & 3u);
   C.G.lwc = C.G.lw > C.G.lh ? C.G.lw : C.G.lh;
@@ -1375,7 +1380,7 @@ __device__ __forceinline__ ChunkDesc chunk_desc(const QLds &L, int ch, int xb) {
   C.cls = (int)((d0 >> 20) & 7u);
   C.G.kc = (int)(d2 & 0xfffu);
   C.t = (int)((d2 >> 12) & 63u);
-  C.qn = (int)(d2 >> 18);
+  C.qn = (int)((d2 >> 18) & 0x1fffu);
   C.toff = d3;
   C.G.out = (size_t)((d1 & 0xffffu) * (uint32_t)xb + (d1 >> 16)) * 64 + (d0 >> 23);
   return C;
@@ -1385,17 +1390,8 @@ __device__ __forceinline__ ChunkDesc chunk_desc(const QLds &L, int ch, int xb) {
 template <bool BIG>
 __device__ __forceinline__ void lane_geom(const ChunkGeom &G, int lane, int &a, int &row, int &col) {
   const int k = G.kc + lane;
-  int v, u;
-  if (G.mode == 0) {
-    u = k >> 3;
-    v = k & 7;
-  } else if (G.mode == 1) {
-    v = k >> G.lw;
-    u = k & ((1 << G.lw) - 1);
-  } else {
-    u = k >> G.lh;
-    v = k & ((1 << G.lh) - 1);
-  }
+  const int hi = k >> G.sh, lo = k & ((1 << G.sh) - 1);
+  int u = G.swap ? lo : hi, v = G.swap ? hi : lo;
   // only the DCT64 family has 64-long axes
   if (BIG && G.d64y) v = dct64_pos(v);
   if (BIG && G.d64x) u = dct64_pos(u);
@@ -1548,7 +1544,8 @@ __device__ __forceinline__ int16_t dc_cfl_q(int c, float xd, float yq, const jxg
 template <bool F, bool BIG>
 __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win_w, int win_h) {
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+  // the wavefront index is uniform: scalar registers, scalar branches
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = blockDim.x >> 6;
   const jxg_derived *D = A.D;
   const int n = L.n;
   if (n == 0) return;

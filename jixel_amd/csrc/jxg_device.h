@@ -25,18 +25,16 @@ __device__ __forceinline__ float fmla(float a, float b, float c) {
 __device__ __forceinline__ float bits_f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t f_bits(float f) { return __float_as_uint(f); }
 
-// Rust `as i32` / `as i16`: saturating, NaN -> 0.
+// Rust `as i32` / `as i16`: saturating, NaN -> 0.  Branch-free: the value is
+// clamped into the target range (2147483520 is the largest float below 2^31;
+// a NaN clamps to the low bound and is then replaced by 0) and truncated.
 __device__ __forceinline__ int32_t sat_i32(float v) {
-  if (v != v) return 0;
-  if (v >= 2147483648.0f) return INT32_MAX;
-  if (v <= -2147483648.0f) return INT32_MIN;
-  return (int32_t)v;
+  const int32_t r = (int32_t)fminf(fmaxf(v, -2147483648.0f), 2147483520.0f);
+  return v != v ? 0 : (v >= 2147483648.0f ? INT32_MAX : r);
 }
 __device__ __forceinline__ int16_t sat_i16(float v) {
-  if (v != v) return 0;
-  if (v >= 32767.0f) return INT16_MAX;
-  if (v <= -32768.0f) return INT16_MIN;
-  return (int16_t)v;
+  const int32_t r = (int32_t)fminf(fmaxf(v, -32768.0f), 32767.0f);
+  return (int16_t)(v != v ? 0 : r);
 }
 
 // f32::round: ties away from zero.

@@ -1120,6 +1120,14 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ x
 #define QT_SUB8_SLOTS 12
 
 // Sized so that four workgroups fit the 160 KB of LDS of a CU.
+// The derived parameters are never written by a kernel: read through the
+// constant address space, uniform reads of them are scalar loads even in
+// loops that store to global memory.
+typedef __attribute__((address_space(4))) const jxg_derived jxg_derived_c;
+__device__ __forceinline__ const jxg_derived_c *derived_c(const jxg_derived *p) {
+  return (const jxg_derived_c *)p;
+}
+
 struct QLds {
   float y2[QT_TILE * QT_PITCH];   // dequantized luma with zeroed LLF (CfL source)
   float cur[QT_TILE * QT_PITCH];  // the channel being transformed
@@ -1546,7 +1554,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   const int tid = threadIdx.x;
   // the wavefront index is uniform: scalar registers, scalar branches
   const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), nwaves = blockDim.x >> 6;
-  const jxg_derived *D = A.D;
+  const jxg_derived_c *D = derived_c(A.D);
   const int n = L.n;
   if (n == 0) return;
   // Prefix sums of the transforms' pass-1 lines, pass-2 lines and
@@ -1684,7 +1692,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
         const int lmin = G.lw > G.lh ? G.lh : G.lw;
         const int cx = 1 << (G.lwc - 3), cy = 1 << (lmin - 3);
         A.co1[G.out + lane] = yv[u];
-        const float *thr = D->thr[1][C[u].cls];
+        const auto *thr = D->thr[1][C[u].cls];
         const float thr0 = thr[0], thr1 = thr[1], thr2 = thr[2], thr3 = thr[3];
         const int hc = 1 << lmin, wc = 1 << G.lwc;
         const float threshold = row >= hc / 2 ? (col >= wc / 2 ? thr3 : thr2)
@@ -1744,7 +1752,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
           const float v = fmla<F>(-f, yq[u], cv[u]);
           L.cur[a] = v;
           co[G.out + lane] = v;
-          const float *thr = D->thr[c][C[u].cls];
+          const auto *thr = D->thr[c][C[u].cls];
           const float thr0 = thr[0], thr1 = thr[1], thr2 = thr[2], thr3 = thr[3];
           const int hc = 1 << (G.lw > G.lh ? G.lh : G.lw), wc = 1 << G.lwc;
           const float threshold = row >= hc / 2 ? (col >= wc / 2 ? thr3 : thr2)
@@ -1764,7 +1772,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
         if (i < cvx * cov_y(L.s[t])) {
           int16_t *dc = c == 0 ? A.dc0 : A.dc2;
           dc[(size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx] =
-              dc_cfl_q<F>(c, L.dc64v[j][i], (float)L.ydc64[j][i], D);
+              dc_cfl_q<F>(c, L.dc64v[j][i], (float)L.ydc64[j][i], A.D);
         }
       }
     }

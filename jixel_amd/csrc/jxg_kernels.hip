@@ -209,32 +209,39 @@ template <bool F>
 __device__ __forceinline__ void aq_pre_body(const float *__restrict__ yp, int stride, int W,
                                                 int H, int pre_w, int pre_h,
                                                 float *__restrict__ pre, int bix, int biy) {
-  const int i = bix * 256 + threadIdx.x;
-  const int j = biy;
+  // Lane = pixel column gx, so a wavefront's loads are consecutive floats;
+  // the four lanes of output i = gx / 4 (which exit together) each run one
+  // column's sum over the four rows, and lane 4 i combines them in the
+  // order (((c0 + c1) + c2) + c3).
+  const int gx = bix * 256 + threadIdx.x;
+  const int i = gx >> 2, j = biy;
   if (i >= pre_w || j >= pre_h) return;
-  float acc[4];
+  const int gxc = min(gx, W - 1), gx1 = min(max(gx - 1, 0), W - 1), gx2 = min(gx + 1, W - 1);
+  float col[6];  // column gxc, rows 4 j - 1 .. 4 j + 4 clamped into the image
+#pragma unroll
+  for (int k = 0; k < 6; k++) col[k] = yp[(size_t)min(max(4 * j - 1 + k, 0), H - 1) * stride + gxc];
+  float lft[4], rgt[4];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
-    const int gy = 4 * j + r;
-    const int gyc = min(gy, H - 1), gy1 = min(max(gy - 1, 0), H - 1), gy2 = min(gy + 1, H - 1);
-    const float *row = yp + (size_t)gyc * stride;
-    const float *row1 = yp + (size_t)gy1 * stride;
-    const float *row2 = yp + (size_t)gy2 * stride;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int gx = 4 * i + c;
-      const int gxc = min(gx, W - 1), gx1 = min(max(gx - 1, 0), W - 1), gx2 = min(gx + 1, W - 1);
-      const float in_y = row[gxc];
-      const float base = 0.25f * (row2[gxc] + row1[gxc] + row[gx1] + row[gx2]);
-      const float gammac = ratio_gamma<false>(in_y + 0.019f);
-      float diff = gammac * (in_y - base);
-      diff *= diff;
-      if (diff >= 0.2f) diff = 0.2f;
-      diff = masking_sqrt<F>(diff);
-      acc[c] = r == 0 ? diff : acc[c] + diff;
-    }
+    const float *row = yp + (size_t)min(4 * j + r, H - 1) * stride;
+    lft[r] = row[gx1];
+    rgt[r] = row[gx2];
   }
-  pre[(size_t)j * pre_w + i] = (acc[0] + acc[1] + acc[2] + acc[3]) * 0.25f;
+  float acc = 0.0f;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    // rows gy - 1, gy, gy + 1 (clamped) are col[r], col[r + 1], col[r + 2]
+    const float in_y = col[r + 1];
+    const float base = 0.25f * (col[r + 2] + col[r] + lft[r] + rgt[r]);
+    const float gammac = ratio_gamma<false>(in_y + 0.019f);
+    float diff = gammac * (in_y - base);
+    diff *= diff;
+    if (diff >= 0.2f) diff = 0.2f;
+    diff = masking_sqrt<F>(diff);
+    acc = r == 0 ? diff : acc + diff;
+  }
+  const float c1 = __shfl_down(acc, 1, 64), c2 = __shfl_down(acc, 2, 64), c3 = __shfl_down(acc, 3, 64);
+  if ((gx & 3) == 0) pre[(size_t)j * pre_w + i] = (((acc + c1) + c2) + c3) * 0.25f;
 }
 
 template <bool F>
@@ -259,24 +266,19 @@ __device__ __forceinline__ void sort4(float *a) {
 #undef JXG_CSWAP
 }
 
+// Inserts v into the ascending m[0..3], keeping the four smallest (v goes
+// before the first element it is strictly below).  Selects on the same
+// comparisons: for an ascending m, v < m[i] implies v < m[i + 1].
 __device__ __forceinline__ void store_min4(float v, float *m) {
-  if (v < m[3]) {
-    if (v < m[0]) {
-      m[3] = m[2];
-      m[2] = m[1];
-      m[1] = m[0];
-      m[0] = v;
-    } else if (v < m[1]) {
-      m[3] = m[2];
-      m[2] = m[1];
-      m[1] = v;
-    } else if (v < m[2]) {
-      m[3] = m[2];
-      m[2] = v;
-    } else {
-      m[3] = v;
-    }
-  }
+  const bool c0 = v < m[0], c1 = v < m[1], c2 = v < m[2], c3 = v < m[3];
+  const float n3 = c3 ? (c2 ? m[2] : v) : m[3];
+  const float n2 = c2 ? (c1 ? m[1] : v) : m[2];
+  const float n1 = c1 ? (c0 ? m[0] : v) : m[1];
+  const float n0 = c0 ? v : m[0];
+  m[0] = n0;
+  m[1] = n1;
+  m[2] = n2;
+  m[3] = n3;
 }
 
 // Per-block modulations (fill_quant_field_scalar stage 3,
@@ -379,22 +381,31 @@ __device__ __forceinline__ void aq_block_body(const float *__restrict__ xyb, int
   const float k_total = 0.29959705784054957f;
 #pragma unroll
   for (int i = 0; i < 4; i++) kmul[i] *= k_total / norm;
+  // The block's four quadrants (fx, fy) = (fx0 + qx, fy0 + qy) and their
+  // neighbours, clamped into the group's pre-map, are exactly the 4x4 window
+  // rows fy0 - 1 .. fy0 + 2, columns fx0 - 1 .. fx0 + 2 clamped the same way
+  // (fx0, fy0 are even and below pw - 1, ph - 1): all sixteen loads are
+  // issued together.
+  const int fy0 = 2 * (by - gy * 256), fx0 = 2 * (bx - gx * 256);
+  float win[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int yy = min(max(fy0 - 1 + r, 0), ph - 1);
+    const float *row = pre + (size_t)(ppy0 + yy) * pre_w + ppx0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) win[r][c] = row[min(max(fx0 - 1 + c, 0), pw - 1)];
+  }
   float aq = 0.0f;
+#pragma unroll
   for (int q = 0; q < 4; q++) {
-    const int fy = 2 * (by - gy * 256) + (q >> 1);
-    const int fx = 2 * (bx - gx * 256) + (q & 1);
-    const int ym1 = fy >= 1 ? fy - 1 : fy, yp1 = fy + 1 < ph ? fy + 1 : fy;
-    const int xm1 = fx >= 1 ? fx - 1 : fx, xp1 = fx + 1 < pw ? fx + 1 : fx;
-    const float *rt = pre + (size_t)(ppy0 + ym1) * pre_w + ppx0;
-    const float *rm = pre + (size_t)(ppy0 + fy) * pre_w + ppx0;
-    const float *rb = pre + (size_t)(ppy0 + yp1) * pre_w + ppx0;
-    float mins[4] = {rm[fx], rm[xm1], rm[xp1], rt[xm1]};
+    const int R = 1 + (q >> 1), C = 1 + (q & 1);  // the quadrant's window position
+    float mins[4] = {win[R][C], win[R][C - 1], win[R][C + 1], win[R - 1][C - 1]};
     sort4(mins);
-    store_min4(rt[fx], mins);
-    store_min4(rt[xp1], mins);
-    store_min4(rb[xm1], mins);
-    store_min4(rb[fx], mins);
-    store_min4(rb[xp1], mins);
+    store_min4(win[R - 1][C], mins);
+    store_min4(win[R - 1][C + 1], mins);
+    store_min4(win[R + 1][C - 1], mins);
+    store_min4(win[R + 1][C], mins);
+    store_min4(win[R + 1][C + 1], mins);
     const float v = kmul[0] * mins[0] + kmul[1] * mins[1] + kmul[2] * mins[2] + kmul[3] * mins[3];
     aq = q == 0 ? v : aq + v;
   }
@@ -2109,7 +2120,7 @@ JXG_HIDDEN void batch_grids(uint32_t W, uint32_t H, uint32_t stride, uint32_t ph
   const uint32_t xb = (W + 7) / 8, yb = (H + 7) / 8, tx = (W + 63) / 64, ty = (H + 63) / 64;
   gw[BS_XYB] = (stride + XYB_PX_WG - 1) / XYB_PX_WG;
   gh[BS_XYB] = (ph + XYB_ROWS - 1) / XYB_ROWS;
-  gw[BS_AQPRE] = (2 * xb + 255) / 256;
+  gw[BS_AQPRE] = (8 * xb + 255) / 256;  // four lanes per pre-map value
   gh[BS_AQPRE] = 2 * yb;
   gw[BS_AQBLK] = (xb + aq_strip - 1) / aq_strip;
   gh[BS_AQBLK] = yb;
@@ -2651,7 +2662,7 @@ static int fill_qf(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t p
   int rc = ensure(&ctx->d_pre, &ctx->pre_cap, (size_t)pre_w * pre_h);
   if (rc) return rc;
   const float *yp = d_xyb + (size_t)stride * padded_h;
-  const dim3 g1((pre_w + 255) / 256, pre_h);
+  const dim3 g1((4 * pre_w + 255) / 256, pre_h);  // four lanes per pre-map value
   if (fused)
     k_aq_pre<true><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, ctx->d_pre);
   else

@@ -63,7 +63,10 @@ __device__ __forceinline__ void cmap_blocks_body(const float *__restrict__ xyb, 
                                                       const float *__restrict__ qm_b, CmapArgs a,
                                                       float *__restrict__ part, int tiles_x_, int bix, int biy) {
   __shared__ float t[3][ROWS * CM_P];  // X, Y, B
+  __shared__ float qms[2][64];         // the X and B quantization weights (LDS: read per step)
   const int tid = threadIdx.x;
+  static_assert(CM_T >= 128, "one thread per weight");
+  if (tid < 128) qms[tid >> 6][tid & 63] = tid < 64 ? qm_x[tid] : qm_b[tid - 64];
   constexpr int PARTS = 64 / ROWS;  // sub-tiles per 64x64 tile
   const int tx = bix, ty = biy / PARTS, quarter = biy % PARTS;
   const int x0 = tx * 64, y0 = ty * 64 + quarter * ROWS;
@@ -108,7 +111,7 @@ __device__ __forceinline__ void cmap_blocks_body(const float *__restrict__ xyb, 
       if (k == 0) y = o = 0.0f;  // the regression ignores the DC
       const float fo = fmla<F>(shift, y, o);
       o = shift != 0.0f ? fo : o;
-      const float qm = which < 2 ? qm_x[k] : qm_b[k];
+      const float qm = qms[which < 2 ? 0 : 1][k];
       const float m = y * qm, s_o = o * qm;
       const float am = K_INV * m;
       // which 0 / 2: a * a; 1: a * (-s_x); 3: a * (m_b - s_b) (fmla(1, m, -s) is exact)

@@ -500,7 +500,6 @@ struct SelLds {
   uint32_t nzp[3][16], msp[3][16];  // per slot 4 j + wave: nonzeros, 1 + last nonzero scan position
   float cost8[16], cost16[4], cost32;
   double red1[4][5], red2[4][5];
-  uint32_t redc[4][5];
   double site_s1[5], site_s2[5];
   uint32_t site_cnt[5];
   double erased[4];
@@ -888,24 +887,28 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
   const bool work = D->run_selection && any_row && any_col;
   const size_t plane = (size_t)stride * padded_h;
   const int px0 = rbx * 8, py0 = rby * 8;
-  // the region's pixels, loaded under the table set-up (one barrier for both)
-  if (work) load_region_nosync(L, xyb, plane, stride, px0, py0);
-  __syncthreads();
+  // cmap_factors (src/ac_strategy.rs:1838-1855): this region's 64x64 tile,
+  // clamped to the DC group's tile grid; read ahead of the barrier.
+  int8_t ytox_t = 0, ytob_t = 0;
   if (work) {
-    // cmap_factors (src/ac_strategy.rs:1838-1855): this region's 64x64 tile,
-    // clamped to the DC group's tile grid.
     const int tiles_x = (W + 63) / 64;
     const int ntx = (xsize + 7) / 8, nty = (ysize + 7) / 8;
     const int tx = min(bx / 8, ntx - 1), ty = min(by / 8, nty - 1);
     const size_t ti = (size_t)(gy * 32 + ty) * tiles_x + gx * 32 + tx;
-    const float cmx = fmla<F>((float)ytox[ti], D->cfl_inv_color_factor, D->cfl_base_x);
-    const float cmb = fmla<F>((float)ytob[ti], D->cfl_inv_color_factor, D->cfl_base_b);
+    ytox_t = ytox[ti];
+    ytob_t = ytob[ti];
+  }
+  // the region's pixels, loaded under the table set-up (one barrier for both)
+  if (work) load_region_nosync(L, xyb, plane, stride, px0, py0);
+  __syncthreads();
+  if (work) {
+    const float cmx = fmla<F>((float)ytox_t, D->cfl_inv_color_factor, D->cfl_base_x);
+    const float cmb = fmla<F>((float)ytob_t, D->cfl_inv_color_factor, D->cfl_base_b);
     // Site luma statistics for MergeTuning::site_bias (site_variance_rect,
     // src/ac_strategy.rs:224-245): f64 sums of the in-image pixels of the
     // four 16x16 sites and the 32x32 region, reduced over the workgroup.
     {
       double s1[5] = {0, 0, 0, 0, 0}, s2[5] = {0, 0, 0, 0, 0};
-      uint32_t cnt[5] = {0, 0, 0, 0, 0};
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int e = tid + SEL_T * j, r = e >> 5, col = e & 31;
@@ -917,28 +920,28 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
             if (s == q) {
               s1[s] += v;
               s2[s] += v * v;
-              cnt[s]++;
             }
           s1[4] += v;
           s2[4] += v * v;
-          cnt[4]++;
         }
       }
 #pragma unroll
       for (int q = 0; q < 5; q++) {
         const double a1 = wave_sum_f64(s1[q]), a2 = wave_sum_f64(s2[q]);
-        const uint32_t cc = wave_sum_u32(cnt[q]);
         if (lane == 0) {
           L.red1[wave][q] = a1;
           L.red2[wave][q] = a2;
-          L.redc[wave][q] = cc;
         }
       }
       __syncthreads();
       if (tid < 5) {
         L.site_s1[tid] = ((L.red1[0][tid] + L.red1[1][tid]) + L.red1[2][tid]) + L.red1[3][tid];
         L.site_s2[tid] = ((L.red2[0][tid] + L.red2[1][tid]) + L.red2[2][tid]) + L.red2[3][tid];
-        L.site_cnt[tid] = L.redc[0][tid] + L.redc[1][tid] + L.redc[2][tid] + L.redc[3][tid];
+        // in-image pixels of the site: 16x16 quadrant tid (0..3) or the region
+        const int oy = tid < 4 ? 16 * (tid >> 1) : 0, ox = tid < 4 ? 16 * (tid & 1) : 0;
+        const int ext = tid < 4 ? 16 : 32;
+        const int ny = min(max(H - py0 - oy, 0), ext), nx = min(max(W - px0 - ox, 0), ext);
+        L.site_cnt[tid] = (uint32_t)(ny * nx);
       }
     }
     float qac8[16], qac16[4], qac32[1];

@@ -925,12 +925,27 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
           s2[4] += v * v;
         }
       }
+      // the ten wavefront sums level by level (each keeps wave_sum_f64's
+      // butterfly), so their shuffles are in flight together
 #pragma unroll
-      for (int q = 0; q < 5; q++) {
-        const double a1 = wave_sum_f64(s1[q]), a2 = wave_sum_f64(s2[q]);
-        if (lane == 0) {
-          L.red1[wave][q] = a1;
-          L.red2[wave][q] = a2;
+      for (int off = 32; off > 0; off >>= 1) {
+        double t1[5], t2[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+          t1[q] = __shfl_xor(s1[q], off, 64);
+          t2[q] = __shfl_xor(s2[q], off, 64);
+        }
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+          s1[q] += t1[q];
+          s2[q] += t2[q];
+        }
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+          L.red1[wave][q] = s1[q];
+          L.red2[wave][q] = s2[q];
         }
       }
       __syncthreads();

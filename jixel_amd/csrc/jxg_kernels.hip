@@ -1161,6 +1161,7 @@ struct QLds {
   float y2[QT_TILE * QT_PITCH];   // dequantized luma with zeroed LLF (CfL source)
   float cur[QT_TILE * QT_PITCH];  // the channel being transformed
   uint4 cdesc[64];                 // 64-coefficient chunk -> packed descriptor (chunk_desc)
+  int qnext[3];                    // per channel: the next chunk a wavefront takes
   int n;
   uint8_t s[QT_MAXT];
   uint16_t bx[QT_MAXT], by[QT_MAXT];  // global first block
@@ -1648,6 +1649,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   if (tid < 64)
     L.quot[tid] = tid >= 2 ? D->dequant_numerator / (float)tid
                            : (tid == 1 ? 1.0f - D->dequant_one[1] : 0.0f);
+  if (tid < 3) L.qnext[tid] = 0;
   __syncthreads();
   // line / chunk -> transform tables
   if (tid < n) {
@@ -1688,18 +1690,24 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     }
   }
   // Y quantize / dequantize round trip (quantize_roundtrip_y_block,
-  // src/group.rs:795) and Y LLF zeroing (:1852).  Each wavefront takes 64
-  // consecutive coefficients of one transform; four chunks per round so the
-  // table loads of a round are in flight together.
+  // src/group.rs:795) and Y LLF zeroing (:1852).  A chunk is 64 consecutive
+  // coefficients of one transform; a wavefront takes four chunks at a time
+  // from the channel's queue (so the wavefront that ran the DC above takes
+  // fewer) and issues their table loads together.
   const int nch = L.prec[n];
   const float num = D->dequant_numerator;
-  for (int base = wave; base < nch; base += 4 * nwaves) {
+  (void)nwaves;
+  for (;;) {
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&L.qnext[1], 4);
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (base >= nch) break;
     float tq[4], td[4], yv[4];
     ChunkDesc C[4];
     int aa[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int ch = base + u * nwaves;
+      const int ch = base + u;
       if (ch < nch) {
         C[u] = chunk_desc(L, ch, A.xb);
         int row, col;
@@ -1712,7 +1720,7 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int ch = base + u * nwaves;
+      const int ch = base + u;
       if (ch < nch) {
         const ChunkGeom &G = C[u].G;
         const int t = C[u].t;
@@ -1750,49 +1758,11 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
     load_plane(L, A, c, px0, py0, win_w, win_h);
     __syncthreads();
     transform_plane<F, BIG>(L, n);
-    int32_t *lv = c == 0 ? A.lv0 : A.lv2;
-    float *co = c == 0 ? A.co0 : A.co2;
-    const float qm = D->qm_mult[c];
-    for (int base = wave; base < nch; base += 4 * nwaves) {
-      float tq[4], yq[4], cv[4];
-      ChunkDesc C[4];
-      int aa[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int ch = base + u * nwaves;
-        if (ch < nch) {
-          C[u] = chunk_desc(L, ch, A.xb);
-          int row, col;
-          lane_geom<BIG>(C[u].G, lane, aa[u], row, col);
-          tq[u] = A.tables[C[u].toff + (3 + c) * C[u].qn + lane];
-          yq[u] = L.y2[aa[u]];
-          cv[u] = L.cur[aa[u]];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int ch = base + u * nwaves;
-        if (ch < nch) {
-          const ChunkGeom &G = C[u].G;
-          const int t = C[u].t;
-          int a, row, col;
-          lane_geom<BIG>(G, lane, a, row, col);
-          const float f = c == 0 ? L.xf[t] : L.bf[t];
-          const float v = fmla<F>(-f, yq[u], cv[u]);
-          L.cur[a] = v;
-          co[G.out + lane] = v;
-          const auto *thr = D->thr[c][C[u].cls];
-          const float thr0 = thr[0], thr1 = thr[1], thr2 = thr[2], thr3 = thr[3];
-          const int hc = 1 << (G.lw > G.lh ? G.lh : G.lw), wc = 1 << G.lwc;
-          const float threshold = row >= hc / 2 ? (col >= wc / 2 ? thr3 : thr2)
-                                                : (col >= wc / 2 ? thr1 : thr0);
-          const float q_scaled = D->scale * (float)L.quant[t] * qm;
-          const float val = (tq[u] * q_scaled) * v;
-          lv[G.out + lane] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
-        }
-      }
-    }
-    __syncthreads();
+    // DC with CfL from the low-frequency coefficients.  The CfL step of the
+    // AC loop subtracts f * y2 and y2 is 0 there, so they hold the same
+    // values before and after it (up to the sign of a zero, which the
+    // integer DC does not see): the DC runs first, beside the other
+    // wavefronts' chunks.
     if constexpr (BIG) dc64_values<F>(L);
     if (BIG && tid < 128) {
       const int j = tid >> 6, i = tid & 63;
@@ -1825,6 +1795,51 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
             qd = sat_i16(round_away(fmla<F>(xd[i], D->inv_factor[2], -yq * D->cfl_factor_b)));
           }
           dc[(size_t)(L.by[t] + i / cvx) * A.xb + L.bx[t] + i % cvx] = qd;
+        }
+      }
+    }
+    int32_t *lv = c == 0 ? A.lv0 : A.lv2;
+    float *co = c == 0 ? A.co0 : A.co2;
+    const float qm = D->qm_mult[c];
+    for (;;) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(&L.qnext[c], 4);
+      base = __builtin_amdgcn_readfirstlane(base);
+      if (base >= nch) break;
+      float tq[4], yq[4], cv[4];
+      ChunkDesc C[4];
+      int aa[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int ch = base + u;
+        if (ch < nch) {
+          C[u] = chunk_desc(L, ch, A.xb);
+          int row, col;
+          lane_geom<BIG>(C[u].G, lane, aa[u], row, col);
+          tq[u] = A.tables[C[u].toff + (3 + c) * C[u].qn + lane];
+          yq[u] = L.y2[aa[u]];
+          cv[u] = L.cur[aa[u]];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int ch = base + u;
+        if (ch < nch) {
+          const ChunkGeom &G = C[u].G;
+          const int t = C[u].t;
+          int a, row, col;
+          lane_geom<BIG>(G, lane, a, row, col);
+          const float f = c == 0 ? L.xf[t] : L.bf[t];
+          const float v = fmla<F>(-f, yq[u], cv[u]);
+          co[G.out + lane] = v;
+          const auto *thr = D->thr[c][C[u].cls];
+          const float thr0 = thr[0], thr1 = thr[1], thr2 = thr[2], thr3 = thr[3];
+          const int hc = 1 << (G.lw > G.lh ? G.lh : G.lw), wc = 1 << G.lwc;
+          const float threshold = row >= hc / 2 ? (col >= wc / 2 ? thr3 : thr2)
+                                                : (col >= wc / 2 ? thr1 : thr0);
+          const float q_scaled = D->scale * (float)L.quant[t] * qm;
+          const float val = (tq[u] * q_scaled) * v;
+          lv[G.out + lane] = fabsf(val) >= threshold ? sat_i32(round_away(val)) : 0;
         }
       }
     }

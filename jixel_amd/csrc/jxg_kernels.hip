@@ -1587,6 +1587,39 @@ __device__ void quantize_list(QLds &L, const QArgs &A, int px0, int py0, int win
   const jxg_derived_c *D = derived_c(A.D);
   const int n = L.n;
   if (n == 0) return;
+  // The list in order of line length (no output depends on the list order,
+  // which the slot atomics leave arbitrary), so the lanes of a wavefront in
+  // the line passes mostly run one length.  Wavefront 0 ranks the
+  // transforms by (length, index) with ballots.
+  if (n > 1) {
+    int s_ = 0, bx_ = 0, by_ = 0, lx_ = 0, ly_ = 0, key = 4;
+    if (tid < n) {
+      s_ = L.s[tid];
+      bx_ = L.bx[tid];
+      by_ = L.by[tid];
+      lx_ = L.lx[tid];
+      ly_ = L.ly[tid];
+      key = is_sub8(s_) ? 0 : ctz32(max(cov_x(s_), cov_y(s_)));
+    }
+    __syncthreads();
+    if (tid < 64) {
+      const uint64_t lt = (1ull << lane) - 1;
+      int pos = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint64_t m = __ballot(key == k);
+        pos += k < key ? __popcll(m) : (k == key ? __popcll(m & lt) : 0);
+      }
+      if (tid < n) {
+        L.s[pos] = (uint8_t)s_;
+        L.bx[pos] = (uint16_t)bx_;
+        L.by[pos] = (uint16_t)by_;
+        L.lx[pos] = (uint8_t)lx_;
+        L.ly[pos] = (uint8_t)ly_;
+      }
+    }
+    __syncthreads();
+  }
   // Prefix sums of the transforms' pass-1 lines, pass-2 lines and
   // 64-coefficient chunks (n <= 64: one wavefront, lane = transform), and the
   // first two DCT64-family transforms in list order.

@@ -484,7 +484,11 @@ __global__ __launch_bounds__(256) void k_aq_block(const float *__restrict__ xyb,
 // ---------------------------------------------------------------------------
 // Candidate costs and square-merge selection
 
-#define SEL_PITCH 33  // LDS row pitch of the 32x32 region (bank-conflict free)
+// LDS row pitch of the 32x32 region: odd, so the 1-D line passes are
+// conflict-free, and 37 rather than 33 because the transposed scan-order
+// coefficient reads of the quantize step then spread over the banks (33
+// puts up to ten lanes of a scan-order wavefront on one bank).
+#define SEL_PITCH 37
 #define SEL_T 256     // threads per region
 #define SELQ_N (64 + 256 + 1024)  // scan-ordered inverse weights per channel
 #define RD_LAMBDA 0.08086717f

@@ -768,7 +768,9 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
         mscan = max(mscan, L.msp[c][site * per + i]);
       }
       const float nzf = (float)nzc;
-      const float header = 0.4f * (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf));
+      // L.rate holds rate_lut[0..255]: the common counts read LDS
+      const float header =
+          0.4f * (nzc < 256u ? L.rate[nzc] : (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf)));
       float visited = 0.0f;
       if (nzc != 0) {
         // maxscan holds 1 + the last nonzero's scan position

@@ -87,7 +87,7 @@ def cpu_baseline(rgb, params, threads):
     import jixel_amd as J
     from oracle import oracle as O
     h, w, _ = rgb.shape
-    tables = J.default_quant_tables()
+    tables = O.tables_for(params)
     O.front_end(rgb[:64, :64].copy(), params, tables, threads)  # warm
     runs = []
     t_end = time.perf_counter() + 12.0

@@ -141,6 +141,11 @@ typedef struct jxg_frame_params {
   int32_t epf_iters;
   int32_t flags;  /* JXG_FLAG_* stage switches */
   float channel_weight[3]; /* EncodingContext::channel_weights (X, Y, B) */
+  int32_t arbiter; /* JXG_ARBITER_*: selects the default quant tables */
+  int32_t xyb_order;  /* JXG_XYB_SCALAR or JXG_XYB_AVX2: the front end's XYB flavour */
+  float xyb_fwd[9];   /* XybMatrix::fwd of the frame (src/xyb.rs:81-96; the
+                         yellow-opsin variants of src/yellow_opsin.rs:724
+                         change it), row-major */
 } jxg_frame_params;
 
 /* jxg_frame_params::flags.  The first two are on after
@@ -190,14 +195,35 @@ float jxg_distance_from_quality(float quality);
  * non-finite or non-positive distance. */
 int jxg_frame_params_init(float distance, int speed, int fused, jxg_frame_params *out);
 
+/* XybMatrix::SPEC's forward matrix (src/xyb.rs:35-47,81-96), row-major. */
+void jxg_spec_xyb_matrix(float fwd[9]);
+
 /* The 256-entry u8 sRGB -> linear table of TransferLut::build
  * (src/color.rs:35-45,295-340): computed in f64, stored as f32. */
 void jxg_srgb_lut(float out[256]);
 
-/* The spec-default quant-matrix blob (JPEG XL library tables: the band
- * parameters of src/quant_weights.rs:123-1230 without distance overrides).
- * A caller with its own DequantMatrices passes those instead. */
-int jxg_default_quant_tables(float *blob /* JXG_QT_BLOB_FLOATS */);
+/* Arbiter: the metric the reference's lossy encoder tunes for
+ * (Arbiter, src/encode_image.rs:292; per-arbiter tuning src/arbiter.rs:36-62).
+ * The default is SSIMULACRA2 (src/encode_image.rs:717). */
+enum { JXG_ARBITER_SSIMULACRA2 = 0, JXG_ARBITER_CVVDP = 1 };
+
+/* The quant-matrix blob the reference codes with at its default effort:
+ * DequantMatrices::new_fast(distance, arbiter) (src/quant_weights.rs:2294-2337),
+ * the table set EncodingContext::new picks for speeds without rectangles
+ * (src/encoding_context.rs:621-625).  Five distance tiers: the spec tables
+ * below d = 0.3, the flat-B DCT8 rows FLAT_B8_BANDS_HQ / _MID
+ * (src/quant_weights.rs:129-130) above it, the SS2 16 / 32 / 32x16 scales from
+ * qm_ss2_distance (2.25, CVVDP 1.75) and the coarse DCT8 luma tilt from 3.5;
+ * every tier carries the spec IDENTITY / DCT2X2 / DCT4X4 / AFV / DCT64
+ * tables, the stored DCT16X8 table, the refitted DCT4X8 table and the
+ * DCT8X32 table derived from the DCT16X8 bands (shared_tables, tables_4x8,
+ * derive_8x32: src/quant_weights.rs:1729-1836,1986-2064).  `fused` is the
+ * fmla flavour of the table formulas, as in jxg_frame_params.  The device
+ * stages use these tables by default, chosen per frame from
+ * jxg_frame_params::distance / arbiter / fused; jxg_ctx_set_tables replaces
+ * them with a caller's DequantMatrices. */
+int jxg_default_quant_tables(float distance, int arbiter, int fused,
+                             float *blob /* JXG_QT_BLOB_FLOATS */);
 
 /* Plane geometry used by the device stages for a W x H image: rows are
  * padded to a multiple of 64 pixels and the stride to a multiple of 64 floats
@@ -215,8 +241,10 @@ void jxg_ctx_destroy(jxg_ctx *ctx);
 int jxg_ctx_synchronize(jxg_ctx *ctx);
 const char *jxg_last_error(void);
 
-/* Uploads a quant-matrix blob (host pointer) into the context; used by the
- * strategy and quantize stages until replaced. */
+/* Uploads a quant-matrix blob (host pointer) into the context; the CfL,
+ * strategy and quantize stages use it for every frame until it is replaced,
+ * instead of the per-frame default tables (jxg_default_quant_tables).  A NULL
+ * blob returns to the defaults. */
 int jxg_ctx_set_tables(jxg_ctx *ctx, const float *blob);
 
 /* ---- device stages ---- */

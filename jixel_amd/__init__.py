@@ -74,6 +74,7 @@ class EncodeConfig:
     xyb_order: int = 0
     fill_invisible: bool = True
     lossy_alpha: bool = True
+    arbiter: int = 0  # ARBITER_SSIMULACRA2 (src/encode_image.rs:717)
 
     def with_fill_invisible(self, on):
         return replace(self, fill_invisible=bool(on))
@@ -89,6 +90,11 @@ class EncodeConfig:
     def with_speed(self, speed):
         return replace(self, speed=Speed(speed))
 
+    def with_arbiter(self, arbiter):
+        if int(arbiter) not in (ARBITER_SSIMULACRA2, ARBITER_CVVDP):
+            raise JxgError(-1, f"arbiter {arbiter}")
+        return replace(self, arbiter=int(arbiter))
+
 
 def frame_params(config: EncodeConfig) -> FrameParams:
     """compute_distance_params + EncodingContext defaults for `config`."""
@@ -97,6 +103,8 @@ def frame_params(config: EncodeConfig) -> FrameParams:
                                       1 if config.fused else 0, ctypes.byref(p)))
     p.flags = ((FLAG_FILL_INVISIBLE if config.fill_invisible else 0) |
                (FLAG_LOSSY_ALPHA if config.lossy_alpha else 0))
+    p.arbiter = int(config.arbiter)
+    p.xyb_order = int(config.xyb_order)
     return p
 
 
@@ -111,9 +119,16 @@ def qt_offset(kind):
 QT_BLOB_FLOATS = 52224
 
 
-def default_quant_tables():
+ARBITER_SSIMULACRA2 = 0  # JXG_ARBITER_SSIMULACRA2 (the reference's default)
+ARBITER_CVVDP = 1        # JXG_ARBITER_CVVDP
+
+
+def default_quant_tables(distance, fused=True, arbiter=ARBITER_SSIMULACRA2):
+    """DequantMatrices::new_fast(distance, arbiter) (src/quant_weights.rs:2294):
+    the tables the device stages use by default for a frame at `distance`."""
     blob = np.zeros(QT_BLOB_FLOATS, np.float32)
-    check(lib().jxg_default_quant_tables(blob.ctypes.data))
+    check(lib().jxg_default_quant_tables(float(distance), int(arbiter), 1 if fused else 0,
+                                         blob.ctypes.data))
     return blob
 
 

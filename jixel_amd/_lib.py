@@ -19,6 +19,7 @@ ERRORS = {
 EXPORTED = [
     "jxg_qt_size", "jxg_qt_offset", "jxg_qt_kind_of", "jxg_distance_from_quality",
     "jxg_frame_params_init", "jxg_srgb_lut", "jxg_default_quant_tables", "jxg_plane_geometry",
+    "jxg_spec_xyb_matrix",
     "jxg_ctx_create", "jxg_ctx_destroy", "jxg_ctx_synchronize", "jxg_last_error",
     "jxg_ctx_set_tables", "jxg_xyb_dev", "jxg_fill_quant_field_dev", "jxg_strategy_costs_dev",
     "jxg_select_strategy_dev", "jxg_forward_quantize_dev", "jxg_front_end_dev", "jxg_front_end",
@@ -55,6 +56,9 @@ class FrameParams(ctypes.Structure):
         ("epf_iters", ctypes.c_int32),
         ("flags", ctypes.c_int32),
         ("channel_weight", ctypes.c_float * 3),
+        ("arbiter", ctypes.c_int32),
+        ("xyb_order", ctypes.c_int32),
+        ("xyb_fwd", ctypes.c_float * 9),
     ]
 
 
@@ -113,7 +117,9 @@ def _declare(L):
     L.jxg_distance_from_quality.argtypes = [c.c_float]
     L.jxg_frame_params_init.argtypes = [c.c_float, c.c_int, c.c_int, c.POINTER(FrameParams)]
     L.jxg_srgb_lut.argtypes = [vp]
-    L.jxg_default_quant_tables.argtypes = [vp]
+    L.jxg_default_quant_tables.argtypes = [c.c_float, c.c_int, c.c_int, vp]
+    L.jxg_spec_xyb_matrix.argtypes = [vp]
+    L.jxg_spec_xyb_matrix.restype = None
     L.jxg_plane_geometry.argtypes = [c.c_uint32, c.c_uint32, c.POINTER(c.c_uint32),
                                      c.POINTER(c.c_uint32)]
     L.jxg_plane_geometry.restype = None

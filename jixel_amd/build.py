@@ -5,6 +5,7 @@ plain C++.  Floating-point contraction is disabled everywhere so that every
 a*b+c is rounded exactly as the reference rounds it, and packed-FP32
 instructions are disabled for the device code.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -34,15 +35,25 @@ def run(cmd):
     subprocess.check_call(cmd)
 
 
-def build(verbose=True):
+HOST_SOURCES = ["jxg_params.cpp", "jxg_tables.cpp"]
+
+
+def build(verbose=True, jobs=None):
     os.makedirs(BUILD, exist_ok=True)
-    params_o = os.path.join(BUILD, "jxg_params.o")
-    run([CXX] + COMMON + ["-c", os.path.join(CSRC, "jxg_params.cpp"), "-o", params_o])
-    objs = [params_o]
+    cmds, objs = [], []
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        cmds.append([CXX] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     for src in HIP_SOURCES:
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        run([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the translation units are independent: compile them side by side
+    jobs = jobs or min(len(cmds), max(1, (os.cpu_count() or 1)))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs)
     os.replace(tmp, LIB)

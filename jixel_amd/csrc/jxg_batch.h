@@ -39,6 +39,7 @@ struct BatchImg {
   float *co[3];
   int16_t *dc[3];
   const jxg_derived *D;
+  const float *tables, *selq;  // quant tables of the image's frame parameters
   float distance, inv_scale, cfl_base_x, cfl_base_b, cfl_color_factor;
 };
 

@@ -23,6 +23,7 @@ struct jxg_batch {
   jxg_ctx *ctx = nullptr;
   int n = 0;
   int fused = 1;
+  int xyb_order = JXG_XYB_SCALAR;
   std::vector<jxg_batch_image> images;
   std::vector<int> order;  // images in map order (largest first)
   std::vector<uint32_t> geo;  // per image: stride, ph
@@ -98,7 +99,7 @@ int enqueue(jxg_batch *b, hipStream_t st) {
   jxg_ctx *ctx = b->ctx;
   int rc;
   if ((rc = batch_stages_pre(ctx, st, b->d_img, b->d_pfx, b->n, b->total, b->fused, b->d_mlut,
-                             b->aq_strip)))
+                             b->aq_strip, b->xyb_order)))
     return rc;
   if ((rc = batch_cmap(ctx, st, b->d_img, b->d_pfx, b->n, b->total, b->fused, b->cm_rows))) return rc;
   return batch_stages_post(ctx, st, b->d_img, b->d_pfx, b->n, b->total, b->fused, b->d_xt,
@@ -128,6 +129,12 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
       return set_err(JXG_ERR_INVALID_ARG, "image %u: row_bytes too small", i);
     if (im.params.fused != fused)
       return set_err(JXG_ERR_INVALID_ARG, "image %u: every image of a batch shares params.fused", i);
+    if (im.params.xyb_order != images[0].params.xyb_order ||
+        memcmp(im.params.xyb_fwd, images[0].params.xyb_fwd, sizeof(im.params.xyb_fwd)) != 0)
+      return set_err(JXG_ERR_INVALID_ARG,
+                     "image %u: every image of a batch shares params.xyb_order and xyb_fwd", i);
+    if (im.params.xyb_order != JXG_XYB_SCALAR && im.params.xyb_order != JXG_XYB_AVX2)
+      return set_err(JXG_ERR_INVALID_ARG, "image %u: xyb_order %d", i, im.params.xyb_order);
     if (im.params.flags & JXG_FLAG_CHROMA_HF)
       return set_err(JXG_ERR_INVALID_ARG, "image %u: chroma-HF protection is single-image only", i);
   }
@@ -136,6 +143,7 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
   b->ctx = ctx;
   b->n = (int)n;
   b->fused = fused ? 1 : 0;
+  b->xyb_order = images[0].params.xyb_order;
   b->images.assign(images, images + n);
   b->order.resize(n);
   std::iota(b->order.begin(), b->order.end(), 0);
@@ -185,17 +193,9 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
       hipMemset(b->d_counters, 0, 8 * sizeof(uint32_t)) != hipSuccess)
     return fail(set_err(JXG_ERR_DEVICE, "batch upload failed"));
   {
-    // XybMatrix::SPEC (src/xyb.rs:35-47,81-96) and the sRGB transfer table
+    // the batch's XybMatrix::fwd and the sRGB transfer table
     float mlut[265];
-    const float M00 = 0.30f, M02 = 0.078f, M01 = 1.0f - M02 - M00;
-    const float M10 = 0.23f, M12 = 0.078f, M11 = 1.0f - M12 - M10;
-    const float B_BIAS = 0.55180986f;
-    const float B_R_RATIO = 0.24342269f / (0.24342269f + 0.20476745f);
-    const float M20 = B_R_RATIO * (1.0f - B_BIAS);
-    const float M21 = (1.0f - B_R_RATIO) * (1.0f - B_BIAS);
-    const float M22 = 1.0f - M20 - M21;
-    const float fwd[9] = {M00, M01, M02, M10, M11, M12, M20, M21, M22};
-    memcpy(mlut, fwd, sizeof(fwd));
+    memcpy(mlut, images[0].params.xyb_fwd, 9 * sizeof(float));
     jxg_srgb_lut(mlut + 9);
     if (hipMemcpy(b->d_mlut, mlut, sizeof(mlut), hipMemcpyHostToDevice) != hipSuccess)
       return fail(set_err(JXG_ERR_DEVICE, "batch upload failed"));
@@ -233,6 +233,10 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
       d.dc[c] = im.d_dc[c];
     }
     d.D = b->d_der + i;
+    {
+      const int trc = tables_for(ctx, &im.params, &d.tables, &d.selq);
+      if (trc) return fail(trc);
+    }
     d.distance = im.params.distance;
     d.inv_scale = der[i].inv_scale;
     d.cfl_base_x = im.params.cfl_base_x;

@@ -181,11 +181,11 @@ __global__ __launch_bounds__(64) void k_cmap_solve(const float *__restrict__ par
 // autotuner chooses.
 template <bool F, int ROWS>
 __global__ __launch_bounds__(CM_T) void k_cmap_blocks_b(const BatchImg *__restrict__ img,
-                                                        const uint32_t *__restrict__ pfx, int n,
-                                                        const float *__restrict__ qm_x,
-                                                        const float *__restrict__ qm_b) {
+                                                        const uint32_t *__restrict__ pfx, int n) {
   const int i = batch_image(pfx, n, blockIdx.x);
   const BatchImg &I = img[i];
+  const float *qm_x = I.tables + 3 * 64;  // DCT8 inv_matrix(0); the DCT8 kind is at offset 0
+  const float *qm_b = I.tables + 5 * 64;  // inv_matrix(2)
   const uint32_t l = blockIdx.x - pfx[i];
   const int gw = I.gw[BS_CMAPB];
   const CmapArgs a{I.distance, I.cfl_base_x, I.cfl_base_b, I.cfl_color_factor};
@@ -207,28 +207,24 @@ __global__ __launch_bounds__(64) void k_cmap_solve_b(const BatchImg *__restrict_
 
 template <bool F>
 static void launch_cmap(hipStream_t st, const BatchImg *d_img, const uint32_t *d_pfx, int n,
-                        const uint32_t total[BS_N], const float *qm_x, const float *qm_b,
-                        int cm_rows) {
+                        const uint32_t total[BS_N], int cm_rows) {
   const int m = n + 1;
   if (cm_rows == 32)
-    k_cmap_blocks_b<F, 32><<<total[BS_CMAPB], CM_T, 0, st>>>(d_img, d_pfx + BS_CMAPB * m, n, qm_x,
-                                                             qm_b);
+    k_cmap_blocks_b<F, 32><<<total[BS_CMAPB], CM_T, 0, st>>>(d_img, d_pfx + BS_CMAPB * m, n);
   else
-    k_cmap_blocks_b<F, 16><<<total[BS_CMAPB], CM_T, 0, st>>>(d_img, d_pfx + BS_CMAPB * m, n, qm_x,
-                                                             qm_b);
+    k_cmap_blocks_b<F, 16><<<total[BS_CMAPB], CM_T, 0, st>>>(d_img, d_pfx + BS_CMAPB * m, n);
   k_cmap_solve_b<F><<<total[BS_CMAPS], 64, 0, st>>>(d_img, d_pfx + BS_CMAPS * m, n);
 }
 
 JXG_HIDDEN int batch_cmap(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
                           const uint32_t *d_pfx, int n, const uint32_t total[BS_N], int fused,
                           int cm_rows) {
-  const uint32_t off = jxg_qt_offset(JXG_QT_DCT8);
-  const float *qm_x = ctx->d_tables + off + 3 * 64;
-  const float *qm_b = ctx->d_tables + off + 5 * 64;
+  (void)ctx;
+  static_assert(JXG_QT_DCT8 == 0, "the batch CfL stage reads the DCT8 kind at blob offset 0");
   if (fused)
-    launch_cmap<true>(st, d_img, d_pfx, n, total, qm_x, qm_b, cm_rows);
+    launch_cmap<true>(st, d_img, d_pfx, n, total, cm_rows);
   else
-    launch_cmap<false>(st, d_img, d_pfx, n, total, qm_x, qm_b, cm_rows);
+    launch_cmap<false>(st, d_img, d_pfx, n, total, cm_rows);
   LAUNCH_CHECK("batch cmap");
   return JXG_OK;
 }
@@ -240,9 +236,11 @@ JXG_HIDDEN int fill_cmap(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint
   const size_t need = (size_t)tiles_x * tiles_y * 64 * 4;
   int rc = ensure(&ctx->d_tmp, &ctx->tmp_cap, need);
   if (rc) return rc;
+  const float *tables, *selq;
+  if ((rc = tables_for(ctx, p, &tables, &selq))) return rc;
   const uint32_t off = jxg_qt_offset(JXG_QT_DCT8);
-  const float *qm_x = ctx->d_tables + off + 3 * 64;  // inv_matrix(0) of DCT8
-  const float *qm_b = ctx->d_tables + off + 5 * 64;  // inv_matrix(2)
+  const float *qm_x = tables + off + 3 * 64;  // inv_matrix(0) of DCT8
+  const float *qm_b = tables + off + 5 * 64;  // inv_matrix(2)
   const CmapArgs a{p->distance, p->cfl_base_x, p->cfl_base_b, p->cfl_color_factor};
   const dim3 grid(tiles_x, 4 * tiles_y);
   const unsigned sgrid = (unsigned)(tiles_x * tiles_y);

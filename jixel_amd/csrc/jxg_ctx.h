@@ -38,8 +38,15 @@ struct jxg_ctx {
   // before the strategy search on `stream`.
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // Quant tables: the caller's blob (jxg_ctx_set_tables; in use while
+  // tables_set) in d_tables / d_selq, and the default tables
+  // (jxg_default_quant_tables), one set per (distance tier, fmla flavour) in
+  // slot 2 * tier + fused, each built and uploaded on first use.
   float *d_tables = nullptr;
   bool tables_set = false;
+  float *d_tier_tables = nullptr;  // 2 * JXG_QUANT_TIERS blobs
+  float *d_tier_selq = nullptr;    // their scan-ordered inverse weights
+  bool tier_valid[2 * JXG_QUANT_TIERS] = {};
   jxg_derived *d_derived = nullptr;
   jxg_derived *h_derived = nullptr;  // pinned staging
   bool derived_valid = false;
@@ -98,6 +105,11 @@ inline int ensure(T **p, size_t *cap, size_t n) {
 JXG_HIDDEN int check_image(uint32_t w, uint32_t h);
 JXG_HIDDEN int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t padded_h);
 JXG_HIDDEN int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p);
+// The quant tables (blob and scan-ordered inverse weights) the stages use for
+// a frame with parameters p: the caller's set if one was given, else the
+// default set of p's distance tier, built on first use (a synchronous upload).
+JXG_HIDDEN int tables_for(jxg_ctx *ctx, const jxg_frame_params *p, const float **tables,
+                          const float **selq);
 JXG_HIDDEN int fill_cmap(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
                          uint32_t width, uint32_t height, const jxg_frame_params *p,
                          int8_t *d_ytox, int8_t *d_ytob, hipStream_t st);  // jxg_cfl.hip
@@ -113,7 +125,7 @@ JXG_HIDDEN int alpha_finish(jxg_ctx *ctx, uint32_t W, uint32_t H, const uint8_t 
 struct BatchImg;
 JXG_HIDDEN int batch_stages_pre(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
                                 const uint32_t *d_pfx, int n, const uint32_t total[],
-                                int fused, const float *mlut, int aq_strip);
+                                int fused, const float *mlut, int aq_strip, int xyb_order);
 JXG_HIDDEN int batch_cmap(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
                           const uint32_t *d_pfx, int n, const uint32_t total[], int fused,
                           int cm_rows);

@@ -44,6 +44,12 @@ extern "C" {
 #endif
 // Builds the derived constants; returns JXG_OK or an error code.
 int jxg_derive(const jxg_frame_params *p, jxg_derived *out);
+// Distance tier (0..4) of the default quant tables (jxg_tables.cpp).
+int jxg_quant_tier(float distance, int arbiter);
+#define JXG_QUANT_TIERS 5
+// The format's f16 round trip (src/util.rs:427-518), jxg_params.cpp.
+uint16_t jxg_f32_to_f16(float v);
+float jxg_f16_to_f32(uint16_t b);
 #ifdef __cplusplus
 }
 #endif

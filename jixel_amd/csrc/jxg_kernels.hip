@@ -1993,13 +1993,13 @@ __global__ __launch_bounds__(256) void k_quantize_xt(QArgs A) {
   const int bx_ = (int)(bl_ % (uint32_t)I.gw[stage]);        \
   const int by_ = (int)(bl_ / (uint32_t)I.gw[stage]);
 
-template <bool F>
+template <bool F, int ORDER>
 __global__ __launch_bounds__(256) void k_xyb_b(const BatchImg *__restrict__ img,
                                                const uint32_t *__restrict__ pfx, int n,
                                                const float *__restrict__ mlut) {
   BATCH_BLOCK(BS_XYB)
-  xyb_body<F, JXG_XYB_SCALAR>(I.rgb, (size_t)I.row_bytes, I.ch, I.W, I.H, mlut, I.xyb, I.stride,
-                              I.ph, bx_, by_);
+  xyb_body<F, ORDER>(I.rgb, (size_t)I.row_bytes, I.ch, I.W, I.H, mlut, I.xyb, I.stride, I.ph,
+                     bx_, by_);
 }
 
 template <bool F>
@@ -2024,16 +2024,13 @@ __global__ __launch_bounds__(STRIP * 8) void k_aq_block_b(const BatchImg *__rest
 template <bool F>
 __global__ __launch_bounds__(SEL_T, 5) void k_select_b(const BatchImg *__restrict__ img,
                                                     const uint32_t *__restrict__ pfx, int n,
-                                                    const float *__restrict__ tables,
-                                                    const uint32_t *__restrict__ scans,
-                                                    const float *__restrict__ selq) {
+                                                    const uint32_t *__restrict__ scans) {
   BATCH_BLOCK(BS_SEL)
-  select_body<F>(I.xyb, I.stride, I.ph, I.W, I.H, I.qf, I.ytox, I.ytob, tables, scans, selq, I.D,
+  select_body<F>(I.xyb, I.stride, I.ph, I.W, I.H, I.qf, I.ytox, I.ytob, I.tables, scans, I.selq, I.D,
                  I.cells, nullptr, nullptr, nullptr, I.xb, I.yb, bx_, by_);
 }
 
 struct BatchQueue {
-  const float *tables;
   uint32_t *xt_count, *xt_list, *xt_img, *status, *bad;
   uint32_t xt_cap;
 };
@@ -2051,7 +2048,7 @@ __device__ __forceinline__ QArgs batch_qargs(const BatchImg &I, const BatchQueue
   A.qf = I.qf;
   A.ytox = I.ytox;
   A.ytob = I.ytob;
-  A.tables = Q.tables;
+  A.tables = I.tables;
   A.D = I.D;
   A.lv0 = I.lv[0];
   A.lv1 = I.lv[1];
@@ -2133,9 +2130,12 @@ __global__ __launch_bounds__(256) void k_quantize_xt_b(const BatchImg *__restric
 template <bool F>
 static void launch_pre(hipStream_t st, const BatchImg *d_img, const uint32_t *p, int n,
                        const uint32_t total[BS_N], const float *mlut, const float *exp2,
-                       int aq_strip) {
+                       int aq_strip, int order) {
   const int m = n + 1;
-  k_xyb_b<F><<<total[BS_XYB], 256, 0, st>>>(d_img, p + BS_XYB * m, n, mlut);
+  if (order == JXG_XYB_AVX2)
+    k_xyb_b<F, JXG_XYB_AVX2><<<total[BS_XYB], 256, 0, st>>>(d_img, p + BS_XYB * m, n, mlut);
+  else
+    k_xyb_b<F, JXG_XYB_SCALAR><<<total[BS_XYB], 256, 0, st>>>(d_img, p + BS_XYB * m, n, mlut);
   k_aq_pre_b<F><<<total[BS_AQPRE], 256, 0, st>>>(d_img, p + BS_AQPRE * m, n);
   if (aq_strip == 16)
     k_aq_block_b<F, 16><<<total[BS_AQBLK], 128, 0, st>>>(d_img, p + BS_AQBLK * m, n, exp2);
@@ -2147,11 +2147,11 @@ static void launch_pre(hipStream_t st, const BatchImg *d_img, const uint32_t *p,
 
 JXG_HIDDEN int batch_stages_pre(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
                                 const uint32_t *d_pfx, int n, const uint32_t total[BS_N],
-                                int fused, const float *mlut, int aq_strip) {
+                                int fused, const float *mlut, int aq_strip, int xyb_order) {
   if (fused)
-    launch_pre<true>(st, d_img, d_pfx, n, total, mlut, ctx->d_exp2, aq_strip);
+    launch_pre<true>(st, d_img, d_pfx, n, total, mlut, ctx->d_exp2, aq_strip, xyb_order);
   else
-    launch_pre<false>(st, d_img, d_pfx, n, total, mlut, ctx->d_exp2, aq_strip);
+    launch_pre<false>(st, d_img, d_pfx, n, total, mlut, ctx->d_exp2, aq_strip, xyb_order);
   LAUNCH_CHECK("batch xyb / aq");
   return JXG_OK;
 }
@@ -2163,7 +2163,6 @@ JXG_HIDDEN int batch_stages_post(jxg_ctx *ctx, hipStream_t st, const BatchImg *d
   const uint32_t *p = d_pfx;
   const int m = n + 1;
   BatchQueue Q;
-  Q.tables = ctx->d_tables;
   Q.xt_count = counters;
   Q.bad = counters + 1;
   Q.status = counters + 4;
@@ -2171,13 +2170,11 @@ JXG_HIDDEN int batch_stages_post(jxg_ctx *ctx, hipStream_t st, const BatchImg *d
   Q.xt_img = xt_img;
   Q.xt_cap = xt_cap;
   if (fused) {
-    k_select_b<true><<<total[BS_SEL], SEL_T, 0, st>>>(d_img, p + BS_SEL * m, n, ctx->d_tables,
-                                                      ctx->d_scans, ctx->d_selq);
+    k_select_b<true><<<total[BS_SEL], SEL_T, 0, st>>>(d_img, p + BS_SEL * m, n, ctx->d_scans);
     k_quantize_b<true><<<total[BS_QUANT], 256, 0, st>>>(d_img, p + BS_QUANT * m, n, Q);
     k_quantize_xt_b<true><<<xt_grid, 256, 0, st>>>(d_img, Q);
   } else {
-    k_select_b<false><<<total[BS_SEL], SEL_T, 0, st>>>(d_img, p + BS_SEL * m, n, ctx->d_tables,
-                                                       ctx->d_scans, ctx->d_selq);
+    k_select_b<false><<<total[BS_SEL], SEL_T, 0, st>>>(d_img, p + BS_SEL * m, n, ctx->d_scans);
     k_quantize_b<false><<<total[BS_QUANT], 256, 0, st>>>(d_img, p + BS_QUANT * m, n, Q);
     k_quantize_xt_b<false><<<xt_grid, 256, 0, st>>>(d_img, Q);
   }
@@ -2452,8 +2449,8 @@ JXG_HIDDEN int fill_u8(jxg_ctx *ctx, uint8_t *p, size_t n, uint8_t v) {
 
 // Uploads a quant-table blob and the scan-ordered copy of the DCT8 /
 // DCT16X16 / DCT32X32 inverse weights the strategy search reads.
-static int upload_tables(jxg_ctx *ctx, const float *blob) {
-  HIP_TRY(hipMemcpy(ctx->d_tables, blob, JXG_QT_BLOB_FLOATS * sizeof(float), hipMemcpyHostToDevice));
+static int upload_tables(jxg_ctx *ctx, const float *blob, float *d_blob, float *d_selq) {
+  HIP_TRY(hipMemcpy(d_blob, blob, JXG_QT_BLOB_FLOATS * sizeof(float), hipMemcpyHostToDevice));
   static const int kinds[3] = {JXG_QT_DCT8, JXG_QT_DCT16X16, JXG_QT_DCT32X32};
   static const int offs[3] = {0, 64, 320};
   float selq[3 * SELQ_N];
@@ -2463,7 +2460,34 @@ static int upload_tables(jxg_ctx *ctx, const float *blob) {
       for (uint32_t p = 0; p < n; p++)
         selq[c * SELQ_N + offs[i] + p] = blob[off + (3 + c) * n + ctx->h_order[offs[i] + p]];
   }
-  HIP_TRY(hipMemcpy(ctx->d_selq, selq, sizeof(selq), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_selq, selq, sizeof(selq), hipMemcpyHostToDevice));
+  return JXG_OK;
+}
+
+JXG_HIDDEN int tables_for(jxg_ctx *ctx, const jxg_frame_params *p, const float **tables,
+                          const float **selq) {
+  if (ctx->tables_set) {
+    *tables = ctx->d_tables;
+    *selq = ctx->d_selq;
+    return JXG_OK;
+  }
+  if (!p) return set_err(JXG_ERR_INVALID_ARG, "null params");
+  const int slot = 2 * jxg_quant_tier(p->distance, p->arbiter) + (p->fused ? 1 : 0);
+  float *d_blob = ctx->d_tier_tables + (size_t)slot * JXG_QT_BLOB_FLOATS;
+  float *d_sq = ctx->d_tier_selq + (size_t)slot * 3 * SELQ_N;
+  if (!ctx->tier_valid[slot]) {
+    float *blob = new float[JXG_QT_BLOB_FLOATS];
+    int rc = jxg_default_quant_tables(p->distance, p->arbiter, p->fused, blob);
+    if (rc == JXG_OK) rc = upload_tables(ctx, blob, d_blob, d_sq);
+    delete[] blob;
+    if (rc != JXG_OK) return rc == JXG_ERR_INVALID_DISTANCE || rc == JXG_ERR_INVALID_ARG
+                                 ? set_err(rc, "invalid table parameters (distance %g, arbiter %d)",
+                                           (double)p->distance, p->arbiter)
+                                 : rc;
+    ctx->tier_valid[slot] = true;
+  }
+  *tables = d_blob;
+  *selq = d_sq;
   return JXG_OK;
 }
 
@@ -2574,6 +2598,10 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
     return rc;
   };
   if (hipMalloc((void **)&ctx->d_tables, JXG_QT_BLOB_FLOATS * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_tier_tables,
+                (size_t)2 * JXG_QUANT_TIERS * JXG_QT_BLOB_FLOATS * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_tier_selq,
+                (size_t)2 * JXG_QUANT_TIERS * 3 * SELQ_N * sizeof(float)) != hipSuccess ||
       hipMalloc((void **)&ctx->d_derived, sizeof(jxg_derived)) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_derived, sizeof(jxg_derived), hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void **)&ctx->d_scans, (64 + 256 + 1024) * sizeof(uint32_t)) != hipSuccess ||
@@ -2636,11 +2664,23 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
     if (hipMemcpy(ctx->d_misc, zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
       return fail(set_err(JXG_ERR_DEVICE, "counter initialisation failed"));
   }
-  float *blob = new float[JXG_QT_BLOB_FLOATS];
-  jxg_default_quant_tables(blob);
-  const int urc = upload_tables(ctx, blob);
-  delete[] blob;
-  if (urc != JXG_OK) return fail(urc);
+  {
+    // The default tables of every distance tier and fmla flavour, built now
+    // so that no later stage call uploads (one representative distance per
+    // tier of jxg_quant_tier).
+    static const float tier_distance[JXG_QUANT_TIERS] = {0.1f, 1.0f, 2.0f, 3.0f, 4.0f};
+    for (int tier = 0; tier < JXG_QUANT_TIERS; tier++)
+      for (int f = 0; f < 2; f++) {
+        jxg_frame_params p;
+        memset(&p, 0, sizeof(p));
+        p.distance = tier_distance[tier];
+        p.arbiter = JXG_ARBITER_SSIMULACRA2;
+        p.fused = f;
+        const float *t, *q;
+        const int trc = tables_for(ctx, &p, &t, &q);
+        if (trc) return fail(trc);
+      }
+  }
   *out = ctx;
   return JXG_OK;
 }
@@ -2657,6 +2697,8 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
   if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   (void)hipFree(ctx->d_tables);
+  (void)hipFree(ctx->d_tier_tables);
+  (void)hipFree(ctx->d_tier_selq);
   (void)hipFree(ctx->d_derived);
   (void)hipHostFree(ctx->h_derived);
   (void)hipFree(ctx->d_scans);
@@ -2691,10 +2733,17 @@ int jxg_ctx_synchronize(jxg_ctx *ctx) {
 }
 
 int jxg_ctx_set_tables(jxg_ctx *ctx, const float *blob) {
-  if (!ctx || !blob) return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  if (!ctx) return set_err(JXG_ERR_INVALID_ARG, "null argument");
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamSynchronize(ctx->stream));  // kernels in flight may read the tables
-  return upload_tables(ctx, blob);
+  if (!blob) {
+    ctx->tables_set = false;
+    return JXG_OK;
+  }
+  ctx->tables_set = false;
+  const int rc = upload_tables(ctx, blob, ctx->d_tables, ctx->d_selq);
+  if (rc == JXG_OK) ctx->tables_set = true;
+  return rc;
 }
 
 int jxg_xyb_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channels,
@@ -2767,12 +2816,14 @@ static int select_impl(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32
   int rc = upload_derived(ctx, p);
   if (rc) return rc;
   if ((rc = zero_cmaps(ctx, W, H, &ytox, &ytob))) return rc;
+  const float *tables, *selq;
+  if ((rc = tables_for(ctx, p, &tables, &selq))) return rc;
   const int xb = (W + 7) / 8, yb = (H + 7) / 8;
   const dim3 grid((xb + 3) / 4, (yb + 3) / 4);
   if (p->fused)
-    k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+    k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
   else
-    k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, ctx->d_tables, ctx->d_scans, ctx->d_selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+    k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
   LAUNCH_CHECK("k_select");
   return JXG_OK;
 }
@@ -2824,6 +2875,8 @@ int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, 
   if ((rc = check_planes(width, height, stride, padded_h))) return rc;
   if ((rc = upload_derived(ctx, params))) return rc;
   if ((rc = zero_cmaps(ctx, width, height, &d_ytox, &d_ytob))) return rc;
+  const float *tables, *selq;
+  if ((rc = tables_for(ctx, params, &tables, &selq))) return rc;
   const int xb = (width + 7) / 8, yb = (height + 7) / 8;
   // At most one crossing transform per 4 blocks (the smallest multi-block
   // transform covers two blocks, and a crossing one at least two).
@@ -2841,7 +2894,7 @@ int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, 
   A.qf = d_qf;
   A.ytox = d_ytox;
   A.ytob = d_ytob;
-  A.tables = ctx->d_tables;
+  A.tables = tables;
   A.D = ctx->d_derived;
   A.lv0 = d_levels[0];
   A.lv1 = d_levels[1];
@@ -2890,6 +2943,8 @@ int jxg_strategy_cost_sites_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t strid
   HIP_TRY(hipSetDevice(ctx->device));
   if ((rc = upload_derived(ctx, params))) return rc;
   if ((rc = zero_cmaps(ctx, width, height, &d_ytox, &d_ytob))) return rc;
+  const float *tables, *selq;
+  if ((rc = tables_for(ctx, params, &tables, &selq))) return rc;
   SiteArgs S;
   S.xyb = d_xyb;
   S.stride = (int)stride;
@@ -2900,7 +2955,7 @@ int jxg_strategy_cost_sites_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t strid
   S.tiles_y = (int)((height + 63) / 64);
   S.ytox = d_ytox;
   S.ytob = d_ytob;
-  S.tables = ctx->d_tables;
+  S.tables = tables;
   S.D = ctx->d_derived;
   S.scanpos = ctx->d_scanpos;
   S.s = strategy;
@@ -2941,19 +2996,10 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
     if (!d_ytox) d_ytox = ctx->d_cmap;
     if (!d_ytob) d_ytob = ctx->d_cmap + tiles;
   }
-  // XybMatrix::SPEC (src/xyb.rs:35-47,81-96)
-  const float M00 = 0.30f, M02 = 0.078f, M01 = 1.0f - M02 - M00;
-  const float M10 = 0.23f, M12 = 0.078f, M11 = 1.0f - M12 - M10;
-  const float B_BIAS = 0.55180986f;
-  const float B_R_RATIO = 0.24342269f / (0.24342269f + 0.20476745f);
-  const float M20 = B_R_RATIO * (1.0f - B_BIAS);
-  const float M21 = (1.0f - B_R_RATIO) * (1.0f - B_BIAS);
-  const float M22 = 1.0f - M20 - M21;
-  const float fwd[9] = {M00, M01, M02, M10, M11, M12, M20, M21, M22};
   float lut[256];
   jxg_srgb_lut(lut);
-  if ((rc = jxg_xyb_dev(ctx, d_rgb, row_bytes, channels, width, height, fwd, lut, JXG_XYB_SCALAR,
-                        params->fused, ctx->d_xyb, stride, ph)))
+  if ((rc = jxg_xyb_dev(ctx, d_rgb, row_bytes, channels, width, height, params->xyb_fwd, lut,
+                        params->xyb_order, params->fused, ctx->d_xyb, stride, ph)))
     return rc;
   // Alpha of gray + alpha and RGBA input: fringe clearing, fill_invisible on
   // XYB and the alpha block factors (src/frame/encoder.rs:60-69,152-157).

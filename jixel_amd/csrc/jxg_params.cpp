@@ -1,6 +1,7 @@
 // jxg_params.cpp - host-side parameters of the front end: quality mapping,
 // distance parameters, transfer LUT, quant-matrix blob layout and defaults,
-// and the per-frame derived constants the kernels read.
+// and the per-frame derived constants the kernels read.  The default quant
+// tables are built in jxg_tables.cpp.
 //
 // Compiled with -ffp-contract=off: every a*b+c stays unfused unless written
 // with fma(), matching the reference's Rust float semantics; `fused`
@@ -168,80 +169,12 @@ void ac_thresholds(int c, int area, float d, float qm_mult, float thr[4]) {
   for (int i = 0; i < 4; i++) thr[i] = hq[i] + t * (normal[i] - hq[i]);
 }
 
-// ---- quant-matrix defaults (src/quant_weights.rs:1002-1230,1372-1380) ----
-// JPEG XL library band parameters (X, Y, B rows): first entry is the DC
-// weight, the rest are the multiplicative steps between radial bands.
-const float kDct8Bands[3][6] = {{3150.0f, 0.0f, -0.4f, -0.4f, -0.4f, -2.0f},
-                                {560.0f, 0.0f, -0.3f, -0.3f, -0.3f, -0.3f},
-                                {512.0f, -2.0f, -1.0f, 0.0f, -1.0f, -2.0f}};
-const float kDct16x16Bands[3][7] = {
-    {8996.872571181412f, -1.3000777393353804f, -0.49424529824571225f, -0.43909377445710344f,
-     -0.6350101832695744f, -0.9017726405082761f, -1.6162099239887414f},
-    {3191.4836629684423f, -0.67424582104194355f, -0.80745813428471f, -0.4492583748484344f,
-     -0.35865440981033403f, -0.31322389111877305f, -0.37615025315725483f},
-    {1157.504081454872f, -2.0531423165804414f, -1.4f, -0.506871300333784f,
-     -0.42708730624733904f, -1.4856834539296244f, -4.92091428844016f}};
-const float kDct32x32Bands[3][8] = {
-    {15718.40830982519f, -1.025f, -0.98f, -0.9012f, -0.4f, -0.48819395f, -0.421064f, -0.27f},
-    {7305.763681069598f, -0.80419582f, -0.76330365f, -0.5566038f, -0.49785305f, -0.43699593f,
-     -0.40180867f, -0.27321683f},
-    {3803.53173721215f, -3.0607336f, -2.041327f, -2.023565f, -0.54953895f, -0.4f, -0.4f, -0.3f}};
-const float kDct16x8Bands[3][7] = {{7240.7734393502f, -0.7f, -0.7f, -0.2f, -0.2f, -0.2f, -0.5f},
-                                   {1448.15468787004f, -0.5f, -0.5f, -0.5f, -0.2f, -0.2f, -0.2f},
-                                   {506.854002029f, -1.4f, -0.2f, -0.5f, -0.5f, -1.5f, -3.6f}};
-const float kDct16x32Bands[3][8] = {
-    {13844.97076442300573f, -0.97113799999999995f, -0.658f, -0.42026f, -0.22712f, -0.2206f,
-     -0.226f, -0.6f},
-    {4798.964084220744293f, -0.61125308982767057f, -0.83770786552491361f, -0.79014862079498627f,
-     -0.2692727459704829f, -0.38272769465388551f, -0.22924222653091453f, -0.20719098826199578f},
-    {1807.236946760964614f, -1.2f, -1.2f, -0.7f, -0.7f, -0.7f, -0.4f, -0.5f}};
-const float kDct64x64Bands[3][8] = {
-    {0.9f * 26629.073922049845f, -1.025f, -0.78f, -0.65012f, -0.19041574084286472f,
-     -0.20819395464f, -0.421064f, -0.3273384553584867f},
-    {0.9f * 9311.323871001005f, -0.3041958212306401f, -0.3633036457487539f,
-     -0.35660379990111464f, -0.3443074455424403f, -0.33699592683512467f, -0.3018086652624211f,
-     -0.27321683125358037f},
-    {0.9f * 4992.248644553863f, -1.2f, -1.2f, -0.8f, -0.7f, -0.7f, -0.4f, -0.5f}};
-const float kDct32x64Bands[3][8] = {
-    {0.65f * 23629.073922049845f, -1.025f, -0.78f, -0.65012f, -0.19041574084286472f,
-     -0.20819395464f, -0.421064f, -0.3273384553584867f},
-    {0.65f * 8611.323871001005f, -0.3041958212306401f, -0.3633036457487539f,
-     -0.35660379990111464f, -0.3443074455424403f, -0.33699592683512467f, -0.3018086652624211f,
-     -0.27321683125358037f},
-    {0.65f * 4492.248644553863f, -1.2f, -1.2f, -0.8f, -0.7f, -0.7f, -0.4f, -0.5f}};
-
-float band_mult(float v) { return v > 0.0f ? 1.0f + v : 1.0f / (1.0f - v); }
-
-float interpolate_bands(float pos, const float *bands) {
-  float idx_f = floorf(pos);
-  float frac = pos - idx_f;
-  int idx = (int)idx_f;
-  float a = bands[idx], b = bands[idx + 1];
-  return powf(b / a, frac) * a;
-}
-
-// GetQuantWeights-style radial band matrix (compute_band_matrix,
-// src/quant_weights.rs:1226-1248): rows x cols step sizes of one channel.
-void band_matrix(bool fused, const float *src, int nb, int rows, int cols, float *out) {
-  float bands[16];
-  bands[0] = src[0];
-  for (int i = 1; i < nb; i++) bands[i] = bands[i - 1] * band_mult(src[i]);
-  float scale = ((float)nb - 1.0f) / (1.41421356237309504880f + 1e-6f);
-  float rcprow = scale / (float)(rows - 1);
-  float rcpcol = scale / (float)(cols - 1);
-  for (int y = 0; y < rows; y++) {
-    float dy = (float)y * rcprow;
-    float dy2 = dy * dy;
-    for (int x = 0; x < cols; x++) {
-      float dx = (float)x * rcpcol;
-      out[y * cols + x] = 1.0f / interpolate_bands(sqrtf(fmla(fused, dx, dx, dy2)), bands);
-    }
-  }
-}
-
 }  // namespace
 
 extern "C" {
+
+uint16_t jxg_f32_to_f16(float v) { return f32_to_f16(v); }
+float jxg_f16_to_f32(uint16_t b) { return f16_to_f32(b); }
 
 uint32_t jxg_qt_size(int kind) {
   return (kind >= 0 && kind < JXG_QT_NUM_KINDS) ? kQtSizes[kind] : 0;
@@ -351,7 +284,23 @@ int jxg_frame_params_init(float distance, int speed, int fused, jxg_frame_params
   out->channel_weight[0] = 0.30f;
   out->channel_weight[1] = 1.0f;
   out->channel_weight[2] = 0.28f;
+  out->arbiter = JXG_ARBITER_SSIMULACRA2;  // EncodeConfig::default (src/encode_image.rs:717)
+  // XybMatrix::SPEC (src/xyb.rs:35-47,81-96), scalar kernel order
+  out->xyb_order = JXG_XYB_SCALAR;
+  jxg_spec_xyb_matrix(out->xyb_fwd);
   return JXG_OK;
+}
+
+void jxg_spec_xyb_matrix(float fwd[9]) {
+  const float M00 = 0.30f, M02 = 0.078f, M01 = 1.0f - M02 - M00;
+  const float M10 = 0.23f, M12 = 0.078f, M11 = 1.0f - M12 - M10;
+  const float B_BIAS = 0.55180986f;
+  const float B_R_RATIO = 0.24342269f / (0.24342269f + 0.20476745f);
+  const float M20 = B_R_RATIO * (1.0f - B_BIAS);
+  const float M21 = (1.0f - B_R_RATIO) * (1.0f - B_BIAS);
+  const float M22 = 1.0f - M20 - M21;
+  const float m[9] = {M00, M01, M02, M10, M11, M12, M20, M21, M22};
+  memcpy(fwd, m, sizeof(m));
 }
 
 void jxg_srgb_lut(float out[256]) {
@@ -359,36 +308,6 @@ void jxg_srgb_lut(float out[256]) {
     double v = (double)i * (1.0 / 255.0);
     out[i] = (float)(v <= 0.04045 ? v / 12.92 : pow((v + 0.055) / 1.055, 2.4));
   }
-}
-
-int jxg_default_quant_tables(float *blob) {
-  if (!blob) return JXG_ERR_INVALID_ARG;
-  memset(blob, 0, JXG_QT_BLOB_FLOATS * sizeof(float));
-  const bool fused = true;
-  auto put = [&](int kind, int rows, int cols, const float *bands, int nb, int stride) {
-    uint32_t n = kQtSizes[kind], off = jxg_qt_offset(kind);
-    for (int c = 0; c < 3; c++) {
-      float *m = blob + off + c * n;
-      band_matrix(fused, bands + c * stride, nb, rows, cols, m);
-      float *inv = blob + off + (3 + c) * n;
-      inv[0] = 0.0f;
-      for (uint32_t i = 1; i < n; i++) inv[i] = 1.0f / m[i];
-    }
-  };
-  put(JXG_QT_DCT8, 8, 8, &kDct8Bands[0][0], 6, 6);
-  put(JXG_QT_DCT16X8, 8, 16, &kDct16x8Bands[0][0], 7, 7);
-  put(JXG_QT_DCT16X16, 16, 16, &kDct16x16Bands[0][0], 7, 7);
-  put(JXG_QT_DCT32X32, 32, 32, &kDct32x32Bands[0][0], 8, 8);
-  put(JXG_QT_DCT32X16, 16, 32, &kDct16x32Bands[0][0], 8, 8);
-  put(JXG_QT_DCT8X32, 8, 32, &kDct16x8Bands[0][0], 7, 7);
-  put(JXG_QT_DCT64X64, 64, 64, &kDct64x64Bands[0][0], 8, 8);
-  put(JXG_QT_DCT64X32, 32, 64, &kDct32x64Bands[0][0], 8, 8);
-  // The sub-8 kinds default to the DCT8 table until their own tables are
-  // restated; callers pass their DequantMatrices for exact parity.
-  for (int kind : {JXG_QT_IDENTITY, JXG_QT_DCT2X2, JXG_QT_DCT4X4, JXG_QT_DCT4X8, JXG_QT_AFV})
-    memcpy(blob + jxg_qt_offset(kind), blob + jxg_qt_offset(JXG_QT_DCT8),
-           6 * 64 * sizeof(float));
-  return JXG_OK;
 }
 
 void jxg_plane_geometry(uint32_t width, uint32_t height, uint32_t *stride, uint32_t *padded_h) {

@@ -504,7 +504,11 @@ int jxg_qf_tuner_create(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint3
   A.yb = yb;
   A.cells = d_cells;
   A.tmap = t->d_tmap;
-  A.tables = ctx->d_tables;
+  {
+    const float *selq;
+    const int trc = tables_for(ctx, params, &A.tables, &selq);
+    if (trc) return fail(trc);
+  }
   A.D = t->d_der;
   A.cmat = t->d_cmat;
   A.coef = t->d_coef;

@@ -44,6 +44,16 @@ void jxo_distance_params(float distance, int32_t *global_scale, float *scale,
                          int32_t *quant_dc, float *scale_dc, uint32_t *x_qm_scale,
                          uint32_t *epf_iters, float dc_step[3]);
 
+/* ---- quant tables ---- */
+/* The stored library tables DEQUANT_MATRIX_8X8 (3 x 64) and
+ * DEQUANT_MATRIX_16X8 (3 x 128) (src/quant_weights.rs:246,455); must be set
+ * before jxo_fast_tables. */
+void jxo_set_static_tables(const float *m8x8, const float *m16x8);
+/* DequantMatrices::new_fast(distance, arbiter) (src/quant_weights.rs:2294)
+ * as a blob in the layout of include/jxg.h, with the current fmla flavour.
+ * Returns 0, or -1 when the static tables are not set. */
+int jxo_fast_tables(float distance, int arbiter, float *blob);
+
 /* ---- XYB ---- */
 /* One pixel, scalar order (src/xyb.rs:172) or AVX2 production order
  * (src/avx/xyb.rs:234, SCALAR_ORDER=false, FUSED=true). */

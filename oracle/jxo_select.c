@@ -674,19 +674,10 @@ static void *fe_worker(void *arg) {
   size_t plane = (size_t)W * H;
   float lut[256];
   jxo_srgb_lut(lut);
+  /* the frame's XybMatrix::fwd and XYB kernel flavour */
   float m[9];
-  /* XybMatrix::SPEC (src/xyb.rs:35-47,81-96) */
-  {
-    const float M00 = 0.30f, M02 = 0.078f, M01 = 1.0f - M02 - M00;
-    const float M10 = 0.23f, M12 = 0.078f, M11 = 1.0f - M12 - M10;
-    const float B_BIAS = 0.55180986f;
-    const float B_R_RATIO = 0.24342269f / (0.24342269f + 0.20476745f);
-    const float M20 = B_R_RATIO * (1.0f - B_BIAS);
-    const float M21 = (1.0f - B_R_RATIO) * (1.0f - B_BIAS);
-    const float M22 = 1.0f - M20 - M21;
-    float t[9] = {M00, M01, M02, M10, M11, M12, M20, M21, M22};
-    memcpy(m, t, sizeof(m));
-  }
+  memcpy(m, j->p->xyb_fwd, sizeof(m));
+  const int order = j->p->xyb_order;
   int i;
   while ((i = take(j)) >= 0) {
     if (j->kind == 0) {
@@ -696,9 +687,9 @@ static void *fe_worker(void *arg) {
           const uint8_t *px = row + (size_t)x * j->channels;
           float o[3];
           if (j->channels <= 2) { /* gray: linearize_gray, snapped after the alpha fill */
-            jxo_xyb_pixel(m, JXG_XYB_SCALAR, lut[px[0]], lut[px[0]], lut[px[0]], o);
+            jxo_xyb_pixel(m, order, lut[px[0]], lut[px[0]], lut[px[0]], o);
           } else {
-            jxo_xyb_pixel(m, JXG_XYB_SCALAR, lut[px[0]], lut[px[1]], lut[px[2]], o);
+            jxo_xyb_pixel(m, order, lut[px[0]], lut[px[1]], lut[px[2]], o);
           }
           size_t k = (size_t)y * W + x;
           j->xyb[k] = o[0];

@@ -40,6 +40,9 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 def _declare(L):
     c = ctypes
     L.jxo_set_fused.argtypes = [c.c_int]
+    L.jxo_get_fused.restype = c.c_int
+    L.jxo_set_static_tables.argtypes = [_f32p, _f32p]
+    L.jxo_fast_tables.argtypes = [c.c_float, c.c_int, _f32p]
     L.jxo_cbrtf.restype = c.c_float
     L.jxo_cbrtf.argtypes = [c.c_float]
     L.jxo_fast_exp2.restype = c.c_float
@@ -219,9 +222,52 @@ def strategy_cost(xyb, strategy, px, py, qac, cmap, tables, params, meta_r=4.0):
                                    ctypes.byref(params))
 
 
-def front_end(rgb, params, tables, threads=1):
+_GOLDEN = os.path.join(os.path.dirname(_HERE), "tests", "golden", "reference_tables.json")
+_static_set = False
+_table_cache = {}
+
+
+def fast_tables(distance, fused=True, arbiter=0):
+    """The oracle's own DequantMatrices::new_fast(distance, arbiter) blob
+    (src/quant_weights.rs:2294), built with the given fmla flavour.  The
+    stored library tables come from the reference-table fixture."""
+    global _static_set
+    key = (np.float32(distance).item(), bool(fused), int(arbiter))
+    if key in _table_cache:
+        return _table_cache[key]
+    L = lib()
+    if not _static_set:
+        import json
+        with open(_GOLDEN) as f:
+            g = json.load(f)
+        m8 = np.array(g["DEQUANT_MATRIX_8X8"], np.float32).reshape(-1)
+        m16 = np.array(g["DEQUANT_MATRIX_16X8"], np.float32).reshape(-1)
+        L.jxo_set_static_tables(_p(m8, _f32p), _p(m16, _f32p))
+        _static_set = True
+    blob = np.zeros(52224, np.float32)
+    old = L.jxo_get_fused()
+    L.jxo_set_fused(1 if fused else 0)
+    try:
+        rc = L.jxo_fast_tables(float(distance), int(arbiter), _p(blob, _f32p))
+    finally:
+        L.jxo_set_fused(old)
+    assert rc == 0
+    _table_cache[key] = blob
+    return blob
+
+
+def tables_for(params, tables=None):
+    """`tables`, or the oracle's default tables of the frame parameters."""
+    if tables is not None:
+        return tables
+    return fast_tables(params.distance, bool(params.fused), int(params.arbiter))
+
+
+def front_end(rgb, params, tables=None, threads=1):
     """Whole front end on the CPU.  rgb: (H, W, C) uint8.  Returns the quant
-    field, strategy cells, CfL maps, levels, coefficients and DC."""
+    field, strategy cells, CfL maps, levels, coefficients and DC.  `tables`
+    defaults to the oracle's new_fast tables of the frame's distance."""
+    tables = tables_for(params, tables)
     rgb = np.ascontiguousarray(rgb)
     h, w, ch = rgb.shape
     xb, yb = (w + 7) // 8, (h + 7) // 8

@@ -163,7 +163,7 @@ def test_front_end_alpha_bit_exact(J, fe, torch, oracle, w, h, ch, q, flags, fus
         img = np.ascontiguousarray(np.stack([img[:, :, 1], a], axis=2))
     p = J.frame_params(J.EncodeConfig(fused=fused).with_quality(q))
     p.flags = flags
-    want = oracle.front_end(img, p, J.default_quant_tables(), threads=4)
+    want = oracle.front_end(img, p, oracle.tables_for(p), threads=4)
     o = fe.run(to_dev(torch, img), w, h, p)
     got_alpha = fe.alpha_plane(w, h).cpu().numpy()
     assert np.array_equal(got_alpha, want["alpha"])
@@ -180,6 +180,6 @@ def test_host_front_end_returns_alpha(J, oracle):
     rgba = synth.smooth_rgba(90, 70, 12)
     cfg = J.EncodeConfig().with_quality(75)
     out = J.encode_front_end(rgba, cfg)
-    want = oracle.front_end(rgba, J.frame_params(cfg), J.default_quant_tables(), threads=2)
+    want = oracle.front_end(rgba, J.frame_params(cfg), threads=2)
     assert np.array_equal(out.alpha, want["alpha"])
     assert same_bits(out.qf, want["qf"]) and same_bits(out.cells, want["cells"])

@@ -84,7 +84,7 @@ def test_batch_members_match_oracle(J, fe, torch, oracle):
         torch.cuda.synchronize()
         for i in (0, 4):  # the large colour image and a gray member
             im, p = imgs[i]
-            want = oracle.front_end(im, p, J.default_quant_tables(), threads=4)
+            want = oracle.front_end(im, p, oracle.tables_for(p), threads=4)
             got = b.outputs(i)
             for k in KEYS:
                 assert same_bits(got[k].cpu().numpy(), want[k]), (i, k)

@@ -66,7 +66,7 @@ def test_front_end_chroma_hf_bit_exact(J, fe, torch, oracle, fused):
     rgb = chromatic_rgb(h, w, 7)
     p = J.frame_params(J.EncodeConfig(fused=fused).with_distance(0.8))
     p.flags = p.flags | J.FLAG_CHROMA_HF
-    want = oracle.front_end(rgb, p, J.default_quant_tables(), threads=4)
+    want = oracle.front_end(rgb, p, oracle.tables_for(p), threads=4)
     o = fe.run(to_dev(torch, rgb), w, h, p)
     for k in ("qf", "cells", "ytox", "ytob"):
         assert same_bits(o[k].cpu().numpy(), want[k]), k

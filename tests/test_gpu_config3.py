@@ -65,7 +65,7 @@ def test_front_end_full_size_bit_exact(J, fe, torch, oracle, rgba):
     dc = [o["dc"][c].cpu().numpy() for c in range(3)]
     del o
     log("oracle front end (16 threads)")
-    want = oracle.front_end(rgba, p, J.default_quant_tables(), threads=16)
+    want = oracle.front_end(rgba, p, oracle.tables_for(p), threads=16)
     log("comparing")
     assert np.array_equal(got_alpha, want["alpha"])
     assert (want["alpha"] == 0).any() and (want["alpha"] == 255).any()

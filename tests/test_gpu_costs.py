@@ -77,7 +77,7 @@ def test_site_costs_bit_exact(J, fe, torch, oracle, name, fused):
     ytob = rng.integers(-40, 41, tiles).astype(np.int8)
     sites = sites_for(name, w, h, rng)
     qac = (p.scale * rng.integers(4, 120, len(sites))).astype(np.float32)
-    tables = J.default_quant_tables()
+    tables = oracle.tables_for(p)
     s = STRATEGY[name]
     for meta_r in (4.0, 0.0):
         got = fe.strategy_cost_sites(xyb_d, w, h, p, s, sites, qac, meta_r=meta_r,

@@ -130,7 +130,7 @@ def _stage_inputs(J, oracle, w, h, distance, seed, fused=True, cmap=False):
 def test_strategy_costs_bit_exact(J, fe, torch, oracle, w, h, distance, fused):
     log("costs", w, h, distance, fused)
     rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, distance, 11, fused, cmap=True)
-    tables = J.default_quant_tables()
+    tables = oracle.tables_for(p)
     want = oracle.strategy_costs(xyb, qf, ytox, ytob, tables, p)
     oracle.set_fused(True)
     got = fe.strategy_costs(to_dev(torch, padded(J, xyb)), w, h, to_dev(torch, qf), p,
@@ -148,7 +148,7 @@ def test_strategy_costs_bit_exact(J, fe, torch, oracle, w, h, distance, fused):
 def test_strategy_selection_bit_exact(J, fe, torch, oracle, w, h, distance, fused):
     log("select", w, h, distance, fused)
     rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, distance, 5, fused, cmap=True)
-    tables = J.default_quant_tables()
+    tables = oracle.tables_for(p)
     cells_w, qf_w = oracle.select_strategy(xyb, qf, ytox, ytob, tables, p)
     oracle.set_fused(True)
     qf_d = to_dev(torch, qf)
@@ -164,7 +164,7 @@ def test_caller_tables_reach_every_stage(J, torch, oracle):
     quantization exactly as the oracle with the same tables."""
     w, h = 200, 136
     rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, 2.35, 12, True, cmap=True)
-    tables = J.default_quant_tables().copy()
+    tables = J.default_quant_tables(p.distance).copy()
     rng = np.random.default_rng(4)
     tables *= rng.uniform(0.8, 1.25, tables.shape).astype(np.float32)
     f2 = J.FrontEnd(0, tables=tables)
@@ -241,7 +241,7 @@ def test_forward_quantize_bit_exact(J, fe, torch, oracle, w, h, distance, crossi
     rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, distance, 9, fused, cmap=True)
     xb, yb = (w + 7) // 8, (h + 7) // 8
     cells = random_layout(xb, yb, seed=w * h, crossing=crossing)
-    tables = J.default_quant_tables()
+    tables = oracle.tables_for(p)
     lv_w, co_w, dc_w = oracle.forward_quantize(xyb, cells, qf, ytox, ytob, tables, p)
     oracle.set_fused(True)
     o = fe.forward_quantize(to_dev(torch, padded(J, xyb)), w, h, to_dev(torch, cells),
@@ -269,7 +269,7 @@ def test_forward_quantize_sub8_bit_exact(J, fe, torch, oracle, kind, fused):
         cells = random_layout(xb, yb, seed=77, crossing=True, sub8=True, big=True)
     else:
         cells = np.full((yb, xb), (kind << 1) | 1, np.uint8)
-    tables = J.default_quant_tables()
+    tables = oracle.tables_for(p)
     lv_w, co_w, dc_w = oracle.forward_quantize(xyb, cells, qf, ytox, ytob, tables, p)
     oracle.set_fused(True)
     o = fe.forward_quantize(to_dev(torch, padded(J, xyb)), w, h, to_dev(torch, cells),
@@ -296,7 +296,7 @@ def test_forward_quantize_dct64_bit_exact(J, fe, torch, oracle, w, h, seed, fuse
     else:
         cells = random_layout(xb, yb, seed=seed, crossing=seed % 2 == 0, big=True)
     assert any((c >> 1) in BIG for c in cells.reshape(-1) if c & 1)
-    tables = J.default_quant_tables()
+    tables = oracle.tables_for(p)
     lv_w, co_w, dc_w = oracle.forward_quantize(xyb, cells, qf, ytox, ytob, tables, p)
     oracle.set_fused(True)
     o = fe.forward_quantize(to_dev(torch, padded(J, xyb)), w, h, to_dev(torch, cells),
@@ -327,7 +327,7 @@ def test_front_end_bit_exact(J, fe, torch, oracle, name, w, h, q, fused):
     rgb = (synth.gradient_sines if name == "gradient" else synth.photo_like)(w, h, 3)
     cfg = J.EncodeConfig(fused=fused).with_quality(q)
     p = J.frame_params(cfg)
-    tables = J.default_quant_tables()
+    tables = oracle.tables_for(p)
     want = oracle.front_end(rgb, p, tables, threads=4)
     o = fe.run(to_dev(torch, rgb), w, h, p)
     assert same_bits(o["qf"].cpu().numpy(), want["qf"])
@@ -347,7 +347,7 @@ def test_front_end_gray_bit_exact(J, fe, torch, oracle, ch):
     rgb = synth.photo_like(136, 72, 8)
     gray = rgb[:, :, :1].copy() if ch == 1 else rgb[:, :, :2].copy()
     p = J.frame_params(J.EncodeConfig().with_quality(80.0))
-    want = oracle.front_end(gray, p, J.default_quant_tables(), threads=2)
+    want = oracle.front_end(gray, p, oracle.tables_for(p), threads=2)
     o = fe.run(to_dev(torch, gray), 136, 72, p)
     for k in ("qf", "cells", "ytox", "ytob"):
         assert same_bits(o[k].cpu().numpy(), want[k]), k
@@ -363,7 +363,7 @@ def test_host_front_end_matches_device_path(J, oracle):
     cfg = J.EncodeConfig().with_quality(75)
     out = J.encode_front_end(rgb, cfg)
     p = J.frame_params(cfg)
-    want = oracle.front_end(rgb, p, J.default_quant_tables(), threads=2)
+    want = oracle.front_end(rgb, p, oracle.tables_for(p), threads=2)
     assert same_bits(out.qf, want["qf"]) and same_bits(out.cells, want["cells"])
     assert same_bits(out.ytox, want["ytox"]) and same_bits(out.ytob, want["ytob"])
     for c in range(3):
@@ -376,7 +376,7 @@ def test_host_front_end_matches_device_path(J, oracle):
 def test_cmap_bit_exact(J, fe, torch, oracle, w, h, distance, fused):
     log("cmap", w, h, distance, fused)
     rgb, xyb, p, qf, _, _ = _stage_inputs(J, oracle, w, h, distance, 31, fused)
-    want_x, want_b = oracle.fill_cmap(xyb, J.default_quant_tables(), p)
+    want_x, want_b = oracle.fill_cmap(xyb, oracle.tables_for(p), p)
     oracle.set_fused(True)
     gx, gb = fe.fill_cmap(to_dev(torch, padded(J, xyb)), w, h, p)
     assert same_bits(gx.cpu().numpy(), want_x), first_mismatch(gx.cpu().numpy(), want_x)

@@ -53,7 +53,7 @@ def test_loss_and_gradient_match_fp64(J, fe, torch, oracle, w, h, q, lam):
         x = xyb[:, :h, :w].cpu().numpy()
         c = cells.cpu().numpy()
         th = theta.cpu().numpy().astype(np.float64)
-        want_loss, want_grad = oracle.tune_loss_grad(x, c, th, J.default_quant_tables(), p, lam,
+        want_loss, want_grad = oracle.tune_loss_grad(x, c, th, oracle.tables_for(p), p, lam,
                                                      9, 3)
         g = grad.cpu().numpy().astype(np.float64)
         assert abs(loss - want_loss) <= 2e-5 * abs(want_loss), (loss, want_loss)
@@ -69,8 +69,8 @@ def test_loss_and_gradient_match_fp64(J, fe, torch, oracle, w, h, q, lam):
             tp, tm = th.copy(), th.copy()
             tp[by, bx] += hstep
             tm[by, bx] -= hstep
-            lp, _ = oracle.tune_loss_grad(x, c, tp, J.default_quant_tables(), p, lam, 9, 3, False)
-            lm, _ = oracle.tune_loss_grad(x, c, tm, J.default_quant_tables(), p, lam, 9, 3, False)
+            lp, _ = oracle.tune_loss_grad(x, c, tp, oracle.tables_for(p), p, lam, 9, 3, False)
+            lm, _ = oracle.tune_loss_grad(x, c, tm, oracle.tables_for(p), p, lam, 9, 3, False)
             fd = (lp - lm) / (2 * hstep)
             assert abs(g[by, bx] - fd) <= 2e-3 * scale + 1e-3 * abs(fd), (bx, by, g[by, bx], fd)
     finally:

@@ -76,32 +76,6 @@ def test_table_blob_layout():
     assert [_lib.lib().jxg_qt_kind_of(s) for s in range(24)][:5] == [0, 1, 1, 2, 3]
 
 
-def test_default_tables_match_reference_library_tables(golden):
-    """The band construction reproduces the reference's stored DCT8 and 16x8
-    tables to the ~4e-6 the reference quotes between them
-    (src/quant_weights.rs:1996-1999)."""
-    blob = J.default_quant_tables()
-    for kind, name, n in [(0, "DEQUANT_MATRIX_8X8", 64), (1, "DEQUANT_MATRIX_16X8", 128)]:
-        off = J.qt_offset(kind)
-        for c in range(3):
-            got = blob[off + c * n: off + (c + 1) * n]
-            want = np.array(golden[name][c], np.float32)
-            assert np.allclose(got, want, rtol=2e-5, atol=0), (name, c)
-            inv = blob[off + (3 + c) * n: off + (4 + c) * n]
-            assert inv[0] == 0.0
-            assert np.abs(got[1:] * inv[1:] - 1.0).max() <= 4 * np.finfo(np.float32).eps
-
-
-def test_dct16_and_32_tables_dc_weight():
-    # dct16x16_matrix_dc_matches_libjxl_polynomial (src/quant_weights.rs:3477)
-    blob = J.default_quant_tables()
-    off = J.qt_offset(2)
-    for c, band0 in enumerate([8996.87257, 3191.48366, 1157.50408]):
-        assert abs(blob[off + c * 256] - 1.0 / band0) < 1e-7
-        m = blob[off + c * 256: off + (c + 1) * 256]
-        assert (m > 0).all() and m[255] > m[0]
-
-
 def test_plane_geometry():
     assert J.plane_geometry(1, 1) == (64, 64)
     assert J.plane_geometry(2048, 2048) == (2048, 2048)
@@ -124,3 +98,14 @@ def test_no_device_is_a_loud_error():
         pytest.skip("a GPU is visible")
     with pytest.raises(J.JxgError):
         J.FrontEnd(0)
+
+
+def test_frame_params_carry_the_spec_xyb_matrix():
+    p = J.frame_params(J.EncodeConfig())
+    assert np.array_equal(np.array(p.xyb_fwd[:], np.float32), J.XYB_SPEC_FWD)
+    assert p.xyb_order == 0 and p.arbiter == J.ARBITER_SSIMULACRA2
+    out = np.zeros(9, np.float32)
+    _lib.lib().jxg_spec_xyb_matrix(out.ctypes.data)
+    assert np.array_equal(out, J.XYB_SPEC_FWD)
+    q = J.frame_params(J.EncodeConfig(xyb_order=1).with_arbiter(J.ARBITER_CVVDP))
+    assert q.xyb_order == 1 and q.arbiter == 1

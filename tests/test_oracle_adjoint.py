@@ -31,7 +31,7 @@ def setup(J, oracle, w, h, seed=3):
     rgb = synth.photo_like(w, h, seed)
     xyb = oracle.xyb_image(rgb, J.XYB_SPEC_FWD)
     p = J.frame_params(J.EncodeConfig().with_quality(75))
-    return xyb, p, J.default_quant_tables()
+    return xyb, p, oracle.tables_for(p)
 
 
 def theta_for(cells, rng):

@@ -183,5 +183,5 @@ def test_front_end_alpha_plane_is_the_cleared_input(oracle):
     from jixel_amd import synth
     rgba = synth.smooth_rgba(96, 80, 3)
     p = J.frame_params(J.EncodeConfig().with_quality(75))
-    out = oracle.front_end(rgba, p, J.default_quant_tables(), threads=2)
+    out = oracle.front_end(rgba, p, oracle.tables_for(p), threads=2)
     assert np.array_equal(out["alpha"], oracle.alpha_fringe_clear(rgba[:, :, 3]))

@@ -78,7 +78,7 @@ def test_cmap_matches_fp64_regression(J, oracle, distance):
     oracle.set_fused(True)
     xyb = oracle.xyb_image(rgb, J.XYB_SPEC_FWD)
     p = J.frame_params(J.EncodeConfig(distance=distance))
-    tables = J.default_quant_tables()
+    tables = oracle.tables_for(p)
     ytox, ytob = oracle.fill_cmap(xyb, tables, p)
     exact = 0
     for ty in range(ytox.shape[0]):
@@ -100,7 +100,7 @@ def test_proportional_chroma_recovers_the_slope(J, oracle, slope):
     y = rng.uniform(0.0, 20.0, (h, w)).astype(np.float32)
     xyb = np.stack([slope * y, y, (1.0 - slope) * y]).astype(np.float32)
     p = J.frame_params(J.EncodeConfig(distance=2.35))
-    ytox, ytob = oracle.fill_cmap(xyb, J.default_quant_tables(), p)
+    ytox, ytob = oracle.fill_cmap(xyb, oracle.tables_for(p), p)
     assert np.all(ytox == round(84 * slope)), ytox
     assert np.all(ytob == round(-84 * slope)), ytob
 
@@ -111,7 +111,7 @@ def test_deadzone_snaps_small_slopes_at_high_quality(J, oracle):
     xyb = np.stack([0.012 * y, y, y]).astype(np.float32)  # slope ~1 step < 1.5
     for d, want in ((0.9, 0), (2.5, 1)):
         p = J.frame_params(J.EncodeConfig(distance=d))
-        ytox, _ = oracle.fill_cmap(xyb, J.default_quant_tables(), p)
+        ytox, _ = oracle.fill_cmap(xyb, oracle.tables_for(p), p)
         assert ytox[0, 0] == want, (d, ytox)
 
 
@@ -120,5 +120,5 @@ def test_flat_and_partial_tiles(J, oracle):
     (or flat content) has slope 0."""
     xyb = np.full((3, 13, 70), 0.3, np.float32)
     p = J.frame_params(J.EncodeConfig(distance=2.35))
-    ytox, ytob = oracle.fill_cmap(xyb, J.default_quant_tables(), p)
+    ytox, ytob = oracle.fill_cmap(xyb, oracle.tables_for(p), p)
     assert ytox.shape == (1, 2) and not ytox.any() and not ytob.any()

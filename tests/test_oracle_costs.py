@@ -20,7 +20,7 @@ def test_flat_image_costs_only_metadata(oracle, name, meta_r):
     p = J.frame_params(J.EncodeConfig().with_distance(1.5))
     s = J.STRATEGY[name]
     cost = oracle.strategy_cost(xyb, s, 0, 0, float(p.scale * 40), [0.0, 0.0, 1.0],
-                                J.default_quant_tables(), p, meta_r=meta_r)
+                                oracle.tables_for(p), p, meta_r=meta_r)
     if name.startswith("AFV"):
         # the AFV basis leaves f32 rounding residue in its AC slots: no level,
         # a squared error of order 1e-15

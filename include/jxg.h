@@ -269,6 +269,21 @@ int jxg_fill_quant_field_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
                              uint32_t padded_h, uint32_t width, uint32_t height,
                              float distance, float inv_scale, int fused, uint8_t *d_qf);
 
+/* jxg_fill_quant_field_dev without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * AQ pre-map lives in a static device array of JXG_NOALLOC_AQ_PRE_FLOATS
+ * floats (16 MiB, one per GPU, held until the process ends), so images up to
+ * 2*ceil(W/8) * 2*ceil(H/8) <= JXG_NOALLOC_AQ_PRE_FLOATS (about 67
+ * megapixels) are accepted and larger ones return JXG_ERR_TOO_LARGE.  Every
+ * call on a GPU shares that array: calls that overlap on one GPU (two
+ * streams, two contexts) race.  Output is identical to
+ * jxg_fill_quant_field_dev's. */
+#define JXG_NOALLOC_AQ_PRE_FLOATS (4u << 20)
+int jxg_fill_quant_field_noalloc_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
+                                     uint32_t padded_h, uint32_t width, uint32_t height,
+                                     float distance, float inv_scale, int fused, uint8_t *d_qf);
+
 /* Coefficient-model RD costs of every square candidate the Fast tiers
  * evaluate: per block its DCT8 cost, per 2x2-aligned site the DCT16X16 cost,
  * per 4x4-aligned site the DCT32X32 cost (NaN where a site cannot be placed).

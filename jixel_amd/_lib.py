@@ -31,6 +31,7 @@ EXPORTED = [
     "jxg_batch_launch", "jxg_batch_status", "jxg_batch_autotune", "jxg_batch_config",
     "jxg_qf_tuner_create", "jxg_qf_tuner_destroy", "jxg_qf_tuner_excluded",
     "jxg_qf_tuner_loss_grad", "jxg_qf_tuner_run", "jxg_dct128_image_dev",
+    "jxg_fill_quant_field_noalloc_dev",
 ]
 
 
@@ -133,6 +134,7 @@ def _declare(L):
                               c.c_int, vp, c.c_uint32, c.c_uint32]
     L.jxg_fill_quant_field_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
                                            c.c_float, c.c_float, c.c_int, vp]
+    L.jxg_fill_quant_field_noalloc_dev.argtypes = L.jxg_fill_quant_field_dev.argtypes
     L.jxg_strategy_costs_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
                                          vp, vp, vp, c.POINTER(FrameParams), vp, vp, vp]
     L.jxg_select_strategy_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,

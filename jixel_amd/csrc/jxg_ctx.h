@@ -61,6 +61,7 @@ struct jxg_ctx {
   float *d_exp2 = nullptr;      // fast_exp2 table
   float *d_pre = nullptr;
   size_t pre_cap = 0;
+  float *d_noalloc_pre = nullptr;  // device address of the static no-allocation scratch
   float *d_xyb = nullptr;
   size_t xyb_cap = 0;
   uint32_t *d_misc = nullptr;  // [0] crossing count, [1] unsupported flag (zero between

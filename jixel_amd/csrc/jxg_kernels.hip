@@ -2447,6 +2447,13 @@ JXG_HIDDEN int fill_u8(jxg_ctx *ctx, uint8_t *p, size_t n, uint8_t v) {
 // ---------------------------------------------------------------------------
 // Context and C ABI
 
+// Device-resident AQ pre-map of jxg_fill_quant_field_noalloc_dev: one copy
+// per GPU, held until the process ends.  Host code never passes the array's
+// name to a kernel; jxg_ctx_create resolves its device address once per
+// context (hipGetSymbolAddress, with the context's device current) into
+// ctx->d_noalloc_pre, and only that address is used.
+static __device__ __align__(256) float scratch[JXG_NOALLOC_AQ_PRE_FLOATS];
+
 // Uploads a quant-table blob and the scan-ordered copy of the DCT8 /
 // DCT16X16 / DCT32X32 inverse weights the strategy search reads.
 static int upload_tables(jxg_ctx *ctx, const float *blob, float *d_blob, float *d_selq) {
@@ -2597,6 +2604,13 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
     jxg_ctx_destroy(ctx);
     return rc;
   };
+  {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(set_err(JXG_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e)));
+    ctx->d_noalloc_pre = (float *)p;
+  }
   if (hipMalloc((void **)&ctx->d_tables, JXG_QT_BLOB_FLOATS * sizeof(float)) != hipSuccess ||
       hipMalloc((void **)&ctx->d_tier_tables,
                 (size_t)2 * JXG_QUANT_TIERS * JXG_QT_BLOB_FLOATS * sizeof(float)) != hipSuccess ||
@@ -2776,26 +2790,36 @@ int jxg_xyb_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int channe
   return JXG_OK;
 }
 
-static int fill_qf(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h, uint32_t W,
-                   uint32_t H, float distance, float inv_scale, int fused, uint8_t *d_qf) {
+// The AQ chain with its pre-map (2 xb x 2 yb floats, every entry written by
+// k_aq_pre before k_aq_block reads it) at `pre`.
+static int fill_qf_into(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
+                        uint32_t W, uint32_t H, float distance, float inv_scale, int fused,
+                        uint8_t *d_qf, float *pre) {
   const int xb = (W + 7) / 8, yb = (H + 7) / 8;
   const int pre_w = 2 * xb, pre_h = 2 * yb;
-  int rc = ensure(&ctx->d_pre, &ctx->pre_cap, (size_t)pre_w * pre_h);
-  if (rc) return rc;
   const float *yp = d_xyb + (size_t)stride * padded_h;
   const dim3 g1((4 * pre_w + 255) / 256, pre_h);  // four lanes per pre-map value
   if (fused)
-    k_aq_pre<true><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, ctx->d_pre);
+    k_aq_pre<true><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, pre);
   else
-    k_aq_pre<false><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, ctx->d_pre);
+    k_aq_pre<false><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, pre);
   LAUNCH_CHECK("k_aq_pre");
   const dim3 g2((xb + AQ_STRIP - 1) / AQ_STRIP, yb);
   if (fused)
-    k_aq_block<true><<<g2, 256, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, ctx->d_pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
+    k_aq_block<true><<<g2, 256, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
   else
-    k_aq_block<false><<<g2, 256, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, ctx->d_pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
+    k_aq_block<false><<<g2, 256, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
   LAUNCH_CHECK("k_aq_block");
   return JXG_OK;
+}
+
+static int fill_qf(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h, uint32_t W,
+                   uint32_t H, float distance, float inv_scale, int fused, uint8_t *d_qf) {
+  const size_t need = (size_t)(2 * ((W + 7) / 8)) * (2 * ((H + 7) / 8));
+  int rc = ensure(&ctx->d_pre, &ctx->pre_cap, need);
+  if (rc) return rc;
+  return fill_qf_into(ctx, d_xyb, stride, padded_h, W, H, distance, inv_scale, fused, d_qf,
+                      ctx->d_pre);
 }
 
 int jxg_fill_quant_field_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
@@ -2808,6 +2832,26 @@ int jxg_fill_quant_field_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, 
   if (!(distance > 0.0f) || !(inv_scale > 0.0f))
     return set_err(JXG_ERR_INVALID_DISTANCE, "distance %g inv_scale %g", (double)distance, (double)inv_scale);
   return fill_qf(ctx, d_xyb, stride, padded_h, width, height, distance, inv_scale, fused, d_qf);
+}
+
+int jxg_fill_quant_field_noalloc_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
+                                     uint32_t padded_h, uint32_t width, uint32_t height,
+                                     float distance, float inv_scale, int fused, uint8_t *d_qf) {
+  int rc = check_image(width, height);
+  if (rc) return rc;
+  if (!ctx || !d_xyb || !d_qf) return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  if ((rc = check_planes(width, height, stride, padded_h))) return rc;
+  if (!(distance > 0.0f) || !(inv_scale > 0.0f))
+    return set_err(JXG_ERR_INVALID_DISTANCE, "distance %g inv_scale %g", (double)distance, (double)inv_scale);
+  const size_t need = (size_t)(2 * ((width + 7) / 8)) * (2 * ((height + 7) / 8));
+  if (need > JXG_NOALLOC_AQ_PRE_FLOATS)
+    return set_err(JXG_ERR_TOO_LARGE,
+                   "image %ux%u needs a %zu-float AQ pre-map; the no-allocation entry point "
+                   "holds at most %u (about 67 megapixels)",
+                   width, height, need, (unsigned)JXG_NOALLOC_AQ_PRE_FLOATS);
+  if (!ctx->d_noalloc_pre) return set_err(JXG_ERR_DEVICE, "no-allocation scratch not mapped");
+  return fill_qf_into(ctx, d_xyb, stride, padded_h, width, height, distance, inv_scale, fused,
+                      d_qf, ctx->d_noalloc_pre);
 }
 
 static int select_impl(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h, uint32_t W,

@@ -1,11 +1,17 @@
 """VarDCT front-end throughput on one MI355X.
 
-A step is one pass of the front end (sRGB u8 -> XYB -> adaptive quant field
--> CfL maps -> AC-strategy search -> forward DCT + quantization) over one 2048x2048 RGB8
-synthetic photo-like image at quality 75, with the input already resident in
-HBM.  Prints one JSON line: whole-job megapixels per second, the dominant
-kernel's roofline position, and the CPU oracle's throughput on the host cores
-of the same machine as a reported baseline.
+A step is one pass of the front end (u8 sRGB(A) -> XYB, alpha fringe
+clearing and fill_invisible -> adaptive quant field -> CfL maps -> AC-strategy
+search -> alpha block factors -> forward DCT + quantization) over one
+7680x4320 RGBA8 synthetic image (1/f noise smoothed over 33-129 px regions,
+alpha 0 / 255 inside 20 soft-edged ellipses) at quality 75, at the
+reference's default speed, with the input already resident in HBM and the
+outputs left there.  Prints one JSON line: whole-job megapixels per second,
+per-kernel device times measured inside the timed region, the dominant
+kernel's roofline position, the pinned-host pipelined rate (PCIe included,
+never the headline), the 2048x2048 RGB8 q75 photo-like workload, the batch
+and quant-field tuning workloads, and the CPU oracle on the host cores of the
+same machine as a reported baseline.
 
 usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu]
 """
@@ -20,18 +26,27 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E, GB/s
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32, TFLOP/s
 
-# Algorithmic HBM bytes per pixel of each stage (reads + writes of the
-# buffers the stage must touch once).
-STAGE_BYTES_PER_PX = {
-    "xyb": 3.0 + 12.0,          # u8 RGB in, 3 f32 planes out
-    "aq": 12.0 + 1.0 / 64,      # 3 planes in (Y twice from cache), u8 field out
-    "cfl": 12.0,                # 3 planes in, two i8 per 64x64 tile out
-    "select": 12.0 + 2.0 / 64,  # 3 planes in, cells + field out
-    "quantize": 12.0 + 24.0 + 6.0 / 64 + 2.0 / 64,  # planes in, i32+f32 out, dc/qf/cells
+# Algorithmic HBM bytes per pixel of each profiled kernel (reads + writes of
+# the buffers it must touch once), for RGBA input.
+KERNEL_BYTES_PER_PX = {
+    "xyb": 4.0 + 12.0,           # u8 RGBA in, 3 f32 planes out
+    "alpha": 4.0 + 12.0 + 12.0,  # alpha in, 3 planes read and written by fill_invisible
+    "cmap": 12.0,                # 3 planes in, two i8 per 64x64 tile out
+    "aq_pre": 4.0 + 1.0 / 4,     # Y plane in, the 1/16-size pre-map out
+    "aq_block": 12.0 + 1.0 / 64, # 3 planes in, u8 field out
+    "select": 12.0 + 2.0 / 64,   # 3 planes in, cells + field out
+    "alpha_qf": 1.0 + 2.0 / 64,  # alpha in, field rewritten
+    "quantize": 12.0 + 24.0 + 6.0 / 64 + 2.0 / 64,  # planes in, i32 + f32 out, DC, qf, cells
+    "quantize_xt": 0.0,          # the tile-crossing queue (empty at this tier)
 }
-
+# FP32 work of the strategy search per pixel: the DCT8 / 16x16 / 32x32
+# candidates' butterflies (4 log2 N flops per pixel per channel for an N x N
+# transform: 12 + 16 + 20) x 3 channels, plus about 10 flops per coefficient
+# for the quantize / SSE / rate terms of 3 candidates x 3 channels.
+SELECT_FLOPS_PER_PX = 3 * (12 + 16 + 20) + 3 * 3 * 10
 
 EXTRA_BYTES_PER_PX = {
     "gaborish": 12.0 + 12.0,  # 3 planes in, 3 planes out
@@ -40,24 +55,41 @@ EXTRA_BYTES_PER_PX = {
 
 CAL_BYTES = 256 << 20  # calibration copy size for the byte counters
 
-# Kernel of each stage, as rocprofv3 names it.
-STAGE_KERNEL = {"xyb": "k_xyb", "aq": "k_aq_block", "cfl": "k_cmap", "select": "k_select",
-                "quantize": "k_quantize"}
+# rocprofv3 kernel name of each profiled stage
+STAGE_KERNEL = {"xyb": "k_xyb", "cmap": "k_cmap_blocks", "aq_pre": "k_aq_pre",
+                "aq_block": "k_aq_block", "select": "k_select", "quantize": "k_quantize",
+                "quantize_xt": "k_quantize_xt"}
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_photo_2048x2048_q75.txt")
+HEADLINE = "smooth_rgba_7680x4320_q75"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_smooth_rgba_7680x4320_q75.txt")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "issue_select_quantize_7680x4320_q75.txt")
 
 
 def read_traffic(stage):
     """HBM bytes per launch of the stage's kernel from the committed counter
     summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy)."""
-    if not os.path.exists(TRAFFIC_FILE):
+    if not os.path.exists(TRAFFIC_FILE) or stage not in STAGE_KERNEL:
         return None
     want = STAGE_KERNEL[stage]
     with open(TRAFFIC_FILE) as f:
         for line in f:
             parts = line.split()
-            if len(parts) >= 2 and parts[0] == "traffic_bytes" and parts[1] == want:
+            if len(parts) >= 3 and parts[0] == "traffic_bytes" and parts[1] == want:
                 return int(parts[2])
+    return None
+
+
+def read_valu_busy(kernel):
+    """VALU-busy fraction of a kernel from the committed SQ-counter summary."""
+    if not os.path.exists(ISSUE_FILE):
+        return None
+    cur = None
+    with open(ISSUE_FILE) as f:
+        for line in f:
+            if line and not line[0].isspace() and not line.startswith("#"):
+                cur = line.split("<")[0].strip()
+            elif cur == kernel and line.strip().startswith("VALU busy"):
+                return float(line.split()[-1].rstrip("%")) / 100.0
     return None
 
 
@@ -66,12 +98,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--width", type=int, default=2048)
-    ap.add_argument("--height", type=int, default=2048)
     ap.add_argument("--quality", type=float, default=75.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--profile", action="store_true", help="few steps, no CPU leg (for rocprofv3)")
+    ap.add_argument("--profile", action="store_true", help="few steps, headline only (for rocprofv3)")
+    ap.add_argument("--photo-only", action="store_true",
+                    help="run only the 2048x2048 RGB8 photo-like workload (for rocprofv3)")
     ap.add_argument("--batch", type=int, default=512,
                     help="images in the batched-front-end workload (0: skip it)")
     ap.add_argument("--batch-only", action="store_true",
@@ -82,27 +114,61 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(rgb, params, threads):
-    """The oracle's whole front end on the host cores (kind "port")."""
-    import jixel_amd as J
+    """The oracle's whole front end on the host cores (kind "port"), built
+    -O3 -march=native for this host (-ffp-contract=off kept: the same bits
+    as the checker build).  All cores on the whole image, and one thread on
+    a 2048x1080 slab of it, each the median of up to 3 runs after a warm-up."""
+    import subprocess
+    native = os.path.join(ROOT, "oracle", "_build", "libjxo_native.so")
+    try:
+        subprocess.check_call(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native"],
+                              stdout=subprocess.DEVNULL)
+        os.environ["JXO_LIB"] = native
+        build = "-O3 -march=native -ffp-contract=off"
+    except (OSError, subprocess.CalledProcessError):
+        build = "-O2 -march=x86-64-v2 -ffp-contract=off (native build failed)"
     from oracle import oracle as O
-    h, w, _ = rgb.shape
     tables = O.tables_for(params)
-    O.front_end(rgb[:64, :64].copy(), params, tables, threads)  # warm
-    runs = []
-    t_end = time.perf_counter() + 12.0
-    while len(runs) < 5 and (not runs or time.perf_counter() < t_end):
-        t0 = time.perf_counter()
-        O.front_end(rgb, params, tables, threads)
-        runs.append(time.perf_counter() - t0)
-    sec = float(np.median(runs))
+    h, w, _ = rgb.shape
+
+    def rate(img, n_threads, budget):
+        O.front_end(img[:64, :64].copy(), params, tables, n_threads)  # warm
+        runs = []
+        t_end = time.perf_counter() + budget
+        while len(runs) < 3 and (not runs or time.perf_counter() < t_end):
+            t0 = time.perf_counter()
+            O.front_end(img, params, tables, n_threads)
+            runs.append(time.perf_counter() - t0)
+        sec = float(np.median(runs))
+        return img.shape[0] * img.shape[1] / 1e6 / sec, sec, len(runs)
+
+    all_mp, all_sec, all_n = rate(rgb, threads, 12.0)
+    slab = np.ascontiguousarray(rgb[:1080, :2048])
+    one_mp, one_sec, one_n = rate(slab, 1, 10.0)
     return {
-        "value": round(w * h / 1e6 / sec, 3),
+        "value": round(all_mp, 3),
         "unit": "MP/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"whole front end of the same {w}x{h} image, median of {len(runs)} runs "
-                  f"({sec:.2f} s each) with {threads} threads",
+        "sample": f"whole front end of the same {w}x{h} RGBA image, median of {all_n} runs "
+                  f"({all_sec:.2f} s each) with {threads} threads",
+        "one_thread": {"value": round(one_mp, 3), "unit": "MP/s",
+                       "sample": f"2048x1080 slab of the image, median of {one_n} runs "
+                                 f"({one_sec:.2f} s each)"},
+        "cpu": cpu_model(),
+        "build": build,
     }
 
 
@@ -204,35 +270,64 @@ def dct128_forms(fe, xyb, w, h, stream, reps):
     return res
 
 
-def host_path_rate(fe, rgb, params, reps=5):
-    """PCIe-inclusive rate of the host-buffer entry point (jxg_front_end):
-    pageable host RGB in, every output copied back to pageable host memory.
-    Reported beside the device-resident figure, never as the headline."""
-    import ctypes
-    from jixel_amd import _lib
-    h, w, ch = rgb.shape
-    xb, yb = (w + 7) // 8, (h + 7) // 8
-    qf = np.zeros((yb, xb), np.uint8)
-    cells = np.zeros((yb, xb), np.uint8)
-    levels = [np.zeros(yb * xb * 64, np.int32) for _ in range(3)]
-    coeffs = [np.zeros(yb * xb * 64, np.float32) for _ in range(3)]
-    dc = [np.zeros(yb * xb, np.int16) for _ in range(3)]
-    arr = lambda xs: (ctypes.c_void_p * 3)(*[x.ctypes.data for x in xs])
-    lv, co, dd = arr(levels), arr(coeffs), arr(dc)
+def host_pipeline_rate(J, torch, fe, d_rgb, params, n_images=6):
+    """PCIe-inclusive rate with pinned host buffers: n_images front ends of
+    the headline image, input copied from pinned host memory and every output
+    (levels, f32 coefficients, DC, field, cells, CfL maps) copied back to
+    pinned host memory, the copies of one image overlapped with the next
+    image's compute (two device slots, an upload stream and a download
+    stream beside the context's).  Never the headline."""
+    h, w, ch = d_rgb.shape
+    host_in = torch.empty((h, w, ch), dtype=torch.uint8, pin_memory=True)
+    host_in.copy_(d_rgb.cpu())
+    up, down = torch.cuda.Stream(), torch.cuda.Stream()
+    comp = torch.cuda.current_stream()
+    slots = []
+    for _ in range(2):
+        o = fe.outputs(w, h)
+        xb, yb = (w + 7) // 8, (h + 7) // 8
+        tiles = ((h + 63) // 64, (w + 63) // 64)
+        o["qf"] = fe.empty((yb, xb), torch.uint8)
+        o["cells"] = fe.empty((yb, xb), torch.uint8)
+        o["ytox"] = torch.zeros(tiles, dtype=torch.int8, device="cuda")
+        o["ytob"] = torch.zeros(tiles, dtype=torch.int8, device="cuda")
+        dev_in = torch.empty_like(d_rgb)
+        flat = [o["qf"], o["cells"], o["ytox"], o["ytob"]] + list(o["levels"]) + \
+            list(o["coeffs"]) + list(o["dc"])
+        host = [torch.empty(t.shape, dtype=t.dtype, pin_memory=True) for t in flat]
+        ev = {k: torch.cuda.Event() for k in ("in", "done", "free")}
+        ev["free"].record(down)
+        slots.append((dev_in, o, flat, host, ev))
+    out_bytes = sum(t.numel() * t.element_size() for t in slots[0][2])
+    torch.cuda.synchronize()
 
-    def once():
-        _lib.check(_lib.lib().jxg_front_end(fe.ctx, rgb.ctypes.data, w * ch, ch, w, h,
-                                            ctypes.byref(params), None, None, qf.ctypes.data,
-                                            cells.ctypes.data, lv, co, dd))
-    once()
-    times = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        once()
-        times.append(time.perf_counter() - t0)
-    sec = float(np.median(times))
-    return {"value": round(w * h / 1e6 / sec, 1), "unit": "MP/s",
-            "ms": round(sec * 1e3, 3), "note": "pageable host buffers, copies included"}
+    def run(n):
+        for i in range(n):
+            dev_in, o, flat, host, ev = slots[i % 2]
+            up.wait_event(ev["free"])
+            with torch.cuda.stream(up):
+                dev_in.copy_(host_in, non_blocking=True)
+            ev["in"].record(up)
+            comp.wait_event(ev["in"])
+            fe.run(dev_in, w, h, params, outputs=o, qf=o["qf"], cells=o["cells"],
+                   ytox=o["ytox"], ytob=o["ytob"])
+            ev["done"].record(comp)
+            down.wait_event(ev["done"])
+            with torch.cuda.stream(down):
+                for dst, src in zip(host, flat):
+                    dst.copy_(src, non_blocking=True)
+            ev["free"].record(down)
+        torch.cuda.synchronize()
+    run(2)
+    t0 = time.perf_counter()
+    run(n_images)
+    sec = (time.perf_counter() - t0) / n_images
+    del slots
+    torch.cuda.empty_cache()
+    return {"value": round(w * h / 1e6 / sec, 1), "unit": "MP/s", "ms_per_image": round(sec * 1e3, 3),
+            "bytes_in_per_image": int(h * w * ch), "bytes_out_per_image": int(out_bytes),
+            "note": f"pinned host buffers, {n_images} images, uploads and downloads overlapped "
+                    "with the next image's compute; PCIe included"}
 
 
 def log(msg):
@@ -363,6 +458,70 @@ def tune_workload(J, torch, fe, stream, n=16, side=1024, steps=20, lr=0.05, lam=
     }
 
 
+def front_end_workload(J, torch, fe, stream, d_rgb, p, steps, warmup):
+    """Times `steps` front-end calls on a device-resident image (HIP events on
+    the context's stream bracketing the whole timed region, and the host
+    clock), with the library's per-kernel event pairs recorded inside that
+    region.  Returns (ms per step, {stage: mean kernel ms})."""
+    h, w, _ = d_rgb.shape
+    dev = d_rgb.device
+    outs = fe.outputs(w, h)
+    xb, yb = (w + 7) // 8, (h + 7) // 8
+    qf = fe.empty((yb, xb), torch.uint8)
+    cells = fe.empty((yb, xb), torch.uint8)
+    tiles = ((h + 63) // 64, (w + 63) // 64)
+    ytox = torch.zeros(tiles, dtype=torch.int8, device=dev)
+    ytob = torch.zeros(tiles, dtype=torch.int8, device=dev)
+
+    def step():
+        fe.run(d_rgb, w, h, p, outputs=outs, qf=qf, cells=cells, ytox=ytox, ytob=ytob)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    fe.profile(True)
+    t0 = time.perf_counter()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = fe.profile_read()
+    fe.profile(False)
+    ms_gpu = e0.elapsed_time(e1) / steps
+    ms_step = max(ms_gpu, wall * 1000.0 / steps)
+    del outs, qf, cells, ytox, ytob
+    return ms_step, {k: v[0] for k, v in prof.items() if v[1] > 0}
+
+
+def roofline_of(kernel_ms, px, traffic_stage_ok=True):
+    dominant = max(kernel_ms, key=kernel_ms.get)
+    alg_bytes = KERNEL_BYTES_PER_PX[dominant] * px
+    achieved = alg_bytes / (kernel_ms[dominant] * 1e-3) / 1e9
+    r = {
+        "kernel": dominant,
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": read_traffic(dominant) if traffic_stage_ok else None,
+        "algorithmic_bytes_per_launch": int(alg_bytes),
+        "kernel_ms": round(kernel_ms[dominant], 4),
+    }
+    if dominant == "select":
+        tflops = SELECT_FLOPS_PER_PX * px / (kernel_ms[dominant] * 1e-3) / 1e12
+        r["limiter"] = "VALU issue, not HBM: the candidate transforms and RD sums run in LDS"
+        r["fp32"] = {"achieved_tflops": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                     "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                     "flops_per_px": SELECT_FLOPS_PER_PX}
+        r["valu_busy"] = read_valu_busy("k_select")
+    return r
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -377,43 +536,19 @@ def main():
 
     if args.profile:
         args.steps, args.warmup, args.no_cpu = min(args.steps, 5), min(args.warmup, 2), True
-    if args.tune_only:
-        dev = torch.device("cuda", 0)
-        torch.cuda.set_device(dev)
-        stream = torch.cuda.current_stream(dev)
-        fe = J.FrontEnd(0, stream=stream)
-        print(json.dumps({"qf_tune": tune_workload(J, torch, fe, stream)}), flush=True)
-        fe.close()
-        return
-    if args.batch_only:
-        dev = torch.device("cuda", 0)
-        torch.cuda.set_device(dev)
-        stream = torch.cuda.current_stream(dev)
-        fe = J.FrontEnd(0, stream=stream)
-        print(json.dumps({"batch": batch_workload(J, torch, fe, stream, max(1, args.batch),
-                                                  max(1, min(args.steps, 3)))}), flush=True)
-        fe.close()
-        return
-    w, h = args.width, args.height
-    rgb = synth.photo_like(w, h, 2)
-    cfg = J.EncodeConfig().with_quality(args.quality)
-    p = J.frame_params(cfg)
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     stream = torch.cuda.current_stream(dev)
     fe = J.FrontEnd(0, stream=stream)
-    d_rgb = torch.from_numpy(rgb).to(dev)
-    outs = fe.outputs(w, h)
-    xb, yb = (w + 7) // 8, (h + 7) // 8
-    qf = fe.empty((yb, xb), torch.uint8)
-    cells = fe.empty((yb, xb), torch.uint8)
-    tiles = ((h + 63) // 64, (w + 63) // 64)
-    ytox = torch.zeros(tiles, dtype=torch.int8, device=dev)
-    ytob = torch.zeros(tiles, dtype=torch.int8, device=dev)
-
-    def step():
-        fe.run(d_rgb, w, h, p, outputs=outs, qf=qf, cells=cells, ytox=ytox, ytob=ytob)
-
+    if args.tune_only:
+        print(json.dumps({"qf_tune": tune_workload(J, torch, fe, stream)}), flush=True)
+        fe.close()
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": batch_workload(J, torch, fe, stream, max(1, args.batch),
+                                                  max(1, min(args.steps, 3)))}), flush=True)
+        fe.close()
+        return
     if args.profile:
         # Calibration for the HBM byte counters: one plain copy kernel that
         # reads and writes exactly CAL_BYTES each (a 256 MiB f32 negation).
@@ -423,70 +558,57 @@ def main():
         torch.neg(cal, out=cal_out)
         torch.cuda.synchronize()
         del cal, cal_out
-
-    for _ in range(args.warmup):
-        step()
+    cfg = J.EncodeConfig().with_quality(args.quality)
+    p = J.frame_params(cfg)
+    # the 2048x2048 RGB8 photo-like workload
+    photo = synth.photo_like(2048, 2048, 2)
+    d_photo = torch.from_numpy(photo).to(dev)
+    photo_ms, photo_k = front_end_workload(J, torch, fe, stream, d_photo, p,
+                                           args.steps if args.photo_only else max(5, args.steps // 2),
+                                           args.warmup)
+    photo_res = {
+        "workload": "photo_like_2048x2048_rgb8_q75",
+        "mp_per_s": round(2048 * 2048 / 1e6 / (photo_ms * 1e-3), 2),
+        "ms_per_step": round(photo_ms, 4),
+        "kernel_ms": {k: round(v, 4) for k, v in photo_k.items()},
+        "roofline": roofline_of(photo_k, 2048 * 2048, traffic_stage_ok=False),
+    }
+    if args.photo_only:
+        print(json.dumps({"photo": photo_res}), flush=True)
+        fe.close()
+        return
+    # the headline: 7680x4320 RGBA q75
+    w, h = 7680, 4320
+    log(f"generating the {w}x{h} RGBA image on the device")
+    d_rgb = synth.smooth_rgba_device(w, h, 3, dev)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_gpu = e0.elapsed_time(e1) / args.steps
-    ms_step = max(ms_gpu, wall * 1000.0 / args.steps)
+    ms_step, kernel_ms = front_end_workload(J, torch, fe, stream, d_rgb, p, args.steps, args.warmup)
+    log(f"front end {w}x{h}: {ms_step:.3f} ms per step")
+    px = w * h
 
-    # Per-stage device time with events on the context's stream.
-    xyb = fe.xyb_buffer(w, h)
-    qf2 = fe.empty((yb, xb), torch.uint8)
-    stage_ms = {k: 0.0 for k in STAGE_BYTES_PER_PX}
-    reps = max(3, min(args.steps, 10))
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
-    for _ in range(reps):
-        ev[0].record(stream)
-        fe.xyb(d_rgb, w, h, fused=bool(p.fused), out=xyb)
-        ev[1].record(stream)
-        fe.fill_quant_field(xyb, w, h, p.distance, 1.0 / p.scale, bool(p.fused), out=qf2)
-        ev[2].record(stream)
-        fe.fill_cmap(xyb, w, h, p, ytox, ytob)
-        ev[3].record(stream)
-        fe.select_strategy(xyb, w, h, qf2, p, ytox, ytob, cells=cells)
-        ev[4].record(stream)
-        fe.forward_quantize(xyb, w, h, cells, qf2, p, ytox, ytob, outputs=outs)
-        ev[5].record(stream)
-        torch.cuda.synchronize()
-        for i, k in enumerate(stage_ms):
-            stage_ms[k] += ev[i].elapsed_time(ev[i + 1]) / reps
     # Kernels beside the measured chain (not part of the Fast-tier step),
     # each timed alone on the same image with its algorithmic bytes.
     extra = {}
+    reps = max(3, min(args.steps, 10))
+    xyb = fe.xyb(d_rgb, w, h, fused=bool(p.fused))
     gab_out = torch.empty_like(xyb)
 
     def gab():
         fe.gaborish_inverse(xyb, w, h, fused=bool(p.fused), out=gab_out)
-    extra["gaborish"] = time_kernel(gab, stream, reps, EXTRA_BYTES_PER_PX["gaborish"] * w * h)
+    extra["gaborish"] = time_kernel(gab, stream, reps, EXTRA_BYTES_PER_PX["gaborish"] * px)
     # EPF proxy: source = XYB luma, candidate = the gaborish-sharpened luma
     proxy = J.LumaProxy(fe, xyb[1, :h, :w])
     cand = gab_out[1, :h, :w]
     extra["epf_proxy_compare"] = time_kernel(lambda: proxy.compare(cand), stream, reps,
-                                             EXTRA_BYTES_PER_PX["epf_proxy_compare"] * w * h)
+                                             EXTRA_BYTES_PER_PX["epf_proxy_compare"] * px)
     extra["epf_proxy_compare"]["note"] = "host-synchronous call (score read back)"
     proxy.close()
     del gab_out
-    extra.update(dct64_forms(fe, xyb, w, h, stream, reps))
-    if w % 128 == 0 and h % 128 == 0:
+    if not args.profile:
+        extra.update(dct64_forms(fe, xyb, w, h, stream, reps))
         extra.update(dct128_forms(fe, xyb, w, h, stream, reps))
-
-    px = w * h
-    dominant = max(stage_ms, key=stage_ms.get)
-    alg_bytes = STAGE_BYTES_PER_PX[dominant] * px
-    achieved = alg_bytes / (stage_ms[dominant] * 1e-3) / 1e9
-    traffic = None
-    if (w, h, args.quality) == (2048, 2048, 75.0):
-        traffic = read_traffic(dominant)
+    del xyb
+    torch.cuda.empty_cache()
 
     result = {
         "metric": "VarDCT front-end MP/s at q75 (1 GPU vs CPU); ssimulacra2 at equal bytes",
@@ -500,39 +622,36 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic",
+        "data": "synthetic (1/f noise smoothed over 33-129 px, 20 soft-edged alpha ellipses; "
+                "generated on the device, seed 3)",
         "config": {
-            "workload": f"photo_like_{w}x{h}_rgb8_q{args.quality:g}",
-            "width": w, "height": h, "quality": args.quality,
+            "workload": HEADLINE,
+            "width": w, "height": h, "channels": 4, "quality": args.quality,
             "distance": round(p.distance, 4), "speed": "fast",
-            "strategy_search": "square merges DCT8/DCT16X16/DCT32X32 + quant-field aggregation",
+            "decisions": "square merges DCT8/DCT16X16/DCT32X32 + adjust_quant_field; the "
+                         "reference's FAST tier also runs layout_dp::retile_coefficient and "
+                         "refine_quant_field, which are not built",
+            "alpha": "alpha_fringe::clear, fill_invisible, alpha block factors",
             "cfl": "per-tile regression maps (fill_cmap, Fast path)",
+            "tables": "DequantMatrices::new_fast tier 3 (SS2 scales, flat-B DCT8 row)",
             "fmla": "fused" if p.fused else "unfused",
-            "input": "u8 RGB resident in HBM; outputs (levels, f32 coefficients, DC, field, "
-                     "strategy) left in HBM",
+            "input": "u8 RGBA resident in HBM; outputs (levels, f32 coefficients, DC, field, "
+                     "strategy, CfL maps) left in HBM",
         },
-        "roofline": {
-            "kernel": dominant,
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": int(alg_bytes),
-        },
-        "stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "roofline": roofline_of(kernel_ms, px),
+        "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "other_kernels": extra,
+        "photo_2048": photo_res,
     }
     if not args.profile:
-        result["host_buffer_path"] = host_path_rate(fe, rgb, p)
+        result["host_pinned_pipeline"] = host_pipeline_rate(J, torch, fe, d_rgb, p)
         if args.batch > 0:
             result["batch"] = batch_workload(J, torch, fe, stream, args.batch, 3)
         if not args.no_tune:
             result["qf_tune"] = tune_workload(J, torch, fe, stream)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        result["cpu_baseline"] = cpu_baseline(rgb, p, threads)
+        result["cpu_baseline"] = cpu_baseline(d_rgb.cpu().numpy(), p, threads)
     else:
         result["cpu_baseline"] = None
     fe.close()

@@ -47,6 +47,9 @@ extern "C" {
 #define JXG_ERR_ALLOC (-5)            /* EncodeError::AllocationFailed    */
 #define JXG_ERR_DEVICE (-6)           /* HIP runtime / launch failure     */
 #define JXG_ERR_NO_DEVICE (-7)        /* no gfx950 device visible         */
+#define JXG_ERR_UNSUPPORTED (-8)      /* a speed tier whose transform-layout
+                                         decisions this library does not run
+                                         (MEDIUM and slower)              */
 
 /* Largest accepted image side (the format's 2^30 limit is far above what a
  * single HBM-resident image needs; this bound keeps block counts in int32). */
@@ -241,6 +244,28 @@ void jxg_ctx_destroy(jxg_ctx *ctx);
 int jxg_ctx_synchronize(jxg_ctx *ctx);
 const char *jxg_last_error(void);
 
+/* Per-kernel timing of the front end for measurement: with profiling on,
+ * jxg_front_end_dev records a pair of HIP events around each of its kernels
+ * (or short kernel chains) on the stream that kernel runs on, for up to
+ * JXG_PROF_CAP calls.  jxg_ctx_profile(ctx, 1) resets and turns it on, 0
+ * turns it off.  jxg_ctx_profile_read synchronizes and returns, per stage,
+ * the summed milliseconds and the number of timed launches. */
+enum {
+  JXG_PROF_XYB = 0,        /* k_xyb                                    */
+  JXG_PROF_ALPHA = 1,      /* alpha fringe + fill_invisible (2/4 ch)   */
+  JXG_PROF_CMAP = 2,       /* k_cmap_blocks + k_cmap_solve (side stream) */
+  JXG_PROF_AQ_PRE = 3,     /* k_aq_pre                                 */
+  JXG_PROF_AQ_BLOCK = 4,   /* k_aq_block                               */
+  JXG_PROF_SELECT = 5,     /* k_select                                 */
+  JXG_PROF_ALPHA_QF = 6,   /* alpha block factors on the final field   */
+  JXG_PROF_QUANTIZE = 7,   /* k_quantize                               */
+  JXG_PROF_QUANTIZE_XT = 8,/* k_quantize_xt                            */
+  JXG_PROF_N = 9
+};
+#define JXG_PROF_CAP 64
+int jxg_ctx_profile(jxg_ctx *ctx, int enable);
+int jxg_ctx_profile_read(jxg_ctx *ctx, double total_ms[JXG_PROF_N], uint32_t count[JXG_PROF_N]);
+
 /* Uploads a quant-matrix blob (host pointer) into the context; the CfL,
  * strategy and quantize stages use it for every frame until it is replaced,
  * instead of the per-frame default tables (jxg_default_quant_tables).  A NULL
@@ -318,6 +343,16 @@ int jxg_strategy_cost_sites_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t strid
                                 const uint32_t *d_sites, const float *d_qac, float meta_r,
                                 float *d_cost);
 
+/* Transform-layout decision over square merges (DCT8 / DCT16X16 /
+ * DCT32X32) and the quant-field aggregation over each merged transform, for
+ * speeds ULTRAFAST (every block DCT8, square_merges off), FASTEST and FAST;
+ * MEDIUM and slower return JXG_ERR_UNSUPPORTED (their rectangle, sub-8,
+ * DCT64, rerank and layout-DP decisions are not built; jxg_strategy_cost_sites_dev
+ * gives their candidate costs).  At FASTEST / FAST the reference follows the
+ * merges with layout_dp::retile_coefficient (src/ac_strategy/selection.rs:2194,
+ * layout_dp.rs:2381) and, at FAST for d >= 1.5, refine_quant_field (:2456,
+ * :801); neither runs here, so the cells and field are those of the merge
+ * core and adjust_quant_field. */
 /* Transform-layout decision over square merges (DCT8 / DCT16X16 /
  * DCT32X32) and the quant-field aggregation over each merged transform.
  * Replaces the square-merge core of fill_ac_strategy

@@ -180,7 +180,23 @@ class FrontEndOutput:
     alpha: np.ndarray = None  # (H, W) uint8 alpha plane as the stages used it
 
 
-def _ptr(t):
+def _dev_ptr(t, device, dtype, numel, what):
+    """The device address of torch tensor `t` once it is checked to be what
+    the C-ABI entry point will read or write: a tensor on `device` of `dtype`,
+    contiguous, with at least `numel` elements.  JxgError (invalid argument)
+    otherwise, before any launch: the library itself checks only for null
+    pointers and plane geometry."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise JxgError(-1, f"{what}: expected a torch tensor, got {type(t).__name__}")
+    if not t.is_cuda or t.device != device:
+        raise JxgError(-1, f"{what}: expected a tensor on {device}, got {t.device}")
+    if t.dtype != dtype:
+        raise JxgError(-1, f"{what}: expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise JxgError(-1, f"{what}: expected a contiguous tensor")
+    if t.numel() < numel:
+        raise JxgError(-1, f"{what}: {t.numel()} elements, the stage needs {numel}")
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -219,6 +235,67 @@ class FrontEnd:
     def synchronize(self):
         check(lib().jxg_ctx_synchronize(self.ctx))
 
+    PROFILE_STAGES = ("xyb", "alpha", "cmap", "aq_pre", "aq_block", "select", "alpha_qf",
+                      "quantize", "quantize_xt")  # JXG_PROF_* order
+
+    def profile(self, enable=True):
+        """Turns the library's per-kernel event timing on (and resets it) or off."""
+        check(lib().jxg_ctx_profile(self.ctx, 1 if enable else 0))
+
+    def profile_read(self):
+        """{stage: (mean ms per timed launch, launches)} of the timed front-end calls."""
+        tot = np.zeros(len(self.PROFILE_STAGES), np.float64)
+        cnt = np.zeros(len(self.PROFILE_STAGES), np.uint32)
+        check(lib().jxg_ctx_profile_read(self.ctx, tot.ctypes.data, cnt.ctypes.data))
+        return {k: (float(t / n) if n else 0.0, int(n))
+                for k, t, n in zip(self.PROFILE_STAGES, tot, cnt)}
+
+    # ---- checked device pointers ----
+    def _buf(self, t, dtype, numel, what):
+        return _dev_ptr(t, self.device, dtype, numel, what)
+
+    def _planes(self, xyb, width, height, what="xyb"):
+        """(3, padded_h, stride) float32 planes: pointer, stride, padded_h."""
+        t = self.torch
+        if not isinstance(xyb, t.Tensor) or xyb.dim() != 3 or xyb.shape[0] != 3:
+            raise JxgError(-1, f"{what}: expected (3, padded_h, stride) planes")
+        return (self._buf(xyb, t.float32, 0, what), int(xyb.shape[2]), int(xyb.shape[1]))
+
+    def _image(self, rgb, width, height, what="rgb"):
+        """(H, W, C) uint8 image with whole-pixel rows (a row stride of its
+        own is allowed): pointer, row bytes, channels."""
+        t = self.torch
+        if not isinstance(rgb, t.Tensor) or rgb.dim() != 3:
+            raise JxgError(-1, f"{what}: expected an (H, W, C) tensor")
+        if not rgb.is_cuda or rgb.device != self.device:
+            raise JxgError(-1, f"{what}: expected a tensor on {self.device}, got {rgb.device}")
+        if rgb.dtype != t.uint8:
+            raise JxgError(-1, f"{what}: expected torch.uint8, got {rgb.dtype}")
+        h, w, ch = (int(v) for v in rgb.shape)
+        if h < height or w < width or ch not in (1, 2, 3, 4):
+            raise JxgError(-1, f"{what}: shape {tuple(rgb.shape)} for a {width}x{height} image")
+        if rgb.stride(2) != 1 or rgb.stride(1) != ch or rgb.stride(0) < w * ch:
+            raise JxgError(-1, f"{what}: pixels must be contiguous (strides {rgb.stride()})")
+        return ctypes.c_void_p(rgb.data_ptr()), rgb.stride(0), ch
+
+    def _blocks(self, width, height):
+        return ((width + 7) // 8) * ((height + 7) // 8)
+
+    def _tiles(self, width, height):
+        return ((width + 63) // 64) * ((height + 63) // 64)
+
+    def _out3(self, ts, dtype, numel, what):
+        if len(ts) != 3:
+            raise JxgError(-1, f"{what}: expected 3 tensors")
+        return (ctypes.c_void_p * 3)(*[self._buf(x, dtype, numel, f"{what}[{c}]").value
+                                       for c, x in enumerate(ts)])
+
+    def _outputs_arrays(self, o, width, height):
+        t, nb = self.torch, self._blocks(width, height)
+        return (self._out3(o["levels"], t.int32, nb * 64, "levels"),
+                self._out3(o["coeffs"], t.float32, nb * 64, "coeffs"),
+                self._out3(o["dc"], t.int16, nb, "dc"))
+
     # ---- buffers ----
     def empty(self, shape, dtype):
         return self.torch.empty(shape, dtype=dtype, device=self.device)
@@ -242,18 +319,20 @@ class FrontEnd:
         fwd = XYB_SPEC_FWD if fwd is None else np.ascontiguousarray(fwd, np.float32)
         lut = srgb_lut() if lut is None else np.ascontiguousarray(lut, np.float32)
         out = self.xyb_buffer(width, height) if out is None else out
-        ch = rgb.shape[-1]
-        check(lib().jxg_xyb_dev(self.ctx, _ptr(rgb), rgb.stride(0) * rgb.element_size(), ch, width,
-                                height, fwd.ctypes.data, lut.ctypes.data, order, 1 if fused else 0,
-                                _ptr(out), out.shape[2], out.shape[1]))
+        prgb, row_bytes, ch = self._image(rgb, width, height)
+        pout, s, ph = self._planes(out, width, height, "out")
+        check(lib().jxg_xyb_dev(self.ctx, prgb, row_bytes, ch, width, height, fwd.ctypes.data,
+                                lut.ctypes.data, order, 1 if fused else 0, pout, s, ph))
         return out
 
     def fill_quant_field(self, xyb, width, height, distance, inv_scale, fused=True, out=None):
         xb, yb = (width + 7) // 8, (height + 7) // 8
         out = self.empty((yb, xb), self.torch.uint8) if out is None else out
-        check(lib().jxg_fill_quant_field_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
-                                             height, float(distance), float(inv_scale),
-                                             1 if fused else 0, _ptr(out)))
+        px, s, ph = self._planes(xyb, width, height)
+        check(lib().jxg_fill_quant_field_dev(self.ctx, px, s, ph, width, height, float(distance),
+                                             float(inv_scale), 1 if fused else 0,
+                                             self._buf(out, self.torch.uint8,
+                                                       self._blocks(width, height), "out")))
         return out
 
     def _cmaps(self, width, height, ytox, ytob):
@@ -268,38 +347,54 @@ class FrontEnd:
     def strategy_costs(self, xyb, width, height, qf, params, ytox=None, ytob=None):
         t = self.torch
         ytox, ytob = self._cmaps(width, height, ytox, ytob)
+        nb, nt = self._blocks(width, height), self._tiles(width, height)
         outs = [self.empty(qf.shape, t.float32) for _ in range(3)]
-        check(lib().jxg_strategy_costs_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
-                                           height, _ptr(qf), _ptr(ytox), _ptr(ytob),
-                                           ctypes.byref(params), *[_ptr(o) for o in outs]))
+        px, s, ph = self._planes(xyb, width, height)
+        check(lib().jxg_strategy_costs_dev(self.ctx, px, s, ph, width, height,
+                                           self._buf(qf, t.uint8, nb, "qf"),
+                                           self._buf(ytox, t.int8, nt, "ytox"),
+                                           self._buf(ytob, t.int8, nt, "ytob"),
+                                           ctypes.byref(params),
+                                           *[self._buf(o, t.float32, nb, "costs") for o in outs]))
         return outs
 
     def select_strategy(self, xyb, width, height, qf, params, ytox=None, ytob=None, cells=None):
         """Writes strategy cells; updates `qf` in place (adjust_quant_field)."""
         ytox, ytob = self._cmaps(width, height, ytox, ytob)
-        cells = self.empty(qf.shape, self.torch.uint8) if cells is None else cells
-        check(lib().jxg_select_strategy_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
-                                            height, _ptr(qf), _ptr(ytox), _ptr(ytob),
-                                            ctypes.byref(params), _ptr(cells)))
+        t, nb, nt = self.torch, self._blocks(width, height), self._tiles(width, height)
+        cells = self.empty(qf.shape, t.uint8) if cells is None else cells
+        px, s, ph = self._planes(xyb, width, height)
+        check(lib().jxg_select_strategy_dev(self.ctx, px, s, ph, width, height,
+                                            self._buf(qf, t.uint8, nb, "qf"),
+                                            self._buf(ytox, t.int8, nt, "ytox"),
+                                            self._buf(ytob, t.int8, nt, "ytob"),
+                                            ctypes.byref(params), self._buf(cells, t.uint8, nb, "cells")))
         return cells
 
     def forward_quantize(self, xyb, width, height, cells, qf, params, ytox=None, ytob=None,
                          outputs=None):
         ytox, ytob = self._cmaps(width, height, ytox, ytob)
         o = self.outputs(width, height) if outputs is None else outputs
-        arr = lambda ts: (ctypes.c_void_p * 3)(*[t.data_ptr() for t in ts])
-        check(lib().jxg_forward_quantize_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
-                                             height, _ptr(cells), _ptr(qf), _ptr(ytox), _ptr(ytob),
-                                             ctypes.byref(params), arr(o["levels"]),
-                                             arr(o["coeffs"]), arr(o["dc"])))
+        t, nb, nt = self.torch, self._blocks(width, height), self._tiles(width, height)
+        px, s, ph = self._planes(xyb, width, height)
+        lv, co, dc = self._outputs_arrays(o, width, height)
+        check(lib().jxg_forward_quantize_dev(self.ctx, px, s, ph, width, height,
+                                             self._buf(cells, t.uint8, nb, "cells"),
+                                             self._buf(qf, t.uint8, nb, "qf"),
+                                             self._buf(ytox, t.int8, nt, "ytox"),
+                                             self._buf(ytob, t.int8, nt, "ytob"),
+                                             ctypes.byref(params), lv, co, dc))
         return o
 
     def gaborish_inverse(self, xyb, width, height, mul=GABORISH_MUL, fused=True, out=None):
         """Gaborish pre-inversion of (3, padded_h, stride) planes; in place
         unless `out` (same geometry) is given."""
         out = xyb if out is None else out
-        check(lib().jxg_gaborish_inverse_dev(self.ctx, _ptr(xyb), _ptr(out), xyb.shape[2],
-                                             xyb.shape[1], width, height, float(mul),
+        px, s, ph = self._planes(xyb, width, height)
+        if tuple(out.shape) != tuple(xyb.shape):
+            raise JxgError(-1, f"out: shape {tuple(out.shape)} != {tuple(xyb.shape)}")
+        po, _, _ = self._planes(out, width, height, "out")
+        check(lib().jxg_gaborish_inverse_dev(self.ctx, px, po, s, ph, width, height, float(mul),
                                              1 if fused else 0))
         return out
 
@@ -310,9 +405,10 @@ class FrontEnd:
         the quantize-stage chunk layout with xblocks = 8 ceil(W/64)."""
         n = 64 * 64 * ((width + 63) // 64) * ((height + 63) // 64)
         out = [self.empty(n, self.torch.float32) for _ in range(3)] if out is None else out
-        arr = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in out])
-        check(lib().jxg_dct64_image_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
-                                        height, int(method), 1 if fused else 0, arr))
+        arr = self._out3(out, self.torch.float32, n, "out")
+        px, s, ph = self._planes(xyb, width, height)
+        check(lib().jxg_dct64_image_dev(self.ctx, px, s, ph, width, height, int(method),
+                                        1 if fused else 0, arr))
         return out
 
     def strategy_cost_sites(self, xyb, width, height, params, strategy, sites, qac,
@@ -327,47 +423,65 @@ class FrontEnd:
         n = int(packed.shape[0])
         out = self.empty(n, t.float32) if out is None else out
         ytox, ytob = self._cmaps(width, height, ytox, ytob)
-        check(lib().jxg_strategy_cost_sites_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1],
-                                                width, height, _ptr(ytox), _ptr(ytob),
+        nt = self._tiles(width, height)
+        px, s, ph = self._planes(xyb, width, height)
+        check(lib().jxg_strategy_cost_sites_dev(self.ctx, px, s, ph, width, height,
+                                                self._buf(ytox, t.int8, nt, "ytox"),
+                                                self._buf(ytob, t.int8, nt, "ytob"),
                                                 ctypes.byref(params), int(strategy), n,
-                                                _ptr(packed), _ptr(qac), float(meta_r), _ptr(out)))
+                                                self._buf(packed, t.int32, n, "sites"),
+                                                self._buf(qac, t.float32, n, "qac"), float(meta_r),
+                                                self._buf(out, t.float32, n, "out")))
         return out
 
     def chroma_hf_protection(self, xyb, width, height, qf, distance, fused=True):
         """apply_chroma_hf_protection on a device quant field, in place."""
-        check(lib().jxg_chroma_hf_protection_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1],
-                                                 width, height, float(distance),
-                                                 1 if fused else 0, _ptr(qf)))
+        px, s, ph = self._planes(xyb, width, height)
+        check(lib().jxg_chroma_hf_protection_dev(self.ctx, px, s, ph, width, height, float(distance),
+                                                 1 if fused else 0,
+                                                 self._buf(qf, self.torch.uint8,
+                                                           self._blocks(width, height), "qf")))
         return qf
 
     def alpha_fringe_clear(self, alpha, width, height):
         """alpha_fringe::clear of a (H, W) uint8 device plane, in place."""
-        check(lib().jxg_alpha_fringe_clear_dev(self.ctx, _ptr(alpha), width, height))
+        check(lib().jxg_alpha_fringe_clear_dev(self.ctx, self._buf(alpha, self.torch.uint8,
+                                                                   width * height, "alpha"),
+                                               width, height))
         return alpha
 
     def fill_invisible(self, xyb, width, height, alpha):
         """fill_invisible of (3, padded_h, stride) planes under a (H, W) uint8
         alpha plane, in place (edge padding rewritten)."""
-        check(lib().jxg_fill_invisible_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
-                                           height, _ptr(alpha)))
+        px, s, ph = self._planes(xyb, width, height)
+        check(lib().jxg_fill_invisible_dev(self.ctx, px, s, ph, width, height,
+                                           self._buf(alpha, self.torch.uint8, width * height,
+                                                     "alpha")))
         return xyb
 
     def alpha_block_factors(self, alpha, width, height, out=None):
         t = self.torch
         out = self.empty(((height + 7) // 8, (width + 7) // 8), t.float32) if out is None else out
-        check(lib().jxg_alpha_block_factors_dev(self.ctx, _ptr(alpha), width, height, _ptr(out)))
+        check(lib().jxg_alpha_block_factors_dev(self.ctx,
+                                                self._buf(alpha, t.uint8, width * height, "alpha"),
+                                                width, height,
+                                                self._buf(out, t.float32, self._blocks(width, height),
+                                                          "out")))
         return out
 
     def apply_alpha_factors(self, cells, factors, width, height, qf):
         """apply_alpha_factors on a device quant field, in place."""
-        check(lib().jxg_apply_alpha_factors_dev(self.ctx, _ptr(cells), _ptr(factors), width,
-                                                height, _ptr(qf)))
+        t, nb = self.torch, self._blocks(width, height)
+        check(lib().jxg_apply_alpha_factors_dev(self.ctx, self._buf(cells, t.uint8, nb, "cells"),
+                                                self._buf(factors, t.float32, nb, "factors"), width,
+                                                height, self._buf(qf, t.uint8, nb, "qf")))
         return qf
 
     def alpha_plane(self, width, height):
         """The alpha plane the last front end call used, as a device tensor."""
         out = self.empty((height, width), self.torch.uint8)
-        check(lib().jxg_alpha_plane(self.ctx, _ptr(out), width * height, 0))
+        check(lib().jxg_alpha_plane(self.ctx, self._buf(out, self.torch.uint8, width * height, "out"),
+                                    width * height, 0))
         return out
 
     def dct128_image(self, xyb, width, height, method=0, fused=True, out=None):
@@ -377,9 +491,10 @@ class FrontEnd:
         tensors in the chunk layout with xblocks = 16 ceil(W/128)."""
         n = 16384 * ((width + 127) // 128) * ((height + 127) // 128)
         out = [self.empty(n, self.torch.float32) for _ in range(3)] if out is None else out
-        arr = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in out])
-        check(lib().jxg_dct128_image_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
-                                         height, int(method), 1 if fused else 0, arr))
+        arr = self._out3(out, self.torch.float32, n, "out")
+        px, s, ph = self._planes(xyb, width, height)
+        check(lib().jxg_dct128_image_dev(self.ctx, px, s, ph, width, height, int(method),
+                                         1 if fused else 0, arr))
         return out
 
     def xyb_buffer128(self, width, height):
@@ -398,8 +513,11 @@ class FrontEnd:
         n = ((height + 63) // 64, (width + 63) // 64)
         ytox = self.empty(n, t.int8) if ytox is None else ytox
         ytob = self.empty(n, t.int8) if ytob is None else ytob
-        check(lib().jxg_fill_cmap_dev(self.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
-                                      height, ctypes.byref(params), _ptr(ytox), _ptr(ytob)))
+        nt = self._tiles(width, height)
+        px, s, ph = self._planes(xyb, width, height)
+        check(lib().jxg_fill_cmap_dev(self.ctx, px, s, ph, width, height, ctypes.byref(params),
+                                      self._buf(ytox, t.int8, nt, "ytox"),
+                                      self._buf(ytob, t.int8, nt, "ytob")))
         return ytox, ytob
 
     def run(self, rgb, width, height, params, outputs=None, qf=None, cells=None, ytox=None,
@@ -415,11 +533,14 @@ class FrontEnd:
         cells = self.empty((yb, xb), t.uint8) if cells is None else cells
         ytox = self.empty(n, t.int8) if ytox is None else ytox
         ytob = self.empty(n, t.int8) if ytob is None else ytob
-        arr = lambda ts: (ctypes.c_void_p * 3)(*[x.data_ptr() for x in ts])
-        check(lib().jxg_front_end_dev(self.ctx, _ptr(rgb), rgb.stride(0) * rgb.element_size(),
-                                      rgb.shape[-1], width, height, ctypes.byref(params),
-                                      _ptr(ytox), _ptr(ytob), _ptr(qf), _ptr(cells),
-                                      arr(o["levels"]), arr(o["coeffs"]), arr(o["dc"])))
+        nb, nt = xb * yb, self._tiles(width, height)
+        prgb, row_bytes, ch = self._image(rgb, width, height)
+        lv, co, dc = self._outputs_arrays(o, width, height)
+        check(lib().jxg_front_end_dev(self.ctx, prgb, row_bytes, ch, width, height,
+                                      ctypes.byref(params), self._buf(ytox, t.int8, nt, "ytox"),
+                                      self._buf(ytob, t.int8, nt, "ytob"),
+                                      self._buf(qf, t.uint8, nb, "qf"),
+                                      self._buf(cells, t.uint8, nb, "cells"), lv, co, dc))
         o = dict(o)
         o["qf"] = qf
         o["cells"] = cells
@@ -455,8 +576,9 @@ class Batch:
             o["ytob"] = fe.empty(tiles, t.int8)
             self.outs.append(o)
             d = arr[i]
-            d.d_rgb = rgb.data_ptr()
-            d.row_bytes = rgb.stride(0) * rgb.element_size()
+            prgb, row_bytes, _ = fe._image(rgb, w, h, f"images[{i}]")
+            d.d_rgb = prgb.value
+            d.row_bytes = row_bytes
             d.channels = ch
             d.width, d.height = w, h
             d.params = params
@@ -518,8 +640,10 @@ class QfTuner:
         self.keep = (xyb, cells)  # the tuner reads them at every step
         self.width, self.height = width, height
         self.handle = ctypes.c_void_p()
-        check(lib().jxg_qf_tuner_create(fe.ctx, _ptr(xyb), xyb.shape[2], xyb.shape[1], width,
-                                        height, _ptr(cells), ctypes.byref(params), float(lam),
+        px, s, ph = fe._planes(xyb, width, height)
+        pc = fe._buf(cells, fe.torch.uint8, fe._blocks(width, height), "cells")
+        check(lib().jxg_qf_tuner_create(fe.ctx, px, s, ph, width,
+                                        height, pc, ctypes.byref(params), float(lam),
                                         int(seed), int(max_steps), ctypes.byref(self.handle)))
 
     def excluded(self):
@@ -532,7 +656,9 @@ class QfTuner:
         theta = theta.to(t.float32).contiguous()
         grad = t.empty_like(theta)
         loss = ctypes.c_float()
-        check(lib().jxg_qf_tuner_loss_grad(self.handle, _ptr(theta), int(step), _ptr(grad),
+        nb = self.fe._blocks(self.width, self.height)
+        check(lib().jxg_qf_tuner_loss_grad(self.handle, self.fe._buf(theta, t.float32, nb, "theta"),
+                                           int(step), self.fe._buf(grad, t.float32, nb, "grad"),
                                            ctypes.byref(loss)))
         return float(loss.value), grad
 
@@ -541,8 +667,10 @@ class QfTuner:
         out = t.empty_like(qf)
         theta = t.empty(qf.shape, dtype=t.float32, device=qf.device)
         trace = (ctypes.c_float * (steps + 1))()
-        check(lib().jxg_qf_tuner_run(self.handle, _ptr(qf), int(steps), float(lr), _ptr(out),
-                                     _ptr(theta), trace))
+        nb = self.fe._blocks(self.width, self.height)
+        check(lib().jxg_qf_tuner_run(self.handle, self.fe._buf(qf, t.uint8, nb, "qf"), int(steps),
+                                     float(lr), self.fe._buf(out, t.uint8, nb, "out"),
+                                     self.fe._buf(theta, t.float32, nb, "theta"), trace))
         return out, theta, [float(v) for v in trace]
 
     def close(self):
@@ -567,14 +695,26 @@ class LumaProxy:
         h, w = y.shape
         self.shape = (h, w)
         self.ptr = ctypes.c_void_p()
-        check(lib().jxg_epf_proxy_new(fe.ctx, _ptr(y), y.stride(0), w, h, ctypes.byref(self.ptr)))
+        check(lib().jxg_epf_proxy_new(fe.ctx, self._rows(y), y.stride(0), w, h,
+                                      ctypes.byref(self.ptr)))
+
+    def _rows(self, y):
+        """(H, W) float32 rows with unit column stride on the context's device."""
+        t = self.fe.torch
+        if not isinstance(y, t.Tensor) or y.dim() != 2:
+            raise JxgError(-1, "y: expected an (H, W) tensor")
+        if not y.is_cuda or y.device != self.fe.device or y.dtype != t.float32:
+            raise JxgError(-1, f"y: expected float32 on {self.fe.device}, got {y.dtype} on {y.device}")
+        if y.stride(1) != 1 or y.stride(0) < y.shape[1]:
+            raise JxgError(-1, f"y: rows must have unit column stride (strides {y.stride()})")
+        return ctypes.c_void_p(y.data_ptr())
 
     def compare(self, y):
         if tuple(y.shape) != self.shape:
             raise JxgError(-1, f"candidate shape {tuple(y.shape)} != {self.shape}")
         score = ctypes.c_float()
         terms = np.zeros(6, np.float32)
-        check(lib().jxg_epf_proxy_compare(self.ptr, _ptr(y), y.stride(0), ctypes.byref(score),
+        check(lib().jxg_epf_proxy_compare(self.ptr, self._rows(y), y.stride(0), ctypes.byref(score),
                                           terms.ctypes.data))
         return float(score.value), terms
 

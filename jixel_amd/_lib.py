@@ -14,6 +14,7 @@ ERRORS = {
     -5: "allocation failed",
     -6: "device error",
     -7: "no gfx950 device",
+    -8: "unsupported speed tier",
 }
 
 EXPORTED = [
@@ -31,7 +32,7 @@ EXPORTED = [
     "jxg_batch_launch", "jxg_batch_status", "jxg_batch_autotune", "jxg_batch_config",
     "jxg_qf_tuner_create", "jxg_qf_tuner_destroy", "jxg_qf_tuner_excluded",
     "jxg_qf_tuner_loss_grad", "jxg_qf_tuner_run", "jxg_dct128_image_dev",
-    "jxg_fill_quant_field_noalloc_dev",
+    "jxg_fill_quant_field_noalloc_dev", "jxg_ctx_profile", "jxg_ctx_profile_read",
 ]
 
 
@@ -135,6 +136,8 @@ def _declare(L):
     L.jxg_fill_quant_field_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
                                            c.c_float, c.c_float, c.c_int, vp]
     L.jxg_fill_quant_field_noalloc_dev.argtypes = L.jxg_fill_quant_field_dev.argtypes
+    L.jxg_ctx_profile.argtypes = [vp, c.c_int]
+    L.jxg_ctx_profile_read.argtypes = [vp, vp, vp]
     L.jxg_strategy_costs_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,
                                          vp, vp, vp, c.POINTER(FrameParams), vp, vp, vp]
     L.jxg_select_strategy_dev.argtypes = [vp, vp, c.c_uint32, c.c_uint32, c.c_uint32, c.c_uint32,

@@ -135,6 +135,7 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
                      "image %u: every image of a batch shares params.xyb_order and xyb_fwd", i);
     if (im.params.xyb_order != JXG_XYB_SCALAR && im.params.xyb_order != JXG_XYB_AVX2)
       return set_err(JXG_ERR_INVALID_ARG, "image %u: xyb_order %d", i, im.params.xyb_order);
+    if ((rc = check_speed(&im.params))) return rc;
     if (im.params.flags & JXG_FLAG_CHROMA_HF)
       return set_err(JXG_ERR_INVALID_ARG, "image %u: chroma-HF protection is single-image only", i);
   }

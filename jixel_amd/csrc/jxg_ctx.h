@@ -87,9 +87,27 @@ struct jxg_ctx {
   size_t alpha2_cap = 0;
   uint32_t *d_vis = nullptr;  // visibility bits
   size_t vis_cap = 0;
+  uint32_t *d_fxchg = nullptr;  // fill_invisible hand-over rows (3 x bands x W)
+  size_t fxchg_cap = 0;
   float *d_afac = nullptr;  // alpha block factors
   size_t afac_cap = 0;
+  // jxg_ctx_profile: event pairs per stage, recorded while prof_on
+  bool prof_on = false;
+  hipEvent_t *prof_ev = nullptr;  // JXG_PROF_N * 2 * JXG_PROF_CAP
+  uint32_t prof_n[JXG_PROF_N] = {};
 };
+
+// Event pair around a profiled kernel (no-ops unless profiling is on).
+inline void prof_begin(jxg_ctx *ctx, int k, hipStream_t st) {
+  if (ctx->prof_on && ctx->prof_n[k] < JXG_PROF_CAP)
+    (void)hipEventRecord(ctx->prof_ev[(k * JXG_PROF_CAP + ctx->prof_n[k]) * 2], st);
+}
+inline void prof_end(jxg_ctx *ctx, int k, hipStream_t st) {
+  if (ctx->prof_on && ctx->prof_n[k] < JXG_PROF_CAP) {
+    (void)hipEventRecord(ctx->prof_ev[(k * JXG_PROF_CAP + ctx->prof_n[k]) * 2 + 1], st);
+    ctx->prof_n[k]++;
+  }
+}
 
 template <typename T>
 inline int ensure(T **p, size_t *cap, size_t n) {
@@ -105,6 +123,8 @@ inline int ensure(T **p, size_t *cap, size_t n) {
 // Shared argument checks and per-frame uploads (jxg_kernels.hip).
 JXG_HIDDEN int check_image(uint32_t w, uint32_t h);
 JXG_HIDDEN int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t padded_h);
+// JXG_ERR_UNSUPPORTED for speeds whose layout decisions are not built.
+JXG_HIDDEN int check_speed(const jxg_frame_params *p);
 JXG_HIDDEN int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p);
 // The quant tables (blob and scan-ordered inverse weights) the stages use for
 // a frame with parameters p: the caller's set if one was given, else the

@@ -2541,6 +2541,15 @@ int check_image(uint32_t w, uint32_t h) {
   return JXG_OK;
 }
 
+int check_speed(const jxg_frame_params *p) {
+  if (!p) return set_err(JXG_ERR_INVALID_ARG, "null params");
+  if (p->speed > JXG_SPEED_FAST)
+    return set_err(JXG_ERR_UNSUPPORTED,
+                   "speed %d: the transform-layout decisions of MEDIUM and slower tiers are not "
+                   "built (ULTRAFAST, FASTEST and FAST are)", p->speed);
+  return JXG_OK;
+}
+
 int check_planes(uint32_t w, uint32_t h, uint32_t stride, uint32_t padded_h) {
   uint32_t s, ph;
   jxg_plane_geometry(w, h, &s, &ph);
@@ -2731,13 +2740,49 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_alpha);
   (void)hipFree(ctx->d_alpha2);
   (void)hipFree(ctx->d_vis);
+  (void)hipFree(ctx->d_fxchg);
   (void)hipFree(ctx->d_afac);
   (void)hipFree(ctx->d_zero_cmap);
   (void)hipFree(ctx->d_rgb);
   (void)hipFree(ctx->d_tmp);
   (void)hipFree(ctx->d_cmap);
   (void)hipFree(ctx->d_c64);
+  if (ctx->prof_ev) {
+    for (int i = 0; i < JXG_PROF_N * 2 * JXG_PROF_CAP; i++)
+      if (ctx->prof_ev[i]) (void)hipEventDestroy(ctx->prof_ev[i]);
+    delete[] ctx->prof_ev;
+  }
   delete ctx;
+}
+
+int jxg_ctx_profile(jxg_ctx *ctx, int enable) {
+  if (!ctx) return set_err(JXG_ERR_INVALID_ARG, "null ctx");
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (enable && !ctx->prof_ev) {
+    const int n = JXG_PROF_N * 2 * JXG_PROF_CAP;
+    ctx->prof_ev = new hipEvent_t[n]();
+    for (int i = 0; i < n; i++) HIP_TRY(hipEventCreate(&ctx->prof_ev[i]));
+  }
+  for (int k = 0; k < JXG_PROF_N; k++) ctx->prof_n[k] = 0;
+  ctx->prof_on = enable != 0;
+  return JXG_OK;
+}
+
+int jxg_ctx_profile_read(jxg_ctx *ctx, double total_ms[JXG_PROF_N], uint32_t count[JXG_PROF_N]) {
+  if (!ctx || !total_ms || !count) return set_err(JXG_ERR_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipDeviceSynchronize());
+  for (int k = 0; k < JXG_PROF_N; k++) {
+    total_ms[k] = 0.0;
+    count[k] = ctx->prof_ev ? ctx->prof_n[k] : 0;
+    for (uint32_t i = 0; i < count[k]; i++) {
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, ctx->prof_ev[(k * JXG_PROF_CAP + i) * 2],
+                                  ctx->prof_ev[(k * JXG_PROF_CAP + i) * 2 + 1]));
+      total_ms[k] += ms;
+    }
+  }
+  return JXG_OK;
 }
 
 int jxg_ctx_synchronize(jxg_ctx *ctx) {
@@ -2799,17 +2844,21 @@ static int fill_qf_into(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint3
   const int pre_w = 2 * xb, pre_h = 2 * yb;
   const float *yp = d_xyb + (size_t)stride * padded_h;
   const dim3 g1((4 * pre_w + 255) / 256, pre_h);  // four lanes per pre-map value
+  prof_begin(ctx, JXG_PROF_AQ_PRE, ctx->stream);
   if (fused)
     k_aq_pre<true><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, pre);
   else
     k_aq_pre<false><<<g1, 256, 0, ctx->stream>>>(yp, stride, W, H, pre_w, pre_h, pre);
   LAUNCH_CHECK("k_aq_pre");
+  prof_end(ctx, JXG_PROF_AQ_PRE, ctx->stream);
   const dim3 g2((xb + AQ_STRIP - 1) / AQ_STRIP, yb);
+  prof_begin(ctx, JXG_PROF_AQ_BLOCK, ctx->stream);
   if (fused)
     k_aq_block<true><<<g2, 256, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
   else
     k_aq_block<false><<<g2, 256, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, pre, pre_w, distance, inv_scale, ctx->d_exp2, d_qf, xb, yb);
   LAUNCH_CHECK("k_aq_block");
+  prof_end(ctx, JXG_PROF_AQ_BLOCK, ctx->stream);
   return JXG_OK;
 }
 
@@ -2864,11 +2913,13 @@ static int select_impl(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32
   if ((rc = tables_for(ctx, p, &tables, &selq))) return rc;
   const int xb = (W + 7) / 8, yb = (H + 7) / 8;
   const dim3 grid((xb + 3) / 4, (yb + 3) / 4);
+  prof_begin(ctx, JXG_PROF_SELECT, ctx->stream);
   if (p->fused)
     k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
   else
     k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
   LAUNCH_CHECK("k_select");
+  prof_end(ctx, JXG_PROF_SELECT, ctx->stream);
   return JXG_OK;
 }
 
@@ -2901,6 +2952,7 @@ int jxg_select_strategy_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, u
   int rc = check_image(width, height);
   if (rc) return rc;
   if ((rc = check_planes(width, height, stride, padded_h))) return rc;
+  if ((rc = check_speed(params))) return rc;
   return select_impl(ctx, d_xyb, stride, padded_h, width, height, d_qf, d_ytox, d_ytob, params,
                      d_cells, nullptr, nullptr, nullptr);
 }
@@ -2955,17 +3007,21 @@ int jxg_forward_quantize_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, 
   A.xt_list = ctx->d_xt;
   A.xt_cap = (uint32_t)cap;
   const dim3 grid((xb + 7) / 8, (yb + 7) / 8);
+  prof_begin(ctx, JXG_PROF_QUANTIZE, ctx->stream);
   if (params->fused)
     k_quantize<true><<<grid, 256, 0, ctx->stream>>>(A);
   else
     k_quantize<false><<<grid, 256, 0, ctx->stream>>>(A);
   LAUNCH_CHECK("k_quantize");
+  prof_end(ctx, JXG_PROF_QUANTIZE, ctx->stream);
   const int xt_grid = 256;
+  prof_begin(ctx, JXG_PROF_QUANTIZE_XT, ctx->stream);
   if (params->fused)
     k_quantize_xt<true><<<xt_grid, 256, 0, ctx->stream>>>(A);
   else
     k_quantize_xt<false><<<xt_grid, 256, 0, ctx->stream>>>(A);
   LAUNCH_CHECK("k_quantize_xt");
+  prof_end(ctx, JXG_PROF_QUANTIZE_XT, ctx->stream);
   return JXG_OK;
 }
 
@@ -3031,6 +3087,7 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
   if (!ctx || !d_rgb || !params || !d_qf || !d_cells) return set_err(JXG_ERR_INVALID_ARG, "null argument");
   int rc = check_image(width, height);
   if (rc) return rc;
+  if ((rc = check_speed(params))) return rc;
   uint32_t stride, ph;
   jxg_plane_geometry(width, height, &stride, &ph);
   if ((rc = ensure(&ctx->d_xyb, &ctx->xyb_cap, (size_t)3 * stride * ph))) return rc;
@@ -3042,9 +3099,11 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
   }
   float lut[256];
   jxg_srgb_lut(lut);
+  prof_begin(ctx, JXG_PROF_XYB, ctx->stream);
   if ((rc = jxg_xyb_dev(ctx, d_rgb, row_bytes, channels, width, height, params->xyb_fwd, lut,
                         params->xyb_order, params->fused, ctx->d_xyb, stride, ph)))
     return rc;
+  prof_end(ctx, JXG_PROF_XYB, ctx->stream);
   // Alpha of gray + alpha and RGBA input: fringe clearing, fill_invisible on
   // XYB and the alpha block factors (src/frame/encoder.rs:60-69,152-157).
   // k_xyb already snapped gray input to X = 0, B = Y; the fill is the same
@@ -3052,9 +3111,11 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
   const bool has_alpha = channels == 2 || channels == 4;
   ctx->alpha_used = 0;
   if (has_alpha) {
+    prof_begin(ctx, JXG_PROF_ALPHA, ctx->stream);
     if ((rc = alpha_front(ctx, d_rgb, row_bytes, channels, width, height, params->flags,
                           ctx->d_xyb, stride, ph)))
       return rc;
+    prof_end(ctx, JXG_PROF_ALPHA, ctx->stream);
     ctx->alpha_used = (size_t)width * height;
   }
   if ((rc = upload_derived(ctx, params))) return rc;
@@ -3063,9 +3124,11 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
   // run on the side stream beside the quant field.
   HIP_TRY(hipEventRecord(ctx->ev_fork, ctx->stream));
   HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+  prof_begin(ctx, JXG_PROF_CMAP, ctx->side);
   if ((rc = fill_cmap(ctx, ctx->d_xyb, stride, ph, width, height, params, d_ytox, d_ytob,
                       ctx->side)))
     return rc;
+  prof_end(ctx, JXG_PROF_CMAP, ctx->side);
   HIP_TRY(hipEventRecord(ctx->ev_join, ctx->side));
   if ((rc = fill_qf(ctx, ctx->d_xyb, stride, ph, width, height, params->distance,
                     ctx->h_derived->inv_scale, params->fused, d_qf)))
@@ -3079,9 +3142,11 @@ int jxg_front_end_dev(jxg_ctx *ctx, const uint8_t *d_rgb, size_t row_bytes, int 
                                     params, d_cells)))
     return rc;
   // once every layout and quant-field decision is made (src/frame/dc_groups.rs:302-312)
-  if (has_alpha && (params->flags & JXG_FLAG_FILL_INVISIBLE) &&
-      (rc = alpha_finish(ctx, width, height, d_cells, d_qf)))
-    return rc;
+  if (has_alpha && (params->flags & JXG_FLAG_FILL_INVISIBLE)) {
+    prof_begin(ctx, JXG_PROF_ALPHA_QF, ctx->stream);
+    if ((rc = alpha_finish(ctx, width, height, d_cells, d_qf))) return rc;
+    prof_end(ctx, JXG_PROF_ALPHA_QF, ctx->stream);
+  }
   return jxg_forward_quantize_dev(ctx, ctx->d_xyb, stride, ph, width, height, d_cells, d_qf, d_ytox,
                                   d_ytob, params, d_levels, d_coeffs, d_dc);
 }

@@ -8,7 +8,8 @@
   amplitude 20 and period 16-64 px in random directions + Gaussian noise
   sigma 4 (the 256x256 q90 workload).
 * smooth_rgba(w, h, seed): 1/f noise box-smoothed over 32-128 px regions with
-  an alpha of 0/255 inside 20 soft-edged ellipses (the 7680x4320 RGBA case).
+  an alpha of 0/255 inside 20 soft-edged ellipses (the 7680x4320 RGBA case);
+  smooth_rgba_device is the same recipe generated on the GPU.
 
 PRNG: numpy PCG64 seeded with the workload's seed; all float math in f64,
 rounded to u8 at the end.
@@ -166,6 +167,59 @@ def smooth_rgba(w, h, seed):
     out[..., :3] = rgb
     out[..., 3] = alpha * 255.0
     return np.clip(np.rint(out), 0, 255).astype(np.uint8)
+
+
+def smooth_rgba_device(w, h, seed, device):
+    """smooth_rgba's recipe on the device (torch, f64): 1/f noise of contrast
+    60 box-smoothed with a seeded radius in [16, 64] (a 33..129 px window),
+    x3 around 128, Gaussian noise sigma 2, and an alpha of 0 / 255 inside 20
+    seeded ellipses with a 4 px linear falloff.  The radius and ellipses are
+    drawn from a PCG64 stream with the workload's seed, the noise fields from
+    torch's generator: the same distribution as smooth_rgba, other pixels.
+    Used for the 7680x4320 benchmark image, whose host generation takes half
+    a minute."""
+    import torch
+    rng = _rng(seed)
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    fy = torch.fft.fftfreq(h, device=device, dtype=torch.float64)[:, None]
+    fx = torch.fft.rfftfreq(w, device=device, dtype=torch.float64)[None, :]
+    k = torch.sqrt(fx * fx + fy * fy)
+    k[0, 0] = 1.0
+    amp = 1.0 / k
+    amp[0, 0] = 0.0
+    chans = []
+    for _ in range(3):
+        phase = torch.rand(amp.shape, generator=g, device=device, dtype=torch.float64) * (2.0 * np.pi)
+        img = torch.fft.irfft2(torch.polar(amp, phase), s=(h, w))
+        chans.append(img / (img.std() + 1e-12) * 60.0)
+    out = torch.stack(chans, dim=2)
+    rgb = 0.6 * out.mean(dim=2, keepdim=True) + 0.4 * out
+    r = int(rng.integers(16, 65))
+    for axis in (0, 1):  # separable running mean with edge replication
+        n = rgb.shape[axis]
+        idx = torch.clamp(torch.arange(-r, n + r, device=device), 0, n - 1)
+        p = torch.index_select(rgb, axis, idx)
+        c = torch.cumsum(p, dim=axis)
+        c = torch.cat([torch.zeros_like(torch.narrow(c, axis, 0, 1)), c], dim=axis)
+        rgb = (torch.narrow(c, axis, 2 * r + 1, n) - torch.narrow(c, axis, 0, n)) / (2 * r + 1)
+    rgb = 128.0 + rgb * 3.0
+    rgb += torch.randn(rgb.shape, generator=g, device=device, dtype=torch.float64) * 2.0
+    alpha = torch.zeros((h, w), device=device, dtype=torch.float64)
+    for _ in range(20):
+        cx, cy = rng.uniform(0, w), rng.uniform(0, h)
+        ax, ay = rng.uniform(w / 40, w / 6), rng.uniform(h / 40, h / 6)
+        x0, x1 = max(0, int(np.floor(cx - ax))), min(w, int(np.ceil(cx + ax)) + 1)
+        y0, y1 = max(0, int(np.floor(cy - ay))), min(h, int(np.ceil(cy + ay)) + 1)
+        if x0 >= x1 or y0 >= y1:
+            continue
+        y = torch.arange(y0, y1, device=device, dtype=torch.float64)[:, None]
+        x = torch.arange(x0, x1, device=device, dtype=torch.float64)[None, :]
+        d = torch.sqrt(((x - cx) / ax) ** 2 + ((y - cy) / ay) ** 2)
+        edge = torch.clamp((1.0 - d) * min(ax, ay) / 4.0, 0.0, 1.0)
+        alpha[y0:y1, x0:x1] = torch.maximum(alpha[y0:y1, x0:x1], edge)
+    img = torch.cat([rgb, (alpha * 255.0)[..., None]], dim=2)
+    return img.round().clamp(0, 255).to(torch.uint8).contiguous()
 
 
 def sha256(img):

@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libjxo.so")
+# JXO_LIB selects another build of the same sources (the benchmark's
+# -march=native CPU-baseline build).
+_LIB_PATH = os.environ.get("JXO_LIB") or os.path.join(_HERE, "_build", "libjxo.so")
 _lib = None
 
 
@@ -20,8 +22,9 @@ def build():
 
 
 def lib():
-    global _lib
+    global _lib, _LIB_PATH
     if _lib is None:
+        _LIB_PATH = os.environ.get("JXO_LIB") or _LIB_PATH
         if not os.path.exists(_LIB_PATH):
             build()
         _lib = ctypes.CDLL(_LIB_PATH)

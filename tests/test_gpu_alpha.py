@@ -122,6 +122,48 @@ def test_fill_invisible_bit_exact(J, fe, torch, oracle, w, h, seed, kind):
         assert same_bits(got[c], pad), c
 
 
+@pytest.mark.parametrize("w,h,seed,scale", [(70, 90, 11, 2.0 ** -118), (150, 200, 12, 2.0 ** -126),
+                                             (33, 140, 13, 2.0 ** -140)])
+def test_fill_invisible_tiny_samples_bit_exact(J, fe, torch, oracle, w, h, seed, scale):
+    """Samples near and below the smallest normal float: some weighted sums
+    fall where the reciprocal division is not exact, and the fill must still
+    equal IEEE division (the oracle's) bit for bit.  The first two rows of
+    bands keep ordinary values, so only some bands meet the tiny sums."""
+    log("fill_invisible tiny", w, h, seed, scale)
+    rng = np.random.default_rng(seed)
+    a = np.where(rng.random((h, w)) < 0.5, 0, 255).astype(np.uint8)
+    s, ph = J.plane_geometry(w, h)
+    planes = rng.uniform(-1.0, 1.0, (3, ph, s))
+    planes[:, min(h, 128):, :] *= scale
+    xyb = torch.from_numpy(planes.astype(np.float32)).cuda()
+    src = xyb[:, :h, :w].cpu().numpy()
+    assert np.count_nonzero(np.abs(src) < 2.0 ** -124) > 0 or scale > 2.0 ** -120
+    want = oracle.fill_invisible(src, a)
+    fe.fill_invisible(xyb, w, h, to_dev(torch, a))
+    got = xyb.cpu().numpy()
+    for c in range(3):
+        assert same_bits(got[c, :h, :w], want[c]), (c, first_mismatch(got[c, :h, :w], want[c]))
+
+
+@pytest.mark.parametrize("offset", [1, 3, 13])
+def test_fill_invisible_unaligned_alpha(J, fe, torch, oracle, offset):
+    """An alpha plane that does not start on a 16-byte boundary (the stats
+    pass reads 16-byte words from the first aligned address)."""
+    w, h = 173, 91
+    log("fill_invisible unaligned alpha", offset)
+    rng = np.random.default_rng(offset)
+    a = np.where(rng.random((h, w)) < 0.3, 0, 255).astype(np.uint8)
+    buf = torch.zeros(w * h + 32, dtype=torch.uint8, device="cuda")
+    buf[offset:offset + w * h] = torch.from_numpy(a.reshape(-1)).cuda()
+    xyb = _xyb_planes(J, fe, torch, w, h, offset)
+    src = xyb[:, :h, :w].cpu().numpy()
+    want = oracle.fill_invisible(src, a)
+    fe.fill_invisible(xyb, w, h, buf[offset:offset + w * h])
+    got = xyb.cpu().numpy()
+    for c in range(3):
+        assert same_bits(got[c, :h, :w], want[c]), (c, first_mismatch(got[c, :h, :w], want[c]))
+
+
 @pytest.mark.parametrize("w,h,seed", [(24, 8, 1), (131, 77, 2), (64, 64, 3)])
 def test_alpha_factors_bit_exact(J, fe, torch, oracle, w, h, seed):
     rng = np.random.default_rng(seed)

@@ -109,3 +109,18 @@ def test_frame_params_carry_the_spec_xyb_matrix():
     assert np.array_equal(out, J.XYB_SPEC_FWD)
     q = J.frame_params(J.EncodeConfig(xyb_order=1).with_arbiter(J.ARBITER_CVVDP))
     assert q.xyb_order == 1 and q.arbiter == 1
+
+
+def test_device_pointer_check_rejects_host_tensors():
+    """The wrappers' pointer check (the C-ABI checks only for null): a host
+    tensor, a wrong dtype, a strided view and a short buffer are refused
+    before any launch."""
+    import torch
+    import jixel_amd as J
+    from jixel_amd import _dev_ptr
+    dev = torch.device("cuda", 0)
+    t = torch.zeros(16, dtype=torch.uint8)
+    for bad, text in ((t, "expected a tensor on"), ([1, 2], "expected a torch tensor")):
+        with pytest.raises(J.JxgError) as e:
+            _dev_ptr(bad, dev, torch.uint8, 16, "buf")
+        assert e.value.code == -1 and text in str(e.value)

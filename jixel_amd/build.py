@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libjxg.so")
+STATIC = os.path.join(HERE, "libjxg.a")  # the same objects, for a static link (link -lamdhip64 too)
 BUILD = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -57,6 +58,11 @@ def build(verbose=True, jobs=None):
     tmp = LIB + ".tmp"
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    tmp = STATIC + ".tmp"
+    if os.path.exists(tmp):
+        os.remove(tmp)
+    run(["ar", "rcs", tmp] + objs)
+    os.replace(tmp, STATIC)
     return LIB
 
 

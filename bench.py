@@ -33,7 +33,8 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32, TFLOP/s
 # the buffers it must touch once), for RGBA input.
 KERNEL_BYTES_PER_PX = {
     "xyb": 4.0 + 12.0,           # u8 RGBA in, 3 f32 planes out
-    "alpha": 4.0 + 12.0 + 12.0,  # alpha in, 3 planes read and written by fill_invisible
+    "alpha_prep": 4.0 + 1.0,     # RGBA in for its alpha bytes, the alpha plane out
+    "fill": 12.0 + 12.0 + 1.0,   # fill_invisible: 3 planes read and written, alpha in
     "cmap": 12.0,                # 3 planes in, two i8 per 64x64 tile out
     "aq_pre": 4.0 + 1.0 / 4,     # Y plane in, the 1/16-size pre-map out
     "aq_block": 12.0 + 1.0 / 64, # 3 planes in, u8 field out
@@ -55,10 +56,20 @@ EXTRA_BYTES_PER_PX = {
 
 CAL_BYTES = 256 << 20  # calibration copy size for the byte counters
 
-# rocprofv3 kernel name of each profiled stage
-STAGE_KERNEL = {"xyb": "k_xyb", "cmap": "k_cmap_blocks", "aq_pre": "k_aq_pre",
-                "aq_block": "k_aq_block", "select": "k_select", "quantize": "k_quantize",
-                "quantize_xt": "k_quantize_xt"}
+# rocprofv3 kernel names of each profiled stage
+STAGE_KERNEL = {"xyb": ["k_xyb"], "cmap": ["k_cmap_blocks", "k_cmap_solve"],
+                "aq_pre": ["k_aq_pre"], "aq_block": ["k_aq_block"], "select": ["k_select"],
+                "quantize": ["k_quantize"], "quantize_xt": ["k_quantize_xt"],
+                "fill": ["k_fill_bands"],
+                "alpha_prep": ["k_alpha_extract", "k_fringe", "k_fringe_commit", "k_astat_init",
+                               "k_alpha_stats", "k_fill_u32", "k_repad", "k_alpha_blocks"],
+                "alpha_qf": ["k_alpha_apply"]}
+# fill_invisible's recurrence: anti-diagonals x + 2 y are its dependent steps,
+# and each step's chain is 14 dependent VALU operations (one DPP wave shift,
+# 4 fused multiply-adds and 3 adds for the sum, 5 FP64 operations for the
+# quotient, one bit select); a wave64 VALU operation takes 4 cycles.
+FILL_CHAIN_OPS = 14
+CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 HEADLINE = "smooth_rgba_7680x4320_q75"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_smooth_rgba_7680x4320_q75.txt")
@@ -66,17 +77,20 @@ ISSUE_FILE = os.path.join(ROOT, "profiles", "issue_select_quantize_7680x4320_q75
 
 
 def read_traffic(stage):
-    """HBM bytes per launch of the stage's kernel from the committed counter
+    """HBM bytes per launch of the stage's kernels from the committed counter
     summary (FETCH_SIZE + WRITE_SIZE, calibrated against a plain copy)."""
     if not os.path.exists(TRAFFIC_FILE) or stage not in STAGE_KERNEL:
         return None
-    want = STAGE_KERNEL[stage]
+    got = {}
     with open(TRAFFIC_FILE) as f:
         for line in f:
             parts = line.split()
-            if len(parts) >= 3 and parts[0] == "traffic_bytes" and parts[1] == want:
-                return int(parts[2])
-    return None
+            if len(parts) >= 3 and parts[0] == "traffic_bytes":
+                got[parts[1]] = got.get(parts[1], 0) + int(parts[2])
+    names = STAGE_KERNEL[stage]
+    if not all(n in got for n in names if n not in ("k_astat_init",)):
+        return None
+    return sum(got.get(n, 0) for n in names)
 
 
 def read_valu_busy(kernel):
@@ -494,11 +508,15 @@ def front_end_workload(J, torch, fe, stream, d_rgb, p, steps, warmup):
     ms_gpu = e0.elapsed_time(e1) / steps
     ms_step = max(ms_gpu, wall * 1000.0 / steps)
     del outs, qf, cells, ytox, ytob
-    return ms_step, {k: v[0] for k, v in prof.items() if v[1] > 0}
+    k = {n: v[0] for n, v in prof.items() if v[1] > 0}
+    if "alpha" in k and "fill" in k:  # the alpha stage without its fill_invisible kernel
+        k["alpha_prep"] = k.pop("alpha") - k["fill"]
+    return ms_step, k
 
 
-def roofline_of(kernel_ms, px, traffic_stage_ok=True):
+def roofline_of(kernel_ms, px_shape, traffic_stage_ok=True):
     dominant = max(kernel_ms, key=kernel_ms.get)
+    px = px_shape[0] * px_shape[1]
     alg_bytes = KERNEL_BYTES_PER_PX[dominant] * px
     achieved = alg_bytes / (kernel_ms[dominant] * 1e-3) / 1e9
     r = {
@@ -512,13 +530,25 @@ def roofline_of(kernel_ms, px, traffic_stage_ok=True):
         "algorithmic_bytes_per_launch": int(alg_bytes),
         "kernel_ms": round(kernel_ms[dominant], 4),
     }
+    if dominant == "fill":
+        # the recurrence's dependent steps bound it, not HBM: the anti-diagonals
+        # times the per-step chain at 4 cycles per dependent operation
+        w, h = px_shape
+        steps = w + 2 * (h - 1)
+        ideal = steps * FILL_CHAIN_OPS * 4 / (CLOCK_GHZ * 1e9) * 1e3
+        r["limiter"] = ("the recurrence's W + 2 H dependent steps, not HBM (3 x 68 bands of 64 "
+                        "rows, one workgroup each)")
+        r["latency_bound"] = {"anti_diagonals": steps, "dependent_ops_per_step": FILL_CHAIN_OPS,
+                              "cycles_per_op": 4, "clock_ghz": CLOCK_GHZ,
+                              "ideal_ms": round(ideal, 4),
+                              "frac": round(ideal / kernel_ms[dominant], 4)}
     if dominant == "select":
         tflops = SELECT_FLOPS_PER_PX * px / (kernel_ms[dominant] * 1e-3) / 1e12
         r["limiter"] = "VALU issue, not HBM: the candidate transforms and RD sums run in LDS"
         r["fp32"] = {"achieved_tflops": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
                      "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
                      "flops_per_px": SELECT_FLOPS_PER_PX}
-        r["valu_busy"] = read_valu_busy("k_select")
+        r["valu_busy"] = read_valu_busy("k_select") if traffic_stage_ok else None
     return r
 
 
@@ -571,7 +601,7 @@ def main():
         "mp_per_s": round(2048 * 2048 / 1e6 / (photo_ms * 1e-3), 2),
         "ms_per_step": round(photo_ms, 4),
         "kernel_ms": {k: round(v, 4) for k, v in photo_k.items()},
-        "roofline": roofline_of(photo_k, 2048 * 2048, traffic_stage_ok=False),
+        "roofline": roofline_of(photo_k, (2048, 2048), traffic_stage_ok=False),
     }
     if args.photo_only:
         print(json.dumps({"photo": photo_res}), flush=True)
@@ -638,7 +668,7 @@ def main():
             "input": "u8 RGBA resident in HBM; outputs (levels, f32 coefficients, DC, field, "
                      "strategy, CfL maps) left in HBM",
         },
-        "roofline": roofline_of(kernel_ms, px),
+        "roofline": roofline_of(kernel_ms, (w, h)),
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "other_kernels": extra,
         "photo_2048": photo_res,

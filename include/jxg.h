@@ -260,7 +260,8 @@ enum {
   JXG_PROF_ALPHA_QF = 6,   /* alpha block factors on the final field   */
   JXG_PROF_QUANTIZE = 7,   /* k_quantize                               */
   JXG_PROF_QUANTIZE_XT = 8,/* k_quantize_xt                            */
-  JXG_PROF_N = 9
+  JXG_PROF_FILL = 9,       /* k_fill_bands alone (inside JXG_PROF_ALPHA) */
+  JXG_PROF_N = 10
 };
 #define JXG_PROF_CAP 64
 int jxg_ctx_profile(jxg_ctx *ctx, int enable);

@@ -785,9 +785,11 @@ int fill_planes(jxg_ctx *ctx, float *xyb, uint32_t stride, uint32_t ph, uint32_t
   const unsigned fg = (unsigned)min((slots + 1023) / 1024, (size_t)2048);
   k_fill_u32<<<fg, 256, 0, ctx->stream>>>(ctx->d_fxchg, slots, FB_EMPTY);
   LAUNCH_CHECK("k_fill_u32");
+  prof_begin(ctx, JXG_PROF_FILL, ctx->stream);
   k_fill_bands<<<3 * nbands, 512, 0, ctx->stream>>>(xyb, (int)stride, (int)ph, (int)W, (int)H,
                                                     alpha, ctx->d_fxchg, ctx->d_astat);
   LAUNCH_CHECK("k_fill_bands");
+  prof_end(ctx, JXG_PROF_FILL, ctx->stream);
   const size_t pad = (size_t)(stride - W) * H + (size_t)stride * (ph - H);
   if (pad) {
     const unsigned rg = (unsigned)min((pad + 255) / 256, (size_t)1024);

@@ -90,6 +90,15 @@ const float *jxo_wc128(void);
 /* 128-family forward transforms: kind 0 128x128, 1 128 tall x 64 wide,
  * 2 64 tall x 128 wide; in = footprint with stride = its width. */
 void jxo_dct128(int kind, const float *in, float *out);
+/* decision pieces of the strategy search (src/ac_strategy/selection.rs) */
+float jxo_partition_cost(const float children[2][2], const float merges[5], int mask); /* :91 */
+int jxo_choose_merge_partition(const float children[2][2], const float baseline[2][2],
+                               const float merges[5], const float accept[5]);        /* :115 */
+int jxo_choose_square32(const float children[2][2], const float baseline[2][2], float merge,
+                        float accept);
+int jxo_aggregate_quant(int max_q, uint32_t sum, int covered, float distance);     /* :484 */
+float jxo_aggregate_qac_2x2(const float qac[2][2], float scale, float distance);   /* :495 */
+void jxo_cmap_factors(int ytox, int ytob, const jxg_frame_params *p, float out[3]); /* ac_strategy.rs:1841 */
 
 /* ---- quantization ---- */
 void jxo_ac_thresholds(int c, int xsize, int ysize, float distance, float qm_mult,

@@ -476,9 +476,63 @@ static uint8_t region_quant(const group_ctx *g, int bx, int by, int w, int h) {
   return aggregate_quant(max_q, sum, w * h, g->p->distance);
 }
 
-/* choose_merge_partition (src/ac_strategy/selection.rs:115-150) restricted
- * to the square merge: rectangles price +inf, so masks other than the
- * square can never pass the `merged < incumbent` gate. */
+/* The eight partitions of four children (src/ac_strategy/selection.rs:86-89):
+ * none, the square, either or both halves in either direction.  Bits: square,
+ * left, right, top, bottom. */
+static const uint8_t MERGE_PARTITIONS[8] = {0, 1, 2, 4, 6, 8, 16, 24};
+
+/* partition_cost (src/ac_strategy/selection.rs:91-108) */
+float jxo_partition_cost(const float children[2][2], const float merges[5], int mask) {
+  if (mask & 1) return merges[0];
+  float cost = 0.0f;
+  for (int k = 1; k < 5; k++)
+    if (mask & (1 << k)) cost += merges[k];
+  for (int y = 0; y < 2; y++)
+    for (int x = 0; x < 2; x++)
+      if ((mask & ((1 << (1 + x)) | (1 << (3 + y)))) == 0) cost += children[y][x];
+  return cost;
+}
+
+/* choose_merge_partition (src/ac_strategy/selection.rs:115-150): the
+ * cheapest partition among those whose merged cost is below both the
+ * children it replaces and their accepted baseline. */
+int jxo_choose_merge_partition(const float children[2][2], const float baseline[2][2],
+                               const float merges[5], const float accept[5]) {
+  float cc[5], bc[5];
+  const float (*t[2])[2] = {children, baseline};
+  float *out[2] = {cc, bc};
+  for (int i = 0; i < 2; i++) {
+    const float (*c)[2] = t[i];
+    out[i][0] = ((c[0][0] + c[0][1]) + c[1][0]) + c[1][1];
+    out[i][1] = c[0][0] + c[1][0];
+    out[i][2] = c[0][1] + c[1][1];
+    out[i][3] = c[0][0] + c[0][1];
+    out[i][4] = c[1][0] + c[1][1];
+  }
+  int best = 0;
+  float best_cost = cc[0];
+  for (int m = 1; m < 8; m++) {
+    const int mask = MERGE_PARTITIONS[m];
+    float merged = 0.0f, incumbent = 0.0f, limit = 0.0f;
+    for (int k = 0; k < 5; k++)
+      if (mask & (1 << k)) {
+        merged += merges[k];
+        incumbent += cc[k];
+        limit += bc[k] * accept[k];
+      }
+    if (!(merged < incumbent && merged < limit)) continue;
+    const float cost = jxo_partition_cost(children, merges, mask);
+    if (cost < best_cost) {
+      best = mask;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
+/* choose_merge_partition restricted to the square merge, the Fast tier's
+ * scope: rectangles price +inf, so no mask but the square can pass the
+ * `merged < incumbent` gate, and the square wins when it passes. */
 static int choose_square32(const float children[2][2], const float baseline[2][2], float merge,
                            float accept) {
   float child_total = children[0][0] + children[0][1] + children[1][0] + children[1][1];
@@ -486,6 +540,25 @@ static int choose_square32(const float children[2][2], const float baseline[2][2
   float limit = base_total * accept;
   if (!(merge < child_total && merge < limit)) return 0;
   return merge < child_total;
+}
+
+/* test hooks for the decision pieces */
+int jxo_choose_square32(const float children[2][2], const float baseline[2][2], float merge,
+                        float accept) {
+  return choose_square32(children, baseline, merge, accept);
+}
+int jxo_aggregate_quant(int max_q, uint32_t sum, int covered, float distance) {
+  return aggregate_quant((uint8_t)max_q, sum, covered, distance);
+}
+float jxo_aggregate_qac_2x2(const float qac[2][2], float scale, float distance) {
+  return aggregate_qac_2x2(qac, scale, distance);
+}
+/* cmap_factors (src/ac_strategy.rs:1841) of one tile's map entries */
+void jxo_cmap_factors(int ytox, int ytob, const jxg_frame_params *p, float out[3]) {
+  const float inv_cf = 1.0f / p->cfl_color_factor;
+  out[0] = jxo_fmla((float)ytox, inv_cf, p->cfl_base_x);
+  out[1] = 0.0f;
+  out[2] = jxo_fmla((float)ytob, inv_cf, p->cfl_base_b);
 }
 
 /* select_band over one DC group (src/ac_strategy/selection.rs:1000-1180),

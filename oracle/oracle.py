@@ -110,6 +110,17 @@ def _declare(L):
     L.jxo_fill_invisible_plane.argtypes = [_f32p, c.c_size_t, c.c_int, c.c_int, _u8p]
     L.jxo_alpha_block_factors.argtypes = [_u8p, c.c_int, c.c_int, _f32p]
     L.jxo_apply_alpha_factors.argtypes = [_u8p, _u8p, c.c_int, c.c_int, _f32p]
+    L.jxo_partition_cost.restype = c.c_float
+    L.jxo_partition_cost.argtypes = [_f32p, _f32p, c.c_int]
+    L.jxo_choose_merge_partition.restype = c.c_int
+    L.jxo_choose_merge_partition.argtypes = [_f32p, _f32p, _f32p, _f32p]
+    L.jxo_choose_square32.restype = c.c_int
+    L.jxo_choose_square32.argtypes = [_f32p, _f32p, c.c_float, c.c_float]
+    L.jxo_aggregate_quant.restype = c.c_int
+    L.jxo_aggregate_quant.argtypes = [c.c_int, c.c_uint32, c.c_int, c.c_float]
+    L.jxo_aggregate_qac_2x2.restype = c.c_float
+    L.jxo_aggregate_qac_2x2.argtypes = [_f32p, c.c_float, c.c_float]
+    L.jxo_cmap_factors.argtypes = [c.c_int, c.c_int, c.c_void_p, _f32p]
 
 
 def _p(a, t):
@@ -288,6 +299,43 @@ def front_end(rgb, params, tables=None, threads=1):
     out = dict(qf=qf, cells=cells, ytox=ytox, ytob=ytob, levels=levels, coeffs=coeffs, dc=dc)
     if ch in (2, 4):
         out["alpha"] = alpha
+    return out
+
+
+def _f32(a, n):
+    a = np.ascontiguousarray(a, np.float32).reshape(-1)
+    assert a.size == n
+    return a
+
+
+def partition_cost(children, merges, mask):
+    c, m = _f32(children, 4), _f32(merges, 5)
+    return float(lib().jxo_partition_cost(_p(c, _f32p), _p(m, _f32p), int(mask)))
+
+
+def choose_merge_partition(children, baseline, merges, accept):
+    c, b, m, a = _f32(children, 4), _f32(baseline, 4), _f32(merges, 5), _f32(accept, 5)
+    return int(lib().jxo_choose_merge_partition(_p(c, _f32p), _p(b, _f32p), _p(m, _f32p),
+                                                _p(a, _f32p)))
+
+
+def choose_square32(children, baseline, merge, accept):
+    c, b = _f32(children, 4), _f32(baseline, 4)
+    return int(lib().jxo_choose_square32(_p(c, _f32p), _p(b, _f32p), float(merge), float(accept)))
+
+
+def aggregate_quant(max_q, total, covered, distance):
+    return int(lib().jxo_aggregate_quant(int(max_q), int(total), int(covered), float(distance)))
+
+
+def aggregate_qac_2x2(qac, scale, distance):
+    q = _f32(qac, 4)
+    return float(lib().jxo_aggregate_qac_2x2(_p(q, _f32p), float(scale), float(distance)))
+
+
+def cmap_factors(ytox, ytob, params):
+    out = np.zeros(3, np.float32)
+    lib().jxo_cmap_factors(int(ytox), int(ytob), ctypes.byref(params), _p(out, _f32p))
     return out
 
 

@@ -70,6 +70,10 @@ void jxo_fill_quant_field_group(const float *xyb, int width, int height, int x0,
 /* All DC groups (2048 px) into one xblocks_total-wide field. */
 void jxo_fill_quant_field_image(const float *xyb, int width, int height, float distance,
                                 float inv_scale, uint8_t *qf);
+/* The same field before scaling and rounding: the per-block value
+ * fast_exp2(out * log2 e) * mul + add of src/adaptive_quant.rs:724. */
+void jxo_quant_field_float_image(const float *xyb, int width, int height, float distance,
+                                 float *fq);
 
 /* ---- transforms ---- */
 /* Forward transform of `strategy` at pixel (px, py) of one plane

@@ -542,9 +542,11 @@ static void store_min4(float v, float m[4]) {
   }
 }
 
-void jxo_fill_quant_field_group(const float *xyb, int img_xsize, int img_ysize, int x0, int y0,
-                                int xblocks, int yblocks, float distance, float inv_scale,
-                                uint8_t *qf) {
+/* One DC group; `fq`, when not NULL, receives the field before it is scaled
+ * and rounded (the value `q` below), for tests of the float chain. */
+static void fill_quant_field_group(const float *xyb, int img_xsize, int img_ysize, int x0, int y0,
+                                   int xblocks, int yblocks, float distance, float inv_scale,
+                                   uint8_t *qf, float *fq) {
   const float *py_ = xyb + (size_t)img_xsize * img_ysize;
   float scale = 0.8294f / distance;
   int region_w = xblocks * 8, region_h = yblocks * 8;
@@ -629,6 +631,7 @@ void jxo_fill_quant_field_group(const float *xyb, int img_xsize, int img_ysize, 
       uint8_t *out = qf + (size_t)by * xblocks + bx;
       if (pxx >= img_xsize || pyy >= img_ysize) {
         *out = 1;
+        if (fq) fq[(size_t)by * xblocks + bx] = 0.0f;
         continue;
       }
       int bx_px = mini(pxx, maxi(img_xsize - 8, 0));
@@ -640,6 +643,7 @@ void jxo_fill_quant_field_group(const float *xyb, int img_xsize, int img_ysize, 
       float blue = blue_modulation(bx_px, by_px, xyb, img_xsize, img_ysize, mask_val);
       out_val = fminf(out_val, blue);
       float q = jxo_fast_exp2(out_val * 1.442695041f) * mul + add;
+      if (fq) fq[(size_t)by * xblocks + bx] = q;
       int32_t qi = jxo_sat_i32(jxo_fmla(q, inv_scale, 0.5f));
       qi = qi < 1 ? 1 : (qi > 255 ? 255 : qi);
       *out = (uint8_t)qi;
@@ -648,6 +652,33 @@ void jxo_fill_quant_field_group(const float *xyb, int img_xsize, int img_ysize, 
   free(row_acc);
   free(pre);
   free(aq_map);
+}
+
+void jxo_fill_quant_field_group(const float *xyb, int img_xsize, int img_ysize, int x0, int y0,
+                                int xblocks, int yblocks, float distance, float inv_scale,
+                                uint8_t *qf) {
+  fill_quant_field_group(xyb, img_xsize, img_ysize, x0, y0, xblocks, yblocks, distance, inv_scale,
+                         qf, NULL);
+}
+
+void jxo_quant_field_float_image(const float *xyb, int width, int height, float distance,
+                                 float *fq) {
+  int xb_total = (width + 7) / 8;
+  for (int gy = 0; gy * 2048 < height; gy++) {
+    for (int gx = 0; gx * 2048 < width; gx++) {
+      int x0 = gx * 2048, y0 = gy * 2048;
+      int xs = mini(2048, width - x0), ys = mini(2048, height - y0);
+      int xb = (xs + 7) / 8, yb = (ys + 7) / 8;
+      uint8_t *tmp = (uint8_t *)malloc((size_t)xb * yb);
+      float *ftmp = (float *)malloc((size_t)xb * yb * sizeof(float));
+      fill_quant_field_group(xyb, width, height, x0, y0, xb, yb, distance, 1.0f, tmp, ftmp);
+      for (int by = 0; by < yb; by++)
+        memcpy(fq + (size_t)(y0 / 8 + by) * xb_total + x0 / 8, ftmp + (size_t)by * xb,
+               xb * sizeof(float));
+      free(tmp);
+      free(ftmp);
+    }
+  }
 }
 
 void jxo_fill_quant_field_image(const float *xyb, int width, int height, float distance,

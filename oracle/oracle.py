@@ -61,6 +61,7 @@ def _declare(L):
     L.jxo_xyb_pixel.argtypes = [_f32p, c.c_int, c.c_float, c.c_float, c.c_float, _f32p]
     L.jxo_xyb_image.argtypes = [_u8p, c.c_size_t, c.c_int, c.c_int, c.c_int, _f32p, c.c_int, _f32p]
     L.jxo_fill_quant_field_image.argtypes = [_f32p, c.c_int, c.c_int, c.c_float, c.c_float, _u8p]
+    L.jxo_quant_field_float_image.argtypes = [_f32p, c.c_int, c.c_int, c.c_float, _f32p]
     L.jxo_gaborish_weights.argtypes = [c.c_float, _f32p]
     L.jxo_gaborish_inverse.argtypes = [_f32p, c.c_int, c.c_int, c.c_float]
     L.jxo_forward_transform.restype = c.c_int
@@ -153,6 +154,15 @@ def fill_quant_field(xyb, distance, inv_scale):
     qf = np.zeros(((h + 7) // 8, (w + 7) // 8), np.uint8)
     lib().jxo_fill_quant_field_image(_p(xyb, _f32p), w, h, distance, inv_scale, _p(qf, _u8p))
     return qf
+
+
+def quant_field_float(xyb, distance):
+    """Per-block field before scaling by inv_scale and rounding."""
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    fq = np.zeros(((h + 7) // 8, (w + 7) // 8), np.float32)
+    lib().jxo_quant_field_float_image(_p(xyb, _f32p), w, h, distance, _p(fq, _f32p))
+    return fq
 
 
 def forward_transform(strategy, plane, px, py):

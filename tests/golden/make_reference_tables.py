@@ -14,6 +14,11 @@ tables against:
 * RESAMPLE_SCALE_64_TO_8 (src/dct.rs:2005): DCT64 DC resampling scales.
 * WC128 (src/dct128.rs:76): the 128-point butterfly constants.
 * AFV_BASIS_TRANSPOSE (src/afv.rs:48): the 16x16 AFV basis, row j = pixel j.
+* AQ (src/adaptive_quant.rs:41-206,410-433,547-729): the adaptive
+  quantization constants, by name: module constants, the numbers of every
+  `let` in the masking, modulation and field functions ("fn.var": [numbers
+  of the initializer]), the knot schedules, and the literals of the
+  per-pixel clamp and the exponent's log2(e).
 
 Usage: python tests/golden/make_reference_tables.py [/root/reference]
 """
@@ -57,6 +62,52 @@ def numbers(s):
     return [float(x.replace("_", "")) for x in re.findall(NUM, strip_comments(s))]
 
 
+def literals(s):
+    """Float literals of an expression (digits glued to a name, such as the
+    32 of an f32 suffix or the 2 of exp2, are not literals)."""
+    found = re.findall(r"(?<![\w.])(" + NUM + ")", strip_comments(s))
+    return [float(x.replace("_", "")) for x in found if re.search(r"[.eE]", x)]
+
+
+def fn_body(text, name):
+    """Body of `fn name` (from its opening brace to the matching one)."""
+    m = re.search(r"fn " + name + r"\b", text)
+    start = text.index("{", m.end())
+    depth = 0
+    for j in range(start, len(text)):
+        if text[j] == "{":
+            depth += 1
+        elif text[j] == "}":
+            depth -= 1
+            if depth == 0:
+                return text[start : j + 1]
+    raise ValueError(name)
+
+
+def aq_constants(aq):
+    out = {}
+    for m in re.finditer(r"(?:pub\(crate\) )?const (\w+): f32 = ([^;]+);", aq):
+        out[m.group(1)] = literals(m.group(2))
+    for name in ("AQ_KNOTS", "AQ_STRENGTH", "AQ_BASE_LEVEL"):
+        out[name] = numbers(block_after(aq, "static " + name))[-7:]
+    for fn in ("ratio_cubic_to_simple_gamma", "compute_mask", "masking_sqrt",
+               "hf_modulation", "blue_modulation", "gamma_modulation",
+               "fill_quant_field_scalar"):
+        body = strip_comments(fn_body(aq, fn))
+        for m in re.finditer(r"let (?:mut )?(\w+)(?:: [^=]+)? = ([^;]+);", body):
+            nums = literals(m.group(2))
+            if nums:
+                out[fn + "." + m.group(1)] = nums
+    body = strip_comments(fn_body(aq, "hf_modulation_strength"))
+    out["hf_modulation_strength"] = literals(body)
+    body = strip_comments(fn_body(aq, "fill_quant_field_scalar"))
+    out["fill_quant_field_scalar.diff_clamp"] = literals(
+        re.search(r"if diff >= (" + NUM + ")", body).group(1))
+    out["fill_quant_field_scalar.exp_mul"] = literals(
+        re.search(r"fast_exp2\(out_val \* (" + NUM + r")\)", body).group(1))
+    return out
+
+
 def main():
     out = {}
     aq = read("src/adaptive_quant.rs")
@@ -64,6 +115,7 @@ def main():
     pairs = re.findall(r"\(0x([0-9A-Fa-f]+),\s*0x([0-9A-Fa-f]+)\)", exp2)
     out["EXP2FT"] = [[int(a, 16), int(b, 16)] for a, b in pairs]
     assert len(out["EXP2FT"]) == 64
+    out["AQ"] = aq_constants(aq)
 
     ctx = read("src/ac_context.rs")
     for name, n in [("K_COEFF_ORDER_8X8", 64), ("K_COEFF_ORDER_16X8", 128),

@@ -106,6 +106,22 @@ def test_quant_field_bit_exact(J, fe, torch, oracle, w, h, distance):
     assert same_bits(got, want), first_mismatch(got, want)
 
 
+@pytest.mark.parametrize("distance", [0.5, 1.0, 1.5])
+def test_quant_field_pinned_images_bit_exact(J, fe, torch, oracle, golden, distance):
+    """The constant, step, blue, checkerboard and textured planes whose float
+    field the oracle matches to an FP64 model (test_oracle_aq)."""
+    from test_oracle_aq import cases
+    p = params_for(J, distance)
+    inv_scale = 1.0 / p.scale
+    for name, xyb in cases(golden).items():
+        log("aq pinned", name, distance)
+        _, h, w = xyb.shape
+        want = oracle.fill_quant_field(xyb, p.distance, inv_scale)
+        got = fe.fill_quant_field(to_dev(torch, padded(J, xyb)), w, h, p.distance,
+                                  inv_scale).cpu().numpy()
+        assert same_bits(got, want), (name, first_mismatch(got, want))
+
+
 def _stage_inputs(J, oracle, w, h, distance, seed, fused=True, cmap=False):
     from jixel_amd.synth import photo_like
     rgb = photo_like(w, h, seed=seed)
@@ -255,6 +271,26 @@ def test_forward_quantize_bit_exact(J, fe, torch, oracle, w, h, distance, crossi
                              ("dc", o["dc"][c], dc_w[c])]:
             g = g.cpu().numpy()
             assert same_bits(g, wnt), (name, c, first_mismatch(g, wnt))
+
+
+def test_forward_quantize_composed_pins_bit_exact(J, fe, torch, oracle):
+    """The single-level CfL, LLF and proportional-chroma DC cases whose
+    oracle outputs follow in closed form (test_oracle_compose)."""
+    from test_oracle_compose import all_inputs
+    for name, a in all_inputs(J, oracle):
+        log("quantize composed", name)
+        lv_w, co_w, dc_w = oracle.forward_quantize(a["xyb"], a["cells"], a["qf"], a["ytox"],
+                                                   a["ytob"], a["tables"], a["p"])
+        oracle.set_fused(True)
+        _, h, w = a["xyb"].shape
+        o = fe.forward_quantize(to_dev(torch, padded(J, a["xyb"])), w, h, to_dev(torch, a["cells"]),
+                                to_dev(torch, a["qf"]), a["p"], to_dev(torch, a["ytox"]),
+                                to_dev(torch, a["ytob"]))
+        for c in range(3):
+            for what, g, wnt in [("levels", o["levels"][c], lv_w[c]),
+                                 ("coeffs", o["coeffs"][c], co_w[c]), ("dc", o["dc"][c], dc_w[c])]:
+                g = g.cpu().numpy()
+                assert same_bits(g, wnt), (name, what, c, first_mismatch(g, wnt))
 
 
 @pytest.mark.parametrize("fused", [True, False])

@@ -166,6 +166,20 @@ def step_y(a, b, at, h=64, w=96):
     return y
 
 
+def checker(a, b, h=64, w=96):
+    yy, xx = np.mgrid[0:h, 0:w]
+    return np.where((yy + xx) % 2 == 0, a, b).astype(np.float32)
+
+
+def textured(h=64, w=96, seed=3):
+    """A checkerboard whose amplitude changes per 4x4 pre-map cell, so the
+    nine neighbours of every erosion site are distinct."""
+    amp = np.random.default_rng(seed).uniform(0.0, 0.03, (h // 4, w // 4))
+    amp = np.kron(amp, np.ones((4, 4)))
+    yy, xx = np.mgrid[0:h, 0:w]
+    return (0.35 + np.where((yy + xx) % 2 == 0, amp, -amp)).astype(np.float32)
+
+
 def cases(golden):
     k_limit = np.float32(golden["AQ"]["blue_modulation.k_limit"][0])
     k_off = np.float32(golden["AQ"]["blue_modulation.k_offset"][0])
@@ -182,6 +196,9 @@ def cases(golden):
         "step_x_block_edge": planes(x0, step_x(0.30, 0.60, 40), np.float32(0.20)),
         "step_y_large": planes(x0, step_y(0.45, 0.50, 20), np.float32(0.40)),
         "step_y_dark": planes(np.float32(0.0), step_y(0.1, 0.0, 44), np.float32(0.0)),
+        "checker_dark": planes(np.float32(0.0), checker(0.0, 0.1), np.float32(0.0)),
+        "checker_small": planes(x0, checker(0.40, 0.42), np.float32(0.30)),
+        "textured": planes(x0, textured(), np.float32(0.30)),
     }
 
 
@@ -196,7 +213,8 @@ REL_TOL = 2e-6
 @pytest.mark.parametrize("d", DISTANCES)
 @pytest.mark.parametrize("name", ["constant", "constant_dark", "blue_partial", "blue_fold",
                                   "step_x_small", "step_x_large", "step_x_dark",
-                                  "step_x_block_edge", "step_y_large", "step_y_dark"])
+                                  "step_x_block_edge", "step_y_large", "step_y_dark",
+                                  "checker_dark", "checker_small", "textured"])
 def test_float_field_matches_fp64_model(oracle, golden, model, name, d):
     xyb = cases(golden)[name]
     got = oracle.quant_field_float(xyb, d).astype(np.float64)
@@ -272,6 +290,19 @@ def test_terms_are_isolated(golden, model):
     base = 0.25 * (P[2:, 1:-1] + P[:-2, 1:-1] + P[1:-1, :-2] + P[1:-1, 2:])
     diff = (model.ratio(Y + model.gamma_offset, False) * (Y - base)) ** 2
     assert diff[0, 35] > 0.2 > diff[0, 36] > 0
+    # on the dark checkerboard every dark pixel is clamped, so the clamp
+    # reaches the erosion's four smallest
+    Y = cs["checker_dark"][1].astype(np.float64)
+    P = np.pad(Y, 1, mode="edge")
+    base = 0.25 * (P[2:, 1:-1] + P[:-2, 1:-1] + P[1:-1, :-2] + P[1:-1, 2:])
+    diff = (model.ratio(Y + model.gamma_offset, False) * (Y - base)) ** 2
+    assert (diff[Y == 0] > 0.2).all() and (diff[Y > 0] < 0.2).all()
+    # the textured pre-map has nine distinct neighbours at every site
+    pre = model.pre_map(cs["textured"][1].astype(np.float64))
+    P = np.pad(pre, 1, mode="edge")
+    nb = np.stack([P[1 + dy:1 + dy + pre.shape[0], 1 + dx:1 + dx + pre.shape[1]]
+                   for dy in (-1, 0, 1) for dx in (-1, 0, 1)])
+    assert (np.diff(np.sort(nb, axis=0)[:5], axis=0)[:, 1:-1, 1:-1] > 0).all()
     # HF: the large step is clamped at valmin_y, the small one is not
     hf_l = model.hf_sum(cs["step_x_large"][1].astype(np.float64))
     hf_s = model.hf_sum(cs["step_x_small"][1].astype(np.float64))

@@ -261,7 +261,8 @@ enum {
   JXG_PROF_QUANTIZE = 7,   /* k_quantize                               */
   JXG_PROF_QUANTIZE_XT = 8,/* k_quantize_xt                            */
   JXG_PROF_FILL = 9,       /* k_fill_bands alone (inside JXG_PROF_ALPHA) */
-  JXG_PROF_N = 10
+  JXG_PROF_RETILE = 10,    /* piece costs, re-tiling passes, field adjustment */
+  JXG_PROF_N = 11
 };
 #define JXG_PROF_CAP 64
 int jxg_ctx_profile(jxg_ctx *ctx, int enable);

@@ -22,6 +22,9 @@ enum {
   BS_CMAPS,
   BS_SEL,
   BS_QUANT,
+  BS_PIECE,
+  BS_RETILE,
+  BS_ADJUST,
   BS_N
 };
 
@@ -33,6 +36,8 @@ struct BatchImg {
   float *xyb;   // 3 planes of stride x ph
   float *pre;   // pre_w x pre_h AQ pre-pass
   float *part;  // CfL block sums, tiles * 64 * 4
+  float *pc;    // re-tiling piece costs, 4 * xb * yb
+  float *c32;   // raw DCT32X32 costs, xb * yb
   uint8_t *qf, *cells;
   int8_t *ytox, *ytob;
   int32_t *lv[3];

@@ -159,7 +159,7 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
   std::vector<jxg_derived> der(n);
   b->geo.resize(2 * n);
   size_t arena = 0;
-  std::vector<size_t> off_xyb(n), off_pre(n), off_part(n);
+  std::vector<size_t> off_xyb(n), off_pre(n), off_part(n), off_pc(n);
   uint64_t queue = 0;
   for (uint32_t i = 0; i < n; i++) {
     const jxg_batch_image &im = images[i];
@@ -177,6 +177,8 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
     arena += round64(4 * xb * yb);
     off_part[i] = arena;
     arena += round64(tiles * 64 * 4);
+    off_pc[i] = arena;  // re-tiling piece costs (4 planes) and 32x32 costs
+    arena += round64(5 * xb * yb);
     queue += xb * yb / 2 + 1;
   }
   if (queue >= (1ull << 32)) return fail(set_err(JXG_ERR_TOO_LARGE, "batch too large"));
@@ -224,6 +226,8 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
     d.xyb = b->d_arena + off_xyb[i];
     d.pre = b->d_arena + off_pre[i];
     d.part = b->d_arena + off_part[i];
+    d.pc = b->d_arena + off_pc[i];
+    d.c32 = d.pc + (size_t)4 * d.xb * d.yb;
     d.qf = im.d_qf;
     d.cells = im.d_cells;
     d.ytox = im.d_ytox;

@@ -91,6 +91,10 @@ struct jxg_ctx {
   size_t fxchg_cap = 0;
   float *d_afac = nullptr;  // alpha block factors
   size_t afac_cap = 0;
+  float *d_pc = nullptr;  // re-tiling piece costs (4 planes of blocks)
+  size_t pc_cap = 0;
+  float *d_c32 = nullptr;  // raw DCT32X32 cost per 32x32 region's first block
+  size_t c32_cap = 0;
   // jxg_ctx_profile: event pairs per stage, recorded while prof_on
   bool prof_on = false;
   hipEvent_t *prof_ev = nullptr;  // JXG_PROF_N * 2 * JXG_PROF_CAP
@@ -159,6 +163,14 @@ JXG_HIDDEN void batch_grids(uint32_t W, uint32_t H, uint32_t stride, uint32_t ph
 // jxg_chroma.hip: chroma-HF protection of the quant field.
 JXG_HIDDEN int chroma_hf(jxg_ctx *ctx, const float *xyb, uint32_t stride, uint32_t ph, uint32_t W,
                          uint32_t H, float distance, int fused, uint8_t *qf);
+// jxg_retile.hip: the coefficient re-tiling after k_select (mode = the
+// derived `retile`), then adjust_quant_field; single image and batch.
+JXG_HIDDEN int retile_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uint32_t stride,
+                             uint32_t ph, uint32_t W, uint32_t H, uint8_t *qf, const int8_t *ytox,
+                             const int8_t *ytob, const float *tables, const jxg_frame_params *p,
+                             int mode, uint8_t *cells, const float *c32, float *pc);
+JXG_HIDDEN int batch_retile(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                            const uint32_t *d_pfx, int n, const uint32_t total[], int fused);
 JXG_HIDDEN int zero_cmaps(jxg_ctx *ctx, uint32_t w, uint32_t h, const int8_t **x,
                           const int8_t **b);
 

@@ -32,7 +32,8 @@ typedef struct jxg_derived {
   int32_t selection_quantizer;  // selection_quantizer::enabled (Fast tiers, 1<=d<=4)
   int32_t run_selection;        // fill_ac_strategy reaches the chooser
   int32_t fused;
-  int32_t pad;
+  int32_t retile;  // coefficient re-tiling: 0 off, 1 aligned grid (Fastest), 2 + unaligned
+                   // pieces and the (4, 4)-shifted grid (Fast); src/effort.rs:311-349
   uint32_t qt_off[JXG_QT_NUM_KINDS];  // quant-matrix blob layout
   uint32_t qt_n[JXG_QT_NUM_KINDS];
   float exp2_table[64];          // fast_exp2's 2^((i-32)/64)

@@ -859,7 +859,8 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
                                                   uint8_t *__restrict__ cells,
                                                   float *__restrict__ c8out,
                                                   float *__restrict__ c16out,
-                                                  float *__restrict__ c32out, int xb, int yb, int bix, int biy) {
+                                                  float *__restrict__ c32out, float *__restrict__ c32raw,
+                                                  int xb, int yb, int bix, int biy) {
   __shared__ SelLds L;
   __shared__ uint8_t q_in[16];
   __shared__ uint8_t cell_out[16];
@@ -1084,7 +1085,9 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
         cell = JXG_CELL(JXG_DCT16X16, tid == first);
       }
       cell_out[tid] = cell;
-      if (cv > 1) {
+      // With the re-tiling after the search, the field is adjusted once the
+      // layout is final (jxg_retile.hip).
+      if (cv > 1 && !D->retile) {
         uint8_t max_q = 0;
         uint32_t sum = 0;
         for (int jy = 0; jy < cv; jy++)
@@ -1097,6 +1100,7 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
       }
     }
     __syncthreads();
+    if (c32raw && tid == 0) c32raw[(size_t)rby * xb + rbx] = (four_row && four_col) ? L.cost32 : NAN;
     if (c8out && tid < 16) {
       const int ix = tid & 3, iy = tid >> 2;
       const int sx = ix >> 1, sy = iy >> 1;
@@ -1141,8 +1145,10 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ x
                                                   uint8_t *__restrict__ cells,
                                                   float *__restrict__ c8out,
                                                   float *__restrict__ c16out,
-                                                  float *__restrict__ c32out, int xb, int yb) {
-  select_body<F>(xyb, stride, padded_h, W, H, qf, ytox, ytob, tables, scans, selq, D, cells, c8out, c16out, c32out, xb, yb, blockIdx.x, blockIdx.y);
+                                                  float *__restrict__ c32out,
+                                                  float *__restrict__ c32raw, int xb, int yb) {
+  select_body<F>(xyb, stride, padded_h, W, H, qf, ytox, ytob, tables, scans, selq, D, cells, c8out,
+                 c16out, c32out, c32raw, xb, yb, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -2027,7 +2033,7 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select_b(const BatchImg *__restric
                                                     const uint32_t *__restrict__ scans) {
   BATCH_BLOCK(BS_SEL)
   select_body<F>(I.xyb, I.stride, I.ph, I.W, I.H, I.qf, I.ytox, I.ytob, I.tables, scans, I.selq, I.D,
-                 I.cells, nullptr, nullptr, nullptr, I.xb, I.yb, bx_, by_);
+                 I.cells, nullptr, nullptr, nullptr, I.c32, I.xb, I.yb, bx_, by_);
 }
 
 struct BatchQueue {
@@ -2169,16 +2175,21 @@ JXG_HIDDEN int batch_stages_post(jxg_ctx *ctx, hipStream_t st, const BatchImg *d
   Q.xt_list = xt_list;
   Q.xt_img = xt_img;
   Q.xt_cap = xt_cap;
-  if (fused) {
+  if (fused)
     k_select_b<true><<<total[BS_SEL], SEL_T, 0, st>>>(d_img, p + BS_SEL * m, n, ctx->d_scans);
+  else
+    k_select_b<false><<<total[BS_SEL], SEL_T, 0, st>>>(d_img, p + BS_SEL * m, n, ctx->d_scans);
+  LAUNCH_CHECK("batch select");
+  int rc = batch_retile(ctx, st, d_img, d_pfx, n, total, fused);
+  if (rc) return rc;
+  if (fused) {
     k_quantize_b<true><<<total[BS_QUANT], 256, 0, st>>>(d_img, p + BS_QUANT * m, n, Q);
     k_quantize_xt_b<true><<<xt_grid, 256, 0, st>>>(d_img, Q);
   } else {
-    k_select_b<false><<<total[BS_SEL], SEL_T, 0, st>>>(d_img, p + BS_SEL * m, n, ctx->d_scans);
     k_quantize_b<false><<<total[BS_QUANT], 256, 0, st>>>(d_img, p + BS_QUANT * m, n, Q);
     k_quantize_xt_b<false><<<xt_grid, 256, 0, st>>>(d_img, Q);
   }
-  LAUNCH_CHECK("batch select / quantize");
+  LAUNCH_CHECK("batch quantize");
   return JXG_OK;
 }
 
@@ -2201,6 +2212,12 @@ JXG_HIDDEN void batch_grids(uint32_t W, uint32_t H, uint32_t stride, uint32_t ph
   gh[BS_SEL] = (yb + 3) / 4;
   gw[BS_QUANT] = (xb + 7) / 8;
   gh[BS_QUANT] = (yb + 7) / 8;
+  gw[BS_PIECE] = (xb + 3) / 4;
+  gh[BS_PIECE] = (yb + 3) / 4;
+  gw[BS_RETILE] = 32 * ((xb + 255) / 256);  // 32 tile slots per DC group and axis
+  gh[BS_RETILE] = 32 * ((yb + 255) / 256);
+  gw[BS_ADJUST] = (xb * yb + 255) / 256;
+  gh[BS_ADJUST] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -2742,6 +2759,8 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_vis);
   (void)hipFree(ctx->d_fxchg);
   (void)hipFree(ctx->d_afac);
+  (void)hipFree(ctx->d_pc);
+  (void)hipFree(ctx->d_c32);
   (void)hipFree(ctx->d_zero_cmap);
   (void)hipFree(ctx->d_rgb);
   (void)hipFree(ctx->d_tmp);
@@ -2913,13 +2932,26 @@ static int select_impl(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32
   if ((rc = tables_for(ctx, p, &tables, &selq))) return rc;
   const int xb = (W + 7) / 8, yb = (H + 7) / 8;
   const dim3 grid((xb + 3) / 4, (yb + 3) / 4);
+  // The Fast / Fastest decision re-tiles the merge search's layout
+  // (strategy decisions only: the cost dump keeps the search's own field).
+  const int retile = cells ? ctx->h_derived->retile : 0;
+  float *c32raw = nullptr;
+  if (retile) {
+    const size_t nb = (size_t)xb * yb;
+    if ((rc = ensure(&ctx->d_pc, &ctx->pc_cap, 4 * nb))) return rc;
+    if ((rc = ensure(&ctx->d_c32, &ctx->c32_cap, nb))) return rc;
+    c32raw = ctx->d_c32;
+  }
   prof_begin(ctx, JXG_PROF_SELECT, ctx->stream);
   if (p->fused)
-    k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+    k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, c32raw, xb, yb);
   else
-    k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, xb, yb);
+    k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, c32raw, xb, yb);
   LAUNCH_CHECK("k_select");
   prof_end(ctx, JXG_PROF_SELECT, ctx->stream);
+  if (retile)
+    return retile_stages(ctx, ctx->stream, d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables,
+                         p, retile, cells, ctx->d_c32, ctx->d_pc);
   return JXG_OK;
 }
 

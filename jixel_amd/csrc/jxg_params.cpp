@@ -372,6 +372,9 @@ int jxg_derive(const jxg_frame_params *p, jxg_derived *o) {
   if (d >= 1.5f) o->bias_32x32 *= 0.94f;
   o->selection_quantizer = (d >= 1.0f && d <= 4.0f && fast) ? 1 : 0;
   o->run_selection = (d > 0.056713393f && p->speed >= JXG_SPEED_FASTEST) ? 1 : 0;
+  // layout_dp_coefficient at Fastest and Fast; layout_dp_unaligned and
+  // layout_crossing at Fast (src/effort.rs:311-349)
+  o->retile = !o->run_selection ? 0 : (p->speed == JXG_SPEED_FAST ? 2 : (p->speed == JXG_SPEED_FASTEST ? 1 : 0));
   o->fused = f ? 1 : 0;
   for (int k = 0; k < JXG_QT_NUM_KINDS; k++) {
     o->qt_off[k] = jxg_qt_offset(k);

@@ -130,12 +130,30 @@ void jxo_natural_scan(int cx, int cy, uint32_t *out);
 float jxo_strategy_cost(const float *xyb, int width, int height, int strategy, int px, int py,
                         float qac, float meta_r, const float cmap[3], const float *tables,
                         const jxg_frame_params *p);
-/* Square-merge selection for the whole image (select_band per DC group,
- * src/ac_strategy/selection.rs:1000) followed by adjust_quant_field (:503).
- * Updates qf in place and writes strategy cells. */
+/* The Fast / Fastest strategy decision for the whole image, per DC group:
+ * select_band's square merges (src/ac_strategy/selection.rs:1000), the
+ * coefficient-model re-tiling (layout_dp::retile_coefficient,
+ * src/ac_strategy/selection/layout_dp.rs:2381) and adjust_quant_field
+ * (selection.rs:503).  Updates qf in place and writes strategy cells. */
 void jxo_select_strategy(const float *xyb, int width, int height, uint8_t *qf,
                          const int8_t *ytox, const int8_t *ytob, const float *tables,
                          const jxg_frame_params *p, uint8_t *cells);
+/* plan_small (src/ac_strategy/selection/layout_dp.rs:1918): exact tiling of
+ * a w x h (<= 8x8) cell grid by leaves, 2x1 / 1x2 pairs and 2x2 squares
+ * around the `fixed` cells (bit y*w+x), after the live-mask pruning.
+ * Returns the optimum; when finite and below replace_below, the plan's
+ * pieces (cell index, choice 1 leaf / 2 wide / 3 tall / 4 square), last
+ * cell first. */
+float jxo_plan_small(const float *leaf, const float *wide, const float *tall, const float *square,
+                     uint64_t fixed, int w, int h, float replace_below, int *n_out,
+                     uint8_t *cells_out, uint8_t *choices_out);
+void jxo_live_masks(const float *leaf, const float *wide, const float *tall, const float *square,
+                    int w, int h, uint8_t *live);
+/* One coefficient re-tiling pass (crossing 0: the aligned 64-px grid; 1:
+ * the grid shifted by (4, 4) blocks) over the given layout, in place. */
+void jxo_retile_pass(const float *xyb, int width, int height, const uint8_t *qf,
+                     const int8_t *ytox, const int8_t *ytob, const float *tables,
+                     const jxg_frame_params *p, uint8_t *cells, int crossing);
 /* Square-candidate cost maps (DCT8 per block, DCT16X16 per 2x2 site,
  * DCT32X32 per 4x4 site; NaN where not evaluated). */
 void jxo_strategy_costs(const float *xyb, int width, int height, const uint8_t *qf,

@@ -6,7 +6,10 @@
  * own, stated here so the GPU kernels can be checked against it:
  *
  *   for every transform T of the layout (first block (bx, by), footprint
- *   w x h px, covered blocks n_T), theta_T = mean of the per-block field
+ *   w x h px, covered blocks n_T) of the DCT8..DCT32 family whose footprint
+ *   lies inside one aligned 32x32-px region (the others, such as the
+ *   re-tiling's unaligned pieces, keep their field and are left out of L),
+ *   theta_T = mean of the per-block field
  *   theta over its covered blocks, and per channel c:
  *     coefficients  C = F_h X F_w^T  (F_n = C_n / n, C_n[k][j] = c_k cos(pi k (2j+1) / 2n),
  *                                     c_0 = 1, c_k = sqrt 2: the front end's scaled DCT-II)
@@ -106,6 +109,7 @@ double jxo_tune_loss_grad(const float *xyb, int W, int H, const uint8_t *cells,
       const int s = cell >> 1;
       if (kind_of(s) < 0) continue;
       const int cvx = jxo_covered_x(s), cvy = jxo_covered_y(s);
+      if ((bx & 3) + cvx > 4 || (by & 3) + cvy > 4) continue;  /* crosses a 32x32 region */
       const int w = cvx * 8, h = cvy * 8;
       double th = 0.0;
       for (int iy = 0; iy < cvy; iy++)

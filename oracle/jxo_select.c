@@ -612,6 +612,381 @@ static void select_group(group_ctx *g) {
   }
 }
 
+/* ------------------------------------------------------------------ */
+/* Coefficient-model re-tiling of the merge search's layout: the Fast and */
+/* Fastest tiers' layout_dp::retile_coefficient                           */
+/* (src/ac_strategy/selection/layout_dp.rs:1737-2430, called from         */
+/* src/ac_strategy/selection.rs:2194-2208).  Every 8x8-block tile of the  */
+/* DC group is re-planned exactly over DCT8 leaves, 8x16 / 16x8 pairs and */
+/* 16x16 squares (plan_small, a broken-profile DP), around or over the    */
+/* incumbent 32-family blocks, and the new pieces are installed when they */
+/* price below the pieces they replace.  Fast adds unaligned pairs and    */
+/* squares and a second pass on the grid shifted by (4, 4) blocks         */
+/* (layout_dp_unaligned, layout_crossing: src/effort.rs:322-349).         */
+
+enum { SM_PRE = 0, SM_LEAF = 1, SM_WIDE = 2, SM_TALL = 3, SM_SQUARE = 4 };
+#define SM_STATES 512 /* 1 << (8 + 1) */
+
+static int small_strategy(int choice) {
+  switch (choice) {
+    case SM_LEAF: return JXG_DCT;
+    case SM_WIDE: return JXG_DCT8X16; /* two blocks wide */
+    case SM_TALL: return JXG_DCT16X8; /* two blocks tall */
+    default: return JXG_DCT16X16;
+  }
+}
+
+/* SmallCosts::live_masks (layout_dp.rs:1860-1916): bit 0 wide, bit 1 tall,
+ * bit 2 square, each kept only when it prices below the smaller pieces
+ * tiling the same cells. */
+void jxo_live_masks(const float *leaf, const float *wide, const float *tall, const float *square,
+                    int w, int h, uint8_t *live) {
+  memset(live, 0, 64);
+  if (w * h == 0) return;
+  for (int y = 0; y < h; y++) {
+    const int s = y * w;
+    if (y + 1 == h) {
+      for (int x = 0; x + 1 < w; x++) live[s + x] = wide[s + x] < leaf[s + x] + leaf[s + x + 1];
+      continue;
+    }
+    const int b = s + w;
+    for (int x = 0; x + 1 < w; x++) {
+      const float top_leaves = leaf[s + x] + leaf[s + x + 1];
+      const float leaves = (top_leaves + leaf[b + x]) + leaf[b + x + 1];
+      const float wides = wide[s + x] + wide[b + x];
+      const float talls = tall[s + x] + tall[s + x + 1];
+      const float best = fminf(fminf(leaves, wides), talls);
+      live[s + x] = (uint8_t)((wide[s + x] < top_leaves) |
+                              ((tall[s + x] < leaf[s + x] + leaf[b + x]) << 1) |
+                              ((square[s + x] < best) << 2));
+    }
+    const int r = w - 1;
+    live[s + r] = (uint8_t)((tall[s + r] < leaf[s + r] + leaf[b + r]) << 1);
+  }
+}
+
+typedef struct {
+  float cost[SM_STATES], next[SM_STATES];
+  uint16_t active[SM_STATES], next_active[SM_STATES];
+  int n_active;
+  uint8_t choices[64 * SM_STATES];
+  int plan_n;
+  uint8_t plan_cell[64], plan_choice[64];
+} small_planner;
+
+static void small_planner_init(small_planner *s) {
+  for (int i = 0; i < SM_STATES; i++) s->cost[i] = s->next[i] = INFINITY;
+  s->n_active = 0;
+  s->plan_n = 0;
+}
+
+/* plan_small (layout_dp.rs:1918-2019): the cells are visited in row-major
+ * order; a state holds the filled flags of the current cell and the w cells
+ * after it.  Predecessors are relaxed in the order the states entered the
+ * layer, each with the leaf, wide, tall, square choices in that order, and a
+ * state keeps the first strictly cheaper arrival, so ties resolve as in the
+ * reference.  Returns the optimum; when it is finite and below
+ * replace_below, the pieces of the plan are saved (last cell first). */
+static float plan_small(const float *leaf, const float *wide, const float *tall,
+                        const float *square, const uint8_t *live, uint64_t fixed, int w, int h,
+                        small_planner *s, float replace_below) {
+  const int cells = w * h;
+  const int states = 1 << (w + 1);
+  const int mask = states - 1;
+  const int below_bit = 1 << w, corner_bit = 2 << w;
+  const int added[5] = {0, 0, 2, below_bit, 2 | below_bit | corner_bit};
+  const int start = (int)(fixed & (uint64_t)mask);
+  s->cost[start] = 0.0f;
+  s->active[0] = (uint16_t)start;
+  s->n_active = 1;
+  uint64_t incoming = fixed >> (w + 1);
+  for (int i = 0; i < cells; i++) {
+    const int corner = (int)(incoming & 1u) << w;
+    incoming >>= 1;
+    const uint8_t shapes = live[i];
+    uint8_t *from = s->choices + (size_t)i * states;
+    int next_len = 0;
+    for (int a = 0; a < s->n_active; a++) {
+      const int state = s->active[a];
+      const float here = s->cost[state];
+      s->cost[state] = INFINITY;
+#define RELAX(filled, c, ch)                                  \
+  do {                                                        \
+    const int to_ = ((((filled) >> 1) | corner) & (SM_STATES - 1)); \
+    const float cc_ = (c);                                    \
+    if (cc_ < s->next[to_]) {                                 \
+      if (s->next[to_] == INFINITY) s->next_active[next_len++] = (uint16_t)to_; \
+      s->next[to_] = cc_;                                     \
+      from[to_] = (uint8_t)(ch);                              \
+    }                                                         \
+  } while (0)
+      if (state & 1) {
+        RELAX(state, here, SM_PRE);
+        continue;
+      }
+      RELAX(state, here + leaf[i], SM_LEAF);
+      const int right = !(state & 2), below = !(state & below_bit);
+      if (right && (shapes & 1)) RELAX(state | 2, here + wide[i], SM_WIDE);
+      if (below && (shapes & 2)) RELAX(state | below_bit, here + tall[i], SM_TALL);
+      if (right && below && corner == 0 && (shapes & 4))
+        RELAX(state | added[SM_SQUARE], here + square[i], SM_SQUARE);
+#undef RELAX
+    }
+    for (int k = 0; k < SM_STATES; k++) {
+      float t = s->cost[k];
+      s->cost[k] = s->next[k];
+      s->next[k] = t;
+    }
+    memcpy(s->active, s->next_active, sizeof(uint16_t) * (size_t)next_len);
+    s->n_active = next_len;
+  }
+  const float total = s->cost[0];
+  for (int a = 0; a < s->n_active; a++) s->cost[s->active[a]] = INFINITY;
+  if (!(isfinite(total) && total < replace_below)) return total;
+  s->plan_n = 0;
+  int state = 0;
+  for (int i = cells - 1; i >= 0; i--) {
+    const int choice = s->choices[(size_t)i * states + state];
+    if (choice != SM_PRE) {
+      s->plan_cell[s->plan_n] = (uint8_t)i;
+      s->plan_choice[s->plan_n] = (uint8_t)choice;
+      s->plan_n++;
+    }
+    state = (((state << 1) ^ added[choice]) & mask) | (choice == SM_PRE);
+  }
+  return total;
+}
+
+/* Exported for the planner's own known-answer and exhaustive checks: the
+ * optimum and the saved pieces (cell index, choice), last cell first. */
+float jxo_plan_small(const float *leaf, const float *wide, const float *tall, const float *square,
+                     uint64_t fixed, int w, int h, float replace_below, int *n_out,
+                     uint8_t *cells_out, uint8_t *choices_out) {
+  static __thread small_planner *sp;
+  if (!sp) {
+    sp = (small_planner *)malloc(sizeof(small_planner));
+    small_planner_init(sp);
+  }
+  uint8_t live[64];
+  jxo_live_masks(leaf, wide, tall, square, w, h, live);
+  float t = plan_small(leaf, wide, tall, square, live, fixed, w, h, sp, replace_below);
+  *n_out = sp->plan_n;
+  memcpy(cells_out, sp->plan_cell, (size_t)sp->plan_n);
+  memcpy(choices_out, sp->plan_choice, (size_t)sp->plan_n);
+  return t;
+}
+
+/* can_place_strategy_crossing (src/dc_group_data.rs:207-218): inside the
+ * group and one 256-px AC group; 64-px tiles may be crossed. */
+static int can_place_crossing(int x, int y, int s, int xsize, int ysize) {
+  int cx = jxo_covered_x(s), cy = jxo_covered_y(s);
+  return x + cx <= xsize && y + cy <= ysize && x / 32 == (x + cx - 1) / 32 &&
+         y / 32 == (y + cy - 1) / 32;
+}
+
+static int is_coefficient_strategy(int s) {
+  return s == JXG_DCT || s == JXG_DCT16X8 || s == JXG_DCT8X16 || s == JXG_DCT16X16;
+}
+
+static uint64_t piece_mask(int x, int y, int s, int stride) {
+  uint64_t row = (1ull << jxo_covered_x(s)) - 1ull, m = 0;
+  for (int r = 0; r < jxo_covered_y(s); r++) m |= row << (y * stride + x + r * stride);
+  return m;
+}
+
+/* footprint_is_self_contained (src/ac_strategy/selection.rs:3235-3258) */
+static int footprint_closed(const group_ctx *g, int bx, int by, int w, int h) {
+  int area = 0;
+  for (int y = by; y < by + h; y++)
+    for (int x = bx; x < bx + w; x++) {
+      uint8_t cell = g->cells[(size_t)(g->gy0 + y) * g->qf_stride + g->gx0 + x];
+      if (!(cell & 1)) continue;
+      int s = cell >> 1, cx = jxo_covered_x(s), cy = jxo_covered_y(s);
+      if (x + cx > bx + w || y + cy > by + h) return 0;
+      area += cx * cy;
+    }
+  return area == w * h;
+}
+
+typedef struct {
+  float *memo; /* [5][ysize][xsize]: DCT8, 16X8, 8X16, 16X16, 32X32; NaN = not yet */
+} retile_memo;
+
+static int memo_slot(int s) {
+  switch (s) {
+    case JXG_DCT: return 0;
+    case JXG_DCT16X8: return 1;
+    case JXG_DCT8X16: return 2;
+    case JXG_DCT16X16: return 3;
+    case JXG_DCT32X32: return 4;
+  }
+  return -1;
+}
+
+/* CoefficientScores::score (layout_dp.rs:1746-1810): strategy_cost at the
+ * piece with region_qac of the (pre-adjustment) field and META_R.  The
+ * reference's caches hold the same values. */
+static float piece_score(group_ctx *g, retile_memo *m, int s, int x, int y) {
+  int slot = memo_slot(s);
+  size_t k = (size_t)y * g->xsize + x;
+  float *cell = slot >= 0 ? &m->memo[(size_t)slot * g->xsize * g->ysize + k] : NULL;
+  if (cell && !isnan(*cell)) return *cell;
+  float cmap[3];
+  cmap_factors(g, x, y, cmap);
+  float qac = g->p->scale *
+              (float)region_quant(g, x, y, jxo_covered_x(s), jxo_covered_y(s));
+  float c = cost_at(g, s, x, y, qac, cmap);
+  if (cell) *cell = c;
+  return c;
+}
+
+typedef struct {
+  int x, y, s;
+} tpiece;
+
+static int tpiece_cmp(const void *a, const void *b) {
+  const tpiece *p = (const tpiece *)a, *q = (const tpiece *)b;
+  if (p->y != q->y) return p->y - q->y;
+  if (p->x != q->x) return p->x - q->x;
+  return p->s - q->s;
+}
+
+static int tpiece_in(const tpiece *set, int n, tpiece p) {
+  for (int i = 0; i < n; i++)
+    if (set[i].x == p.x && set[i].y == p.y && set[i].s == p.s) return 1;
+  return 0;
+}
+
+/* search_tile_coefficient (layout_dp.rs:2033-2199) on the w x h tile at
+ * group block (bx, by); the accepted pieces are installed at once (tiles of
+ * one pass are disjoint and read only their own cells). */
+static void retile_tile(group_ctx *g, retile_memo *m, small_planner *sp, int bx, int by, int w,
+                        int h, int crossing, int unaligned) {
+  tpiece cur[64];
+  int ncur = 0;
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      uint8_t cell = g->cells[(size_t)(g->gy0 + by + y) * g->qf_stride + g->gx0 + bx + x];
+      if (!(cell & 1)) continue;
+      int s = cell >> 1;
+      if (!crossing || (x + jxo_covered_x(s) <= w && y + jxo_covered_y(s) <= h)) {
+        tpiece p = {x, y, s};
+        cur[ncur++] = p;
+      }
+    }
+  uint64_t blocked = 0;
+  if (crossing) {
+    uint64_t full = (w * h == 64) ? ~0ull : ((1ull << (w * h)) - 1ull);
+    blocked = full;
+    for (int i = 0; i < ncur; i++) blocked &= ~piece_mask(cur[i].x, cur[i].y, cur[i].s, w);
+  }
+  for (int i = 0; i < ncur; i++)
+    if (cur[i].s == JXG_IDENTITY || cur[i].s == JXG_DCT2X2) return;
+  float costs[4][64];
+  for (int k = 0; k < 4; k++)
+    for (int i = 0; i < 64; i++) costs[k][i] = INFINITY;
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      const int i = y * w + x;
+      for (int choice = SM_LEAF; choice <= SM_SQUARE; choice++) {
+        const int s = small_strategy(choice);
+        if (x + jxo_covered_x(s) > w || y + jxo_covered_y(s) > h) continue;
+        if (blocked && (piece_mask(x, y, s, w) & blocked)) continue;
+        const int gx = bx + x, gy = by + y;
+        const int off_grid = choice == SM_WIDE   ? (gx % 2 != 0)
+                             : choice == SM_TALL ? (gy % 2 != 0)
+                             : choice == SM_SQUARE ? (gx % 2 != 0 || gy % 2 != 0)
+                                                   : 0;
+        if (off_grid && !unaligned) continue;
+        const int legal = crossing ? can_place_crossing(gx, gy, s, g->xsize, g->ysize)
+                                   : can_place(gx, gy, s, g->xsize, g->ysize);
+        if (legal) costs[choice - 1][i] = piece_score(g, m, s, gx, gy);
+      }
+    }
+  uint64_t fixed = 0;
+  float fixed_cost = 0.0f;
+  for (int i = 0; i < ncur; i++) {
+    if (is_coefficient_strategy(cur[i].s)) continue;
+    fixed |= piece_mask(cur[i].x, cur[i].y, cur[i].s, w);
+    fixed_cost += piece_score(g, m, cur[i].s, bx + cur[i].x, by + cur[i].y);
+  }
+  uint8_t live[64];
+  jxo_live_masks(costs[0], costs[1], costs[2], costs[3], w, h, live);
+  const float around = plan_small(costs[0], costs[1], costs[2], costs[3], live, fixed | blocked, w,
+                                  h, sp, INFINITY) +
+                       fixed_cost;
+  int keep_fixed = 1;
+  float winner = around;
+  if (fixed) {
+    const float over =
+        plan_small(costs[0], costs[1], costs[2], costs[3], live, blocked, w, h, sp, around);
+    if (over < around) {
+      winner = over;
+      keep_fixed = 0;
+    }
+  }
+  if (!isfinite(winner)) return;
+  tpiece lay[64];
+  int nlay = 0;
+  for (int k = 0; k < sp->plan_n; k++) {
+    tpiece p = {sp->plan_cell[k] % w, sp->plan_cell[k] / w, small_strategy(sp->plan_choice[k])};
+    lay[nlay++] = p;
+  }
+  if (keep_fixed)
+    for (int i = 0; i < ncur; i++)
+      if (!is_coefficient_strategy(cur[i].s)) lay[nlay++] = cur[i];
+  qsort(lay, (size_t)nlay, sizeof(tpiece), tpiece_cmp);
+  float old_cost = 0.0f, new_cost = 0.0f;
+  for (int i = 0; i < ncur; i++)
+    if (!tpiece_in(lay, nlay, cur[i]))
+      old_cost += piece_score(g, m, cur[i].s, bx + cur[i].x, by + cur[i].y);
+  for (int i = 0; i < nlay; i++)
+    if (!tpiece_in(cur, ncur, lay[i]))
+      new_cost += piece_score(g, m, lay[i].s, bx + lay[i].x, by + lay[i].y);
+  if (!(new_cost < old_cost)) return;
+  for (int i = 0; i < nlay; i++)
+    if (!tpiece_in(cur, ncur, lay[i])) set_first(g, bx + lay[i].x, by + lay[i].y, lay[i].s);
+}
+
+/* layout_dp_coefficient / _unaligned / layout_crossing per speed
+ * (src/effort.rs:311-349). */
+static void retile_flags(const jxg_frame_params *p, int *on, int *unaligned, int *crossing) {
+  *on = p->speed == JXG_SPEED_FASTEST || p->speed == JXG_SPEED_FAST;
+  *unaligned = p->speed == JXG_SPEED_FAST;
+  *crossing = p->speed == JXG_SPEED_FAST;
+}
+
+/* One re-tiling pass over a DC group: the aligned grid (crossing = 0, only
+ * tiles whose transforms stay inside them) or the grid shifted by (4, 4)
+ * blocks (find_coefficient, layout_dp.rs:2340-2379). */
+static void retile_pass(group_ctx *g, retile_memo *m, small_planner *sp, int crossing,
+                        int unaligned) {
+  const int o = crossing ? 4 : 0;
+  for (int by = o; by < g->ysize; by += 8)
+    for (int bx = o; bx < g->xsize; bx += 8) {
+      const int w = g->xsize - bx < 8 ? g->xsize - bx : 8;
+      const int h = g->ysize - by < 8 ? g->ysize - by : 8;
+      if (w * h < 2 || (!crossing && !footprint_closed(g, bx, by, w, h))) continue;
+      retile_tile(g, m, sp, bx, by, w, h, crossing, unaligned);
+    }
+}
+
+static void retile_coefficient(group_ctx *g, int passes_mask) {
+  int on, unaligned, crossing;
+  retile_flags(g->p, &on, &unaligned, &crossing);
+  if (!on) return;
+  retile_memo m;
+  size_t n = (size_t)g->xsize * g->ysize * 5;
+  m.memo = (float *)malloc(n * sizeof(float));
+  for (size_t i = 0; i < n; i++) m.memo[i] = NAN;
+  small_planner *sp = (small_planner *)malloc(sizeof(small_planner));
+  small_planner_init(sp);
+  if (passes_mask & 1) retile_pass(g, &m, sp, 0, unaligned);
+  if ((passes_mask & 2) && crossing) retile_pass(g, &m, sp, 1, unaligned);
+  free(sp);
+  free(m.memo);
+}
+
 static void adjust_quant_field(group_ctx *g, uint8_t *qf_out) {
   for (int by = 0; by < g->ysize; by++)
     for (int bx = 0; bx < g->xsize; bx++) {
@@ -685,9 +1060,24 @@ void jxo_select_strategy(const float *xyb, int width, int height, uint8_t *qf,
       group_ctx g;
       group_init(&g, xyb, width, height, gx, gy, qf_in, ytox, ytob, tables, p, cells);
       select_group(&g);
+      retile_coefficient(&g, 3);
       adjust_quant_field(&g, qf);
     }
   free(qf_in);
+}
+
+/* One re-tiling pass (aligned: crossing = 0; shifted: 1) over every DC group
+ * of a given layout, the field left as it is (test entry point). */
+void jxo_retile_pass(const float *xyb, int width, int height, const uint8_t *qf,
+                     const int8_t *ytox, const int8_t *ytob, const float *tables,
+                     const jxg_frame_params *p, uint8_t *cells, int crossing) {
+  jxo_set_fused(p->fused);
+  for (int gy = 0; gy * 2048 < height; gy++)
+    for (int gx = 0; gx * 2048 < width; gx++) {
+      group_ctx g;
+      group_init(&g, xyb, width, height, gx, gy, qf, ytox, ytob, tables, p, cells);
+      retile_coefficient(&g, crossing ? 2 : 1);
+    }
 }
 
 void jxo_strategy_costs(const float *xyb, int width, int height, const uint8_t *qf,
@@ -786,6 +1176,7 @@ static void *fe_worker(void *arg) {
       group_ctx g;
       group_init(&g, j->xyb, W, H, gx, gy, j->qf_in, j->ytox, j->ytob, j->tables, j->p, j->cells);
       select_group(&g);
+      retile_coefficient(&g, 3);
       adjust_quant_field(&g, j->qf);
     } else if (j->kind == 4) {
       const int tiles_x = (W + 63) / 64;

@@ -122,6 +122,12 @@ def _declare(L):
     L.jxo_aggregate_qac_2x2.restype = c.c_float
     L.jxo_aggregate_qac_2x2.argtypes = [_f32p, c.c_float, c.c_float]
     L.jxo_cmap_factors.argtypes = [c.c_int, c.c_int, c.c_void_p, _f32p]
+    L.jxo_plan_small.restype = c.c_float
+    L.jxo_plan_small.argtypes = [_f32p, _f32p, _f32p, _f32p, c.c_uint64, c.c_int, c.c_int,
+                                 c.c_float, c.POINTER(c.c_int), _u8p, _u8p]
+    L.jxo_live_masks.argtypes = [_f32p, _f32p, _f32p, _f32p, c.c_int, c.c_int, _u8p]
+    L.jxo_retile_pass.argtypes = [_f32p, c.c_int, c.c_int, _u8p, _i8p, _i8p, _f32p, c.c_void_p,
+                                  _u8p, c.c_int]
 
 
 def _p(a, t):
@@ -222,6 +228,41 @@ def select_strategy(xyb, qf, ytox, ytob, tables, params):
                               _p(np.ascontiguousarray(tables, np.float32), _f32p),
                               ctypes.byref(params), _p(cells, _u8p))
     return cells, qf
+
+
+def plan_small(leaf, wide, tall, square, fixed, w, h, replace_below=float("inf")):
+    """plan_small over a w x h grid: (optimum, [(cell, choice), ...] last
+    cell first, or None when no plan was saved).  Costs are 64-entry
+    arrays indexed y * w + x."""
+    arrs = [np.ascontiguousarray(np.asarray(a, np.float32).reshape(64)) for a in
+            (leaf, wide, tall, square)]
+    n = ctypes.c_int(0)
+    cells = np.zeros(64, np.uint8)
+    choices = np.zeros(64, np.uint8)
+    t = lib().jxo_plan_small(*[_p(a, _f32p) for a in arrs], int(fixed), w, h, replace_below,
+                             ctypes.byref(n), _p(cells, _u8p), _p(choices, _u8p))
+    return float(t), [(int(cells[i]), int(choices[i])) for i in range(n.value)]
+
+
+def live_masks(leaf, wide, tall, square, w, h):
+    arrs = [np.ascontiguousarray(np.asarray(a, np.float32).reshape(64)) for a in
+            (leaf, wide, tall, square)]
+    live = np.zeros(64, np.uint8)
+    lib().jxo_live_masks(*[_p(a, _f32p) for a in arrs], w, h, _p(live, _u8p))
+    return live
+
+
+def retile_pass(xyb, qf, ytox, ytob, tables, params, cells, crossing):
+    """One coefficient re-tiling pass over `cells` (returned updated)."""
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    cells = np.array(cells, np.uint8, copy=True)
+    lib().jxo_retile_pass(_p(xyb, _f32p), w, h, _p(np.ascontiguousarray(qf, np.uint8), _u8p),
+                          _p(np.ascontiguousarray(ytox, np.int8), _i8p),
+                          _p(np.ascontiguousarray(ytob, np.int8), _i8p),
+                          _p(np.ascontiguousarray(tables, np.float32), _f32p),
+                          ctypes.byref(params), _p(cells, _u8p), 1 if crossing else 0)
+    return cells
 
 
 def strategy_costs(xyb, qf, ytox, ytob, tables, params):

@@ -67,7 +67,9 @@ def test_batch_matches_single_image_path(J, fe, torch, fused):
     try:
         b.run()
         torch.cuda.synchronize()
-        assert b.status() == (0, 0)
+        # tile-crossing transforms (the shifted re-tiling grid places them)
+        # are queued, none is unsupported
+        assert b.status()[1] == 0
         for i, (rgb, p) in enumerate(dev):
             want = fe.run(rgb, rgb.shape[1], rgb.shape[0], p)
             same_outputs(b.outputs(i), want, f"image {i} {tuple(rgb.shape)}")

@@ -41,13 +41,19 @@ KERNEL_BYTES_PER_PX = {
     "select": 12.0 + 2.0 / 64,   # 3 planes in, cells + field out
     "alpha_qf": 1.0 + 2.0 / 64,  # alpha in, field rewritten
     "quantize": 12.0 + 24.0 + 6.0 / 64 + 2.0 / 64,  # planes in, i32 + f32 out, DC, qf, cells
-    "quantize_xt": 0.0,          # the tile-crossing queue (empty at this tier)
+    "quantize_xt": 0.0,          # the tile-crossing queue (its bytes are a sliver of quantize's)
+    "retile": 12.0 + 1.0 / 64,   # 3 planes in (the piece costs), field and cells rewritten
 }
 # FP32 work of the strategy search per pixel: the DCT8 / 16x16 / 32x32
 # candidates' butterflies (4 log2 N flops per pixel per channel for an N x N
 # transform: 12 + 16 + 20) x 3 channels, plus about 10 flops per coefficient
 # for the quantize / SSE / rate terms of 3 candidates x 3 channels.
 SELECT_FLOPS_PER_PX = 3 * (12 + 16 + 20) + 3 * 3 * 10
+# FP32 work of the re-tiling's piece costs per pixel: at every block a DCT8,
+# a 16x8, an 8x16 and a 16x16 (2 log2 N flops per pixel per axis: 768 +
+# 1792 + 1792 + 4096 per block and channel) and about 10 flops per
+# coefficient (576 per block and channel) for the RD terms, 3 channels.
+RETILE_FLOPS_PER_PX = 3 * (768 + 2 * 1792 + 4096 + 10 * 576) / 64
 
 EXTRA_BYTES_PER_PX = {
     "gaborish": 12.0 + 12.0,  # 3 planes in, 3 planes out
@@ -63,7 +69,8 @@ STAGE_KERNEL = {"xyb": ["k_xyb"], "cmap": ["k_cmap_blocks", "k_cmap_solve"],
                 "fill": ["k_fill_bands"],
                 "alpha_prep": ["k_alpha_extract", "k_fringe", "k_fringe_commit", "k_astat_init",
                                "k_alpha_stats", "k_fill_u32", "k_repad", "k_alpha_blocks"],
-                "alpha_qf": ["k_alpha_apply"]}
+                "alpha_qf": ["k_alpha_apply"],
+                "retile": ["k_piece_costs", "k_retile", "k_adjust_qf"]}
 # fill_invisible's recurrence: anti-diagonals x + 2 y are its dependent steps,
 # and each step's chain is 14 dependent VALU operations (one DPP wave shift,
 # 4 fused multiply-adds and 3 adds for the sum, 5 FP64 operations for the
@@ -542,13 +549,15 @@ def roofline_of(kernel_ms, px_shape, traffic_stage_ok=True):
                               "cycles_per_op": 4, "clock_ghz": CLOCK_GHZ,
                               "ideal_ms": round(ideal, 4),
                               "frac": round(ideal / kernel_ms[dominant], 4)}
-    if dominant == "select":
-        tflops = SELECT_FLOPS_PER_PX * px / (kernel_ms[dominant] * 1e-3) / 1e12
+    if dominant in ("select", "retile"):
+        fpp = SELECT_FLOPS_PER_PX if dominant == "select" else RETILE_FLOPS_PER_PX
+        tflops = fpp * px / (kernel_ms[dominant] * 1e-3) / 1e12
         r["limiter"] = "VALU issue, not HBM: the candidate transforms and RD sums run in LDS"
         r["fp32"] = {"achieved_tflops": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
                      "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
-                     "flops_per_px": SELECT_FLOPS_PER_PX}
-        r["valu_busy"] = read_valu_busy("k_select") if traffic_stage_ok else None
+                     "flops_per_px": round(fpp, 1)}
+        if dominant == "select":
+            r["valu_busy"] = read_valu_busy("k_select") if traffic_stage_ok else None
     return r
 
 
@@ -658,9 +667,10 @@ def main():
             "workload": HEADLINE,
             "width": w, "height": h, "channels": 4, "quality": args.quality,
             "distance": round(p.distance, 4), "speed": "fast",
-            "decisions": "square merges DCT8/DCT16X16/DCT32X32 + adjust_quant_field; the "
-                         "reference's FAST tier also runs layout_dp::retile_coefficient and "
-                         "refine_quant_field, which are not built",
+            "decisions": "square merges DCT8/DCT16X16/DCT32X32, layout_dp::retile_coefficient "
+                         "(aligned + (4,4)-shifted grids, unaligned pairs and squares), "
+                         "adjust_quant_field; the reference's FAST tier also runs "
+                         "refine_quant_field at d >= 1.5, which is not built",
             "alpha": "alpha_fringe::clear, fill_invisible, alpha block factors",
             "cfl": "per-tile regression maps (fill_cmap, Fast path)",
             "tables": "DequantMatrices::new_fast tier 3 (SS2 scales, flat-B DCT8 row)",

@@ -860,7 +860,8 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
                                                   float *__restrict__ c8out,
                                                   float *__restrict__ c16out,
                                                   float *__restrict__ c32out, float *__restrict__ c32raw,
-                                                  int xb, int yb, int bix, int biy) {
+                                                  float *__restrict__ c8raw, int xb, int yb, int bix,
+                                                  int biy) {
   __shared__ SelLds L;
   __shared__ uint8_t q_in[16];
   __shared__ uint8_t cell_out[16];
@@ -1101,6 +1102,12 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
     }
     __syncthreads();
     if (c32raw && tid == 0) c32raw[(size_t)rby * xb + rbx] = (four_row && four_col) ? L.cost32 : NAN;
+    if (c8raw && tid < 16) {
+      // the DCT8 costs for the re-tiling (NaN where the search did not price one)
+      const int ix = tid & 3, iy = tid >> 2;
+      const bool eval = (four_row ? true : iy < 2) && (bx + (ix >> 1) * 2 + 1 < xsize);
+      if (rbx + ix < xb && rby + iy < yb) c8raw[(size_t)(rby + iy) * xb + rbx + ix] = eval ? L.cost8[tid] : NAN;
+    }
     if (c8out && tid < 16) {
       const int ix = tid & 3, iy = tid >> 2;
       const int sx = ix >> 1, sy = iy >> 1;
@@ -1113,7 +1120,11 @@ __device__ __forceinline__ void select_body(const float *__restrict__ xyb, int s
         c32out[o] = (tid == 0 && four_row && four_col) ? L.cost32 : NAN;
       }
     }
-  } else if (c8out && tid < 16) {
+  } else if (c8raw && tid < 16) {
+    const int ix = tid & 3, iy = tid >> 2;
+    if (rbx + ix < xb && rby + iy < yb) c8raw[(size_t)(rby + iy) * xb + rbx + ix] = NAN;
+  }
+  if (!work && c8out && tid < 16) {
     const int ix = tid & 3, iy = tid >> 2;
     if (rbx + ix < xb && rby + iy < yb) {
       const size_t o = (size_t)(rby + iy) * xb + rbx + ix;
@@ -1146,9 +1157,10 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select(const float *__restrict__ x
                                                   float *__restrict__ c8out,
                                                   float *__restrict__ c16out,
                                                   float *__restrict__ c32out,
-                                                  float *__restrict__ c32raw, int xb, int yb) {
+                                                  float *__restrict__ c32raw,
+                                                  float *__restrict__ c8raw, int xb, int yb) {
   select_body<F>(xyb, stride, padded_h, W, H, qf, ytox, ytob, tables, scans, selq, D, cells, c8out,
-                 c16out, c32out, c32raw, xb, yb, blockIdx.x, blockIdx.y);
+                 c16out, c32out, c32raw, c8raw, xb, yb, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -2033,7 +2045,8 @@ __global__ __launch_bounds__(SEL_T, 5) void k_select_b(const BatchImg *__restric
                                                     const uint32_t *__restrict__ scans) {
   BATCH_BLOCK(BS_SEL)
   select_body<F>(I.xyb, I.stride, I.ph, I.W, I.H, I.qf, I.ytox, I.ytob, I.tables, scans, I.selq, I.D,
-                 I.cells, nullptr, nullptr, nullptr, I.c32, I.xb, I.yb, bx_, by_);
+                 I.cells, nullptr, nullptr, nullptr, I.c32, I.D->retile ? I.pc : nullptr, I.xb,
+                 I.yb, bx_, by_);
 }
 
 struct BatchQueue {
@@ -2935,18 +2948,19 @@ static int select_impl(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32
   // The Fast / Fastest decision re-tiles the merge search's layout
   // (strategy decisions only: the cost dump keeps the search's own field).
   const int retile = cells ? ctx->h_derived->retile : 0;
-  float *c32raw = nullptr;
+  float *c32raw = nullptr, *c8raw = nullptr;
   if (retile) {
     const size_t nb = (size_t)xb * yb;
     if ((rc = ensure(&ctx->d_pc, &ctx->pc_cap, 4 * nb))) return rc;
     if ((rc = ensure(&ctx->d_c32, &ctx->c32_cap, nb))) return rc;
     c32raw = ctx->d_c32;
+    c8raw = ctx->d_pc;  // the leaf plane of the piece costs
   }
   prof_begin(ctx, JXG_PROF_SELECT, ctx->stream);
   if (p->fused)
-    k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, c32raw, xb, yb);
+    k_select<true><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, c32raw, c8raw, xb, yb);
   else
-    k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, c32raw, xb, yb);
+    k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, c32raw, c8raw, xb, yb);
   LAUNCH_CHECK("k_select");
   prof_end(ctx, JXG_PROF_SELECT, ctx->stream);
   if (retile)

@@ -7,27 +7,30 @@
 //   k_piece_costs  one 256-thread workgroup per 32x32-px region: the
 //                  coefficient-model cost (strategy_cost with META_R and the
 //                  region quantizer of the field before its adjustment) of a
-//                  DCT8, a DCT8X16 (two blocks wide), a DCT16X8 (two tall)
-//                  and a DCT16X16 anchored at each of its 16 blocks.  The
-//                  region and its one-block right / bottom margin are staged
-//                  in LDS once; each shape's sites are transformed there in
-//                  the reference's pass order and scored like k_select's
-//                  candidates (sums in storage order, f32).
-//   k_retile       one wavefront per 8x8-block tile of a DC group (the
-//                  aligned grid, then at Fast the grid shifted by (4, 4)):
-//                  the exact tiling of plan_small (a broken-profile DP over
-//                  the cells in row-major order, 2^(w+1) frontier states)
-//                  around and over the incumbent 32x32 blocks; the new pieces
-//                  replace the old ones when they price lower.
+//                  DCT8X16 (two blocks wide), a DCT16X8 (two tall) and a
+//                  DCT16X16 anchored at each of its 16 blocks (and a DCT8
+//                  where the strategy search has not priced one).  Each
+//                  shape's sites are staged from the planes into LDS,
+//                  transformed in the reference's pass order and scored like
+//                  k_select's candidates; the squared errors are summed in
+//                  storage order, the rate terms over the nonzero levels
+//                  only (a zero term adds nothing to a non-negative f32 sum).
+//   k_retile       one 256-thread workgroup per 8x8-block tile of a DC
+//                  group (the aligned grid, then at Fast the grid shifted by
+//                  (4, 4)): the exact tiling of plan_small (a broken-profile
+//                  DP over the cells in row-major order, 2^(w+1) frontier
+//                  states) around and over the incumbent 32x32 blocks; the
+//                  new pieces replace the old ones when they price lower.
 //   k_adjust_qf    per transform: the aggregate quant value over its blocks.
 //
-// The DP runs one lane per frontier state, pulling each state's at most five
-// predecessors.  The reference relaxes states in the order they entered the
-// layer and keeps the first strictly cheaper arrival, so each state keeps
-// the minimum of (cost, rank of the predecessor * 8 + choice), and its rank
-// in the next layer is the rank of its first arrival among all states, read
-// off a bitmap of arrival keys with population counts.  That reproduces the
-// reference's choice among equal-cost tilings.
+// The DP gives each thread two frontier states and has each state pull its
+// at most five predecessors.  The reference relaxes states in the order they
+// entered the layer and keeps the first strictly cheaper arrival, so each
+// state keeps the minimum of (cost, rank of the predecessor * 8 + choice),
+// and its rank in the next layer is the rank of its first arrival among all
+// states, read off a bitmap of arrival keys with population counts.  That
+// reproduces the reference's choice among equal-cost tilings.  A cell that
+// is filled before only shifts the states (cost and order kept).
 #include <hip/hip_runtime.h>
 
 #include "jxg_batch.h"
@@ -74,20 +77,19 @@ struct PieceShape<3> {
   static constexpr bool ROWS_FIRST = false, TRANS = true;
 };
 
-#define PC_WIN 40
-#define PC_WP 41
 #define PC_WORK 2304  // the largest batch: 16 sites of 16 x 9 (tall pieces)
 
 struct PcLds {
-  float win[3][PC_WIN * PC_WP];  // the region and its right / bottom block margin
-  float wy[PC_WORK];             // luma coefficients of the batch's sites
-  float wc[PC_WORK];             // X or B coefficients
-  float sq[2048], mg[2048];      // per coefficient: squared error, rate term (storage order)
-  float quot[3][64];             // 0, 1 - one, numerator / a
-  float rate[256];               // log2(1 + i)
-  float qac[16];                 // per site of the shape: region quantizer
-  uint32_t nz[16], mx[16];       // nonzero count, 1 + last nonzero scan position
+  float wy[PC_WORK];         // luma coefficients of the batch's sites
+  float wc[PC_WORK];         // X or B coefficients
+  float sq[2048], mg[2048];  // per coefficient: squared error, rate term (storage order)
+  uint8_t sp[2048];          // per coefficient: its scan position (< 256 for these shapes)
+  uint64_t nzm[16][4];       // per site and 64-coefficient chunk: the nonzero levels
+  float quot[3][64];         // 0, 1 - one, numerator / a
+  float rate[256];           // log2(1 + i)
+  float qac[16];             // per site of the shape: region quantizer
   float sse[3][16], bits[3][16];
+  float cost[4][16];
 };
 
 struct PieceArgs {
@@ -112,10 +114,10 @@ __device__ __forceinline__ uint8_t rt_aggregate_quant(uint8_t max_q, uint32_t su
 
 // One batch of one shape: sites b * NS .. b * NS + NS - 1 of the region's 16
 // anchors (anchor a at block (a & 3, a >> 2)).  Writes the sites' costs to
-// cost[] (NaN where the piece leaves the image or a 256-px AC group).
+// L.cost[SH] (NaN where the piece leaves the image or a 256-px AC group).
 template <bool F, int SH>
-__device__ void piece_batch(PcLds &L, const PieceArgs &A, int b, int rbx, int rby, float fx,
-                            float fb, float *cost) {
+__device__ __forceinline__ void piece_batch(PcLds &L, const PieceArgs &A, int b, int rbx, int rby,
+                                            float fx, float fb) {
   using P = PieceShape<SH>;
   constexpr int W = P::W, H = P::H, NS = P::NS, PITCH = W + 1, SS = H * PITCH, SIZE = W * H;
   constexpr int SW = W > H ? W : H, SHH = W > H ? H : W;  // storage width / height
@@ -127,6 +129,7 @@ __device__ void piece_batch(PcLds &L, const PieceArgs &A, int b, int rbx, int rb
   const jxg_derived *D = A.D;
   const int cls = jxg_area_class(CX, CY);
   const uint32_t qoff = D->qt_off[KIND], qn = D->qt_n[KIND];
+  const size_t plane = (size_t)A.stride * A.ph;
   // the sites' quantizers: region_qac of the footprint (max starts at 1)
   if (tid < NS) {
     const int a = b * NS + tid, ax = rbx + (a & 3), ay = rby + (a >> 2);
@@ -139,14 +142,16 @@ __device__ void piece_batch(PcLds &L, const PieceArgs &A, int b, int rbx, int rb
         sum += q;
       }
     L.qac[tid] = D->scale * (float)rt_aggregate_quant(max_q, sum, CW * CH, D->distance);
-    L.nz[tid] = 0;
-    L.mx[tid] = 0;
   }
+  // the sites' pixels from the (L2-resident) planes; past the padded
+  // planes 0 (no piece that reads them lies inside the image)
   auto stage = [&](float *dst, int c) {
     for (int e = tid; e < NS * SIZE; e += 256) {
       const int site = e / SIZE, rem = e - site * SIZE, r = rem / W, col = rem - r * W;
       const int a = b * NS + site;
-      dst[site * SS + r * PITCH + col] = L.win[c][((a >> 2) * 8 + r) * PC_WP + (a & 3) * 8 + col];
+      const int y = (rby + (a >> 2)) * 8 + r, x = (rbx + (a & 3)) * 8 + col;
+      dst[site * SS + r * PITCH + col] =
+          (y < A.ph && x < A.stride) ? A.xyb[c * plane + (size_t)y * A.stride + x] : 0.0f;
     }
   };
   auto transform = [&](float *buf) {
@@ -168,6 +173,21 @@ __device__ void piece_batch(PcLds &L, const PieceArgs &A, int b, int rbx, int rb
       __syncthreads();
     }
   };
+  // Per thread, its coefficients' geometry (the same for every channel):
+  // offset in the transform buffer, AC flag, threshold quadrant; and
+  // their scan positions into LDS.
+  constexpr int ITERS = NS * SIZE / 256;
+  int geo[ITERS];
+#pragma unroll
+  for (int it = 0; it < ITERS; it++) {
+    const int e = it * 256 + tid, site = e / SIZE, k = e - site * SIZE;
+    const int r = P::TRANS ? k % H : k / W, col = P::TRANS ? k / H : k % W;
+    const int srow = k / SW, scol = k - srow * SW;
+    const int ac = !(scol < CX && srow < CY);
+    const int tsel = (srow >= SHH / 2 ? 2 : 0) + (scol >= SW / 2 ? 1 : 0);
+    geo[it] = (site * SS + r * PITCH + col) | (tsel << 12) | (ac << 14);
+    L.sp[e] = (uint8_t)A.scanpos[SPOFF + k];
+  }
   // Coefficient terms of channel c (sse_and_rate_scalar_impl,
   // src/inflated_cost.rs:435-500) and the per-site sums in storage order.
   auto channel = [&](int c) {
@@ -176,20 +196,24 @@ __device__ void piece_batch(PcLds &L, const PieceArgs &A, int b, int rbx, int rb
     const float f = c == 0 ? fx : fb;
     const float t0 = D->thr[c][cls][0], t1 = D->thr[c][cls][1];
     const float t2 = D->thr[c][cls][2], t3 = D->thr[c][cls][3];
-    for (int e0 = 0; e0 < NS * SIZE; e0 += 256) {
+    float invk[ITERS];
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+      const int e = it * 256 + tid;
+      invk[it] = inv[e - (e / SIZE) * SIZE];
+    }
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+      const int e0 = it * 256;
       const int e = e0 + tid;
       const int site = __builtin_amdgcn_readfirstlane((e0 + (tid & ~63)) / SIZE);
-      const int k = e - site * SIZE;
-      const int r = P::TRANS ? k % H : k / W, col = P::TRANS ? k / H : k % W;
-      const int srow = k / SW, scol = k - srow * SW;
-      const int off = site * SS + r * PITCH + col;
+      const int off = geo[it] & 4095, tsel = (geo[it] >> 12) & 3;
+      const bool ac = (geo[it] >> 14) & 1;
       const float yv = L.wy[off];
       const float coef = c == 1 ? yv : fmla<F>(-f, yv, L.wc[off]);
-      const bool ac = !(scol < CX && srow < CY);
-      const int yfix = srow >= SHH / 2 ? 2 : 0;
-      const float thr = scol >= SW / 2 ? (yfix ? t3 : t1) : (yfix ? t2 : t0);
+      const float thr = tsel >= 2 ? (tsel == 3 ? t3 : t2) : (tsel == 1 ? t1 : t0);
       const float qs = L.qac[site] * qm;
-      const float a = inv[k] * qs * coef;
+      const float a = invk[it] * qs * coef;
       const float q = (ac && fabsf(a) >= thr) ? round_away(a) : 0.0f;
       const float aq = fabsf(q);
       float lev;
@@ -210,18 +234,12 @@ __device__ void piece_batch(PcLds &L, const PieceArgs &A, int b, int rbx, int rb
       }
       L.mg[e] = rt;
       const uint64_t m = __ballot(q != 0.0f);
-      uint32_t pos1 = q != 0.0f ? (uint32_t)A.scanpos[SPOFF + k] + 1u : 0u;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) pos1 = max(pos1, (uint32_t)__shfl_xor((int)pos1, o));
-      if (lane == 0) {
-        atomicAdd(&L.nz[site], (uint32_t)__popcll(m));
-        atomicMax(&L.mx[site], pos1);
-      }
+      if (lane == 0) L.nzm[site][(e - site * SIZE) >> 6] = m;
     }
     __syncthreads();
-    if (tid < 2 * NS) {
-      const int site = tid >> 1, which = tid & 1;
-      const float4 *src = reinterpret_cast<const float4 *>((which ? L.mg : L.sq) + site * SIZE);
+    if (tid < NS) {
+      // squared errors: every term, in storage order
+      const float4 *src = reinterpret_cast<const float4 *>(L.sq + tid * SIZE);
       float acc = 0.0f;
       for (int k4 = 0; k4 < SIZE / 4; k4 += 4) {
         float4 v[4];
@@ -235,27 +253,38 @@ __device__ void piece_batch(PcLds &L, const PieceArgs &A, int b, int rbx, int rb
           acc += v[i].w;
         }
       }
-      if (which == 0) {
-        L.sse[c][site] = acc;
-      } else {
-        // model_bits (src/inflated_cost.rs:540-546)
-        const uint32_t nzc = L.nz[site];
-        const float nzf = (float)nzc;
-        const float header =
-            0.4f * (nzc < 256u ? L.rate[nzc] : (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf)));
-        float visited = 0.0f;
-        if (nzc != 0) {
-          const int64_t span = (int64_t)L.mx[site] - (int64_t)(CX * CY + nzc);
-          visited = span > 0 ? (float)span : 0.0f;
+      L.sse[c][tid] = acc;
+    } else if (tid >= 64 && tid < 64 + NS) {
+      // rate terms: only the nonzero levels contribute (adding a zero term
+      // to a non-negative sum changes no bit), in storage order; their
+      // count and the last one's scan position on the way
+      const int site = tid - 64;
+      float acc = 0.0f;
+      uint32_t nzc = 0, mx = 0;
+#pragma unroll
+      for (int ch = 0; ch < SIZE / 64; ch++) {
+        uint64_t m = L.nzm[site][ch];
+        while (m) {
+          const int i = __builtin_ctzll(m);
+          m &= m - 1;
+          const int e = site * SIZE + ch * 64 + i;
+          acc += L.mg[e];
+          mx = max(mx, (uint32_t)L.sp[e] + 1u);
+          nzc++;
         }
-        L.bits[c][site] = nzf * 1.6f + 1.0f * acc + header + 0.5f * visited;
       }
+      // model_bits (src/inflated_cost.rs:540-546)
+      const float nzf = (float)nzc;
+      const float header =
+          0.4f * (nzc < 256u ? L.rate[nzc] : (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf)));
+      float visited = 0.0f;
+      if (nzc != 0) {
+        const int64_t span = (int64_t)mx - (int64_t)(CX * CY + nzc);
+        visited = span > 0 ? (float)span : 0.0f;
+      }
+      L.bits[c][site] = nzf * 1.6f + 1.0f * acc + header + 0.5f * visited;
     }
     __syncthreads();
-    if (tid < NS) {
-      L.nz[tid] = 0;
-      L.mx[tid] = 0;
-    }
   };
   // Y: transform and keep; then X and B against it.
   stage(L.wy, 1);
@@ -281,15 +310,14 @@ __device__ void piece_batch(PcLds &L, const PieceArgs &A, int b, int rbx, int rb
     const float visible = (vw == W && vh == H) ? 1.0f : (float)(vw * vh) / (float)(W * H);
     const bool valid = ax + CW <= A.xb && ay + CH <= A.yb && (ax >> 5) == ((ax + CW - 1) >> 5) &&
                        (ay >> 5) == ((ay + CH - 1) >> 5);
-    cost[a] = valid ? fmla<F>(RT_RD_LAMBDA, r_total + RT_META_R, d_total * visible) : __int_as_float(0x7fc00000);
+    L.cost[SH][a] = valid ? fmla<F>(RT_RD_LAMBDA, r_total + RT_META_R, d_total * visible) : __int_as_float(0x7fc00000);
   }
   __syncthreads();
 }
 
 template <bool F>
-__device__ void piece_costs_body(const PieceArgs &A, int bix, int biy) {
+__device__ __forceinline__ void piece_costs_body(const PieceArgs &A, int bix, int biy) {
   __shared__ PcLds L;
-  __shared__ float cost[4][16];
   const int tid = threadIdx.x;
   const int rbx = bix * 4, rby = biy * 4;
   const jxg_derived *D = A.D;
@@ -302,22 +330,6 @@ __device__ void piece_costs_body(const PieceArgs &A, int bix, int biy) {
     }
     L.rate[tid] = D->rate_lut[tid];
   }
-  // the region's 40x40 window; float4 chunks past the padded planes are 0
-  // (no piece that reads them lies inside the image)
-  const size_t plane = (size_t)A.stride * A.ph;
-  const int px0 = rbx * 8, py0 = rby * 8;
-  for (int e = tid; e < 3 * PC_WIN * (PC_WIN / 4); e += 256) {
-    const int c = e / (PC_WIN * 10), rem = e - c * PC_WIN * 10, r = rem / 10, q = rem - r * 10;
-    const int y = py0 + r, x = px0 + 4 * q;
-    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (y < A.ph && x + 4 <= A.stride)
-      v = *reinterpret_cast<const float4 *>(A.xyb + c * plane + (size_t)y * A.stride + x);
-    float *dst = &L.win[c][r * PC_WP + 4 * q];
-    dst[0] = v.x;
-    dst[1] = v.y;
-    dst[2] = v.z;
-    dst[3] = v.w;
-  }
   // cmap_factors (src/ac_strategy.rs:1838-1855): the region's 64x64 tile,
   // clamped to its DC group's tile grid
   const int gx = rbx >> 8, gy = rby >> 8;
@@ -329,15 +341,24 @@ __device__ void piece_costs_body(const PieceArgs &A, int bix, int biy) {
   const size_t ti = (size_t)(gy * 32 + ty) * tiles_x + gx * 32 + tx;
   const float fx = fmla<F>((float)A.ytox[ti], D->cfl_inv_color_factor, D->cfl_base_x);
   const float fb = fmla<F>((float)A.ytob[ti], D->cfl_inv_color_factor, D->cfl_base_b);
-  __syncthreads();
-  piece_batch<F, 0>(L, A, 0, rbx, rby, fx, fb, cost[0]);
-  piece_batch<F, 1>(L, A, 0, rbx, rby, fx, fb, cost[1]);
-  piece_batch<F, 2>(L, A, 0, rbx, rby, fx, fb, cost[2]);
-  piece_batch<F, 3>(L, A, 0, rbx, rby, fx, fb, cost[3]);
-  piece_batch<F, 3>(L, A, 1, rbx, rby, fx, fb, cost[3]);
+  // The strategy search has priced the DCT8 of (nearly) every block in the
+  // leaf plane already (the same strategy_cost); the leaves are priced here
+  // only when one of the region's blocks lacks one.
+  bool need_leaf = false;
+  if (tid < 16) {
+    const int x = rbx + (tid & 3), y = rby + (tid >> 2);
+    if (x < A.xb && y < A.yb) need_leaf = isnan(A.pc[(size_t)y * A.xb + x]);
+  }
+  need_leaf = __syncthreads_or(need_leaf);
+  if (need_leaf) piece_batch<F, 0>(L, A, 0, rbx, rby, fx, fb);
+  piece_batch<F, 1>(L, A, 0, rbx, rby, fx, fb);
+  piece_batch<F, 2>(L, A, 0, rbx, rby, fx, fb);
+  piece_batch<F, 3>(L, A, 0, rbx, rby, fx, fb);
+  piece_batch<F, 3>(L, A, 1, rbx, rby, fx, fb);
   if (tid < 64) {
     const int s = tid >> 4, a = tid & 15, x = rbx + (a & 3), y = rby + (a >> 2);
-    if (x < A.xb && y < A.yb) A.pc[(size_t)s * A.xb * A.yb + (size_t)y * A.xb + x] = cost[s][a];
+    if ((s > 0 || need_leaf) && x < A.xb && y < A.yb)
+      A.pc[(size_t)s * A.xb * A.yb + (size_t)y * A.xb + x] = L.cost[s][a];
   }
 }
 
@@ -351,16 +372,22 @@ __global__ __launch_bounds__(256) void k_piece_costs(PieceArgs A) {
 
 #define RT_STATES 512
 
+#define RT_T 256  // threads per tile: two frontier states each
+
 struct RtLds {
-  uint8_t choices[64 * RT_STATES];  // per cell and state: the winning choice
-  float cost[2][RT_STATES];         // per state: path cost (INF: not reached), double-buffered
-  uint16_t rank[2][RT_STATES];      // per state: its rank in the layer
-  uint64_t bits[2][64];             // arrival keys of the layer being built, double-buffered
+  // per cell and thread: the winning choices of the thread's two states
+  // (state tid in bits 0..3, state tid + 256 in bits 4..7)
+  uint8_t choices[64][RT_T];
+  uint2 st[2][RT_STATES];  // per state: path cost bits (INF: not reached), rank in the layer
+  uint64_t bits[2][64];    // arrival keys of the layer being built, double-buffered
   uint32_t pfx[64];
   float pcost[4][64];  // per cell: leaf, wide, tall, square (INF: not a candidate)
-  float score[64];     // per cell: the score of the piece anchored there (current / new)
+  float score[64];     // per cell: the score of the piece anchored there
   uint8_t live[64];
   uint8_t plan[64];  // per cell: the saved plan's choice (0: none)
+  uint64_t fixed, blocked;
+  float fixed_cost;
+  int go;
 };
 
 struct RetileArgs {
@@ -394,80 +421,92 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
 
 // plan_small on a w x h tile around the `fixed` cells.  Returns the optimum;
 // when it is finite and below replace_below, L.plan receives the plan.
-__device__ float rt_plan(RtLds &L, uint64_t fixed, int w, int h, float replace_below) {
-  const int lane = threadIdx.x;
+// Thread t holds states t and t + 256; each state pulls its (at most five)
+// predecessors, all of a layer's reads issued before any is used.  A cell
+// that is filled before (fixed) only shifts the states, keeping their cost
+// and order.
+__device__ __forceinline__ float rt_plan(RtLds &L, uint64_t fixed, int w, int h, float replace_below) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int cells = w * h, states = 1 << (w + 1), mask = states - 1;
   const int below_bit = 1 << w;
   const int start = (int)(fixed & (uint64_t)mask);
-  for (int t = lane; t < RT_STATES; t += 64) {
-    L.cost[0][t] = t == start ? 0.0f : INFINITY;
-    L.rank[0][t] = 0;
-  }
-  L.bits[0][lane] = 0;
-  L.bits[1][lane] = 0;
+  const uint32_t inf_bits = 0x7f800000u;
+  for (int t = tid; t < RT_STATES; t += RT_T) L.st[0][t] = make_uint2(t == start ? 0u : inf_bits, 0u);
+  if (tid < 128) (&L.bits[0][0])[tid] = 0;
   __syncthreads();
-  const int jn = (states + 63) >> 6;
+  const int jn = (states + RT_T - 1) / RT_T;  // 1 or 2, uniform
   for (int i = 0; i < cells; i++) {
     const int cur = i & 1, nxt = cur ^ 1;
     // the fixed flag of cell i + w + 1, entering the window (0 past the tile)
     const int corner = (w + 1 + i < 64) ? (int)((fixed >> (w + 1 + i)) & 1ull) : 0;
-    const int shapes = L.live[i];
-    const float c_leaf = L.pcost[0][i], c_wide = L.pcost[1][i];
-    const float c_tall = L.pcost[2][i], c_square = L.pcost[3][i];
-    float best[8];
-    uint32_t first[8];
-    uint8_t choice[8];
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      best[j] = INFINITY;
-      first[j] = 0xffffffffu;
-      choice[j] = 0;
-      const int t = lane + 64 * j;
-      if (j >= jn || t >= states) continue;
-      const int tw = (t >> w) & 1;
-      const int low = (t & (below_bit - 1)) << 1;
-      // Candidate (predecessor, choice): the least cost wins, then the
-      // earliest arrival (predecessor rank, then choice order); the state's
-      // own arrival key is the earliest finite candidate.
-      uint32_t wkey = 0xffffffffu;
-      float wcost = INFINITY;
-      uint8_t wch = 0;
-      uint32_t fkey = 0xffffffffu;
-      auto take = [&](int from, float add, bool plain, int ch) {
-        const float c0 = L.cost[cur][from];
-        if (!(c0 < INFINITY)) return;
-        const float c = plain ? c0 : c0 + add;
-        if (!(c < INFINITY)) return;
-        const uint32_t key = (uint32_t)L.rank[cur][from] * 8u + (uint32_t)ch;
-        if (c < wcost || (c == wcost && key < wkey)) {
-          wcost = c;
-          wkey = key;
-          wch = (uint8_t)ch;
+    if ((fixed >> i) & 1ull) {
+#pragma clang loop unroll(full)
+      for (int j = 0; j < 2; j++) {
+        const int t = tid + RT_T * j;
+        if (j < jn && t < states) {
+          const int low = (t & (below_bit - 1)) << 1;
+          L.st[nxt][t] = ((t >> w) & 1) == corner ? L.st[cur][low | 1] : make_uint2(inf_bits, 0u);
         }
-        fkey = min(fkey, key);
-      };
-      if (tw == corner) {
-        take(low | 1, 0.0f, true, 0);  // the cell was filled before
-        take(low, c_leaf, false, 1);
-        if ((t & 1) && (shapes & 1)) take(low & ~2, c_wide, false, 2);
-        if (((t >> (w - 1)) & 1) && (shapes & 2)) take(low & ~below_bit, c_tall, false, 3);
       }
-      if (corner == 0 && tw == 1 && (t & 1) && ((t >> (w - 1)) & 1) && (shapes & 4))
-        take(low & ~2 & ~below_bit, c_square, false, 4);
+      L.choices[i][tid] = 0;
+      __syncthreads();
+      continue;
+    }
+    const int shapes = L.live[i];
+    const float add[5] = {0.0f, L.pcost[0][i], L.pcost[1][i], L.pcost[2][i], L.pcost[3][i]};
+    uint2 sv[2][5];
+    int valid[2];
+#pragma clang loop unroll(full)
+    for (int j = 0; j < 2; j++) {
+      const int t = tid + RT_T * j;
+      const int low = (t & (below_bit - 1)) << 1;
+      const int tw = (t >> w) & 1, b0 = t & 1, bw = (t >> (w - 1)) & 1;
+      const bool in = j < jn && t < states;
+      const bool same = in && tw == corner;
+      valid[j] = (same ? 3 : 0) | ((same && b0 && (shapes & 1)) ? 4 : 0) |
+                 ((same && bw && (shapes & 2)) ? 8 : 0) |
+                 ((in && corner == 0 && tw == 1 && b0 && bw && (shapes & 4)) ? 16 : 0);
+      if (j < jn) {
+        sv[j][0] = L.st[cur][low | 1];                // the cell was filled before
+        sv[j][1] = L.st[cur][low];                    // leaf
+        sv[j][2] = L.st[cur][low & ~2];               // wide
+        sv[j][3] = L.st[cur][low & ~below_bit];       // tall
+        sv[j][4] = L.st[cur][low & ~2 & ~below_bit];  // square
+      }
+    }
+    float best[2];
+    uint32_t first[2], choice[2];
+#pragma clang loop unroll(full)
+    for (int j = 0; j < 2; j++) {
+      // the least cost wins, then the earliest arrival (predecessor rank,
+      // then choice order); the state's own arrival key is its earliest
+      // finite candidate
+      float wcost = INFINITY;
+      uint32_t wkey = 0xffffffffu, wch = 0, fkey = 0xffffffffu;
+#pragma clang loop unroll(full)
+      for (int k = 0; k < 5; k++) {
+        const float c0 = __uint_as_float(sv[j][k].x);
+        const float c = k == 0 ? c0 : c0 + add[k];
+        const bool ok = ((valid[j] >> k) & 1) && c0 < INFINITY && c < INFINITY;
+        const uint32_t key = sv[j][k].y * 8u + (uint32_t)k;
+        const bool better = ok && (c < wcost || (c == wcost && key < wkey));
+        wcost = better ? c : wcost;
+        wkey = better ? key : wkey;
+        wch = better ? (uint32_t)k : wch;
+        fkey = ok ? min(fkey, key) : fkey;
+      }
       best[j] = wcost;
       first[j] = fkey;
       choice[j] = wch;
+      if (fkey != 0xffffffffu)
+        atomicOr((unsigned long long *)&L.bits[cur][fkey >> 6], 1ull << (fkey & 63));
     }
-    // arrival bitmap of this layer
-#pragma unroll
-    for (int j = 0; j < 8; j++)
-      if (first[j] != 0xffffffffu)
-        atomicOr((unsigned long long *)&L.bits[cur][first[j] >> 6], 1ull << (first[j] & 63));
     __syncthreads();
-    {
+    if (wave == 0) {
+      // ranks: the arrival keys in increasing order
       const uint32_t cnt = (uint32_t)__popcll(L.bits[cur][lane]);
       uint32_t inc = cnt;
-#pragma unroll
+#pragma clang loop unroll(full)
       for (int o = 1; o < 64; o <<= 1) {
         const uint32_t v = (uint32_t)__shfl_up((int)inc, o);
         if (lane >= o) inc += v;
@@ -475,35 +514,36 @@ __device__ float rt_plan(RtLds &L, uint64_t fixed, int w, int h, float replace_b
       L.pfx[lane] = inc - cnt;
     }
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int t = lane + 64 * j;
+    uint32_t packed = 0;
+#pragma clang loop unroll(full)
+    for (int j = 0; j < 2; j++) {
+      const int t = tid + RT_T * j;
       if (j >= jn || t >= states) continue;
-      float c = INFINITY;
-      uint16_t r = 0;
+      uint2 out = make_uint2(inf_bits, 0u);
       if (first[j] != 0xffffffffu) {
         const uint32_t f = first[j], wd = f >> 6;
-        r = (uint16_t)(L.pfx[wd] + (uint32_t)__popcll(L.bits[cur][wd] & ((1ull << (f & 63)) - 1ull)));
-        c = best[j];
-        L.choices[i * RT_STATES + t] = choice[j];
+        out.y = L.pfx[wd] + (uint32_t)__popcll(L.bits[cur][wd] & ((1ull << (f & 63)) - 1ull));
+        out.x = __float_as_uint(best[j]);
+        packed |= choice[j] << (4 * j);
       }
-      L.cost[nxt][t] = c;
-      L.rank[nxt][t] = r;
+      L.st[nxt][t] = out;
     }
+    L.choices[i][tid] = (uint8_t)packed;
     __syncthreads();
-    L.bits[cur][lane] = 0;
+    if (tid < 64) L.bits[cur][tid] = 0;
   }
-  const float total = L.cost[cells & 1][0];
+  const float total = __uint_as_float(L.st[cells & 1][0].x);
   __syncthreads();
   if (!(total < INFINITY && total < replace_below)) return total;
-  if (lane == 0) {
-    const int added[5] = {0, 0, 2, below_bit, 2 | below_bit | (2 << w)};
+  if (tid == 0) {
     for (int i = 0; i < 64; i++) L.plan[i] = 0;
     int state = 0;
     for (int i = cells - 1; i >= 0; i--) {
-      const int ch = L.choices[i * RT_STATES + state];
+      const int ch = (int)((L.choices[i][state & (RT_T - 1)] >> (4 * (state >> 8))) & 15u);
       if (ch != 0) L.plan[i] = (uint8_t)ch;
-      state = (((state << 1) ^ added[ch]) & mask) | (ch == 0 ? 1 : 0);
+      // the predecessor: undo the shift, remove the cells the choice filled
+      const int added = ch == 2 ? 2 : (ch == 3 ? below_bit : (ch == 4 ? (2 | below_bit) : 0));
+      state = (((state << 1) ^ added) & mask) | (ch == 0 ? 1 : 0);
     }
   }
   __syncthreads();
@@ -512,8 +552,8 @@ __device__ float rt_plan(RtLds &L, uint64_t fixed, int w, int h, float replace_b
 
 // search_tile_coefficient (layout_dp.rs:2033-2199) for tile slot (sx, sy)
 // of the pass: 32 slots per DC group and axis.
-__device__ void retile_body(RtLds &L, const RetileArgs A, int sx, int sy) {
-  const int lane = threadIdx.x;
+__device__ __forceinline__ void retile_body(RtLds &L, const RetileArgs A, int sx, int sy) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int gx = sx >> 5, gy = sy >> 5;
   const int o = A.crossing ? 4 : 0;
   const int lx = o + 8 * (sx & 31), ly = o + 8 * (sy & 31);
@@ -523,9 +563,10 @@ __device__ void retile_body(RtLds &L, const RetileArgs A, int sx, int sy) {
   if (cells < 2) return;
   const int X0 = gx * 256 + lx, Y0 = gy * 256 + ly;
   const size_t plane = (size_t)A.xb * A.yb;
-  // the tile's current pieces (first blocks; on a shifted grid, those that
-  // fit inside the tile)
-  const bool in = lane < cells;
+  // Wave 0, one lane per cell: the tile's current pieces (first blocks; on a
+  // shifted grid, those that fit inside the tile), the candidate costs and
+  // the live masks.
+  const bool in = wave == 0 && lane < cells;
   const int x = in ? lane % w : 0, y = in ? lane / w : 0;
   uint8_t cell = 0;
   if (in) cell = A.cells[(size_t)(Y0 + y) * A.xb + X0 + x];
@@ -533,20 +574,22 @@ __device__ void retile_body(RtLds &L, const RetileArgs A, int sx, int sy) {
   const bool first = in && (cell & 1);
   const bool fits = first && x + cov_x(s) <= w && y + cov_y(s) <= h;
   const bool curp = first && (!A.crossing || fits);
-  if (!A.crossing) {
-    // footprint_is_self_contained (src/ac_strategy/selection.rs:3235-3258)
-    if (__ballot(first && !fits) != 0) return;
-    uint32_t area = first ? (uint32_t)(cov_x(s) * cov_y(s)) : 0u;
+  const bool fixedp = curp && rt_slot(s) < 0;
+  float my_score = 0.0f;
+  if (wave == 0) {
+    bool go = true;
+    if (!A.crossing) {
+      // footprint_is_self_contained (src/ac_strategy/selection.rs:3235-3258)
+      uint32_t area = first ? (uint32_t)(cov_x(s) * cov_y(s)) : 0u;
 #pragma unroll
-    for (int k = 32; k > 0; k >>= 1) area += (uint32_t)__shfl_xor((int)area, k);
-    if ((int)area != cells) return;
-  }
-  if (__ballot(curp && (s == JXG_IDENTITY || s == JXG_DCT2X2)) != 0) return;
-  const uint64_t covered = wave_or64(curp ? rt_mask(x, y, s, w) : 0ull);
-  const uint64_t full = cells == 64 ? ~0ull : ((1ull << cells) - 1ull);
-  const uint64_t blocked = A.crossing ? (full & ~covered) : 0ull;
-  // candidate costs of the four shapes at each cell
-  {
+      for (int k = 32; k > 0; k >>= 1) area += (uint32_t)__shfl_xor((int)area, k);
+      go = __ballot(first && !fits) == 0 && (int)area == cells;
+    }
+    go = go && __ballot(curp && (s == JXG_IDENTITY || s == JXG_DCT2X2)) == 0;
+    const uint64_t covered = wave_or64(curp ? rt_mask(x, y, s, w) : 0ull);
+    const uint64_t full = cells == 64 ? ~0ull : ((1ull << cells) - 1ull);
+    const uint64_t blocked = A.crossing ? (full & ~covered) : 0ull;
+    // candidate costs of the four shapes at each cell
     const int gxl = lx + x, gyl = ly + y;  // group-relative
 #pragma unroll
     for (int ch = 1; ch <= 4; ch++) {
@@ -565,20 +608,34 @@ __device__ void retile_body(RtLds &L, const RetileArgs A, int sx, int sy) {
       }
       L.pcost[ch - 1][lane] = v;
     }
+    // the incumbent's scores
+    if (curp) {
+      const int sl = rt_slot(s);
+      const size_t at = (size_t)(Y0 + y) * A.xb + X0 + x;
+      my_score = sl >= 0 ? A.pc[(size_t)sl * plane + at] : A.c32[at];
+    }
+    const uint64_t fixed = wave_or64(fixedp ? rt_mask(x, y, s, w) : 0ull);
+    // fixed_cost: the incumbents outside the candidate shapes, in raster order
+    float fixed_cost = 0.0f;
+    {
+      uint64_t fm = __ballot(fixedp);
+      while (fm) {
+        const int i = __builtin_ctzll(fm);
+        fm &= fm - 1;
+        fixed_cost += __shfl(my_score, i);
+      }
+    }
+    if (lane == 0) {
+      L.go = go;
+      L.fixed = fixed;
+      L.blocked = blocked;
+      L.fixed_cost = fixed_cost;
+    }
   }
-  // the incumbent's scores
-  float my_score = 0.0f;
-  if (curp) {
-    const int sl = rt_slot(s);
-    const size_t at = (size_t)(Y0 + y) * A.xb + X0 + x;
-    my_score = sl >= 0 ? A.pc[(size_t)sl * plane + at] : A.c32[at];
-  }
-  const bool fixedp = curp && rt_slot(s) < 0;
-  const uint64_t fixed = wave_or64(fixedp ? rt_mask(x, y, s, w) : 0ull);
-  L.score[lane] = my_score;
   __syncthreads();
-  // SmallCosts::live_masks (layout_dp.rs:1860-1916)
-  {
+  if (!L.go) return;
+  if (wave == 0) {
+    // SmallCosts::live_masks (layout_dp.rs:1860-1916)
     uint8_t lv = 0;
     if (in) {
       const int i = lane;
@@ -599,17 +656,9 @@ __device__ void retile_body(RtLds &L, const RetileArgs A, int sx, int sy) {
     }
     L.live[lane] = lv;
   }
-  // fixed_cost: the incumbents outside the candidate shapes, in raster order
-  float fixed_cost = 0.0f;
-  {
-    uint64_t fm = __ballot(fixedp);
-    while (fm) {
-      const int i = __builtin_ctzll(fm);
-      fm &= fm - 1;
-      fixed_cost += __shfl(my_score, i);
-    }
-  }
-  __syncthreads();
+  const uint64_t fixed = L.fixed, blocked = L.blocked;
+  const float fixed_cost = L.fixed_cost;
+  // (rt_plan's first barrier publishes the live masks)
   const float around = rt_plan(L, fixed | blocked, w, h, INFINITY) + fixed_cost;
   bool keep_fixed = true;
   float winner = around;
@@ -620,7 +669,7 @@ __device__ void retile_body(RtLds &L, const RetileArgs A, int sx, int sy) {
       keep_fixed = false;
     }
   }
-  if (!(winner < INFINITY)) return;  // winner is never NaN: sums of costs
+  if (!(winner < INFINITY) || wave != 0) return;  // winner is never NaN: sums of costs
   // the new layout at each cell, and the old / new sums over the pieces
   // that differ, both in raster order (the sorted layout's order)
   const int pch = in ? L.plan[lane] : 0;
@@ -655,7 +704,7 @@ __device__ void retile_body(RtLds &L, const RetileArgs A, int sx, int sy) {
   }
 }
 
-__global__ __launch_bounds__(64) void k_retile(RetileArgs A) {
+__global__ __launch_bounds__(RT_T) void k_retile(RetileArgs A) {
   __shared__ RtLds L;
   retile_body(L, A, blockIdx.x, blockIdx.y);
 }
@@ -718,7 +767,7 @@ __global__ __launch_bounds__(256) void k_piece_costs_b(const BatchImg *__restric
   piece_costs_body<F>(A, (int)(bl % (uint32_t)gw), (int)(bl / (uint32_t)gw));
 }
 
-__global__ __launch_bounds__(64) void k_retile_b(const BatchImg *__restrict__ img,
+__global__ __launch_bounds__(RT_T) void k_retile_b(const BatchImg *__restrict__ img,
                                                  const uint32_t *__restrict__ pfx, int n,
                                                  int crossing) {
   __shared__ RtLds L;
@@ -786,11 +835,11 @@ JXG_HIDDEN int retile_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uin
   R.unaligned = mode >= 2;
   const dim3 g2(32 * ((xb + 255) / 256), 32 * ((yb + 255) / 256));
   R.crossing = 0;
-  k_retile<<<g2, 64, 0, st>>>(R);
+  k_retile<<<g2, RT_T, 0, st>>>(R);
   LAUNCH_CHECK("k_retile");
   if (mode >= 2) {
     R.crossing = 1;
-    k_retile<<<g2, 64, 0, st>>>(R);
+    k_retile<<<g2, RT_T, 0, st>>>(R);
     LAUNCH_CHECK("k_retile (shifted)");
   }
   k_adjust_qf<<<(xb * yb + 255) / 256, 256, 0, st>>>(cells, qf, xb, yb, p->distance);
@@ -809,8 +858,8 @@ JXG_HIDDEN int batch_retile(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
     k_piece_costs_b<true><<<total[BS_PIECE], 256, 0, st>>>(d_img, p + BS_PIECE * m, n, ctx->d_scanpos);
   else
     k_piece_costs_b<false><<<total[BS_PIECE], 256, 0, st>>>(d_img, p + BS_PIECE * m, n, ctx->d_scanpos);
-  k_retile_b<<<total[BS_RETILE], 64, 0, st>>>(d_img, p + BS_RETILE * m, n, 0);
-  k_retile_b<<<total[BS_RETILE], 64, 0, st>>>(d_img, p + BS_RETILE * m, n, 1);
+  k_retile_b<<<total[BS_RETILE], RT_T, 0, st>>>(d_img, p + BS_RETILE * m, n, 0);
+  k_retile_b<<<total[BS_RETILE], RT_T, 0, st>>>(d_img, p + BS_RETILE * m, n, 1);
   k_adjust_qf_b<<<total[BS_ADJUST], 256, 0, st>>>(d_img, p + BS_ADJUST * m, n);
   LAUNCH_CHECK("batch retile");
   return JXG_OK;

@@ -262,7 +262,8 @@ enum {
   JXG_PROF_QUANTIZE_XT = 8,/* k_quantize_xt                            */
   JXG_PROF_FILL = 9,       /* k_fill_bands alone (inside JXG_PROF_ALPHA) */
   JXG_PROF_RETILE = 10,    /* piece costs, re-tiling passes, field adjustment */
-  JXG_PROF_N = 11
+  JXG_PROF_REFINE = 11,    /* quant-field refinement: candidate costs, QF trellis */
+  JXG_PROF_N = 12
 };
 #define JXG_PROF_CAP 64
 int jxg_ctx_profile(jxg_ctx *ctx, int enable);

@@ -28,7 +28,7 @@ DEVICE = ["--offload-arch=gfx950", "-fno-gpu-flush-denormals-to-zero",
 
 HIP_SOURCES = ["jxg_kernels.hip", "jxg_filters.hip", "jxg_cfl.hip", "jxg_large.hip",
                "jxg_epf.hip", "jxg_alpha.hip", "jxg_chroma.hip",
-               "jxg_batch.hip", "jxg_tune.hip", "jxg_retile.hip"]
+               "jxg_batch.hip", "jxg_tune.hip", "jxg_retile.hip", "jxg_refine.hip"]
 
 
 def run(cmd):

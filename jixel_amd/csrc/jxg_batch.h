@@ -25,6 +25,10 @@ enum {
   BS_PIECE,
   BS_RETILE,
   BS_ADJUST,
+  BS_RLIST,
+  BS_RCOST_S,
+  BS_RCOST_B,
+  BS_TRELLIS,
   BS_N
 };
 
@@ -38,6 +42,7 @@ struct BatchImg {
   float *part;  // CfL block sums, tiles * 64 * 4
   float *pc;    // re-tiling piece costs, 4 * xb * yb
   float *c32;   // raw DCT32X32 costs, xb * yb
+  uint8_t *rf;  // quant-field refinement scratch (refine_scratch_bytes)
   uint8_t *qf, *cells;
   int8_t *ytox, *ytob;
   int32_t *lv[3];

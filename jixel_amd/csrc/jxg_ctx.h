@@ -95,6 +95,8 @@ struct jxg_ctx {
   size_t pc_cap = 0;
   float *d_c32 = nullptr;  // raw DCT32X32 cost per 32x32 region's first block
   size_t c32_cap = 0;
+  uint8_t *d_rf = nullptr;  // quant-field refinement: candidate costs / values, transform lists
+  size_t rf_cap = 0;
   // jxg_ctx_profile: event pairs per stage, recorded while prof_on
   bool prof_on = false;
   hipEvent_t *prof_ev = nullptr;  // JXG_PROF_N * 2 * JXG_PROF_CAP
@@ -169,8 +171,17 @@ JXG_HIDDEN int retile_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uin
                              uint32_t ph, uint32_t W, uint32_t H, uint8_t *qf, const int8_t *ytox,
                              const int8_t *ytob, const float *tables, const jxg_frame_params *p,
                              int mode, uint8_t *cells, const float *c32, float *pc);
+// jxg_refine.hip: refine_quant_field + qf_trellis after the adjustment.
+JXG_HIDDEN int refine_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uint32_t stride,
+                             uint32_t ph, uint32_t W, uint32_t H, uint8_t *qf, const int8_t *ytox,
+                             const int8_t *ytob, const float *tables, const jxg_frame_params *p,
+                             const uint8_t *cells);
 JXG_HIDDEN int batch_retile(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
                             const uint32_t *d_pfx, int n, const uint32_t total[], int fused);
+JXG_HIDDEN int batch_refine(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                            const uint32_t *d_pfx, int n, const uint32_t total[], int fused);
+JXG_HIDDEN void refine_grids(uint32_t W, uint32_t H, uint32_t *g_list, uint32_t *g0, uint32_t *g1,
+                             uint32_t *g_trellis);
 JXG_HIDDEN int zero_cmaps(jxg_ctx *ctx, uint32_t w, uint32_t h, const int8_t **x,
                           const int8_t **b);
 

@@ -296,6 +296,58 @@ __device__ __forceinline__ void inv_dct8(float *b) {
   }
 }
 
+// inv_dct1d_16 / inv_dct1d_32 (src/dct.rs:1547-1627).
+__device__ __forceinline__ void inv_dct16(float *b) {
+  const float IS2 = 0.70710678118654752440f;
+  const float iwc[8] = {1.0f / 0.5024193f, 1.0f / 0.5224986f, 1.0f / 0.566944f, 1.0f / 0.6468218f,
+                        1.0f / 0.7881546f, 1.0f / 1.0606777f, 1.0f / 1.7224471f, 1.0f / 5.1011486f};
+  float t[16];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    t[i] = b[2 * i];
+    t[8 + i] = b[2 * i + 1];
+  }
+#pragma unroll
+  for (int i = 14; i >= 9; i--) t[i] -= t[i + 1];
+  t[8] = (t[8] - t[9]) * IS2;
+  inv_dct8(t + 8);
+#pragma unroll
+  for (int i = 0; i < 8; i++) t[8 + i] *= iwc[i];
+  inv_dct8(t);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    b[i] = (t[i] + t[8 + i]) * 0.5f;
+    b[15 - i] = (t[i] - t[8 + i]) * 0.5f;
+  }
+}
+__device__ __forceinline__ void inv_dct32(float *b) {
+  const float IS2 = 0.70710678118654752440f;
+  const float iwc[16] = {1.0f / 0.50060298f, 1.0f / 0.50547096f, 1.0f / 0.5154473f,
+                         1.0f / 0.5310426f,  1.0f / 0.5531039f,  1.0f / 0.582935f,
+                         1.0f / 0.6225041f,  1.0f / 0.67480834f, 1.0f / 0.7445363f,
+                         1.0f / 0.83934965f, 1.0f / 0.97256825f, 1.0f / 1.16944f,
+                         1.0f / 1.4841646f,  1.0f / 2.057781f,   1.0f / 3.4076084f,
+                         1.0f / 10.190008f};
+  float t[32];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    t[i] = b[2 * i];
+    t[16 + i] = b[2 * i + 1];
+  }
+#pragma unroll
+  for (int i = 30; i >= 17; i--) t[i] -= t[i + 1];
+  t[16] = (t[16] - t[17]) * IS2;
+  inv_dct16(t + 16);
+#pragma unroll
+  for (int i = 0; i < 16; i++) t[16 + i] *= iwc[i];
+  inv_dct16(t);
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    b[i] = (t[i] + t[16 + i]) * 0.5f;
+    b[31 - i] = (t[i] - t[16 + i]) * 0.5f;
+  }
+}
+
 // RESAMPLE_SCALE_64_TO_8 (src/dct.rs:2005-2014)
 static __constant__ float RS64_8_DEV[8] = {
     1.0f, 0.9936866f, 0.97488683f, 0.94401807f, 0.9017642f, 0.8490575f, 0.7870549f, 0.71710813f};

@@ -34,6 +34,8 @@ typedef struct jxg_derived {
   int32_t fused;
   int32_t retile;  // coefficient re-tiling: 0 off, 1 aligned grid (Fastest), 2 + unaligned
                    // pieces and the (4, 4)-shifted grid (Fast); src/effort.rs:311-349
+  int32_t refine_steps;  // refine_quant_field: 0 off, 1 or 2 coarser steps (Fast, d >= 1.5)
+  float recon_lambda;    // reconstruction_lambda(distance) (src/ac_strategy.rs:1667)
   uint32_t qt_off[JXG_QT_NUM_KINDS];  // quant-matrix blob layout
   uint32_t qt_n[JXG_QT_NUM_KINDS];
   float exp2_table[64];          // fast_exp2's 2^((i-32)/64)
@@ -60,6 +62,11 @@ float jxg_f16_to_f32(uint16_t b);
 #else
 #define JXG_HD
 #endif
+
+// Scratch of the quant-field refinement for nb blocks: per block 4 candidate
+// costs (f32), 4 values and a count (u8); two transform lists (u32) and two
+// counters; the trellis chain (block index u32, value, field, back u8).
+static inline JXG_HD size_t jxg_refine_scratch_bytes(size_t nb) { return 32 * nb + 4 * nb + 128; }
 
 static inline JXG_HD int jxg_area_class(int cx, int cy) {
   int a = cx * cy, k = 0;

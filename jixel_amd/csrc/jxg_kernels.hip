@@ -2195,6 +2195,7 @@ JXG_HIDDEN int batch_stages_post(jxg_ctx *ctx, hipStream_t st, const BatchImg *d
   LAUNCH_CHECK("batch select");
   int rc = batch_retile(ctx, st, d_img, d_pfx, n, total, fused);
   if (rc) return rc;
+  if ((rc = batch_refine(ctx, st, d_img, d_pfx, n, total, fused))) return rc;
   if (fused) {
     k_quantize_b<true><<<total[BS_QUANT], 256, 0, st>>>(d_img, p + BS_QUANT * m, n, Q);
     k_quantize_xt_b<true><<<xt_grid, 256, 0, st>>>(d_img, Q);
@@ -2231,6 +2232,8 @@ JXG_HIDDEN void batch_grids(uint32_t W, uint32_t H, uint32_t stride, uint32_t ph
   gh[BS_RETILE] = 32 * ((yb + 255) / 256);
   gw[BS_ADJUST] = (xb * yb + 255) / 256;
   gh[BS_ADJUST] = 1;
+  refine_grids(W, H, &gw[BS_RLIST], &gw[BS_RCOST_S], &gw[BS_RCOST_B], &gw[BS_TRELLIS]);
+  gh[BS_RLIST] = gh[BS_RCOST_S] = gh[BS_RCOST_B] = gh[BS_TRELLIS] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -2661,7 +2664,7 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
       hipMalloc((void **)&ctx->d_selq, 3 * SELQ_N * sizeof(float)) != hipSuccess ||
       hipMalloc((void **)&ctx->d_mlut, (9 + 256) * sizeof(float)) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_mlut, (9 + 256) * sizeof(float), hipHostMallocDefault) != hipSuccess ||
-      hipMalloc((void **)&ctx->d_misc, 8 * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_misc, 16 * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc((void **)&ctx->d_exp2, 64 * sizeof(float)) != hipSuccess)
     return fail(set_err(JXG_ERR_ALLOC, "context allocation failed"));
   if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
@@ -2773,6 +2776,7 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_fxchg);
   (void)hipFree(ctx->d_afac);
   (void)hipFree(ctx->d_pc);
+  (void)hipFree(ctx->d_rf);
   (void)hipFree(ctx->d_c32);
   (void)hipFree(ctx->d_zero_cmap);
   (void)hipFree(ctx->d_rgb);
@@ -2963,9 +2967,14 @@ static int select_impl(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32
     k_select<false><<<grid, SEL_T, 0, ctx->stream>>>(d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables, ctx->d_scans, selq, ctx->d_derived, cells, c8, c16, c32, c32raw, c8raw, xb, yb);
   LAUNCH_CHECK("k_select");
   prof_end(ctx, JXG_PROF_SELECT, ctx->stream);
-  if (retile)
-    return retile_stages(ctx, ctx->stream, d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob, tables,
-                         p, retile, cells, ctx->d_c32, ctx->d_pc);
+  if (retile) {
+    if ((rc = retile_stages(ctx, ctx->stream, d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob,
+                            tables, p, retile, cells, ctx->d_c32, ctx->d_pc)))
+      return rc;
+    if (ctx->h_derived->refine_steps)
+      return refine_stages(ctx, ctx->stream, d_xyb, stride, padded_h, W, H, d_qf, ytox, ytob,
+                           tables, p, cells);
+  }
   return JXG_OK;
 }
 

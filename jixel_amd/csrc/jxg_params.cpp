@@ -375,6 +375,20 @@ int jxg_derive(const jxg_frame_params *p, jxg_derived *o) {
   // layout_dp_coefficient at Fastest and Fast; layout_dp_unaligned and
   // layout_crossing at Fast (src/effort.rs:311-349)
   o->retile = !o->run_selection ? 0 : (p->speed == JXG_SPEED_FAST ? 2 : (p->speed == JXG_SPEED_FASTEST ? 1 : 0));
+  // quant_refine at Fast, quant_refinement_steps (src/ac_strategy/selection.rs:534-542)
+  o->refine_steps = (o->run_selection && p->speed == JXG_SPEED_FAST) ? (d < 1.5f ? 0 : (d >= 2.0f ? 2 : 1)) : 0;
+  {
+    // reconstruction_lambda (src/ac_strategy.rs:1667-1686)
+    const float recip = 1.0f / (4.0f - 1.0f);
+    const float ramp = clampf((d - 1.0f) * recip, 0.0f, 1.0f);
+    float mult = fmla(f, ramp, 10.0f - 0.25f, 0.25f);
+    if (d > 1.0f && d < 2.0f) {
+      const float knee = (2.0f - 1.0f) * recip;
+      const float t = (d - 1.0f) / (2.0f - 1.0f);
+      mult = fmla(f, powf(t, 1.33f), knee * (10.0f - 0.25f), 0.25f);
+    }
+    o->recon_lambda = 0.08086717f * mult;
+  }
   o->fused = f ? 1 : 0;
   for (int k = 0; k < JXG_QT_NUM_KINDS; k++) {
     o->qt_off[k] = jxg_qt_offset(k);

@@ -1,0 +1,932 @@
+// jxg_refine.hip - refine_quant_field with qf_trellis, the last stage of the
+// Fast tier's strategy decision (src/ac_strategy/selection.rs:534-896),
+// run on the field after adjust_quant_field.
+//
+//   k_refine_list   per block: clears the block's refinement record and puts
+//                   every eligible transform (a first block, not the DCT64
+//                   family, field value above 1) on the list of its size
+//                   class (up to 16x16, or 32x32).
+//   k_refine_costs  one workgroup per listed transform (64 threads up to
+//                   16x16, 256 for 32x32): the transform's three channels
+//                   with the tile's CfL, then for the current field value
+//                   and each coarser candidate the reconstruction cost of
+//                   reconstruction_dist_and_rate (src/ac_strategy.rs:1538;
+//                   src/inflated_cost.rs:1136-1500) in the reference's scalar
+//                   order: the biased quantizer's coefficient error of Y, of
+//                   X and B against the luma error (combine_error), the
+//                   coefficient rate model, the scalar inverse transforms
+//                   and the 8x8-window SSIM deficit per channel, then
+//                   rd_cost with the reconstruction lambda.
+//   k_qf_trellis    one workgroup per DC group: the raster chain of its first
+//                   blocks, two rounds of QF residual prices (a histogram of
+//                   the chain) and the Viterbi search over the candidates,
+//                   then the chosen values written over each transform.
+//
+// The Viterbi recursion is a sequential f32 computation whose every
+// rounding decides ties downstream, so one lane runs it, reading states
+// that the other waves stage into LDS a chunk ahead.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "jxg_batch.h"
+#include "jxg_ctx.h"
+#include "jxg_device.h"
+#include "jxg_internal.h"
+
+using namespace jxg;
+
+namespace {
+
+struct RefineArgs {
+  const float *xyb;
+  int stride, ph, W, H, xb, yb;
+  const uint8_t *cells;
+  const uint8_t *qf;  // the adjusted field
+  const int8_t *ytox, *ytob;
+  const float *tables;
+  const jxg_derived *D;
+  const uint16_t *scanpos;  // storage index -> scan position per shape
+  uint32_t *list[2];        // transforms by size class (block indices)
+  uint32_t *count;          // [0], [1]: list lengths
+  float *rcost;             // 4 per block
+  uint8_t *rq;              // 4 per block
+  uint8_t *rlen;            // 1 per block: candidate count (0: not refined)
+};
+
+__device__ __forceinline__ int rf_scanpos_offset(int cx, int cy) {
+  const int a = cx * cy;
+  return a == 1 ? 0 : (a == 2 ? 64 : (a == 4 ? (cx == 2 ? 192 : 448) : (a == 16 ? 1216 : 0)));
+}
+
+__device__ __forceinline__ void refine_list_body(const RefineArgs &A, int i) {
+  if (i >= A.xb * A.yb) return;
+  const uint8_t cell = A.cells[i];
+  if (!(cell & 1)) return;
+  A.rlen[i] = 0;
+  const int s = cell >> 1;
+  const bool small = s == JXG_DCT || s == JXG_DCT16X8 || s == JXG_DCT8X16 || s == JXG_DCT16X16;
+  if (!(small || s == JXG_DCT32X32) || A.qf[i] <= 1) return;
+  const int cls = small ? 0 : 1;
+  const uint32_t slot = atomicAdd(&A.count[cls], 1u);
+  A.list[cls][slot] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_refine_list(RefineArgs A) {
+  refine_list_body(A, blockIdx.x * 256 + threadIdx.x);
+}
+
+// Buffers of one transform (NMAX coefficients, in-place transform pitch W + 1).
+template <int NMAX, int NPITCH>
+struct RfLds {
+  float pix[3][NMAX];   // source pixels, row-major (pitch W)
+  float co[3][NPITCH];  // coefficients after CfL, in-place layout (pitch W + 1)
+  float err[3][NMAX];   // quantizer errors, storage order
+  float sp[NPITCH];     // spatial error of one channel (pitch W + 1)
+  float mg[NMAX];       // rate terms, storage order
+  uint64_t nzm[NMAX / 64];
+  float win[16];        // per window: SSIM deficit
+  float quot[3][64];
+  float rate[256];
+  float chan_rate[3], chan_dist[3];
+};
+
+template <bool F, bool BIG>
+__device__ __forceinline__ void refine_costs_body(RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> &L,
+                                                  const RefineArgs &A, uint32_t item) {
+  constexpr int NMAX = BIG ? 1024 : 256, T = BIG ? 256 : 64;
+  (void)NMAX;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const jxg_derived *D = A.D;
+  const uint32_t blk = A.list[BIG ? 1 : 0][item];
+  const int bx = (int)(blk % (uint32_t)A.xb), by = (int)(blk / (uint32_t)A.xb);
+  const int s = A.cells[blk] >> 1;
+  const int q0 = A.qf[blk];
+  const int cvx = cov_x(s), cvy = cov_y(s);
+  const int W = cvx * 8, H = cvy * 8, P = W + 1, N = W * H;
+  const int cx = max(cvx, cvy), cy = min(cvx, cvy), SW = cx * 8, SH = cy * 8;
+  const bool rows_first = s == JXG_DCT || s == JXG_DCT8X16;
+  const bool trans = s != JXG_DCT8X16;  // storage k = c * H + r (else r * W + c)
+  const int kind = qt_kind(s);
+  const uint32_t qoff = D->qt_off[kind], qn = D->qt_n[kind];
+  const int cls = jxg_area_class(cx, cy);
+  const int spoff = rf_scanpos_offset(cx, cy);
+  const int px = bx * 8, py = by * 8;
+  const size_t plane = (size_t)A.stride * A.ph;
+  {
+    const float num = D->dequant_numerator;
+    for (int e = tid; e < 192; e += T) {
+      const int c = e >> 6, a = e & 63;
+      L.quot[c][a] = a >= 2 ? num / (float)a : (a == 1 ? 1.0f - D->dequant_one[c] : 0.0f);
+    }
+    for (int e = tid; e < 256; e += T) L.rate[e] = D->rate_lut[e];
+  }
+  // cmap_factors (src/ac_strategy.rs:1838-1855): the first block's 64x64
+  // tile, clamped to its DC group's tile grid
+  float fx, fb;
+  {
+    const int gx = bx >> 8, gy = by >> 8;
+    const int xsize = min(256, A.xb - gx * 256), ysize = min(256, A.yb - gy * 256);
+    const int lbx = bx - gx * 256, lby = by - gy * 256;
+    const int tiles_x = (A.W + 63) / 64;
+    const int tx = min(lbx / 8, (xsize + 7) / 8 - 1), ty = min(lby / 8, (ysize + 7) / 8 - 1);
+    const size_t ti = (size_t)(gy * 32 + ty) * tiles_x + gx * 32 + tx;
+    fx = fmla<F>((float)A.ytox[ti], D->cfl_inv_color_factor, D->cfl_base_x);
+    fb = fmla<F>((float)A.ytob[ti], D->cfl_inv_color_factor, D->cfl_base_b);
+  }
+  // source pixels (the padded planes replicate the image edge, as
+  // prepare_reconstruction and the forward gather do)
+  for (int e = tid; e < 3 * N; e += T) {
+    const int c = e / N, rem = e - c * N, r = rem / W, col = rem - r * W;
+    const float v = A.xyb[c * plane + (size_t)(py + r) * A.stride + px + col];
+    L.pix[c][rem] = v;
+    L.co[c][r * P + col] = v;
+  }
+  __syncthreads();
+  // forward transform of each channel in place (the reference's pass order)
+  {
+    const float scale = 1.0f / (float)N;
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+      const bool rows = rows_first ? (pass == 0) : (pass == 1);
+      const int nl = rows ? 3 * H : 3 * W;
+      for (int e = tid; e < nl; e += T) {
+        const int c = rows ? e / H : e / W, l = rows ? e - c * H : e - c * W;
+        float *base = rows ? &L.co[c][l * P] : &L.co[c][l];
+        const int step = rows ? 1 : P, len = rows ? W : H;
+        if (len == 8)
+          dct_line<F, 8>(base, step, scale, pass == 1);
+        else if (len == 16)
+          dct_line<F, 16>(base, step, scale, pass == 1);
+        else
+          dct_line<F, 32>(base, step, scale, pass == 1);
+      }
+      __syncthreads();
+    }
+  }
+  auto coef_off = [&](int k) {
+    const int r = trans ? k % H : k / W, c = trans ? k / H : k % W;
+    return r * P + c;
+  };
+  // CfL with the tile's factors (apply_cfl, src/ac_strategy.rs:1104)
+  for (int k = tid; k < N; k += T) {
+    const int o = coef_off(k);
+    const float y = L.co[1][o];
+    L.co[0][o] = fmla<F>(-fx, y, L.co[0][o]);
+    L.co[2][o] = fmla<F>(-fb, y, L.co[2][o]);
+  }
+  __syncthreads();
+  // the candidates: the current value, then one or two coarser ones
+  // (find_quant_refinements, src/ac_strategy/selection.rs:556-690)
+  int qs_list[4] = {q0, 0, 0, 0};
+  int len = 1;
+  {
+    const int steps = D->refine_steps;
+    const int cand[3] = {q0 - 1, steps == 2 ? q0 - 2 : q0, q0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int c = cand[k] < 0 ? 0 : cand[k];
+      bool dup = c == 0;
+      for (int m = 0; m < len; m++) dup = dup || qs_list[m] == c;
+      if (!dup) qs_list[len++] = c;
+    }
+  }
+  const bool full = px + W <= A.W && py + H <= A.H;
+  const int vw = min(W, A.W - px), vh = min(H, A.H - py);
+  const int nwx = full ? W / 8 : (vw + 7) / 8, nwy = full ? H / 8 : (vh + 7) / 8;
+  float costs[4];
+  for (int j = 0; j < len; j++) {
+    const float qac = D->scale * (float)qs_list[j];
+    // quantize Y, then X and B against the luma error
+    for (int ci = 0; ci < 3; ci++) {
+      const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+      const float *inv = A.tables + qoff + (3 + c) * qn;
+      const float *mat = A.tables + qoff + c * qn;
+      const float qs = qac * D->qm_mult[c];
+      const float inv_scale = 1.0f / qs;
+      const float one = D->dequant_one[c], num = D->dequant_numerator;
+      const float t0 = D->thr[c][cls][0], t1 = D->thr[c][cls][1];
+      const float t2 = D->thr[c][cls][2], t3 = D->thr[c][cls][3];
+      const float factor = c == 0 ? fx : fb;
+      for (int k0 = 0; k0 < N; k0 += T) {
+        const int k = k0 + tid;
+        const int srow = k / SW, scol = k - srow * SW;
+        float e = 0.0f, rt = 0.0f;
+        bool nz = false;
+        if (!(scol < cx && srow < cy)) {
+          const int o = coef_off(k);
+          const float coef = c == 1 ? L.co[1][o] : fmla<F>(factor, L.err[1][k], L.co[c][o]);
+          const int yfix = srow >= SH / 2 ? 2 : 0;
+          const float thr = scol >= SW / 2 ? (yfix ? t3 : t1) : (yfix ? t2 : t0);
+          const float scaled = (inv[k] * qs) * coef;
+          const float quantized = fabsf(scaled) >= thr ? round_away(scaled) : 0.0f;
+          const float aq = fabsf(quantized);
+          const float lev =
+              aq < 64.0f ? quantized - copysignf(L.quot[c][(int)aq], quantized) : quantized - num / quantized;
+          e = (scaled - lev) * (mat[k] * inv_scale);
+          if (quantized != 0.0f) {
+            nz = true;
+            if (aq < 256.0f)
+              rt = L.rate[(int)aq];
+            else {
+              const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
+              rt = ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
+            }
+          }
+        }
+        L.err[c][k] = e;
+        L.mg[k] = rt;
+        const uint64_t m = __ballot(nz);
+        if (lane == 0) L.nzm[k >> 6] = m;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        // the coefficient rate model over the nonzero levels, storage order
+        float mag = 0.0f;
+        uint32_t nzc = 0, mx = 0;
+        for (int ch = 0; ch < N / 64; ch++) {
+          uint64_t m = L.nzm[ch];
+          while (m) {
+            const int i = __builtin_ctzll(m);
+            m &= m - 1;
+            const int k = ch * 64 + i;
+            mag += L.mg[k];
+            mx = max(mx, (uint32_t)A.scanpos[spoff + k] + 1u);
+            nzc++;
+          }
+        }
+        const float nzf = (float)nzc;
+        const float header =
+            0.4f * (nzc < 256u ? L.rate[nzc] : (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf)));
+        float visited = 0.0f;
+        if (nzc != 0) {
+          const int64_t span = (int64_t)mx - (int64_t)(cx * cy + nzc);
+          visited = span > 0 ? (float)span : 0.0f;
+        }
+        L.chan_rate[c] = nzf * 1.6f + 1.0f * mag + header + 0.5f * visited;
+      }
+      __syncthreads();
+    }
+    // inverse transform and SSIM deficit of each channel
+    for (int c = 0; c < 3; c++) {
+      // reconstruct_error with the scalar inverse transforms (src/dct.rs:
+      // 1594-1680): squares take the storage transposed into place, scaled
+      // by N, columns then rows; DCT8X16 columns (8) then rows (16); DCT16X8
+      // 8-point rows of the transposed storage, then 16-point columns.
+      const float nsc = (float)N;
+      for (int k = tid; k < N; k += T) {
+        float v = L.err[c][k] * nsc;
+        int r, col;
+        if (s == JXG_DCT8X16) {
+          r = k / 16;
+          col = k % 16;
+        } else if (s == JXG_DCT16X8) {
+          // storage u * 16 + v -> (row v, column u)
+          r = k % 16;
+          col = k / 16;
+        } else {
+          r = k % W;  // out[b * n + a] = coef[a * n + b]: a = k / n, b = k % n
+          col = k / W;
+        }
+        L.sp[r * P + col] = v;
+      }
+      __syncthreads();
+      if (s == JXG_DCT16X8) {
+        // rows (8 points) first, then columns (16 points)
+        for (int e = tid; e < H; e += T) {
+          float b[8];
+#pragma unroll
+          for (int i = 0; i < 8; i++) b[i] = L.sp[e * P + i];
+          inv_dct8(b);
+#pragma unroll
+          for (int i = 0; i < 8; i++) L.sp[e * P + i] = b[i];
+        }
+        __syncthreads();
+        for (int e = tid; e < W; e += T) {
+          float b[16];
+#pragma unroll
+          for (int i = 0; i < 16; i++) b[i] = L.sp[i * P + e];
+          inv_dct16(b);
+#pragma unroll
+          for (int i = 0; i < 16; i++) L.sp[i * P + e] = b[i];
+        }
+      } else {
+        // columns first, then rows
+        for (int e = tid; e < W; e += T) {
+          if (H == 8) {
+            float b[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) b[i] = L.sp[i * P + e];
+            inv_dct8(b);
+#pragma unroll
+            for (int i = 0; i < 8; i++) L.sp[i * P + e] = b[i];
+          } else if (H == 16) {
+            float b[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) b[i] = L.sp[i * P + e];
+            inv_dct16(b);
+#pragma unroll
+            for (int i = 0; i < 16; i++) L.sp[i * P + e] = b[i];
+          } else {
+            float b[32];
+#pragma unroll
+            for (int i = 0; i < 32; i++) b[i] = L.sp[i * P + e];
+            inv_dct32(b);
+#pragma unroll
+            for (int i = 0; i < 32; i++) L.sp[i * P + e] = b[i];
+          }
+        }
+        __syncthreads();
+        for (int e = tid; e < H; e += T) {
+          if (W == 8) {
+            float b[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) b[i] = L.sp[e * P + i];
+            inv_dct8(b);
+#pragma unroll
+            for (int i = 0; i < 8; i++) L.sp[e * P + i] = b[i];
+          } else if (W == 16) {
+            float b[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) b[i] = L.sp[e * P + i];
+            inv_dct16(b);
+#pragma unroll
+            for (int i = 0; i < 16; i++) L.sp[e * P + i] = b[i];
+          } else {
+            float b[32];
+#pragma unroll
+            for (int i = 0; i < 32; i++) b[i] = L.sp[e * P + i];
+            inv_dct32(b);
+#pragma unroll
+            for (int i = 0; i < 32; i++) L.sp[e * P + i] = b[i];
+          }
+        }
+      }
+      __syncthreads();
+      // SSIM deficit per 8x8 window (ssim_deficit_scalar_kernel, or the
+      // visible variant for footprints past the image edge)
+      if (tid < nwx * nwy) {
+        const int wx = tid % nwx, wy = tid / nwx, x0 = wx * 8, y0 = wy * 8;
+        const int ww = full ? 8 : min(8, vw - x0), wh = full ? 8 : min(8, vh - y0);
+        const float *o = L.pix[c];
+        const float bo = o[y0 * W + x0], br = bo - L.sp[y0 * P + x0];
+        float so = 0.0f, sr = 0.0f;
+        for (int y = y0; y < y0 + wh; y++)
+          for (int x = x0; x < x0 + ww; x++) {
+            const float ov = o[y * W + x], rv = ov - L.sp[y * P + x];
+            so += ov - bo;
+            sr += rv - br;
+          }
+        const float count = (float)(ww * wh);
+        const float inv_count = full ? (1.0f / 64.0f) : 1.0f / count;
+        const float mo = bo + so * inv_count, mr = br + sr * inv_count;
+        float vo = 0.0f, vr = 0.0f, cv = 0.0f;
+        for (int y = y0; y < y0 + wh; y++)
+          for (int x = x0; x < x0 + ww; x++) {
+            const float ov = o[y * W + x], rv = ov - L.sp[y * P + x];
+            const float co_ = ov - mo, cr = rv - mr;
+            vo = fmla<F>(co_, co_, vo);
+            vr = fmla<F>(cr, cr, vr);
+            cv = fmla<F>(co_, cr, cv);
+          }
+        vo *= inv_count;
+        vr *= inv_count;
+        cv *= inv_count;
+        const float lum = (2.0f * mo * mr + 1e-4f) / (mo * mo + mr * mr + 1e-4f);
+        const float st = (2.0f * cv + 9e-4f) / (vo + vr + 9e-4f);
+        L.win[tid] = (1.0f - lum * st) * (full ? 64.0f : count);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        float deficit = 0.0f;
+        for (int w = 0; w < nwx * nwy; w++) deficit += L.win[w];
+        L.chan_dist[c] = D->channel_weight[c] * deficit;
+      }
+      __syncthreads();
+    }
+    // recon rate Y, X, B; distortion X, Y, B; rd_cost (src/ac_strategy.rs:1688)
+    float rate = 0.0f;
+    rate += L.chan_rate[1];
+    rate += L.chan_rate[0];
+    rate += L.chan_rate[2];
+    float distortion = 0.0f;
+    distortion += L.chan_dist[0];
+    distortion += L.chan_dist[1];
+    distortion += L.chan_dist[2];
+    costs[j] = fmla<F>(D->recon_lambda, rate + 0.0f, distortion);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    for (int j = 0; j < 4; j++) {
+      A.rq[(size_t)blk * 4 + j] = (uint8_t)qs_list[j < len ? j : 0];
+      A.rcost[(size_t)blk * 4 + j] = costs[j < len ? j : 0];
+    }
+    A.rlen[blk] = (uint8_t)(len > 1 ? len : 0);
+  }
+}
+
+// A fixed grid walks the list (its length is known on the device only).
+template <bool F, bool BIG>
+__global__ __launch_bounds__(BIG ? 256 : 64) void k_refine_costs(RefineArgs A) {
+  __shared__ RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> L;
+  const uint32_t n = A.count[BIG ? 1 : 0];
+  for (uint32_t item = blockIdx.x; item < n; item += gridDim.x) {
+    refine_costs_body<F, BIG>(L, A, item);
+    __syncthreads();
+  }
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// The QF trellis per DC group
+
+#define TR_T 256
+#define TR_CHUNK 1024
+
+struct TrellisArgs {
+  const uint8_t *cells;
+  uint8_t *qf;
+  int xb, yb;
+  const float *rcost;
+  const uint8_t *rq, *rlen;
+  uint32_t *chain_blk;  // scratch, xb * yb
+  uint8_t *chain_q, *chain_fixed, *chain_back;
+  const jxg_derived *D;
+};
+
+struct TrLds {
+  float prices[4][512];
+  uint32_t counts[4][512];
+  uint32_t totals[4];
+  uint32_t scan[TR_T];
+  // one chunk of the chain, staged ahead of the lane that runs the recursion
+  float cost[2][TR_CHUNK][4];
+  uint8_t q[2][TR_CHUNK][4];
+  uint8_t len[2][TR_CHUNK];
+  uint8_t fixed[2][TR_CHUNK];
+  uint8_t back[2][TR_CHUNK];
+  int n;
+};
+
+// STRATEGY_CODE_LUT (src/dc_group_data.rs:73-78)
+__constant__ uint8_t STRATEGY_CODE_DEV[24] = {0,  6,  7,  4,  5,  3,  12, 13, 10, 11, 14, 15,
+                                              16, 17, 18, 19, 20, 1,  2,  21, 22, 23, 8,  9};
+__device__ __forceinline__ int strategy_code(int s) { return STRATEGY_CODE_DEV[s]; }
+
+// qf_symbol (src/ac_strategy/selection.rs:725-733): the writer's context of
+// the left value and the packed residual, clamped to the table.
+__device__ __forceinline__ int qf_slot(int left, int qf) {
+  const int c = left > 11 ? 0 : (left > 5 ? 1 : (left > 3 ? 2 : 3));
+  const int64_t d = (int64_t)qf - left;
+  const uint32_t pk = (uint32_t)((d << 1) ^ (d >> 63));
+  return c * 512 + (int)(pk < 511u ? pk : 511u);
+}
+
+__device__ void stage_chunk(TrLds &L, const TrellisArgs &A, size_t base, int n, int chunk, int buf) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < TR_CHUNK; e += TR_T) {
+    const int i = chunk * TR_CHUNK + e;
+    if (i >= n) break;
+    const uint32_t blk = A.chain_blk[base + i];
+    const int ln = A.rlen[blk];
+    L.len[buf][e] = (uint8_t)ln;
+    L.fixed[buf][e] = A.chain_fixed[base + i];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      L.q[buf][e][k] = A.rq[(size_t)blk * 4 + k];
+      L.cost[buf][e][k] = A.rcost[(size_t)blk * 4 + k];
+    }
+  }
+}
+
+template <bool F>
+__device__ __forceinline__ void qf_trellis_body(TrLds &L, const TrellisArgs &A, int gx, int gy) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int x0 = gx * 256, y0 = gy * 256;
+  const int xs = min(256, A.xb - x0), ys = min(256, A.yb - y0);
+  if (xs <= 0 || ys <= 0) return;
+  const jxg_derived *D = A.D;
+  const size_t base = (size_t)y0 * A.xb + (size_t)x0 * ys;  // this group's scratch
+  // the raster chain of first blocks
+  int n = 0;
+  for (int r = 0; r < ys; r++) {
+    const size_t row = (size_t)(y0 + r) * A.xb + x0;
+    const bool first = tid < xs && (A.cells[row + tid] & 1);
+    const uint64_t m = __ballot(first);
+    if (lane == 0) L.scan[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < wave; w++) before += L.scan[w];
+    before += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const int total = (int)(L.scan[0] + L.scan[1] + L.scan[2] + L.scan[3]);
+    if (first) {
+      const uint32_t blk = (uint32_t)(row + tid);
+      A.chain_blk[base + n + before] = blk;
+      A.chain_fixed[base + n + before] = A.qf[blk];
+    }
+    n += total;
+    __syncthreads();
+  }
+  if (n == 0) return;
+  __threadfence_block();
+  // independent_choice (:695-704)
+  for (int i = tid; i < n; i += TR_T) {
+    const uint32_t blk = A.chain_blk[base + i];
+    const int ln = A.rlen[blk];
+    uint8_t q = A.chain_fixed[base + i];
+    if (ln > 1) {
+      int b = 0;
+      float bc = A.rcost[(size_t)blk * 4];
+      for (int k = 1; k < ln; k++) {
+        const float c = A.rcost[(size_t)blk * 4 + k];
+        if (c < bc) {
+          bc = c;
+          b = k;
+        }
+      }
+      q = A.rq[(size_t)blk * 4 + b];
+    }
+    A.chain_q[base + i] = q;
+  }
+  const int first_left = strategy_code(A.cells[(size_t)y0 * A.xb + x0] >> 1);
+  const float lambda_bits = fmla<F>(D->recon_lambda, 1.0f + 0.0f, 0.0f);
+  const int nchunks = (n + TR_CHUNK - 1) / TR_CHUNK;
+  for (int round = 0; round < 2; round++) {
+    // qf_residual_prices (:707-723): the histogram of the chain
+    for (int e = tid; e < 2048; e += TR_T) (&L.counts[0][0])[e] = 0;
+    if (tid < 4) L.totals[tid] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += TR_T) {
+      const int left = i == 0 ? first_left : (int)A.chain_q[base + i - 1] - 1;
+      const int slot = qf_slot(left, (int)A.chain_q[base + i] - 1);
+      atomicAdd(&(&L.counts[0][0])[slot], 1u);
+      atomicAdd(&L.totals[slot >> 9], 1u);
+    }
+    __syncthreads();
+    for (int e = tid; e < 2048; e += TR_T) {
+      const int c = e >> 9, v = e & 511;
+      const float denom = (float)L.totals[c] + 2.0f;
+      const float prior = 2.0f * fast_exp2<F>(-fminf((float)v + 1.0f, 100.0f), D->exp2_table);
+      (&L.prices[0][0])[e] = -dirty_log2f<F>(((float)L.counts[c][v] + prior) / denom);
+    }
+    // qf_trellis (:737-800): one lane runs the recursion over staged chunks
+    stage_chunk(L, A, base, n, 0, 0);
+    __syncthreads();
+    float prev_cost[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int prev_q[4] = {0, 0, 0, 0};
+    int prev_len = 0;
+    for (int ck = 0; ck < nchunks; ck++) {
+      const int buf = ck & 1;
+      if (tid == 0) {
+        const int cn = min(TR_CHUNK, n - ck * TR_CHUNK);
+        for (int e = 0; e < cn; e++) {
+          const int i = ck * TR_CHUNK + e;
+          int len = L.len[buf][e];
+          int cq[4];
+          float cc[4];
+          if (len > 1) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              cq[k] = L.q[buf][e][k];
+              cc[k] = L.cost[buf][e][k];
+            }
+          } else {
+            len = 1;
+            cq[0] = L.fixed[buf][e];
+            cc[0] = 0.0f;
+            cq[1] = cq[2] = cq[3] = cq[0];
+            cc[1] = cc[2] = cc[3] = 0.0f;
+          }
+          float cost[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+          uint32_t bk = 0;
+#pragma unroll
+          for (int st = 0; st < 4; st++) {
+            if (st >= len) continue;
+            const int qf = cq[st] - 1;
+            if (i == 0) {
+              cost[st] = cc[st] + lambda_bits * (&L.prices[0][0])[qf_slot(first_left, qf)];
+              continue;
+            }
+#pragma unroll
+            for (int pp = 0; pp < 4; pp++) {
+              if (pp >= prev_len) continue;
+              const float total =
+                  prev_cost[pp] + cc[st] + lambda_bits * (&L.prices[0][0])[qf_slot(prev_q[pp] - 1, qf)];
+              if (total < cost[st]) {
+                cost[st] = total;
+                bk = (bk & ~(3u << (2 * st))) | ((uint32_t)pp << (2 * st));
+              }
+            }
+          }
+          float fl = INFINITY;
+#pragma unroll
+          for (int st = 0; st < 4; st++)
+            if (st < len) fl = fminf(fl, cost[st]);
+#pragma unroll
+          for (int st = 0; st < 4; st++) {
+            prev_q[st] = cq[st];
+            prev_cost[st] = st < len ? cost[st] - fl : INFINITY;
+          }
+          prev_len = len;
+          L.back[buf][e] = (uint8_t)bk;
+        }
+      } else if (ck + 1 < nchunks) {
+        // the other threads stage the next chunk meanwhile (thread 0 reads
+        // only the current buffer)
+        const int tid2 = tid - 1;
+        for (int e = tid2; e < TR_CHUNK; e += TR_T - 1) {
+          const int i = (ck + 1) * TR_CHUNK + e;
+          if (i >= n) break;
+          const uint32_t blk = A.chain_blk[base + i];
+          L.len[buf ^ 1][e] = A.rlen[blk];
+          L.fixed[buf ^ 1][e] = A.chain_fixed[base + i];
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            L.q[buf ^ 1][e][k] = A.rq[(size_t)blk * 4 + k];
+            L.cost[buf ^ 1][e][k] = A.rcost[(size_t)blk * 4 + k];
+          }
+        }
+      }
+      __syncthreads();
+      // back pointers of this chunk out to the scratch
+      {
+        const int cn = min(TR_CHUNK, n - ck * TR_CHUNK);
+        for (int e = tid; e < cn; e += TR_T) A.chain_back[base + ck * TR_CHUNK + e] = L.back[buf][e];
+      }
+      __syncthreads();
+    }
+    // the end state: the first minimum (every cost is >= +0 after the floor)
+    if (tid == 0) {
+      int sb = 0;
+      for (int k = 1; k < prev_len; k++)
+        if (prev_cost[k] < prev_cost[sb]) sb = k;
+      L.n = sb;
+    }
+    __syncthreads();
+    // backtrack, one chunk at a time from the end: the chunk's back
+    // pointers and candidates staged in LDS first
+    int sstate = L.n;
+    for (int ck = nchunks - 1; ck >= 0; ck--) {
+      const int cn = min(TR_CHUNK, n - ck * TR_CHUNK);
+      __syncthreads();
+      stage_chunk(L, A, base, n, ck, 0);
+      for (int e = tid; e < cn; e += TR_T) L.back[0][e] = A.chain_back[base + ck * TR_CHUNK + e];
+      __syncthreads();
+      if (tid == 0) {
+        for (int e = cn - 1; e >= 0; e--) {
+          const int i = ck * TR_CHUNK + e;
+          const uint8_t q = L.len[0][e] > 1 ? L.q[0][e][sstate] : L.fixed[0][e];
+          A.chain_q[base + i] = q;
+          sstate = (L.back[0][e] >> (2 * sstate)) & 3;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // the chosen values over each transform whose value changed
+  for (int i = tid; i < n; i += TR_T) {
+    const uint8_t q = A.chain_q[base + i], old = A.chain_fixed[base + i];
+    if (q == old) continue;
+    const uint32_t blk = A.chain_blk[base + i];
+    const int s = A.cells[blk] >> 1, cw = cov_x(s), ch = cov_y(s);
+    const int bx = (int)(blk % (uint32_t)A.xb), by = (int)(blk / (uint32_t)A.xb);
+    for (int y = 0; y < ch; y++)
+      for (int x = 0; x < cw; x++) A.qf[(size_t)(by + y) * A.xb + bx + x] = q;
+  }
+}
+
+template <bool F>
+__global__ __launch_bounds__(TR_T) void k_qf_trellis(TrellisArgs A) {
+  __shared__ TrLds L;
+  qf_trellis_body<F>(L, A, blockIdx.x, blockIdx.y);
+}
+
+
+struct RfPtrs {
+  float *rcost;
+  uint8_t *rq, *rlen;
+  uint32_t *list0, *list1, *count, *chain_blk;
+  uint8_t *chain_q, *chain_fixed, *chain_back;
+};
+
+__host__ __device__ inline RfPtrs rf_ptrs(uint8_t *base, size_t nb) {
+  RfPtrs r;
+  r.rcost = (float *)base;
+  r.rq = base + 16 * nb;
+  r.rlen = r.rq + 4 * nb;
+  uint8_t *a = (uint8_t *)(((uintptr_t)(r.rlen + nb) + 15) & ~(uintptr_t)15);
+  r.list0 = (uint32_t *)a;
+  r.list1 = r.list0 + nb;
+  r.count = r.list1 + nb;
+  r.chain_blk = r.count + 4;
+  r.chain_q = (uint8_t *)(r.chain_blk + nb);
+  r.chain_fixed = r.chain_q + nb;
+  r.chain_back = r.chain_fixed + nb;
+  return r;
+}
+
+__host__ __device__ inline void rf_grids(size_t nb, unsigned *g0, unsigned *g1) {
+  *g0 = (unsigned)(nb < 16384 ? nb : 16384);
+  *g1 = (unsigned)(nb / 16 + 1 < 2048 ? nb / 16 + 1 : 2048);
+}
+
+inline RefineArgs refine_args(const float *xyb, int stride, int ph, int W, int H, const uint8_t *cells,
+                       const uint8_t *qf, const int8_t *ytox, const int8_t *ytob,
+                       const float *tables, const jxg_derived *D, const uint16_t *scanpos,
+                       const RfPtrs &r) {
+  RefineArgs A;
+  A.xyb = xyb;
+  A.stride = stride;
+  A.ph = ph;
+  A.W = W;
+  A.H = H;
+  A.xb = (W + 7) / 8;
+  A.yb = (H + 7) / 8;
+  A.cells = cells;
+  A.qf = qf;
+  A.ytox = ytox;
+  A.ytob = ytob;
+  A.tables = tables;
+  A.D = D;
+  A.scanpos = scanpos;
+  A.list[0] = r.list0;
+  A.list[1] = r.list1;
+  A.count = r.count;
+  A.rcost = r.rcost;
+  A.rq = r.rq;
+  A.rlen = r.rlen;
+  return A;
+}
+
+__device__ inline RefineArgs refine_args_dev(const BatchImg &I, const uint16_t *scanpos) {
+  const RfPtrs r = rf_ptrs(I.rf, (size_t)I.xb * I.yb);
+  RefineArgs A;
+  A.xyb = I.xyb;
+  A.stride = I.stride;
+  A.ph = I.ph;
+  A.W = I.W;
+  A.H = I.H;
+  A.xb = I.xb;
+  A.yb = I.yb;
+  A.cells = I.cells;
+  A.qf = I.qf;
+  A.ytox = I.ytox;
+  A.ytob = I.ytob;
+  A.tables = I.tables;
+  A.D = I.D;
+  A.scanpos = scanpos;
+  A.list[0] = r.list0;
+  A.list[1] = r.list1;
+  A.count = r.count;
+  A.rcost = r.rcost;
+  A.rq = r.rq;
+  A.rlen = r.rlen;
+  return A;
+}
+
+__host__ __device__ inline TrellisArgs trellis_args(const uint8_t *cells, uint8_t *qf, int xb, int yb, const jxg_derived *D,
+                         const RfPtrs &r) {
+  TrellisArgs T;
+  T.cells = cells;
+  T.qf = qf;
+  T.xb = xb;
+  T.yb = yb;
+  T.rcost = r.rcost;
+  T.rq = r.rq;
+  T.rlen = r.rlen;
+  T.chain_blk = r.chain_blk;
+  T.chain_q = r.chain_q;
+  T.chain_fixed = r.chain_fixed;
+  T.chain_back = r.chain_back;
+  T.D = D;
+  return T;
+}
+
+// ---- batch forms: the same bodies behind the flat index map ----
+
+__global__ void k_refine_zero_b(const BatchImg *__restrict__ img, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const BatchImg &I = img[i];
+    if (!I.D->refine_steps) continue;
+    uint32_t *c = rf_ptrs(I.rf, (size_t)I.xb * I.yb).count;
+    c[0] = 0;
+    c[1] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_refine_list_b(const BatchImg *__restrict__ img,
+                                                       const uint32_t *__restrict__ pfx, int n) {
+  const int bi = batch_image(pfx, n, blockIdx.x);
+  const BatchImg &I = img[bi];
+  if (!I.D->refine_steps) return;
+  const RefineArgs A = refine_args_dev(I, nullptr);
+  refine_list_body(A, (int)((blockIdx.x - pfx[bi]) * 256 + threadIdx.x));
+}
+
+template <bool F, bool BIG>
+__global__ __launch_bounds__(BIG ? 256 : 64) void k_refine_costs_b(const BatchImg *__restrict__ img,
+                                                                   const uint32_t *__restrict__ pfx,
+                                                                   int n,
+                                                                   const uint16_t *__restrict__ scanpos) {
+  __shared__ RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> L;
+  const int bi = batch_image(pfx, n, blockIdx.x);
+  const BatchImg &I = img[bi];
+  if (!I.D->refine_steps) return;
+  const RefineArgs A = refine_args_dev(I, scanpos);
+  const uint32_t first = blockIdx.x - pfx[bi], grid = pfx[bi + 1] - pfx[bi];
+  const uint32_t cnt = A.count[BIG ? 1 : 0];
+  for (uint32_t item = first; item < cnt; item += grid) {
+    refine_costs_body<F, BIG>(L, A, item);
+    __syncthreads();
+  }
+}
+
+template <bool F>
+__global__ __launch_bounds__(TR_T) void k_qf_trellis_b(const BatchImg *__restrict__ img,
+                                                       const uint32_t *__restrict__ pfx, int n) {
+  const int bi = batch_image(pfx, n, blockIdx.x);
+  const BatchImg &I = img[bi];
+  if (!I.D->refine_steps) return;
+  const RfPtrs r = rf_ptrs(I.rf, (size_t)I.xb * I.yb);
+  const TrellisArgs T = trellis_args(I.cells, I.qf, I.xb, I.yb, I.D, r);
+  const uint32_t bl = blockIdx.x - pfx[bi];
+  const int gw = (I.xb + 255) / 256;
+  __shared__ TrLds L;
+  qf_trellis_body<F>(L, T, (int)(bl % (uint32_t)gw), (int)(bl / (uint32_t)gw));
+}
+
+}  // namespace
+
+// Host side: the refinement after adjust_quant_field (single image), on st.
+JXG_HIDDEN int refine_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uint32_t stride,
+                             uint32_t ph, uint32_t W, uint32_t H, uint8_t *qf, const int8_t *ytox,
+                             const int8_t *ytob, const float *tables, const jxg_frame_params *p,
+                             const uint8_t *cells) {
+  const int xb = (int)((W + 7) / 8), yb = (int)((H + 7) / 8);
+  const size_t nb = (size_t)xb * yb;
+  int rc;
+  if ((rc = ensure(&ctx->d_rf, &ctx->rf_cap, jxg_refine_scratch_bytes(nb)))) return rc;
+  const RfPtrs r = rf_ptrs(ctx->d_rf, nb);
+  const RefineArgs A = refine_args(xyb, (int)stride, (int)ph, (int)W, (int)H, cells, qf, ytox, ytob,
+                                   tables, ctx->d_derived, ctx->d_scanpos, r);
+  unsigned g0, g1;
+  rf_grids(nb, &g0, &g1);
+  prof_begin(ctx, JXG_PROF_REFINE, st);
+  k_zero_u32<<<1, 64, 0, st>>>(r.count, 2);
+  k_refine_list<<<(unsigned)((nb + 255) / 256), 256, 0, st>>>(A);
+  LAUNCH_CHECK("k_refine_list");
+  if (p->fused) {
+    k_refine_costs<true, false><<<g0, 64, 0, st>>>(A);
+    k_refine_costs<true, true><<<g1, 256, 0, st>>>(A);
+  } else {
+    k_refine_costs<false, false><<<g0, 64, 0, st>>>(A);
+    k_refine_costs<false, true><<<g1, 256, 0, st>>>(A);
+  }
+  LAUNCH_CHECK("k_refine_costs");
+  const TrellisArgs T = trellis_args(cells, qf, xb, yb, ctx->d_derived, r);
+  const dim3 g((xb + 255) / 256, (yb + 255) / 256);
+  if (p->fused)
+    k_qf_trellis<true><<<g, TR_T, 0, st>>>(T);
+  else
+    k_qf_trellis<false><<<g, TR_T, 0, st>>>(T);
+  LAUNCH_CHECK("k_qf_trellis");
+  prof_end(ctx, JXG_PROF_REFINE, st);
+  return JXG_OK;
+}
+
+// Batch: the same stages over every image (images whose tier does not
+// refine have no workgroups).
+JXG_HIDDEN int batch_refine(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                            const uint32_t *d_pfx, int n, const uint32_t total[], int fused) {
+  const uint32_t *p = d_pfx;
+  const int m = n + 1;
+  if (!total[BS_TRELLIS]) return JXG_OK;
+  k_refine_zero_b<<<1, 256, 0, st>>>(d_img, n);
+  k_refine_list_b<<<total[BS_RLIST], 256, 0, st>>>(d_img, p + BS_RLIST * m, n);
+  if (fused) {
+    k_refine_costs_b<true, false><<<total[BS_RCOST_S], 64, 0, st>>>(d_img, p + BS_RCOST_S * m, n, ctx->d_scanpos);
+    k_refine_costs_b<true, true><<<total[BS_RCOST_B], 256, 0, st>>>(d_img, p + BS_RCOST_B * m, n, ctx->d_scanpos);
+    k_qf_trellis_b<true><<<total[BS_TRELLIS], TR_T, 0, st>>>(d_img, p + BS_TRELLIS * m, n);
+  } else {
+    k_refine_costs_b<false, false><<<total[BS_RCOST_S], 64, 0, st>>>(d_img, p + BS_RCOST_S * m, n, ctx->d_scanpos);
+    k_refine_costs_b<false, true><<<total[BS_RCOST_B], 256, 0, st>>>(d_img, p + BS_RCOST_B * m, n, ctx->d_scanpos);
+    k_qf_trellis_b<false><<<total[BS_TRELLIS], TR_T, 0, st>>>(d_img, p + BS_TRELLIS * m, n);
+  }
+  LAUNCH_CHECK("batch refine");
+  return JXG_OK;
+}
+
+// Workgroups of the refinement stages of one image (the batch grids).
+JXG_HIDDEN void refine_grids(uint32_t W, uint32_t H, uint32_t *g_list, uint32_t *g0, uint32_t *g1,
+                             uint32_t *g_trellis) {
+  const size_t xb = (W + 7) / 8, yb = (H + 7) / 8, nb = xb * yb;
+  unsigned a, b;
+  rf_grids(nb, &a, &b);
+  *g_list = (uint32_t)((nb + 255) / 256);
+  *g0 = a;
+  *g1 = b;
+  *g_trellis = (uint32_t)(((xb + 255) / 256) * ((yb + 255) / 256));
+}

@@ -849,11 +849,12 @@ JXG_HIDDEN int retile_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uin
 }
 
 // Batch: the same stages over every image (images whose tier does not
-// re-tile return at once).
+// re-tile have no workgroups).
 JXG_HIDDEN int batch_retile(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
                             const uint32_t *d_pfx, int n, const uint32_t total[], int fused) {
   const uint32_t *p = d_pfx;
   const int m = n + 1;
+  if (!total[BS_RETILE]) return JXG_OK;
   if (fused)
     k_piece_costs_b<true><<<total[BS_PIECE], 256, 0, st>>>(d_img, p + BS_PIECE * m, n, ctx->d_scanpos);
   else

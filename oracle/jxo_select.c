@@ -1037,6 +1037,8 @@ static void group_init(group_ctx *g, const float *xyb, int width, int height, in
   g->cells = cells;
 }
 
+static void refine_quant_field(group_ctx *g, uint8_t *qf_out);
+
 /* fill_ac_strategy returns before scoping when the speed has no square
  * merges or the distance takes the DCT8-only shortcut
  * (src/ac_strategy/selection.rs:1930-1935, src/ac_strategy.rs:71-79). */
@@ -1062,8 +1064,257 @@ void jxo_select_strategy(const float *xyb, int width, int height, uint8_t *qf,
       select_group(&g);
       retile_coefficient(&g, 3);
       adjust_quant_field(&g, qf);
+      refine_quant_field(&g, qf);
     }
   free(qf_in);
+}
+
+/* ------------------------------------------------------------------ */
+/* refine_quant_field with qf_trellis (src/ac_strategy/selection.rs:534-896): */
+/* after adjust_quant_field, the Fast tier tries one or two coarser steps of  */
+/* each transform's quantizer, prices every candidate by its reconstruction   */
+/* (quantize, inverse transform, SSIM deficit; jxo_recon.c), and picks the    */
+/* field along the DC group's raster chain of first blocks with a Viterbi     */
+/* search that also prices the QF residual stream (two rounds, the prices     */
+/* re-estimated from the chain each time).                                    */
+
+int jxo_recon_coeffs(const float *xyb, int width, int height, int s, int px, int py,
+                     const float cmap[3], float *co);
+float jxo_recon_cost(const float *xyb, int width, int height, int s, int px, int py, float qac,
+                     const float cmap[3], const float *co, const float *tables,
+                     const jxg_frame_params *p, float *dist_out, float *rate_out);
+float jxo_reconstruction_lambda(float d);
+
+/* quant_refinement_steps (:534-542) */
+static int refinement_steps(float d) { return d < 1.5f ? 0 : (d >= 2.0f ? 2 : 1); }
+
+/* STRATEGY_CODE_LUT (src/dc_group_data.rs:73-78) */
+static const uint8_t STRATEGY_CODE[24] = {0, 6, 7, 4, 5, 3, 12, 13, 10, 11, 14, 15,
+                                          16, 17, 18, 19, 20, 1, 2, 21, 22, 23, 8, 9};
+
+/* pack_signed (src/entropy/token.rs:159) */
+static uint32_t pack_signed(int32_t v) {
+  int64_t x = v;
+  return (uint32_t)((x << 1) ^ (x >> 63));
+}
+
+/* qf_symbol (:725-733) with ac_metadata_context (src/frame/metadata.rs:46) */
+static void qf_symbol(int left, int qf, int *c, int *v) {
+  *c = left > 11 ? 0 : (left > 5 ? 1 : (left > 3 ? 2 : 3));
+  uint32_t pk = pack_signed(qf - left);
+  *v = pk < 511 ? (int)pk : 511;
+}
+
+/* qf_residual_prices (:707-723) */
+static void qf_prices(int first_left, const uint8_t *chain, int n, float prices[4][512]) {
+  static __thread uint32_t counts[4][512];
+  uint32_t totals[4] = {0, 0, 0, 0};
+  memset(counts, 0, sizeof(counts));
+  int left = first_left;
+  for (int i = 0; i < n; i++) {
+    int qf = (int)chain[i] - 1, c, v;
+    qf_symbol(left, qf, &c, &v);
+    counts[c][v]++;
+    totals[c]++;
+    left = qf;
+  }
+  for (int c = 0; c < 4; c++) {
+    const float denom = (float)totals[c] + 2.0f;
+    for (int v = 0; v < 512; v++) {
+      const float prior = 2.0f * jxo_fast_exp2(-fminf((float)v + 1.0f, 100.0f));
+      prices[c][v] = -jxo_dirty_log2f(((float)counts[c][v] + prior) / denom);
+    }
+  }
+}
+
+typedef struct {
+  int len;
+  uint8_t q[4];
+  float cost[4];
+} qrefine;
+
+/* qf_trellis (:737-800) */
+static void qf_trellis(int first_left, const uint8_t *fixed, const qrefine *const *states, int n,
+                       float lambda_bits, const float prices[4][512], uint8_t *chain,
+                       uint8_t (*back)[4]) {
+  uint8_t prev_q[4] = {0, 0, 0, 0};
+  float prev_cost[4] = {0, 0, 0, 0};
+  int prev_len = 0;
+  for (int i = 0; i < n; i++) {
+    uint8_t cq[4];
+    float cc[4];
+    int len;
+    if (states[i]) {
+      len = states[i]->len;
+      for (int s = 0; s < 4; s++) {
+        cq[s] = states[i]->q[s];
+        cc[s] = states[i]->cost[s];
+      }
+    } else {
+      len = 1;
+      cq[0] = fixed[i];
+      cc[0] = 0.0f;
+    }
+    float cost[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+    back[i][0] = back[i][1] = back[i][2] = back[i][3] = 0;
+    for (int s = 0; s < len; s++) {
+      const int qf = (int)cq[s] - 1;
+      int c, v;
+      if (i == 0) {
+        qf_symbol(first_left, qf, &c, &v);
+        cost[s] = cc[s] + lambda_bits * prices[c][v];
+        continue;
+      }
+      for (int pp = 0; pp < prev_len; pp++) {
+        qf_symbol((int)prev_q[pp] - 1, qf, &c, &v);
+        const float total = prev_cost[pp] + cc[s] + lambda_bits * prices[c][v];
+        if (total < cost[s]) {
+          cost[s] = total;
+          back[i][s] = (uint8_t)pp;
+        }
+      }
+    }
+    float floor_ = INFINITY;
+    for (int s = 0; s < len; s++) floor_ = fminf(floor_, cost[s]);
+    for (int s = 0; s < len; s++) {
+      prev_q[s] = cq[s];
+      cost[s] -= floor_;
+    }
+    for (int s = 0; s < 4; s++) prev_cost[s] = cost[s];
+    prev_len = len;
+  }
+  if (n == 0) return;
+  int s = 0;
+  for (int k = 1; k < prev_len; k++)
+    if (prev_cost[k] < prev_cost[s]) s = k; /* first minimum (all >= +0) */
+  for (int i = n - 1; i >= 0; i--) {
+    chain[i] = states[i] ? states[i]->q[s] : fixed[i];
+    s = back[i][s];
+  }
+}
+
+/* Test entry points: the QF price table of a chain, and the trellis over
+ * n blocks whose states (up to 4 (q, cost) pairs; len 0 = a fixed block) are
+ * given in arrays. */
+void jxo_qf_prices(int first_left, const uint8_t *chain, int n, float *prices /* 4 x 512 */) {
+  qf_prices(first_left, chain, n, (float(*)[512])prices);
+}
+void jxo_qf_trellis(int first_left, const uint8_t *fixed, const int *lens, const uint8_t *qs,
+                    const float *costs, int n, float lambda_bits, const float *prices,
+                    uint8_t *chain) {
+  qrefine *refs = (qrefine *)malloc(sizeof(qrefine) * (n ? n : 1));
+  const qrefine **st = (const qrefine **)malloc(sizeof(qrefine *) * (n ? n : 1));
+  uint8_t(*back)[4] = (uint8_t(*)[4])malloc(4 * (size_t)(n ? n : 1));
+  for (int i = 0; i < n; i++) {
+    st[i] = NULL;
+    if (lens[i] > 0) {
+      refs[i].len = lens[i];
+      for (int k = 0; k < 4; k++) {
+        refs[i].q[k] = qs[i * 4 + k];
+        refs[i].cost[k] = costs[i * 4 + k];
+      }
+      st[i] = &refs[i];
+    }
+  }
+  qf_trellis(first_left, fixed, st, n, lambda_bits, (const float(*)[512])prices, chain, back);
+  free(refs);
+  free(st);
+  free(back);
+}
+
+/* refine_quant_field (:802-896) at the Fast tier (no learned prices, no
+ * cached rerank costs, no X-heavy upward step) on the adjusted field. */
+static void refine_quant_field(group_ctx *g, uint8_t *qf_out) {
+  const jxg_frame_params *p = g->p;
+  const int steps = refinement_steps(p->distance);
+  if (!steps || p->speed != JXG_SPEED_FAST) return;
+  const int xs = g->xsize, ys = g->ysize;
+  const int cap = xs * ys;
+  int *fbx = (int *)malloc(sizeof(int) * cap), *fby = (int *)malloc(sizeof(int) * cap);
+  uint8_t *fs = (uint8_t *)malloc(cap), *fixed = (uint8_t *)malloc(cap),
+          *chain = (uint8_t *)malloc(cap);
+  qrefine *refs = (qrefine *)malloc(sizeof(qrefine) * cap);
+  const qrefine **states = (const qrefine **)malloc(sizeof(qrefine *) * cap);
+  uint8_t(*back)[4] = (uint8_t(*)[4])malloc(4 * (size_t)cap);
+  static __thread float co[3 * 4096];
+  int n = 0;
+  for (int by = 0; by < ys; by++)
+    for (int bx = 0; bx < xs; bx++) {
+      const size_t i = (size_t)(g->gy0 + by) * g->qf_stride + g->gx0 + bx;
+      const uint8_t cell = g->cells[i];
+      if (!(cell & 1)) continue;
+      const int s = cell >> 1;
+      fbx[n] = bx;
+      fby[n] = by;
+      fs[n] = (uint8_t)s;
+      fixed[n] = qf_out[i];
+      states[n] = NULL;
+      const uint8_t q = qf_out[i];
+      if (!(s == JXG_DCT64X64 || s == JXG_DCT64X32 || s == JXG_DCT32X64) && q > 1) {
+        float cmap[3];
+        cmap_factors(g, bx, by, cmap);
+        const int px = (g->gx0 + bx) * 8, py = (g->gy0 + by) * 8;
+        jxo_recon_coeffs(g->xyb, g->width, g->height, s, px, py, cmap, co);
+        qrefine *r = &refs[n];
+        r->len = 1;
+        r->q[0] = q;
+        r->cost[0] = jxo_recon_cost(g->xyb, g->width, g->height, s, px, py,
+                                    p->scale * (float)q, cmap, co, g->tables, p, NULL, NULL);
+        for (int k = 1; k < 4; k++) {
+          r->q[k] = q;
+          r->cost[k] = r->cost[0];
+        }
+        const int cand[3] = {q - 1, steps == 2 ? q - 2 : q, q};
+        for (int k = 0; k < 3; k++) {
+          const int cq = cand[k] < 0 ? 0 : cand[k];
+          int dup = cq == 0;
+          for (int m = 0; m < r->len && !dup; m++) dup = r->q[m] == cq;
+          if (dup) continue;
+          r->q[r->len] = (uint8_t)cq;
+          r->cost[r->len] = jxo_recon_cost(g->xyb, g->width, g->height, s, px, py,
+                                           p->scale * (float)cq, cmap, co, g->tables, p, NULL,
+                                           NULL);
+          r->len++;
+        }
+        if (r->len > 1) states[n] = r;
+      }
+      n++;
+    }
+  /* independent_choice (:695-704) */
+  for (int i = 0; i < n; i++) {
+    if (!states[i]) {
+      chain[i] = fixed[i];
+      continue;
+    }
+    int b = 0;
+    for (int k = 1; k < states[i]->len; k++)
+      if (states[i]->cost[k] < states[i]->cost[b]) b = k;
+    chain[i] = states[i]->q[b];
+  }
+  if (n > 0) {
+    const int first_left = STRATEGY_CODE[g->cells[(size_t)g->gy0 * g->qf_stride + g->gx0] >> 1];
+    const float lambda_bits = jxo_fmla(jxo_reconstruction_lambda(p->distance), 1.0f + 0.0f, 0.0f);
+    static __thread float prices[4][512];
+    for (int round = 0; round < 2; round++) {
+      qf_prices(first_left, chain, n, prices);
+      qf_trellis(first_left, fixed, states, n, lambda_bits, (const float(*)[512])prices, chain, back);
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    if (chain[i] == fixed[i]) continue;
+    const int cw = jxo_covered_x(fs[i]), ch = jxo_covered_y(fs[i]);
+    for (int y = 0; y < ch; y++)
+      for (int x = 0; x < cw; x++)
+        qf_out[(size_t)(g->gy0 + fby[i] + y) * g->qf_stride + g->gx0 + fbx[i] + x] = chain[i];
+  }
+  free(fbx);
+  free(fby);
+  free(fs);
+  free(fixed);
+  free(chain);
+  free(refs);
+  free(states);
+  free(back);
 }
 
 /* One re-tiling pass (aligned: crossing = 0; shifted: 1) over every DC group
@@ -1178,6 +1429,7 @@ static void *fe_worker(void *arg) {
       select_group(&g);
       retile_coefficient(&g, 3);
       adjust_quant_field(&g, j->qf);
+      refine_quant_field(&g, j->qf);
     } else if (j->kind == 4) {
       const int tiles_x = (W + 63) / 64;
       for (int tx = 0; tx < tiles_x; tx++) {

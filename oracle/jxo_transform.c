@@ -610,6 +610,103 @@ static void inv_dct1d_8(float *b) {
   }
 }
 
+/* inv_dct1d_16 / inv_dct1d_32 (src/dct.rs:1547-1627) */
+static void inv_dct1d_16(float *b) {
+  const float IS2 = 0.70710678118654752440f;
+  float inv_wc[8];
+  for (int i = 0; i < 8; i++) inv_wc[i] = 1.0f / WC16[i];
+  float t[16];
+  for (int i = 0; i < 8; i++) {
+    t[i] = b[2 * i];
+    t[8 + i] = b[2 * i + 1];
+  }
+  for (int i = 14; i >= 9; i--) t[i] -= t[i + 1];
+  t[8] = (t[8] - t[9]) * IS2;
+  inv_dct1d_8(t + 8);
+  for (int i = 0; i < 8; i++) t[8 + i] *= inv_wc[i];
+  inv_dct1d_8(t);
+  for (int i = 0; i < 8; i++) {
+    b[i] = (t[i] + t[8 + i]) * 0.5f;
+    b[15 - i] = (t[i] - t[8 + i]) * 0.5f;
+  }
+}
+static void inv_dct1d_32(float *b) {
+  const float IS2 = 0.70710678118654752440f;
+  float inv_wc[16];
+  for (int i = 0; i < 16; i++) inv_wc[i] = 1.0f / WC32[i];
+  float t[32];
+  for (int i = 0; i < 16; i++) {
+    t[i] = b[2 * i];
+    t[16 + i] = b[2 * i + 1];
+  }
+  for (int i = 30; i >= 17; i--) t[i] -= t[i + 1];
+  t[16] = (t[16] - t[17]) * IS2;
+  inv_dct1d_16(t + 16);
+  for (int i = 0; i < 16; i++) t[16 + i] *= inv_wc[i];
+  inv_dct1d_16(t);
+  for (int i = 0; i < 16; i++) {
+    b[i] = (t[i] + t[16 + i]) * 0.5f;
+    b[31 - i] = (t[i] - t[16 + i]) * 0.5f;
+  }
+}
+static void inv_line(float *b, int n) {
+  if (n == 8) inv_dct1d_8(b);
+  else if (n == 16) inv_dct1d_16(b);
+  else inv_dct1d_32(b);
+}
+
+/* reconstruct_error (src/inflated_cost.rs:741-788) with the scalar inverse
+ * transforms of IdctMethods::scalar (src/dct.rs:1594-1680): inv_dct_square
+ * for DCT8 / 16X16 / 32X32 (coefficients transposed into place and scaled
+ * by N, columns then rows), inv_dct8x16 (rows of 16 px: 8-point columns,
+ * then 16-point rows) and inv_dct16x8 (16 tall: 8-point rows of the
+ * transposed storage, then 16-point columns).  out is the pixel footprint,
+ * row-major.  Returns 0, or -1 for a strategy these Fast layouts never hold. */
+int jxo_inverse_transform(int s, const float *coef, float *out) {
+  float col[32], acd[128];
+  switch (s) {
+    case JXG_DCT:
+    case JXG_DCT16X16:
+    case JXG_DCT32X32: {
+      const int n = s == JXG_DCT ? 8 : (s == JXG_DCT16X16 ? 16 : 32);
+      const float scale = (float)(n * n);
+      for (int a = 0; a < n; a++)
+        for (int b = 0; b < n; b++) out[b * n + a] = coef[a * n + b] * scale;
+      for (int u = 0; u < n; u++) {
+        for (int v = 0; v < n; v++) col[v] = out[v * n + u];
+        inv_line(col, n);
+        for (int v = 0; v < n; v++) out[v * n + u] = col[v];
+      }
+      for (int v = 0; v < n; v++) inv_line(out + v * n, n);
+      return 0;
+    }
+    case JXG_DCT8X16:
+      for (int v = 0; v < 8; v++)
+        for (int u = 0; u < 16; u++) out[v * 16 + u] = coef[v * 16 + u] * 128.0f;
+      for (int u = 0; u < 16; u++) {
+        for (int v = 0; v < 8; v++) col[v] = out[v * 16 + u];
+        inv_dct1d_8(col);
+        for (int v = 0; v < 8; v++) out[v * 16 + u] = col[v];
+      }
+      for (int r = 0; r < 8; r++) inv_dct1d_16(out + r * 16);
+      return 0;
+    case JXG_DCT16X8:
+      for (int v = 0; v < 16; v++) {
+        float row[8];
+        for (int u = 0; u < 8; u++) row[u] = coef[u * 16 + v] * 128.0f;
+        inv_dct1d_8(row);
+        for (int u = 0; u < 8; u++) acd[v * 8 + u] = row[u];
+      }
+      for (int u = 0; u < 8; u++) {
+        for (int v = 0; v < 16; v++) col[v] = acd[v * 8 + u];
+        inv_dct1d_16(col);
+        for (int i = 0; i < 16; i++) out[i * 8 + u] = col[i];
+      }
+      return 0;
+  }
+  return -1;
+}
+
 /* RESAMPLE_SCALE_64_TO_8 (src/dct.rs:2005-2014) */
 static const float RS64_8[8] = {1.0f,       0.9936866f, 0.97488683f, 0.94401807f,
                                 0.9017642f, 0.8490575f, 0.7870549f,  0.71710813f};

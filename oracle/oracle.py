@@ -126,6 +126,20 @@ def _declare(L):
     L.jxo_plan_small.argtypes = [_f32p, _f32p, _f32p, _f32p, c.c_uint64, c.c_int, c.c_int,
                                  c.c_float, c.POINTER(c.c_int), _u8p, _u8p]
     L.jxo_live_masks.argtypes = [_f32p, _f32p, _f32p, _f32p, c.c_int, c.c_int, _u8p]
+    L.jxo_inverse_transform.restype = c.c_int
+    L.jxo_inverse_transform.argtypes = [c.c_int, _f32p, _f32p]
+    L.jxo_reconstruction_lambda.restype = c.c_float
+    L.jxo_reconstruction_lambda.argtypes = [c.c_float]
+    L.jxo_ssim_deficit.restype = c.c_float
+    L.jxo_ssim_deficit.argtypes = [_f32p, _f32p, c.c_int, c.c_int, c.c_int, c.c_int]
+    L.jxo_qf_prices.argtypes = [c.c_int, _u8p, c.c_int, _f32p]
+    L.jxo_qf_trellis.argtypes = [c.c_int, _u8p, _i32p, _u8p, _f32p, c.c_int, c.c_float, _f32p, _u8p]
+    L.jxo_recon_coeffs.restype = c.c_int
+    L.jxo_recon_coeffs.argtypes = [_f32p, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, _f32p, _f32p]
+    L.jxo_recon_cost.restype = c.c_float
+    L.jxo_recon_cost.argtypes = [_f32p, c.c_int, c.c_int, c.c_int, c.c_int, c.c_int, c.c_float,
+                                 _f32p, _f32p, _f32p, c.c_void_p, c.POINTER(c.c_float),
+                                 c.POINTER(c.c_float)]
     L.jxo_retile_pass.argtypes = [_f32p, c.c_int, c.c_int, _u8p, _i8p, _i8p, _f32p, c.c_void_p,
                                   _u8p, c.c_int]
 
@@ -250,6 +264,72 @@ def live_masks(leaf, wide, tall, square, w, h):
     live = np.zeros(64, np.uint8)
     lib().jxo_live_masks(*[_p(a, _f32p) for a in arrs], w, h, _p(live, _u8p))
     return live
+
+
+def inverse_transform(strategy, coef):
+    """The scalar inverse transform of a coefficient block (storage order) to
+    its pixel footprint (row-major)."""
+    coef = np.ascontiguousarray(coef, np.float32).reshape(-1)
+    out = np.zeros(64 * 64, np.float32)
+    rc = lib().jxo_inverse_transform(strategy, _p(coef, _f32p), _p(out, _f32p))
+    assert rc == 0
+    return out[:coef.size]
+
+
+def reconstruction_lambda(d):
+    return float(lib().jxo_reconstruction_lambda(d))
+
+
+def ssim_deficit(orig, rec, visible=False, w=None, h=None):
+    orig = np.ascontiguousarray(orig, np.float32)
+    rec = np.ascontiguousarray(rec, np.float32)
+    hh, ww = orig.shape
+    return float(lib().jxo_ssim_deficit(_p(orig, _f32p), _p(rec, _f32p), ww,
+                                        ww if w is None else w, hh if h is None else h,
+                                        1 if visible else 0))
+
+
+def qf_prices(first_left, chain):
+    chain = np.ascontiguousarray(chain, np.uint8)
+    out = np.zeros((4, 512), np.float32)
+    lib().jxo_qf_prices(first_left, _p(chain, _u8p), chain.size, _p(out, _f32p))
+    return out
+
+
+def qf_trellis(first_left, fixed, states, lambda_bits, prices):
+    """states: per block None (fixed) or [(q, cost), ...] (up to 4)."""
+    n = len(fixed)
+    fixed = np.ascontiguousarray(fixed, np.uint8)
+    lens = np.zeros(n, np.int32)
+    qs = np.zeros((n, 4), np.uint8)
+    costs = np.zeros((n, 4), np.float32)
+    for i, st in enumerate(states):
+        if st:
+            lens[i] = len(st)
+            for k in range(4):
+                q, cst = st[min(k, len(st) - 1)]
+                qs[i, k] = q
+                costs[i, k] = cst
+    chain = np.zeros(n, np.uint8)
+    lib().jxo_qf_trellis(first_left, _p(fixed, _u8p), _p(lens, _i32p), _p(qs, _u8p),
+                         _p(costs, _f32p), n, lambda_bits,
+                         _p(np.ascontiguousarray(prices, np.float32), _f32p), _p(chain, _u8p))
+    return chain
+
+
+def recon_cost(xyb, strategy, px, py, qac, cmap, tables, params):
+    """(cost, distortion, rate) of the reconstruction-domain RD model."""
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    cm = np.ascontiguousarray(cmap, np.float32)
+    co = np.zeros(3 * 4096, np.float32)
+    lib().jxo_recon_coeffs(_p(xyb, _f32p), w, h, strategy, px, py, _p(cm, _f32p), _p(co, _f32p))
+    d = ctypes.c_float()
+    r = ctypes.c_float()
+    cost = lib().jxo_recon_cost(_p(xyb, _f32p), w, h, strategy, px, py, qac, _p(cm, _f32p),
+                                _p(co, _f32p), _p(np.ascontiguousarray(tables, np.float32), _f32p),
+                                ctypes.byref(params), ctypes.byref(d), ctypes.byref(r))
+    return float(cost), float(d.value), float(r.value)
 
 
 def retile_pass(xyb, qf, ytox, ytob, tables, params, cells, crossing):

@@ -32,7 +32,8 @@ def torch():
 
 
 def batch_images(J, torch, n, seed, fused=True):
-    """Uneven sizes (not multiples of 8), qualities 50..95, photo content."""
+    """Uneven sizes (not multiples of 8), qualities 50..95, photo content,
+    mixed speed tiers."""
     from jixel_amd import synth
     rng = np.random.default_rng(seed)
     out = []
@@ -44,7 +45,11 @@ def batch_images(J, torch, n, seed, fused=True):
         img = synth.photo_like(w, h, 1000 + i)
         if i % 5 == 4:
             img = np.ascontiguousarray(img[:, :, 1:2])  # a gray member
-        p = J.frame_params(J.EncodeConfig(fused=fused).with_quality(q))
+        # mixed tiers: Fast (re-tiling, and the quant-field refinement at
+        # d >= 1.5), Fastest (aligned re-tiling only) and Ultrafast (the fixed
+        # DCT8 layout)
+        speed = (J.Speed.FAST, J.Speed.FASTEST, J.Speed.FAST, J.Speed.ULTRAFAST)[i % 4]
+        p = J.frame_params(J.EncodeConfig(fused=fused, speed=speed).with_quality(q))
         out.append((img, p))
     return out
 

@@ -43,6 +43,7 @@ KERNEL_BYTES_PER_PX = {
     "quantize": 12.0 + 24.0 + 6.0 / 64 + 2.0 / 64,  # planes in, i32 + f32 out, DC, qf, cells
     "quantize_xt": 0.0,          # the tile-crossing queue (its bytes are a sliver of quantize's)
     "retile": 12.0 + 1.0 / 64,   # 3 planes in (the piece costs), field and cells rewritten
+    "refine": 12.0 + 2.0 / 64,   # 3 planes in, cells in, field rewritten
 }
 # FP32 work of the strategy search per pixel: the DCT8 / 16x16 / 32x32
 # candidates' butterflies (4 log2 N flops per pixel per channel for an N x N
@@ -54,6 +55,14 @@ SELECT_FLOPS_PER_PX = 3 * (12 + 16 + 20) + 3 * 3 * 10
 # 1792 + 1792 + 4096 per block and channel) and about 10 flops per
 # coefficient (576 per block and channel) for the RD terms, 3 channels.
 RETILE_FLOPS_PER_PX = 3 * (768 + 2 * 1792 + 4096 + 10 * 576) / 64
+# FP32 work of the quant-field refinement per pixel of a refined transform:
+# the forward transform (about 16 flops per pixel and channel) and, for each
+# of up to 4 candidate field values, the biased quantizer's error and rate
+# terms (about 10 flops per coefficient), the inverse transform (about 16 per
+# pixel) and the SSIM window sums (about 12 per pixel), 3 channels.
+REFINE_FLOPS_PER_PX = 3 * (16 + 4 * (10 + 16 + 12))
+VALU_FLOPS_PER_PX = {"select": SELECT_FLOPS_PER_PX, "retile": RETILE_FLOPS_PER_PX,
+                     "refine": REFINE_FLOPS_PER_PX}
 
 EXTRA_BYTES_PER_PX = {
     "gaborish": 12.0 + 12.0,  # 3 planes in, 3 planes out
@@ -70,7 +79,8 @@ STAGE_KERNEL = {"xyb": ["k_xyb"], "cmap": ["k_cmap_blocks", "k_cmap_solve"],
                 "alpha_prep": ["k_alpha_extract", "k_fringe", "k_fringe_commit", "k_astat_init",
                                "k_alpha_stats", "k_fill_u32", "k_repad", "k_alpha_blocks"],
                 "alpha_qf": ["k_alpha_apply"],
-                "retile": ["k_piece_costs", "k_retile", "k_adjust_qf"]}
+                "retile": ["k_piece_costs", "k_retile", "k_adjust_qf"],
+                "refine": ["k_zero_u32", "k_refine_list", "k_refine_costs", "k_qf_trellis"]}
 # fill_invisible's recurrence: anti-diagonals x + 2 y are its dependent steps,
 # and each step's chain is 14 dependent VALU operations (one DPP wave shift,
 # 4 fused multiply-adds and 3 adds for the sum, 5 FP64 operations for the
@@ -522,21 +532,22 @@ def front_end_workload(J, torch, fe, stream, d_rgb, p, steps, warmup):
 
 
 def roofline_of(kernel_ms, px_shape, traffic_stage_ok=True):
+    """Roofline of the dominant kernel.  The decision kernels (strategy
+    search, re-tiling, quant-field refinement) keep their tiles in LDS and
+    are bound by VALU issue: their bound is FP32 vector throughput, with the
+    HBM rate kept as a secondary figure.  The others are HBM-bound."""
     dominant = max(kernel_ms, key=kernel_ms.get)
     px = px_shape[0] * px_shape[1]
+    sec = kernel_ms[dominant] * 1e-3
     alg_bytes = KERNEL_BYTES_PER_PX[dominant] * px
-    achieved = alg_bytes / (kernel_ms[dominant] * 1e-3) / 1e9
-    r = {
-        "kernel": dominant,
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": read_traffic(dominant) if traffic_stage_ok else None,
-        "algorithmic_bytes_per_launch": int(alg_bytes),
-        "kernel_ms": round(kernel_ms[dominant], 4),
-    }
+    gbs = alg_bytes / sec / 1e9
+    hbm = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    r = {"kernel": dominant, "bound": "hbm"}
+    r.update(hbm)
+    r["traffic"] = read_traffic(dominant) if traffic_stage_ok else None
+    r["algorithmic_bytes_per_launch"] = int(alg_bytes)
+    r["kernel_ms"] = round(kernel_ms[dominant], 4)
     if dominant == "fill":
         # the recurrence's dependent steps bound it, not HBM: the anti-diagonals
         # times the per-step chain at 4 cycles per dependent operation
@@ -549,13 +560,13 @@ def roofline_of(kernel_ms, px_shape, traffic_stage_ok=True):
                               "cycles_per_op": 4, "clock_ghz": CLOCK_GHZ,
                               "ideal_ms": round(ideal, 4),
                               "frac": round(ideal / kernel_ms[dominant], 4)}
-    if dominant in ("select", "retile"):
-        fpp = SELECT_FLOPS_PER_PX if dominant == "select" else RETILE_FLOPS_PER_PX
-        tflops = fpp * px / (kernel_ms[dominant] * 1e-3) / 1e12
+    if dominant in VALU_FLOPS_PER_PX:
+        fpp = VALU_FLOPS_PER_PX[dominant]
+        tflops = fpp * px / sec / 1e12
+        r.update({"bound": "valu", "achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                  "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
+                  "flops_per_px": round(fpp, 1), "hbm": hbm})
         r["limiter"] = "VALU issue, not HBM: the candidate transforms and RD sums run in LDS"
-        r["fp32"] = {"achieved_tflops": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
-                     "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
-                     "flops_per_px": round(fpp, 1)}
         if dominant == "select":
             r["valu_busy"] = read_valu_busy("k_select") if traffic_stage_ok else None
     return r
@@ -669,8 +680,8 @@ def main():
             "distance": round(p.distance, 4), "speed": "fast",
             "decisions": "square merges DCT8/DCT16X16/DCT32X32, layout_dp::retile_coefficient "
                          "(aligned + (4,4)-shifted grids, unaligned pairs and squares), "
-                         "adjust_quant_field; the reference's FAST tier also runs "
-                         "refine_quant_field at d >= 1.5, which is not built",
+                         "adjust_quant_field, refine_quant_field with qf_trellis "
+                         "(2 steps at d >= 2): the FAST tier's decision at d > 1.709",
             "alpha": "alpha_fringe::clear, fill_invisible, alpha block factors",
             "cfl": "per-tile regression maps (fill_cmap, Fast path)",
             "tables": "DequantMatrices::new_fast tier 3 (SS2 scales, flat-B DCT8 row)",

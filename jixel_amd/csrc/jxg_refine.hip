@@ -60,16 +60,29 @@ __device__ __forceinline__ int rf_scanpos_offset(int cx, int cy) {
 }
 
 __device__ __forceinline__ void refine_list_body(const RefineArgs &A, int i) {
-  if (i >= A.xb * A.yb) return;
-  const uint8_t cell = A.cells[i];
-  if (!(cell & 1)) return;
-  A.rlen[i] = 0;
-  const int s = cell >> 1;
-  const bool small = s == JXG_DCT || s == JXG_DCT16X8 || s == JXG_DCT8X16 || s == JXG_DCT16X16;
-  if (!(small || s == JXG_DCT32X32) || A.qf[i] <= 1) return;
-  const int cls = small ? 0 : 1;
-  const uint32_t slot = atomicAdd(&A.count[cls], 1u);
-  A.list[cls][slot] = (uint32_t)i;
+  int cls = -1;
+  if (i < A.xb * A.yb) {
+    const uint8_t cell = A.cells[i];
+    if (cell & 1) {
+      A.rlen[i] = 0;
+      const int s = cell >> 1;
+      const bool small = s == JXG_DCT || s == JXG_DCT16X8 || s == JXG_DCT8X16 || s == JXG_DCT16X16;
+      if ((small || s == JXG_DCT32X32) && A.qf[i] > 1) cls = small ? 0 : 1;
+    }
+  }
+  // one atomic per wave and class (the list order is free: each entry
+  // writes its own block's record)
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    const uint64_t m = __ballot(cls == c);
+    if (!m) continue;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t first = 0;
+    if (lane == leader) first = atomicAdd(&A.count[c], (uint32_t)__popcll(m));
+    first = __shfl(first, leader);
+    if (cls == c) A.list[c][first + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)i;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_refine_list(RefineArgs A) {
@@ -444,7 +457,7 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 // The QF trellis per DC group
 
 #define TR_T 256
-#define TR_CHUNK 1024
+#define TR_WARM 16  // guess steps before a segment (measured: <= 5 reach the exact vector)
 
 struct TrellisArgs {
   const uint8_t *cells;
@@ -462,13 +475,12 @@ struct TrLds {
   uint32_t counts[4][512];
   uint32_t totals[4];
   uint32_t scan[TR_T];
-  // one chunk of the chain, staged ahead of the lane that runs the recursion
-  float cost[2][TR_CHUNK][4];
-  uint8_t q[2][TR_CHUNK][4];
-  uint8_t len[2][TR_CHUNK];
-  uint8_t fixed[2][TR_CHUNK];
-  uint8_t back[2][TR_CHUNK];
-  int n;
+  float sc[TR_T][4];    // per segment: start vector (survivor costs)
+  float ec[TR_T][4];    // per segment: end vector
+  uint8_t elen[TR_T];   // per segment: states at its last step
+  uint8_t map[TR_T];    // per segment: last-step state -> state before it (2 bits each)
+  uint8_t send[TR_T];   // per segment: state at its last step on the chosen path
+  int redo;             // the last verification pass that found a mismatch
 };
 
 // STRATEGY_CODE_LUT (src/dc_group_data.rs:73-78)
@@ -485,21 +497,87 @@ __device__ __forceinline__ int qf_slot(int left, int qf) {
   return c * 512 + (int)(pk < 511u ? pk : 511u);
 }
 
-__device__ void stage_chunk(TrLds &L, const TrellisArgs &A, size_t base, int n, int chunk, int buf) {
-  const int tid = threadIdx.x;
-  for (int e = tid; e < TR_CHUNK; e += TR_T) {
-    const int i = chunk * TR_CHUNK + e;
-    if (i >= n) break;
-    const uint32_t blk = A.chain_blk[base + i];
-    const int ln = A.rlen[blk];
-    L.len[buf][e] = (uint8_t)ln;
-    L.fixed[buf][e] = A.chain_fixed[base + i];
+// The survivors after one chain step: their values, costs (less the floor)
+// and count.
+struct TrVec {
+  int q[4];
+  float c[4];
+  int len;
+};
+
+// The candidates of chain step i: a refined transform's states, else its
+// fixed value at cost 0.
+__device__ __forceinline__ void tr_cands(const TrellisArgs &A, size_t base, int i, int *cq, float *cc,
+                                         int &len) {
+  const uint32_t blk = A.chain_blk[base + i];
+  len = A.rlen[blk];
+  if (len > 1) {
+    const uint32_t qq = *(const uint32_t *)(A.rq + (size_t)blk * 4);
+    const float4 c4 = *(const float4 *)(A.rcost + (size_t)blk * 4);
+#pragma unroll
+    for (int k = 0; k < 4; k++) cq[k] = (int)((qq >> (8 * k)) & 255u);
+    cc[0] = c4.x;
+    cc[1] = c4.y;
+    cc[2] = c4.z;
+    cc[3] = c4.w;
+  } else {
+    len = 1;
+    const int q = A.chain_fixed[base + i];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      L.q[buf][e][k] = A.rq[(size_t)blk * 4 + k];
-      L.cost[buf][e][k] = A.rcost[(size_t)blk * 4 + k];
+      cq[k] = q;
+      cc[k] = 0.0f;
     }
   }
+}
+
+// The survivor values of step i with arbitrary costs (the costs are set by
+// the caller).
+__device__ __forceinline__ void tr_load(const TrellisArgs &A, size_t base, int i, TrVec &v) {
+  float cc[4];
+  tr_cands(A, base, i, v.q, cc, v.len);
+}
+
+// One step of the recursion (:750-786) on v; the back pointers (2 bits per
+// state) are stored when `keep`.  Step 0 prices against first_left.
+template <bool F>
+__device__ __forceinline__ void tr_step(const TrLds &L, const TrellisArgs &A, size_t base, int i,
+                                        int first_left, float lambda_bits, TrVec &v, bool keep) {
+  int cq[4];
+  float cc[4];
+  int len;
+  tr_cands(A, base, i, cq, cc, len);
+  float cost[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  uint32_t bk = 0;
+#pragma unroll
+  for (int st = 0; st < 4; st++) {
+    if (st >= len) continue;
+    const int qf = cq[st] - 1;
+    if (i == 0) {
+      cost[st] = cc[st] + lambda_bits * (&L.prices[0][0])[qf_slot(first_left, qf)];
+      continue;
+    }
+#pragma unroll
+    for (int pp = 0; pp < 4; pp++) {
+      if (pp >= v.len) continue;
+      const float total = v.c[pp] + cc[st] + lambda_bits * (&L.prices[0][0])[qf_slot(v.q[pp] - 1, qf)];
+      if (total < cost[st]) {
+        cost[st] = total;
+        bk = (bk & ~(3u << (2 * st))) | ((uint32_t)pp << (2 * st));
+      }
+    }
+  }
+  float fl = INFINITY;
+#pragma unroll
+  for (int st = 0; st < 4; st++)
+    if (st < len) fl = fminf(fl, cost[st]);
+#pragma unroll
+  for (int st = 0; st < 4; st++) {
+    v.q[st] = cq[st];
+    v.c[st] = st < len ? cost[st] - fl : INFINITY;
+  }
+  v.len = len;
+  if (keep) A.chain_back[base + i] = (uint8_t)bk;
 }
 
 template <bool F>
@@ -553,7 +631,6 @@ __device__ __forceinline__ void qf_trellis_body(TrLds &L, const TrellisArgs &A, 
   }
   const int first_left = strategy_code(A.cells[(size_t)y0 * A.xb + x0] >> 1);
   const float lambda_bits = fmla<F>(D->recon_lambda, 1.0f + 0.0f, 0.0f);
-  const int nchunks = (n + TR_CHUNK - 1) / TR_CHUNK;
   for (int round = 0; round < 2; round++) {
     // qf_residual_prices (:707-723): the histogram of the chain
     for (int e = tid; e < 2048; e += TR_T) (&L.counts[0][0])[e] = 0;
@@ -572,116 +649,88 @@ __device__ __forceinline__ void qf_trellis_body(TrLds &L, const TrellisArgs &A, 
       const float prior = 2.0f * fast_exp2<F>(-fminf((float)v + 1.0f, 100.0f), D->exp2_table);
       (&L.prices[0][0])[e] = -dirty_log2f<F>(((float)L.counts[c][v] + prior) / denom);
     }
-    // qf_trellis (:737-800): one lane runs the recursion over staged chunks
-    stage_chunk(L, A, base, n, 0, 0);
     __syncthreads();
-    float prev_cost[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    int prev_q[4] = {0, 0, 0, 0};
-    int prev_len = 0;
-    for (int ck = 0; ck < nchunks; ck++) {
-      const int buf = ck & 1;
-      if (tid == 0) {
-        const int cn = min(TR_CHUNK, n - ck * TR_CHUNK);
-        for (int e = 0; e < cn; e++) {
-          const int i = ck * TR_CHUNK + e;
-          int len = L.len[buf][e];
-          int cq[4];
-          float cc[4];
-          if (len > 1) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-              cq[k] = L.q[buf][e][k];
-              cc[k] = L.cost[buf][e][k];
-            }
-          } else {
-            len = 1;
-            cq[0] = L.fixed[buf][e];
-            cc[0] = 0.0f;
-            cq[1] = cq[2] = cq[3] = cq[0];
-            cc[1] = cc[2] = cc[3] = 0.0f;
-          }
-          float cost[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-          uint32_t bk = 0;
-#pragma unroll
-          for (int st = 0; st < 4; st++) {
-            if (st >= len) continue;
-            const int qf = cq[st] - 1;
-            if (i == 0) {
-              cost[st] = cc[st] + lambda_bits * (&L.prices[0][0])[qf_slot(first_left, qf)];
-              continue;
-            }
-#pragma unroll
-            for (int pp = 0; pp < 4; pp++) {
-              if (pp >= prev_len) continue;
-              const float total =
-                  prev_cost[pp] + cc[st] + lambda_bits * (&L.prices[0][0])[qf_slot(prev_q[pp] - 1, qf)];
-              if (total < cost[st]) {
-                cost[st] = total;
-                bk = (bk & ~(3u << (2 * st))) | ((uint32_t)pp << (2 * st));
-              }
-            }
-          }
-          float fl = INFINITY;
-#pragma unroll
-          for (int st = 0; st < 4; st++)
-            if (st < len) fl = fminf(fl, cost[st]);
-#pragma unroll
-          for (int st = 0; st < 4; st++) {
-            prev_q[st] = cq[st];
-            prev_cost[st] = st < len ? cost[st] - fl : INFINITY;
-          }
-          prev_len = len;
-          L.back[buf][e] = (uint8_t)bk;
-        }
-      } else if (ck + 1 < nchunks) {
-        // the other threads stage the next chunk meanwhile (thread 0 reads
-        // only the current buffer)
-        const int tid2 = tid - 1;
-        for (int e = tid2; e < TR_CHUNK; e += TR_T - 1) {
-          const int i = (ck + 1) * TR_CHUNK + e;
-          if (i >= n) break;
-          const uint32_t blk = A.chain_blk[base + i];
-          L.len[buf ^ 1][e] = A.rlen[blk];
-          L.fixed[buf ^ 1][e] = A.chain_fixed[base + i];
-#pragma unroll
-          for (int k = 0; k < 4; k++) {
-            L.q[buf ^ 1][e][k] = A.rq[(size_t)blk * 4 + k];
-            L.cost[buf ^ 1][e][k] = A.rcost[(size_t)blk * 4 + k];
-          }
-        }
+    // qf_trellis (:737-800), segmented: lane t owns steps [t*seg, t*seg+seg)
+    const int seg = (n + TR_T - 1) / TR_T;
+    const int nseg = (n + seg - 1) / seg;
+    const int s0 = tid * seg, s1 = min(n, s0 + seg);
+    const bool own = tid < nseg;
+    TrVec v;
+    if (tid == 0) L.redo = 0;
+    if (own) {
+      // the start vector: exact at step 0, else a guess from TR_WARM steps
+      // of the recursion begun at zero costs
+      const int w0 = max(0, s0 - TR_WARM);
+      if (w0 == 0) {
+        v.len = 0;
+      } else {
+        tr_load(A, base, w0 - 1, v);
+        for (int k = 0; k < 4; k++) v.c[k] = 0.0f;
       }
-      __syncthreads();
-      // back pointers of this chunk out to the scratch
-      {
-        const int cn = min(TR_CHUNK, n - ck * TR_CHUNK);
-        for (int e = tid; e < cn; e += TR_T) A.chain_back[base + ck * TR_CHUNK + e] = L.back[buf][e];
-      }
-      __syncthreads();
+      for (int i = w0; i < s0; i++) tr_step<F>(L, A, base, i, first_left, lambda_bits, v, false);
+      for (int k = 0; k < 4; k++) L.sc[tid][k] = v.c[k];
+      for (int i = s0; i < s1; i++) tr_step<F>(L, A, base, i, first_left, lambda_bits, v, true);
+      for (int k = 0; k < 4; k++) L.ec[tid][k] = v.c[k];
+      L.elen[tid] = (uint8_t)v.len;
     }
-    // the end state: the first minimum (every cost is >= +0 after the floor)
+    // verify: a segment's result stands once its start vector equals its
+    // predecessor's end vector bit for bit (step 0 is exact); a mismatch is
+    // re-run from the predecessor's end, until none is left
+    for (int pass = 1;; pass++) {
+      __syncthreads();
+      bool redo = false;
+      float start[4];
+      if (own && tid > 0) {
+        const int ln = L.elen[tid - 1];
+        for (int k = 0; k < 4; k++) {
+          start[k] = L.ec[tid - 1][k];
+          if (k < ln && __float_as_uint(start[k]) != __float_as_uint(L.sc[tid][k])) redo = true;
+        }
+      }
+      if (redo) L.redo = pass;
+      __syncthreads();
+      if (L.redo != pass) break;
+      if (redo) {
+        tr_load(A, base, s0 - 1, v);
+        for (int k = 0; k < 4; k++) {
+          v.c[k] = start[k];
+          L.sc[tid][k] = start[k];
+        }
+        for (int i = s0; i < s1; i++) tr_step<F>(L, A, base, i, first_left, lambda_bits, v, true);
+        for (int k = 0; k < 4; k++) L.ec[tid][k] = v.c[k];
+      }
+    }
+    // backtrack: per segment the map from its last step's state to the
+    // state before its first step, composed from the end by one lane
+    uint32_t map = 0;
+    if (own) {
+      int st[4] = {0, 1, 2, 3};
+      for (int i = s1 - 1; i >= s0; i--) {
+        const uint32_t bk = A.chain_back[base + i];
+#pragma unroll
+        for (int k = 0; k < 4; k++) st[k] = (int)((bk >> (2 * st[k])) & 3u);
+      }
+      map = (uint32_t)(st[0] | st[1] << 2 | st[2] << 4 | st[3] << 6);
+      L.map[tid] = (uint8_t)map;
+    }
+    __syncthreads();
     if (tid == 0) {
+      // the end state: the first minimum (every cost is >= +0 after the floor)
+      const int last = nseg - 1, ln = L.elen[last];
       int sb = 0;
-      for (int k = 1; k < prev_len; k++)
-        if (prev_cost[k] < prev_cost[sb]) sb = k;
-      L.n = sb;
+      for (int k = 1; k < ln; k++)
+        if (L.ec[last][k] < L.ec[last][sb]) sb = k;
+      L.send[last] = (uint8_t)sb;
+      for (int t = last; t > 0; t--) L.send[t - 1] = (uint8_t)((L.map[t] >> (2 * L.send[t])) & 3u);
     }
     __syncthreads();
-    // backtrack, one chunk at a time from the end: the chunk's back
-    // pointers and candidates staged in LDS first
-    int sstate = L.n;
-    for (int ck = nchunks - 1; ck >= 0; ck--) {
-      const int cn = min(TR_CHUNK, n - ck * TR_CHUNK);
-      __syncthreads();
-      stage_chunk(L, A, base, n, ck, 0);
-      for (int e = tid; e < cn; e += TR_T) L.back[0][e] = A.chain_back[base + ck * TR_CHUNK + e];
-      __syncthreads();
-      if (tid == 0) {
-        for (int e = cn - 1; e >= 0; e--) {
-          const int i = ck * TR_CHUNK + e;
-          const uint8_t q = L.len[0][e] > 1 ? L.q[0][e][sstate] : L.fixed[0][e];
-          A.chain_q[base + i] = q;
-          sstate = (L.back[0][e] >> (2 * sstate)) & 3;
-        }
+    if (own) {
+      int sstate = L.send[tid];
+      for (int i = s1 - 1; i >= s0; i--) {
+        const uint32_t blk = A.chain_blk[base + i];
+        const uint8_t q = A.rlen[blk] > 1 ? A.rq[(size_t)blk * 4 + sstate] : A.chain_fixed[base + i];
+        A.chain_q[base + i] = q;
+        sstate = (int)((A.chain_back[base + i] >> (2 * sstate)) & 3u);
       }
     }
     __syncthreads();

@@ -90,6 +90,7 @@ struct PcLds {
   float qac[16];             // per site of the shape: region quantizer
   float sse[3][16], bits[3][16];
   float cost[4][16];
+  int need_leaf;             // some block of the region lacks its DCT8 cost
 };
 
 struct PieceArgs {
@@ -344,12 +345,17 @@ __device__ __forceinline__ void piece_costs_body(const PieceArgs &A, int bix, in
   // The strategy search has priced the DCT8 of (nearly) every block in the
   // leaf plane already (the same strategy_cost); the leaves are priced here
   // only when one of the region's blocks lacks one.
-  bool need_leaf = false;
-  if (tid < 16) {
-    const int x = rbx + (tid & 3), y = rby + (tid >> 2);
-    if (x < A.xb && y < A.yb) need_leaf = isnan(A.pc[(size_t)y * A.xb + x]);
+  if (tid < 64) {
+    bool nl = false;
+    if (tid < 16) {
+      const int x = rbx + (tid & 3), y = rby + (tid >> 2);
+      if (x < A.xb && y < A.yb) nl = isnan(A.pc[(size_t)y * A.xb + x]);
+    }
+    const uint64_t m = __ballot(nl);
+    if (tid == 0) L.need_leaf = m != 0 ? 1 : 0;
   }
-  need_leaf = __syncthreads_or(need_leaf);
+  __syncthreads();
+  const bool need_leaf = L.need_leaf != 0;
   if (need_leaf) piece_batch<F, 0>(L, A, 0, rbx, rby, fx, fb);
   piece_batch<F, 1>(L, A, 0, rbx, rby, fx, fb);
   piece_batch<F, 2>(L, A, 0, rbx, rby, fx, fb);

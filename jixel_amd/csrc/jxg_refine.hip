@@ -54,6 +54,13 @@ struct RefineArgs {
   uint8_t *rlen;            // 1 per block: candidate count (0: not refined)
 };
 
+// A wave-uniform 64-bit value in scalar registers.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ int rf_scanpos_offset(int cx, int cy) {
   const int a = cx * cy;
   return a == 1 ? 0 : (a == 2 ? 64 : (a == 4 ? (cx == 2 ? 192 : 448) : (a == 16 ? 1216 : 0)));
@@ -95,31 +102,34 @@ struct RfLds {
   float pix[3][NMAX];   // source pixels, row-major (pitch W)
   float co[3][NPITCH];  // coefficients after CfL, in-place layout (pitch W + 1)
   float err[3][NMAX];   // quantizer errors, storage order
-  float sp[NPITCH];     // spatial error of one channel (pitch W + 1)
-  float mg[NMAX];       // rate terms, storage order
-  uint64_t nzm[NMAX / 64];
-  float win[16];        // per window: SSIM deficit
+  float sp[3][NPITCH];  // per channel: the nonzero levels' rate terms, then the spatial error
+  uint64_t nzm[3][NMAX / 64];
+  uint32_t mx[3];       // per channel: largest scan position of a nonzero level + 1
+  float win[3][16];     // per channel and window: SSIM deficit
   float quot[3][64];
   float rate[256];
   float chan_rate[3], chan_dist[3];
 };
 
-template <bool F, bool BIG>
-__device__ __forceinline__ void refine_costs_body(RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> &L,
-                                                  const RefineArgs &A, uint32_t item) {
+// One listed transform of strategy S (the shape fixed at compile time, so
+// that every index is a shift or a constant).
+template <bool F, bool BIG, int S>
+__device__ __forceinline__ void refine_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> &L,
+                                             const RefineArgs &A, uint32_t blk) {
   constexpr int NMAX = BIG ? 1024 : 256, T = BIG ? 256 : 64;
   (void)NMAX;
   const int tid = threadIdx.x, lane = tid & 63;
   const jxg_derived *D = A.D;
-  const uint32_t blk = A.list[BIG ? 1 : 0][item];
   const int bx = (int)(blk % (uint32_t)A.xb), by = (int)(blk / (uint32_t)A.xb);
-  const int s = A.cells[blk] >> 1;
+  constexpr int s = S;
   const int q0 = A.qf[blk];
-  const int cvx = cov_x(s), cvy = cov_y(s);
-  const int W = cvx * 8, H = cvy * 8, P = W + 1, N = W * H;
-  const int cx = max(cvx, cvy), cy = min(cvx, cvy), SW = cx * 8, SH = cy * 8;
-  const bool rows_first = s == JXG_DCT || s == JXG_DCT8X16;
-  const bool trans = s != JXG_DCT8X16;  // storage k = c * H + r (else r * W + c)
+  constexpr int cvx = (S == JXG_DCT8X16 || S == JXG_DCT16X16) ? 2 : (S == JXG_DCT32X32 ? 4 : 1);
+  constexpr int cvy = (S == JXG_DCT16X8 || S == JXG_DCT16X16) ? 2 : (S == JXG_DCT32X32 ? 4 : 1);
+  constexpr int W = cvx * 8, H = cvy * 8, P = W + 1, N = W * H;
+  constexpr int cx = cvx > cvy ? cvx : cvy, cy = cvx > cvy ? cvy : cvx, SW = cx * 8, SH = cy * 8;
+  constexpr bool rows_first = S == JXG_DCT || S == JXG_DCT8X16;
+  constexpr bool trans = S != JXG_DCT8X16;  // storage k = c * H + r (else r * W + c)
+  static_assert(N % T == 0, "whole coefficient rows per thread");
   const int kind = qt_kind(s);
   const uint32_t qoff = D->qt_off[kind], qn = D->qt_n[kind];
   const int cls = jxg_area_class(cx, cy);
@@ -208,66 +218,85 @@ __device__ __forceinline__ void refine_costs_body(RfLds<BIG ? 1024 : 256, BIG ? 
   const int vw = min(W, A.W - px), vh = min(H, A.H - py);
   const int nwx = full ? W / 8 : (vw + 7) / 8, nwy = full ? H / 8 : (vh + 7) / 8;
   float costs[4];
+  constexpr int KPT = NMAX / T;  // coefficients per thread and channel (at most)
   for (int j = 0; j < len; j++) {
     const float qac = D->scale * (float)qs_list[j];
+    if (tid < 3) L.mx[tid] = 0u;
     // quantize Y, then X and B against the luma error
-    for (int ci = 0; ci < 3; ci++) {
-      const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
-      const float *inv = A.tables + qoff + (3 + c) * qn;
-      const float *mat = A.tables + qoff + c * qn;
-      const float qs = qac * D->qm_mult[c];
-      const float inv_scale = 1.0f / qs;
-      const float one = D->dequant_one[c], num = D->dequant_numerator;
-      const float t0 = D->thr[c][cls][0], t1 = D->thr[c][cls][1];
-      const float t2 = D->thr[c][cls][2], t3 = D->thr[c][cls][3];
-      const float factor = c == 0 ? fx : fb;
-      for (int k0 = 0; k0 < N; k0 += T) {
-        const int k = k0 + tid;
-        const int srow = k / SW, scol = k - srow * SW;
-        float e = 0.0f, rt = 0.0f;
-        bool nz = false;
-        if (!(scol < cx && srow < cy)) {
-          const int o = coef_off(k);
-          const float coef = c == 1 ? L.co[1][o] : fmla<F>(factor, L.err[1][k], L.co[c][o]);
-          const int yfix = srow >= SH / 2 ? 2 : 0;
-          const float thr = scol >= SW / 2 ? (yfix ? t3 : t1) : (yfix ? t2 : t0);
-          const float scaled = (inv[k] * qs) * coef;
-          const float quantized = fabsf(scaled) >= thr ? round_away(scaled) : 0.0f;
-          const float aq = fabsf(quantized);
-          const float lev =
-              aq < 64.0f ? quantized - copysignf(L.quot[c][(int)aq], quantized) : quantized - num / quantized;
-          e = (scaled - lev) * (mat[k] * inv_scale);
-          if (quantized != 0.0f) {
-            nz = true;
-            if (aq < 256.0f)
-              rt = L.rate[(int)aq];
-            else {
-              const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
-              rt = ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
+    uint32_t mxl[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int phase = 0; phase < 2; phase++) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        if ((phase == 0) != (c == 1)) continue;
+        const float *inv = A.tables + qoff + (3 + c) * qn;
+        const float *mat = A.tables + qoff + c * qn;
+        const float qs = qac * D->qm_mult[c];
+        const float inv_scale = 1.0f / qs;
+        const float num = D->dequant_numerator;
+        const float t0 = D->thr[c][cls][0], t1 = D->thr[c][cls][1];
+        const float t2 = D->thr[c][cls][2], t3 = D->thr[c][cls][3];
+        const float factor = c == 0 ? fx : fb;
+#pragma unroll
+        for (int i = 0; i < KPT; i++) {
+          const int k = tid + T * i;
+          if (T * i >= N) continue;  // uniform: N is a multiple of T
+          const int srow = k / SW, scol = k - srow * SW;
+          float e = 0.0f, r = 0.0f;
+          bool nz = false;
+          if (!(scol < cx && srow < cy)) {
+            const int o = coef_off(k);
+            const float coef = c == 1 ? L.co[1][o] : fmla<F>(factor, L.err[1][k], L.co[c][o]);
+            const int yfix = srow >= SH / 2 ? 2 : 0;
+            const float thr = scol >= SW / 2 ? (yfix ? t3 : t1) : (yfix ? t2 : t0);
+            const float scaled = (inv[k] * qs) * coef;
+            const float quantized = fabsf(scaled) >= thr ? round_away(scaled) : 0.0f;
+            const float aq = fabsf(quantized);
+            const float lev =
+                aq < 64.0f ? quantized - copysignf(L.quot[c][(int)aq], quantized) : quantized - num / quantized;
+            e = (scaled - lev) * (mat[k] * inv_scale);
+            if (quantized != 0.0f) {
+              nz = true;
+              if (aq < 256.0f)
+                r = L.rate[(int)aq];
+              else {
+                const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
+                r = ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
+              }
             }
           }
+          L.err[c][k] = e;
+          L.sp[c][k] = r;  // the rate term, until the inverse transform
+          if (nz) mxl[c] = max(mxl[c], (uint32_t)A.scanpos[spoff + k] + 1u);
+          const uint64_t m = __ballot(nz);
+          if (lane == 0) L.nzm[c][k >> 6] = m;
         }
-        L.err[c][k] = e;
-        L.mg[k] = rt;
-        const uint64_t m = __ballot(nz);
-        if (lane == 0) L.nzm[k >> 6] = m;
       }
       __syncthreads();
-      if (tid == 0) {
-        // the coefficient rate model over the nonzero levels, storage order
-        float mag = 0.0f;
-        uint32_t nzc = 0, mx = 0;
-        for (int ch = 0; ch < N / 64; ch++) {
-          uint64_t m = L.nzm[ch];
-          while (m) {
-            const int i = __builtin_ctzll(m);
-            m &= m - 1;
-            const int k = ch * 64 + i;
-            mag += L.mg[k];
-            mx = max(mx, (uint32_t)A.scanpos[spoff + k] + 1u);
-            nzc++;
-          }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      if (mxl[c]) atomicMax(&L.mx[c], mxl[c]);
+    __syncthreads();
+    // the coefficient rate model: one wave per channel walks the nonzero
+    // levels in storage order (the order of the sum), taking each term from
+    // its lane
+    for (int c = tid >> 6; c < 3; c += T / 64) {
+      float mag = 0.0f;
+      uint32_t nzc = 0;
+      for (int ch = 0; ch < N / 64; ch++) {
+        uint64_t m = uniform64(L.nzm[c][ch]);
+        if (!m) continue;
+        const int val = __float_as_int(L.sp[c][ch * 64 + lane]);
+        nzc += (uint32_t)__popcll(m);
+        while (m) {
+          const int b = __builtin_ctzll(m);
+          m &= m - 1ull;
+          mag += __int_as_float(__builtin_amdgcn_readlane(val, b));
         }
+      }
+      if (lane == 0) {
+        const uint32_t mx = L.mx[c];
         const float nzf = (float)nzc;
         const float header =
             0.4f * (nzc < 256u ? L.rate[nzc] : (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf)));
@@ -278,17 +307,18 @@ __device__ __forceinline__ void refine_costs_body(RfLds<BIG ? 1024 : 256, BIG ? 
         }
         L.chan_rate[c] = nzf * 1.6f + 1.0f * mag + header + 0.5f * visited;
       }
-      __syncthreads();
     }
-    // inverse transform and SSIM deficit of each channel
-    for (int c = 0; c < 3; c++) {
-      // reconstruct_error with the scalar inverse transforms (src/dct.rs:
-      // 1594-1680): squares take the storage transposed into place, scaled
-      // by N, columns then rows; DCT8X16 columns (8) then rows (16); DCT16X8
-      // 8-point rows of the transposed storage, then 16-point columns.
+    __syncthreads();
+    // reconstruct_error with the scalar inverse transforms (src/dct.rs:
+    // 1594-1680), the three channels side by side: squares take the storage
+    // transposed into place, scaled by N, columns then rows; DCT8X16
+    // columns (8) then rows (16); DCT16X8 8-point rows of the transposed
+    // storage, then 16-point columns.
+    {
       const float nsc = (float)N;
-      for (int k = tid; k < N; k += T) {
-        float v = L.err[c][k] * nsc;
+      for (int e = tid; e < 3 * N; e += T) {
+        const int c = e / N, k = e - c * N;
+        const float v = L.err[c][k] * nsc;
         int r, col;
         if (s == JXG_DCT8X16) {
           r = k / 16;
@@ -301,122 +331,170 @@ __device__ __forceinline__ void refine_costs_body(RfLds<BIG ? 1024 : 256, BIG ? 
           r = k % W;  // out[b * n + a] = coef[a * n + b]: a = k / n, b = k % n
           col = k / W;
         }
-        L.sp[r * P + col] = v;
+        L.sp[c][r * P + col] = v;
       }
       __syncthreads();
       if (s == JXG_DCT16X8) {
         // rows (8 points) first, then columns (16 points)
-        for (int e = tid; e < H; e += T) {
+        for (int e = tid; e < 3 * H; e += T) {
+          float *sp = L.sp[e / H];
+          const int row = e % H;
           float b[8];
 #pragma unroll
-          for (int i = 0; i < 8; i++) b[i] = L.sp[e * P + i];
+          for (int i = 0; i < 8; i++) b[i] = sp[row * P + i];
           inv_dct8(b);
 #pragma unroll
-          for (int i = 0; i < 8; i++) L.sp[e * P + i] = b[i];
+          for (int i = 0; i < 8; i++) sp[row * P + i] = b[i];
         }
         __syncthreads();
-        for (int e = tid; e < W; e += T) {
+        for (int e = tid; e < 3 * W; e += T) {
+          float *sp = L.sp[e / W];
+          const int col = e % W;
           float b[16];
 #pragma unroll
-          for (int i = 0; i < 16; i++) b[i] = L.sp[i * P + e];
+          for (int i = 0; i < 16; i++) b[i] = sp[i * P + col];
           inv_dct16(b);
 #pragma unroll
-          for (int i = 0; i < 16; i++) L.sp[i * P + e] = b[i];
+          for (int i = 0; i < 16; i++) sp[i * P + col] = b[i];
         }
       } else {
         // columns first, then rows
-        for (int e = tid; e < W; e += T) {
+        for (int e = tid; e < 3 * W; e += T) {
+          float *sp = L.sp[e / W];
+          const int col = e % W;
           if (H == 8) {
             float b[8];
 #pragma unroll
-            for (int i = 0; i < 8; i++) b[i] = L.sp[i * P + e];
+            for (int i = 0; i < 8; i++) b[i] = sp[i * P + col];
             inv_dct8(b);
 #pragma unroll
-            for (int i = 0; i < 8; i++) L.sp[i * P + e] = b[i];
+            for (int i = 0; i < 8; i++) sp[i * P + col] = b[i];
           } else if (H == 16) {
             float b[16];
 #pragma unroll
-            for (int i = 0; i < 16; i++) b[i] = L.sp[i * P + e];
+            for (int i = 0; i < 16; i++) b[i] = sp[i * P + col];
             inv_dct16(b);
 #pragma unroll
-            for (int i = 0; i < 16; i++) L.sp[i * P + e] = b[i];
+            for (int i = 0; i < 16; i++) sp[i * P + col] = b[i];
           } else {
             float b[32];
 #pragma unroll
-            for (int i = 0; i < 32; i++) b[i] = L.sp[i * P + e];
+            for (int i = 0; i < 32; i++) b[i] = sp[i * P + col];
             inv_dct32(b);
 #pragma unroll
-            for (int i = 0; i < 32; i++) L.sp[i * P + e] = b[i];
+            for (int i = 0; i < 32; i++) sp[i * P + col] = b[i];
           }
         }
         __syncthreads();
-        for (int e = tid; e < H; e += T) {
+        for (int e = tid; e < 3 * H; e += T) {
+          float *sp = L.sp[e / H];
+          const int row = e % H;
           if (W == 8) {
             float b[8];
 #pragma unroll
-            for (int i = 0; i < 8; i++) b[i] = L.sp[e * P + i];
+            for (int i = 0; i < 8; i++) b[i] = sp[row * P + i];
             inv_dct8(b);
 #pragma unroll
-            for (int i = 0; i < 8; i++) L.sp[e * P + i] = b[i];
+            for (int i = 0; i < 8; i++) sp[row * P + i] = b[i];
           } else if (W == 16) {
             float b[16];
 #pragma unroll
-            for (int i = 0; i < 16; i++) b[i] = L.sp[e * P + i];
+            for (int i = 0; i < 16; i++) b[i] = sp[row * P + i];
             inv_dct16(b);
 #pragma unroll
-            for (int i = 0; i < 16; i++) L.sp[e * P + i] = b[i];
+            for (int i = 0; i < 16; i++) sp[row * P + i] = b[i];
           } else {
             float b[32];
 #pragma unroll
-            for (int i = 0; i < 32; i++) b[i] = L.sp[e * P + i];
+            for (int i = 0; i < 32; i++) b[i] = sp[row * P + i];
             inv_dct32(b);
 #pragma unroll
-            for (int i = 0; i < 32; i++) L.sp[e * P + i] = b[i];
+            for (int i = 0; i < 32; i++) sp[row * P + i] = b[i];
           }
         }
       }
       __syncthreads();
-      // SSIM deficit per 8x8 window (ssim_deficit_scalar_kernel, or the
-      // visible variant for footprints past the image edge)
-      if (tid < nwx * nwy) {
-        const int wx = tid % nwx, wy = tid / nwx, x0 = wx * 8, y0 = wy * 8;
-        const int ww = full ? 8 : min(8, vw - x0), wh = full ? 8 : min(8, vh - y0);
-        const float *o = L.pix[c];
-        const float bo = o[y0 * W + x0], br = bo - L.sp[y0 * P + x0];
-        float so = 0.0f, sr = 0.0f;
+    }
+    // SSIM deficit per 8x8 window and channel (ssim_deficit_scalar_kernel,
+    // or the visible variant for footprints past the image edge)
+    const int nw = nwx * nwy;
+    if (tid < 3 * nw) {
+      const int c = tid / nw, wi = tid - c * nw;
+      const int wx = wi % nwx, wy = wi / nwx, x0 = wx * 8, y0 = wy * 8;
+      const int ww = full ? 8 : min(8, vw - x0), wh = full ? 8 : min(8, vh - y0);
+      const float *o = L.pix[c];
+      const float *sp = L.sp[c];
+      const float bo = o[y0 * W + x0], br = bo - sp[y0 * P + x0];
+      float so = 0.0f, sr = 0.0f;
+      if (full) {
+        // whole window: rows of 8 loaded together, summed in row-major order
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+          float ov[8], rv[8];
+#pragma unroll
+          for (int x = 0; x < 8; x++) {
+            ov[x] = o[(y0 + y) * W + x0 + x];
+            rv[x] = ov[x] - sp[(y0 + y) * P + x0 + x];
+          }
+#pragma unroll
+          for (int x = 0; x < 8; x++) {
+            so += ov[x] - bo;
+            sr += rv[x] - br;
+          }
+        }
+      } else {
         for (int y = y0; y < y0 + wh; y++)
           for (int x = x0; x < x0 + ww; x++) {
-            const float ov = o[y * W + x], rv = ov - L.sp[y * P + x];
+            const float ov = o[y * W + x], rv = ov - sp[y * P + x];
             so += ov - bo;
             sr += rv - br;
           }
-        const float count = (float)(ww * wh);
-        const float inv_count = full ? (1.0f / 64.0f) : 1.0f / count;
-        const float mo = bo + so * inv_count, mr = br + sr * inv_count;
-        float vo = 0.0f, vr = 0.0f, cv = 0.0f;
+      }
+      const float count = (float)(ww * wh);
+      const float inv_count = full ? (1.0f / 64.0f) : 1.0f / count;
+      const float mo = bo + so * inv_count, mr = br + sr * inv_count;
+      float vo = 0.0f, vr = 0.0f, cv = 0.0f;
+      if (full) {
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+          float ov[8], rv[8];
+#pragma unroll
+          for (int x = 0; x < 8; x++) {
+            ov[x] = o[(y0 + y) * W + x0 + x];
+            rv[x] = ov[x] - sp[(y0 + y) * P + x0 + x];
+          }
+#pragma unroll
+          for (int x = 0; x < 8; x++) {
+            const float co_ = ov[x] - mo, cr = rv[x] - mr;
+            vo = fmla<F>(co_, co_, vo);
+            vr = fmla<F>(cr, cr, vr);
+            cv = fmla<F>(co_, cr, cv);
+          }
+        }
+      } else {
         for (int y = y0; y < y0 + wh; y++)
           for (int x = x0; x < x0 + ww; x++) {
-            const float ov = o[y * W + x], rv = ov - L.sp[y * P + x];
+            const float ov = o[y * W + x], rv = ov - sp[y * P + x];
             const float co_ = ov - mo, cr = rv - mr;
             vo = fmla<F>(co_, co_, vo);
             vr = fmla<F>(cr, cr, vr);
             cv = fmla<F>(co_, cr, cv);
           }
-        vo *= inv_count;
-        vr *= inv_count;
-        cv *= inv_count;
-        const float lum = (2.0f * mo * mr + 1e-4f) / (mo * mo + mr * mr + 1e-4f);
-        const float st = (2.0f * cv + 9e-4f) / (vo + vr + 9e-4f);
-        L.win[tid] = (1.0f - lum * st) * (full ? 64.0f : count);
       }
-      __syncthreads();
-      if (tid == 0) {
-        float deficit = 0.0f;
-        for (int w = 0; w < nwx * nwy; w++) deficit += L.win[w];
-        L.chan_dist[c] = D->channel_weight[c] * deficit;
-      }
-      __syncthreads();
+      vo *= inv_count;
+      vr *= inv_count;
+      cv *= inv_count;
+      const float lum = (2.0f * mo * mr + 1e-4f) / (mo * mo + mr * mr + 1e-4f);
+      const float st = (2.0f * cv + 9e-4f) / (vo + vr + 9e-4f);
+      L.win[c][wi] = (1.0f - lum * st) * (full ? 64.0f : count);
     }
+    __syncthreads();
+    if (tid < 3) {
+      float deficit = 0.0f;
+      for (int w = 0; w < nw; w++) deficit += L.win[tid][w];
+      L.chan_dist[tid] = D->channel_weight[tid] * deficit;
+    }
+    __syncthreads();
     // recon rate Y, X, B; distortion X, Y, B; rd_cost (src/ac_strategy.rs:1688)
     float rate = 0.0f;
     rate += L.chan_rate[1];
@@ -435,6 +513,22 @@ __device__ __forceinline__ void refine_costs_body(RfLds<BIG ? 1024 : 256, BIG ? 
       A.rcost[(size_t)blk * 4 + j] = costs[j < len ? j : 0];
     }
     A.rlen[blk] = (uint8_t)(len > 1 ? len : 0);
+  }
+}
+
+template <bool F, bool BIG>
+__device__ __forceinline__ void refine_costs_body(RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> &L,
+                                                  const RefineArgs &A, uint32_t item) {
+  const uint32_t blk = A.list[BIG ? 1 : 0][item];
+  if constexpr (BIG) {
+    refine_shape<F, true, JXG_DCT32X32>(L, A, blk);
+  } else {
+    switch (A.cells[blk] >> 1) {
+      case JXG_DCT: refine_shape<F, false, JXG_DCT>(L, A, blk); break;
+      case JXG_DCT16X8: refine_shape<F, false, JXG_DCT16X8>(L, A, blk); break;
+      case JXG_DCT8X16: refine_shape<F, false, JXG_DCT8X16>(L, A, blk); break;
+      default: refine_shape<F, false, JXG_DCT16X16>(L, A, blk); break;
+    }
   }
 }
 

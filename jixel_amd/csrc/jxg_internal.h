@@ -64,9 +64,10 @@ float jxg_f16_to_f32(uint16_t b);
 #endif
 
 // Scratch of the quant-field refinement for nb blocks: per block 4 candidate
-// costs (f32), 4 values and a count (u8); two transform lists (u32) and two
-// counters; the trellis chain (block index u32, value, field, back u8).
-static inline JXG_HD size_t jxg_refine_scratch_bytes(size_t nb) { return 32 * nb + 4 * nb + 128; }
+// costs (f32), 4 values and a count (u8); two transform lists (u32) and their
+// counters; per chain step its block (u32), candidate costs (4 f32), values
+// (4 u8), count, chosen and fixed value and back pointers (u8 each).
+static inline JXG_HD size_t jxg_refine_scratch_bytes(size_t nb) { return 61 * nb + 128; }
 
 static inline JXG_HD int jxg_area_class(int cx, int cy) {
   int a = cx * cy, k = 0;

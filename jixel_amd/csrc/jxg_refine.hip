@@ -550,7 +550,7 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 // ---------------------------------------------------------------------------
 // The QF trellis per DC group
 
-#define TR_T 256
+#define TR_T 1024
 #define TR_WARM 16  // guess steps before a segment (measured: <= 5 reach the exact vector)
 
 struct TrellisArgs {
@@ -559,7 +559,10 @@ struct TrellisArgs {
   int xb, yb;
   const float *rcost;
   const uint8_t *rq, *rlen;
-  uint32_t *chain_blk;  // scratch, xb * yb
+  uint32_t *chain_blk;  // scratch, xb * yb each: the chain's blocks,
+  float4 *rec_c;        // per step: candidate costs,
+  uint32_t *rec_q;      // candidate values (a byte each),
+  uint8_t *rec_len;     // candidate count (1: the fixed value at cost 0)
   uint8_t *chain_q, *chain_fixed, *chain_back;
   const jxg_derived *D;
 };
@@ -599,48 +602,41 @@ struct TrVec {
   int len;
 };
 
-// The candidates of chain step i: a refined transform's states, else its
-// fixed value at cost 0.
-__device__ __forceinline__ void tr_cands(const TrellisArgs &A, size_t base, int i, int *cq, float *cc,
-                                         int &len) {
-  const uint32_t blk = A.chain_blk[base + i];
-  len = A.rlen[blk];
-  if (len > 1) {
-    const uint32_t qq = *(const uint32_t *)(A.rq + (size_t)blk * 4);
-    const float4 c4 = *(const float4 *)(A.rcost + (size_t)blk * 4);
-#pragma unroll
-    for (int k = 0; k < 4; k++) cq[k] = (int)((qq >> (8 * k)) & 255u);
-    cc[0] = c4.x;
-    cc[1] = c4.y;
-    cc[2] = c4.z;
-    cc[3] = c4.w;
-  } else {
-    len = 1;
-    const int q = A.chain_fixed[base + i];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      cq[k] = q;
-      cc[k] = 0.0f;
-    }
-  }
+// The candidates of one chain step.
+struct TrRec {
+  float4 c;
+  uint32_t q;
+  int len;
+};
+
+__device__ __forceinline__ TrRec tr_rec(const TrellisArgs &A, size_t base, int i) {
+  TrRec r;
+  r.c = A.rec_c[base + i];
+  r.q = A.rec_q[base + i];
+  r.len = A.rec_len[base + i];
+  return r;
 }
 
 // The survivor values of step i with arbitrary costs (the costs are set by
 // the caller).
 __device__ __forceinline__ void tr_load(const TrellisArgs &A, size_t base, int i, TrVec &v) {
-  float cc[4];
-  tr_cands(A, base, i, v.q, cc, v.len);
+  const TrRec r = tr_rec(A, base, i);
+#pragma unroll
+  for (int k = 0; k < 4; k++) v.q[k] = (int)((r.q >> (8 * k)) & 255u);
+  v.len = r.len;
 }
 
 // One step of the recursion (:750-786) on v; the back pointers (2 bits per
 // state) are stored when `keep`.  Step 0 prices against first_left.
 template <bool F>
 __device__ __forceinline__ void tr_step(const TrLds &L, const TrellisArgs &A, size_t base, int i,
-                                        int first_left, float lambda_bits, TrVec &v, bool keep) {
+                                        int first_left, float lambda_bits, TrVec &v, bool keep,
+                                        const TrRec &rec) {
   int cq[4];
-  float cc[4];
-  int len;
-  tr_cands(A, base, i, cq, cc, len);
+  const float cc[4] = {rec.c.x, rec.c.y, rec.c.z, rec.c.w};
+  const int len = rec.len;
+#pragma unroll
+  for (int k = 0; k < 4; k++) cq[k] = (int)((rec.q >> (8 * k)) & 255u);
   float cost[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
   uint32_t bk = 0;
 #pragma unroll
@@ -674,6 +670,22 @@ __device__ __forceinline__ void tr_step(const TrLds &L, const TrellisArgs &A, si
   if (keep) A.chain_back[base + i] = (uint8_t)bk;
 }
 
+// Steps [from, to) of the recursion, each step's record loaded one step
+// ahead.
+template <bool F>
+__device__ __forceinline__ void tr_run(const TrLds &L, const TrellisArgs &A, size_t base, int from,
+                                       int to, int first_left, float lambda_bits, TrVec &v,
+                                       bool keep) {
+  if (from >= to) return;
+  TrRec r = tr_rec(A, base, from);
+  for (int i = from; i < to; i++) {
+    TrRec nx = r;
+    if (i + 1 < to) nx = tr_rec(A, base, i + 1);
+    tr_step<F>(L, A, base, i, first_left, lambda_bits, v, keep, r);
+    r = nx;
+  }
+}
+
 template <bool F>
 __device__ __forceinline__ void qf_trellis_body(TrLds &L, const TrellisArgs &A, int gx, int gy) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -682,44 +694,73 @@ __device__ __forceinline__ void qf_trellis_body(TrLds &L, const TrellisArgs &A, 
   if (xs <= 0 || ys <= 0) return;
   const jxg_derived *D = A.D;
   const size_t base = (size_t)y0 * A.xb + (size_t)x0 * ys;  // this group's scratch
-  // the raster chain of first blocks
+  // the raster chain of first blocks: each thread counts a quarter row
+  // (64 cells), a workgroup scan places them in raster order
+  const int parts = ys * 4;
+  int cnt = 0;
+  size_t cell0 = 0;
+  int cw = 0;
+  if (tid < parts) {
+    const int r = tid >> 2, q = tid & 3;
+    cw = max(0, min(64, xs - q * 64));
+    cell0 = (size_t)(y0 + r) * A.xb + x0 + q * 64;
+    for (int x = 0; x < cw; x++) cnt += A.cells[cell0 + x] & 1;
+  }
+  int incl = cnt;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) L.scan[wave] = (uint32_t)incl;
+  __syncthreads();
+  int before = incl - cnt;
   int n = 0;
-  for (int r = 0; r < ys; r++) {
-    const size_t row = (size_t)(y0 + r) * A.xb + x0;
-    const bool first = tid < xs && (A.cells[row + tid] & 1);
-    const uint64_t m = __ballot(first);
-    if (lane == 0) L.scan[wave] = (uint32_t)__popcll(m);
-    __syncthreads();
-    uint32_t before = 0;
-    for (int w = 0; w < wave; w++) before += L.scan[w];
-    before += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    const int total = (int)(L.scan[0] + L.scan[1] + L.scan[2] + L.scan[3]);
-    if (first) {
-      const uint32_t blk = (uint32_t)(row + tid);
-      A.chain_blk[base + n + before] = blk;
-      A.chain_fixed[base + n + before] = A.qf[blk];
+  for (int w = 0; w < TR_T / 64; w++) {
+    const int t = (int)L.scan[w];
+    if (w < wave) before += t;
+    n += t;
+  }
+  if (cnt) {
+    int o = before;
+    for (int x = 0; x < cw; x++) {
+      const size_t c = cell0 + x;
+      if (A.cells[c] & 1) {
+        A.chain_blk[base + o] = (uint32_t)c;
+        A.chain_fixed[base + o] = A.qf[c];
+        o++;
+      }
     }
-    n += total;
-    __syncthreads();
   }
   if (n == 0) return;
-  __threadfence_block();
-  // independent_choice (:695-704)
+  __syncthreads();
+  // per chain step: its candidates (a refined transform's states, else its
+  // fixed value at cost 0) in chain order; independent_choice (:695-704)
   for (int i = tid; i < n; i += TR_T) {
     const uint32_t blk = A.chain_blk[base + i];
     const int ln = A.rlen[blk];
-    uint8_t q = A.chain_fixed[base + i];
+    const uint8_t fq = A.chain_fixed[base + i];
+    uint8_t q = fq;
     if (ln > 1) {
-      int b = 0;
-      float bc = A.rcost[(size_t)blk * 4];
+      const uint32_t qq = *(const uint32_t *)(A.rq + (size_t)blk * 4);
+      const float4 c4 = *(const float4 *)(A.rcost + (size_t)blk * 4);
+      A.rec_c[base + i] = c4;
+      A.rec_q[base + i] = qq;
+      A.rec_len[base + i] = (uint8_t)ln;
+      const float cc[4] = {c4.x, c4.y, c4.z, c4.w};
+      int bi = 0;
+      float bc = cc[0];
       for (int k = 1; k < ln; k++) {
-        const float c = A.rcost[(size_t)blk * 4 + k];
-        if (c < bc) {
-          bc = c;
-          b = k;
+        if (cc[k] < bc) {
+          bc = cc[k];
+          bi = k;
         }
       }
-      q = A.rq[(size_t)blk * 4 + b];
+      q = (uint8_t)((qq >> (8 * bi)) & 255u);
+    } else {
+      A.rec_c[base + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      A.rec_q[base + i] = (uint32_t)fq * 0x01010101u;
+      A.rec_len[base + i] = 1;
     }
     A.chain_q[base + i] = q;
   }
@@ -761,9 +802,9 @@ __device__ __forceinline__ void qf_trellis_body(TrLds &L, const TrellisArgs &A, 
         tr_load(A, base, w0 - 1, v);
         for (int k = 0; k < 4; k++) v.c[k] = 0.0f;
       }
-      for (int i = w0; i < s0; i++) tr_step<F>(L, A, base, i, first_left, lambda_bits, v, false);
+      tr_run<F>(L, A, base, w0, s0, first_left, lambda_bits, v, false);
       for (int k = 0; k < 4; k++) L.sc[tid][k] = v.c[k];
-      for (int i = s0; i < s1; i++) tr_step<F>(L, A, base, i, first_left, lambda_bits, v, true);
+      tr_run<F>(L, A, base, s0, s1, first_left, lambda_bits, v, true);
       for (int k = 0; k < 4; k++) L.ec[tid][k] = v.c[k];
       L.elen[tid] = (uint8_t)v.len;
     }
@@ -790,7 +831,7 @@ __device__ __forceinline__ void qf_trellis_body(TrLds &L, const TrellisArgs &A, 
           v.c[k] = start[k];
           L.sc[tid][k] = start[k];
         }
-        for (int i = s0; i < s1; i++) tr_step<F>(L, A, base, i, first_left, lambda_bits, v, true);
+        tr_run<F>(L, A, base, s0, s1, first_left, lambda_bits, v, true);
         for (int k = 0; k < 4; k++) L.ec[tid][k] = v.c[k];
       }
     }
@@ -808,21 +849,36 @@ __device__ __forceinline__ void qf_trellis_body(TrLds &L, const TrellisArgs &A, 
       L.map[tid] = (uint8_t)map;
     }
     __syncthreads();
-    if (tid == 0) {
-      // the end state: the first minimum (every cost is >= +0 after the floor)
+    // the state at each segment's last step: the suffix compositions
+    // h_t = map_{t+1} o ... o map_last (a doubling scan of 2-bit state maps)
+    // applied to the end state, the first minimum (every cost is >= +0
+    // after the floor)
+    if (own) L.send[tid] = tid + 1 < nseg ? L.map[tid + 1] : (uint8_t)0xE4;  // 0xE4: identity
+    __syncthreads();
+    for (int d = 1; d < nseg; d <<= 1) {
+      uint32_t h = 0;
+      const bool upd = own && tid + d < nseg;
+      if (upd) {
+        const uint32_t f = L.send[tid], g = L.send[tid + d];
+#pragma unroll
+        for (int k = 0; k < 4; k++) h |= ((f >> (2 * ((g >> (2 * k)) & 3u))) & 3u) << (2 * k);
+      }
+      __syncthreads();
+      if (upd) L.send[tid] = (uint8_t)h;
+      __syncthreads();
+    }
+    {
       const int last = nseg - 1, ln = L.elen[last];
       int sb = 0;
       for (int k = 1; k < ln; k++)
         if (L.ec[last][k] < L.ec[last][sb]) sb = k;
-      L.send[last] = (uint8_t)sb;
-      for (int t = last; t > 0; t--) L.send[t - 1] = (uint8_t)((L.map[t] >> (2 * L.send[t])) & 3u);
+      if (own) L.send[tid] = (uint8_t)((L.send[tid] >> (2 * sb)) & 3u);
     }
     __syncthreads();
     if (own) {
       int sstate = L.send[tid];
       for (int i = s1 - 1; i >= s0; i--) {
-        const uint32_t blk = A.chain_blk[base + i];
-        const uint8_t q = A.rlen[blk] > 1 ? A.rq[(size_t)blk * 4 + sstate] : A.chain_fixed[base + i];
+        const uint8_t q = (uint8_t)((A.rec_q[base + i] >> (8 * sstate)) & 255u);
         A.chain_q[base + i] = q;
         sstate = (int)((A.chain_back[base + i] >> (2 * sstate)) & 3u);
       }
@@ -852,22 +908,34 @@ struct RfPtrs {
   float *rcost;
   uint8_t *rq, *rlen;
   uint32_t *list0, *list1, *count, *chain_blk;
-  uint8_t *chain_q, *chain_fixed, *chain_back;
+  float4 *rec_c;
+  uint32_t *rec_q;
+  uint8_t *chain_q, *chain_fixed, *chain_back, *rec_len;
 };
 
+__host__ __device__ inline uint8_t *align16(uint8_t *p) {
+  return (uint8_t *)(((uintptr_t)p + 15) & ~(uintptr_t)15);
+}
+
+// The refinement scratch of nb blocks (jxg_refine_scratch_bytes): per block
+// 4 candidate costs, values and a count; two transform lists and their
+// counters; per chain step its block, candidates (costs, values, count),
+// chosen and fixed value and back pointers.
 __host__ __device__ inline RfPtrs rf_ptrs(uint8_t *base, size_t nb) {
   RfPtrs r;
   r.rcost = (float *)base;
   r.rq = base + 16 * nb;
   r.rlen = r.rq + 4 * nb;
-  uint8_t *a = (uint8_t *)(((uintptr_t)(r.rlen + nb) + 15) & ~(uintptr_t)15);
-  r.list0 = (uint32_t *)a;
+  r.list0 = (uint32_t *)align16(r.rlen + nb);
   r.list1 = r.list0 + nb;
-  r.count = r.list1 + nb;
+  r.count = (uint32_t *)align16((uint8_t *)(r.list1 + nb));
   r.chain_blk = r.count + 4;
-  r.chain_q = (uint8_t *)(r.chain_blk + nb);
+  r.rec_c = (float4 *)align16((uint8_t *)(r.chain_blk + nb));
+  r.rec_q = (uint32_t *)(r.rec_c + nb);
+  r.chain_q = (uint8_t *)(r.rec_q + nb);
   r.chain_fixed = r.chain_q + nb;
   r.chain_back = r.chain_fixed + nb;
+  r.rec_len = r.chain_back + nb;
   return r;
 }
 
@@ -941,6 +1009,9 @@ __host__ __device__ inline TrellisArgs trellis_args(const uint8_t *cells, uint8_
   T.rq = r.rq;
   T.rlen = r.rlen;
   T.chain_blk = r.chain_blk;
+  T.rec_c = r.rec_c;
+  T.rec_q = r.rec_q;
+  T.rec_len = r.rec_len;
   T.chain_q = r.chain_q;
   T.chain_fixed = r.chain_fixed;
   T.chain_back = r.chain_back;

@@ -346,22 +346,18 @@ int jxg_strategy_cost_sites_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t strid
                                 const uint32_t *d_sites, const float *d_qac, float meta_r,
                                 float *d_cost);
 
-/* Transform-layout decision over square merges (DCT8 / DCT16X16 /
- * DCT32X32) and the quant-field aggregation over each merged transform, for
- * speeds ULTRAFAST (every block DCT8, square_merges off), FASTEST and FAST;
- * MEDIUM and slower return JXG_ERR_UNSUPPORTED (their rectangle, sub-8,
- * DCT64, rerank and layout-DP decisions are not built; jxg_strategy_cost_sites_dev
- * gives their candidate costs).  At FASTEST / FAST the reference follows the
- * merges with layout_dp::retile_coefficient (src/ac_strategy/selection.rs:2194,
- * layout_dp.rs:2381) and, at FAST for d >= 1.5, refine_quant_field (:2456,
- * :801); neither runs here, so the cells and field are those of the merge
- * core and adjust_quant_field. */
-/* Transform-layout decision over square merges (DCT8 / DCT16X16 /
- * DCT32X32) and the quant-field aggregation over each merged transform.
- * Replaces the square-merge core of fill_ac_strategy
- * (src/ac_strategy/selection.rs:1000-1180 select_band, :238 select_super_block)
- * and adjust_quant_field (:503).  Writes strategy cells and updates d_qf in
- * place. */
+/* The transform-layout decision and the final quant field of speeds
+ * ULTRAFAST (every block DCT8), FASTEST and FAST; MEDIUM and slower return
+ * JXG_ERR_UNSUPPORTED (their rectangle, sub-8, DCT64 and layout-DP
+ * decisions are not built; jxg_strategy_cost_sites_dev gives their
+ * candidate costs).  Replaces the Fast / Fastest path of fill_ac_strategy:
+ * the square-merge core (src/ac_strategy/selection.rs:1000-1180 select_band,
+ * :238 select_super_block), layout_dp::retile_coefficient (:2194-2208,
+ * selection/layout_dp.rs:2381; the (4, 4)-shifted grid and unaligned pieces
+ * at FAST only), adjust_quant_field (:503) and, at FAST for d >= 1.5,
+ * refine_quant_field with qf_trellis (:534-896).  The SSIM rerank
+ * (d <= 1.709) and leaf-first (d <= 1.5) of FAST are not run.  Writes
+ * strategy cells and updates d_qf in place. */
 int jxg_select_strategy_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
                             uint32_t padded_h, uint32_t width, uint32_t height,
                             uint8_t *d_qf, const int8_t *d_ytox, const int8_t *d_ytob,

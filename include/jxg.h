@@ -263,7 +263,8 @@ enum {
   JXG_PROF_FILL = 9,       /* k_fill_bands alone (inside JXG_PROF_ALPHA) */
   JXG_PROF_RETILE = 10,    /* piece costs, re-tiling passes, field adjustment */
   JXG_PROF_REFINE = 11,    /* quant-field refinement: candidate costs, QF trellis */
-  JXG_PROF_N = 12
+  JXG_PROF_RERANK = 12,    /* SSIM rerank (inside JXG_PROF_RETILE)     */
+  JXG_PROF_N = 13
 };
 #define JXG_PROF_CAP 64
 int jxg_ctx_profile(jxg_ctx *ctx, int enable);

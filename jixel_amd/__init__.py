@@ -236,7 +236,7 @@ class FrontEnd:
         check(lib().jxg_ctx_synchronize(self.ctx))
 
     PROFILE_STAGES = ("xyb", "alpha", "cmap", "aq_pre", "aq_block", "select", "alpha_qf",
-                      "quantize", "quantize_xt", "fill", "retile", "refine")  # JXG_PROF_* order ("fill" is inside "alpha")
+                      "quantize", "quantize_xt", "fill", "retile", "refine", "rerank")  # JXG_PROF_* order ("fill" is inside "alpha")
 
     def profile(self, enable=True):
         """Turns the library's per-kernel event timing on (and resets it) or off."""

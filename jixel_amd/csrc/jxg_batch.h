@@ -29,6 +29,10 @@ enum {
   BS_RCOST_S,
   BS_RCOST_B,
   BS_TRELLIS,
+  BS_RRLIST,
+  BS_RRCOST_S,
+  BS_RRCOST_B,
+  BS_RRDECIDE,
   BS_N
 };
 

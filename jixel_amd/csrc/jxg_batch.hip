@@ -37,7 +37,7 @@ struct jxg_batch {
   uint32_t xt_cap = 0;
   uint32_t *d_counters = nullptr;  // [0] queue count, [1] bad, [4..6] status
   uint32_t total[BS_N] = {};
-  std::vector<uint8_t> stage_on;  // per image: re-tiling, quant-field refinement
+  std::vector<uint8_t> stage_on;  // per image: re-tiling, quant-field refinement, rerank
   int aq_strip = 32, cm_rows = 16, xt_grid = 256;
   hipStream_t stream = nullptr;  // the graph's stream
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
@@ -81,9 +81,11 @@ int upload_layout(jxg_batch *b, const std::vector<BatchImg> &host_img) {
     batch_grids(im.width, im.height, b->geo[2 * i], b->geo[2 * i + 1], b->aq_strip, b->cm_rows, gw,
                 gh);
     // stages this image's tier skips get no workgroups
-    if (!b->stage_on[2 * i]) gw[BS_PIECE] = gw[BS_RETILE] = gw[BS_ADJUST] = 0;
-    if (!b->stage_on[2 * i + 1])
+    if (!b->stage_on[3 * i]) gw[BS_PIECE] = gw[BS_RETILE] = gw[BS_ADJUST] = 0;
+    if (!b->stage_on[3 * i + 1])
       gw[BS_RLIST] = gw[BS_RCOST_S] = gw[BS_RCOST_B] = gw[BS_TRELLIS] = 0;
+    if (!b->stage_on[3 * i + 2])
+      gw[BS_RRLIST] = gw[BS_RRCOST_S] = gw[BS_RRCOST_B] = gw[BS_RRDECIDE] = 0;
     for (int s = 0; s < BS_N; s++) {
       img[k].gw[s] = (int)gw[s];
       pfx[(size_t)s * (n + 1) + k] = (uint32_t)tot[s];
@@ -163,7 +165,7 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
   // derived parameters, arena layout
   std::vector<jxg_derived> der(n);
   b->geo.resize(2 * n);
-  b->stage_on.resize(2 * n);
+  b->stage_on.resize(3 * n);
   size_t arena = 0;
   std::vector<size_t> off_xyb(n), off_pre(n), off_part(n), off_pc(n), off_rf(n);
   uint64_t queue = 0;
@@ -171,8 +173,9 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
     const jxg_batch_image &im = images[i];
     const int rc = jxg_derive(&im.params, &der[i]);
     if (rc) return fail(set_err(rc, "image %u: invalid frame parameters", i));
-    b->stage_on[2 * i] = der[i].retile != 0;
-    b->stage_on[2 * i + 1] = der[i].refine_steps != 0;
+    b->stage_on[3 * i] = der[i].retile != 0;
+    b->stage_on[3 * i + 1] = der[i].refine_steps != 0;
+    b->stage_on[3 * i + 2] = der[i].rerank != 0;
     uint32_t s, ph;
     jxg_plane_geometry(im.width, im.height, &s, &ph);
     b->geo[2 * i] = s;

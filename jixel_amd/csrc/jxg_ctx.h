@@ -178,6 +178,14 @@ JXG_HIDDEN int refine_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uin
                              const uint8_t *cells);
 JXG_HIDDEN int batch_retile(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
                             const uint32_t *d_pfx, int n, const uint32_t total[], int fused);
+JXG_HIDDEN int rerank_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uint32_t stride,
+                             uint32_t ph, uint32_t W, uint32_t H, const uint8_t *qf,
+                             const int8_t *ytox, const int8_t *ytob, const float *tables,
+                             const jxg_frame_params *p, uint8_t *cells);
+JXG_HIDDEN int batch_rerank(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                            const uint32_t *d_pfx, int n, const uint32_t total[], int fused);
+JXG_HIDDEN void rerank_grids(uint32_t W, uint32_t H, uint32_t *g_list, uint32_t *g_small,
+                             uint32_t *g_big, uint32_t *g_decide);
 JXG_HIDDEN int batch_refine(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
                             const uint32_t *d_pfx, int n, const uint32_t total[], int fused);
 JXG_HIDDEN void refine_grids(uint32_t W, uint32_t H, uint32_t *g_list, uint32_t *g0, uint32_t *g1,

@@ -508,4 +508,15 @@ __device__ __forceinline__ float level_f32(float one, float num, float q) {
   return q - num / q;
 }
 
+// aggregate_quant (src/ac_strategy/selection.rs:484-498): the quantizer of
+// a merged footprint from its blocks' maximum and sum.
+__device__ __forceinline__ uint8_t jxg_aggregate_quant(uint8_t max_q, uint32_t sum, int covered,
+                                                       float d) {
+  if (covered < 4) return max_q;
+  const float mixer = fmaxf(1.0f - fmaxf(d - 1.54138f, 0.0f) * 0.56391f, 0.0f);
+  const float mean = (float)sum / (float)covered;
+  const float mixed = (float)max_q * mixer + mean * (1.0f - mixer);
+  return (uint8_t)fminf(fmaxf(mixed + 0.5f, 1.0f), 255.0f);
+}
+
 }  // namespace jxg

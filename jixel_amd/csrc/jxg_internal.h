@@ -29,13 +29,15 @@ typedef struct jxg_derived {
   float thr[3][JXG_AREA_CLASSES][4];
   // MergeTuning at this distance/speed (src/ac_strategy.rs:288-350)
   float bias_16x16, bias_32x32, accept_16, accept_32, risk_k, site_k;
-  int32_t selection_quantizer;  // selection_quantizer::enabled (Fast tiers, 1<=d<=4)
+  int32_t selection_quantizer;  // selection_quantizer::enabled (Fast tiers, 1<=d<=4, SSIMULACRA2)
   int32_t run_selection;        // fill_ac_strategy reaches the chooser
   int32_t fused;
   int32_t retile;  // coefficient re-tiling: 0 off, 1 aligned grid (Fastest), 2 + unaligned
                    // pieces and the (4, 4)-shifted grid (Fast); src/effort.rs:311-349
   int32_t refine_steps;  // refine_quant_field: 0 off, 1 or 2 coarser steps (Fast, d >= 1.5)
   float recon_lambda;    // reconstruction_lambda(distance) (src/ac_strategy.rs:1667)
+  int32_t rerank;        // the SSIM rerank's downgrade pass (Fast, d <= 1.7090991)
+  float rerank_margin;   // MergeTuning::rerank_margin: 1, or 0 without the arbiter's downgrades
   uint32_t qt_off[JXG_QT_NUM_KINDS];  // quant-matrix blob layout
   uint32_t qt_n[JXG_QT_NUM_KINDS];
   float exp2_table[64];          // fast_exp2's 2^((i-32)/64)

@@ -816,11 +816,7 @@ __device__ __forceinline__ float mean_max_mixer(float d) {
 
 __device__ __forceinline__ uint8_t aggregate_quant(uint8_t max_q, uint32_t sum, int covered,
                                                    float d) {
-  if (covered < 4) return max_q;
-  const float mixer = mean_max_mixer(d);
-  const float mean = (float)sum / (float)covered;
-  const float mixed = (float)max_q * mixer + mean * (1.0f - mixer);
-  return (uint8_t)fminf(fmaxf(mixed + 0.5f, 1.0f), 255.0f);
+  return jxg_aggregate_quant(max_q, sum, covered, d);
 }
 
 template <bool F>
@@ -2234,6 +2230,8 @@ JXG_HIDDEN void batch_grids(uint32_t W, uint32_t H, uint32_t stride, uint32_t ph
   gh[BS_ADJUST] = 1;
   refine_grids(W, H, &gw[BS_RLIST], &gw[BS_RCOST_S], &gw[BS_RCOST_B], &gw[BS_TRELLIS]);
   gh[BS_RLIST] = gh[BS_RCOST_S] = gh[BS_RCOST_B] = gh[BS_TRELLIS] = 1;
+  rerank_grids(W, H, &gw[BS_RRLIST], &gw[BS_RRCOST_S], &gw[BS_RRCOST_B], &gw[BS_RRDECIDE]);
+  gh[BS_RRLIST] = gh[BS_RRCOST_S] = gh[BS_RRCOST_B] = gh[BS_RRDECIDE] = 1;
 }
 
 // ---------------------------------------------------------------------------

@@ -370,13 +370,19 @@ int jxg_derive(const jxg_frame_params *p, jxg_derived *o) {
     o->accept_32 = fmla(f, vlq_t, (1.0f - 0.02925f) - o->accept_32, o->accept_32);
   }
   if (d >= 1.5f) o->bias_32x32 *= 0.94f;
-  o->selection_quantizer = (d >= 1.0f && d <= 4.0f && fast) ? 1 : 0;
+  // selection_quantizer::enabled (src/ac_strategy/selection_quantizer.rs:46-66)
+  o->selection_quantizer =
+      (p->arbiter == JXG_ARBITER_SSIMULACRA2 && d >= 1.0f && d <= 4.0f && fast) ? 1 : 0;
   o->run_selection = (d > 0.056713393f && p->speed >= JXG_SPEED_FASTEST) ? 1 : 0;
   // layout_dp_coefficient at Fastest and Fast; layout_dp_unaligned and
   // layout_crossing at Fast (src/effort.rs:311-349)
   o->retile = !o->run_selection ? 0 : (p->speed == JXG_SPEED_FAST ? 2 : (p->speed == JXG_SPEED_FASTEST ? 1 : 0));
   // quant_refine at Fast, quant_refinement_steps (src/ac_strategy/selection.rs:534-542)
   o->refine_steps = (o->run_selection && p->speed == JXG_SPEED_FAST) ? (d < 1.5f ? 0 : (d >= 2.0f ? 2 : 1)) : 0;
+  // SearchScope::rerank without full_rerank (src/ac_strategy.rs:71-73,410-412)
+  // and MergeTuning::rerank_margin (:320,346-348; src/arbiter.rs:50-62)
+  o->rerank = (o->run_selection && p->speed == JXG_SPEED_FAST && d <= 1.7090991223128462f) ? 1 : 0;
+  o->rerank_margin = p->arbiter == JXG_ARBITER_SSIMULACRA2 ? 1.0f : 0.0f;
   {
     // reconstruction_lambda (src/ac_strategy.rs:1667-1686)
     const float recip = 1.0f / (4.0f - 1.0f);

@@ -109,20 +109,27 @@ struct RfLds {
   float quot[3][64];
   float rate[256];
   float chan_rate[3], chan_dist[3];
+  double erased[4];     // per wavefront: the selection quantizer's erased energy
 };
 
-// One listed transform of strategy S (the shape fixed at compile time, so
-// that every index is a shift or a constant).
+// The reconstruction-domain cost of the transform of strategy S whose first
+// block is blk, at each of the len field values qs_list (the shape fixed at
+// compile time, so that every index is a shift or a constant):
+// reconstruction_dist_and_rate then rd_cost(Reconstruction, d, meta, ...)
+// (src/ac_strategy.rs:1538,1691).  With selq the 32-family's quantizer is
+// first scaled by the selection quantizer (reconstruction_strategy_cost_
+// and_base, :774-860).  Every thread returns the costs.
 template <bool F, bool BIG, int S>
-__device__ __forceinline__ void refine_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> &L,
-                                             const RefineArgs &A, uint32_t blk) {
+__device__ __forceinline__ void recon_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> &L,
+                                            const RefineArgs &A, uint32_t blk,
+                                            const int *qs_list, int len, float meta, bool selq,
+                                            float *costs) {
   constexpr int NMAX = BIG ? 1024 : 256, T = BIG ? 256 : 64;
   (void)NMAX;
   const int tid = threadIdx.x, lane = tid & 63;
   const jxg_derived *D = A.D;
   const int bx = (int)(blk % (uint32_t)A.xb), by = (int)(blk / (uint32_t)A.xb);
   constexpr int s = S;
-  const int q0 = A.qf[blk];
   constexpr int cvx = (S == JXG_DCT8X16 || S == JXG_DCT16X16) ? 2 : (S == JXG_DCT32X32 ? 4 : 1);
   constexpr int cvy = (S == JXG_DCT16X8 || S == JXG_DCT16X16) ? 2 : (S == JXG_DCT32X32 ? 4 : 1);
   constexpr int W = cvx * 8, H = cvy * 8, P = W + 1, N = W * H;
@@ -199,28 +206,38 @@ __device__ __forceinline__ void refine_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 
     L.co[2][o] = fmla<F>(-fb, y, L.co[2][o]);
   }
   __syncthreads();
-  // the candidates: the current value, then one or two coarser ones
-  // (find_quant_refinements, src/ac_strategy/selection.rs:556-690)
-  int qs_list[4] = {q0, 0, 0, 0};
-  int len = 1;
-  {
-    const int steps = D->refine_steps;
-    const int cand[3] = {q0 - 1, steps == 2 ? q0 - 2 : q0, q0};
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const int c = cand[k] < 0 ? 0 : cand[k];
-      bool dup = c == 0;
-      for (int m = 0; m < len; m++) dup = dup || qs_list[m] == c;
-      if (!dup) qs_list[len++] = c;
-    }
-  }
   const bool full = px + W <= A.W && py + H <= A.H;
   const int vw = min(W, A.W - px), vh = min(H, A.H - py);
   const int nwx = full ? W / 8 : (vw + 7) / 8, nwy = full ? H / 8 : (vh + 7) / 8;
-  float costs[4];
   constexpr int KPT = NMAX / T;  // coefficients per thread and channel (at most)
   for (int j = 0; j < len; j++) {
-    const float qac = D->scale * (float)qs_list[j];
+    float qac = D->scale * (float)qs_list[j];
+    if constexpr (S == JXG_DCT32X32) {
+      if (selq && D->selection_quantizer) {
+        // selection_quantizer::scale (src/ac_strategy/selection_quantizer.rs:
+        // 82-135): the luma energy the coding quantizer erases outside the
+        // lowest quarter of each axis, in f64 (per-thread terms in storage
+        // order, then the wavefronts' tree sums in order)
+        const float *inv_y = A.tables + qoff + 4 * qn;
+        double energy = 0.0;
+#pragma unroll
+        for (int i = 0; i < KPT; i++) {
+          const int k = tid + T * i;
+          const int y = k >> 5, x = k & 31;
+          if (x < 8 && y < 8) continue;
+          const float l = L.co[1][coef_off(k)];
+          if (fabsf(l * inv_y[k] * qac) < 0.5f) energy += (double)l * (double)l;
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) energy += __shfl_xor(energy, off, 64);
+        if (lane == 0) L.erased[tid >> 6] = energy;
+        __syncthreads();
+        const float erased = (float)(((L.erased[0] + L.erased[1]) + L.erased[2]) + L.erased[3]);
+        const float dd = D->distance;
+        const float t = fminf(fmaxf(erased / (2e-4f * dd * dd) - 1.0f, 0.0f), 1.0f);
+        qac *= 1.0f + 0.6f * t;
+      }
+    }
     if (tid < 3) L.mx[tid] = 0u;
     // quantize Y, then X and B against the luma error
     uint32_t mxl[3] = {0u, 0u, 0u};
@@ -504,10 +521,34 @@ __device__ __forceinline__ void refine_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 
     distortion += L.chan_dist[0];
     distortion += L.chan_dist[1];
     distortion += L.chan_dist[2];
-    costs[j] = fmla<F>(D->recon_lambda, rate + 0.0f, distortion);
+    costs[j] = fmla<F>(D->recon_lambda, rate + meta, distortion);
     __syncthreads();
   }
-  if (tid == 0) {
+}
+
+// One listed transform of the refinement: the current value, then one or
+// two coarser ones (find_quant_refinements, src/ac_strategy/selection.rs:
+// 556-690), priced without the metadata charge.
+template <bool F, bool BIG, int S>
+__device__ __forceinline__ void refine_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> &L,
+                                             const RefineArgs &A, uint32_t blk) {
+  const int q0 = A.qf[blk];
+  int qs_list[4] = {q0, 0, 0, 0};
+  int len = 1;
+  {
+    const int steps = A.D->refine_steps;
+    const int cand[3] = {q0 - 1, steps == 2 ? q0 - 2 : q0, q0};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const int c = cand[k] < 0 ? 0 : cand[k];
+      bool dup = c == 0;
+      for (int m = 0; m < len; m++) dup = dup || qs_list[m] == c;
+      if (!dup) qs_list[len++] = c;
+    }
+  }
+  float costs[4];
+  recon_shape<F, BIG, S>(L, A, blk, qs_list, len, 0.0f, false, costs);
+  if (threadIdx.x == 0) {
     for (int j = 0; j < 4; j++) {
       A.rq[(size_t)blk * 4 + j] = (uint8_t)qs_list[j < len ? j : 0];
       A.rcost[(size_t)blk * 4 + j] = costs[j < len ? j : 0];
@@ -545,6 +586,302 @@ __global__ __launch_bounds__(BIG ? 256 : 64) void k_refine_costs(RefineArgs A) {
 
 __global__ void k_zero_u32(uint32_t *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// The SSIM rerank at the Fast tier (d <= 1.709; rerank_large_transforms and
+// find_rerank_downgrades, src/ac_strategy/selection.rs:2718-3232,3568-3611),
+// between the re-tiling and adjust_quant_field, on the field before the
+// adjustment.  Every merged transform (2 or more blocks) is priced in the
+// reconstruction domain with the rerank's metadata charge, against its
+// DCT8 tiling and, for a 32x32, mixtures with its four aligned 16x16
+// children (best_reconstruction_plan, :2636-2716); where the cheaper of
+// those beats the merge (times MergeTuning::rerank_margin) the merge is
+// split.  At Fast the scope has no rectangles and no fine mosaic, the saved
+// child layouts hold no 1x1 leaf but DCT8, no learned prices exist and the
+// upgrade passes need d >= 2, so these arms are all there is.
+//
+//   k_rerank_list    per block: each merge gets its record and its cost
+//                    jobs (the merge, its DCT8 children in raster order, a
+//                    32x32's 16x16 children), one atomic per wavefront.
+//   k_rerank_costs   one workgroup per job, the refinement's cost kernel
+//                    with the metadata charge and the selection quantizer.
+//   k_rerank_decide  one lane per merge: the plan DP in LDS, the decision,
+//                    the split written into the cells.
+
+#define RR_META_R 2.6465739748323758f
+
+struct RrJob {
+  uint32_t blk;  // first block of the priced transform
+  uint8_t s, q;  // its strategy and the footprint's aggregated field value
+  uint16_t pad;
+};
+
+struct RrMerge {
+  uint32_t blk;
+  uint32_t small;  // first small job: [the merge unless 32x32], DCT8 children, 16x16 children
+  uint32_t big;    // the 32x32's own job in the big list
+  uint32_t s;
+};
+
+struct RerankArgs {
+  RefineArgs A;  // the planes, maps, tables and derived parameters (qf: before adjustment)
+  uint8_t *cells;
+  RrMerge *merges;
+  RrJob *jobs[2];  // small (<= 16x16), big (32x32)
+  float *cost[2];
+  uint32_t *count;  // [0] merges, [1] small jobs, [2] big jobs
+};
+
+__device__ __forceinline__ uint8_t rr_region_quant(const RerankArgs &R, int bx, int by, int w,
+                                                   int h) {
+  uint8_t max_q = 1;
+  uint32_t sum = 0;
+  for (int y = by; y < by + h; y++)
+    for (int x = bx; x < bx + w; x++) {
+      const uint8_t q = R.A.qf[(size_t)y * R.A.xb + x];
+      max_q = q > max_q ? q : max_q;
+      sum += q;
+    }
+  return jxg_aggregate_quant(max_q, sum, w * h, R.A.D->distance);
+}
+
+// Exclusive wavefront prefix of v and the wavefront's total.
+__device__ __forceinline__ uint32_t wave_excl(uint32_t v, uint32_t &total) {
+  const int lane = threadIdx.x & 63;
+  uint32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_up((int)incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  total = (uint32_t)__shfl((int)incl, 63, 64);
+  return incl - v;
+}
+
+__device__ __forceinline__ void rerank_list_body(const RerankArgs &R, int i) {
+  const int xb = R.A.xb, yb = R.A.yb;
+  int s = -1, cxs = 1, cys = 1;
+  uint32_t nm = 0, ns = 0, nbg = 0;
+  if (i < xb * yb) {
+    const uint8_t cell = R.cells[i];
+    if (cell & 1) {
+      s = cell >> 1;
+      cxs = cov_x(s);
+      cys = cov_y(s);
+      // the merged shapes of the Fast layout (the rectangles, DCT64 family
+      // and the rest come only from slower tiers)
+      const bool fast_merge = s == JXG_DCT16X8 || s == JXG_DCT8X16 || s == JXG_DCT16X16 ||
+                              s == JXG_DCT32X32;
+      if (fast_merge) {
+        nm = 1;
+        ns = s == JXG_DCT32X32 ? 20u : 1u + (uint32_t)(cxs * cys);
+        nbg = s == JXG_DCT32X32 ? 1u : 0u;
+      }
+    }
+  }
+  uint32_t tm, ts, tb;
+  const uint32_t em = wave_excl(nm, tm), es = wave_excl(ns, ts), eb = wave_excl(nbg, tb);
+  uint32_t bm = 0, bs = 0, bb = 0;
+  if ((threadIdx.x & 63) == 63) {
+    if (tm) bm = atomicAdd(&R.count[0], tm);
+    if (ts) bs = atomicAdd(&R.count[1], ts);
+    if (tb) bb = atomicAdd(&R.count[2], tb);
+  }
+  bm = (uint32_t)__shfl((int)bm, 63, 64);
+  bs = (uint32_t)__shfl((int)bs, 63, 64);
+  bb = (uint32_t)__shfl((int)bb, 63, 64);
+  if (!nm) return;
+  const int bx = i % xb, by = i / xb;
+  RrMerge m;
+  m.blk = (uint32_t)i;
+  m.small = bs + es;
+  m.big = s == JXG_DCT32X32 ? bb + eb : 0xffffffffu;
+  m.s = (uint32_t)s;
+  R.merges[bm + em] = m;
+  uint32_t k = m.small;
+  auto job = [&](int x, int y, int js, int w, int h) {
+    RrJob j;
+    j.blk = (uint32_t)(y * xb + x);
+    j.s = (uint8_t)js;
+    j.q = rr_region_quant(R, x, y, w, h);
+    j.pad = 0;
+    return j;
+  };
+  if (s == JXG_DCT32X32)
+    R.jobs[1][m.big] = job(bx, by, s, 4, 4);
+  else
+    R.jobs[0][k++] = job(bx, by, s, cxs, cys);
+  for (int iy = 0; iy < cys; iy++)
+    for (int ix = 0; ix < cxs; ix++) R.jobs[0][k++] = job(bx + ix, by + iy, JXG_DCT, 1, 1);
+  if (s == JXG_DCT32X32)
+    for (int iy = 0; iy < 4; iy += 2)
+      for (int ix = 0; ix < 4; ix += 2) R.jobs[0][k++] = job(bx + ix, by + iy, JXG_DCT16X16, 2, 2);
+}
+
+__global__ __launch_bounds__(256) void k_rerank_list(RerankArgs R) {
+  rerank_list_body(R, (int)(blockIdx.x * 256 + threadIdx.x));
+}
+
+template <bool F, bool BIG>
+__device__ __forceinline__ void rerank_cost_body(RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> &L,
+                                                 const RerankArgs &R, uint32_t item) {
+  const RrJob j = R.jobs[BIG ? 1 : 0][item];
+  const int q = j.q;
+  float c[1];
+  if constexpr (BIG) {
+    recon_shape<F, true, JXG_DCT32X32>(L, R.A, j.blk, &q, 1, RR_META_R, true, c);
+  } else {
+    switch (j.s) {
+      case JXG_DCT: recon_shape<F, false, JXG_DCT>(L, R.A, j.blk, &q, 1, RR_META_R, true, c); break;
+      case JXG_DCT16X8: recon_shape<F, false, JXG_DCT16X8>(L, R.A, j.blk, &q, 1, RR_META_R, true, c); break;
+      case JXG_DCT8X16: recon_shape<F, false, JXG_DCT8X16>(L, R.A, j.blk, &q, 1, RR_META_R, true, c); break;
+      default: recon_shape<F, false, JXG_DCT16X16>(L, R.A, j.blk, &q, 1, RR_META_R, true, c); break;
+    }
+  }
+  if (threadIdx.x == 0) R.cost[BIG ? 1 : 0][item] = c[0];
+}
+
+template <bool F, bool BIG>
+__global__ __launch_bounds__(BIG ? 256 : 64) void k_rerank_costs(RerankArgs R) {
+  __shared__ RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> L;
+  const uint32_t n = R.count[BIG ? 2 : 1];
+  for (uint32_t item = blockIdx.x; item < n; item += gridDim.x) {
+    rerank_cost_body<F, BIG>(L, R, item);
+    __syncthreads();
+  }
+}
+
+// best_reconstruction_plan's table for one footprint (at most 4x4 blocks):
+// per (log2 width, log2 height, origin) the plan's cost and how it was
+// formed (0 none, 1 a tile, 2 left + right halves, 3 top + bottom halves).
+struct RrPlan {
+  float cost[9 * 16];
+  uint8_t kind[9 * 16];
+  uint8_t tile[9 * 16];
+};
+
+__device__ __forceinline__ int rr_index(int lx, int ly, int x, int y) {
+  return (ly * 3 + lx) * 16 + y * 4 + x;
+}
+
+// The decision for one merge, in one lane (its plan table in LDS).
+template <bool F>
+__device__ __forceinline__ void rerank_decide_one(RrPlan &P, const RerankArgs &R, uint32_t item) {
+  const jxg_derived *D = R.A.D;
+  const RrMerge m = R.merges[item];
+  const int s = (int)m.s, cxb = cov_x(s), cyb = cov_y(s);
+  const int xb = R.A.xb, bx = (int)(m.blk % (uint32_t)xb), by = (int)(m.blk / (uint32_t)xb);
+  const float j_big = s == JXG_DCT32X32 ? R.cost[1][m.big] : R.cost[0][m.small];
+  const uint32_t c8 = m.small + (s == JXG_DCT32X32 ? 0u : 1u);
+  // the tiles: DCT8 children in raster order, then the 16x16 children
+  int tx[20], ty[20], tw[20];
+  float tc[20];
+  int nt = 0;
+  float j_dct8 = 0.0f;
+  for (int iy = 0; iy < cyb; iy++)
+    for (int ix = 0; ix < cxb; ix++) {
+      const float c = R.cost[0][c8 + iy * cxb + ix];
+      tx[nt] = ix;
+      ty[nt] = iy;
+      tw[nt] = 1;
+      tc[nt++] = c;
+      j_dct8 += c;
+    }
+  if (s == JXG_DCT32X32)
+    for (int k = 0; k < 4; k++) {
+      tx[nt] = 2 * (k & 1);
+      ty[nt] = 2 * (k >> 1);
+      tw[nt] = 2;
+      tc[nt++] = R.cost[0][m.small + 16 + k];
+    }
+  const int lw = cxb == 4 ? 2 : cxb - 1, lh = cyb == 4 ? 2 : cyb - 1;
+  for (int i = 0; i < 9 * 16; i++) {
+    P.cost[i] = INFINITY;
+    P.kind[i] = 0;
+  }
+  for (int i = 0; i < nt; i++) {
+    const int l = tw[i] == 2 ? 1 : 0;
+    const int slot = rr_index(l, l, tx[i], ty[i]);
+    if (tc[i] < P.cost[slot]) {
+      P.cost[slot] = tc[i];
+      P.kind[slot] = 1;
+      P.tile[slot] = (uint8_t)i;
+    }
+  }
+  for (int ly = 0; ly <= lh; ly++) {
+    const int h = 1 << ly;
+    for (int lx = 0; lx <= lw; lx++) {
+      const int w = 1 << lx;
+      for (int y = 0; y < cyb; y += h)
+        for (int x = 0; x < cxb; x += w) {
+          float best = INFINITY;
+          int kind = 0;
+          if (lx > 0) {
+            best = P.cost[rr_index(lx - 1, ly, x, y)] + P.cost[rr_index(lx - 1, ly, x + w / 2, y)];
+            kind = 2;
+          }
+          if (ly > 0) {
+            const float split =
+                P.cost[rr_index(lx, ly - 1, x, y)] + P.cost[rr_index(lx, ly - 1, x, y + h / 2)];
+            if (split < best) {
+              best = split;
+              kind = 3;
+            }
+          }
+          const int slot = rr_index(lx, ly, x, y);
+          if (!(P.cost[slot] < best)) {
+            P.cost[slot] = best;
+            P.kind[slot] = (uint8_t)kind;
+          }
+        }
+    }
+  }
+  const float j_child = P.cost[rr_index(lw, lh, 0, 0)];
+  const bool child_wins = j_child < j_dct8;
+  const float j_alt = child_wins ? j_child : j_dct8;
+  if (!(j_alt < j_big * D->rerank_margin)) return;
+  // the split: the plan's tiles (expanded from the top entry), or DCT8s
+  auto put = [&](int x, int y, int st) {
+    const int cw = cov_x(st), ch = cov_y(st);
+    for (int iy = 0; iy < ch; iy++)
+      for (int ix = 0; ix < cw; ix++)
+        R.cells[(size_t)(by + y + iy) * xb + bx + x + ix] = (uint8_t)((st << 1) | (ix == 0 && iy == 0 ? 1 : 0));
+  };
+  if (!child_wins) {
+    for (int iy = 0; iy < cyb; iy++)
+      for (int ix = 0; ix < cxb; ix++) put(ix, iy, JXG_DCT);
+    return;
+  }
+  int stack[32];
+  int sp = 0;
+  stack[sp++] = rr_index(lw, lh, 0, 0);
+  while (sp > 0) {
+    const int e = stack[--sp];
+    const int lv = e >> 4, ly = lv / 3, lx = lv - ly * 3, y = (e >> 2) & 3, x = e & 3;
+    const int kind = P.kind[e];
+    if (kind == 1) {
+      const int t = P.tile[e];
+      put(tx[t], ty[t], tw[t] == 2 ? JXG_DCT16X16 : JXG_DCT);
+    } else if (kind == 2) {
+      stack[sp++] = rr_index(lx - 1, ly, x, y);
+      stack[sp++] = rr_index(lx - 1, ly, x + (1 << lx) / 2, y);
+    } else if (kind == 3) {
+      stack[sp++] = rr_index(lx, ly - 1, x, y);
+      stack[sp++] = rr_index(lx, ly - 1, x, y + (1 << ly) / 2);
+    }
+  }
+}
+
+// Four waves per workgroup, lane 0 of each deciding one merge at a time.
+template <bool F>
+__global__ __launch_bounds__(256) void k_rerank_decide(RerankArgs R) {
+  __shared__ RrPlan P[4];
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) != 0) return;
+  const uint32_t n = R.count[0];
+  for (uint32_t item = blockIdx.x * 4 + wave; item < n; item += gridDim.x * 4)
+    rerank_decide_one<F>(P[wave], R, item);
 }
 
 // ---------------------------------------------------------------------------
@@ -1072,7 +1409,175 @@ __global__ __launch_bounds__(TR_T) void k_qf_trellis_b(const BatchImg *__restric
   qf_trellis_body<F>(L, T, (int)(bl % (uint32_t)gw), (int)(bl / (uint32_t)gw));
 }
 
+struct RrPtrs {
+  RrMerge *merges;
+  RrJob *jobs0, *jobs1;
+  float *cost0, *cost1;
+  uint32_t *count;
+};
+
+// The rerank's scratch inside the refinement scratch of nb blocks (the
+// refinement runs after it): merges <= nb / 2, small jobs <= 1.5 nb (a pair
+// has 3 jobs), big jobs <= nb / 16.
+__host__ __device__ inline RrPtrs rr_ptrs(uint8_t *base, size_t nb) {
+  RrPtrs r;
+  r.merges = (RrMerge *)align16(base);
+  r.jobs0 = (RrJob *)(r.merges + nb / 2 + 1);
+  r.jobs1 = r.jobs0 + (3 * nb) / 2 + 1;
+  r.cost0 = (float *)(r.jobs1 + nb / 16 + 1);
+  r.cost1 = r.cost0 + (3 * nb) / 2 + 1;
+  r.count = (uint32_t *)align16((uint8_t *)(r.cost1 + nb / 16 + 1));
+  return r;
+}
+
+__host__ __device__ inline void rr_grids(size_t nb, unsigned *g_list, unsigned *g_small,
+                                         unsigned *g_big, unsigned *g_decide) {
+  *g_list = (unsigned)((nb + 255) / 256);
+  const size_t ns = (3 * nb) / 2 + 1, nbg = nb / 16 + 1, nm = nb / 8 + 1;
+  *g_small = (unsigned)(ns < 16384 ? ns : 16384);
+  *g_big = (unsigned)(nbg < 2048 ? nbg : 2048);
+  *g_decide = (unsigned)(nm < 4096 ? nm : 4096);
+}
+
+__host__ __device__ inline RerankArgs rerank_args(const RefineArgs &A, uint8_t *cells,
+                                                  const RrPtrs &r) {
+  RerankArgs R;
+  R.A = A;
+  R.cells = cells;
+  R.merges = r.merges;
+  R.jobs[0] = r.jobs0;
+  R.jobs[1] = r.jobs1;
+  R.cost[0] = r.cost0;
+  R.cost[1] = r.cost1;
+  R.count = r.count;
+  return R;
+}
+
+__device__ inline RerankArgs rerank_args_dev(const BatchImg &I, const uint16_t *scanpos) {
+  return rerank_args(refine_args_dev(I, scanpos), I.cells, rr_ptrs(I.rf, (size_t)I.xb * I.yb));
+}
+
+__global__ void k_rerank_zero_b(const BatchImg *__restrict__ img, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const BatchImg &I = img[i];
+    if (!I.D->rerank) continue;
+    uint32_t *c = rr_ptrs(I.rf, (size_t)I.xb * I.yb).count;
+    c[0] = c[1] = c[2] = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rerank_list_b(const BatchImg *__restrict__ img,
+                                                       const uint32_t *__restrict__ pfx, int n) {
+  const int bi = batch_image(pfx, n, blockIdx.x);
+  const BatchImg &I = img[bi];
+  if (!I.D->rerank) return;
+  rerank_list_body(rerank_args_dev(I, nullptr), (int)((blockIdx.x - pfx[bi]) * 256 + threadIdx.x));
+}
+
+template <bool F, bool BIG>
+__global__ __launch_bounds__(BIG ? 256 : 64) void k_rerank_costs_b(const BatchImg *__restrict__ img,
+                                                                   const uint32_t *__restrict__ pfx,
+                                                                   int n,
+                                                                   const uint16_t *__restrict__ scanpos) {
+  __shared__ RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> L;
+  const int bi = batch_image(pfx, n, blockIdx.x);
+  const BatchImg &I = img[bi];
+  if (!I.D->rerank) return;
+  const RerankArgs R = rerank_args_dev(I, scanpos);
+  const uint32_t first = blockIdx.x - pfx[bi], grid = pfx[bi + 1] - pfx[bi];
+  const uint32_t cnt = R.count[BIG ? 2 : 1];
+  for (uint32_t item = first; item < cnt; item += grid) {
+    rerank_cost_body<F, BIG>(L, R, item);
+    __syncthreads();
+  }
+}
+
+template <bool F>
+__global__ __launch_bounds__(256) void k_rerank_decide_b(const BatchImg *__restrict__ img,
+                                                         const uint32_t *__restrict__ pfx, int n) {
+  __shared__ RrPlan P[4];
+  const int bi = batch_image(pfx, n, blockIdx.x);
+  const BatchImg &I = img[bi];
+  if (!I.D->rerank) return;
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) != 0) return;
+  const RerankArgs R = rerank_args_dev(I, nullptr);
+  const uint32_t first = blockIdx.x - pfx[bi], grid = pfx[bi + 1] - pfx[bi];
+  const uint32_t cnt = R.count[0];
+  for (uint32_t item = first * 4 + wave; item < cnt; item += grid * 4)
+    rerank_decide_one<F>(P[wave], R, item);
+}
+
 }  // namespace
+
+// Host side: the SSIM rerank (Fast, d <= 1.709) after the re-tiling and
+// before adjust_quant_field, on st; qf is the field before the adjustment.
+JXG_HIDDEN int rerank_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uint32_t stride,
+                             uint32_t ph, uint32_t W, uint32_t H, const uint8_t *qf,
+                             const int8_t *ytox, const int8_t *ytob, const float *tables,
+                             const jxg_frame_params *p, uint8_t *cells) {
+  const int xb = (int)((W + 7) / 8), yb = (int)((H + 7) / 8);
+  const size_t nb = (size_t)xb * yb;
+  int rc;
+  if ((rc = ensure(&ctx->d_rf, &ctx->rf_cap, jxg_refine_scratch_bytes(nb)))) return rc;
+  const RrPtrs r = rr_ptrs(ctx->d_rf, nb);
+  const RfPtrs rp = rf_ptrs(ctx->d_rf, nb);
+  const RefineArgs A = refine_args(xyb, (int)stride, (int)ph, (int)W, (int)H, cells, qf, ytox, ytob,
+                                   tables, ctx->d_derived, ctx->d_scanpos, rp);
+  const RerankArgs R = rerank_args(A, cells, r);
+  unsigned gl, gs, gb, gd;
+  rr_grids(nb, &gl, &gs, &gb, &gd);
+  prof_begin(ctx, JXG_PROF_RERANK, st);
+  k_zero_u32<<<1, 64, 0, st>>>(r.count, 3);
+  k_rerank_list<<<gl, 256, 0, st>>>(R);
+  LAUNCH_CHECK("k_rerank_list");
+  if (p->fused) {
+    k_rerank_costs<true, false><<<gs, 64, 0, st>>>(R);
+    k_rerank_costs<true, true><<<gb, 256, 0, st>>>(R);
+    k_rerank_decide<true><<<gd, 256, 0, st>>>(R);
+  } else {
+    k_rerank_costs<false, false><<<gs, 64, 0, st>>>(R);
+    k_rerank_costs<false, true><<<gb, 256, 0, st>>>(R);
+    k_rerank_decide<false><<<gd, 256, 0, st>>>(R);
+  }
+  LAUNCH_CHECK("k_rerank");
+  prof_end(ctx, JXG_PROF_RERANK, st);
+  return JXG_OK;
+}
+
+// Batch: the same stages over every image (images whose tier does not
+// rerank have no workgroups).
+JXG_HIDDEN int batch_rerank(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
+                            const uint32_t *d_pfx, int n, const uint32_t total[], int fused) {
+  const uint32_t *p = d_pfx;
+  const int m = n + 1;
+  if (!total[BS_RRDECIDE]) return JXG_OK;
+  k_rerank_zero_b<<<1, 256, 0, st>>>(d_img, n);
+  k_rerank_list_b<<<total[BS_RRLIST], 256, 0, st>>>(d_img, p + BS_RRLIST * m, n);
+  if (fused) {
+    k_rerank_costs_b<true, false><<<total[BS_RRCOST_S], 64, 0, st>>>(d_img, p + BS_RRCOST_S * m, n, ctx->d_scanpos);
+    k_rerank_costs_b<true, true><<<total[BS_RRCOST_B], 256, 0, st>>>(d_img, p + BS_RRCOST_B * m, n, ctx->d_scanpos);
+    k_rerank_decide_b<true><<<total[BS_RRDECIDE], 256, 0, st>>>(d_img, p + BS_RRDECIDE * m, n);
+  } else {
+    k_rerank_costs_b<false, false><<<total[BS_RRCOST_S], 64, 0, st>>>(d_img, p + BS_RRCOST_S * m, n, ctx->d_scanpos);
+    k_rerank_costs_b<false, true><<<total[BS_RRCOST_B], 256, 0, st>>>(d_img, p + BS_RRCOST_B * m, n, ctx->d_scanpos);
+    k_rerank_decide_b<false><<<total[BS_RRDECIDE], 256, 0, st>>>(d_img, p + BS_RRDECIDE * m, n);
+  }
+  LAUNCH_CHECK("batch rerank");
+  return JXG_OK;
+}
+
+// Workgroups of the rerank stages of one image (the batch grids).
+JXG_HIDDEN void rerank_grids(uint32_t W, uint32_t H, uint32_t *g_list, uint32_t *g_small,
+                             uint32_t *g_big, uint32_t *g_decide) {
+  const size_t nb = (size_t)((W + 7) / 8) * ((H + 7) / 8);
+  unsigned a, b, c, d;
+  rr_grids(nb, &a, &b, &c, &d);
+  *g_list = a;
+  *g_small = b;
+  *g_big = c;
+  *g_decide = d;
+}
 
 // Host side: the refinement after adjust_quant_field (single image), on st.
 JXG_HIDDEN int refine_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uint32_t stride,

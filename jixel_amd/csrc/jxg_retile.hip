@@ -106,11 +106,7 @@ struct PieceArgs {
 
 __device__ __forceinline__ uint8_t rt_aggregate_quant(uint8_t max_q, uint32_t sum, int covered,
                                                       float d) {
-  if (covered < 4) return max_q;
-  const float mixer = fmaxf(1.0f - fmaxf(d - 1.54138f, 0.0f) * 0.56391f, 0.0f);
-  const float mean = (float)sum / (float)covered;
-  const float mixed = (float)max_q * mixer + mean * (1.0f - mixer);
-  return (uint8_t)fminf(fmaxf(mixed + 0.5f, 1.0f), 255.0f);
+  return jxg_aggregate_quant(max_q, sum, covered, d);
 }
 
 // One batch of one shape: sites b * NS .. b * NS + NS - 1 of the region's 16
@@ -848,6 +844,10 @@ JXG_HIDDEN int retile_stages(jxg_ctx *ctx, hipStream_t st, const float *xyb, uin
     k_retile<<<g2, RT_T, 0, st>>>(R);
     LAUNCH_CHECK("k_retile (shifted)");
   }
+  if (ctx->h_derived->rerank) {
+    const int rc = rerank_stages(ctx, st, xyb, stride, ph, W, H, qf, ytox, ytob, tables, p, cells);
+    if (rc) return rc;
+  }
   k_adjust_qf<<<(xb * yb + 255) / 256, 256, 0, st>>>(cells, qf, xb, yb, p->distance);
   LAUNCH_CHECK("k_adjust_qf");
   prof_end(ctx, JXG_PROF_RETILE, st);
@@ -867,6 +867,8 @@ JXG_HIDDEN int batch_retile(jxg_ctx *ctx, hipStream_t st, const BatchImg *d_img,
     k_piece_costs_b<false><<<total[BS_PIECE], 256, 0, st>>>(d_img, p + BS_PIECE * m, n, ctx->d_scanpos);
   k_retile_b<<<total[BS_RETILE], RT_T, 0, st>>>(d_img, p + BS_RETILE * m, n, 0);
   k_retile_b<<<total[BS_RETILE], RT_T, 0, st>>>(d_img, p + BS_RETILE * m, n, 1);
+  const int rc = batch_rerank(ctx, st, d_img, d_pfx, n, total, fused);
+  if (rc) return rc;
   k_adjust_qf_b<<<total[BS_ADJUST], 256, 0, st>>>(d_img, p + BS_ADJUST * m, n);
   LAUNCH_CHECK("batch retile");
   return JXG_OK;

@@ -154,6 +154,18 @@ void jxo_live_masks(const float *leaf, const float *wide, const float *tall, con
 void jxo_retile_pass(const float *xyb, int width, int height, const uint8_t *qf,
                      const int8_t *ytox, const int8_t *ytob, const float *tables,
                      const jxg_frame_params *p, uint8_t *cells, int crossing);
+/* best_reconstruction_plan over nt tiles (x, y, w, h, cost as 5 floats
+ * each) of a width x height footprint: the plan's cost, and per grid cell
+ * (y * 4 + x) the index of the tile whose origin it is (0xFF: none). */
+float jxo_rerank_plan(const float *tile_xywh_cost, int nt, int width, int height,
+                      uint8_t *tiles_out);
+/* Tests only: turn the rerank off (0) or back on (1) in jxo_select_strategy. */
+void jxo_rerank_enable(int on);
+/* The SSIM rerank's downgrade pass (Fast, d <= 1.709) over a given layout
+ * and field (the field before adjust_quant_field), in place. */
+void jxo_rerank_pass(const float *xyb, int width, int height, const uint8_t *qf,
+                     const int8_t *ytox, const int8_t *ytob, const float *tables,
+                     const jxg_frame_params *p, uint8_t *cells);
 /* Square-candidate cost maps (DCT8 per block, DCT16X16 per 2x2 site,
  * DCT32X32 per 4x4 site; NaN where not evaluated). */
 void jxo_strategy_costs(const float *xyb, int width, int height, const uint8_t *qf,

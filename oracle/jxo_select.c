@@ -182,7 +182,7 @@ static float model_bits(uint32_t nzeros, float mag_bits, uint32_t max_scan, int 
  * (src/ac_strategy/selection_quantizer.rs:46-120).  The gate `enabled`
  * holds at the Fast tiers for 1 <= d <= 4 with the SSIMULACRA2 arbiter. */
 static int selection_quantizer_enabled(const jxg_frame_params *p) {
-  if (!(p->distance >= 1.0f && p->distance <= 4.0f)) return 0;
+  if (p->arbiter != JXG_ARBITER_SSIMULACRA2 || !(p->distance >= 1.0f && p->distance <= 4.0f)) return 0;
   return p->speed == JXG_SPEED_FAST || p->speed == JXG_SPEED_FASTEST;
 }
 
@@ -1038,6 +1038,7 @@ static void group_init(group_ctx *g, const float *xyb, int width, int height, in
 }
 
 static void refine_quant_field(group_ctx *g, uint8_t *qf_out);
+static void rerank_large_transforms(group_ctx *g);
 
 /* fill_ac_strategy returns before scoping when the speed has no square
  * merges or the distance takes the DCT8-only shortcut
@@ -1045,6 +1046,18 @@ static void refine_quant_field(group_ctx *g, uint8_t *qf_out);
 static int selection_runs(const jxg_frame_params *p) {
   if (p->distance <= 0.056713393f) return 0;
   return p->speed >= JXG_SPEED_FASTEST;
+}
+
+/* The Fast / Fastest decision of one DC group (fill_ac_strategy's stages,
+ * src/ac_strategy/selection.rs:1917-2481): the merge search, the re-tiling,
+ * the rerank (Fast, d <= 1.709), adjust_quant_field into qf_out, then the
+ * quant-field refinement (Fast, d >= 1.5). */
+static void decide_group(group_ctx *g, uint8_t *qf_out) {
+  select_group(g);
+  retile_coefficient(g, 3);
+  rerank_large_transforms(g);
+  adjust_quant_field(g, qf_out);
+  refine_quant_field(g, qf_out);
 }
 
 void jxo_select_strategy(const float *xyb, int width, int height, uint8_t *qf,
@@ -1061,10 +1074,7 @@ void jxo_select_strategy(const float *xyb, int width, int height, uint8_t *qf,
     for (int gx = 0; gx * 2048 < width; gx++) {
       group_ctx g;
       group_init(&g, xyb, width, height, gx, gy, qf_in, ytox, ytob, tables, p, cells);
-      select_group(&g);
-      retile_coefficient(&g, 3);
-      adjust_quant_field(&g, qf);
-      refine_quant_field(&g, qf);
+      decide_group(&g, qf);
     }
   free(qf_in);
 }
@@ -1426,10 +1436,7 @@ static void *fe_worker(void *arg) {
       int gx = i % gxs, gy = i / gxs;
       group_ctx g;
       group_init(&g, j->xyb, W, H, gx, gy, j->qf_in, j->ytox, j->ytob, j->tables, j->p, j->cells);
-      select_group(&g);
-      retile_coefficient(&g, 3);
-      adjust_quant_field(&g, j->qf);
-      refine_quant_field(&g, j->qf);
+      decide_group(&g, j->qf);
     } else if (j->kind == 4) {
       const int tiles_x = (W + 63) / 64;
       for (int tx = 0; tx < tiles_x; tx++) {
@@ -1534,4 +1541,230 @@ void jxo_front_end(const uint8_t *rgb, size_t row_bytes, int channels, int width
   run_pool(&j, 3, (yb + 7) / 8, threads);
   free(j.xyb);
   pthread_mutex_destroy(&j.mu);
+}
+
+/* ------------------------------------------------------------------ */
+/* The SSIM reconstruction rerank at the Fast tier (scope.rerank:        */
+/* d <= FAST_RERANK_MAX_DISTANCE, src/ac_strategy.rs:71-73,410-412;      */
+/* rerank_large_transforms, src/ac_strategy/selection.rs:3568-3611, and  */
+/* find_rerank_downgrades, :2718-3232).  Every merged transform (2 or    */
+/* more blocks) of the re-tiled layout is priced in the reconstruction   */
+/* domain against its DCT8 tiling and its best subdivision into aligned  */
+/* square children; where either is cheaper the merge is split.  At Fast */
+/* the search scope has no rectangles and no fine mosaic, the saved      */
+/* child layouts hold no 1x1 leaf other than DCT8 (those arms add        */
+/* nothing), no learned prices exist, and the upgrade passes need        */
+/* d >= 2.  The rerank's cached base costs, which the refinement reuses, */
+/* equal the refinement's own evaluation of the same transform and       */
+/* quantizer, so the refinement recomputes them.                         */
+
+#define RERANK_META_R 2.6465739748323758f
+#define FAST_RERANK_MAX_DISTANCE 1.7090991223128462f
+#define NO_CHILD 0xFF
+
+/* reconstruction_strategy_cost_and_base's cost (src/ac_strategy.rs:
+ * 774-1052, unbounded): the selection quantizer's scale on qac, the
+ * reconstruction distortion and rate, rd_cost with the metadata charge. */
+static float rerank_cost(group_ctx *g, int s, int bx, int by, float qac) {
+  static __thread float co[3 * 4096];
+  float cmap[3];
+  cmap_factors(g, bx, by, cmap);
+  const int px = (g->gx0 + bx) * 8, py = (g->gy0 + by) * 8;
+  jxo_recon_coeffs(g->xyb, g->width, g->height, s, px, py, cmap, co);
+  const int kind = jxo_qt_kind(s);
+  const uint32_t n = jxo_qt_size(kind), off = jxo_qt_offset(kind);
+  qac *= selection_scale(s, co + 4096, g->tables + off + 4 * n, qac, g->p, RERANK_META_R);
+  float dist, rate;
+  jxo_recon_cost(g->xyb, g->width, g->height, s, px, py, qac, cmap, co, g->tables, g->p, &dist,
+                 &rate);
+  return jxo_fmla(jxo_reconstruction_lambda(g->p->distance), rate + RERANK_META_R, dist);
+}
+
+typedef struct {
+  int x, y, w, h;
+  float cost;
+} rr_tile;
+
+typedef struct {
+  float cost;
+  uint8_t tiles[16];
+} rr_plan;
+
+static int ilog2i(int v) {
+  int r = 0;
+  while ((1 << (r + 1)) <= v) r++;
+  return r;
+}
+
+static rr_plan rr_join(rr_plan a, rr_plan b) {
+  for (int i = 0; i < 16; i++)
+    if (b.tiles[i] != NO_CHILD) a.tiles[i] = b.tiles[i];
+  a.cost = a.cost + b.cost;
+  return a;
+}
+
+/* best_reconstruction_plan (src/ac_strategy/selection.rs:2636-2716): the
+ * cheapest tiling of a width x height block footprint by aligned
+ * power-of-two tiles, halving horizontally first, then vertically, the
+ * single tile last; strict comparisons keep the first candidate. */
+static rr_plan best_reconstruction_plan(const rr_tile *tiles, int nt, int width, int height) {
+  rr_plan empty;
+  empty.cost = INFINITY;
+  memset(empty.tiles, NO_CHILD, 16);
+  rr_plan plans[9 * 16];
+  for (int i = 0; i < 9 * 16; i++) plans[i] = empty;
+#define RR_INDEX(lx, ly, x, y) (((ly) * 3 + (lx)) * 16 + (y) * 4 + (x))
+  for (int i = 0; i < nt; i++) {
+    const rr_tile t = tiles[i];
+    if ((t.w & (t.w - 1)) || (t.h & (t.h - 1)) || t.w > width || t.h > height || t.x >= width ||
+        t.y >= height || t.x % t.w || t.y % t.h)
+      continue;
+    const int slot = RR_INDEX(ilog2i(t.w), ilog2i(t.h), t.x, t.y);
+    if (t.cost < plans[slot].cost) {
+      plans[slot].cost = t.cost;
+      plans[slot].tiles[t.y * 4 + t.x] = (uint8_t)i;
+    }
+  }
+  const int lw = ilog2i(width), lh = ilog2i(height);
+  for (int ly = 0; ly <= lh; ly++) {
+    const int h = 1 << ly;
+    for (int lx = 0; lx <= lw; lx++) {
+      const int w = 1 << lx;
+      for (int y = 0; y < height; y += h)
+        for (int x = 0; x < width; x += w) {
+          rr_plan best = empty;
+          if (lx > 0)
+            best = rr_join(plans[RR_INDEX(lx - 1, ly, x, y)], plans[RR_INDEX(lx - 1, ly, x + w / 2, y)]);
+          if (ly > 0) {
+            rr_plan split =
+                rr_join(plans[RR_INDEX(lx, ly - 1, x, y)], plans[RR_INDEX(lx, ly - 1, x, y + h / 2)]);
+            if (split.cost < best.cost) best = split;
+          }
+          const int slot = RR_INDEX(lx, ly, x, y);
+          if (plans[slot].cost < best.cost) best = plans[slot];
+          plans[slot] = best;
+        }
+    }
+  }
+  return plans[RR_INDEX(lw, lh, 0, 0)];
+#undef RR_INDEX
+}
+
+typedef struct {
+  int bx, by, cx, cy;
+  uint8_t restore[16];
+} rr_downgrade;
+
+/* The decision for the merge s at (bx, by): 1 and the restore grid when it
+ * is split. */
+static int rerank_merge(group_ctx *g, int s, int bx, int by, float margin, uint8_t restore[16]) {
+  const jxg_frame_params *p = g->p;
+  const int cxb = jxo_covered_x(s), cyb = jxo_covered_y(s);
+  const float j_big = rerank_cost(g, s, bx, by, p->scale * (float)region_quant(g, bx, by, cxb, cyb));
+  rr_tile tiles[24];
+  uint8_t strat[24];
+  int nt = 0;
+  float j_dct8 = 0.0f;
+  for (int iy = 0; iy < cyb; iy++)
+    for (int ix = 0; ix < cxb; ix++) {
+      const float c = rerank_cost(g, JXG_DCT, bx + ix, by + iy,
+                                  p->scale * (float)region_quant(g, bx + ix, by + iy, 1, 1));
+      tiles[nt] = (rr_tile){ix, iy, 1, 1, c};
+      strat[nt++] = JXG_DCT;
+      j_dct8 += c;
+    }
+  /* the children: squares only (no rectangles in this scope), aligned
+   * inside the footprint and smaller than it */
+  for (int iy = 0; iy < cyb; iy++)
+    for (int ix = 0; ix < cxb; ix++) {
+      const int cs = JXG_DCT16X16, ccx = 2, ccy = 2;
+      if (ix + ccx > cxb || iy + ccy > cyb || ix % ccx || iy % ccy || (ccx == cxb && ccy == cyb) ||
+          !can_place(bx + ix, by + iy, cs, g->xsize, g->ysize))
+        continue;
+      const float c = rerank_cost(g, cs, bx + ix, by + iy,
+                                  p->scale * (float)region_quant(g, bx + ix, by + iy, ccx, ccy));
+      tiles[nt] = (rr_tile){ix, iy, ccx, ccy, c};
+      strat[nt++] = (uint8_t)cs;
+    }
+  const rr_plan plan = best_reconstruction_plan(tiles, nt, cxb, cyb);
+  const int child_wins = plan.cost < j_dct8;
+  const float j_alt = child_wins ? plan.cost : j_dct8;
+  if (!(j_alt < j_big * margin)) return 0;
+  memset(restore, NO_CHILD, 16);
+  for (int iy = 0; iy < cyb; iy++)
+    for (int ix = 0; ix < cxb; ix++) {
+      if (child_wins) {
+        const uint8_t t = plan.tiles[iy * 4 + ix];
+        if (t != NO_CHILD) restore[iy * 4 + ix] = strat[t];
+      } else {
+        restore[iy * 4 + ix] = JXG_DCT;
+      }
+    }
+  return 1;
+}
+
+static int rerank_runs(const jxg_frame_params *p) {
+  return p->speed == JXG_SPEED_FAST && p->distance <= FAST_RERANK_MAX_DISTANCE;
+}
+
+static int g_rerank_off; /* tests: the decision without the rerank */
+void jxo_rerank_enable(int on) { g_rerank_off = !on; }
+
+static void rerank_large_transforms(group_ctx *g) {
+  if (g_rerank_off || !rerank_runs(g->p)) return;
+  /* MergeTuning::rerank_margin: 0 without the arbiter's rerank downgrades
+   * (src/ac_strategy.rs:320,346-348; src/arbiter.rs:50-62) */
+  const float margin = g->p->arbiter == JXG_ARBITER_SSIMULACRA2 ? 1.0f : 0.0f;
+  rr_downgrade *dg = (rr_downgrade *)malloc(sizeof(rr_downgrade) * (size_t)g->xsize * g->ysize);
+  int nd = 0;
+  /* every merge is judged on the layout before any split */
+  for (int by = 0; by < g->ysize; by++)
+    for (int bx = 0; bx < g->xsize; bx++) {
+      const uint8_t cell = g->cells[(size_t)(g->gy0 + by) * g->qf_stride + g->gx0 + bx];
+      if (!(cell & 1)) continue;
+      const int s = cell >> 1;
+      if (s == JXG_DCT64X64 || s == JXG_DCT64X32 || s == JXG_DCT32X64) continue; /* recon64, not at Fast */
+      if (jxo_covered_x(s) * jxo_covered_y(s) <= 1) continue;
+      if (rerank_merge(g, s, bx, by, margin, dg[nd].restore)) {
+        dg[nd].bx = bx;
+        dg[nd].by = by;
+        dg[nd].cx = jxo_covered_x(s);
+        dg[nd].cy = jxo_covered_y(s);
+        nd++;
+      }
+    }
+  for (int i = 0; i < nd; i++)
+    for (int iy = 0; iy < dg[i].cy; iy++)
+      for (int ix = 0; ix < dg[i].cx; ix++) {
+        const uint8_t r = dg[i].restore[iy * 4 + ix];
+        if (r != NO_CHILD) set_first(g, dg[i].bx + ix, dg[i].by + iy, r);
+      }
+  free(dg);
+}
+
+/* Test entry points. */
+float jxo_rerank_plan(const float *tile_xywh_cost, int nt, int width, int height,
+                      uint8_t *tiles_out) {
+  rr_tile t[64];
+  for (int i = 0; i < nt && i < 64; i++)
+    t[i] = (rr_tile){(int)tile_xywh_cost[5 * i], (int)tile_xywh_cost[5 * i + 1],
+                     (int)tile_xywh_cost[5 * i + 2], (int)tile_xywh_cost[5 * i + 3],
+                     tile_xywh_cost[5 * i + 4]};
+  rr_plan r = best_reconstruction_plan(t, nt < 64 ? nt : 64, width, height);
+  memcpy(tiles_out, r.tiles, 16);
+  return r.cost;
+}
+
+/* The rerank pass (Fast, d <= 1.709) over a given layout and field, in
+ * place. */
+void jxo_rerank_pass(const float *xyb, int width, int height, const uint8_t *qf,
+                     const int8_t *ytox, const int8_t *ytob, const float *tables,
+                     const jxg_frame_params *p, uint8_t *cells) {
+  jxo_set_fused(p->fused);
+  for (int gy = 0; gy * 2048 < height; gy++)
+    for (int gx = 0; gx * 2048 < width; gx++) {
+      group_ctx g;
+      group_init(&g, xyb, width, height, gx, gy, qf, ytox, ytob, tables, p, cells);
+      rerank_large_transforms(&g);
+    }
 }

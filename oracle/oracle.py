@@ -142,6 +142,11 @@ def _declare(L):
                                  c.POINTER(c.c_float)]
     L.jxo_retile_pass.argtypes = [_f32p, c.c_int, c.c_int, _u8p, _i8p, _i8p, _f32p, c.c_void_p,
                                   _u8p, c.c_int]
+    L.jxo_rerank_enable.argtypes = [c.c_int]
+    L.jxo_rerank_plan.restype = c.c_float
+    L.jxo_rerank_plan.argtypes = [_f32p, c.c_int, c.c_int, c.c_int, _u8p]
+    L.jxo_rerank_pass.argtypes = [_f32p, c.c_int, c.c_int, _u8p, _i8p, _i8p, _f32p, c.c_void_p,
+                                  _u8p]
 
 
 def _p(a, t):
@@ -342,6 +347,33 @@ def retile_pass(xyb, qf, ytox, ytob, tables, params, cells, crossing):
                           _p(np.ascontiguousarray(ytob, np.int8), _i8p),
                           _p(np.ascontiguousarray(tables, np.float32), _f32p),
                           ctypes.byref(params), _p(cells, _u8p), 1 if crossing else 0)
+    return cells
+
+
+def rerank_enable(on):
+    """Tests only: the decision with (True) or without the SSIM rerank."""
+    lib().jxo_rerank_enable(1 if on else 0)
+
+
+def rerank_plan(tiles, width, height):
+    """best_reconstruction_plan: (cost, 16 tile indices by grid cell, 255 none)
+    for tiles [(x, y, w, h, cost), ...]."""
+    arr = np.ascontiguousarray(np.asarray(tiles, np.float32).reshape(-1))
+    out = np.zeros(16, np.uint8)
+    cost = lib().jxo_rerank_plan(_p(arr, _f32p), len(tiles), width, height, _p(out, _u8p))
+    return float(cost), out
+
+
+def rerank_pass(xyb, qf, ytox, ytob, tables, params, cells):
+    """The SSIM rerank's downgrade pass over `cells` (returned updated)."""
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    _, h, w = xyb.shape
+    cells = np.array(cells, np.uint8, copy=True)
+    lib().jxo_rerank_pass(_p(xyb, _f32p), w, h, _p(np.ascontiguousarray(qf, np.uint8), _u8p),
+                          _p(np.ascontiguousarray(ytox, np.int8), _i8p),
+                          _p(np.ascontiguousarray(ytob, np.int8), _i8p),
+                          _p(np.ascontiguousarray(tables, np.float32), _f32p),
+                          ctypes.byref(params), _p(cells, _u8p))
     return cells
 
 

@@ -44,6 +44,7 @@ KERNEL_BYTES_PER_PX = {
     "quantize_xt": 0.0,          # the tile-crossing queue (its bytes are a sliver of quantize's)
     "retile": 12.0 + 1.0 / 64,   # 3 planes in (the piece costs), field and cells rewritten
     "refine": 12.0 + 2.0 / 64,   # 3 planes in, cells in, field rewritten
+    "rerank": 12.0 + 2.0 / 64,   # 3 planes in, field in, cells rewritten (inside "retile")
 }
 # FP32 work of the strategy search per pixel: the DCT8 / 16x16 / 32x32
 # candidates' butterflies (4 log2 N flops per pixel per channel for an N x N
@@ -623,6 +624,19 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in photo_k.items()},
         "roofline": roofline_of(photo_k, (2048, 2048), traffic_stage_ok=False),
     }
+    # the same photo at q90 (d = 1.0): the decision adds the SSIM rerank and
+    # drops the quant-field refinement
+    p90 = J.frame_params(J.EncodeConfig().with_quality(90.0))
+    q90_ms, q90_k = front_end_workload(J, torch, fe, stream, d_photo, p90,
+                                       max(3, args.steps // 4), args.warmup)
+    photo_res["q90"] = {
+        "workload": "photo_like_2048x2048_rgb8_q90",
+        "distance": round(p90.distance, 4),
+        "mp_per_s": round(2048 * 2048 / 1e6 / (q90_ms * 1e-3), 2),
+        "ms_per_step": round(q90_ms, 4),
+        "kernel_ms": {k: round(v, 4) for k, v in q90_k.items()},
+    }
+    del d_photo
     if args.photo_only:
         print(json.dumps({"photo": photo_res}), flush=True)
         fe.close()
@@ -681,7 +695,8 @@ def main():
             "decisions": "square merges DCT8/DCT16X16/DCT32X32, layout_dp::retile_coefficient "
                          "(aligned + (4,4)-shifted grids, unaligned pairs and squares), "
                          "adjust_quant_field, refine_quant_field with qf_trellis "
-                         "(2 steps at d >= 2): the FAST tier's decision at d > 1.709",
+                         "(2 steps at d >= 2); the SSIM rerank runs at d <= 1.709: "
+                         "the FAST tier's whole decision",
             "alpha": "alpha_fringe::clear, fill_invisible, alpha block factors",
             "cfl": "per-tile regression maps (fill_cmap, Fast path)",
             "tables": "DequantMatrices::new_fast tier 3 (SS2 scales, flat-B DCT8 row)",

@@ -355,9 +355,10 @@ int jxg_strategy_cost_sites_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t strid
  * the square-merge core (src/ac_strategy/selection.rs:1000-1180 select_band,
  * :238 select_super_block), layout_dp::retile_coefficient (:2194-2208,
  * selection/layout_dp.rs:2381; the (4, 4)-shifted grid and unaligned pieces
- * at FAST only), adjust_quant_field (:503) and, at FAST for d >= 1.5,
- * refine_quant_field with qf_trellis (:534-896).  The SSIM rerank
- * (d <= 1.709) and leaf-first (d <= 1.5) of FAST are not run.  Writes
+ * at FAST only), at FAST for d <= 1.709 the SSIM rerank (:2221-2330,
+ * 2718-3232, 3568), adjust_quant_field (:503) and, at FAST for d >= 1.5,
+ * refine_quant_field with qf_trellis (:534-896).  select_band_leaf_first
+ * (d <= 1.5) gives the same layout as select_band at these tiers.  Writes
  * strategy cells and updates d_qf in place. */
 int jxg_select_strategy_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride,
                             uint32_t padded_h, uint32_t width, uint32_t height,

@@ -160,7 +160,8 @@ def test_strategy_costs_bit_exact(J, fe, torch, oracle, w, h, distance, fused):
 
 @pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("w,h,distance", [(64, 64, 2.35), (100, 75, 1.0), (256, 256, 2.35),
-                                          (256, 200, 1.2), (2100, 72, 2.35), (130, 94, 6.0)])
+                                          (256, 200, 1.2), (2100, 72, 2.35), (130, 94, 6.0),
+                                          (160, 96, 0.1), (160, 96, 0.05)])
 def test_strategy_selection_bit_exact(J, fe, torch, oracle, w, h, distance, fused):
     log("select", w, h, distance, fused)
     rgb, xyb, p, qf, ytox, ytob = _stage_inputs(J, oracle, w, h, distance, 5, fused, cmap=True)
@@ -173,6 +174,32 @@ def test_strategy_selection_bit_exact(J, fe, torch, oracle, w, h, distance, fuse
     assert same_bits(cells, cells_w), first_mismatch(cells, cells_w)
     assert same_bits(qf_d.cpu().numpy(), qf_w), first_mismatch(qf_d.cpu().numpy(), qf_w)
     log("  strategies", np.bincount(cells_w.reshape(-1) >> 1, minlength=5)[:5])
+
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("distance", [0.1, 0.05])
+def test_front_end_low_distance_bit_exact(J, fe, torch, oracle, distance, fused):
+    """The finest distances: d = 0.1 still runs the decision (with the
+    rerank), d = 0.05 takes the DCT8-only shortcut below 0.056713393
+    (src/ac_strategy.rs:71-79)."""
+    log("front_end low distance", distance, fused)
+    from jixel_amd import synth
+    w, h = 136, 104
+    rgb = synth.photo_like(w, h, 4)
+    p = J.frame_params(J.EncodeConfig(fused=fused).with_distance(distance))
+    want = oracle.front_end(rgb, p, oracle.tables_for(p), threads=2)
+    o = fe.run(to_dev(torch, rgb), w, h, p)
+    for k in ("qf", "cells", "ytox", "ytob"):
+        assert same_bits(o[k].cpu().numpy(), want[k]), (k, first_mismatch(o[k].cpu().numpy(), want[k]))
+    for c in range(3):
+        assert same_bits(o["levels"][c].cpu().numpy(), want["levels"][c]), c
+        assert same_bits(o["coeffs"][c].cpu().numpy(), want["coeffs"][c]), c
+        assert same_bits(o["dc"][c].cpu().numpy(), want["dc"][c]), c
+    cells = want["cells"] >> 1
+    if distance < 0.056713393:
+        assert (cells == 0).all()  # every block DCT8
+    else:
+        assert (cells != 0).any()
 
 
 def test_caller_tables_reach_every_stage(J, torch, oracle):

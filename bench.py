@@ -223,26 +223,29 @@ def time_kernel(fn, stream, reps, alg_bytes):
 
 
 def dct64_forms(fe, xyb, w, h, stream, reps):
-    """DCT64X64 of every 64x64 block of the image in both forms (FP32
-    butterflies; split-bf16 x3 on the matrix cores), each with its HBM and
+    """DCT64X64 of every 64x64 block of the image in its three forms (FP32
+    butterflies; split-bf16 x3 and the three-term split x6 on the matrix
+    cores), each with its HBM and
     matrix-core roofline position and its error against the FP64 closed form
     on a sample of blocks."""
     import math
     import torch
     nblk = ((w + 63) // 64) * ((h + 63) // 64)
     px = 64 * 64 * nblk
-    outs = {m: fe.dct64_image(xyb, w, h, m) for m in (0, 1)}
+    outs = {m: fe.dct64_image(xyb, w, h, m) for m in (0, 1, 2)}
     k = np.arange(64)[:, None]
     cm = np.cos(np.pi * k * (2 * np.arange(64)[None, :] + 1) / 128)
     cm[1:] *= math.sqrt(2.0)
     planes = xyb.cpu().numpy()
     xbk = 8 * ((w + 63) // 64)
     res = {}
-    for m, name in ((0, "dct64_fp32_butterflies"), (1, "dct64_mfma_bf16x3")):
+    for m, name in ((0, "dct64_fp32_butterflies"), (1, "dct64_mfma_bf16x3"),
+                    (2, "dct64_mfma_bf16x6")):
         r = time_kernel(lambda: fe.dct64_image(xyb, w, h, m, out=outs[m]), stream, reps,
                         24.0 * px)
-        flops = 3 * nblk * 2 * 3 * 2 * 64 ** 3  # 3 channels, 2 products, 3 split terms
-        if m == 1:
+        # 3 channels, 2 products of 2 * 64^3 flops, 3 or 6 split terms
+        flops = 3 * nblk * 2 * (3 if m == 1 else 6) * 2 * 64 ** 3
+        if m >= 1:
             r["mfma_tflops"] = round(flops / (r["ms"] * 1e-3) / 1e12, 1)
             r["mfma_frac_bf16_peak"] = round(flops / (r["ms"] * 1e-3) / 2.5e15, 4)
         got = [o.cpu().numpy().reshape(-1, xbk, 64) for o in outs[m]]

@@ -512,12 +512,14 @@ int jxg_quantize_status(jxg_ctx *ctx, uint32_t out[2]);
  * to the quantize stage's DCT64X64); JXG_DCT64_MFMA_BF16X3 computes
  * C X C^T / 4096 on the bf16 matrix cores with split-precision operands
  * (hi + lo bf16, three products, f32 accumulation), an approximation whose
- * error against FP64 the caller measures.  d_out[c] receives
+ * error against FP64 the caller measures; JXG_DCT64_MFMA_BF16X6 splits
+ * every operand in three bf16 terms (exact for an f32) and keeps the six
+ * products above 2^-24 of the leading one.  d_out[c] receives
  * 64 * (8 ceil(W/64)) * (8 ceil(H/64)) floats: the DCT64X64 coefficient
  * storage (u-major) in the chunk layout of jxg_forward_quantize_dev with
  * xblocks = 8 ceil(W/64).  No counterpart entry point in the reference (its
  * DCT64 runs per transform inside write_ac_group, src/group.rs:1534-1640). */
-enum { JXG_DCT64_PLAIN = 0, JXG_DCT64_MFMA_BF16X3 = 1 };
+enum { JXG_DCT64_PLAIN = 0, JXG_DCT64_MFMA_BF16X3 = 1, JXG_DCT64_MFMA_BF16X6 = 2 };
 int jxg_dct64_image_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint32_t padded_h,
                         uint32_t width, uint32_t height, int method, int fused,
                         float *d_out[3]);

@@ -400,7 +400,8 @@ class FrontEnd:
 
     def dct64_image(self, xyb, width, height, method=0, fused=True, out=None):
         """DCT64X64 of every 64x64 block (method 0: FP32 butterflies in the
-        reference's order, 1: split-bf16 x3 on the matrix cores).  Returns 3
+        reference's order, 1: split-bf16 x3 on the matrix cores, 2: the
+        three-term split with six products).  Returns 3
         float tensors of 64 * (8 ceil(W/64)) * (8 ceil(H/64)) coefficients in
         the quantize-stage chunk layout with xblocks = 8 ceil(W/64)."""
         n = 64 * 64 * ((width + 63) // 64) * ((height + 63) // 64)

@@ -77,6 +77,7 @@ struct jxg_ctx {
   int8_t *d_cmap = nullptr;  // CfL maps when the caller does not want them
   size_t cmap_cap = 0;
   void *d_c64 = nullptr;  // split-bf16 DCT64 matrix (hi, lo), built on first use
+  void *d_c64x6 = nullptr;  // the same in three terms (hi, mid, lo)
   void *d_c128 = nullptr;  // split-bf16 DCT128 matrix (hi, lo), built on first use
   float *d_wc128 = nullptr;  // 128-point butterfly constants, built on first use
   uint32_t *d_astat = nullptr;  // alpha-stage counters

@@ -2781,6 +2781,7 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_tmp);
   (void)hipFree(ctx->d_cmap);
   (void)hipFree(ctx->d_c64);
+  (void)hipFree(ctx->d_c64x6);
   if (ctx->prof_ev) {
     for (int i = 0; i < JXG_PROF_N * 2 * JXG_PROF_CAP; i++)
       if (ctx->prof_ev[i]) (void)hipEventDestroy(ctx->prof_ev[i]);

@@ -155,6 +155,99 @@ __global__ __launch_bounds__(256) void k_dct64_mfma(const float *__restrict__ xy
   }
 }
 
+// acc += A(32 x 64) * B(64 x 32) with each operand split in three bf16
+// terms (hi + mid + lo holds an f32 exactly): the six products above 2^-24
+// of the leading one (lo*hi, mid*mid, hi*lo, mid*hi, hi*mid, hi*hi, the
+// smallest first), f32 accumulation.
+__device__ __forceinline__ f32x16 gemm_split3(const __bf16 *a0, const __bf16 *a1,
+                                              const __bf16 *a2, const __bf16 *b0,
+                                              const __bf16 *b1, const __bf16 *b2) {
+  const int l = threadIdx.x & 63;
+  const int r = l & 31, kg = (l >> 5) * 8;
+  f32x16 acc = {};
+#pragma unroll
+  for (int kk = 0; kk < 64; kk += 16) {
+    const int o = r * LP + kk + kg;
+    const bf16x8 ah = lds8(a0 + o), am = lds8(a1 + o), al = lds8(a2 + o);
+    const bf16x8 bh = lds8(b0 + o), bm = lds8(b1 + o), bl = lds8(b2 + o);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bm, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bm, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// f = h + m + l exactly for a finite f32 (8 significand bits each).
+__device__ __forceinline__ void split3(float f, __bf16 &h, __bf16 &m, __bf16 &l) {
+  h = (__bf16)f;
+  const float r1 = f - (float)h;
+  m = (__bf16)r1;
+  l = (__bf16)(r1 - (float)m);
+}
+
+// Y = C X C^T / 4096 with three-term operands (JXG_DCT64_MFMA_BF16X6).  The
+// X tiles are reused for T^T once the first product has read them.
+__global__ __launch_bounds__(256) void k_dct64_mfma6(const float *__restrict__ xyb, int stride,
+                                                     int padded_h, int xb,
+                                                     const __bf16 *__restrict__ cmat,
+                                                     float *out0, float *out1, float *out2) {
+  __shared__ __attribute__((aligned(16))) __bf16 x0s[64 * LP], x1s[64 * LP], x2s[64 * LP];
+  __shared__ __attribute__((aligned(16))) __bf16 c0s[64 * LP], c1s[64 * LP], c2s[64 * LP];
+  const int tid = threadIdx.x, c = blockIdx.z;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 64;
+  const float *src = xyb + (size_t)c * stride * padded_h;
+  for (int e = tid; e < 64 * 16; e += 256) {
+    const int r = e >> 4, q = e & 15;
+    const float4 v = *reinterpret_cast<const float4 *>(src + (size_t)(y0 + r) * stride + x0 + 4 * q);
+    const float f[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int o = r * LP + 4 * q + i;
+      split3(f[i], x0s[o], x1s[o], x2s[o]);
+    }
+  }
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e >> 6, q = e & 63;
+    c0s[r * LP + q] = cmat[e];
+    c1s[r * LP + q] = cmat[4096 + e];
+    c2s[r * LP + q] = cmat[8192 + e];
+  }
+  __syncthreads();
+  const int w = tid >> 6, l = tid & 63;
+  const int qr = (w >> 1) * 32, qc = (w & 1) * 32;
+  // T[y][u] = sum_x X[y][x] C[u][x]
+  f32x16 acc = gemm_split3(x0s + qr * LP, x1s + qr * LP, x2s + qr * LP, c0s + qc * LP,
+                           c1s + qc * LP, c2s + qc * LP);
+  __syncthreads();  // every wave has read X
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int y = qr + (i >> 2) * 8 + (l >> 5) * 4 + (i & 3), u = qc + (l & 31);
+    const int o = u * LP + y;  // T^T: row u, K = y
+    split3(acc[i], x0s[o], x1s[o], x2s[o]);
+  }
+  __syncthreads();
+  // Y[v][u] = sum_y C[v][y] T^T[u][y]
+  acc = gemm_split3(c0s + qr * LP, c1s + qr * LP, c2s + qr * LP, x0s + qc * LP, x1s + qc * LP,
+                    x2s + qc * LP);
+  float *out = c == 0 ? out0 : (c == 1 ? out1 : out2);
+  const int bx = blockIdx.x * 8, by = blockIdx.y * 8;
+  const int u = qc + (l & 31);
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const int v = qr + g * 8 + (l >> 5) * 4;
+    const int k = u * 64 + v;
+    float4 o;
+    o.x = acc[g * 4 + 0] * (1.0f / 4096.0f);
+    o.y = acc[g * 4 + 1] * (1.0f / 4096.0f);
+    o.z = acc[g * 4 + 2] * (1.0f / 4096.0f);
+    o.w = acc[g * 4 + 3] * (1.0f / 4096.0f);
+    *reinterpret_cast<float4 *>(out + out_index(xb, bx, by, k)) = o;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // DCT128X128 (the 128 family's square member) in the same two forms.
 
@@ -356,7 +449,8 @@ int jxg_dct64_image_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint3
   int rc = check_image(width, height);
   if (rc) return rc;
   if ((rc = check_planes(width, height, stride, padded_h))) return rc;
-  if (method != JXG_DCT64_PLAIN && method != JXG_DCT64_MFMA_BF16X3)
+  if (method != JXG_DCT64_PLAIN && method != JXG_DCT64_MFMA_BF16X3 &&
+      method != JXG_DCT64_MFMA_BF16X6)
     return set_err(JXG_ERR_INVALID_ARG, "unknown DCT64 method %d", method);
   HIP_TRY(hipSetDevice(ctx->device));
   const int tx = (int)((width + 63) / 64), ty = (int)((height + 63) / 64);
@@ -370,6 +464,29 @@ int jxg_dct64_image_dev(jxg_ctx *ctx, const float *d_xyb, uint32_t stride, uint3
       k_dct64_plain<false><<<grid, 256, 0, ctx->stream>>>(d_xyb, (int)stride, (int)padded_h, xb,
                                                           d_out[0], d_out[1], d_out[2]);
     LAUNCH_CHECK("k_dct64_plain");
+    return JXG_OK;
+  }
+  if (method == JXG_DCT64_MFMA_BF16X6) {
+    if (!ctx->d_c64x6) {
+      // The scaled DCT-II matrix in FP64, split into bf16 hi + mid + lo.
+      std::vector<uint16_t> h(3 * 4096);
+      for (int k = 0; k < 64; k++)
+        for (int i = 0; i < 64; i++) {
+          const double ck = k == 0 ? 1.0 : sqrt(2.0);
+          double v = ck * cos(M_PI * k * (2 * i + 1) / 128.0);
+          for (int t = 0; t < 3; t++) {
+            const uint16_t b = bf16_bits((float)v);
+            h[t * 4096 + k * 64 + i] = b;
+            v -= (double)bf16_value(b);
+          }
+        }
+      HIP_TRY(hipMalloc((void **)&ctx->d_c64x6, h.size() * sizeof(uint16_t)));
+      HIP_TRY(hipMemcpy(ctx->d_c64x6, h.data(), h.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    }
+    k_dct64_mfma6<<<grid, 256, 0, ctx->stream>>>(d_xyb, (int)stride, (int)padded_h, xb,
+                                                 (const __bf16 *)ctx->d_c64x6, d_out[0], d_out[1],
+                                                 d_out[2]);
+    LAUNCH_CHECK("k_dct64_mfma6");
     return JXG_OK;
   }
   if (!ctx->d_c64) {

@@ -94,6 +94,7 @@ def test_large_transforms_full_size_properties(J, fe, torch, rgba, n):
     log(f"DCT{n}X{n} of the {W}x{H} planes, plain and split-bf16")
     plain = run(xyb, W, H, 0)
     mfma = run(xyb, W, H, 1)
+    mfma6 = run(xyb, W, H, 2) if n == 64 else None
     torch.cuda.synchronize()
     for c in range(3):
         x = xyb[c, :ty * n, :tx * n].double().view(ty, n, tx, n).permute(0, 2, 1, 3)
@@ -117,3 +118,10 @@ def test_large_transforms_full_size_properties(J, fe, torch, rgba, n):
         diff = ((m - p).abs().amax(dim=2) / scale).max().item()
         log(f"  c{c} split-bf16 vs plain {diff:.2e}")
         assert diff <= 3e-5
+        if n == 64:
+            # the three-term split with six products
+            m6 = blocks(torch, mfma6[c], n, ty, tx).double()
+            d6 = ((m6 - p).abs().amax(dim=2) / scale).max().item()
+            pars6 = ((n * n * (m6 * m6).sum(dim=2) - energy).abs() / denom).max().item()
+            log(f"  c{c} split-bf16x6 vs plain {d6:.2e}, Parseval {pars6:.2e}")
+            assert d6 <= 1e-5 and pars6 <= 1e-5

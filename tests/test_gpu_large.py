@@ -1,7 +1,8 @@
 """DCT64X64 over whole images: the plain FP32 butterfly form against the
-oracle bit for bit, and the split-bf16 x3 matrix-core form against the FP64
-closed form Y = C X C^T / 4096 (stated tolerance 2e-5 of the block's largest
-coefficient; the measured error is printed)."""
+oracle bit for bit, and the split-bf16 matrix-core forms against the FP64
+closed form Y = C X C^T / 4096 (stated tolerance of the block's largest
+coefficient: 2e-5 for the two-term split with three products, 1e-5 for the
+three-term split with six; the measured errors are printed)."""
 import math
 
 import numpy as np
@@ -80,6 +81,40 @@ def test_dct64_mfma_split_bf16_error(J, fe, torch, oracle, w, h):
         f"FP32 butterflies {worst_plain:.3e}")
     assert worst <= 2e-5
     assert worst_plain <= 1e-5
+
+
+@pytest.mark.parametrize("w,h,content", [(64, 64, "photo"), (256, 192, "photo"),
+                                         (320, 256, "smooth")])
+def test_dct64_mfma_split3_bf16x6_error(J, fe, torch, oracle, w, h, content):
+    """The three-term split with six products: within 1e-5 of the block's
+    largest coefficient against FP64 (the two-term form sits near 2e-5 on
+    blocks with strong AC content)."""
+    if content == "photo":
+        xyb = image(w, h, 6)
+    else:
+        from jixel_amd.synth import smooth_rgba
+        from oracle import oracle as O
+        rgb = np.ascontiguousarray(smooth_rgba(w, h, seed=3)[..., :3])
+        xyb = O.xyb_image(rgb, J_mod().XYB_SPEC_FWD)
+    pad = padded(J, xyb)
+    out = [o.cpu().numpy() for o in fe.dct64_image(torch.from_numpy(pad).cuda(), w, h, 2)]
+    x3 = [o.cpu().numpy() for o in fe.dct64_image(torch.from_numpy(pad).cuda(), w, h, 1)]
+    Cm = C64()
+    worst = worst3 = 0.0
+    for ty in range((h + 63) // 64):
+        for tx in range((w + 63) // 64):
+            for c in range(3):
+                x = pad[c, ty * 64:ty * 64 + 64, tx * 64:tx * 64 + 64].astype(np.float64)
+                want = (Cm @ x @ Cm.T / 4096.0).T.reshape(-1)
+                scale = np.abs(want).max()
+                got = block_coeffs(out[c], w, h, tx, ty).astype(np.float64)
+                g3 = block_coeffs(x3[c], w, h, tx, ty).astype(np.float64)
+                worst = max(worst, np.abs(got - want).max() / scale)
+                worst3 = max(worst3, np.abs(g3 - want).max() / scale)
+    log(f"dct64 {content} {w}x{h}: max error / max |coef| vs FP64: bf16x6 {worst:.3e}, "
+        f"bf16x3 {worst3:.3e}")
+    assert worst <= 1e-5
+    assert worst <= worst3
 
 
 def padded128(xyb):

@@ -173,6 +173,7 @@ int jxg_batch_create(jxg_ctx *ctx, uint32_t n, const jxg_batch_image *images, jx
     const jxg_batch_image &im = images[i];
     const int rc = jxg_derive(&im.params, &der[i]);
     if (rc) return fail(set_err(rc, "image %u: invalid frame parameters", i));
+    der[i].rate_big = ctx->d_rate_big;
     b->stage_on[3 * i] = der[i].retile != 0;
     b->stage_on[3 * i + 1] = der[i].refine_steps != 0;
     b->stage_on[3 * i + 2] = der[i].rerank != 0;

@@ -78,6 +78,7 @@ struct jxg_ctx {
   size_t cmap_cap = 0;
   void *d_c64 = nullptr;  // split-bf16 DCT64 matrix (hi, lo), built on first use
   void *d_c64x6 = nullptr;  // the same in three terms (hi, mid, lo)
+  float *d_rate_big = nullptr;  // log2f(1 + i), i < JXG_RATE_BIG_N, built at creation
   void *d_c128 = nullptr;  // split-bf16 DCT128 matrix (hi, lo), built on first use
   float *d_wc128 = nullptr;  // 128-point butterfly constants, built on first use
   uint32_t *d_astat = nullptr;  // alpha-stage counters

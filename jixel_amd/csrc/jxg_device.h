@@ -519,4 +519,14 @@ __device__ __forceinline__ uint8_t jxg_aggregate_quant(uint8_t max_q, uint32_t s
   return (uint8_t)fminf(fmaxf(mixed + 0.5f, 1.0f), 255.0f);
 }
 
+// rate_log2 (src/inflated_cost.rs:52-60) of a level magnitude aq (an
+// integer-valued float): the small table, the context's large table, then
+// the device log2f beyond both.
+__device__ __forceinline__ float jxg_rate_log2(const jxg_derived *D, float aq) {
+  const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
+  if (ki < JXG_RATE_LUT_N) return D->rate_lut[ki];
+  if (ki < JXG_RATE_BIG_N && D->rate_big) return D->rate_big[ki];
+  return log2f(1.0f + aq);
+}
+
 }  // namespace jxg

@@ -9,6 +9,10 @@
 #include "../../include/jxg.h"
 
 #define JXG_RATE_LUT_N 1024
+// log2f(1 + i) for the large levels, computed on the host by the C library's
+// log2f (the function the reference's f32::log2 calls), so that no device
+// log2f rounding enters a rate below 65536.
+#define JXG_RATE_BIG_N 65536
 #define JXG_AREA_CLASSES 7 /* cx*cy in {1,2,4,8,16,32,64} */
 
 // Per-frame constants derived once on the host, in the reference's float
@@ -42,6 +46,7 @@ typedef struct jxg_derived {
   uint32_t qt_n[JXG_QT_NUM_KINDS];
   float exp2_table[64];          // fast_exp2's 2^((i-32)/64)
   float rate_lut[JXG_RATE_LUT_N];  // log2(1+i) (src/inflated_cost.rs:57)
+  const float *rate_big;  // device table of log2f(1+i), i < JXG_RATE_BIG_N (set by the context)
 } jxg_derived;
 
 #ifdef __cplusplus

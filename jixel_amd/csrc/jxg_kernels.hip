@@ -545,8 +545,7 @@ __device__ __forceinline__ float level_f32_tab(float one, float num, float q, co
 }
 
 __device__ __forceinline__ float rate_log2_big(const jxg_derived *D, float aq) {
-  const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
-  return ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
+  return jxg_rate_log2(D, aq);
 }
 
 __device__ __noinline__ float rate_log2_slow(const jxg_derived *D, float aq) { return rate_log2_big(D, aq); }
@@ -770,7 +769,7 @@ __device__ void eval_squares(SelLds &L, const float *qac_in, float cmx, float cm
       const float nzf = (float)nzc;
       // L.rate holds rate_lut[0..255]: the common counts read LDS
       const float header =
-          0.4f * (nzc < 256u ? L.rate[nzc] : (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf)));
+          0.4f * (nzc < 256u ? L.rate[nzc] : jxg_rate_log2(D, nzf));
       float visited = 0.0f;
       if (nzc != 0) {
         // maxscan holds 1 + the last nonzero's scan position
@@ -2438,7 +2437,7 @@ __global__ __launch_bounds__(256) void k_site_cost(SiteArgs S) {
       d_total += D->channel_weight[c] * sums[c][0];
       const uint32_t nzc = nz[c];
       const float nzf = (float)nzc;
-      const float header = 0.4f * (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf));
+      const float header = 0.4f * jxg_rate_log2(D, nzf);
       float visited = 0.0f;
       if (nzc != 0) {  // mx holds 1 + the last nonzero's scan position
         const int64_t span = (int64_t)mx[c] - (int64_t)(cx * cy + nzc);
@@ -2597,6 +2596,7 @@ int upload_derived(jxg_ctx *ctx, const jxg_frame_params *p) {
   const int rc = jxg_derive(p, &tmp);
   if (rc != JXG_OK) return set_err(rc, "invalid frame parameters (distance %g)", (double)p->distance);
   HIP_TRY(hipStreamSynchronize(ctx->stream));  // staging buffer may still be in flight
+  tmp.rate_big = ctx->d_rate_big;
   *ctx->h_derived = tmp;
   HIP_TRY(hipMemcpyAsync(ctx->d_derived, ctx->h_derived, sizeof(jxg_derived), hipMemcpyHostToDevice,
                          ctx->stream));
@@ -2663,8 +2663,16 @@ int jxg_ctx_create(int device, void *stream, jxg_ctx **out) {
       hipMalloc((void **)&ctx->d_mlut, (9 + 256) * sizeof(float)) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_mlut, (9 + 256) * sizeof(float), hipHostMallocDefault) != hipSuccess ||
       hipMalloc((void **)&ctx->d_misc, 16 * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc((void **)&ctx->d_exp2, 64 * sizeof(float)) != hipSuccess)
+      hipMalloc((void **)&ctx->d_exp2, 64 * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&ctx->d_rate_big, JXG_RATE_BIG_N * sizeof(float)) != hipSuccess)
     return fail(set_err(JXG_ERR_ALLOC, "context allocation failed"));
+  {
+    std::vector<float> big(JXG_RATE_BIG_N);
+    for (int i = 0; i < JXG_RATE_BIG_N; i++) big[i] = log2f(1.0f + (float)i);
+    if (hipMemcpy(ctx->d_rate_big, big.data(), big.size() * sizeof(float), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return fail(set_err(JXG_ERR_DEVICE, "rate table upload failed"));
+  }
   if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess)
@@ -2782,6 +2790,7 @@ void jxg_ctx_destroy(jxg_ctx *ctx) {
   (void)hipFree(ctx->d_cmap);
   (void)hipFree(ctx->d_c64);
   (void)hipFree(ctx->d_c64x6);
+  (void)hipFree(ctx->d_rate_big);
   if (ctx->prof_ev) {
     for (int i = 0; i < JXG_PROF_N * 2 * JXG_PROF_CAP; i++)
       if (ctx->prof_ev[i]) (void)hipEventDestroy(ctx->prof_ev[i]);

@@ -278,7 +278,7 @@ __device__ __forceinline__ void recon_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 :
                 r = L.rate[(int)aq];
               else {
                 const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
-                r = ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
+                r = jxg_rate_log2(D, aq);
               }
             }
           }
@@ -316,7 +316,7 @@ __device__ __forceinline__ void recon_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 :
         const uint32_t mx = L.mx[c];
         const float nzf = (float)nzc;
         const float header =
-            0.4f * (nzc < 256u ? L.rate[nzc] : (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf)));
+            0.4f * (nzc < 256u ? L.rate[nzc] : jxg_rate_log2(D, nzf));
         float visited = 0.0f;
         if (nzc != 0) {
           const int64_t span = (int64_t)mx - (int64_t)(cx * cy + nzc);

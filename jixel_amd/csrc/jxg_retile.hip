@@ -226,7 +226,7 @@ __device__ __forceinline__ void piece_batch(PcLds &L, const PieceArgs &A, int b,
           rt = L.rate[(int)aq];
         else {
           const uint32_t ki = aq >= 4294967295.0f ? 0xffffffffu : (uint32_t)aq;
-          rt = ki < JXG_RATE_LUT_N ? D->rate_lut[ki] : log2f(1.0f + aq);
+          rt = jxg_rate_log2(D, aq);
         }
       }
       L.mg[e] = rt;
@@ -273,7 +273,7 @@ __device__ __forceinline__ void piece_batch(PcLds &L, const PieceArgs &A, int b,
       // model_bits (src/inflated_cost.rs:540-546)
       const float nzf = (float)nzc;
       const float header =
-          0.4f * (nzc < 256u ? L.rate[nzc] : (nzc < JXG_RATE_LUT_N ? D->rate_lut[nzc] : log2f(1.0f + nzf)));
+          0.4f * (nzc < 256u ? L.rate[nzc] : jxg_rate_log2(D, nzf));
       float visited = 0.0f;
       if (nzc != 0) {
         const int64_t span = (int64_t)mx - (int64_t)(CX * CY + nzc);

@@ -176,16 +176,31 @@ def test_strategy_selection_bit_exact(J, fe, torch, oracle, w, h, distance, fuse
     log("  strategies", np.bincount(cells_w.reshape(-1) >> 1, minlength=5)[:5])
 
 
+def high_contrast(w, h):
+    """8-px and 1-px black / white checkerboards and a saturated red block:
+    levels above 1024 at d = 0.1, so the rate of the decision's candidates
+    takes the large-level log2 table."""
+    y, x = np.mgrid[0:h, 0:w]
+    img = np.zeros((h, w, 3), np.uint8)
+    img[((x // 8 + y // 8) % 2) == 0] = 255
+    img[:, 70:] = np.where(((x[:, 70:] + y[:, 70:]) % 2)[..., None] == 0, 255, 0)
+    img[50:, :40, 0] = 255
+    img[50:, :40, 1:] = 0
+    return img
+
+
 @pytest.mark.parametrize("fused", [True, False])
-@pytest.mark.parametrize("distance", [0.1, 0.05])
-def test_front_end_low_distance_bit_exact(J, fe, torch, oracle, distance, fused):
+@pytest.mark.parametrize("distance,content", [(0.1, "photo"), (0.05, "photo"), (0.1, "contrast"),
+                                              (0.05, "contrast")])
+def test_front_end_low_distance_bit_exact(J, fe, torch, oracle, distance, content, fused):
     """The finest distances: d = 0.1 still runs the decision (with the
     rerank), d = 0.05 takes the DCT8-only shortcut below 0.056713393
-    (src/ac_strategy.rs:71-79)."""
-    log("front_end low distance", distance, fused)
+    (src/ac_strategy.rs:71-79); high-contrast content reaches levels above
+    1024."""
+    log("front_end low distance", distance, content, fused)
     from jixel_amd import synth
     w, h = 136, 104
-    rgb = synth.photo_like(w, h, 4)
+    rgb = synth.photo_like(w, h, 4) if content == "photo" else high_contrast(w, h)
     p = J.frame_params(J.EncodeConfig(fused=fused).with_distance(distance))
     want = oracle.front_end(rgb, p, oracle.tables_for(p), threads=2)
     o = fe.run(to_dev(torch, rgb), w, h, p)

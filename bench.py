@@ -91,7 +91,7 @@ CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 HEADLINE = "smooth_rgba_7680x4320_q75"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_smooth_rgba_7680x4320_q75.txt")
-ISSUE_FILE = os.path.join(ROOT, "profiles", "issue_select_quantize_7680x4320_q75.txt")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "issue_decision_quantize_7680x4320_q75.txt")
 
 
 def read_traffic(stage):

@@ -571,8 +571,10 @@ def roofline_of(kernel_ms, px_shape, traffic_stage_ok=True):
                   "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4),
                   "flops_per_px": round(fpp, 1), "hbm": hbm})
         r["limiter"] = "VALU issue, not HBM: the candidate transforms and RD sums run in LDS"
-        if dominant == "select":
-            r["valu_busy"] = read_valu_busy("k_select") if traffic_stage_ok else None
+        # the stage's main kernel in the committed issue-counter summary
+        main = {"select": "k_select", "retile": "k_piece_costs", "refine": "k_refine_costs"}
+        r["valu_busy"] = read_valu_busy(main[dominant]) if traffic_stage_ok else None
+        r["valu_busy_kernel"] = main[dominant]
     return r
 
 

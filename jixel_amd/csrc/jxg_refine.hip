@@ -99,7 +99,9 @@ __global__ __launch_bounds__(256) void k_refine_list(RefineArgs A) {
 // Buffers of one transform (NMAX coefficients, in-place transform pitch W + 1).
 template <int NMAX, int NPITCH>
 struct RfLds {
-  float pix[3][NMAX];   // source pixels, row-major (pitch W)
+  // source pixels, row-major (pitch W); the 32x32 kernels read the planes
+  // instead (L2-resident), which leaves LDS for four workgroups per CU
+  float pix[3][NMAX == 1024 ? 1 : NMAX];
   float co[3][NPITCH];  // coefficients after CfL, in-place layout (pitch W + 1)
   float err[3][NMAX];   // quantizer errors, storage order
   float sp[3][NPITCH];  // per channel: the nonzero levels' rate terms, then the spatial error
@@ -169,7 +171,7 @@ __device__ __forceinline__ void recon_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 :
   for (int e = tid; e < 3 * N; e += T) {
     const int c = e / N, rem = e - c * N, r = rem / W, col = rem - r * W;
     const float v = A.xyb[c * plane + (size_t)(py + r) * A.stride + px + col];
-    L.pix[c][rem] = v;
+    if constexpr (!BIG) L.pix[c][rem] = v;
     L.co[c][r * P + col] = v;
   }
   __syncthreads();
@@ -439,9 +441,11 @@ __device__ __forceinline__ void recon_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 :
       const int c = tid / nw, wi = tid - c * nw;
       const int wx = wi % nwx, wy = wi / nwx, x0 = wx * 8, y0 = wy * 8;
       const int ww = full ? 8 : min(8, vw - x0), wh = full ? 8 : min(8, vh - y0);
-      const float *o = L.pix[c];
+      // the source pixels: from the (L2-resident) planes for 32x32, else LDS
+      const float *o = BIG ? A.xyb + c * plane + (size_t)py * A.stride + px : L.pix[c];
+      const size_t ostride = BIG ? (size_t)A.stride : (size_t)W;
       const float *sp = L.sp[c];
-      const float bo = o[y0 * W + x0], br = bo - sp[y0 * P + x0];
+      const float bo = o[(size_t)y0 * ostride + x0], br = bo - sp[y0 * P + x0];
       float so = 0.0f, sr = 0.0f;
       if (full) {
         // whole window: rows of 8 loaded together, summed in row-major order
@@ -450,7 +454,7 @@ __device__ __forceinline__ void recon_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 :
           float ov[8], rv[8];
 #pragma unroll
           for (int x = 0; x < 8; x++) {
-            ov[x] = o[(y0 + y) * W + x0 + x];
+            ov[x] = o[(size_t)(y0 + y) * ostride + x0 + x];
             rv[x] = ov[x] - sp[(y0 + y) * P + x0 + x];
           }
 #pragma unroll
@@ -462,7 +466,7 @@ __device__ __forceinline__ void recon_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 :
       } else {
         for (int y = y0; y < y0 + wh; y++)
           for (int x = x0; x < x0 + ww; x++) {
-            const float ov = o[y * W + x], rv = ov - sp[y * P + x];
+            const float ov = o[(size_t)y * ostride + x], rv = ov - sp[y * P + x];
             so += ov - bo;
             sr += rv - br;
           }
@@ -477,7 +481,7 @@ __device__ __forceinline__ void recon_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 :
           float ov[8], rv[8];
 #pragma unroll
           for (int x = 0; x < 8; x++) {
-            ov[x] = o[(y0 + y) * W + x0 + x];
+            ov[x] = o[(size_t)(y0 + y) * ostride + x0 + x];
             rv[x] = ov[x] - sp[(y0 + y) * P + x0 + x];
           }
 #pragma unroll
@@ -491,7 +495,7 @@ __device__ __forceinline__ void recon_shape(RfLds<BIG ? 1024 : 256, BIG ? 1056 :
       } else {
         for (int y = y0; y < y0 + wh; y++)
           for (int x = x0; x < x0 + ww; x++) {
-            const float ov = o[y * W + x], rv = ov - sp[y * P + x];
+            const float ov = o[(size_t)y * ostride + x], rv = ov - sp[y * P + x];
             const float co_ = ov - mo, cr = rv - mr;
             vo = fmla<F>(co_, co_, vo);
             vr = fmla<F>(cr, cr, vr);
@@ -575,7 +579,7 @@ __device__ __forceinline__ void refine_costs_body(RfLds<BIG ? 1024 : 256, BIG ? 
 
 // A fixed grid walks the list (its length is known on the device only).
 template <bool F, bool BIG>
-__global__ __launch_bounds__(BIG ? 256 : 64) void k_refine_costs(RefineArgs A) {
+__global__ __launch_bounds__(BIG ? 256 : 64, BIG ? 4 : 1) void k_refine_costs(RefineArgs A) {
   __shared__ RfLds<BIG ? 1024 : 256, BIG ? 1056 : 272> L;
   const uint32_t n = A.count[BIG ? 1 : 0];
   for (uint32_t item = blockIdx.x; item < n; item += gridDim.x) {

@@ -82,7 +82,8 @@ struct PieceShape<3> {
 struct PcLds {
   float wy[PC_WORK];         // luma coefficients of the batch's sites
   float wc[PC_WORK];         // X or B coefficients
-  float sq[2048], mg[2048];  // per coefficient: squared error, rate term (storage order)
+  float mg[2048];            // per coefficient: rate term (storage order); the squared
+                             // errors go to wc once its coefficients are in registers
   uint8_t sp[2048];          // per coefficient: its scan position (< 256 for these shapes)
   uint64_t nzm[16][4];       // per site and 64-coefficient chunk: the nonzero levels
   float quot[3][64];         // 0, 1 - one, numerator / a
@@ -199,15 +200,22 @@ __device__ __forceinline__ void piece_batch(PcLds &L, const PieceArgs &A, int b,
       const int e = it * 256 + tid;
       invk[it] = inv[e - (e / SIZE) * SIZE];
     }
+    float coefs[ITERS];
+#pragma unroll
+    for (int it = 0; it < ITERS; it++) {
+      const int off = geo[it] & 4095;
+      const float yv = L.wy[off];
+      coefs[it] = c == 1 ? yv : fmla<F>(-f, yv, L.wc[off]);
+    }
+    __syncthreads();  // every coefficient read before wc takes the squared errors
 #pragma unroll
     for (int it = 0; it < ITERS; it++) {
       const int e0 = it * 256;
       const int e = e0 + tid;
       const int site = __builtin_amdgcn_readfirstlane((e0 + (tid & ~63)) / SIZE);
-      const int off = geo[it] & 4095, tsel = (geo[it] >> 12) & 3;
+      const int tsel = (geo[it] >> 12) & 3;
       const bool ac = (geo[it] >> 14) & 1;
-      const float yv = L.wy[off];
-      const float coef = c == 1 ? yv : fmla<F>(-f, yv, L.wc[off]);
+      const float coef = coefs[it];
       const float thr = tsel >= 2 ? (tsel == 3 ? t3 : t2) : (tsel == 1 ? t1 : t0);
       const float qs = L.qac[site] * qm;
       const float a = invk[it] * qs * coef;
@@ -219,7 +227,7 @@ __device__ __forceinline__ void piece_batch(PcLds &L, const PieceArgs &A, int b,
       else
         lev = q - one_num / q;
       const float diff = a - lev;
-      L.sq[e] = ac ? diff * diff : 0.0f;
+      L.wc[e] = ac ? diff * diff : 0.0f;
       float rt = 0.0f;
       if (q != 0.0f) {
         if (aq < 256.0f)
@@ -236,7 +244,7 @@ __device__ __forceinline__ void piece_batch(PcLds &L, const PieceArgs &A, int b,
     __syncthreads();
     if (tid < NS) {
       // squared errors: every term, in storage order
-      const float4 *src = reinterpret_cast<const float4 *>(L.sq + tid * SIZE);
+      const float4 *src = reinterpret_cast<const float4 *>(L.wc + tid * SIZE);
       float acc = 0.0f;
       for (int k4 = 0; k4 < SIZE / 4; k4 += 4) {
         float4 v[4];

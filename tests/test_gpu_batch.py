@@ -102,6 +102,56 @@ def test_batch_members_match_oracle(J, fe, torch, oracle):
         b.close()
 
 
+def same_dev(a, b):
+    """Bit equality of two device tensors (floats compared as their bits)."""
+    import torch
+    if a.dtype == torch.float32 and b.dtype == torch.float32:
+        a, b = a.view(torch.int32), b.view(torch.int32)
+    return a.shape == b.shape and a.dtype == b.dtype and bool((a == b).all())
+
+
+def test_batch_recipe_matches_single_image_path(J, fe, torch, oracle):
+    """The first 32 images of the timed batch recipe (sides uniform in
+    64..2048, not multiples of 8, quality 50..95, photo-like content with
+    seeds 1000 + i, Fast tier): after the autotuner has picked the tile
+    shapes, every member equals the single-image path bit for bit, and the
+    largest member equals the oracle."""
+    from jixel_amd import synth
+    dev = torch.device("cuda", 0)
+    sizes = synth.batch_sizes(32, 4)  # the 512-image recipe's first 32 draws
+    assert max(max(w, h) for w, h, _ in sizes) > 1536
+    field = synth.one_over_f_field(4096, 4, dev)
+    images = [(synth.photo_like_device(w, h, 1000 + i, dev, field),
+               J.frame_params(J.EncodeConfig().with_quality(q)))
+              for i, (w, h, q) in enumerate(sizes)]
+    del field
+    b = J.Batch(fe, images)
+    try:
+        b.autotune(reps=1)
+        b.run()
+        torch.cuda.synchronize()
+        assert b.status()[1] == 0
+        for i, (rgb, p) in enumerate(images):
+            got, want = b.outputs(i), fe.run(rgb, rgb.shape[1], rgb.shape[0], p)
+            for k in KEYS:
+                assert same_dev(got[k], want[k]), (i, sizes[i], k)
+            for c in range(3):
+                for k in ("levels", "coeffs", "dc"):
+                    assert same_dev(got[k][c], want[k][c]), (i, sizes[i], k, c)
+        big = max(range(len(sizes)), key=lambda i: sizes[i][0] * sizes[i][1])
+        rgb, p = images[big]
+        im = rgb.cpu().numpy()
+        want = oracle.front_end(im, p, oracle.tables_for(p), threads=8)
+        got = b.outputs(big)
+        for k in KEYS:
+            assert same_bits(got[k].cpu().numpy(), want[k]), (big, sizes[big], k)
+        for c in range(3):
+            assert same_bits(got["levels"][c].cpu().numpy(), want["levels"][c]), (big, c)
+            assert same_bits(got["dc"][c].cpu().numpy(), want["dc"][c]), (big, c)
+    finally:
+        b.close()
+
+
 def test_batch_graph_replays_and_autotune(J, fe, torch):
     """The captured graph, replayed several times with eager work and
     synchronizations in between, and after the autotuner changes the tile
